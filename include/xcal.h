@@ -1,0 +1,285 @@
+/*
+ * xcal.h -- C ABI of libxcalhip.so, the gfx950 (MI355X) implementation of
+ * XCALibre's per-iteration finite-volume SIMPLE hot path.
+ *
+ * The entry points are what a Julia package extension (a sibling of
+ * ext/XCALibre_AMDExt.jl) binds with `ccall` behind XCALibre's own API:
+ * `(...) -> ScalarEquation(phi, BCs)`, `-> VectorEquation(psi, BCs)` and
+ * `run!(model, config; pref)`.  Each declaration cites the reference
+ * function it replaces.  Conventions:
+ *   - Mesh arrays arrive in the reference layout (1-based Int64 indices,
+ *     Float64 geometry, boundary-first faces; src/Mesh/Mesh_0_types.jl).
+ *     Host pointers are borrowed for the duration of the call only.
+ *   - The library owns every device allocation behind opaque handles.
+ *   - Every call returns 0 on success, < 0 on error (message from
+ *     xcal_last_error(), thread-local), > 0 for warnings (XCAL_ITMAX).
+ *   - Calls on one context are ordered on its HIP stream; calls that return
+ *     host scalars synchronise.  Contexts must not be shared between threads
+ *     without external locking.
+ *   - Cell numbering, CSR pattern (columns ascending within a row) and face
+ *     numbering are the reference's; CSR arrays cross the boundary 0-based.
+ */
+#ifndef XCAL_H
+#define XCAL_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define XCAL_OK 0
+#define XCAL_ITMAX 1          /* warning: Krylov itmax reached (Solve_1_api.jl:302) */
+#define XCAL_ERR_ARG -1
+#define XCAL_ERR_HIP -2
+#define XCAL_ERR_UNSUPPORTED -3
+#define XCAL_ERR_NOMEM -4
+#define XCAL_ERR_STATE -5
+
+typedef struct xcal_ctx xcal_ctx;
+typedef struct xcal_mesh xcal_mesh;
+typedef struct xcal_field xcal_field;
+typedef struct xcal_eqn xcal_eqn;
+typedef struct xcal_simple xcal_simple;
+
+/* ---------------------------------------------------------------- mesh in */
+/* The fields of Mesh2/Mesh3 that the hot path reads
+ * (src/Mesh/Mesh_0_types.jl:135-185).  dim = 2 for Mesh2, 3 for Mesh3. */
+typedef struct {
+  int32_t dim;
+  int32_t n_patches;
+  int64_t n_cells, n_faces, n_bfaces, n_cell_faces;
+  const double* cell_centre;        /* 3*n_cells                         */
+  const double* cell_volume;        /* n_cells                           */
+  const int64_t* cell_faces_range;  /* 2*n_cells: first,last (1-based)   */
+  const int64_t* cell_faces;        /* n_cell_faces, 1-based face ids    */
+  const int64_t* cell_neighbours;   /* n_cell_faces, 1-based cell ids    */
+  const int64_t* cell_nsign;        /* n_cell_faces, +1 owner / -1 neigh */
+  const int64_t* face_owner;        /* 2*n_faces: ownerCells (1-based)   */
+  const double* face_centre;        /* 3*n_faces                         */
+  const double* face_normal;        /* 3*n_faces                         */
+  const double* face_e;             /* 3*n_faces                         */
+  const double* face_area;          /* n_faces                           */
+  const double* face_delta;         /* n_faces                           */
+  const double* face_weight;        /* n_faces                           */
+  const int64_t* boundary_cellsID;  /* n_bfaces (1-based)                */
+  const int64_t* patch_range;       /* 2*n_patches: IDs_range (1-based)  */
+} xcal_mesh_host;
+
+/* ------------------------------------------------------- operator spec */
+/* Operator types (ModelFramework_0_types.jl:18-78, schemes in
+ * Discretise_1_schemes.jl). */
+enum {
+  XCAL_TIME_STEADY = 0,       /* Time{SteadyState}          :12-22 */
+  XCAL_TIME_EULER = 1,        /* Time{Euler}                :25-42 */
+  XCAL_LAPLACIAN_LINEAR = 2,  /* Laplacian{Linear}          :68-110 */
+  XCAL_DIV_LINEAR = 3,        /* Divergence{Linear}         :114-133 */
+  XCAL_DIV_UPWIND = 4,        /* Divergence{Upwind}         :137-151 */
+  XCAL_DIV_LUST = 5,          /* Divergence{LUST}           :155-179 */
+  XCAL_DIV_BOUNDED_UPWIND = 6,/* Divergence{BoundedUpwind}  :183-199 */
+  XCAL_SI = 7                 /* Si (implicit source)       :204-218 */
+};
+
+/* One operator term: Operator(flux, phi, sign, type).  flux is a face scalar
+ * field for Laplacian/Divergence and a cell scalar field for Time/Si; a NULL
+ * flux means ConstantScalar(flux_const). */
+typedef struct {
+  int32_t type;
+  int32_t sign; /* +1 or -1 */
+  const xcal_field* flux;
+  double flux_const;
+} xcal_term;
+
+/* Src(field, sign) (ModelFramework_0_types.jl:80-88); the field's kind
+ * (scalar or vector) must match the equation. */
+typedef struct {
+  const xcal_field* field;
+  int32_t sign;
+} xcal_source;
+
+/* Boundary conditions (files under Discretise/boundary_conditions/).  One entry per
+ * mesh patch, in patch order, as `assign(region, (U=[...], ...))` produces. */
+enum {
+  XCAL_BC_DIRICHLET = 0,
+  XCAL_BC_NEUMANN = 1,
+  XCAL_BC_ZEROGRADIENT = 2,
+  XCAL_BC_EXTRAPOLATED = 3,
+  XCAL_BC_WALL = 4,
+  XCAL_BC_SYMMETRY = 5
+};
+typedef struct {
+  int32_t kind;
+  int32_t patch;    /* 0-based mesh patch index */
+  double value[3];  /* scalar BCs use value[0] */
+} xcal_bc;
+
+/* Linear solver settings: SolverSetup (Solve_1_api.jl:8-83). */
+enum { XCAL_SOLVER_CG = 0, XCAL_SOLVER_BICGSTAB = 1 };
+enum { XCAL_PRECON_NONE = 0, XCAL_PRECON_JACOBI = 1, XCAL_PRECON_DILU = 2 };
+enum { XCAL_PREC_FP64 = 0,      /* plain FP64 Krylov                         */
+       XCAL_PREC_FP32_IR = 1 }; /* FP32 inner Krylov in FP64 refinement loop */
+typedef struct {
+  int32_t solver;
+  int32_t precon;
+  int32_t precision;
+  int32_t itmax;
+  double convergence;
+  double relax;
+  double atol;
+  double rtol;
+  double ir_inner_rtol; /* FP32_IR: relative tolerance of each inner solve */
+} xcal_solver_setup;
+
+typedef struct {
+  int32_t niter;     /* Krylov iterations (summed over IR sweeps)        */
+  int32_t nouter;    /* IR sweeps (1 for FP64)                           */
+  double residual;   /* residual() of Solve_1_api.jl:441-464            */
+} xcal_solve_stats;
+
+/* ------------------------------------------------------------ context */
+const char* xcal_last_error(void);
+int32_t xcal_abi_version(void);
+int xcal_ctx_create(int32_t device, xcal_ctx** out);
+int xcal_ctx_destroy(xcal_ctx* ctx);
+int xcal_ctx_synchronize(xcal_ctx* ctx);
+
+/* -------------------------------------------------------------- mesh */
+/* Replaces adapt(backend, mesh) plus the host CSR build of
+ * sparse_matrix_connectivity / _build_A (ModelFramework_0_types.jl:106,
+ * 223-242; ext/XCALibre_AMDExt.jl:25-28).  Builds the device SoA mesh, the
+ * CSR pattern, face->nonzero slot maps (replacing every spindex search,
+ * ModelFramework_0_types.jl:246-258) and the DILU level schedule. */
+int xcal_mesh_create(xcal_ctx* ctx, const xcal_mesh_host* host, xcal_mesh** out);
+int xcal_mesh_destroy(xcal_mesh* mesh);
+/* n_cells, nnz, number of forward/backward DILU levels, device bytes */
+int xcal_mesh_info(const xcal_mesh* mesh, int64_t* n_cells, int64_t* nnz,
+                   int32_t* fwd_levels, int32_t* bwd_levels, int64_t* device_bytes);
+/* CSR pattern, 0-based (rowptr n+1, colval nnz) */
+int xcal_mesh_csr_pattern(const xcal_mesh* mesh, int32_t* rowptr, int32_t* colval);
+
+/* ------------------------------------------------------------- fields */
+/* ScalarField/VectorField/FaceScalarField/FaceVectorField/TensorField
+ * (src/Fields/Fields_0_types.jl:56-203), SoA per component. */
+enum {
+  XCAL_CELL_SCALAR = 0,
+  XCAL_CELL_VECTOR = 1,
+  XCAL_FACE_SCALAR = 2,
+  XCAL_FACE_VECTOR = 3,
+  XCAL_CELL_TENSOR = 4
+};
+int xcal_field_create(xcal_mesh* mesh, int32_t kind, xcal_field** out);
+int xcal_field_destroy(xcal_field* f);
+int xcal_field_upload(xcal_field* f, int32_t component, const double* host);
+int xcal_field_download(const xcal_field* f, int32_t component, double* host);
+/* initialise!(field, value) (Fields_0_types.jl:376-402) */
+int xcal_field_fill(xcal_field* f, int32_t component, double value);
+
+/* ---------------------------------------------------------- equations */
+/* ScalarEquation(phi, BCs) / VectorEquation(psi, BCs)
+ * (ModelFramework_0_types.jl:110-215). */
+int xcal_eqn_create(xcal_mesh* mesh, int32_t vector, xcal_eqn** out);
+int xcal_eqn_destroy(xcal_eqn* e);
+/* discretise! (Discretise_2_generated_distretisation.jl:3-167): cell-gather
+ * assembly into nzval (A for scalar, A0 for vector) and b (bx,by,bz).
+ * prev: the field (scalar or vector) used by Time terms. */
+int xcal_discretise(xcal_eqn* e, const xcal_term* terms, int32_t nterms,
+                    const xcal_source* sources, int32_t nsources, const xcal_field* prev);
+/* update_equation! (Discretise_2_generated_distretisation.jl:305-330): A <- A0 */
+int xcal_update_equation(xcal_eqn* e);
+/* apply_boundary_conditions! (Discretise_5_apply_bcs.jl:5-123) for the
+ * given component (0 for scalar equations, 0..2 for vector equations).
+ * phi is the equation's unknown (read by Extrapolated/Wall/Symmetry). */
+int xcal_apply_bcs(xcal_eqn* e, const xcal_term* terms, int32_t nterms,
+                   const xcal_bc* bcs, int32_t nbcs, const xcal_field* phi,
+                   int32_t component);
+/* implicit_relaxation_diagdom! (Solve_1_api.jl:370-411) */
+int xcal_relax_implicit_diagdom(xcal_eqn* e, const xcal_field* phi, int32_t component,
+                                double alpha);
+/* setReference! (Solve_1_api.jl:414-439); cell is 1-based */
+int xcal_set_reference(xcal_eqn* e, double pref, int64_t cell);
+/* copy A (which=0) or A0 (which=1) in CSR order and b of `component` out */
+int xcal_eqn_download(const xcal_eqn* e, int32_t which, int32_t component, double* nzval,
+                      double* b);
+/* update_preconditioner! (preconditioners_2_functions.jl:65-124) */
+int xcal_precond_update(xcal_eqn* e, int32_t precon);
+/* solve_system! (Solve_1_api.jl:268-307): Krylov solve warm-started from
+ * phi, copy into phi, residual.  Returns XCAL_ITMAX when itmax was hit. */
+int xcal_solve_system(xcal_eqn* e, const xcal_solver_setup* s, xcal_field* phi,
+                      int32_t component, xcal_solve_stats* stats);
+
+/* ---------------------------------------------------------- calculate */
+/* interpolate! (Calculate_2_interpolation.jl:40-76, 347-397) */
+int xcal_interpolate(xcal_field* face, const xcal_field* cell);
+/* correct_boundaries! (boundary_conditions/1_boundary_interpolation.jl:3-37) */
+int xcal_correct_boundaries(xcal_field* face, const xcal_field* cell, const xcal_bc* bcs,
+                            int32_t nbcs);
+/* green_gauss! (Calculate_1_green_gauss.jl:5-173): scalar->vector or
+ * vector->tensor, selected by the field kinds */
+int xcal_green_gauss(xcal_field* grad, const xcal_field* face);
+/* grad!{Gauss} = interpolate! + correct_boundaries! + green_gauss!
+ * (Calculate_0_gradient.jl:105-117) */
+int xcal_grad(xcal_field* grad, xcal_field* face, const xcal_field* cell, const xcal_bc* bcs,
+              int32_t nbcs);
+/* div! for a FaceScalarField (Calculate_0_divergence.jl:99-171) */
+int xcal_div(xcal_field* cell, const xcal_field* face);
+/* flux! (Solvers_0_functions.jl:41-64): mdot_f = psi_f . S_f */
+int xcal_flux(xcal_field* mdot, const xcal_field* face_vector);
+/* explicit_relaxation! (Solve_1_api.jl:317-333) */
+int xcal_relax_explicit(xcal_field* phi, const xcal_field* phi0, double alpha);
+
+/* ------------------------------------------------------ SIMPLE stages */
+/* inverse_diagonal! (Solvers_0_functions.jl:96-122) */
+int xcal_inverse_diagonal(xcal_field* rD, const xcal_eqn* e);
+/* remove_pressure_source! (Solvers_0_functions.jl:156-181) */
+int xcal_remove_pressure_source(xcal_eqn* e, const xcal_field* gradp, int32_t source_sign);
+/* H! (Solvers_0_functions.jl:184-249) */
+int xcal_H(xcal_field* Hv, const xcal_field* U, const xcal_eqn* e);
+/* correct_mass_flux! + correct_boundary_mass_flux!
+ * (Solvers_1_SIMPLE.jl:340-388, 474-557).  p_flux/p_sign: the pressure
+ * Laplacian term's flux (rDf) and sign. */
+int xcal_correct_mass_flux(xcal_field* mdot, const xcal_eqn* peqn, const xcal_field* p,
+                           const xcal_field* p_flux, int32_t p_sign, const xcal_bc* bcs,
+                           int32_t nbcs);
+/* correct_velocity! (Solvers_0_functions.jl:126-152) */
+int xcal_correct_velocity(xcal_field* U, const xcal_field* Hv, const xcal_field* gradp,
+                          const xcal_field* rD);
+
+/* ------------------------------------------------------ SIMPLE driver */
+/* The steady incompressible SIMPLE loop of Solvers_1_SIMPLE.jl:103-263 with
+ * laminar viscosity, Gauss gradients and no non-orthogonal correctors,
+ * entirely on the device (run!/simple!, Solvers_3_solver_dispatch.jl:120-135). */
+typedef struct {
+  double nu;
+  int32_t div_scheme;   /* XCAL_DIV_* for the momentum convection term */
+  int32_t has_pref;     /* run!(...; pref) given                       */
+  double pref;
+  xcal_solver_setup U;
+  xcal_solver_setup p;
+  const xcal_bc* bc_U;  /* n_patches entries */
+  const xcal_bc* bc_p;  /* n_patches entries */
+  int32_t n_bcs;
+} xcal_simple_setup;
+
+enum { XCAL_SIMPLE_U = 0, XCAL_SIMPLE_P = 1, XCAL_SIMPLE_MDOTF = 2 };
+
+int xcal_simple_create(xcal_mesh* mesh, const xcal_simple_setup* setup, xcal_simple** out);
+int xcal_simple_destroy(xcal_simple* s);
+/* U: 3*n_cells (x block, y block, z block); p: n_cells */
+int xcal_simple_set_field(xcal_simple* s, int32_t which, const double* host);
+int xcal_simple_get_field(const xcal_simple* s, int32_t which, double* host);
+/* Run up to `iterations` SIMPLE iterations (stops early on convergence as
+ * the reference does).  residuals: 4*iterations doubles laid out
+ * [Ux..., Uy..., Uz..., p...] (the NamedTuple of Solvers_1_SIMPLE.jl:262);
+ * done: iterations performed.  krylov_iters (optional, 4*iterations ints)
+ * gets the Krylov iteration counts of the Ux, Uy, Uz and p solves. */
+int xcal_simple_run(xcal_simple* s, int32_t iterations, double* residuals, int32_t* done,
+                    int32_t* krylov_iters);
+/* Device time (ms, HIP events) spent in the last xcal_simple_run by stage:
+ * [0] momentum assembly+BCs, [1] momentum solves, [2] pressure assembly,
+ * [3] pressure solve, [4] gradients/interpolation/flux/corrections, [5] total */
+int xcal_simple_stage_times(const xcal_simple* s, double* ms6);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,127 @@
+/*
+ * oracle.h -- CPU restatement of XCALibre's FP64 SIMPLE hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the parity checker for the
+ * gfx950 implementation (libxcalhip.so) and the timed CPU baseline; it is
+ * never part of the product path.  Each function restates the reference
+ * function named beside it, in the same loop order and floating-point
+ * evaluation order (built with -ffp-contract=off).  Cell and face loops that
+ * the reference runs as independent work-items may run in OpenMP threads;
+ * every accumulation that the reference performs with atomics or in a
+ * sequential loop is performed sequentially here in ascending index order.
+ *
+ * Index conventions: all ids crossing this ABI are 0-based; the mesh comes
+ * in as xcal_mesh_host (1-based, reference layout) and is converted once.
+ */
+#ifndef XCAL_ORACLE_H
+#define XCAL_ORACLE_H
+
+#include <stdint.h>
+
+#include "xcal.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_mesh orc_mesh;
+
+/* operator term over host arrays; flux == NULL means constant flux_const */
+typedef struct {
+  int32_t type; /* XCAL_TIME_STEADY ... XCAL_SI */
+  int32_t sign;
+  const double* flux;
+  double flux_const;
+} orc_term;
+
+/* source term; y,z are used by vector equations only */
+typedef struct {
+  const double* x;
+  const double* y;
+  const double* z;
+  int32_t sign;
+} orc_source;
+
+const char* orc_last_error(void);
+int orc_set_threads(int32_t n);
+
+int orc_mesh_create(const xcal_mesh_host* host, orc_mesh** out);
+void orc_mesh_destroy(orc_mesh* m);
+int64_t orc_nnz(const orc_mesh* m);
+/* CSR pattern (0-based): rowptr n+1, colval nnz */
+void orc_csr(const orc_mesh* m, int64_t* rowptr, int64_t* colval);
+
+/* discretise! scalar (Discretise_2_generated_distretisation.jl:89-167) */
+int orc_discretise_scalar(const orc_mesh* m, const orc_term* t, int32_t nt,
+                          const orc_source* s, int32_t ns, const double* prev, double dt,
+                          double* nzval, double* b);
+/* discretise! vector (Discretise_2_generated_distretisation.jl:3-87) -> A0 */
+int orc_discretise_vector(const orc_mesh* m, const orc_term* t, int32_t nt,
+                          const orc_source* s, int32_t ns, const double* prevx,
+                          const double* prevy, const double* prevz, double dt, double* nzval0,
+                          double* bx, double* by, double* bz);
+/* apply_boundary_conditions! (Discretise_5_apply_bcs.jl:5-123).
+ * component: -1 scalar equation, 0..2 vector component.  phi: the unknown
+ * (x,y,z arrays for vectors; phix only for scalars). */
+int orc_apply_bcs(const orc_mesh* m, const orc_term* t, int32_t nt, const xcal_bc* bcs,
+                  int32_t nbc, int32_t component, const double* phix, const double* phiy,
+                  const double* phiz, double* nzval, double* b);
+/* implicit_relaxation_diagdom! (Solve_1_api.jl:370-411) */
+void orc_relax_implicit_diagdom(const orc_mesh* m, double* nzval, double* b,
+                                const double* field, double alpha);
+/* setReference! (Solve_1_api.jl:414-439) */
+void orc_set_reference(const orc_mesh* m, double* nzval, double* b, double pref, int64_t cell);
+/* SpMV y = A x (Multithread/spmvm.jl:35-59) */
+void orc_spmv(const orc_mesh* m, const double* nzval, const double* x, double* y);
+/* residual (Solve_1_api.jl:441-464) */
+double orc_residual(const orc_mesh* m, const double* nzval, const double* b, const double* x);
+/* DILU (preconditioners_1_DILU.jl:43-120): D from A, then z = P \ r */
+void orc_dilu_diagonal(const orc_mesh* m, const double* nzval, double* D);
+void orc_dilu_apply(const orc_mesh* m, const double* nzval, const double* D, const double* r,
+                    double* z);
+/* Krylov CG / BiCGSTAB as called from solve_system! (Solve_1_api.jl:268-307).
+ * x: warm start in, solution out.  Returns 1 when itmax was reached. */
+int orc_krylov(const orc_mesh* m, int32_t solver, int32_t precon, const double* nzval,
+               const double* b, double* x, int32_t itmax, double atol, double rtol,
+               int32_t* niter);
+
+/* interpolate! scalar (Calculate_2_interpolation.jl:40-76) */
+void orc_interpolate(const orc_mesh* m, const double* phi, double* phif);
+/* correct_boundaries! (1_boundary_interpolation.jl); vector: component 0..2
+ * arrays given via phix/phiy/phiz and outputs fx/fy/fz; scalar: fy==NULL */
+int orc_correct_boundaries(const orc_mesh* m, const xcal_bc* bcs, int32_t nbc,
+                           const double* phix, const double* phiy, const double* phiz,
+                           double* fx, double* fy, double* fz);
+/* green_gauss! scalar -> vector (Calculate_1_green_gauss.jl:5-78) */
+void orc_green_gauss(const orc_mesh* m, const double* phif, double* gx, double* gy,
+                     double* gz);
+/* green_gauss! vector -> tensor (Calculate_1_green_gauss.jl:82-173);
+ * t: 9 arrays xx,xy,xz,yx,yy,yz,zx,zy,zz */
+void orc_green_gauss_tensor(const orc_mesh* m, const double* fx, const double* fy,
+                            const double* fz, double* const* t);
+/* div! (Calculate_0_divergence.jl:99-171) */
+void orc_div(const orc_mesh* m, const double* psif, double* div);
+/* flux! (Solvers_0_functions.jl:41-64) */
+void orc_flux(const orc_mesh* m, const double* fx, const double* fy, const double* fz,
+              double* mdot);
+/* H! (Solvers_0_functions.jl:184-249) */
+void orc_H(const orc_mesh* m, const double* nzval, const double* bx, const double* by,
+           const double* bz, const double* Ux, const double* Uy, const double* Uz,
+           double* Hx, double* Hy, double* Hz);
+/* correct_mass_flux! + correct_boundary_mass_flux! (Solvers_1_SIMPLE.jl:340-557) */
+int orc_correct_mass_flux(const orc_mesh* m, const double* p_nzval, const double* p,
+                          const double* pflux, int32_t psign, const xcal_bc* bcs,
+                          int32_t nbc, double* mdot);
+
+/* Whole steady SIMPLE run (Solvers_1_SIMPLE.jl:45-263, laminar).
+ * U: 3 arrays (x,y,z) in/out, p in/out.  residuals: 4*iterations
+ * [Ux.., Uy.., Uz.., p..]; krylov_iters: 4*iterations (may be NULL);
+ * mdotf: n_faces output (may be NULL). */
+int orc_simple_run(const orc_mesh* m, const xcal_simple_setup* setup, double* Ux, double* Uy,
+                   double* Uz, double* p, int32_t iterations, double* residuals,
+                   int32_t* done, int32_t* krylov_iters, double* mdotf);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

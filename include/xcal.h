@@ -202,6 +202,14 @@ int xcal_eqn_download(const xcal_eqn* e, int32_t which, int32_t component, doubl
                       double* b);
 /* update_preconditioner! (preconditioners_2_functions.jl:65-124) */
 int xcal_precond_update(xcal_eqn* e, int32_t precon);
+/* z = P \ r with the equation's current preconditioner: ldiv!(x, P, b) for
+ * DILU (preconditioners_1_DILU.jl:116-120, ext/XCALibre_AMDExt.jl:149-156),
+ * mul!(z, opDiagonal(1/|a_ii|), r) for Jacobi.  r and z: cell scalar fields
+ * or components of cell vector fields. */
+int xcal_precond_apply(xcal_eqn* e, const xcal_field* r, int32_t rcomp, xcal_field* z,
+                       int32_t zcomp);
+/* y = A x with the equation's A (mul! on SparseXCSR, Multithread/spmvm.jl:85-91) */
+int xcal_spmv(xcal_eqn* e, const xcal_field* x, int32_t xcomp, xcal_field* y, int32_t ycomp);
 /* solve_system! (Solve_1_api.jl:268-307): Krylov solve warm-started from
  * phi, copy into phi, residual.  Returns XCAL_ITMAX when itmax was hit. */
 int xcal_solve_system(xcal_eqn* e, const xcal_solver_setup* s, xcal_field* phi,

@@ -1,0 +1,340 @@
+"""Parity of the gfx950 library (through its C ABI) with the CPU oracle.
+
+Assembly, boundary conditions, relaxation, interpolation, gradients,
+divergence, fluxes, SIMPLE stages, SpMV and DILU follow the reference's
+evaluation order and must match the oracle bit for bit.  Krylov solves and
+whole SIMPLE runs differ only by the order of dot-product reductions and are
+held to stated relative tolerances.
+"""
+import numpy as np
+import pytest
+
+from conftest import cavity_case, channel_case, rel
+
+pytestmark = pytest.mark.gpu
+
+MESHES = ["laplace3", "cavity40", "box12", "cavity_hex", "step", "trig40"]
+
+
+@pytest.fixture(scope="module")
+def ctx(x):
+    return x.hip.Context(0)
+
+
+def _rng(seed=1):
+    return np.random.default_rng(seed)
+
+
+def _scalar_bcs(A, mesh):
+    kinds = [A.BC_DIRICHLET, A.BC_EXTRAPOLATED, A.BC_ZEROGRADIENT, A.BC_NEUMANN, A.BC_WALL,
+             A.BC_SYMMETRY]
+    return [(kinds[i % len(kinds)], i, 0.7 + i) for i in range(len(mesh.patch_names))]
+
+
+def _vector_bcs(A, mesh):
+    kinds = [A.BC_DIRICHLET, A.BC_EXTRAPOLATED, A.BC_WALL, A.BC_SYMMETRY, A.BC_ZEROGRADIENT]
+    return [(kinds[i % len(kinds)], i, [0.3 + i, -0.2, 0.1 * i]) for i in range(len(mesh.patch_names))]
+
+
+def _fields(mesh, rng):
+    n, nf = mesh.n_cells, mesh.n_faces
+    return dict(gamma=0.5 + rng.random(nf), mdot=rng.standard_normal(nf) * mesh.face_area,
+                src=rng.standard_normal(n), U=rng.standard_normal((3, n)),
+                phi=rng.standard_normal(n), si=rng.standard_normal(n))
+
+
+@pytest.mark.parametrize("name", MESHES)
+def test_csr_pattern_bit_exact(x, A, O, ctx, meshes, name):
+    mesh = meshes(name)
+    dm = x.hip.DeviceMesh(ctx, mesh)
+    rp, cv = dm.csr_pattern()
+    orp, ocv = O.OracleMesh(mesh, A).csr()
+    assert np.array_equal(rp, orp) and np.array_equal(cv, ocv)
+
+
+@pytest.mark.parametrize("div", ["DIV_UPWIND", "DIV_LINEAR", "DIV_LUST", "DIV_BOUNDED_UPWIND"])
+@pytest.mark.parametrize("name", MESHES)
+def test_scalar_assembly_and_bcs_bit_exact(x, A, O, ctx, meshes, name, div):
+    mesh = meshes(name)
+    rng = _rng(2)
+    F = _fields(mesh, rng)
+    H = x.hip
+    dm = H.DeviceMesh(ctx, mesh)
+    om = O.OracleMesh(mesh, A)
+    g = H.Field(dm, A.FACE_SCALAR).upload(F["gamma"])
+    md = H.Field(dm, A.FACE_SCALAR).upload(F["mdot"])
+    si = H.Field(dm, A.CELL_SCALAR).upload(F["si"])
+    src = H.Field(dm, A.CELL_SCALAR).upload(F["src"])
+    phi = H.Field(dm, A.CELL_SCALAR).upload(F["phi"])
+    dterms = [(A.TIME_STEADY, 1, 1.0), (getattr(A, div), 1, md), (A.LAPLACIAN_LINEAR, -1, g),
+              (A.SI, 1, si)]
+    oterms = [(A.TIME_STEADY, 1, 1.0), (getattr(A, div), 1, F["mdot"]),
+              (A.LAPLACIAN_LINEAR, -1, F["gamma"]), (A.SI, 1, F["si"])]
+    eq = H.Equation(dm, False)
+    eq.discretise(dterms, [(src, -1)], phi)
+    onz, ob = om.discretise_scalar(oterms, [(F["src"], -1)], F["phi"])
+    nz, b = eq.download()
+    assert np.array_equal(nz, onz) and np.array_equal(b, ob)
+    bcs = A.bc_array(_scalar_bcs(A, mesh))
+    eq.apply_bcs(dterms, bcs, phi)
+    om.apply_bcs(oterms, bcs, -1, F["phi"], onz, ob)
+    nz, b = eq.download()
+    assert np.array_equal(nz, onz) and np.array_equal(b, ob)
+    eq.set_reference(0.25, 1)
+    om.set_reference(onz, ob, 0.25, 0)
+    nz, b = eq.download()
+    assert np.array_equal(nz, onz) and np.array_equal(b, ob)
+
+
+@pytest.mark.parametrize("div", ["DIV_UPWIND", "DIV_LINEAR", "DIV_LUST"])
+@pytest.mark.parametrize("name", MESHES)
+def test_vector_assembly_bcs_relax_bit_exact(x, A, O, ctx, meshes, name, div):
+    mesh = meshes(name)
+    rng = _rng(3)
+    F = _fields(mesh, rng)
+    H = x.hip
+    dm = H.DeviceMesh(ctx, mesh)
+    om = O.OracleMesh(mesh, A)
+    nu = H.Field(dm, A.FACE_SCALAR).upload(F["gamma"] * 1e-2)
+    md = H.Field(dm, A.FACE_SCALAR).upload(F["mdot"])
+    U = H.Field(dm, A.CELL_VECTOR).upload(F["U"])
+    gp = H.Field(dm, A.CELL_VECTOR).upload(rng.standard_normal((3, mesh.n_cells)))
+    gph = gp.download()
+    dterms = [(A.TIME_STEADY, 1, 1.0), (getattr(A, div), 1, md), (A.LAPLACIAN_LINEAR, -1, nu)]
+    oterms = [(A.TIME_STEADY, 1, 1.0), (getattr(A, div), 1, F["mdot"]),
+              (A.LAPLACIAN_LINEAR, -1, F["gamma"] * 1e-2)]
+    eq = H.Equation(dm, True)
+    eq.discretise(dterms, [(gp, -1)], U)
+    onz0, ob = om.discretise_vector(oterms, [([gph[0], gph[1], gph[2]], -1)], F["U"])
+    nz0, _ = eq.download(which=1)
+    assert np.array_equal(nz0, onz0)
+    bcs = A.bc_array(_vector_bcs(A, mesh))
+    Uh = [np.ascontiguousarray(F["U"][c]) for c in range(3)]
+    for c in range(3):
+        eq.update_equation()
+        eq.apply_bcs(dterms, bcs, U, c)
+        onz = onz0.copy()
+        obc = ob[c].copy()
+        om.apply_bcs(oterms, bcs, c, Uh, onz, obc)
+        nz, b = eq.download(0, c)
+        assert np.array_equal(nz, onz) and np.array_equal(b, obc), f"component {c}"
+        eq.relax_implicit_diagdom(U, c, 0.7)
+        om.relax_implicit_diagdom(onz, obc, Uh[c], 0.7)
+        nz, b = eq.download(0, c)
+        assert np.array_equal(nz, onz) and np.array_equal(b, obc), f"relax component {c}"
+
+
+@pytest.mark.parametrize("name", MESHES)
+def test_calculate_bit_exact(x, A, O, ctx, meshes, name):
+    mesh = meshes(name)
+    rng = _rng(4)
+    H = x.hip
+    dm = H.DeviceMesh(ctx, mesh)
+    om = O.OracleMesh(mesh, A)
+    n, nf = mesh.n_cells, mesh.n_faces
+    phi_h = rng.standard_normal(n)
+    U_h = rng.standard_normal((3, n))
+    phi = H.Field(dm, A.CELL_SCALAR).upload(phi_h)
+    U = H.Field(dm, A.CELL_VECTOR).upload(U_h)
+    pf = H.Field(dm, A.FACE_SCALAR)
+    Uf = H.Field(dm, A.FACE_VECTOR)
+    grad = H.Field(dm, A.CELL_VECTOR)
+    gradU = H.Field(dm, A.CELL_TENSOR)
+    sb = A.bc_array(_scalar_bcs(A, mesh))
+    vb = A.bc_array(_vector_bcs(A, mesh))
+    # scalar: interpolate!, correct_boundaries!, green_gauss!, div!
+    H.check(H.lib().xcal_grad(grad.h, pf.h, phi.h, sb, len(sb)))
+    opf = om.interpolate(phi_h)
+    om.correct_boundaries(sb, phi_h, opf)
+    assert np.array_equal(pf.download(), opf)
+    assert np.array_equal(grad.download(), np.stack(om.green_gauss(opf)))
+    d = H.Field(dm, A.CELL_SCALAR)
+    H.check(H.lib().xcal_div(d.h, pf.h))
+    assert np.array_equal(d.download(), om.div(opf))
+    # vector: interpolate!, correct_boundaries!, tensor green_gauss!, flux!
+    H.check(H.lib().xcal_grad(gradU.h, Uf.h, U.h, vb, len(vb)))
+    oUf = [np.ascontiguousarray(om.interpolate(np.ascontiguousarray(U_h[c]))) for c in range(3)]
+    om.correct_boundaries(vb, [np.ascontiguousarray(U_h[c]) for c in range(3)], oUf)
+    assert np.array_equal(Uf.download(), np.stack(oUf))
+    assert np.array_equal(gradU.download(), np.stack(om.green_gauss_tensor(*oUf)))
+    m = H.Field(dm, A.FACE_SCALAR)
+    H.check(H.lib().xcal_flux(m.h, Uf.h))
+    assert np.array_equal(m.download(), om.flux(*oUf))
+    # explicit_relaxation!
+    p0 = H.Field(dm, A.CELL_SCALAR).upload(phi_h * 0.5)
+    H.check(H.lib().xcal_relax_explicit(phi.h, p0.h, 0.3))
+    assert np.array_equal(phi.download(), phi_h * 0.5 + 0.3 * (phi_h - phi_h * 0.5))
+
+
+@pytest.mark.parametrize("name", ["cavity40", "box12", "step", "cavity_hex"])
+def test_simple_stages_bit_exact(x, A, O, ctx, meshes, name):
+    mesh = meshes(name)
+    rng = _rng(5)
+    H = x.hip
+    L = H.lib()
+    dm = H.DeviceMesh(ctx, mesh)
+    om = O.OracleMesh(mesh, A)
+    n, nf = mesh.n_cells, mesh.n_faces
+    F = _fields(mesh, rng)
+    nu = H.Field(dm, A.FACE_SCALAR).upload(F["gamma"] * 1e-2)
+    md = H.Field(dm, A.FACE_SCALAR).upload(F["mdot"])
+    U_h = F["U"]
+    U = H.Field(dm, A.CELL_VECTOR).upload(U_h)
+    gp_h = rng.standard_normal((3, n))
+    gp = H.Field(dm, A.CELL_VECTOR).upload(gp_h)
+    dterms = [(A.TIME_STEADY, 1, 1.0), (A.DIV_UPWIND, 1, md), (A.LAPLACIAN_LINEAR, -1, nu)]
+    oterms = [(A.TIME_STEADY, 1, 1.0), (A.DIV_UPWIND, 1, F["mdot"]),
+              (A.LAPLACIAN_LINEAR, -1, F["gamma"] * 1e-2)]
+    eq = H.Equation(dm, True)
+    eq.discretise(dterms, [(gp, -1)], U)
+    onz, ob = om.discretise_vector(oterms, [([gp_h[0], gp_h[1], gp_h[2]], -1)], U_h)
+    eq.update_equation()
+    # inverse_diagonal!
+    rD = H.Field(dm, A.CELL_SCALAR)
+    H.check(L.xcal_inverse_diagonal(rD.h, eq.h))
+    rows, cols = om.csr()
+    diag_idx = np.array([rows[i] + np.searchsorted(cols[rows[i]:rows[i + 1]], i) for i in range(n)])
+    orD = mesh.cell_volume / onz[diag_idx]
+    assert np.array_equal(rD.download(), orD)
+    # remove_pressure_source! then H!
+    H.check(L.xcal_remove_pressure_source(eq.h, gp.h, -1))
+    obs = [ob[c] - (-1.0 * gp_h[c]) * mesh.cell_volume for c in range(3)]
+    Hv = H.Field(dm, A.CELL_VECTOR)
+    H.check(L.xcal_H(Hv.h, U.h, eq.h))
+    oH = om.H(onz, [np.ascontiguousarray(b) for b in obs], [np.ascontiguousarray(u) for u in U_h])
+    assert np.array_equal(Hv.download(), np.stack(oH))
+    # correct_mass_flux! with a pressure matrix
+    pterms_d = [(A.LAPLACIAN_LINEAR, -1, nu)]
+    peq = H.Equation(dm, False)
+    divH = H.Field(dm, A.CELL_SCALAR).upload(F["src"])
+    p_h = F["phi"]
+    p = H.Field(dm, A.CELL_SCALAR).upload(p_h)
+    peq.discretise(pterms_d, [(divH, -1)], p)
+    bcs = A.bc_array(_scalar_bcs(A, mesh))
+    peq.apply_bcs(pterms_d, bcs, p)
+    opnz, opb = om.discretise_scalar([(A.LAPLACIAN_LINEAR, -1, F["gamma"] * 1e-2)],
+                                     [(F["src"], -1)], p_h)
+    om.apply_bcs([(A.LAPLACIAN_LINEAR, -1, F["gamma"] * 1e-2)], bcs, -1, p_h, opnz, opb)
+    H.check(L.xcal_correct_mass_flux(md.h, peq.h, p.h, nu.h, -1, bcs, len(bcs)))
+    omd = F["mdot"].copy()
+    om.correct_mass_flux(opnz, p_h, F["gamma"] * 1e-2, -1, bcs, omd)
+    assert np.array_equal(md.download(), omd)
+    # correct_velocity!
+    H.check(L.xcal_correct_velocity(U.h, Hv.h, gp.h, rD.h))
+    oU = np.stack([oH[c] - gp_h[c] * orD for c in range(3)])
+    assert np.array_equal(U.download(), oU)
+
+
+@pytest.mark.parametrize("name", ["cavity40", "box12", "box20", "step"])
+@pytest.mark.parametrize("precon", ["PRECON_JACOBI", "PRECON_DILU"])
+def test_spmv_and_preconditioner_bit_exact(x, A, O, ctx, meshes, name, precon):
+    mesh = meshes(name)
+    rng = _rng(6)
+    H = x.hip
+    dm = H.DeviceMesh(ctx, mesh)
+    om = O.OracleMesh(mesh, A)
+    F = _fields(mesh, rng)
+    g = H.Field(dm, A.FACE_SCALAR).upload(F["gamma"])
+    md = H.Field(dm, A.FACE_SCALAR).upload(F["mdot"] * 0.1)
+    terms = [(A.DIV_UPWIND, 1, md), (A.LAPLACIAN_LINEAR, -1, g)]
+    oterms = [(A.DIV_UPWIND, 1, F["mdot"] * 0.1), (A.LAPLACIAN_LINEAR, -1, F["gamma"])]
+    eq = H.Equation(dm, False)
+    zero = H.Field(dm, A.CELL_SCALAR)
+    eq.discretise(terms, [(zero, 1)], zero)
+    onz, _ = om.discretise_scalar(oterms, [(np.zeros(mesh.n_cells), 1)])
+    r_h = rng.standard_normal(mesh.n_cells)
+    r = H.Field(dm, A.CELL_SCALAR).upload(r_h)
+    y = H.Field(dm, A.CELL_SCALAR)
+    eq.spmv(r, y)
+    assert np.array_equal(y.download(), om.spmv(onz, r_h))
+    eq.precond_update(getattr(A, precon))
+    z = H.Field(dm, A.CELL_SCALAR)
+    eq.precond_apply(r, z)
+    if precon == "PRECON_DILU":
+        oz = om.dilu_apply(onz, om.dilu_diagonal(onz), r_h)
+    else:
+        rows, cols = om.csr()
+        dg = np.array([onz[rows[i] + np.searchsorted(cols[rows[i]:rows[i + 1]], i)]
+                       for i in range(mesh.n_cells)])
+        oz = (1.0 / np.abs(dg)) * r_h
+    assert np.array_equal(z.download(), oz)
+
+
+@pytest.mark.parametrize("solver,precon", [("SOLVER_CG", "PRECON_DILU"), ("SOLVER_CG", "PRECON_JACOBI"),
+                                           ("SOLVER_BICGSTAB", "PRECON_DILU"),
+                                           ("SOLVER_BICGSTAB", "PRECON_JACOBI")])
+def test_krylov_matches_oracle(x, A, O, ctx, meshes, solver, precon):
+    """Same Krylov recurrences; only reduction order differs.  Tolerance:
+    solutions agree to 1e-9 relative when both converge to rtol 1e-12."""
+    mesh = meshes("box20")
+    rng = _rng(7)
+    H = x.hip
+    dm = H.DeviceMesh(ctx, mesh)
+    om = O.OracleMesh(mesh, A)
+    F = _fields(mesh, rng)
+    g = H.Field(dm, A.FACE_SCALAR).upload(F["gamma"])
+    sym = solver == "SOLVER_CG"
+    mdh = np.zeros(mesh.n_faces) if sym else F["mdot"] * 0.2
+    md = H.Field(dm, A.FACE_SCALAR).upload(mdh)
+    terms = [(A.DIV_UPWIND, 1, md), (A.LAPLACIAN_LINEAR, -1, g)]
+    oterms = [(A.DIV_UPWIND, 1, mdh), (A.LAPLACIAN_LINEAR, -1, F["gamma"])]
+    src = H.Field(dm, A.CELL_SCALAR).upload(F["src"])
+    phi = H.Field(dm, A.CELL_SCALAR).upload(F["phi"] * 0.01)
+    eq = H.Equation(dm, False)
+    eq.discretise(terms, [(src, 1)], phi)
+    bcs = A.bc_array([(A.BC_DIRICHLET, i, 1.0) for i in range(len(mesh.patch_names))])
+    eq.apply_bcs(terms, bcs, phi)
+    onz, ob = om.discretise_scalar(oterms, [(F["src"], 1)])
+    om.apply_bcs(oterms, bcs, -1, F["phi"] * 0.01, onz, ob)
+    pre = getattr(A, precon)
+    eq.precond_update(pre)
+    setup = A.SolverSetupC(getattr(A, solver), pre, A.PREC_FP64, 2000, 1e-7, 1.0, 1e-30, 1e-12,
+                           0.0)
+    it, res, rc = eq.solve(setup, phi)
+    ox, oit, orc = om.krylov(getattr(A, solver), pre, onz, ob, F["phi"] * 0.01, itmax=2000,
+                             atol=1e-30, rtol=1e-12)
+    assert rc == 0 and orc == 0
+    assert abs(it - oit) <= max(2, oit // 20)
+    assert rel(phi.download(), ox) < 1e-9
+    assert res < 1e-9
+
+
+def test_cavity_simple_matches_oracle(x, A, O, meshes):
+    """40x40 lid-driven cavity, Re=100, 200 SIMPLE iterations, DILU.
+    Stated tolerance: relative L2 of U and p against the oracle <= 1e-6."""
+    api = x.api
+    mesh = meshes("cavity40")
+    model, config = cavity_case(api, mesh, iterations=200)
+    setup = api.simple_setup(model, config, pref=0.0)
+    ref = O.OracleMesh(mesh, A).simple_run(setup, model.momentum.U, model.momentum.p, 200)
+    res = api.run(model, config, pref=0.0)
+    assert len(res["p"]) == ref["done"]
+    eU = rel(model.momentum.U, ref["U"])
+    ep = rel(model.momentum.p, ref["p"])
+    print(f"cavity40: rel L2 U {eU:.3e} p {ep:.3e}")
+    assert eU < 1e-6 and ep < 1e-6
+    assert np.allclose(res["Ux"], ref["residuals"][0], rtol=1e-4, atol=1e-12)
+
+
+@pytest.mark.parametrize("name", ["box12", "box20"])
+def test_channel_simple_matches_oracle(x, A, O, meshes, name):
+    """Jittered, renumbered hex channel, 10 SIMPLE iterations, p to rtol 1e-10.
+    Stated tolerance: relative L2 of U, p and mdotf against the oracle <= 1e-8."""
+    api = x.api
+    mesh = meshes(name)
+    model, config = channel_case(api, mesh, iterations=10)
+    setup = api.simple_setup(model, config)
+    ref = O.OracleMesh(mesh, A).simple_run(setup, model.momentum.U, model.momentum.p, 10)
+    H = x.hip
+    dm = H.DeviceMesh(api._context(0), mesh)
+    sim = H.Simple(dm, setup)
+    sim.set_U(model.momentum.U)
+    sim.set_p(model.momentum.p)
+    out = sim.run(10)
+    eU = rel(sim.get("U"), ref["U"])
+    ep = rel(sim.get("p"), ref["p"])
+    em = rel(sim.get("mdotf"), ref["mdotf"])
+    print(f"{name}: rel L2 U {eU:.3e} p {ep:.3e} mdot {em:.3e}; krylov its gpu "
+          f"{out['krylov_iters'][:, -1]} oracle {ref['krylov_iters'][:, -1]}")
+    assert out["done"] == ref["done"]
+    assert eU < 1e-8 and ep < 1e-8 and em < 1e-8

@@ -1,0 +1,253 @@
+"""Pinning the CPU oracle against the reference's golden vectors and against
+what the mathematics fixes (CPU only).  A plausible slip in the restatement
+(a dropped boundary term, a wrong sign or index, a transposed operand) fails
+at least one of these."""
+import json
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, cavity_case, channel_case, rel
+
+
+def dense(om, nz):
+    rp, cv = om.csr()
+    M = np.zeros((om.n, om.n))
+    for i in range(om.n):
+        M[i, cv[rp[i]:rp[i + 1]]] = nz[rp[i]:rp[i + 1]]
+    return M
+
+
+def test_laplace_golden_A_b(A, O, meshes):
+    """unit_test_laplace.jl:115-130 on laplace_unit_3by3.unv (k=1, rDf=1)."""
+    g = json.load(open(GOLDEN / "laplace_3by3.json"))
+    m = meshes("laplace3")
+    om = O.OracleMesh(m, A)
+    idx = {n: i for i, n in enumerate(m.patch_names)}
+    bcs = A.bc_array([(A.BC_DIRICHLET, idx["left_wall"], 50.0),
+                      (A.BC_ZEROGRADIENT, idx["right_wall"], 0.0),
+                      (A.BC_DIRICHLET, idx["bottom_wall"], 10.0),
+                      (A.BC_ZEROGRADIENT, idx["upper_wall"], 0.0)])
+    T = np.full(m.n_cells, 15.0)
+    terms = [(A.TIME_STEADY, 1, 0.0), (A.LAPLACIAN_LINEAR, -1, 1.0)]
+    nz, b = om.discretise_scalar(terms, [(np.zeros(m.n_cells), -1)], T)
+    om.apply_bcs(terms, bcs, -1, T, nz, b)
+    assert np.allclose(dense(om, nz), np.array(g["A"], float), atol=g["A_atol"])
+    assert np.allclose(dense(om, nz), np.array(g["A"], float), atol=1e-12)
+    assert np.allclose(b, g["b"], rtol=1e-12)
+
+
+@pytest.mark.parametrize("name", ["cavity40", "box12", "trig40", "cavity_hex"])
+def test_csr_pattern_closed_form(A, O, meshes, name):
+    """row i = {i} U neighbours(i), ascending (sparse_matrix_connectivity)."""
+    m = meshes(name)
+    rp, cv = O.OracleMesh(m, A).csr()
+    own = m.face_owner.reshape(-1, 2)[m.n_bfaces:] - 1
+    rows = [{i} for i in range(m.n_cells)]
+    for a, b in own:
+        rows[a].add(b)
+        rows[b].add(a)
+    for i in range(m.n_cells):
+        assert list(cv[rp[i]:rp[i + 1]]) == sorted(rows[i])
+
+
+def test_uniform_laplacian_and_upwind_closed_form(A, O, meshes):
+    """On the uniform 40x40 cavity: Laplacian off-diagonals are -Gamma*A/delta
+    = -Gamma, diagonals the negated row sum; upwind rows sum to sum of outflow
+    minus inflow of the face fluxes."""
+    m = meshes("cavity40")
+    om = O.OracleMesh(m, A)
+    nz, _ = om.discretise_scalar([(A.LAPLACIAN_LINEAR, -1, 0.01)], [])
+    D = dense(om, nz)
+    off = D - np.diag(np.diag(D))
+    assert np.allclose(off[off != 0], -0.01, rtol=1e-12)
+    assert np.allclose(np.diag(D), -off.sum(1), rtol=1e-12)
+    rng = np.random.default_rng(0)
+    mdot = rng.standard_normal(m.n_faces)
+    nz, _ = om.discretise_scalar([(A.DIV_UPWIND, 1, mdot)], [])
+    D = dense(om, nz)
+    net = np.zeros(m.n_cells)
+    cf = m.cell_faces - 1
+    rng_ = m.cell_faces_range.reshape(-1, 2)
+    cell_of = np.repeat(np.arange(m.n_cells), rng_[:, 1] - rng_[:, 0] + 1)
+    np.add.at(net, cell_of, mdot[cf] * m.cell_nsign)
+    assert np.allclose(D.sum(1), net, atol=1e-12)           # upwind row sum = net outflow
+    assert np.all(np.diag(D) >= 0) and np.all(D - np.diag(np.diag(D)) <= 0)
+
+
+@pytest.mark.parametrize("name", ["cavity40", "box12_natural", "trig40", "cavity_hex"])
+def test_green_gauss_exact_for_linear_fields(A, O, meshes, name):
+    """With exact face-centroid values of phi = a.x + c, Green-Gauss returns a
+    in every closed cell with planar faces; the tensor variant returns the
+    outer product for a linear vector field.  (Jittered hexes have non-planar
+    faces, for which the identity holds only to O(jitter).)"""
+    m = meshes(name)
+    om = O.OracleMesh(m, A)
+    a = np.array([0.3, -1.7, 0.9 if m.dim == 3 else 0.0])
+    fc = m.face_centre.reshape(-1, 3)
+    phif = fc @ a + 2.5
+    g = np.stack(om.green_gauss(phif))
+    tol = 1e-9 if name in ("cavity_hex", "trig40") else 1e-10
+    assert np.allclose(g, a[:, None], atol=tol * np.abs(a).max() * 10)
+    B = np.array([[1.0, 2.0, 0.5], [-0.5, 0.25, 1.0], [0.0, 3.0, -1.0]])
+    if m.dim == 2:
+        B[:, 2] = 0.0
+    Uf = [np.ascontiguousarray(fc @ B[p]) for p in range(3)]
+    T = np.stack(om.green_gauss_tensor(*Uf)).reshape(3, 3, -1)
+    assert np.allclose(T, B[:, :, None], atol=1e-8)
+
+
+@pytest.mark.parametrize("name", ["cavity40", "box12", "step"])
+def test_interpolation_and_divergence(A, O, meshes, name):
+    """Uniform flow has zero divergence; linear interpolation reproduces a
+    field that is linear along the owner-neighbour line at the weight point."""
+    m = meshes(name)
+    om = O.OracleMesh(m, A)
+    U = np.array([0.7, -0.2, 0.4 if m.dim == 3 else 0.0])
+    ones = np.ones(m.n_faces)
+    md = om.flux(U[0] * ones, U[1] * ones, U[2] * ones)
+    assert np.allclose(om.div(md), 0.0, atol=1e-10)
+    cc = m.cell_centre.reshape(-1, 3)
+    phi = cc @ U
+    pf = om.interpolate(phi)
+    own = m.face_owner.reshape(-1, 2) - 1
+    w = m.face_weight
+    expect = w * phi[own[:, 0]] + (1 - w) * phi[own[:, 1]]
+    assert np.allclose(pf, expect, rtol=1e-14, atol=1e-14)
+    if name == "cavity40":   # uniform: weight 1/2 at the face midpoint -> exact
+        fc = m.face_centre.reshape(-1, 3)
+        assert np.allclose(pf[m.n_bfaces:], (fc @ U)[m.n_bfaces:], atol=1e-13)
+
+
+def _random_mesh_matrix(A, O, m, seed=0, asym=True):
+    """Diagonally dominant matrix with the mesh's sparsity pattern."""
+    om = O.OracleMesh(m, A)
+    rng = np.random.default_rng(seed)
+    rp, cv = om.csr()
+    nz = -rng.random(len(cv)) * (1.0 if asym else 0.0)
+    if not asym:
+        own = m.face_owner.reshape(-1, 2)[m.n_bfaces:] - 1
+        for a, b in own:
+            v = -rng.random()
+            nz[rp[a] + np.searchsorted(cv[rp[a]:rp[a + 1]], b)] = v
+            nz[rp[b] + np.searchsorted(cv[rp[b]:rp[b + 1]], a)] = v
+    for i in range(om.n):
+        k = rp[i] + np.searchsorted(cv[rp[i]:rp[i + 1]], i)
+        nz[k] = 0.0
+        nz[k] = -nz[rp[i]:rp[i + 1]].sum() + 0.75 + rng.random()
+    return om, nz
+
+
+@pytest.mark.parametrize("name", ["cavity16", "box12"])
+def test_dilu_identities(A, O, meshes, name):
+    """test_DILU.jl:24-40: diag((D*+L) D*^-1 (D*+U)) = diag(A); forward and
+    backward substitution equal dense triangular solves."""
+    m = meshes(name)
+    om, nz = _random_mesh_matrix(A, O, m, seed=1)
+    Ad = dense(om, nz)
+    D = om.dilu_diagonal(nz)
+    L = np.tril(Ad, -1)
+    U = np.triu(Ad, 1)
+    PL = (np.diag(D) + L) @ np.diag(1.0 / D)
+    PU = np.diag(D) + U
+    P = PL @ PU
+    assert np.allclose(np.diag(P), np.diag(Ad), rtol=1e-12)
+    b = P @ np.ones(om.n)
+    x = np.linalg.solve(PU, np.linalg.solve(PL, b))
+    assert np.allclose(om.dilu_apply(nz, D, b), x, rtol=1e-10)
+    assert np.allclose(x, 1.0, rtol=1e-8)
+
+
+@pytest.mark.parametrize("solver,precon,asym", [(0, 2, False), (0, 1, False), (0, 0, False),
+                                               (1, 2, True), (1, 1, True), (1, 0, True)])
+def test_krylov_against_dense_solve(A, O, meshes, solver, precon, asym):
+    m = meshes("box12")
+    om, nz = _random_mesh_matrix(A, O, m, seed=2, asym=asym)
+    rng = np.random.default_rng(3)
+    b = rng.standard_normal(om.n)
+    xs = np.linalg.solve(dense(om, nz), b)
+    x0 = rng.standard_normal(om.n) * 0.1
+    x, it, rc = om.krylov(solver, precon, nz, b, x0, itmax=1000, atol=1e-14, rtol=1e-13)
+    assert rc == 0 and 0 < it < 1000
+    assert rel(x, xs) < 1e-10
+    assert om.residual(nz, b, x) < 1e-11
+
+
+def test_krylov_itmax_flag(A, O, meshes):
+    m = meshes("box12")
+    om, nz = _random_mesh_matrix(A, O, m, seed=4, asym=False)
+    b = np.ones(om.n)
+    _, it, rc = om.krylov(0, 1, nz, b, np.zeros(om.n), itmax=2, atol=0.0, rtol=1e-14)
+    assert it == 2 and rc == 1
+
+
+def test_residual_definition(A, O, meshes):
+    """residual = sqrt(sum (b-Ax)^2)/||b|| with denominator 1 when b = 0."""
+    m = meshes("cavity16")
+    om, nz = _random_mesh_matrix(A, O, m, seed=5)
+    rng = np.random.default_rng(6)
+    x = rng.standard_normal(om.n)
+    b = rng.standard_normal(om.n)
+    r = b - dense(om, nz) @ x
+    assert np.isclose(om.residual(nz, b, x), np.linalg.norm(r) / np.linalg.norm(b), rtol=1e-12)
+    z = np.zeros(om.n)
+    assert np.isclose(om.residual(nz, z, x), np.linalg.norm(dense(om, nz) @ x), rtol=1e-12)
+
+
+def test_implicit_relaxation_and_reference(A, O, meshes):
+    m = meshes("cavity16")
+    om, nz = _random_mesh_matrix(A, O, m, seed=7)
+    Ad = dense(om, nz)
+    rng = np.random.default_rng(8)
+    b = rng.standard_normal(om.n)
+    phi = rng.standard_normal(om.n)
+    nz2, b2 = nz.copy(), b.copy()
+    om.relax_implicit_diagdom(nz2, b2, phi, 0.7)
+    d0 = np.diag(Ad)
+    dm = np.maximum(np.abs(d0), np.abs(Ad).sum(1) - np.abs(d0)) / 0.7
+    assert np.allclose(np.diag(dense(om, nz2)), dm, rtol=1e-14)
+    assert np.allclose(b2, b + (dm - d0) * phi, rtol=1e-12)
+    nz3, b3 = nz.copy(), b.copy()
+    om.set_reference(nz3, b3, 2.0, 0)
+    assert b3[0] == d0[0] * 2.0 and dense(om, nz3)[0, 0] == 2 * d0[0]
+
+
+def test_cavity_simple_physics(x, A, O, meshes):
+    """40x40 lid-driven cavity at Re=100: residuals fall and the u-velocity on
+    the vertical centreline reaches the Ghia et al. (1982) minimum of about
+    -0.21 near y = 0.45 (first-order upwind on 40x40 gives about -0.19)."""
+    api = x.api
+    m = meshes("cavity40")
+    model, config = cavity_case(api, m, iterations=400)
+    setup = api.simple_setup(model, config, pref=0.0)
+    out = O.OracleMesh(m, A).simple_run(setup, model.momentum.U, model.momentum.p, 400)
+    r = out["residuals"]
+    assert r[0, -1] < 1e-3 * r[0, 0] and r[3, -1] < 1e-2 * r[3, 0]
+    cc = m.cell_centre.reshape(-1, 3)
+    col = np.abs(cc[:, 0] - 0.5 - 1 / 80) < 1e-9
+    u = out["U"][0][col]
+    assert -0.24 < u.min() < -0.15
+    assert 0.35 < cc[col][np.argmin(u), 1] < 0.55
+
+
+def test_channel_simple_mass_conservation(x, A, O, meshes):
+    """Developing laminar flow in the jittered box: the outlet mass flux
+    approaches the inlet's as SIMPLE converges (the corrected flux uses the
+    under-relaxed pressure, so the balance closes only at convergence) and
+    the velocity stays bounded."""
+    api = x.api
+    m = meshes("box12")
+    om = O.OracleMesh(m, A)
+    pr = m.patch_range.reshape(-1, 2)
+    imbalance = []
+    for its in (30, 150):
+        model, config = channel_case(api, m, iterations=its)
+        out = om.simple_run(api.simple_setup(model, config), model.momentum.U,
+                            model.momentum.p, its)
+        md = out["mdotf"]
+        inflow = md[pr[0, 0] - 1:pr[0, 1]].sum()
+        outflow = md[pr[1, 0] - 1:pr[1, 1]].sum()
+        assert np.isclose(inflow, -1.0, rtol=1e-12)          # Dirichlet U=1 on a unit inlet
+        imbalance.append(abs(outflow + inflow))
+        assert np.abs(out["U"]).max() < 3.0
+    assert imbalance[1] < 2e-3 and imbalance[1] < imbalance[0] / 3

@@ -1,0 +1,277 @@
+"""Host-side mirror of XCALibre's user API for the SIMPLE path.
+
+The names, keyword arguments, defaults and error behaviour follow the
+reference so that a case script reads the same:
+
+    mesh = block_mesh_2d(...)                      # or FOAM3D_mesh / UNV2D_mesh
+    model = Physics(time=Steady(), fluid=Fluid(Incompressible, nu=0.01),
+                    turbulence=RANS(Laminar), energy=Energy(Isothermal),
+                    domain=mesh)
+    BCs = assign(region=mesh, U=[Wall("top", [1, 0, 0]), ...], p=[...])
+    solvers = dict(U=SolverSetup(solver=Bicgstab(), preconditioner=DILU(),
+                                 convergence=1e-7, relax=0.7), p=...)
+    schemes = dict(U=Schemes(divergence=Upwind), p=Schemes())
+    config = Configuration(solvers=solvers, schemes=schemes,
+                           runtime=Runtime(iterations=200, write_interval=-1,
+                                           time_step=1),
+                           hardware=Hardware(backend=HIPBackend(), workgroup=256),
+                           boundaries=BCs)
+    initialise(model.momentum.U, [0, 0, 0])
+    residuals = run(model, config, pref=0.0)       # run!(model, config; pref)
+
+References: Physics (ModelPhysics/0_type_definition.jl:196-229), assign
+(Discretise_4_assign_boundaries.jl:5-49), SolverSetup/Runtime/Schemes
+(Solve_1_api.jl:8-211), Configuration/Hardware (Simulate_0_types.jl:32-73),
+run! (Solvers_3_solver_dispatch.jl:120-135), residual NamedTuple
+(Solvers_1_SIMPLE.jl:262).  Execution always goes through libxcalhip.so.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from typing import Any, Optional
+
+import numpy as np
+
+from . import _abi as A
+from . import hip
+
+
+# ------------------------------------------------------------- schemes
+class SteadyState: ...
+class Euler: ...
+class Linear: ...
+class Upwind: ...
+class LUST: ...
+class BoundedUpwind: ...
+class Gauss: ...
+class Midpoint: ...
+
+
+_DIV = {Linear: A.DIV_LINEAR, Upwind: A.DIV_UPWIND, LUST: A.DIV_LUST,
+        BoundedUpwind: A.DIV_BOUNDED_UPWIND}
+
+
+@dataclass
+class Schemes:
+    time: Any = SteadyState
+    divergence: Any = Linear
+    laplacian: Any = Linear
+    gradient: Any = Gauss
+    limiter: Any = None
+
+
+# ------------------------------------------------------- solver setup
+class Cg: ...
+class Bicgstab: ...
+class Jacobi: ...
+class DILU: ...
+
+
+@dataclass
+class SolverSetup:
+    solver: Any
+    preconditioner: Any
+    convergence: float
+    relax: float
+    float_type: Any = np.float64
+    smoother: Any = None
+    limit: Any = None
+    itmax: int = 1000
+    atol: Optional[float] = None
+    rtol: float = 0.1
+    precision: str = "fp64"      # "fp64" or "fp32-ir" (FP32 inner Krylov, FP64 refinement)
+    inner_rtol: float = 1e-2     # fp32-ir: relative tolerance of each inner solve
+
+    def __post_init__(self):
+        if self.atol is None:
+            self.atol = float(np.finfo(self.float_type).eps) ** 0.9
+        if self.smoother is not None:
+            raise NotImplementedError("smoothers are not part of the device path")
+
+    def c(self) -> A.SolverSetupC:
+        solver = {Cg: A.SOLVER_CG, Bicgstab: A.SOLVER_BICGSTAB}.get(type(self.solver))
+        if solver is None:
+            raise NotImplementedError(f"solver {self.solver!r} is not supported")
+        pre = {Jacobi: A.PRECON_JACOBI, DILU: A.PRECON_DILU, type(None): A.PRECON_NONE}.get(
+            type(self.preconditioner))
+        if pre is None:
+            raise NotImplementedError(f"preconditioner {self.preconditioner!r} is not supported")
+        prec = {"fp64": A.PREC_FP64, "fp32-ir": A.PREC_FP32_IR}[self.precision]
+        return A.SolverSetupC(solver, pre, prec, int(self.itmax), float(self.convergence),
+                              float(self.relax), float(self.atol), float(self.rtol),
+                              float(self.inner_rtol))
+
+
+@dataclass
+class Runtime:
+    iterations: int
+    write_interval: int
+    time_step: float = 1.0
+
+
+class HIPBackend:
+    """The gfx950 device backend (sibling of CPU()/ROCBackend())."""
+
+    def __init__(self, device: int = 0):
+        self.device = device
+
+
+@dataclass
+class Hardware:
+    backend: Any
+    workgroup: int = 256
+
+
+@dataclass
+class Configuration:
+    solvers: dict
+    schemes: dict
+    runtime: Runtime
+    hardware: Hardware
+    boundaries: dict
+    postprocess: Any = None
+
+
+# ------------------------------------------------------------- physics
+class Steady: ...
+class Incompressible: ...
+class Laminar: ...
+class Isothermal: ...
+
+
+@dataclass
+class Fluid:
+    kind: Any = Incompressible
+    nu: float = 0.0
+
+
+def RANS(kind=Laminar):
+    if kind is not Laminar:
+        raise NotImplementedError("only RANS{Laminar} is on the device path")
+    return kind()
+
+
+def Energy(kind=Isothermal):
+    return kind()
+
+
+@dataclass
+class Momentum:
+    U: np.ndarray
+    p: np.ndarray
+
+
+@dataclass
+class Physics:
+    domain: Any
+    fluid: Fluid
+    time: Any = field(default_factory=Steady)
+    turbulence: Any = field(default_factory=Laminar)
+    energy: Any = field(default_factory=Isothermal)
+    momentum: Momentum = None
+
+    def __post_init__(self):
+        n = self.domain.n_cells
+        self.momentum = Momentum(np.zeros((3, n)), np.zeros(n))
+
+
+def initialise(field_values: np.ndarray, value):
+    """initialise!(field, value) (Fields_0_types.jl:376-402)"""
+    if field_values.ndim == 2:
+        v = np.asarray(value, dtype=np.float64)
+        if v.shape != (3,):
+            raise ValueError("Vectors must have 3 components")
+        field_values[:] = v[:, None]
+    else:
+        field_values[:] = float(value)
+
+
+# -------------------------------------------------- boundary conditions
+@dataclass
+class _BC:
+    ID: str
+    value: Any = 0.0
+    kind: int = -1
+
+
+def Dirichlet(name, value): return _BC(name, value, A.BC_DIRICHLET)
+def Neumann(name, value=0.0): return _BC(name, value, A.BC_NEUMANN)
+def Zerogradient(name): return _BC(name, 0.0, A.BC_ZEROGRADIENT)
+def Extrapolated(name): return _BC(name, 0.0, A.BC_EXTRAPOLATED)
+def Wall(name, value=0.0): return _BC(name, value, A.BC_WALL)
+def Symmetry(name): return _BC(name, 0.0, A.BC_SYMMETRY)
+
+
+def assign(region, **fields):
+    """assign(region, (U=[...], p=[...])): resolve patch names, require one BC
+    per patch (Discretise_4_assign_boundaries.jl:5-49)."""
+    out = {}
+    for name, bcs in fields.items():
+        resolved = []
+        for bc in bcs:
+            patch = region.patch_index(bc.ID)  # KeyError: not a recognised boundary name
+            v = bc.value
+            if np.ndim(v):
+                if len(v) != 3:
+                    raise ValueError("Vectors must have 3 components")
+                v = [float(x) for x in v]
+            else:
+                v = float(v)
+            resolved.append((bc.kind, patch, v))
+        if len(resolved) != len(region.patch_names):
+            raise AssertionError(
+                f"Inconsistent number of boundaries assigned to field {name}")
+        out[name] = resolved
+    return out
+
+
+def simple_setup(model: Physics, config: Configuration, pref=None) -> A.SimpleSetupC:
+    """xcal_simple_setup for run!(model, config; pref)."""
+    if type(model.turbulence) is not Laminar:
+        raise NotImplementedError("only laminar SIMPLE is on the device path")
+    sU, sp = config.schemes["U"], config.schemes["p"]
+    for s in (sU, sp):
+        if s.time is not SteadyState:
+            raise NotImplementedError("SIMPLE is a steady solver (Time{SteadyState})")
+        if s.gradient is not Gauss or s.limiter is not None or s.laplacian is not Linear:
+            raise NotImplementedError("device path: Gauss gradient, Linear laplacian, no limiter")
+    div = _DIV.get(sU.divergence)
+    if div is None:
+        raise NotImplementedError(f"divergence scheme {sU.divergence!r}")
+    bU = A.bc_array(config.boundaries["U"])
+    bp = A.bc_array(config.boundaries["p"])
+    setup = A.SimpleSetupC(float(model.fluid.nu), div, int(pref is not None),
+                           float(pref or 0.0), config.solvers["U"].c(), config.solvers["p"].c(),
+                           bU, bp, len(config.boundaries["U"]))
+    setup._keep = (bU, bp)
+    return setup
+
+
+_ctx_cache = {}
+
+
+def _context(device: int) -> hip.Context:
+    if device not in _ctx_cache:
+        _ctx_cache[device] = hip.Context(device)
+    return _ctx_cache[device]
+
+
+def run(model: Physics, config: Configuration, pref=None, ncorrectors: int = 0,
+        inner_loops: int = 0):
+    """run!(model, config; pref, ncorrectors) for steady incompressible
+    laminar SIMPLE.  Returns the residual histories (Ux, Uy, Uz, p) and
+    leaves the converged fields in model.momentum."""
+    if ncorrectors != 0:
+        raise NotImplementedError("non-orthogonal correctors are not on the device path")
+    if not isinstance(config.hardware.backend, HIPBackend):
+        raise TypeError("the device path runs on HIPBackend()")
+    ctx = _context(config.hardware.backend.device)
+    dmesh = hip.DeviceMesh(ctx, model.domain)
+    sim = hip.Simple(dmesh, simple_setup(model, config, pref))
+    sim.set_U(model.momentum.U)
+    sim.set_p(model.momentum.p)
+    out = sim.run(config.runtime.iterations)
+    model.momentum.U[:] = sim.get("U")
+    model.momentum.p[:] = sim.get("p")
+    r = out["residuals"]
+    return dict(Ux=r[0], Uy=r[1], Uz=r[2], p=r[3])

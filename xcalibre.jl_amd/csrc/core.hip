@@ -1,0 +1,429 @@
+// Context, mesh upload and fields of libxcalhip.so.
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <numeric>
+
+#include "xcal_internal.h"
+
+namespace xcal {
+
+thread_local std::string g_last_error;
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+template <class T>
+static void upload(DBuf<T>& d, const std::vector<T>& h, int64_t& bytes) {
+  d.alloc((int64_t)h.size());
+  if (!h.empty()) XCAL_HIP(hipMemcpy(d.p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice));
+  bytes += d.bytes();
+}
+
+TermsK make_terms(const xcal_term* terms, int n) {
+  XCAL_CHECK(n >= 1 && n <= kMaxTerms, "number of operator terms must be 1..4");
+  TermsK t{};
+  t.n = n;
+  for (int i = 0; i < n; ++i) {
+    XCAL_CHECK(terms[i].type >= XCAL_TIME_STEADY && terms[i].type <= XCAL_SI, "unknown operator type");
+    XCAL_CHECK(terms[i].sign == 1 || terms[i].sign == -1, "operator sign must be +1 or -1");
+    t.t[i].type = terms[i].type;
+    t.t[i].sign = terms[i].sign;
+    t.t[i].flux = terms[i].flux ? terms[i].flux->comp(0) : nullptr;
+    t.t[i].flux_const = terms[i].flux_const;
+  }
+  return t;
+}
+
+BCsK make_bcs(const xcal_mesh* m, const xcal_bc* bcs, int n) {
+  XCAL_CHECK(m->n_patches <= kMaxPatches, "mesh has more than 32 patches");
+  BCsK b{};
+  b.npatch = m->n_patches;
+  for (int p = 0; p < kMaxPatches; ++p) b.p[p].kind = -1;
+  for (int i = 0; i < n; ++i) {
+    XCAL_CHECK(bcs[i].patch >= 0 && bcs[i].patch < m->n_patches, "BC patch index out of range");
+    XCAL_CHECK(bcs[i].kind >= XCAL_BC_DIRICHLET && bcs[i].kind <= XCAL_BC_SYMMETRY,
+               "unknown boundary condition kind");
+    // the reference applies the first BC whose range holds the face
+    if (b.p[bcs[i].patch].kind != -1) continue;
+    b.p[bcs[i].patch].kind = bcs[i].kind;
+    for (int k = 0; k < 3; ++k) b.p[bcs[i].patch].v[k] = bcs[i].value[k];
+  }
+  return b;
+}
+
+xcal_field* check_field(const xcal_field* f, const xcal_mesh* m, int kind, const char* what) {
+  XCAL_CHECK(f != nullptr, std::string(what) + ": field is NULL");
+  XCAL_CHECK(f->mesh == m, std::string(what) + ": field belongs to another mesh");
+  XCAL_CHECK(f->kind == kind, std::string(what) + ": wrong field kind");
+  return const_cast<xcal_field*>(f);
+}
+
+__global__ void k_fill(double* p, int64_t n, double v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+void fill(hipStream_t s, double* p, int64_t n, double v) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_fill, dim3(grid_for(n)), dim3(kBlock), 0, s, p, n, v);
+}
+
+}  // namespace xcal
+
+using namespace xcal;
+
+extern "C" {
+
+const char* xcal_last_error(void) { return g_last_error.c_str(); }
+int32_t xcal_abi_version(void) { return 1; }
+
+int xcal_ctx_create(int32_t device, xcal_ctx** out) {
+  return guarded([&] {
+    XCAL_CHECK(out != nullptr, "out is NULL");
+    int ndev = 0;
+    XCAL_HIP(hipGetDeviceCount(&ndev));
+    XCAL_CHECK(device >= 0 && device < ndev, "device index out of range");
+    XCAL_HIP(hipSetDevice(device));
+    auto* c = new xcal_ctx();
+    c->device = device;
+    try {
+      XCAL_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+      c->red_partial.alloc((int64_t)kRedBlocks * 8);
+      c->red_ticket.alloc(8);
+      c->red_result.alloc(64);
+      XCAL_HIP(hipMemset(c->red_ticket.p, 0, sizeof(unsigned) * 8));
+      XCAL_HIP(hipHostMalloc((void**)&c->host_pinned, sizeof(double) * 64, hipHostMallocDefault));
+    } catch (...) {
+      if (c->stream) (void)hipStreamDestroy(c->stream);
+      if (c->host_pinned) (void)hipHostFree(c->host_pinned);
+      delete c;
+      throw;
+    }
+    *out = c;
+    return 0;
+  });
+}
+
+int xcal_ctx_destroy(xcal_ctx* c) {
+  return guarded([&] {
+    if (!c) return 0;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipStreamDestroy(c->stream);
+    if (c->host_pinned) (void)hipHostFree(c->host_pinned);
+    delete c;
+    return 0;
+  });
+}
+
+int xcal_ctx_synchronize(xcal_ctx* c) {
+  return guarded([&] {
+    XCAL_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+  });
+}
+
+int xcal_mesh_create(xcal_ctx* ctx, const xcal_mesh_host* h, xcal_mesh** out) {
+  return guarded([&] {
+    XCAL_CHECK(ctx && h && out, "NULL argument");
+    XCAL_CHECK(h->dim == 2 || h->dim == 3, "dim must be 2 or 3");
+    XCAL_CHECK(h->n_cells > 0 && h->n_faces > 0, "empty mesh");
+    XCAL_CHECK(h->n_cells < (1ll << 30) && h->n_faces < (1ll << 30), "mesh too large for int32 indices");
+    XCAL_CHECK(h->n_bfaces >= 0 && h->n_bfaces <= h->n_faces, "bad n_bfaces");
+    XCAL_HIP(hipSetDevice(ctx->device));
+    auto m = std::make_unique<xcal_mesh>();
+    m->ctx = ctx;
+    m->dim = h->dim;
+    const int64_t n = h->n_cells, nf = h->n_faces, nbf = h->n_bfaces;
+    m->n = n;
+    m->nf = nf;
+    m->nbf = nbf;
+    m->n_patches = h->n_patches;
+    XCAL_CHECK(h->n_patches <= kMaxPatches, "mesh has more than 32 patches");
+    for (int p = 0; p < h->n_patches; ++p) {
+      int64_t a = h->patch_range[2 * p] - 1, b = h->patch_range[2 * p + 1] - 1;
+      XCAL_CHECK(a >= 0 && b < nbf && a <= b + 1, "patch range outside boundary faces");
+      m->patch_first.push_back(a);
+      m->patch_last.push_back(b);
+    }
+    int64_t bytes = 0;
+    // cell -> internal face lists in the reference's faces_range order
+    std::vector<int32_t> cf_ptr(n + 1, 0), cf_face, cf_nbr;
+    std::vector<int8_t> cf_sign;
+    cf_face.reserve(h->n_cell_faces);
+    cf_nbr.reserve(h->n_cell_faces);
+    cf_sign.reserve(h->n_cell_faces);
+    for (int64_t c = 0; c < n; ++c) {
+      int64_t a = h->cell_faces_range[2 * c], b = h->cell_faces_range[2 * c + 1];
+      for (int64_t k = a; k <= b; ++k) {
+        XCAL_CHECK(k >= 1 && k <= h->n_cell_faces, "faces_range outside cell_faces");
+        int64_t f = h->cell_faces[k - 1] - 1, nb = h->cell_neighbours[k - 1] - 1;
+        XCAL_CHECK(f >= nbf && f < nf, "cell face is not an internal face");
+        XCAL_CHECK(nb >= 0 && nb < n, "neighbour index out of range");
+        XCAL_CHECK(h->cell_nsign[k - 1] == 1 || h->cell_nsign[k - 1] == -1, "nsign must be +-1");
+        cf_face.push_back((int32_t)f);
+        cf_nbr.push_back((int32_t)nb);
+        cf_sign.push_back((int8_t)h->cell_nsign[k - 1]);
+      }
+      cf_ptr[c + 1] = (int32_t)cf_face.size();
+    }
+    m->ncf = (int64_t)cf_face.size();
+    // CSR pattern: row i = {i} U neighbours(i), ascending, unique
+    // (sparse_matrix_connectivity + sparsecsr, ModelFramework_0_types.jl:223-242)
+    std::vector<int32_t> rowptr(n + 1, 0), colval;
+    colval.reserve(n + cf_face.size());
+    std::vector<int32_t> row;
+    for (int64_t c = 0; c < n; ++c) {
+      row.clear();
+      row.push_back((int32_t)c);
+      for (int32_t k = cf_ptr[c]; k < cf_ptr[c + 1]; ++k) row.push_back(cf_nbr[k]);
+      std::sort(row.begin(), row.end());
+      row.erase(std::unique(row.begin(), row.end()), row.end());
+      colval.insert(colval.end(), row.begin(), row.end());
+      rowptr[c + 1] = (int32_t)colval.size();
+    }
+    const int64_t nnz = (int64_t)colval.size();
+    m->nnz = nnz;
+    auto find = [&](int64_t i, int32_t j) -> int32_t {
+      const int32_t* b = colval.data() + rowptr[i];
+      const int32_t* e = colval.data() + rowptr[i + 1];
+      const int32_t* it = std::lower_bound(b, e, j);
+      return (it != e && *it == j) ? (int32_t)(it - colval.data()) : -1;
+    };
+    std::vector<int32_t> diag(n), cf_slot(cf_face.size()), trans(nnz);
+    for (int64_t c = 0; c < n; ++c) {
+      diag[c] = find(c, (int32_t)c);
+      for (int32_t k = cf_ptr[c]; k < cf_ptr[c + 1]; ++k) cf_slot[k] = find(c, cf_nbr[k]);
+      for (int32_t k = rowptr[c]; k < rowptr[c + 1]; ++k) trans[k] = find(colval[k], (int32_t)c);
+    }
+    // faces
+    std::vector<int32_t> own(nf), nei(nf);
+    std::vector<double> area(h->face_area, h->face_area + nf), weight(h->face_weight, h->face_weight + nf),
+        delta(h->face_delta, h->face_delta + nf), nx(nf), ny(nf), nz(nf), ex(nf), ey(nf), ez(nf),
+        sfx(nf), sfy(nf), sfz(nf), lapEf(nf);
+    for (int64_t f = 0; f < nf; ++f) {
+      own[f] = (int32_t)(h->face_owner[2 * f] - 1);
+      nei[f] = (int32_t)(h->face_owner[2 * f + 1] - 1);
+      XCAL_CHECK(own[f] >= 0 && own[f] < n && nei[f] >= 0 && nei[f] < n, "ownerCells out of range");
+      nx[f] = h->face_normal[3 * f];
+      ny[f] = h->face_normal[3 * f + 1];
+      nz[f] = h->face_normal[3 * f + 2];
+      ex[f] = h->face_e[3 * f];
+      ey[f] = h->face_e[3 * f + 1];
+      ez[f] = h->face_e[3 * f + 2];
+      // Sf = area*normal (flux!, green_gauss!), same rounding as the reference
+      sfx[f] = area[f] * nx[f];
+      sfy[f] = area[f] * ny[f];
+      sfz[f] = area[f] * nz[f];
+      // Laplacian{Linear} |E_f| (Discretise_1_schemes.jl:74-88); independent
+      // of the visiting side's nsign because every negation there is exact
+      double s0 = area[f] * nx[f], s1 = area[f] * ny[f], s2 = area[f] * nz[f];
+      double cfac = (s0 * s0 + s1 * s1 + s2 * s2) / (s0 * ex[f] + s1 * ey[f] + s2 * ez[f]);
+      double e0 = cfac * ex[f], e1 = cfac * ey[f], e2 = cfac * ez[f];
+      lapEf[f] = std::sqrt(e0 * e0 + e1 * e1 + e2 * e2);
+    }
+    std::vector<int32_t> fslot(nf, -1);
+    for (int64_t f = nbf; f < nf; ++f) fslot[f] = find(own[f], nei[f]);
+    // boundary faces grouped per cell, ascending face id
+    std::vector<int8_t> bpatch(nbf, -1);
+    for (int p = 0; p < h->n_patches; ++p)
+      for (int64_t f = m->patch_first[p]; f <= m->patch_last[p]; ++f)
+        if (bpatch[f] < 0) bpatch[f] = (int8_t)p;
+    std::vector<int32_t> bcount(n, 0);
+    for (int64_t f = 0; f < nbf; ++f) {
+      int64_t c = h->boundary_cellsID[f] - 1;
+      XCAL_CHECK(c >= 0 && c < n, "boundary_cellsID out of range");
+      bcount[c]++;
+    }
+    std::vector<int32_t> bc_cells, bc_ptr(1, 0), bc_face;
+    std::vector<int32_t> bstart(n, -1);
+    for (int64_t c = 0; c < n; ++c)
+      if (bcount[c] > 0) {
+        bstart[c] = (int32_t)bc_cells.size();
+        bc_cells.push_back((int32_t)c);
+        bc_ptr.push_back(bc_ptr.back() + bcount[c]);
+      }
+    bc_face.assign(nbf, 0);
+    {
+      std::vector<int32_t> cur(bc_cells.size(), 0);
+      for (int64_t f = 0; f < nbf; ++f) {
+        int64_t c = h->boundary_cellsID[f] - 1;
+        int32_t s = bstart[c];
+        bc_face[bc_ptr[s] + cur[s]++] = (int32_t)f;
+      }
+    }
+    m->nbcell = (int64_t)bc_cells.size();
+    // DILU level schedules: forward level(i) = 1 + max level of lower
+    // neighbours; backward symmetric over upper neighbours.  Rows inside a
+    // level are independent; level order preserves every i<j dependency.
+    std::vector<int32_t> lev(n, 0);
+    int maxl = 0;
+    for (int64_t i = 0; i < n; ++i) {
+      int l = 0;
+      for (int32_t k = rowptr[i]; k < diag[i]; ++k) l = std::max(l, lev[colval[k]] + 1);
+      lev[i] = l;
+      maxl = std::max(maxl, l);
+    }
+    auto bucket = [&](int nl, std::vector<int32_t>& ptr, std::vector<int32_t>& rows, bool asc) {
+      ptr.assign(nl + 1, 0);
+      for (int64_t i = 0; i < n; ++i) ptr[lev[i] + 1]++;
+      for (int l = 0; l < nl; ++l) ptr[l + 1] += ptr[l];
+      rows.assign(n, 0);
+      std::vector<int32_t> cur(ptr.begin(), ptr.end() - 1);
+      if (asc)
+        for (int64_t i = 0; i < n; ++i) rows[cur[lev[i]]++] = (int32_t)i;
+      else
+        for (int64_t i = n - 1; i >= 0; --i) rows[cur[lev[i]]++] = (int32_t)i;
+    };
+    std::vector<int32_t> rows_f, rows_b;
+    m->nlev_f = maxl + 1;
+    bucket(m->nlev_f, m->lev_ptr_f, rows_f, true);
+    maxl = 0;
+    for (int64_t i = n - 1; i >= 0; --i) {
+      int l = 0;
+      for (int32_t k = diag[i] + 1; k < rowptr[i + 1]; ++k) l = std::max(l, lev[colval[k]] + 1);
+      lev[i] = l;
+      maxl = std::max(maxl, l);
+    }
+    m->nlev_b = maxl + 1;
+    bucket(m->nlev_b, m->lev_ptr_b, rows_b, false);
+
+    std::vector<double> vol(h->cell_volume, h->cell_volume + n);
+    upload(m->vol, vol, bytes);
+    upload(m->cf_ptr, cf_ptr, bytes);
+    upload(m->cf_face, cf_face, bytes);
+    upload(m->cf_nbr, cf_nbr, bytes);
+    upload(m->cf_slot, cf_slot, bytes);
+    upload(m->cf_sign, cf_sign, bytes);
+    upload(m->diag, diag, bytes);
+    upload(m->own, own, bytes);
+    upload(m->nei, nei, bytes);
+    upload(m->fslot, fslot, bytes);
+    upload(m->area, area, bytes);
+    upload(m->weight, weight, bytes);
+    upload(m->delta, delta, bytes);
+    upload(m->nx, nx, bytes);
+    upload(m->ny, ny, bytes);
+    upload(m->nz, nz, bytes);
+    upload(m->ex, ex, bytes);
+    upload(m->ey, ey, bytes);
+    upload(m->ez, ez, bytes);
+    upload(m->sfx, sfx, bytes);
+    upload(m->sfy, sfy, bytes);
+    upload(m->sfz, sfz, bytes);
+    upload(m->lapEf, lapEf, bytes);
+    upload(m->bc_cells, bc_cells, bytes);
+    upload(m->bc_ptr, bc_ptr, bytes);
+    upload(m->bc_face, bc_face, bytes);
+    upload(m->bface_patch, bpatch, bytes);
+    upload(m->rowptr, rowptr, bytes);
+    upload(m->colval, colval, bytes);
+    upload(m->trans, trans, bytes);
+    upload(m->lev_rows_f, rows_f, bytes);
+    upload(m->lev_rows_b, rows_b, bytes);
+    m->h_rowptr = std::move(rowptr);
+    m->h_colval = std::move(colval);
+    m->device_bytes = bytes;
+    *out = m.release();
+    return 0;
+  });
+}
+
+int xcal_mesh_destroy(xcal_mesh* m) {
+  return guarded([&] {
+    if (m) {
+      (void)hipSetDevice(m->ctx->device);
+      (void)hipStreamSynchronize(m->ctx->stream);
+      delete m;
+    }
+    return 0;
+  });
+}
+
+int xcal_mesh_info(const xcal_mesh* m, int64_t* n_cells, int64_t* nnz, int32_t* fl, int32_t* bl,
+                   int64_t* bytes) {
+  return guarded([&] {
+    XCAL_CHECK(m, "mesh is NULL");
+    if (n_cells) *n_cells = m->n;
+    if (nnz) *nnz = m->nnz;
+    if (fl) *fl = m->nlev_f;
+    if (bl) *bl = m->nlev_b;
+    if (bytes) *bytes = m->device_bytes;
+    return 0;
+  });
+}
+
+int xcal_mesh_csr_pattern(const xcal_mesh* m, int32_t* rowptr, int32_t* colval) {
+  return guarded([&] {
+    XCAL_CHECK(m && rowptr && colval, "NULL argument");
+    std::copy(m->h_rowptr.begin(), m->h_rowptr.end(), rowptr);
+    std::copy(m->h_colval.begin(), m->h_colval.end(), colval);
+    return 0;
+  });
+}
+
+int xcal_field_create(xcal_mesh* m, int32_t kind, xcal_field** out) {
+  return guarded([&] {
+    XCAL_CHECK(m && out, "NULL argument");
+    XCAL_HIP(hipSetDevice(m->ctx->device));
+    auto f = std::make_unique<xcal_field>();
+    f->mesh = m;
+    f->kind = kind;
+    switch (kind) {
+      case XCAL_CELL_SCALAR: f->ncomp = 1; f->len = m->n; break;
+      case XCAL_CELL_VECTOR: f->ncomp = 3; f->len = m->n; break;
+      case XCAL_FACE_SCALAR: f->ncomp = 1; f->len = m->nf; break;
+      case XCAL_FACE_VECTOR: f->ncomp = 3; f->len = m->nf; break;
+      case XCAL_CELL_TENSOR: f->ncomp = 9; f->len = m->n; break;
+      default: throw Error(XCAL_ERR_ARG, "unknown field kind");
+    }
+    f->data.alloc((int64_t)f->ncomp * f->len);
+    XCAL_HIP(hipMemsetAsync(f->data.p, 0, f->data.bytes(), m->ctx->stream));
+    XCAL_HIP(hipStreamSynchronize(m->ctx->stream));
+    *out = f.release();
+    return 0;
+  });
+}
+
+int xcal_field_destroy(xcal_field* f) {
+  return guarded([&] {
+    if (f) {
+      (void)hipStreamSynchronize(f->mesh->ctx->stream);
+      delete f;
+    }
+    return 0;
+  });
+}
+
+int xcal_field_upload(xcal_field* f, int32_t c, const double* host) {
+  return guarded([&] {
+    XCAL_CHECK(f && host, "NULL argument");
+    XCAL_CHECK(c >= 0 && c < f->ncomp, "component out of range");
+    XCAL_HIP(hipMemcpyAsync(f->comp(c), host, sizeof(double) * f->len, hipMemcpyHostToDevice,
+                            f->mesh->ctx->stream));
+    XCAL_HIP(hipStreamSynchronize(f->mesh->ctx->stream));
+    return 0;
+  });
+}
+
+int xcal_field_download(const xcal_field* f, int32_t c, double* host) {
+  return guarded([&] {
+    XCAL_CHECK(f && host, "NULL argument");
+    XCAL_CHECK(c >= 0 && c < f->ncomp, "component out of range");
+    XCAL_HIP(hipMemcpyAsync(host, f->comp(c), sizeof(double) * f->len, hipMemcpyDeviceToHost,
+                            f->mesh->ctx->stream));
+    XCAL_HIP(hipStreamSynchronize(f->mesh->ctx->stream));
+    return 0;
+  });
+}
+
+int xcal_field_fill(xcal_field* f, int32_t c, double v) {
+  return guarded([&] {
+    XCAL_CHECK(f, "NULL argument");
+    XCAL_CHECK(c >= 0 && c < f->ncomp, "component out of range");
+    fill(f->mesh->ctx->stream, f->comp(c), f->len, v);
+    XCAL_HIP(hipGetLastError());
+    return 0;
+  });
+}
+
+}  // extern "C"

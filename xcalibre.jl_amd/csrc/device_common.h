@@ -1,0 +1,263 @@
+// Device-side views and the per-face coefficient functions shared by the
+// assembly kernels.  Every expression keeps the reference's evaluation order
+// (the library is built with -ffp-contract=off) so that results match the
+// oracle bit for bit where the reference's order is defined.
+#pragma once
+
+#include "xcal_internal.h"
+
+namespace xcal {
+
+struct MeshK {
+  int64_t n, nf, nbf;
+  const double* vol;
+  const int32_t *cf_ptr, *cf_face, *cf_nbr, *cf_slot;
+  const int8_t* cf_sign;
+  const int32_t* diag;
+  const int32_t *own, *nei, *fslot;
+  const double *area, *weight, *delta, *nx, *ny, *nz, *ex, *ey, *ez, *sfx, *sfy, *sfz, *lapEf;
+  int64_t nbcell;
+  const int32_t *bc_cells, *bc_ptr, *bc_face;
+  const int8_t* bface_patch;
+  const int32_t *rowptr, *colval, *trans;
+};
+
+inline MeshK meshk(const xcal_mesh* m) {
+  MeshK k;
+  k.n = m->n;
+  k.nf = m->nf;
+  k.nbf = m->nbf;
+  k.vol = m->vol.p;
+  k.cf_ptr = m->cf_ptr.p;
+  k.cf_face = m->cf_face.p;
+  k.cf_nbr = m->cf_nbr.p;
+  k.cf_slot = m->cf_slot.p;
+  k.cf_sign = m->cf_sign.p;
+  k.diag = m->diag.p;
+  k.own = m->own.p;
+  k.nei = m->nei.p;
+  k.fslot = m->fslot.p;
+  k.area = m->area.p;
+  k.weight = m->weight.p;
+  k.delta = m->delta.p;
+  k.nx = m->nx.p;
+  k.ny = m->ny.p;
+  k.nz = m->nz.p;
+  k.ex = m->ex.p;
+  k.ey = m->ey.p;
+  k.ez = m->ez.p;
+  k.sfx = m->sfx.p;
+  k.sfy = m->sfy.p;
+  k.sfz = m->sfz.p;
+  k.lapEf = m->lapEf.p;
+  k.nbcell = m->nbcell;
+  k.bc_cells = m->bc_cells.p;
+  k.bc_ptr = m->bc_ptr.p;
+  k.bc_face = m->bc_face.p;
+  k.bface_patch = m->bface_patch.p;
+  k.rowptr = m->rowptr.p;
+  k.colval = m->colval.p;
+  k.trans = m->trans.p;
+  return k;
+}
+
+// Julia's max for Float64: NaN-propagating, +0.0 preferred over -0.0
+__device__ __host__ inline double jmax(double a, double b) {
+  if (a != a || b != b) return __builtin_nan("");
+  if (a == b) return __builtin_signbit(a) ? b : a;
+  return a > b ? a : b;
+}
+
+__device__ inline double term_flux(const TermK& t, int64_t i) {
+  return t.flux ? t.flux[i] : t.flux_const;
+}
+
+// scheme! (Discretise_1_schemes.jl) for one internal face visit
+__device__ inline void scheme(const MeshK& m, const TermK& t, int f, double ns, double& ac,
+                              double& an) {
+  const double sign = (double)t.sign;
+  switch (t.type) {
+    case XCAL_LAPLACIAN_LINEAR: {  // :68-104
+      double ap = (sign * (term_flux(t, f) * m.lapEf[f])) / m.delta[f];
+      ac = -ap;
+      an = ap;
+      return;
+    }
+    case XCAL_DIV_LINEAR: {  // :114-129
+      double w = 0.5 + ns * (m.weight[f] - 0.5);
+      double ap = sign * (term_flux(t, f) * ns);
+      ac = ap * w;
+      an = ap * (1.0 - w);
+      return;
+    }
+    case XCAL_DIV_UPWIND: {  // :137-147
+      double ap = sign * (term_flux(t, f) * ns);
+      ac = jmax(ap, 0.0);
+      an = -jmax(-ap, 0.0);
+      return;
+    }
+    case XCAL_DIV_LUST: {  // :155-175
+      double w = m.weight[f];
+      if (ns < 0.0) w = 1.0 - w;
+      double ap = sign * (term_flux(t, f) * ns);
+      double acL = ap * w, anL = ap * (1.0 - w);
+      double acU = jmax(ap, 0.0), anU = -jmax(-ap, 0.0);
+      ac = 0.75 * acL + 0.25 * acU;
+      an = 0.75 * anL + 0.25 * anU;
+      return;
+    }
+    case XCAL_DIV_BOUNDED_UPWIND: {  // :183-195
+      double ap = sign * (term_flux(t, f) * ns);
+      ac = jmax(-ap, 0.0);
+      an = -jmax(-ap, 0.0);
+      return;
+    }
+    default:  // Time, Si: no face contribution
+      ac = 0.0;
+      an = 0.0;
+      return;
+  }
+}
+
+// scheme_source! (Discretise_1_schemes.jl) per cell
+__device__ inline void scheme_source(const MeshK& m, const TermK& t, int64_t c,
+                                     const double* prev, double dt, double& ac, double& b) {
+  if (t.type == XCAL_TIME_EULER) {  // :30-42
+    double vol_rdt = (term_flux(t, c) * m.vol[c]) / dt;
+    ac = vol_rdt;
+    b = prev[c] * vol_rdt;
+  } else if (t.type == XCAL_SI) {  // :208-218
+    ac = ((double)t.sign * term_flux(t, c)) * m.vol[c];
+    b = 0.0;
+  } else {
+    ac = 0.0;
+    b = 0.0;
+  }
+}
+
+// Boundary-condition functors (files under Discretise/boundary_conditions/).
+// comp < 0: scalar equation; 0..2: vector component.
+__device__ inline void bc_coeffs(const MeshK& m, const BCK& bc, const TermK& t, int f, int c,
+                                 int comp, const double* px, const double* py,
+                                 const double* pz, double& ap_out, double& bp_out) {
+  ap_out = 0.0;
+  bp_out = 0.0;
+  if (t.type == XCAL_TIME_STEADY || t.type == XCAL_TIME_EULER || t.type == XCAL_SI) return;
+  const double sign = (double)t.sign;
+  const bool vec = comp >= 0;
+  const double value = vec ? bc.v[comp] : bc.v[0];
+  const bool lap = t.type == XCAL_LAPLACIAN_LINEAR;
+  const bool divBU = t.type == XCAL_DIV_BOUNDED_UPWIND;
+  const bool divLUL = !lap && !divBU;
+  const double J = term_flux(t, f);
+  switch (bc.kind) {
+    case XCAL_BC_DIRICHLET:  // dirichlet.jl:19-99
+      if (lap) {
+        double flux = (J * m.area[f]) / m.delta[f];
+        double ap = sign * (-flux);
+        ap_out = ap;
+        bp_out = ap * value;
+      } else if (divLUL) {
+        double ap = sign * (-J);
+        bp_out = ap * value;
+      } else {
+        double ap = sign * J;
+        ap_out = jmax(-ap, 0.0);
+        double an = -jmax(-ap, 0.0);
+        bp_out = -an * value;
+      }
+      return;
+    case XCAL_BC_NEUMANN:  // neumann.jl:23-63 (scalar)
+      if (lap) {
+        bp_out = (J * m.area[f]) * value;
+      } else {
+        double ap = sign * J;
+        ap_out = ap;
+        bp_out = ((-value) * ap) * m.delta[f];
+      }
+      return;
+    case XCAL_BC_ZEROGRADIENT:  // zerogradient.jl:25-76
+      if (divLUL) ap_out = sign * J;
+      else if (divBU) ap_out = sign * J - J;
+      return;
+    case XCAL_BC_EXTRAPOLATED: {  // extrapolated.jl:21-72
+      if (lap) {
+        double flux = ((-J) * m.area[f]) / m.delta[f];
+        double ap = sign * flux;
+        const double* ph = vec ? (comp == 0 ? px : (comp == 1 ? py : pz)) : px;
+        ap_out = ap;
+        bp_out = ap * ph[c];
+      } else if (divLUL) {
+        ap_out = sign * J;
+      } else {
+        ap_out = sign * J - J;
+      }
+      return;
+    }
+    case XCAL_BC_WALL:  // wall.jl:26-120
+      if (vec) {
+        if (lap) {
+          double nx = m.nx[f], ny = m.ny[f], nz = m.nz[f];
+          double flux = (J * m.area[f]) / m.delta[f];
+          double ap = sign * (-flux);
+          double vcn = px[c] * nx + py[c] * ny + pz[c] * nz;
+          double vbn = bc.v[0] * nx + bc.v[1] * ny + bc.v[2] * nz;
+          double nk = comp == 0 ? nx : (comp == 1 ? ny : nz);
+          double vb_p = bc.v[comp] - vbn * nk;
+          double vc_n = vcn * nk;
+          ap_out = ap;
+          bp_out = ap * (vb_p + vc_n);
+        }
+      } else {
+        if (divLUL) {
+          ap_out = sign * J;
+        } else if (divBU) {
+          double ap = sign * J;
+          ap_out = jmax(-ap, 0.0);
+          double an = -jmax(-ap, 0.0);
+          bp_out = -an * value;
+        }
+      }
+      return;
+    case XCAL_BC_SYMMETRY:  // symmetry.jl:23-151
+      if (vec) {
+        double nx = m.nx[f], ny = m.ny[f], nz = m.nz[f];
+        double vcn = px[c] * nx + py[c] * ny + pz[c] * nz;
+        double nk = comp == 0 ? nx : (comp == 1 ? ny : nz);
+        double vck = comp == 0 ? px[c] : (comp == 1 ? py[c] : pz[c]);
+        double vn = vcn * nk;
+        if (lap) {
+          double flux = (J * m.area[f]) / m.delta[f];
+          double ap = sign * (-flux);
+          ap_out = ap;
+          bp_out = ap * (vck - vn);
+        } else {
+          double ap = sign * J;
+          ap_out = ap;
+          bp_out = ap * vn;
+        }
+      } else {
+        if (!lap) ap_out = sign * J;
+      }
+      return;
+    default:
+      return;
+  }
+}
+
+// Deterministic block-level sum (wave shuffles + LDS), result in thread 0.
+__device__ inline double block_sum(double v, double* lds) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) lds[wid] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0) {
+    const int nw = (blockDim.x + 63) >> 6;
+    for (int w = 0; w < nw; ++w) s += lds[w];
+  }
+  __syncthreads();
+  return s;
+}
+
+}  // namespace xcal

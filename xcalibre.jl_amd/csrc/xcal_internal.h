@@ -1,0 +1,225 @@
+// Internal declarations of libxcalhip.so (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "xcal.h"
+
+namespace xcal {
+
+// ------------------------------------------------------------------ errors
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+void set_error(const std::string& msg);
+
+#define XCAL_HIP(call)                                                               \
+  do {                                                                               \
+    hipError_t e_ = (call);                                                          \
+    if (e_ != hipSuccess)                                                            \
+      throw ::xcal::Error(XCAL_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+#define XCAL_CHECK(cond, msg)                                      \
+  do {                                                             \
+    if (!(cond)) throw ::xcal::Error(XCAL_ERR_ARG, (msg));         \
+  } while (0)
+
+// Runs fn, mapping exceptions to status codes (returns fn's int result).
+template <class Fn>
+int guarded(Fn&& fn) {
+  try {
+    return fn();
+  } catch (const Error& e) {
+    set_error(e.what());
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    set_error("host allocation failed");
+    return XCAL_ERR_NOMEM;
+  } catch (const std::exception& e) {
+    set_error(e.what());
+    return XCAL_ERR_ARG;
+  }
+}
+
+constexpr int kBlock = 256;
+constexpr int kMaxTerms = 4;
+constexpr int kMaxSources = 2;
+constexpr int kMaxPatches = 32;
+constexpr int kRedBlocks = 1024;  // partial-sum slots of one device reduction
+
+inline int grid_for(int64_t n, int block = kBlock) {
+  int64_t g = (n + block - 1) / block;
+  return (int)(g < 1 ? 1 : g);
+}
+
+// device buffer with RAII
+template <class T>
+struct DBuf {
+  T* p = nullptr;
+  int64_t n = 0;
+  DBuf() = default;
+  explicit DBuf(int64_t count) { alloc(count); }
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  DBuf(DBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+  DBuf& operator=(DBuf&& o) noexcept {
+    if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+    return *this;
+  }
+  void alloc(int64_t count) {
+    release();
+    n = count;
+    if (count > 0) XCAL_HIP(hipMalloc(&p, sizeof(T) * (size_t)count));
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  ~DBuf() { release(); }
+  int64_t bytes() const { return n * (int64_t)sizeof(T); }
+};
+
+}  // namespace xcal
+
+// ----------------------------------------------------------------- handles
+struct xcal_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  // scratch for device reductions: partial sums + tickets + results
+  xcal::DBuf<double> red_partial;   // kRedBlocks * 8 slots
+  xcal::DBuf<unsigned> red_ticket;  // 8 slots
+  xcal::DBuf<double> red_result;    // 64 slots
+  double* host_pinned = nullptr;    // 64 doubles, pinned
+};
+
+// Device mesh in SoA form, 0-based int32 indices, reference numbering.
+struct xcal_mesh {
+  xcal_ctx* ctx = nullptr;
+  int dim = 3;
+  int64_t n = 0, nf = 0, nbf = 0, nnz = 0, ncf = 0;
+  int n_patches = 0;
+  std::vector<int64_t> patch_first, patch_last;  // 0-based inclusive
+  // cells
+  xcal::DBuf<double> vol;
+  xcal::DBuf<int32_t> cf_ptr, cf_face, cf_nbr, cf_slot;
+  xcal::DBuf<int8_t> cf_sign;
+  xcal::DBuf<int32_t> diag;
+  // faces
+  xcal::DBuf<int32_t> own, nei, fslot;  // fslot: nz index of (owner, neighbour)
+  xcal::DBuf<double> area, weight, delta, nx, ny, nz, ex, ey, ez, sfx, sfy, sfz, lapEf;
+  // boundary faces grouped by cell (ascending face id within a cell)
+  int64_t nbcell = 0;
+  xcal::DBuf<int32_t> bc_cells, bc_ptr, bc_face;
+  xcal::DBuf<int8_t> bface_patch;  // per boundary face
+  // CSR
+  std::vector<int32_t> h_rowptr, h_colval;
+  xcal::DBuf<int32_t> rowptr, colval, trans;
+  // DILU level schedules
+  int nlev_f = 0, nlev_b = 0;
+  std::vector<int32_t> lev_ptr_f, lev_ptr_b;
+  xcal::DBuf<int32_t> lev_rows_f, lev_rows_b;
+  int64_t device_bytes = 0;
+};
+
+struct xcal_field {
+  xcal_mesh* mesh = nullptr;
+  int kind = 0;
+  int ncomp = 1;
+  int64_t len = 0;  // entries per component
+  xcal::DBuf<double> data;
+  double* comp(int c) const { return data.p + (int64_t)c * len; }
+};
+
+struct xcal_eqn {
+  xcal_mesh* mesh = nullptr;
+  bool vector = false;
+  xcal::DBuf<double> nzval, nzval0;
+  xcal::DBuf<double> b;  // 1 or 3 components of length n
+  double* bc(int c) const { return b.p + (int64_t)c * mesh->n; }
+  // preconditioner storage
+  int precon = XCAL_PRECON_NONE;
+  xcal::DBuf<double> pS;   // Jacobi 1/|a_ii|
+  xcal::DBuf<double> pD;   // DILU D
+  xcal::DBuf<double> prD;  // DILU 1/D
+  // Krylov workspace (allocated lazily)
+  xcal::DBuf<double> work;  // 10 vectors of length n
+  xcal::DBuf<float> work32; // FP32 inner solve workspace
+  xcal::DBuf<float> nz32;   // FP32 copy of nzval
+};
+
+namespace xcal {
+
+// kernel-side descriptors (passed by value)
+struct TermK {
+  int type;
+  int sign;
+  const double* flux;
+  double flux_const;
+};
+struct TermsK {
+  int n;
+  TermK t[kMaxTerms];
+};
+struct SourcesK {
+  int n;
+  int sign[kMaxSources];
+  const double* x[kMaxSources];
+  const double* y[kMaxSources];
+  const double* z[kMaxSources];
+};
+struct BCK {
+  int kind;
+  double v[3];
+};
+struct BCsK {
+  int npatch;
+  BCK p[kMaxPatches];  // indexed by patch id; kind -1 = no BC on patch
+};
+
+TermsK make_terms(const xcal_term* terms, int n);
+BCsK make_bcs(const xcal_mesh* m, const xcal_bc* bcs, int n);
+xcal_field* check_field(const xcal_field* f, const xcal_mesh* m, int kind, const char* what);
+
+// ---------------------------------------------------------------- kernels
+// assembly
+void discretise(xcal_eqn* e, const TermsK& t, const SourcesK& s, const xcal_field* prev, double dt);
+void apply_bcs(xcal_eqn* e, const TermsK& t, const BCsK& b, const xcal_field* phi, int comp);
+void relax_diagdom(xcal_eqn* e, const double* field, double* b, double alpha);
+void set_reference(xcal_eqn* e, double pref, int64_t cell);
+void copy_nz(xcal_mesh* m, double* dst, const double* src);
+// calculate
+void interpolate(const xcal_mesh* m, const double* phi, double* phif);
+void correct_boundaries(const xcal_mesh* m, const BCsK& b, const double* const* phi,
+                        double* const* phif, int ncomp);
+void green_gauss(const xcal_mesh* m, const double* phif, double* gx, double* gy, double* gz);
+void green_gauss_tensor(const xcal_mesh* m, const double* fx, const double* fy, const double* fz,
+                        double* t0);
+void divergence(const xcal_mesh* m, const double* psif, double* div);
+void flux(const xcal_mesh* m, const double* fx, const double* fy, const double* fz, double* mdot);
+void relax_explicit(const xcal_mesh* m, int64_t len, double* phi, const double* phi0, double alpha);
+void fill(hipStream_t s, double* p, int64_t n, double v);
+void inverse_diagonal(const xcal_eqn* e, double* rD);
+void remove_pressure_source(xcal_eqn* e, const double* gx, const double* gy, const double* gz,
+                            int sign);
+void Hop(const xcal_eqn* e, const double* Ux, const double* Uy, const double* Uz, double* Hx,
+         double* Hy, double* Hz);
+void correct_mass_flux(const xcal_eqn* pe, const double* p, const double* pflux, int psign,
+                       const BCsK& b, double* mdot);
+void correct_velocity(const xcal_mesh* m, double* Ux, double* Uy, double* Uz, const double* Hx,
+                      const double* Hy, const double* Hz, const double* gx, const double* gy,
+                      const double* gz, const double* rD);
+// linear algebra
+void precond_update(xcal_eqn* e, int precon);
+int solve_system(xcal_eqn* e, const xcal_solver_setup* s, double* x, const double* b,
+                 xcal_solve_stats* st);
+double residual(xcal_eqn* e, const double* x, const double* b);
+
+}  // namespace xcal

@@ -1,0 +1,302 @@
+"""ctypes bindings of libxcalhip.so (include/xcal.h).
+
+Thin wrappers only: every call goes to the gfx950 library; a missing library
+or a failing call raises.  Handles free their device memory when collected.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _abi as A
+from . import _libs
+
+_vp = C.c_void_p
+_pd = C.POINTER(C.c_double)
+_pi32 = C.POINTER(C.c_int32)
+_lib = None
+
+
+class XcalError(RuntimeError):
+    pass
+
+
+class ItmaxWarning(UserWarning):
+    pass
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        _libs._need(_libs.HIP_SO)
+        L = C.CDLL(str(_libs.HIP_SO))
+        L.xcal_last_error.restype = C.c_char_p
+        L.xcal_abi_version.restype = C.c_int32
+        sig = {
+            "xcal_ctx_create": [C.c_int32, C.POINTER(_vp)],
+            "xcal_ctx_destroy": [_vp],
+            "xcal_ctx_synchronize": [_vp],
+            "xcal_mesh_create": [_vp, C.POINTER(A.MeshHost), C.POINTER(_vp)],
+            "xcal_mesh_destroy": [_vp],
+            "xcal_mesh_info": [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), _pi32, _pi32,
+                               C.POINTER(C.c_int64)],
+            "xcal_mesh_csr_pattern": [_vp, _pi32, _pi32],
+            "xcal_field_create": [_vp, C.c_int32, C.POINTER(_vp)],
+            "xcal_field_destroy": [_vp],
+            "xcal_field_upload": [_vp, C.c_int32, _pd],
+            "xcal_field_download": [_vp, C.c_int32, _pd],
+            "xcal_field_fill": [_vp, C.c_int32, C.c_double],
+            "xcal_eqn_create": [_vp, C.c_int32, C.POINTER(_vp)],
+            "xcal_eqn_destroy": [_vp],
+            "xcal_discretise": [_vp, _vp, C.c_int32, _vp, C.c_int32, _vp],
+            "xcal_update_equation": [_vp],
+            "xcal_apply_bcs": [_vp, _vp, C.c_int32, _vp, C.c_int32, _vp, C.c_int32],
+            "xcal_relax_implicit_diagdom": [_vp, _vp, C.c_int32, C.c_double],
+            "xcal_set_reference": [_vp, C.c_double, C.c_int64],
+            "xcal_eqn_download": [_vp, C.c_int32, C.c_int32, _pd, _pd],
+            "xcal_precond_update": [_vp, C.c_int32],
+            "xcal_precond_apply": [_vp, _vp, C.c_int32, _vp, C.c_int32],
+            "xcal_spmv": [_vp, _vp, C.c_int32, _vp, C.c_int32],
+            "xcal_solve_system": [_vp, C.POINTER(A.SolverSetupC), _vp, C.c_int32,
+                                  C.POINTER(A.SolveStats)],
+            "xcal_interpolate": [_vp, _vp],
+            "xcal_correct_boundaries": [_vp, _vp, _vp, C.c_int32],
+            "xcal_green_gauss": [_vp, _vp],
+            "xcal_grad": [_vp, _vp, _vp, _vp, C.c_int32],
+            "xcal_div": [_vp, _vp],
+            "xcal_flux": [_vp, _vp],
+            "xcal_relax_explicit": [_vp, _vp, C.c_double],
+            "xcal_inverse_diagonal": [_vp, _vp],
+            "xcal_remove_pressure_source": [_vp, _vp, C.c_int32],
+            "xcal_H": [_vp, _vp, _vp],
+            "xcal_correct_mass_flux": [_vp, _vp, _vp, _vp, C.c_int32, _vp, C.c_int32],
+            "xcal_correct_velocity": [_vp, _vp, _vp, _vp],
+            "xcal_simple_create": [_vp, C.POINTER(A.SimpleSetupC), C.POINTER(_vp)],
+            "xcal_simple_destroy": [_vp],
+            "xcal_simple_set_field": [_vp, C.c_int32, _pd],
+            "xcal_simple_get_field": [_vp, C.c_int32, _pd],
+            "xcal_simple_run": [_vp, C.c_int32, _pd, _pi32, _pi32],
+            "xcal_simple_stage_times": [_vp, _pd],
+        }
+        for name, argt in sig.items():
+            fn = getattr(L, name)
+            fn.argtypes = argt
+            fn.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def check(rc: int) -> int:
+    if rc < 0:
+        raise XcalError(lib().xcal_last_error().decode())
+    return rc
+
+
+def _pd_of(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_pd)
+
+
+class Context:
+    """One HIP stream on one device (xcal_ctx)."""
+
+    def __init__(self, device: int = 0):
+        self.h = _vp()
+        check(lib().xcal_ctx_create(device, C.byref(self.h)))
+
+    def synchronize(self):
+        check(lib().xcal_ctx_synchronize(self.h))
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.xcal_ctx_destroy(self.h)
+            self.h = None
+
+
+class DeviceMesh:
+    def __init__(self, ctx: Context, mesh):
+        self.ctx = ctx
+        self.mesh = mesh
+        host = A.mesh_host(mesh)
+        self.h = _vp()
+        check(lib().xcal_mesh_create(ctx.h, C.byref(host), C.byref(self.h)))
+        n, nnz, bytes_ = C.c_int64(), C.c_int64(), C.c_int64()
+        fl, bl = C.c_int32(), C.c_int32()
+        check(lib().xcal_mesh_info(self.h, C.byref(n), C.byref(nnz), C.byref(fl), C.byref(bl),
+                                   C.byref(bytes_)))
+        self.n, self.nnz = n.value, nnz.value
+        self.levels = (fl.value, bl.value)
+        self.device_bytes = bytes_.value
+
+    def csr_pattern(self):
+        rp = np.zeros(self.n + 1, np.int32)
+        cv = np.zeros(self.nnz, np.int32)
+        check(lib().xcal_mesh_csr_pattern(self.h, rp.ctypes.data_as(_pi32), cv.ctypes.data_as(_pi32)))
+        return rp, cv
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.xcal_mesh_destroy(self.h)
+            self.h = None
+
+
+_NCOMP = {A.CELL_SCALAR: 1, A.CELL_VECTOR: 3, A.FACE_SCALAR: 1, A.FACE_VECTOR: 3,
+          A.CELL_TENSOR: 9}
+
+
+class Field:
+    def __init__(self, dmesh: DeviceMesh, kind: int):
+        self.dmesh = dmesh
+        self.kind = kind
+        self.ncomp = _NCOMP[kind]
+        self.len = dmesh.mesh.n_faces if kind in (A.FACE_SCALAR, A.FACE_VECTOR) else dmesh.n
+        self.h = _vp()
+        check(lib().xcal_field_create(dmesh.h, kind, C.byref(self.h)))
+
+    def upload(self, values):
+        v = np.asarray(values, dtype=np.float64).reshape(self.ncomp, self.len)
+        for c in range(self.ncomp):
+            check(lib().xcal_field_upload(self.h, c, _pd_of(np.ascontiguousarray(v[c]))))
+        return self
+
+    def download(self) -> np.ndarray:
+        out = np.zeros((self.ncomp, self.len))
+        for c in range(self.ncomp):
+            row = np.zeros(self.len)
+            check(lib().xcal_field_download(self.h, c, _pd_of(row)))
+            out[c] = row
+        return out[0] if self.ncomp == 1 else out
+
+    def fill(self, value, component: int = 0):
+        check(lib().xcal_field_fill(self.h, component, float(value)))
+        return self
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.xcal_field_destroy(self.h)
+            self.h = None
+
+
+class _TermC(C.Structure):
+    _fields_ = [("type", C.c_int32), ("sign", C.c_int32), ("flux", _vp), ("flux_const", C.c_double)]
+
+
+class _SrcC(C.Structure):
+    _fields_ = [("field", _vp), ("sign", C.c_int32)]
+
+
+def terms_c(terms):
+    """terms: list of (type, sign, Field-or-constant)."""
+    arr = (_TermC * len(terms))()
+    for i, (typ, sign, flux) in enumerate(terms):
+        if isinstance(flux, Field):
+            arr[i] = _TermC(typ, sign, flux.h, 0.0)
+        else:
+            arr[i] = _TermC(typ, sign, None, float(flux))
+    return arr
+
+
+def sources_c(srcs):
+    arr = (_SrcC * max(1, len(srcs)))()
+    for i, (field, sign) in enumerate(srcs):
+        arr[i] = _SrcC(field.h, sign)
+    return arr
+
+
+class Equation:
+    """ScalarEquation / VectorEquation storage on the device."""
+
+    def __init__(self, dmesh: DeviceMesh, vector: bool):
+        self.dmesh = dmesh
+        self.vector = vector
+        self.h = _vp()
+        check(lib().xcal_eqn_create(dmesh.h, int(vector), C.byref(self.h)))
+
+    def discretise(self, terms, srcs, prev=None):
+        check(lib().xcal_discretise(self.h, terms_c(terms), len(terms), sources_c(srcs), len(srcs),
+                                    prev.h if prev is not None else None))
+
+    def update_equation(self):
+        check(lib().xcal_update_equation(self.h))
+
+    def apply_bcs(self, terms, bcs, phi, component=0):
+        check(lib().xcal_apply_bcs(self.h, terms_c(terms), len(terms), bcs, len(bcs), phi.h,
+                                   component))
+
+    def relax_implicit_diagdom(self, phi, component, alpha):
+        check(lib().xcal_relax_implicit_diagdom(self.h, phi.h, component, alpha))
+
+    def set_reference(self, pref, cell=1):
+        check(lib().xcal_set_reference(self.h, pref, cell))
+
+    def download(self, which=0, component=0):
+        nz = np.zeros(self.dmesh.nnz)
+        b = np.zeros(self.dmesh.n)
+        check(lib().xcal_eqn_download(self.h, which, component, _pd_of(nz), _pd_of(b)))
+        return nz, b
+
+    def precond_update(self, precon):
+        check(lib().xcal_precond_update(self.h, precon))
+
+    def precond_apply(self, r: Field, z: Field, rc=0, zc=0):
+        check(lib().xcal_precond_apply(self.h, r.h, rc, z.h, zc))
+
+    def spmv(self, xf: Field, y: Field, xc=0, yc=0):
+        check(lib().xcal_spmv(self.h, xf.h, xc, y.h, yc))
+
+    def solve(self, setup: A.SolverSetupC, phi: Field, component=0):
+        st = A.SolveStats()
+        rc = check(lib().xcal_solve_system(self.h, C.byref(setup), phi.h, component, C.byref(st)))
+        return st.niter, st.residual, rc
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.xcal_eqn_destroy(self.h)
+            self.h = None
+
+
+class Simple:
+    """Device-resident SIMPLE run (xcal_simple_*)."""
+
+    def __init__(self, dmesh: DeviceMesh, setup: A.SimpleSetupC):
+        self.dmesh = dmesh
+        self._setup = setup  # keeps BC arrays alive
+        self.h = _vp()
+        check(lib().xcal_simple_create(dmesh.h, C.byref(setup), C.byref(self.h)))
+
+    def set_U(self, U):
+        U = np.ascontiguousarray(np.asarray(U, dtype=np.float64).reshape(3, self.dmesh.n))
+        check(lib().xcal_simple_set_field(self.h, 0, _pd_of(U)))
+
+    def set_p(self, p):
+        p = np.ascontiguousarray(np.asarray(p, dtype=np.float64).reshape(self.dmesh.n))
+        check(lib().xcal_simple_set_field(self.h, 1, _pd_of(p)))
+
+    def get(self, which: str) -> np.ndarray:
+        sel = {"U": 0, "p": 1, "mdotf": 2}[which]
+        n = self.dmesh.mesh.n_faces if which == "mdotf" else self.dmesh.n
+        out = np.zeros((3 if which == "U" else 1) * n)
+        check(lib().xcal_simple_get_field(self.h, sel, _pd_of(out)))
+        return out.reshape(3, n) if which == "U" else out
+
+    def run(self, iterations: int):
+        res = np.zeros(4 * iterations)
+        kit = np.zeros(4 * iterations, np.int32)
+        done = C.c_int32()
+        check(lib().xcal_simple_run(self.h, iterations, _pd_of(res), C.byref(done),
+                                    kit.ctypes.data_as(_pi32)))
+        d = done.value
+        return dict(done=d, residuals=res.reshape(4, iterations)[:, :d],
+                    krylov_iters=kit.reshape(4, iterations)[:, :d])
+
+    def stage_times(self):
+        t = np.zeros(6)
+        check(lib().xcal_simple_stage_times(self.h, _pd_of(t)))
+        return dict(momentum_assembly=t[0], momentum_solve=t[1], pressure_assembly=t[2],
+                    pressure_solve=t[3], other=t[4], total=t[5])
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.xcal_simple_destroy(self.h)
+            self.h = None

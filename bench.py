@@ -1,0 +1,169 @@
+#!/usr/bin/env python3
+"""SIMPLE cell-iterations per second on one MI355X.
+
+Workload (a "step" is one SIMPLE iteration over the whole mesh):
+  3D laminar channel, steady incompressible SIMPLE, FP64, on a
+  100x100x100 hex box (1M cells) whose interior nodes are jittered by +-10 %
+  of the spacing and whose cells are renumbered by a seeded random
+  permutation.  Inlet U = (1,0,0), outlet p = 0, no-slip walls, nu = 1e-3,
+  upwind convection, linear Laplacian, Gauss gradients.  U: BiCGSTAB + DILU
+  (rtol 0.1), relax 0.7; p: CG + DILU to rtol 1e-10, relax 0.3.
+
+value     = cells x steps / seconds of the timed region (inputs resident in HBM)
+roofline  = algorithmic bytes of the dominant pressure-solve kernel per launch
+            / its mean launch time from HIP events on the library's stream
+cpu_baseline = the CPU oracle (restatement of the reference's FP64 path)
+            timed on the host's cores over a bounded sample of the same run.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "SIMPLE cell-iters/s FP64 vs mixed@1e-8; batched solves/s + p50/p99 latency"
+SEED_JITTER, SEED_PERM = 0x5EED0002, 0x5EED0003
+
+
+def channel(api, mesh, iterations):
+    model = api.Physics(domain=mesh, fluid=api.Fluid(nu=1e-3))
+    api.initialise(model.momentum.U, [1.0, 0.0, 0.0])
+    api.initialise(model.momentum.p, 0.0)
+    BCs = api.assign(region=mesh,
+                     U=[api.Dirichlet("inlet", [1.0, 0.0, 0.0]), api.Extrapolated("outlet"),
+                        api.Wall("walls", [0.0, 0.0, 0.0])],
+                     p=[api.Extrapolated("inlet"), api.Dirichlet("outlet", 0.0),
+                        api.Extrapolated("walls")])
+    solvers = dict(U=api.SolverSetup(solver=api.Bicgstab(), preconditioner=api.DILU(),
+                                     convergence=1e-7, relax=0.7, rtol=0.1),
+                   p=api.SolverSetup(solver=api.Cg(), preconditioner=api.DILU(),
+                                     convergence=1e-7, relax=0.3, rtol=1e-10))
+    config = api.Configuration(
+        solvers=solvers, schemes=dict(U=api.Schemes(divergence=api.Upwind), p=api.Schemes()),
+        runtime=api.Runtime(iterations=iterations, write_interval=-1),
+        hardware=api.Hardware(backend=api.HIPBackend(0)), boundaries=BCs)
+    return model, config
+
+
+def kernel_bytes(name, n, nnz):
+    """Algorithmic HBM bytes of one launch (FP64 values, int32 indices; every
+    array element counted once, gathers included once)."""
+    nlow = (nnz - n) // 2
+    if name == "cg_spmv":   # rowptr, colval+nz, z and pold read, pnew and q written
+        return 4 * (n + 1) + 12 * nnz + 8 * 4 * n
+    if name == "dilu_fwd":  # rows, rowptr, diag; lower colval+nz; rD, y gathers; y write;
+        # p, q read; x, r read+write; z reset
+        return 4 * 3 * n + 12 * nlow + 8 * n * (1 + 1 + 1 + 1 + 1 + 2 + 2 + 1)
+    if name == "dilu_bwd":  # rows, rowptr, diag; upper colval+nz; D; y read+reset;
+        # z gathers + write; r read
+        return 4 * 3 * n + 12 * nlow + 8 * n * (1 + 2 + 1 + 1 + 1)
+    raise KeyError(name)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--cells", type=int, default=100, help="cells per box edge")
+    ap.add_argument("--roofline-kernel", default="dilu_fwd",
+                    choices=["cg_spmv", "dilu_fwd", "dilu_bwd"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs on one GPU only (the path has no multi-GPU sharding); "
+              "use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import __graft_entry__ as ge
+    x = ge.load_package()
+    api, H = x.api, x.hip
+    N = args.cells
+    t0 = time.perf_counter()
+    mesh = x.mesh.box3d(N, N, N, jitter=0.1, seed_jitter=SEED_JITTER, seed_perm=SEED_PERM)
+    t_mesh = time.perf_counter() - t0
+    model, config = channel(api, mesh, args.warmup + args.steps)
+    setup = api.simple_setup(model, config)
+    ctx = H.Context(0)
+    t0 = time.perf_counter()
+    dm = H.DeviceMesh(ctx, mesh)
+    sim = H.Simple(dm, setup)
+    sim.set_U(model.momentum.U)
+    sim.set_p(model.momentum.p)
+    ctx.synchronize()
+    t_upload = time.perf_counter() - t0
+    if args.warmup:
+        sim.run(args.warmup)
+    ctx.time_kernel(args.roofline_kernel, True)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    out = sim.run(args.steps)
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    launches, kms = ctx.kernel_time()
+    ctx.time_kernel(args.roofline_kernel, False)
+    if out["done"] != args.steps:
+        raise RuntimeError(f"SIMPLE stopped after {out['done']} of {args.steps} steps")
+    wall = t1 - t0
+    n = mesh.n_cells
+    value = n * args.steps / wall
+    stages = sim.stage_times()
+    kit = out["krylov_iters"]
+    nbytes = kernel_bytes(args.roofline_kernel, n, dm.nnz)
+    avg_s = kms / 1e3 / max(launches, 1)
+    achieved = nbytes / avg_s / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": args.roofline_kernel, "bytes_per_launch": nbytes,
+                "launches": launches, "avg_us": round(avg_s * 1e6, 2)}
+
+    cpu = None
+    if not args.no_cpu:
+        O = ge.load_oracle()
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        O.set_threads(threads)
+        om = O.OracleMesh(mesh, x._abi)
+        m2, c2 = channel(api, mesh, args.cpu_steps)
+        s2 = api.simple_setup(m2, c2)
+        t0 = time.perf_counter()
+        ref = om.simple_run(s2, m2.momentum.U, m2.momentum.p, args.cpu_steps)
+        tc = time.perf_counter() - t0
+        cpu = {"value": round(n * ref["done"] / tc, 1), "unit": "cell-iters/s",
+               "cores": threads, "kind": "port",
+               "sample": f"first {ref['done']} SIMPLE iterations of the same 1M-cell FP64 "
+                         f"channel, C++ oracle with OpenMP ({tc:.1f} s)"}
+
+    line = {
+        "metric": METRIC, "value": round(value, 1), "unit": "cell-iters/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * wall / args.steps, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded jittered, renumbered hex box)",
+        "config": {"workload": f"3D laminar channel SIMPLE FP64, {N}x{N}x{N} jittered+renumbered "
+                               f"hex box ({n} cells)",
+                   "cells": n, "nnz": dm.nnz, "dilu_levels": list(dm.levels),
+                   "p_solver": "CG+DILU rtol 1e-10", "U_solver": "BiCGSTAB+DILU rtol 0.1",
+                   "parallelism": "single GPU"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "detail": {"mesh_build_s": round(t_mesh, 2), "upload_s": round(t_upload, 2),
+                   "device_ms": {k: round(v, 2) for k, v in stages.items()},
+                   "p_cg_iters_mean": float(np.mean(kit[3])),
+                   "U_bicgstab_iters_mean": float(np.mean(kit[:3]))},
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -11,6 +11,10 @@
  *     Float64 geometry, boundary-first faces; src/Mesh/Mesh_0_types.jl).
  *     Host pointers are borrowed for the duration of the call only.
  *   - The library owns every device allocation behind opaque handles.
+ *     A *_destroy call frees only its own handle and never reads the parent
+ *     handles, so handles may be released in any order (garbage collectors
+ *     finalise in arbitrary order); after a parent is destroyed its children
+ *     may only be destroyed.
  *   - Every call returns 0 on success, < 0 on error (message from
  *     xcal_last_error(), thread-local), > 0 for warnings (XCAL_ITMAX).
  *   - Calls on one context are ordered on its HIP stream; calls that return
@@ -286,6 +290,16 @@ int xcal_simple_run(xcal_simple* s, int32_t iterations, double* residuals, int32
  * [0] momentum assembly+BCs, [1] momentum solves, [2] pressure assembly,
  * [3] pressure solve, [4] gradients/interpolation/flux/corrections, [5] total */
 int xcal_simple_stage_times(const xcal_simple* s, double* ms6);
+
+/* Kernel timing with HIP events on the context's stream (no reference
+ * counterpart; measurement support).  While enabled, every launch of the
+ * selected pressure-solve kernel is bracketed by an event pair:
+ *   0 = CG SpMV (p = z + beta p; q = A p; p.q),
+ *   1 = forward DILU sweep (fused x, r update), 2 = backward DILU sweep.
+ * xcal_kernel_timing_read returns launches and summed milliseconds since
+ * the last enable. */
+int xcal_kernel_timing_enable(xcal_ctx* ctx, int32_t kernel, int32_t enable);
+int xcal_kernel_timing_read(xcal_ctx* ctx, int64_t* launches, double* total_ms);
 
 #ifdef __cplusplus
 }

@@ -265,7 +265,9 @@ def test_spmv_and_preconditioner_bit_exact(x, A, O, ctx, meshes, name, precon):
                                            ("SOLVER_BICGSTAB", "PRECON_JACOBI")])
 def test_krylov_matches_oracle(x, A, O, ctx, meshes, solver, precon):
     """Same Krylov recurrences; only reduction order differs.  Tolerance:
-    solutions agree to 1e-9 relative when both converge to rtol 1e-12."""
+    solutions agree to 1e-9 relative when both converge to rtol 1e-12.
+    CG iteration counts agree within 5%; BiCGSTAB's irregular residual
+    history makes its count sensitive to rounding, so within 25%."""
     mesh = meshes("box20")
     rng = _rng(7)
     H = x.hip
@@ -294,7 +296,8 @@ def test_krylov_matches_oracle(x, A, O, ctx, meshes, solver, precon):
     ox, oit, orc = om.krylov(getattr(A, solver), pre, onz, ob, F["phi"] * 0.01, itmax=2000,
                              atol=1e-30, rtol=1e-12)
     assert rc == 0 and orc == 0
-    assert abs(it - oit) <= max(2, oit // 20)
+    slack = oit // 20 if sym else oit // 4
+    assert abs(it - oit) <= max(2, slack), (it, oit)
     assert rel(phi.download(), ox) < 1e-9
     assert res < 1e-9
 

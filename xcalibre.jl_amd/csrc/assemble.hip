@@ -227,7 +227,7 @@ int xcal_eqn_create(xcal_mesh* m, int32_t vector, xcal_eqn** out) {
 int xcal_eqn_destroy(xcal_eqn* e) {
   return guarded([&] {
     if (e) {
-      (void)hipStreamSynchronize(e->mesh->ctx->stream);
+      (void)hipDeviceSynchronize();
       delete e;
     }
     return 0;
@@ -293,7 +293,7 @@ int xcal_set_reference(xcal_eqn* e, double pref, int64_t cell) {
   return guarded([&] {
     XCAL_CHECK(e && !e->vector, "setReference! applies to scalar equations");
     XCAL_CHECK(cell >= 1 && cell <= e->mesh->n, "reference cell out of range");
-    set_reference(e, pref, cell - 1);
+    set_reference(e, pref, e->mesh->cperm[cell - 1]);
     return 0;
   });
 }
@@ -303,13 +303,9 @@ int xcal_eqn_download(const xcal_eqn* e, int32_t which, int32_t comp, double* nz
     XCAL_CHECK(e, "equation is NULL");
     XCAL_CHECK(which == 0 || (which == 1 && e->vector), "which: 0 = A, 1 = A0 (vector only)");
     XCAL_CHECK(comp >= 0 && comp < (e->vector ? 3 : 1), "component out of range");
-    hipStream_t s = e->mesh->ctx->stream;
-    if (nzval)
-      XCAL_HIP(hipMemcpyAsync(nzval, which ? e->nzval0.p : e->nzval.p, sizeof(double) * e->mesh->nnz,
-                              hipMemcpyDeviceToHost, s));
-    if (b)
-      XCAL_HIP(hipMemcpyAsync(b, e->bc(comp), sizeof(double) * e->mesh->n, hipMemcpyDeviceToHost, s));
-    XCAL_HIP(hipStreamSynchronize(s));
+    const xcal_mesh* m = e->mesh;
+    if (nzval) download_permuted(m, m->nzperm, which ? e->nzval0.p : e->nzval.p, nzval, m->nnz);
+    if (b) download_permuted(m, m->cperm, e->bc(comp), b, m->n);
     return 0;
   });
 }

@@ -108,6 +108,8 @@ int xcal_ctx_destroy(xcal_ctx* c) {
     if (!c) return 0;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
+    for (auto& pr : c->prof_pending) c->prof_pool.insert(c->prof_pool.end(), {pr.first, pr.second});
+    for (hipEvent_t ev : c->prof_pool) (void)hipEventDestroy(ev);
     (void)hipStreamDestroy(c->stream);
     if (c->host_pinned) (void)hipHostFree(c->host_pinned);
     delete c;
@@ -286,8 +288,120 @@ int xcal_mesh_create(xcal_ctx* ctx, const xcal_mesh_host* h, xcal_mesh** out) {
     }
     m->nlev_b = maxl + 1;
     bucket(m->nlev_b, m->lev_ptr_b, rows_b, false);
-
     std::vector<double> vol(h->cell_volume, h->cell_volume + n);
+
+    // ---- locality renumbering of the device copy -------------------------
+    // Cells are stored in Morton order of their centres and internal faces
+    // in order of their renumbered cells, so that neighbour gathers hit
+    // nearby cache lines instead of random ones.  Every CSR row keeps its
+    // entries in the reference's (original) column order, so row sums, the
+    // DILU lower/upper split and all per-row summation orders are unchanged;
+    // fields are permuted only when they cross the C ABI.
+    std::vector<int32_t> cperm(n), cinv(n), fperm(nf), finv(nf);
+    {
+      double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+      for (int64_t c = 0; c < n; ++c)
+        for (int a = 0; a < 3; ++a) {
+          lo[a] = std::min(lo[a], h->cell_centre[3 * c + a]);
+          hi[a] = std::max(hi[a], h->cell_centre[3 * c + a]);
+        }
+      const int naxis = h->dim == 3 ? 3 : 2;
+      const int bits = naxis == 3 ? 21 : 31;
+      auto spread = [&](uint64_t v) {  // interleave with naxis-1 zero bits
+        uint64_t r = 0;
+        for (int b = 0; b < bits; ++b) r |= ((v >> b) & 1ull) << (naxis * b);
+        return r;
+      };
+      std::vector<std::pair<uint64_t, int32_t>> key(n);
+      for (int64_t c = 0; c < n; ++c) {
+        uint64_t k = 0;
+        for (int a = 0; a < naxis; ++a) {
+          double span = hi[a] - lo[a];
+          double t = span > 0 ? (h->cell_centre[3 * c + a] - lo[a]) / span : 0.0;
+          uint64_t q = (uint64_t)std::min(t * (double)((1ull << bits) - 1), (double)((1ull << bits) - 1));
+          k |= spread(q) << a;
+        }
+        key[c] = {k, (int32_t)c};
+      }
+      std::sort(key.begin(), key.end());
+      for (int64_t c = 0; c < n; ++c) {
+        cinv[c] = key[c].second;
+        cperm[key[c].second] = (int32_t)c;
+      }
+      for (int64_t f = 0; f < nbf; ++f) fperm[f] = finv[f] = (int32_t)f;  // boundary faces keep ids
+      std::vector<std::pair<std::pair<int32_t, int32_t>, int32_t>> fk;
+      fk.reserve(nf - nbf);
+      for (int64_t f = nbf; f < nf; ++f) {
+        int32_t a = cperm[own[f]], b = cperm[nei[f]];
+        fk.push_back({{std::min(a, b), std::max(a, b)}, (int32_t)f});
+      }
+      std::sort(fk.begin(), fk.end());
+      for (int64_t k = 0; k < nf - nbf; ++k) {
+        finv[nbf + k] = fk[k].second;
+        fperm[fk[k].second] = (int32_t)(nbf + k);
+      }
+    }
+    std::vector<int32_t> rowptr_n(n + 1, 0), colval_n(nnz), nzperm(nnz), diag_n(n), trans_n(nnz);
+    for (int64_t c = 0; c < n; ++c) rowptr_n[c + 1] = rowptr_n[c] + (rowptr[cinv[c] + 1] - rowptr[cinv[c]]);
+    for (int64_t oc = 0; oc < n; ++oc)
+      for (int32_t k = rowptr[oc]; k < rowptr[oc + 1]; ++k) nzperm[k] = rowptr_n[cperm[oc]] + (k - rowptr[oc]);
+    for (int64_t k = 0; k < nnz; ++k) colval_n[nzperm[k]] = cperm[colval[k]];
+    for (int64_t k = 0; k < nnz; ++k) trans_n[nzperm[k]] = trans[k] >= 0 ? nzperm[trans[k]] : -1;
+    for (int64_t oc = 0; oc < n; ++oc) diag_n[cperm[oc]] = nzperm[diag[oc]];
+    std::vector<int32_t> cf_ptr_n(n + 1, 0), cf_face_n(cf_face.size()), cf_nbr_n(cf_face.size()),
+        cf_slot_n(cf_face.size());
+    std::vector<int8_t> cf_sign_n(cf_face.size());
+    for (int64_t c = 0; c < n; ++c) {
+      const int32_t oc = cinv[c];
+      int32_t w = cf_ptr_n[c];
+      for (int32_t k = cf_ptr[oc]; k < cf_ptr[oc + 1]; ++k, ++w) {
+        cf_face_n[w] = fperm[cf_face[k]];
+        cf_nbr_n[w] = cperm[cf_nbr[k]];
+        cf_slot_n[w] = nzperm[cf_slot[k]];
+        cf_sign_n[w] = cf_sign[k];
+      }
+      cf_ptr_n[c + 1] = w;
+    }
+    auto permf = [&](std::vector<double>& a) {
+      std::vector<double> t(nf);
+      for (int64_t f = 0; f < nf; ++f) t[fperm[f]] = a[f];
+      a.swap(t);
+    };
+    for (auto* a : {&area, &weight, &delta, &nx, &ny, &nz, &ex, &ey, &ez, &sfx, &sfy, &sfz, &lapEf})
+      permf(*a);
+    std::vector<int32_t> own_n(nf), nei_n(nf), fslot_n(nf, -1);
+    for (int64_t f = 0; f < nf; ++f) {
+      own_n[fperm[f]] = cperm[own[f]];
+      nei_n[fperm[f]] = cperm[nei[f]];
+      if (f >= nbf) fslot_n[fperm[f]] = nzperm[fslot[f]];
+    }
+    for (auto& c : bc_cells) c = cperm[c];
+    for (auto& r : rows_f) r = cperm[r];
+    for (auto& r : rows_b) r = cperm[r];
+    {
+      std::vector<double> t(n);
+      for (int64_t c = 0; c < n; ++c) t[cperm[c]] = vol[c];
+      vol.swap(t);
+    }
+    cf_ptr.swap(cf_ptr_n);
+    cf_face.swap(cf_face_n);
+    cf_nbr.swap(cf_nbr_n);
+    cf_slot.swap(cf_slot_n);
+    cf_sign.swap(cf_sign_n);
+    own.swap(own_n);
+    nei.swap(nei_n);
+    fslot.swap(fslot_n);
+    diag.swap(diag_n);
+    trans.swap(trans_n);
+    m->cperm = cperm;
+    m->fperm = fperm;
+    m->nzperm = nzperm;
+    // original-numbering CSR stays on the host for xcal_mesh_csr_pattern
+    m->h_rowptr = rowptr;
+    m->h_colval = colval;
+    rowptr.swap(rowptr_n);
+    colval.swap(colval_n);
+
     upload(m->vol, vol, bytes);
     upload(m->cf_ptr, cf_ptr, bytes);
     upload(m->cf_face, cf_face, bytes);
@@ -320,8 +434,6 @@ int xcal_mesh_create(xcal_ctx* ctx, const xcal_mesh_host* h, xcal_mesh** out) {
     upload(m->trans, trans, bytes);
     upload(m->lev_rows_f, rows_f, bytes);
     upload(m->lev_rows_b, rows_b, bytes);
-    m->h_rowptr = std::move(rowptr);
-    m->h_colval = std::move(colval);
     m->device_bytes = bytes;
     *out = m.release();
     return 0;
@@ -330,9 +442,10 @@ int xcal_mesh_create(xcal_ctx* ctx, const xcal_mesh_host* h, xcal_mesh** out) {
 
 int xcal_mesh_destroy(xcal_mesh* m) {
   return guarded([&] {
+    // Destroy touches only the handle's own memory (never its context), so
+    // handles may be released in any order; hipFree waits for the device.
     if (m) {
-      (void)hipSetDevice(m->ctx->device);
-      (void)hipStreamSynchronize(m->ctx->stream);
+      (void)hipDeviceSynchronize();
       delete m;
     }
     return 0;
@@ -387,20 +500,44 @@ int xcal_field_create(xcal_mesh* m, int32_t kind, xcal_field** out) {
 int xcal_field_destroy(xcal_field* f) {
   return guarded([&] {
     if (f) {
-      (void)hipStreamSynchronize(f->mesh->ctx->stream);
+      (void)hipDeviceSynchronize();
       delete f;
     }
     return 0;
   });
 }
 
+}  // extern "C"
+
+namespace xcal {
+// reference id -> device id map of a field's entities
+const std::vector<int32_t>& field_perm(const xcal_field* f) {
+  return (f->kind == XCAL_FACE_SCALAR || f->kind == XCAL_FACE_VECTOR) ? f->mesh->fperm
+                                                                      : f->mesh->cperm;
+}
+void upload_permuted(const xcal_mesh* m, const std::vector<int32_t>& perm, double* dev,
+                     const double* host, int64_t len) {
+  std::vector<double> t(len);
+  for (int64_t i = 0; i < len; ++i) t[perm[i]] = host[i];
+  XCAL_HIP(hipMemcpyAsync(dev, t.data(), sizeof(double) * len, hipMemcpyHostToDevice, m->ctx->stream));
+  XCAL_HIP(hipStreamSynchronize(m->ctx->stream));
+}
+void download_permuted(const xcal_mesh* m, const std::vector<int32_t>& perm, const double* dev,
+                       double* host, int64_t len) {
+  std::vector<double> t(len);
+  XCAL_HIP(hipMemcpyAsync(t.data(), dev, sizeof(double) * len, hipMemcpyDeviceToHost, m->ctx->stream));
+  XCAL_HIP(hipStreamSynchronize(m->ctx->stream));
+  for (int64_t i = 0; i < len; ++i) host[i] = t[perm[i]];
+}
+}  // namespace xcal
+
+extern "C" {
+
 int xcal_field_upload(xcal_field* f, int32_t c, const double* host) {
   return guarded([&] {
     XCAL_CHECK(f && host, "NULL argument");
     XCAL_CHECK(c >= 0 && c < f->ncomp, "component out of range");
-    XCAL_HIP(hipMemcpyAsync(f->comp(c), host, sizeof(double) * f->len, hipMemcpyHostToDevice,
-                            f->mesh->ctx->stream));
-    XCAL_HIP(hipStreamSynchronize(f->mesh->ctx->stream));
+    upload_permuted(f->mesh, field_perm(f), f->comp(c), host, f->len);
     return 0;
   });
 }
@@ -409,9 +546,7 @@ int xcal_field_download(const xcal_field* f, int32_t c, double* host) {
   return guarded([&] {
     XCAL_CHECK(f && host, "NULL argument");
     XCAL_CHECK(c >= 0 && c < f->ncomp, "component out of range");
-    XCAL_HIP(hipMemcpyAsync(host, f->comp(c), sizeof(double) * f->len, hipMemcpyDeviceToHost,
-                            f->mesh->ctx->stream));
-    XCAL_HIP(hipStreamSynchronize(f->mesh->ctx->stream));
+    download_permuted(f->mesh, field_perm(f), f->comp(c), host, f->len);
     return 0;
   });
 }

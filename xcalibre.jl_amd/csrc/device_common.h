@@ -245,6 +245,16 @@ __device__ inline void bc_coeffs(const MeshK& m, const BCK& bc, const TermK& t, 
   }
 }
 
+// XCD-aware block order: the dispatcher deals workgroups round-robin to the
+// 8 XCDs, so block b lands on XCD b%8.  Remapping gives each XCD one
+// contiguous range of rows, so the cells it gathers stay in its own L2.
+__device__ inline int64_t xcd_block() {
+  const int64_t G = gridDim.x, b = blockIdx.x;
+  const int64_t q = G >> 3, r = G & 7, x = b & 7, l = b >> 3;
+  return x * q + (x < r ? x : r) + l;
+}
+__device__ inline int64_t xcd_row() { return xcd_block() * blockDim.x + threadIdx.x; }
+
 // Deterministic block-level sum (wave shuffles + LDS), result in thread 0.
 __device__ inline double block_sum(double v, double* lds) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);

@@ -22,7 +22,7 @@ namespace xcal {
 __global__ void __launch_bounds__(kBlock)
     k_spmv(int64_t n, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colval,
            const double* __restrict__ nz, const double* __restrict__ x, double* __restrict__ y) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t i = xcd_row();
   if (i >= n) return;
   double acc = 0.0;
   for (int32_t k = rowptr[i]; k < rowptr[i + 1]; ++k) acc += nz[k] * x[colval[k]];
@@ -34,7 +34,7 @@ __global__ void __launch_bounds__(kBlock)
     k_residual_vec(int64_t n, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colval,
                    const double* __restrict__ nz, const double* __restrict__ x,
                    const double* __restrict__ b, double* __restrict__ r) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t i = xcd_row();
   if (i >= n) return;
   double acc = 0.0;
   for (int32_t k = rowptr[i]; k < rowptr[i + 1]; ++k) acc += nz[k] * x[colval[k]];
@@ -251,7 +251,7 @@ struct Ctx {
 
 static void launch_check() { XCAL_HIP(hipGetLastError()); }
 
-static void spmv(const xcal_eqn* e, const double* x, double* y) {
+void spmv(const xcal_eqn* e, const double* x, double* y) {
   const xcal_mesh* m = e->mesh;
   hipLaunchKernelGGL(k_spmv, dim3(grid_for(m->n)), dim3(kBlock), 0, m->ctx->stream, m->n,
                      m->rowptr.p, m->colval.p, e->nzval.p, x, y);
@@ -261,7 +261,7 @@ static void spmv(const xcal_eqn* e, const double* x, double* y) {
 static int red_grid(int64_t n) { return std::min<int>(grid_for(n), kRedBlocks); }
 
 // returns {a0.b0, a1.b1} (host values; synchronises)
-static void dots(const xcal_eqn* e, const double* a0, const double* b0, const double* a1,
+[[maybe_unused]] static void dots(const xcal_eqn* e, const double* a0, const double* b0, const double* a1,
                  const double* b1, double* out) {
   const xcal_mesh* m = e->mesh;
   xcal_ctx* c = m->ctx;
@@ -328,7 +328,7 @@ void precond_update(xcal_eqn* e, int precon) {
 }
 
 // mulorldiv!(z, M, r, ldiv)
-static void apply_M(const xcal_eqn* e, const double* r, double* z) {
+void apply_M_levels(const xcal_eqn* e, const double* r, double* z) {
   const xcal_mesh* m = e->mesh;
   hipStream_t s = m->ctx->stream;
   const int64_t n = m->n;
@@ -353,135 +353,6 @@ static void apply_M(const xcal_eqn* e, const double* r, double* z) {
   }
 }
 
-#define L1(kern, ...) \
-  hipLaunchKernelGGL(kern, dim3(grid_for(n)), dim3(kBlock), 0, s, n, __VA_ARGS__)
-
-static int cg(xcal_eqn* e, const double* b, double* x0, int itmax, double atol, double rtol,
-              int& niter) {
-  const xcal_mesh* m = e->mesh;
-  const int64_t n = m->n;
-  hipStream_t s = m->ctx->stream;
-  double* w = e->work.p;
-  double *r = w, *z = w + n, *p = w + 2 * n, *Ap = w + 3 * n, *x = w + 4 * n;
-  hipLaunchKernelGGL(k_residual_vec, dim3(grid_for(n)), dim3(kBlock), 0, s, n, m->rowptr.p,
-                     m->colval.p, e->nzval.p, x0, b, r);
-  fill(s, x, n, 0.0);
-  apply_M(e, r, z);
-  L1(k_copy, z, p);
-  double d[2];
-  dots(e, r, z, nullptr, nullptr, d);
-  double gamma = d[0];
-  double rNorm = std::sqrt(gamma);
-  niter = 0;
-  if (gamma == 0.0) return 0;
-  if (itmax == 0) itmax = (int)(2 * n);
-  double pNorm2 = gamma;
-  const double epsm = 2.220446049250313e-16;
-  const double epsilon = atol + rtol * rNorm;
-  bool solved = rNorm <= epsilon, tired = niter >= itmax, zero_curv = false;
-  while (!(solved || tired || zero_curv)) {
-    spmv(e, p, Ap);
-    dots(e, p, Ap, nullptr, nullptr, d);
-    const double pAp = d[0];
-    if (pAp <= epsm * pNorm2 && std::fabs(pAp) <= epsm * pNorm2) {
-      zero_curv = true;
-      continue;
-    }
-    const double alpha = gamma / pAp;
-    L1(k_cg_update, alpha, p, Ap, x, r);
-    apply_M(e, r, z);
-    dots(e, r, z, nullptr, nullptr, d);
-    const double gamma_next = d[0];
-    rNorm = std::sqrt(gamma_next);
-    solved = rNorm <= epsilon || (rNorm + 1.0 <= 1.0);
-    if (!solved) {
-      const double beta = gamma_next / gamma;
-      pNorm2 = gamma_next + beta * beta * pNorm2;
-      gamma = gamma_next;
-      L1(k_xpby, z, beta, p);
-    }
-    ++niter;
-    tired = niter >= itmax;
-  }
-  L1(k_axpy, 1.0, x, x0);  // x0 = x0 + x (warm start)
-  launch_check();
-  return niter >= itmax ? 1 : 0;
-}
-
-static int bicgstab(xcal_eqn* e, const double* b, double* x0, int itmax, double atol, double rtol,
-                    int& niter) {
-  const xcal_mesh* m = e->mesh;
-  const int64_t n = m->n;
-  hipStream_t s = m->ctx->stream;
-  double* w = e->work.p;
-  double *r0 = w, *r = w + n, *p = w + 2 * n, *v = w + 3 * n, *sv = w + 4 * n, *q = w + 5 * n,
-         *t = w + 6 * n, *dd = w + 7 * n, *x = w + 8 * n;
-  hipLaunchKernelGGL(k_residual_vec, dim3(grid_for(n)), dim3(kBlock), 0, s, n, m->rowptr.p,
-                     m->colval.p, e->nzval.p, x0, b, r0);
-  fill(s, x, n, 0.0);
-  fill(s, v, n, 0.0);
-  fill(s, sv, n, 0.0);
-  apply_M(e, r0, r);
-  L1(k_copy, r, p);
-  double d[2];
-  dots(e, r, r, b, r, d);  // ||r||^2 and <c, r> with c = b
-  double rNorm = std::sqrt(d[0]);
-  niter = 0;
-  if (rNorm == 0.0) return 0;
-  if (itmax == 0) itmax = (int)(2 * n);
-  const double epsilon = atol + rtol * rNorm;
-  double next_rho = d[1];
-  if (next_rho == 0.0) return 0;
-  bool solved = rNorm <= epsilon, tired = niter >= itmax, breakdown = false;
-  while (!(solved || tired || breakdown)) {
-    ++niter;
-    const double rho = next_rho;
-    spmv(e, p, q);
-    apply_M(e, q, v);
-    dots(e, b, v, nullptr, nullptr, d);
-    const double alpha = rho / d[0];
-    L1(k_waxpy, r, -alpha, v, sv);  // s = r + (-alpha) v
-    L1(k_axpy, alpha, p, x);        // x = x + alpha p
-    spmv(e, sv, dd);
-    apply_M(e, dd, t);
-    dots(e, t, sv, t, t, d);
-    const double omega = d[0] / d[1];
-    L1(k_bicg_update, omega, sv, t, v, x, r, p);
-    dots(e, r, r, b, r, d);
-    next_rho = d[1];
-    const double beta = (next_rho / rho) * (alpha / omega);
-    L1(k_xpby, r, beta, p);  // p = r + beta p
-    rNorm = std::sqrt(d[0]);
-    solved = rNorm <= epsilon || (rNorm + 1.0 <= 1.0);
-    tired = niter >= itmax;
-    breakdown = (alpha == 0.0 || alpha != alpha);
-  }
-  L1(k_axpy, 1.0, x, x0);
-  launch_check();
-  return niter >= itmax ? 1 : 0;
-}
-
-int solve_system(xcal_eqn* e, const xcal_solver_setup* st, double* x, const double* b,
-                 xcal_solve_stats* stats) {
-  const int64_t n = e->mesh->n;
-  if (e->work.n < 10 * n) e->work.alloc(10 * n);
-  XCAL_CHECK(st->precision == XCAL_PREC_FP64 || st->precision == XCAL_PREC_FP32_IR,
-             "unknown precision mode");
-  XCAL_CHECK(st->precon == e->precon,
-             "preconditioner of the solver setup has not been updated on this equation");
-  int niter = 0, rc;
-  if (st->solver == XCAL_SOLVER_CG) rc = cg(e, b, x, st->itmax, st->atol, st->rtol, niter);
-  else if (st->solver == XCAL_SOLVER_BICGSTAB)
-    rc = bicgstab(e, b, x, st->itmax, st->atol, st->rtol, niter);
-  else throw Error(XCAL_ERR_UNSUPPORTED, "unknown Krylov solver");
-  if (stats) {
-    stats->niter = niter;
-    stats->nouter = 1;
-    stats->residual = residual(e, x, b);
-  }
-  return rc;
-}
-
 }  // namespace xcal
 
 using namespace xcal;
@@ -504,17 +375,6 @@ static const double* cell_comp(const xcal_eqn* e, const xcal_field* f, int c, co
   return f->comp(c);
 }
 
-int xcal_precond_apply(xcal_eqn* e, const xcal_field* r, int32_t rc, xcal_field* z, int32_t zc) {
-  return guarded([&] {
-    XCAL_CHECK(e, "equation is NULL");
-    const double* rp = cell_comp(e, r, rc, "r");
-    double* zp = const_cast<double*>(cell_comp(e, z, zc, "z"));
-    XCAL_CHECK(rp != zp, "r and z must not alias");
-    apply_M(e, rp, zp);
-    return 0;
-  });
-}
-
 int xcal_spmv(xcal_eqn* e, const xcal_field* x, int32_t xc, xcal_field* y, int32_t yc) {
   return guarded([&] {
     XCAL_CHECK(e, "equation is NULL");
@@ -523,19 +383,6 @@ int xcal_spmv(xcal_eqn* e, const xcal_field* x, int32_t xc, xcal_field* y, int32
     XCAL_CHECK(xp != yp, "x and y must not alias");
     spmv(e, xp, yp);
     return 0;
-  });
-}
-
-int xcal_solve_system(xcal_eqn* e, const xcal_solver_setup* s, xcal_field* phi, int32_t comp,
-                      xcal_solve_stats* stats) {
-  return guarded([&] {
-    XCAL_CHECK(e && s, "NULL argument");
-    check_field(phi, e->mesh, e->vector ? XCAL_CELL_VECTOR : XCAL_CELL_SCALAR, "phi");
-    XCAL_CHECK(comp >= 0 && comp < (e->vector ? 3 : 1), "component out of range");
-    xcal_solve_stats st{};
-    int rc = solve_system(e, s, phi->comp(comp), e->bc(e->vector ? comp : 0), &st);
-    if (stats) *stats = st;
-    return rc;
   });
 }
 

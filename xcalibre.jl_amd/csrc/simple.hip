@@ -246,7 +246,7 @@ int xcal_simple_run(xcal_simple* S, int32_t iterations, double* residuals, int32
       // solve_equation! scalar (Solve_1_api.jl:214-231)
       discretise(pe, PT, PS, S->p, 1.0);
       apply_bcs(pe, PT, S->bp, S->p, 0);
-      if (cfg.has_pref) set_reference(pe, cfg.pref, 0);
+      if (cfg.has_pref) set_reference(pe, cfg.pref, m->cperm[0]);  // reference cell 1
       precond_update(pe, cfg.p.precon);
       mark(2);
       xcal_solve_stats ps{};

@@ -98,6 +98,13 @@ struct xcal_ctx {
   xcal::DBuf<unsigned> red_ticket;  // 8 slots
   xcal::DBuf<double> red_result;    // 64 slots
   double* host_pinned = nullptr;    // 64 doubles, pinned
+  // optional HIP-event timing of one pressure-solve kernel
+  int prof_kernel = -1;
+  std::vector<hipEvent_t> prof_pool;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_pending;
+  int64_t prof_launches = 0;
+  double prof_ms = 0.0;
+  int coop_max_blocks = 0;  // co-resident blocks of the sweep kernels (0: not yet queried)
 };
 
 // Device mesh in SoA form, 0-based int32 indices, reference numbering.
@@ -119,7 +126,10 @@ struct xcal_mesh {
   int64_t nbcell = 0;
   xcal::DBuf<int32_t> bc_cells, bc_ptr, bc_face;
   xcal::DBuf<int8_t> bface_patch;  // per boundary face
-  // CSR
+  // device numbering: cperm/fperm map reference cell/face ids to device ids,
+  // nzperm maps reference CSR slots to device slots
+  std::vector<int32_t> cperm, fperm, nzperm;
+  // CSR (h_* in the reference's numbering)
   std::vector<int32_t> h_rowptr, h_colval;
   xcal::DBuf<int32_t> rowptr, colval, trans;
   // DILU level schedules
@@ -150,9 +160,18 @@ struct xcal_eqn {
   xcal::DBuf<double> pD;   // DILU D
   xcal::DBuf<double> prD;  // DILU 1/D
   // Krylov workspace (allocated lazily)
-  xcal::DBuf<double> work;  // 10 vectors of length n
-  xcal::DBuf<float> work32; // FP32 inner solve workspace
-  xcal::DBuf<float> nz32;   // FP32 copy of nzval
+  xcal::DBuf<double> work;    // solver vectors, 12 x n
+  xcal::DBuf<double> kpart;   // reduction partial sums
+  xcal::DBuf<unsigned> kcount;  // [0] sweep chunk ticket [1] reduction ticket [2] sweep error
+  xcal::DBuf<char> kstate;    // device-resident Krylov scalars
+  void* kstate_host = nullptr;  // pinned copy
+  xcal::DBuf<double> yflag, zflag;  // DILU sweep outputs published through sentinels
+  int cg_batch = 8, bi_batch = 2;   // iterations enqueued per state read
+  xcal::DBuf<float> work32;   // FP32 inner solve workspace
+  xcal::DBuf<float> nz32;     // FP32 copy of nzval
+  ~xcal_eqn() {
+    if (kstate_host) (void)hipHostFree(kstate_host);
+  }
 };
 
 namespace xcal {
@@ -184,6 +203,10 @@ struct BCsK {
   BCK p[kMaxPatches];  // indexed by patch id; kind -1 = no BC on patch
 };
 
+void upload_permuted(const xcal_mesh* m, const std::vector<int32_t>& perm, double* dev,
+                     const double* host, int64_t len);
+void download_permuted(const xcal_mesh* m, const std::vector<int32_t>& perm, const double* dev,
+                       double* host, int64_t len);
 TermsK make_terms(const xcal_term* terms, int n);
 BCsK make_bcs(const xcal_mesh* m, const xcal_bc* bcs, int n);
 xcal_field* check_field(const xcal_field* f, const xcal_mesh* m, int kind, const char* what);
@@ -217,6 +240,8 @@ void correct_velocity(const xcal_mesh* m, double* Ux, double* Uy, double* Uz, co
                       const double* Hy, const double* Hz, const double* gx, const double* gy,
                       const double* gz, const double* rD);
 // linear algebra
+void spmv(const xcal_eqn* e, const double* x, double* y);
+void apply_M_levels(const xcal_eqn* e, const double* r, double* z);
 void precond_update(xcal_eqn* e, int precon);
 int solve_system(xcal_eqn* e, const xcal_solver_setup* s, double* x, const double* b,
                  xcal_solve_stats* st);

@@ -78,6 +78,8 @@ def lib():
             "xcal_simple_get_field": [_vp, C.c_int32, _pd],
             "xcal_simple_run": [_vp, C.c_int32, _pd, _pi32, _pi32],
             "xcal_simple_stage_times": [_vp, _pd],
+            "xcal_kernel_timing_enable": [_vp, C.c_int32, C.c_int32],
+            "xcal_kernel_timing_read": [_vp, C.POINTER(C.c_int64), _pd],
         }
         for name, argt in sig.items():
             fn = getattr(L, name)
@@ -107,6 +109,18 @@ class Context:
 
     def synchronize(self):
         check(lib().xcal_ctx_synchronize(self.h))
+
+    KERNELS = {"cg_spmv": 0, "dilu_fwd": 1, "dilu_bwd": 2}
+
+    def time_kernel(self, name: str, enable: bool = True):
+        """Bracket every launch of one pressure-solve kernel with HIP events."""
+        check(lib().xcal_kernel_timing_enable(self.h, self.KERNELS[name], int(enable)))
+
+    def kernel_time(self):
+        """(launches, total milliseconds) since time_kernel()."""
+        n, ms = C.c_int64(), C.c_double()
+        check(lib().xcal_kernel_timing_read(self.h, C.byref(n), C.byref(ms)))
+        return n.value, ms.value
 
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:

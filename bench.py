@@ -36,7 +36,7 @@ METRIC = "SIMPLE cell-iters/s FP64 vs mixed@1e-8; batched solves/s + p50/p99 lat
 SEED_JITTER, SEED_PERM = 0x5EED0002, 0x5EED0003
 
 
-def channel(api, mesh, iterations):
+def channel(api, mesh, iterations, precision="fp64"):
     model = api.Physics(domain=mesh, fluid=api.Fluid(nu=1e-3))
     api.initialise(model.momentum.U, [1.0, 0.0, 0.0])
     api.initialise(model.momentum.p, 0.0)
@@ -48,7 +48,8 @@ def channel(api, mesh, iterations):
     solvers = dict(U=api.SolverSetup(solver=api.Bicgstab(), preconditioner=api.DILU(),
                                      convergence=1e-7, relax=0.7, rtol=0.1),
                    p=api.SolverSetup(solver=api.Cg(), preconditioner=api.DILU(),
-                                     convergence=1e-7, relax=0.3, rtol=1e-10))
+                                     convergence=1e-7, relax=0.3, rtol=1e-10,
+                                     precision=precision))
     config = api.Configuration(
         solvers=solvers, schemes=dict(U=api.Schemes(divergence=api.Upwind), p=api.Schemes()),
         runtime=api.Runtime(iterations=iterations, write_interval=-1),
@@ -57,17 +58,19 @@ def channel(api, mesh, iterations):
 
 
 def kernel_bytes(name, n, nnz):
-    """Algorithmic HBM bytes of one launch (FP64 values, int32 indices; every
-    array element counted once, gathers included once)."""
+    """Algorithmic HBM bytes of one timed unit (FP64 values, int32 indices;
+    every array element counted once, gathers included once, slice padding
+    not counted).  cg_spmv is one launch; dilu_fwd/dilu_bwd are one whole
+    level-synchronous sweep (one launch per large level)."""
     nlow = (nnz - n) // 2
-    if name == "cg_spmv":   # rowptr, colval+nz, z and pold read, pnew and q written
-        return 4 * (n + 1) + 12 * nnz + 8 * 4 * n
-    if name == "dilu_fwd":  # rows, rowptr, diag; lower colval+nz; rD, y gathers; y write;
-        # p, q read; x, r read+write; z reset
-        return 4 * 3 * n + 12 * nlow + 8 * n * (1 + 1 + 1 + 1 + 1 + 2 + 2 + 1)
-    if name == "dilu_bwd":  # rows, rowptr, diag; upper colval+nz; D; y read+reset;
-        # z gathers + write; r read
-        return 4 * 3 * n + 12 * nlow + 8 * n * (1 + 2 + 1 + 1 + 1)
+    if name == "cg_spmv":   # row begin/end; colval+nz; z and pold read; pnew and q written
+        return 8 * n + 12 * nnz + 8 * 4 * n
+    if name == "dilu_fwd":  # row begin, diag; lower colval+nz; rD and y gathered; y written;
+        # p, q read; x, r read+written
+        return 8 * n + 12 * nlow + 8 * n * (1 + 1 + 1 + 2 + 4)
+    if name == "dilu_bwd":  # level rows, diag, row end; upper colval+nz; D, y read;
+        # z gathered + written; r read for r.z
+        return 12 * n + 12 * nlow + 8 * n * (1 + 1 + 2 + 1)
     raise KeyError(name)
 
 
@@ -77,8 +80,10 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--cells", type=int, default=100, help="cells per box edge")
-    ap.add_argument("--roofline-kernel", default="dilu_fwd",
+    ap.add_argument("--roofline-kernel", default="cg_spmv",
                     choices=["cg_spmv", "dilu_fwd", "dilu_bwd"])
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32-ir"],
+                    help="pressure solve: FP64, or FP32 inner Krylov in FP64 refinement")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-steps", type=int, default=2)
     args = ap.parse_args()
@@ -94,7 +99,7 @@ def main():
     t0 = time.perf_counter()
     mesh = x.mesh.box3d(N, N, N, jitter=0.1, seed_jitter=SEED_JITTER, seed_perm=SEED_PERM)
     t_mesh = time.perf_counter() - t0
-    model, config = channel(api, mesh, args.warmup + args.steps)
+    model, config = channel(api, mesh, args.warmup + args.steps, args.precision)
     setup = api.simple_setup(model, config)
     ctx = H.Context(0)
     t0 = time.perf_counter()
@@ -129,8 +134,22 @@ def main():
                 "kernel": args.roofline_kernel, "bytes_per_launch": nbytes,
                 "launches": launches, "avg_us": round(avg_s * 1e6, 2)}
 
+    err = None
+    if args.precision != "fp64":
+        # per-field error of the mixed-precision run against the FP64 path
+        # (which matches the CPU oracle to 1e-8 on the parity meshes)
+        m64, c64 = channel(api, mesh, args.warmup + args.steps)
+        sim64 = H.Simple(dm, api.simple_setup(m64, c64))
+        sim64.set_U(m64.momentum.U)
+        sim64.set_p(m64.momentum.p)
+        sim64.run(args.warmup + args.steps)
+
+        def rel(a, b):
+            return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+        err = {f: rel(sim.get(f), sim64.get(f)) for f in ("U", "p", "mdotf")}
+
     cpu = None
-    if not args.no_cpu:
+    if not args.no_cpu and args.precision == "fp64":
         O = ge.load_oracle()
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         O.set_threads(threads)
@@ -148,19 +167,23 @@ def main():
     line = {
         "metric": METRIC, "value": round(value, 1), "unit": "cell-iters/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * wall / args.steps, 3),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f64 (pressure Krylov f32 in f64 refinement)",
         "data": "synthetic (seeded jittered, renumbered hex box)",
-        "config": {"workload": f"3D laminar channel SIMPLE FP64, {N}x{N}x{N} jittered+renumbered "
-                               f"hex box ({n} cells)",
+        "config": {"workload": f"3D laminar channel SIMPLE {args.precision.upper()}, {N}x{N}x{N} "
+                               f"jittered+renumbered hex box ({n} cells)",
                    "cells": n, "nnz": dm.nnz, "dilu_levels": list(dm.levels),
-                   "p_solver": "CG+DILU rtol 1e-10", "U_solver": "BiCGSTAB+DILU rtol 0.1",
+                   "p_solver": "CG+DILU rtol 1e-10" if args.precision == "fp64" else
+                               "FP32 CG+DILU (inner rtol 1e-4) in FP64 refinement to 1e-10",
+                   "U_solver": "BiCGSTAB+DILU rtol 0.1",
                    "parallelism": "single GPU"},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "detail": {"mesh_build_s": round(t_mesh, 2), "upload_s": round(t_upload, 2),
                    "device_ms": {k: round(v, 2) for k, v in stages.items()},
                    "p_cg_iters_mean": float(np.mean(kit[3])),
-                   "U_bicgstab_iters_mean": float(np.mean(kit[:3]))},
+                   "U_bicgstab_iters_mean": float(np.mean(kit[:3])),
+                   "error_vs_fp64": err},
     }
     print(json.dumps(line))
 

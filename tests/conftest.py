@@ -91,7 +91,8 @@ def rel(a, b):
     return np.linalg.norm(a - b) / (d if d > 0 else 1.0)
 
 
-def cavity_case(api, mesh, nu=0.01, iterations=200, precon="DILU", rtol_p=0.1):
+def cavity_case(api, mesh, nu=0.01, iterations=200, precon="DILU", rtol_p=0.1,
+                precision_p="fp64"):
     P = getattr(api, precon)
     model = api.Physics(domain=mesh, fluid=api.Fluid(nu=nu))
     BCs = api.assign(region=mesh,
@@ -101,7 +102,8 @@ def cavity_case(api, mesh, nu=0.01, iterations=200, precon="DILU", rtol_p=0.1):
     solvers = dict(U=api.SolverSetup(solver=api.Bicgstab(), preconditioner=P(),
                                      convergence=1e-7, relax=0.7),
                    p=api.SolverSetup(solver=api.Cg(), preconditioner=P(),
-                                     convergence=1e-7, relax=0.3, rtol=rtol_p))
+                                     convergence=1e-7, relax=0.3, rtol=rtol_p,
+                                     precision=precision_p))
     config = api.Configuration(solvers=solvers,
                                schemes=dict(U=api.Schemes(divergence=api.Upwind), p=api.Schemes()),
                                runtime=api.Runtime(iterations=iterations, write_interval=-1),
@@ -110,7 +112,7 @@ def cavity_case(api, mesh, nu=0.01, iterations=200, precon="DILU", rtol_p=0.1):
 
 
 def channel_case(api, mesh, nu=1e-3, iterations=10, precon="DILU", rtol_p=1e-10,
-                 div=None):
+                 div=None, precision_p="fp64"):
     P = getattr(api, precon)
     model = api.Physics(domain=mesh, fluid=api.Fluid(nu=nu))
     api.initialise(model.momentum.U, [1.0, 0.0, 0.0])
@@ -122,7 +124,8 @@ def channel_case(api, mesh, nu=1e-3, iterations=10, precon="DILU", rtol_p=1e-10,
     solvers = dict(U=api.SolverSetup(solver=api.Bicgstab(), preconditioner=P(),
                                      convergence=1e-7, relax=0.7),
                    p=api.SolverSetup(solver=api.Cg(), preconditioner=P(),
-                                     convergence=1e-7, relax=0.3, rtol=rtol_p))
+                                     convergence=1e-7, relax=0.3, rtol=rtol_p,
+                                     precision=precision_p))
     config = api.Configuration(
         solvers=solvers,
         schemes=dict(U=api.Schemes(divergence=div or api.Upwind), p=api.Schemes()),

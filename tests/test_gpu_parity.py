@@ -302,6 +302,50 @@ def test_krylov_matches_oracle(x, A, O, ctx, meshes, solver, precon):
     assert res < 1e-9
 
 
+@pytest.mark.parametrize("solver,precon", [("SOLVER_CG", "PRECON_DILU"), ("SOLVER_CG", "PRECON_JACOBI"),
+                                           ("SOLVER_BICGSTAB", "PRECON_DILU"),
+                                           ("SOLVER_CG", "PRECON_NONE")])
+def test_fp32_refinement_matches_fp64_oracle(x, A, O, ctx, meshes, solver, precon):
+    """Mixed precision: FP32 Krylov inner solves (inner rtol 1e-4) inside an
+    FP64 refinement loop driven to a true-residual reduction of 1e-12.
+    Stated tolerance: relative L2 against the oracle's FP64 solve (rtol
+    1e-12) <= 1e-8, reached in more than one refinement sweep."""
+    mesh = meshes("box20")
+    rng = _rng(8)
+    H = x.hip
+    dm = H.DeviceMesh(ctx, mesh)
+    om = O.OracleMesh(mesh, A)
+    F = _fields(mesh, rng)
+    g = H.Field(dm, A.FACE_SCALAR).upload(F["gamma"])
+    sym = solver == "SOLVER_CG"
+    mdh = np.zeros(mesh.n_faces) if sym else F["mdot"] * 0.2
+    md = H.Field(dm, A.FACE_SCALAR).upload(mdh)
+    terms = [(A.DIV_UPWIND, 1, md), (A.LAPLACIAN_LINEAR, -1, g)]
+    oterms = [(A.DIV_UPWIND, 1, mdh), (A.LAPLACIAN_LINEAR, -1, F["gamma"])]
+    src = H.Field(dm, A.CELL_SCALAR).upload(F["src"])
+    phi = H.Field(dm, A.CELL_SCALAR).upload(F["phi"] * 0.01)
+    eq = H.Equation(dm, False)
+    eq.discretise(terms, [(src, 1)], phi)
+    bcs = A.bc_array([(A.BC_DIRICHLET, i, 1.0) for i in range(len(mesh.patch_names))])
+    eq.apply_bcs(terms, bcs, phi)
+    onz, ob = om.discretise_scalar(oterms, [(F["src"], 1)])
+    om.apply_bcs(oterms, bcs, -1, F["phi"] * 0.01, onz, ob)
+    pre = getattr(A, precon)
+    eq.precond_update(pre)
+    setup = A.SolverSetupC(getattr(A, solver), pre, A.PREC_FP32_IR, 4000, 1e-7, 1.0, 0.0, 1e-12,
+                           1e-4)
+    it, res, rc = eq.solve(setup, phi)
+    ox, oit, orc = om.krylov(getattr(A, solver), pre, onz, ob, F["phi"] * 0.01, itmax=4000,
+                             atol=1e-30, rtol=1e-12)
+    assert rc == 0 and orc == 0
+    e = rel(phi.download(), ox)
+    print(f"{solver}+{precon}: fp32-ir {it} inner its in {eq.last_nouter} sweeps "
+          f"(fp64 oracle {oit}); rel L2 {e:.2e}; residual {res:.2e}")
+    assert eq.last_nouter >= 2
+    assert e < 1e-8
+    assert res < 1e-10
+
+
 def test_cavity_simple_matches_oracle(x, A, O, meshes):
     """40x40 lid-driven cavity, Re=100, 200 SIMPLE iterations, DILU.
     Stated tolerance: relative L2 of U and p against the oracle <= 1e-6."""
@@ -339,5 +383,31 @@ def test_channel_simple_matches_oracle(x, A, O, meshes, name):
     em = rel(sim.get("mdotf"), ref["mdotf"])
     print(f"{name}: rel L2 U {eU:.3e} p {ep:.3e} mdot {em:.3e}; krylov its gpu "
           f"{out['krylov_iters'][:, -1]} oracle {ref['krylov_iters'][:, -1]}")
+    assert out["done"] == ref["done"]
+    assert eU < 1e-8 and ep < 1e-8 and em < 1e-8
+
+
+def test_channel_simple_fp32_pressure_matches_fp64_oracle(x, A, O, meshes):
+    """Mixed-precision form of the channel: the pressure solve runs as FP32
+    CG+DILU sweeps inside FP64 refinement to a 1e-10 true-residual reduction;
+    assembly, residuals and corrections stay FP64.  Stated tolerance:
+    relative L2 of U, p and mdotf against the FP64 oracle run <= 1e-8."""
+    api = x.api
+    mesh = meshes("box12")
+    model, config = channel_case(api, mesh, iterations=10, precision_p="fp32-ir")
+    setup = api.simple_setup(model, config)
+    model64, config64 = channel_case(api, mesh, iterations=10)
+    setup64 = api.simple_setup(model64, config64)
+    ref = O.OracleMesh(mesh, A).simple_run(setup64, model64.momentum.U, model64.momentum.p, 10)
+    H = x.hip
+    dm = H.DeviceMesh(api._context(0), mesh)
+    sim = H.Simple(dm, setup)
+    sim.set_U(model.momentum.U)
+    sim.set_p(model.momentum.p)
+    out = sim.run(10)
+    eU = rel(sim.get("U"), ref["U"])
+    ep = rel(sim.get("p"), ref["p"])
+    em = rel(sim.get("mdotf"), ref["mdotf"])
+    print(f"box12 fp32-ir: rel L2 U {eU:.3e} p {ep:.3e} mdot {em:.3e}")
     assert out["done"] == ref["done"]
     assert eU < 1e-8 and ep < 1e-8 and em < 1e-8

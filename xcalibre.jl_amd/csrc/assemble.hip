@@ -99,13 +99,13 @@ __global__ void __launch_bounds__(kBlock)
 
 // _implicit_relaxation_diagdom! (Solve_1_api.jl:388-411)
 __global__ void __launch_bounds__(kBlock)
-    k_relax_diagdom(int64_t n, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ diag,
-                    double* __restrict__ nzval, double* __restrict__ b,
-                    const double* __restrict__ field, double alpha) {
+    k_relax_diagdom(int64_t n, const int32_t* __restrict__ rb, const int32_t* __restrict__ re,
+                    const int32_t* __restrict__ diag, double* __restrict__ nzval,
+                    double* __restrict__ b, const double* __restrict__ field, double alpha) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double sumv = 0.0;
-  for (int32_t k = rowptr[i]; k < rowptr[i + 1]; ++k) sumv += fabs(nzval[k]);
+  for (int32_t k = rb[i]; k < re[i]; k += kSlice) sumv += fabs(nzval[k]);
   const int c = diag[i];
   const double D0 = nzval[c];
   sumv -= fabs(D0);
@@ -154,7 +154,7 @@ void apply_bcs(xcal_eqn* e, const TermsK& t, const BCsK& b, const xcal_field* ph
 void relax_diagdom(xcal_eqn* e, const double* field, double* b, double alpha) {
   xcal_mesh* m = e->mesh;
   hipLaunchKernelGGL(k_relax_diagdom, dim3(grid_for(m->n)), dim3(kBlock), 0, m->ctx->stream, m->n,
-                     m->rowptr.p, m->diag.p, e->nzval.p, b, field, alpha);
+                     m->rbeg.p, m->rend.p, m->diag.p, e->nzval.p, b, field, alpha);
   XCAL_HIP(hipGetLastError());
 }
 
@@ -166,7 +166,7 @@ void set_reference(xcal_eqn* e, double pref, int64_t cell) {
 }
 
 void copy_nz(xcal_mesh* m, double* dst, const double* src) {
-  XCAL_HIP(hipMemcpyAsync(dst, src, sizeof(double) * m->nnz, hipMemcpyDeviceToDevice,
+  XCAL_HIP(hipMemcpyAsync(dst, src, sizeof(double) * m->nzlen, hipMemcpyDeviceToDevice,
                           m->ctx->stream));
 }
 
@@ -212,8 +212,8 @@ int xcal_eqn_create(xcal_mesh* m, int32_t vector, xcal_eqn** out) {
     auto e = std::make_unique<xcal_eqn>();
     e->mesh = m;
     e->vector = vector != 0;
-    e->nzval.alloc(m->nnz);
-    if (e->vector) e->nzval0.alloc(m->nnz);
+    e->nzval.alloc(m->nzlen);
+    if (e->vector) e->nzval0.alloc(m->nzlen);
     e->b.alloc((e->vector ? 3 : 1) * m->n);
     XCAL_HIP(hipMemsetAsync(e->nzval.p, 0, e->nzval.bytes(), m->ctx->stream));
     if (e->vector) XCAL_HIP(hipMemsetAsync(e->nzval0.p, 0, e->nzval0.bytes(), m->ctx->stream));
@@ -304,7 +304,8 @@ int xcal_eqn_download(const xcal_eqn* e, int32_t which, int32_t comp, double* nz
     XCAL_CHECK(which == 0 || (which == 1 && e->vector), "which: 0 = A, 1 = A0 (vector only)");
     XCAL_CHECK(comp >= 0 && comp < (e->vector ? 3 : 1), "component out of range");
     const xcal_mesh* m = e->mesh;
-    if (nzval) download_permuted(m, m->nzperm, which ? e->nzval0.p : e->nzval.p, nzval, m->nnz);
+    if (nzval) download_permuted(m, m->nzperm, which ? e->nzval0.p : e->nzval.p, nzval, m->nnz,
+                                  m->nzlen);
     if (b) download_permuted(m, m->cperm, e->bc(comp), b, m->n);
     return 0;
   });

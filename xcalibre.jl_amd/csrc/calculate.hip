@@ -222,15 +222,16 @@ __global__ void __launch_bounds__(kBlock)
 
 // _H! (Solvers_0_functions.jl:201-249)
 __global__ void __launch_bounds__(kBlock)
-    k_H(int64_t n, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colval,
-        const int32_t* __restrict__ diag, const double* __restrict__ nz,
+    k_H(int64_t n, const int32_t* __restrict__ rb, const int32_t* __restrict__ re,
+        const int32_t* __restrict__ colval, const int32_t* __restrict__ diag,
+        const double* __restrict__ nz,
         const double* __restrict__ bx, const double* __restrict__ by, const double* __restrict__ bz,
         const double* __restrict__ Ux, const double* __restrict__ Uy, const double* __restrict__ Uz,
         double* __restrict__ Hx, double* __restrict__ Hy, double* __restrict__ Hz) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double sx = 0.0, sy = 0.0, sz = 0.0;
-  for (int32_t k = rowptr[i]; k < rowptr[i + 1]; ++k) {
+  for (int32_t k = rb[i]; k < re[i]; k += kSlice) {
     const int j = colval[k];
     const double v = nz[k];
     sx += v * Ux[j];
@@ -356,7 +357,7 @@ void remove_pressure_source(xcal_eqn* e, const double* gx, const double* gy, con
 void Hop(const xcal_eqn* e, const double* Ux, const double* Uy, const double* Uz, double* Hx,
          double* Hy, double* Hz) {
   const xcal_mesh* m = e->mesh;
-  LAUNCH(k_H, m->n, m->n, m->rowptr.p, m->colval.p, m->diag.p, e->nzval.p, e->bc(0), e->bc(1),
+  LAUNCH(k_H, m->n, m->n, m->rbeg.p, m->rend.p, m->colval.p, m->diag.p, e->nzval.p, e->bc(0), e->bc(1),
          e->bc(2), Ux, Uy, Uz, Hx, Hy, Hz);
 }
 

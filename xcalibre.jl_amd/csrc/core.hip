@@ -1,5 +1,6 @@
 // Context, mesh upload and fields of libxcalhip.so.
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <numeric>
@@ -62,6 +63,56 @@ __global__ void k_fill(double* p, int64_t n, double v) {
   if (i < n) p[i] = v;
 }
 
+// Launch plans of the DILU sweeps.  A level with at least `small` rows gets a
+// launch of its own over many workgroups; a run of consecutive smaller levels
+// is done by one workgroup that steps through them with barriers, which costs
+// far less than a launch per level.  Each launch owns a range of
+// partial-sum slots (one per workgroup) for the dot products that the
+// backward sweep folds in.
+static std::vector<SweepStep> plan_levels(const std::vector<int32_t>& ptr, int nlev, int small) {
+  std::vector<SweepStep> plan;
+  int32_t slot = 0;
+  for (int l = 0; l < nlev;) {
+    const int32_t cnt = ptr[l + 1] - ptr[l];
+    SweepStep st{};
+    st.lev0 = l;
+    st.row0 = ptr[l];
+    if (cnt >= small) {
+      st.lev1 = l + 1;
+      st.grid = grid_for(cnt);
+    } else {
+      int e = l;
+      while (e < nlev && ptr[e + 1] - ptr[e] < small) ++e;
+      st.lev1 = e;
+      st.grid = 1;
+    }
+    st.row1 = ptr[st.lev1];
+    st.slot = slot;
+    slot += st.grid;
+    plan.push_back(st);
+    l = st.lev1;
+  }
+  return plan;
+}
+
+void make_sweep_plans(xcal_mesh* m, int small) {
+  if (small <= 0) {
+    small = kSmallLevel;
+    if (const char* env = std::getenv("XCAL_SMALL_LEVEL")) small = std::max(1, std::atoi(env));
+  }
+  m->small_level = small;
+  m->plan_f = plan_levels(m->lev_ptr_f, m->nlev_f, small);
+  m->plan_b = plan_levels(m->lev_ptr_b, m->nlev_b, small);
+  m->sweep_slots = 0;
+  for (const auto& st : m->plan_b) m->sweep_slots = std::max(m->sweep_slots, st.slot + st.grid);
+  int64_t bytes = 0;
+  if (m->lev_ptr_f_d.n == 0) {
+    upload(m->lev_ptr_f_d, m->lev_ptr_f, bytes);
+    upload(m->lev_ptr_b_d, m->lev_ptr_b, bytes);
+    m->device_bytes += bytes;
+  }
+}
+
 void fill(hipStream_t s, double* p, int64_t n, double v) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_fill, dim3(grid_for(n)), dim3(kBlock), 0, s, p, n, v);
@@ -87,10 +138,7 @@ int xcal_ctx_create(int32_t device, xcal_ctx** out) {
     c->device = device;
     try {
       XCAL_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-      c->red_partial.alloc((int64_t)kRedBlocks * 8);
-      c->red_ticket.alloc(8);
       c->red_result.alloc(64);
-      XCAL_HIP(hipMemset(c->red_ticket.p, 0, sizeof(unsigned) * 8));
       XCAL_HIP(hipHostMalloc((void**)&c->host_pinned, sizeof(double) * 64, hipHostMallocDefault));
     } catch (...) {
       if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -257,46 +305,34 @@ int xcal_mesh_create(xcal_ctx* ctx, const xcal_mesh_host* h, xcal_mesh** out) {
     // DILU level schedules: forward level(i) = 1 + max level of lower
     // neighbours; backward symmetric over upper neighbours.  Rows inside a
     // level are independent; level order preserves every i<j dependency.
-    std::vector<int32_t> lev(n, 0);
+    std::vector<int32_t> flev(n, 0), blev(n, 0);
     int maxl = 0;
     for (int64_t i = 0; i < n; ++i) {
       int l = 0;
-      for (int32_t k = rowptr[i]; k < diag[i]; ++k) l = std::max(l, lev[colval[k]] + 1);
-      lev[i] = l;
+      for (int32_t k = rowptr[i]; k < diag[i]; ++k) l = std::max(l, flev[colval[k]] + 1);
+      flev[i] = l;
       maxl = std::max(maxl, l);
     }
-    auto bucket = [&](int nl, std::vector<int32_t>& ptr, std::vector<int32_t>& rows, bool asc) {
-      ptr.assign(nl + 1, 0);
-      for (int64_t i = 0; i < n; ++i) ptr[lev[i] + 1]++;
-      for (int l = 0; l < nl; ++l) ptr[l + 1] += ptr[l];
-      rows.assign(n, 0);
-      std::vector<int32_t> cur(ptr.begin(), ptr.end() - 1);
-      if (asc)
-        for (int64_t i = 0; i < n; ++i) rows[cur[lev[i]]++] = (int32_t)i;
-      else
-        for (int64_t i = n - 1; i >= 0; --i) rows[cur[lev[i]]++] = (int32_t)i;
-    };
-    std::vector<int32_t> rows_f, rows_b;
     m->nlev_f = maxl + 1;
-    bucket(m->nlev_f, m->lev_ptr_f, rows_f, true);
     maxl = 0;
     for (int64_t i = n - 1; i >= 0; --i) {
       int l = 0;
-      for (int32_t k = diag[i] + 1; k < rowptr[i + 1]; ++k) l = std::max(l, lev[colval[k]] + 1);
-      lev[i] = l;
+      for (int32_t k = diag[i] + 1; k < rowptr[i + 1]; ++k) l = std::max(l, blev[colval[k]] + 1);
+      blev[i] = l;
       maxl = std::max(maxl, l);
     }
     m->nlev_b = maxl + 1;
-    bucket(m->nlev_b, m->lev_ptr_b, rows_b, false);
     std::vector<double> vol(h->cell_volume, h->cell_volume + n);
 
-    // ---- locality renumbering of the device copy -------------------------
-    // Cells are stored in Morton order of their centres and internal faces
-    // in order of their renumbered cells, so that neighbour gathers hit
-    // nearby cache lines instead of random ones.  Every CSR row keeps its
-    // entries in the reference's (original) column order, so row sums, the
-    // DILU lower/upper split and all per-row summation orders are unchanged;
-    // fields are permuted only when they cross the C ABI.
+    // ---- device numbering ------------------------------------------------
+    // Cells are stored level-major: by forward DILU level, and in Morton
+    // order of their centres inside a level.  Each forward level is then one
+    // contiguous row range, and rows that are neighbours in a level are
+    // neighbours in space, so neighbour gathers hit nearby cache lines.
+    // Internal faces follow their renumbered cells.  Every matrix row keeps
+    // its entries in the reference's (original) column order, so row sums,
+    // the DILU lower/upper split and all per-row summation orders are
+    // unchanged; fields are permuted only when they cross the C ABI.
     std::vector<int32_t> cperm(n), cinv(n), fperm(nf), finv(nf);
     {
       double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
@@ -312,7 +348,7 @@ int xcal_mesh_create(xcal_ctx* ctx, const xcal_mesh_host* h, xcal_mesh** out) {
         for (int b = 0; b < bits; ++b) r |= ((v >> b) & 1ull) << (naxis * b);
         return r;
       };
-      std::vector<std::pair<uint64_t, int32_t>> key(n);
+      std::vector<std::pair<std::pair<int32_t, uint64_t>, int32_t>> key(n);
       for (int64_t c = 0; c < n; ++c) {
         uint64_t k = 0;
         for (int a = 0; a < naxis; ++a) {
@@ -321,7 +357,7 @@ int xcal_mesh_create(xcal_ctx* ctx, const xcal_mesh_host* h, xcal_mesh** out) {
           uint64_t q = (uint64_t)std::min(t * (double)((1ull << bits) - 1), (double)((1ull << bits) - 1));
           k |= spread(q) << a;
         }
-        key[c] = {k, (int32_t)c};
+        key[c] = {{flev[c], k}, (int32_t)c};
       }
       std::sort(key.begin(), key.end());
       for (int64_t c = 0; c < n; ++c) {
@@ -341,13 +377,49 @@ int xcal_mesh_create(xcal_ctx* ctx, const xcal_mesh_host* h, xcal_mesh** out) {
         fperm[fk[k].second] = (int32_t)(nbf + k);
       }
     }
-    std::vector<int32_t> rowptr_n(n + 1, 0), colval_n(nnz), nzperm(nnz), diag_n(n), trans_n(nnz);
-    for (int64_t c = 0; c < n; ++c) rowptr_n[c + 1] = rowptr_n[c] + (rowptr[cinv[c] + 1] - rowptr[cinv[c]]);
+    // Matrix storage: sliced ELL with slices of kSlice consecutive rows, each
+    // slice column-major and as wide as its longest row.  Entry t of device
+    // row c sits at rbeg[c] + t*kSlice, so a wavefront reading entry t of 64
+    // rows reads one contiguous segment.  Padding entries are never read.
+    const int64_t nslice = (n + kSlice - 1) / kSlice;
+    std::vector<int32_t> rbeg(n), rend(n);
+    int64_t nzlen = 0;
+    for (int64_t s = 0; s < nslice; ++s) {
+      int32_t w = 0;
+      const int64_t c1 = std::min<int64_t>(n, (s + 1) * kSlice);
+      for (int64_t c = s * kSlice; c < c1; ++c) w = std::max(w, rowptr[cinv[c] + 1] - rowptr[cinv[c]]);
+      for (int64_t c = s * kSlice; c < c1; ++c) {
+        rbeg[c] = (int32_t)(nzlen + (c - s * kSlice));
+        rend[c] = rbeg[c] + kSlice * (rowptr[cinv[c] + 1] - rowptr[cinv[c]]);
+      }
+      nzlen += (int64_t)w * kSlice;
+      XCAL_CHECK(nzlen < (1ll << 31), "matrix storage exceeds int32 indices");
+    }
+    m->nzlen = nzlen;
+    std::vector<int32_t> colval_n(nzlen), nzperm(nnz), diag_n(n), trans_n(nzlen, -1);
     for (int64_t oc = 0; oc < n; ++oc)
-      for (int32_t k = rowptr[oc]; k < rowptr[oc + 1]; ++k) nzperm[k] = rowptr_n[cperm[oc]] + (k - rowptr[oc]);
+      for (int32_t k = rowptr[oc]; k < rowptr[oc + 1]; ++k)
+        nzperm[k] = rbeg[cperm[oc]] + kSlice * (k - rowptr[oc]);
     for (int64_t k = 0; k < nnz; ++k) colval_n[nzperm[k]] = cperm[colval[k]];
     for (int64_t k = 0; k < nnz; ++k) trans_n[nzperm[k]] = trans[k] >= 0 ? nzperm[trans[k]] : -1;
     for (int64_t oc = 0; oc < n; ++oc) diag_n[cperm[oc]] = nzperm[diag[oc]];
+    // level schedules in device numbering: forward levels are the contiguous
+    // ranges of the level-major order; backward levels list their rows
+    std::vector<int32_t> rows_b;
+    {
+      m->lev_ptr_f.assign(m->nlev_f + 1, 0);
+      for (int64_t i = 0; i < n; ++i) m->lev_ptr_f[flev[i] + 1]++;
+      for (int l = 0; l < m->nlev_f; ++l) m->lev_ptr_f[l + 1] += m->lev_ptr_f[l];
+      for (int64_t c = 0; c < n; ++c)
+        XCAL_CHECK(c >= m->lev_ptr_f[flev[cinv[c]]] && c < m->lev_ptr_f[flev[cinv[c]] + 1],
+                   "internal: level-major numbering");
+      m->lev_ptr_b.assign(m->nlev_b + 1, 0);
+      for (int64_t i = 0; i < n; ++i) m->lev_ptr_b[blev[i] + 1]++;
+      for (int l = 0; l < m->nlev_b; ++l) m->lev_ptr_b[l + 1] += m->lev_ptr_b[l];
+      rows_b.assign(n, 0);
+      std::vector<int32_t> cur(m->lev_ptr_b.begin(), m->lev_ptr_b.end() - 1);
+      for (int64_t c = 0; c < n; ++c) rows_b[cur[blev[cinv[c]]]++] = (int32_t)c;
+    }
     std::vector<int32_t> cf_ptr_n(n + 1, 0), cf_face_n(cf_face.size()), cf_nbr_n(cf_face.size()),
         cf_slot_n(cf_face.size());
     std::vector<int8_t> cf_sign_n(cf_face.size());
@@ -376,8 +448,6 @@ int xcal_mesh_create(xcal_ctx* ctx, const xcal_mesh_host* h, xcal_mesh** out) {
       if (f >= nbf) fslot_n[fperm[f]] = nzperm[fslot[f]];
     }
     for (auto& c : bc_cells) c = cperm[c];
-    for (auto& r : rows_f) r = cperm[r];
-    for (auto& r : rows_b) r = cperm[r];
     {
       std::vector<double> t(n);
       for (int64_t c = 0; c < n; ++c) t[cperm[c]] = vol[c];
@@ -399,7 +469,6 @@ int xcal_mesh_create(xcal_ctx* ctx, const xcal_mesh_host* h, xcal_mesh** out) {
     // original-numbering CSR stays on the host for xcal_mesh_csr_pattern
     m->h_rowptr = rowptr;
     m->h_colval = colval;
-    rowptr.swap(rowptr_n);
     colval.swap(colval_n);
 
     upload(m->vol, vol, bytes);
@@ -429,12 +498,13 @@ int xcal_mesh_create(xcal_ctx* ctx, const xcal_mesh_host* h, xcal_mesh** out) {
     upload(m->bc_ptr, bc_ptr, bytes);
     upload(m->bc_face, bc_face, bytes);
     upload(m->bface_patch, bpatch, bytes);
-    upload(m->rowptr, rowptr, bytes);
+    upload(m->rbeg, rbeg, bytes);
+    upload(m->rend, rend, bytes);
     upload(m->colval, colval, bytes);
     upload(m->trans, trans, bytes);
-    upload(m->lev_rows_f, rows_f, bytes);
     upload(m->lev_rows_b, rows_b, bytes);
     m->device_bytes = bytes;
+    make_sweep_plans(m.get(), 0);
     *out = m.release();
     return 0;
   });
@@ -523,9 +593,11 @@ void upload_permuted(const xcal_mesh* m, const std::vector<int32_t>& perm, doubl
   XCAL_HIP(hipStreamSynchronize(m->ctx->stream));
 }
 void download_permuted(const xcal_mesh* m, const std::vector<int32_t>& perm, const double* dev,
-                       double* host, int64_t len) {
-  std::vector<double> t(len);
-  XCAL_HIP(hipMemcpyAsync(t.data(), dev, sizeof(double) * len, hipMemcpyDeviceToHost, m->ctx->stream));
+                       double* host, int64_t len, int64_t dev_len) {
+  if (dev_len < 0) dev_len = len;
+  std::vector<double> t(dev_len);
+  XCAL_HIP(hipMemcpyAsync(t.data(), dev, sizeof(double) * dev_len, hipMemcpyDeviceToHost,
+                          m->ctx->stream));
   XCAL_HIP(hipStreamSynchronize(m->ctx->stream));
   for (int64_t i = 0; i < len; ++i) host[i] = t[perm[i]];
 }

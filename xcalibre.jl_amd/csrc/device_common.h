@@ -19,8 +19,22 @@ struct MeshK {
   int64_t nbcell;
   const int32_t *bc_cells, *bc_ptr, *bc_face;
   const int8_t* bface_patch;
-  const int32_t *rowptr, *colval, *trans;
 };
+
+// Matrix view (sliced-ELL storage): the entries of row i sit at
+// rb[i], rb[i] + kSlice, ... < re[i], in the reference's column order, and
+// diag[i] is the slot of the diagonal entry.
+template <class T>
+struct MatKT {
+  int64_t n;
+  const int32_t *rb, *re, *colval, *diag;
+  const T* nz;
+};
+using MatK = MatKT<double>;
+template <class T>
+inline MatKT<T> matk(const xcal_mesh* m, const T* nz) {
+  return MatKT<T>{m->n, m->rbeg.p, m->rend.p, m->colval.p, m->diag.p, nz};
+}
 
 inline MeshK meshk(const xcal_mesh* m) {
   MeshK k;
@@ -55,9 +69,6 @@ inline MeshK meshk(const xcal_mesh* m) {
   k.bc_ptr = m->bc_ptr.p;
   k.bc_face = m->bc_face.p;
   k.bface_patch = m->bface_patch.p;
-  k.rowptr = m->rowptr.p;
-  k.colval = m->colval.p;
-  k.trans = m->trans.p;
   return k;
 }
 

@@ -3,43 +3,34 @@
 // cg!/bicgstab!, compat 0.10, restated) with Jacobi or DILU preconditioning.
 //
 // Design for gfx950:
-//  * All Krylov scalars live in device memory (KS).  Each reduction ends in
-//    its kernel's last-finishing block, which updates the scalars
-//    (alpha, beta, convergence flags) in place; every kernel returns at entry
-//    once the solve has stopped.  The host enqueues batches of iterations
-//    and reads the state once per batch instead of once per dot product.
-//  * BLAS-1 work is fused into the SpMV and preconditioner sweeps:
+//  * All Krylov scalars live in device memory (KS).  A reduction writes one
+//    partial sum per workgroup with plain stores; a one-workgroup finalize
+//    kernel then adds the partials in slot order and updates the scalars
+//    (alpha, beta, convergence flags) in place.  The kernel boundary orders
+//    the two, so no device-scope fences or atomics are needed (a fence in
+//    every workgroup writes back and invalidates the XCD's L2, which costs
+//    more than the kernel's own memory traffic).  Every kernel returns at
+//    entry once the solve has stopped, so the host enqueues batches of
+//    iterations and reads the state once per batch.
+//  * BLAS-1 work is fused into the SpMV and the preconditioner sweeps:
 //      CG+DILU per iteration: [p = z + beta p ; q = A p ; p.q]
 //                             [x += a p ; r -= a q ; forward DILU sweep]
-//                             [backward DILU sweep ; r.z ; beta]
-//  * DILU sweeps are single sync-free kernels over the level-ordered rows:
-//    a row's result is published with one 64-bit atomic store into a buffer
-//    pre-filled with a signalling-NaN sentinel, and dependants spin on the
-//    value itself (no separate flags, no fences).  Chunks of 64 rows are taken
-//    in level order from a ticket counter, so every chunk a wave waits on was
-//    taken earlier by a resident wave and the oldest unfinished row can
-//    always proceed; the spin is bounded and sets an error flag instead of
-//    hanging.  The summation order inside every row is the reference's
-//    (ascending column), so the sweeps are bit-identical to
-//    forward_substitution!/backward_substitution!
+//                             [backward DILU sweep ; r.z]
+//  * DILU sweeps are level-synchronous: each level is a set of independent
+//    rows; large levels get a launch each, runs of small levels are swept by
+//    one workgroup with a barrier between levels.  The summation order in
+//    every row is the reference's (ascending column), so the sweeps are
+//    bit-identical to forward_substitution!/backward_substitution!
 //    (preconditioners_1_DILU.jl:84-114).
-//  * Reductions are deterministic: fixed partial-sum slots per block or per
-//    chunk, summed in a fixed order by the finishing block.
+//  * Reductions are deterministic: fixed partial-sum slots per workgroup,
+//    summed in a fixed order.
 #include <cmath>
 #include <cstring>
-#include <cstdlib>
 #include <functional>
-#include <tuple>
-#include <utility>
 
 #include "device_common.h"
 
 namespace xcal {
-
-constexpr uint64_t kSentinel = 0x7FF4DEADBEEF0001ull;  // never produced by arithmetic
-constexpr int kArrive = 64;         // first-level arrival counters of a reduction
-constexpr unsigned kSpinLimit = 1u << 22;
-constexpr int kMaxSweepBlocks = 2048;  // partial-sum slots reserved for the sweep reductions
 
 enum : int { STOP_NONE = 0, STOP_SOLVED = 1, STOP_ITMAX = 2, STOP_BREAKDOWN = 3,
              STOP_ZERO_RHS = 4 };
@@ -127,254 +118,210 @@ __device__ inline void finalize(KS* ks, int fin, double d0, double d1) {
   }
 }
 
-// ------------------------------------------------------- reduction tail
-__device__ inline void st_agent(double* p, double v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ inline double ld_agent(const double* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
 
-// Arrival of one of `count` blocks at a reduction.  Same-address atomics
-// serialise, so blocks first count in one of 64 first-level counters
-// (slot % 64) and only the block completing a group touches the second
-// level; returns true in exactly one block, the last to arrive.
-__device__ inline bool arrive(unsigned* cnt, int slot, int count) {
-  const int groups = count < kArrive ? count : kArrive;
-  const int g = slot % kArrive;
-  const unsigned gsize = (unsigned)((count - g + kArrive - 1) / kArrive);
-  __threadfence();
-  if (atomicAdd(&cnt[g], 1u) != gsize - 1u) return false;
-  cnt[g] = 0u;  // every member of this group has arrived
-  __threadfence();
-  if (atomicAdd(&cnt[kArrive], 1u) != (unsigned)groups - 1u) return false;
-  cnt[kArrive] = 0u;
-  __threadfence();
-  return true;
-}
+// ---------------------------------------------------------- reductions
+// The kernels below are templates on the value type T of the matrix and the
+// vectors: double for the FP64 solve, float for the inner solves of the
+// mixed-precision (FP32 inner, FP64 refinement) mode.  Dot products are
+// accumulated in double in both cases; the Krylov scalars are double.
+__device__ inline bool stopped(const KS* ks) { return ks->stop != STOP_NONE; }
 
-// Block `slot` publishes its two partial sums; the last of `count` blocks to
-// arrive sums all slots in slot order and runs the scalar update.
-__device__ void reduce_tail(double s0, double s1, int slot, int count, double* partial,
-                            unsigned* done, KS* ks, int fin, double* lds) {
-  __shared__ bool last;
+// workgroup sums of s0, s1 -> partial[2 slot], partial[2 slot + 1]
+template <int NT>
+__device__ inline void store_partial(double s0, double s1, double* partial, int slot) {
+  __shared__ double lds[NT / 64];
   s0 = block_sum(s0, lds);
   s1 = block_sum(s1, lds);
   if (threadIdx.x == 0) {
-    st_agent(&partial[2 * slot], s0);
-    st_agent(&partial[2 * slot + 1], s1);
-    last = arrive(done, slot, count);
+    partial[2 * slot] = s0;
+    partial[2 * slot + 1] = s1;
   }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
+}
+
+// Sums `count` partial pairs in slot order (fixed per-thread strides, fixed
+// tree) and runs the scalar update `fin`.
+__global__ void __launch_bounds__(kRunBlock)
+    k_finalize(const double* __restrict__ partial, int count, KS* ks, int fin) {
+  __shared__ double lds[kRunBlock / 64];
+  if (stopped(ks)) return;
   double t0 = 0.0, t1 = 0.0;
   for (int b = threadIdx.x; b < count; b += blockDim.x) {
-    t0 += ld_agent(&partial[2 * b]);
-    t1 += ld_agent(&partial[2 * b + 1]);
+    t0 += partial[2 * b];
+    t1 += partial[2 * b + 1];
   }
   t0 = block_sum(t0, lds);
   t1 = block_sum(t1, lds);
-  if (threadIdx.x == 0) {
-    finalize(ks, fin, t0, t1);
-    __threadfence();
-  }
+  if (threadIdx.x == 0) finalize(ks, fin, t0, t1);
 }
 
-__device__ inline bool stopped(const KS* ks) {
-  return __hip_atomic_load(&ks->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != STOP_NONE;
-}
-
-__device__ inline double ld_flag(const double* p) {
-  return __longlong_as_double(
-      (long long)__hip_atomic_load((const unsigned long long*)p, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT));
-}
-__device__ inline void st_flag(double* p, double v) {
-  __hip_atomic_store((unsigned long long*)p, (unsigned long long)__double_as_longlong(v),
-                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ inline bool is_sentinel(double v) {
-  return (unsigned long long)__double_as_longlong(v) == kSentinel;
-}
-
-struct MatK {
-  int64_t n;
-  const int32_t *rowptr, *colval, *diag;
-  const double* nz;
-};
+__device__ inline double dd(double a, double b) { return a * b; }  // products of dots in double
 
 // --------------------------------------------------------- static kernels
-// r = b - A x0 ; x = 0 ; pold = 0 (CG / BiCGSTAB start)
+// r = b - A x0 (r = b when x0 is NULL: zero start) ; x = 0 ; pold = 0
+template <class T>
 __global__ void __launch_bounds__(kBlock)
-    k_start(MatK A, const double* __restrict__ b, const double* __restrict__ x0,
-            double* __restrict__ r, double* __restrict__ x, double* __restrict__ pold) {
+    k_start(MatKT<T> A, const T* __restrict__ b, const T* __restrict__ x0, T* __restrict__ r,
+            T* __restrict__ x, T* __restrict__ pold) {
   const int64_t i = xcd_row();
   if (i >= A.n) return;
-  double acc = 0.0;
-  for (int32_t k = A.rowptr[i]; k < A.rowptr[i + 1]; ++k) acc += A.nz[k] * x0[A.colval[k]];
-  r[i] = b[i] - acc;
-  x[i] = 0.0;
-  pold[i] = 0.0;
+  if (x0) {
+    T acc = 0;
+    for (int32_t k = A.rb[i]; k < A.re[i]; k += kSlice) acc += A.nz[k] * x0[A.colval[k]];
+    r[i] = b[i] - acc;
+  } else {
+    r[i] = b[i];
+  }
+  x[i] = 0;
+  pold[i] = 0;
 }
 
 // z = S r (Jacobi) or z = r; partials r.z  [CG start]   or
 // r = S r0 ; partials r.r, c.r               [BiCGSTAB start]
-template <bool BICG>
+template <bool BICG, class T>
 __global__ void __launch_bounds__(kBlock)
-    k_precond_diag_start(int64_t n, const double* __restrict__ S, double* __restrict__ r,
-                         double* __restrict__ z, const double* __restrict__ c, double* partial,
-                         unsigned* done, KS* ks) {
-  __shared__ double lds[kBlock / 64];
+    k_precond_diag_start(int64_t n, const T* __restrict__ S, T* __restrict__ r, T* __restrict__ z,
+                         const T* __restrict__ c, double* partial) {
   const int64_t i = xcd_row();
   double s0 = 0.0, s1 = 0.0;
   if (i < n) {
     if (BICG) {
-      const double ri = S ? S[i] * r[i] : r[i];
+      const T ri = S ? S[i] * r[i] : r[i];
       r[i] = ri;
-      s0 = ri * ri;
-      s1 = c[i] * ri;
+      s0 = dd(ri, ri);
+      s1 = dd(c[i], ri);
     } else {
-      const double zi = S ? S[i] * r[i] : r[i];
+      const T zi = S ? S[i] * r[i] : r[i];
       z[i] = zi;
-      s0 = r[i] * zi;
+      s0 = dd(r[i], zi);
     }
   }
-  reduce_tail(s0, s1, blockIdx.x, gridDim.x, partial, done, ks, BICG ? FIN_BI_INIT : FIN_CG_INIT,
-              lds);
+  store_partial<kBlock>(s0, s1, partial, blockIdx.x);
 }
 
 // CG: pnew = z + beta pold (on the fly for neighbours) ; q = A pnew ; p.q
+template <class T>
 __global__ void __launch_bounds__(kBlock)
-    k_cg_spmv(MatK A, const double* __restrict__ z, const double* __restrict__ pold,
-              double* __restrict__ pnew, double* __restrict__ q, double* partial, unsigned* done,
-              KS* ks) {
-  __shared__ double lds[kBlock / 64];
+    k_cg_spmv(MatKT<T> A, const T* __restrict__ z, const T* __restrict__ pold,
+              T* __restrict__ pnew, T* __restrict__ q, double* partial, const KS* ks) {
   if (stopped(ks)) return;
-  const double beta = ks->beta;
+  const T beta = (T)ks->beta;
   const int64_t i = xcd_row();
   double s0 = 0.0;
   if (i < A.n) {
-    double acc = 0.0;
-    for (int32_t k = A.rowptr[i]; k < A.rowptr[i + 1]; ++k) {
+    T acc = 0;
+    for (int32_t k = A.rb[i]; k < A.re[i]; k += kSlice) {
       const int j = A.colval[k];
       acc += A.nz[k] * (z[j] + beta * pold[j]);
     }
-    const double pi = z[i] + beta * pold[i];
+    const T pi = z[i] + beta * pold[i];
     pnew[i] = pi;
     q[i] = acc;
-    s0 = pi * acc;
+    s0 = dd(pi, acc);
   }
-  reduce_tail(s0, 0.0, blockIdx.x, gridDim.x, partial, done, ks, FIN_CG_PAP, lds);
+  store_partial<kBlock>(s0, 0.0, partial, blockIdx.x);
 }
 
 // CG with a diagonal preconditioner: x += a p ; r -= a q ; z = S r ; r.z
+template <class T>
 __global__ void __launch_bounds__(kBlock)
-    k_cg_update_diag(int64_t n, const double* __restrict__ S, const double* __restrict__ p,
-                     const double* __restrict__ q, double* __restrict__ x, double* __restrict__ r,
-                     double* __restrict__ z, double* partial, unsigned* done, KS* ks) {
-  __shared__ double lds[kBlock / 64];
+    k_cg_update_diag(int64_t n, const T* __restrict__ S, const T* __restrict__ p,
+                     const T* __restrict__ q, T* __restrict__ x, T* __restrict__ r,
+                     T* __restrict__ z, double* partial, const KS* ks) {
   if (stopped(ks)) return;
-  const double a = ks->alpha;
+  const T a = (T)ks->alpha;
   const int64_t i = xcd_row();
   double s0 = 0.0;
   if (i < n) {
     x[i] = x[i] + a * p[i];
-    const double ri = r[i] + (-a) * q[i];
+    const T ri = r[i] + (-a) * q[i];
     r[i] = ri;
-    const double zi = S ? S[i] * ri : ri;
+    const T zi = S ? S[i] * ri : ri;
     z[i] = zi;
-    s0 = ri * zi;
+    s0 = dd(ri, zi);
   }
-  reduce_tail(s0, 0.0, blockIdx.x, gridDim.x, partial, done, ks, FIN_CG_ITER, lds);
+  store_partial<kBlock>(s0, 0.0, partial, blockIdx.x);
 }
 
 // BiCGSTAB: pnew = r + beta pold (on the fly) ; q = A pnew ; v = S q (diag
 // preconditioner, fused when VDIAG) ; c.v
-template <bool VDIAG>
+template <bool VDIAG, class T>
 __global__ void __launch_bounds__(kBlock)
-    k_bi_spmv_p(MatK A, const double* __restrict__ r, const double* __restrict__ pold,
-                double* __restrict__ pnew, const double* __restrict__ S, double* __restrict__ qv,
-                const double* __restrict__ c, double* partial, unsigned* done, KS* ks) {
-  __shared__ double lds[kBlock / 64];
+    k_bi_spmv_p(MatKT<T> A, const T* __restrict__ r, const T* __restrict__ pold,
+                T* __restrict__ pnew, const T* __restrict__ S, T* __restrict__ qv,
+                const T* __restrict__ c, double* partial, const KS* ks) {
   if (stopped(ks)) return;
-  const double beta = ks->beta;
+  const T beta = (T)ks->beta;
   const int64_t i = xcd_row();
   double s0 = 0.0;
   if (i < A.n) {
-    double acc = 0.0;
-    for (int32_t k = A.rowptr[i]; k < A.rowptr[i + 1]; ++k) {
+    T acc = 0;
+    for (int32_t k = A.rb[i]; k < A.re[i]; k += kSlice) {
       const int j = A.colval[k];
       acc += A.nz[k] * (r[j] + beta * pold[j]);
     }
     pnew[i] = r[i] + beta * pold[i];
     if (VDIAG) {
-      const double vi = S ? S[i] * acc : acc;
+      const T vi = S ? S[i] * acc : acc;
       qv[i] = vi;
-      s0 = c[i] * vi;
+      s0 = dd(c[i], vi);
     } else {
       qv[i] = acc;
     }
   }
-  if (VDIAG)
-    reduce_tail(s0, 0.0, blockIdx.x, gridDim.x, partial, done, ks, FIN_BI_ALPHA, lds);
+  if (VDIAG) store_partial<kBlock>(s0, 0.0, partial, blockIdx.x);
 }
 
 // BiCGSTAB: s = r - a v (on the fly) ; d = A s ; x += a p ;
 // t = S d and the dots t.s, t.t when TDIAG
-template <bool TDIAG>
+template <bool TDIAG, class T>
 __global__ void __launch_bounds__(kBlock)
-    k_bi_spmv_s(MatK A, const double* __restrict__ r, const double* __restrict__ v,
-                const double* __restrict__ p, double* __restrict__ s, double* __restrict__ x,
-                const double* __restrict__ S, double* __restrict__ dt, double* partial,
-                unsigned* done, KS* ks) {
-  __shared__ double lds[kBlock / 64];
+    k_bi_spmv_s(MatKT<T> A, const T* __restrict__ r, const T* __restrict__ v,
+                const T* __restrict__ p, T* __restrict__ s, T* __restrict__ x,
+                const T* __restrict__ S, T* __restrict__ dt, double* partial, const KS* ks) {
   if (stopped(ks)) return;
-  const double a = ks->alpha;
+  const T a = (T)ks->alpha;
   const int64_t i = xcd_row();
   double s0 = 0.0, s1 = 0.0;
   if (i < A.n) {
-    double acc = 0.0;
-    for (int32_t k = A.rowptr[i]; k < A.rowptr[i + 1]; ++k) {
+    T acc = 0;
+    for (int32_t k = A.rb[i]; k < A.re[i]; k += kSlice) {
       const int j = A.colval[k];
       acc += A.nz[k] * (r[j] + (-a) * v[j]);
     }
-    const double si = r[i] + (-a) * v[i];
+    const T si = r[i] + (-a) * v[i];
     s[i] = si;
     x[i] = x[i] + a * p[i];
     if (TDIAG) {
-      const double ti = S ? S[i] * acc : acc;
+      const T ti = S ? S[i] * acc : acc;
       dt[i] = ti;
-      s0 = ti * si;
-      s1 = ti * ti;
+      s0 = dd(ti, si);
+      s1 = dd(ti, ti);
     } else {
       dt[i] = acc;
     }
   }
-  if (TDIAG)
-    reduce_tail(s0, s1, blockIdx.x, gridDim.x, partial, done, ks, FIN_BI_OMEGA, lds);
+  if (TDIAG) store_partial<kBlock>(s0, s1, partial, blockIdx.x);
 }
 
 // BiCGSTAB tail: x += w s ; r = s - w t ; p = p - w v ; c.r and r.r
+template <class T>
 __global__ void __launch_bounds__(kBlock)
-    k_bi_tail(int64_t n, const double* __restrict__ s, const double* __restrict__ t,
-              const double* __restrict__ v, double* __restrict__ x, double* __restrict__ r,
-              double* __restrict__ p, const double* __restrict__ c, double* partial,
-              unsigned* done, KS* ks) {
-  __shared__ double lds[kBlock / 64];
+    k_bi_tail(int64_t n, const T* __restrict__ s, const T* __restrict__ t, const T* __restrict__ v,
+              T* __restrict__ x, T* __restrict__ r, T* __restrict__ p, const T* __restrict__ c,
+              double* partial, const KS* ks) {
   if (stopped(ks)) return;
-  const double w = ks->omega;
+  const T w = (T)ks->omega;
   const int64_t i = xcd_row();
   double s0 = 0.0, s1 = 0.0;
   if (i < n) {
     x[i] = x[i] + w * s[i];
-    const double ri = s[i] + (-w) * t[i];
+    const T ri = s[i] + (-w) * t[i];
     r[i] = ri;
     p[i] = p[i] + (-w) * v[i];
-    s0 = c[i] * ri;
-    s1 = ri * ri;
+    s0 = dd(c[i], ri);
+    s1 = dd(ri, ri);
   }
-  reduce_tail(s0, s1, blockIdx.x, gridDim.x, partial, done, ks, FIN_BI_END, lds);
+  store_partial<kBlock>(s0, s1, partial, blockIdx.x);
 }
 
 // x0 = x0 + x (warm start, Krylov.jl adds the start vector last)
@@ -384,274 +331,306 @@ __global__ void __launch_bounds__(kBlock)
   if (i < n) x0[i] = x[i] + x0[i];
 }
 
-__global__ void __launch_bounds__(kBlock) k_fill_bits(int64_t n, double* p, uint64_t bits) {
+// --------------------------------------------- mixed-precision refinement
+// r = b - A x in FP64, stored as FP32 for the inner solve; partials of
+// |r|^2 and |b|^2
+__global__ void __launch_bounds__(kBlock)
+    k_ir_residual(MatK A, const double* __restrict__ x, const double* __restrict__ b,
+                  float* __restrict__ r32, double* partial) {
   const int64_t i = xcd_row();
-  if (i < n) p[i] = __longlong_as_double((long long)bits);
+  double s0 = 0.0, s1 = 0.0;
+  if (i < A.n) {
+    double acc = 0.0;
+    for (int32_t k = A.rb[i]; k < A.re[i]; k += kSlice) acc += A.nz[k] * x[A.colval[k]];
+    const double r = b[i] - acc;
+    r32[i] = (float)r;
+    s0 = r * r;
+    s1 = b[i] * b[i];
+  }
+  store_partial<kBlock>(s0, s1, partial, blockIdx.x);
+}
+
+__global__ void __launch_bounds__(kRunBlock)
+    k_sum2(const double* __restrict__ partial, int count, double* __restrict__ out) {
+  __shared__ double lds[kRunBlock / 64];
+  double t0 = 0.0, t1 = 0.0;
+  for (int b = threadIdx.x; b < count; b += blockDim.x) {
+    t0 += partial[2 * b];
+    t1 += partial[2 * b + 1];
+  }
+  t0 = block_sum(t0, lds);
+  t1 = block_sum(t1, lds);
+  if (threadIdx.x == 0) {
+    out[0] = t0;
+    out[1] = t1;
+  }
+}
+
+// x += d (FP32 correction into the FP64 solution)
+__global__ void __launch_bounds__(kBlock)
+    k_ir_update(int64_t n, const float* __restrict__ d, double* __restrict__ x) {
+  const int64_t i = xcd_row();
+  if (i < n) x[i] = x[i] + (double)d[i];
 }
 
 __global__ void __launch_bounds__(kBlock)
-    k_copy_vec(int64_t n, const double* __restrict__ a, double* __restrict__ b) {
+    k_to_float(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
   const int64_t i = xcd_row();
-  if (i < n) b[i] = a[i];
+  if (i < n) b[i] = (float)a[i];
 }
 
-// ------------------------------------------------------ sync-free sweeps
-// The DILU sweeps run as persistent, co-resident (cooperative) kernels.  The
-// level-ordered rows are cut into 64-row chunks; wave w of W processes chunks
-// w, w+W, w+2W, ... in order, so the waves work on a moving window of about
-// W chunks that follows the dependency front.  Deadlock freedom: the smallest
-// unfinished chunk belongs to a wave that has finished all its earlier
-// (smaller) chunks and whose rows depend only on finished rows, so it can
-// always proceed; co-residency is guaranteed by the cooperative launch.
-// Polling backs off with s_sleep and is bounded (error flag, no hang).
-struct SweepK {
-  int64_t n;
-  int nchunks;          // 64-row chunks
-  const int32_t* rows;  // level-ordered rows
-  unsigned* done;
-  double* partial;
-  KS* ks;
-  int* err;
+// ------------------------------------------------------------ DILU sweeps
+// Forward sweep y = (I + L D^-1)^-1 u: row order and per-row summation order
+// of forward_substitution!.  UPD fuses the CG update of the row's own x and
+// r first (u = r after r -= a q).
+template <class T>
+struct FwdK {
+  MatKT<T> A;
+  const T* rD;
+  const T* u;
+  T* Y;
+  const T* p;
+  const T* q;
+  T* x;
+  T* r;
+  const KS* ks;  // NULL outside a Krylov solve
 };
 
-// Forward DILU sweep y = (I + L D^-1)^-1 u, row order and per-row summation
-// order of forward_substitution!.  UPD fuses the CG update of the row's own
-// x and r first (u = r after r -= a q).  `reset` (may be NULL) is the output
-// buffer of the following backward sweep: its row entries are set back to
-// the sentinel here.
-template <bool UPD>
-__global__ void __launch_bounds__(kBlock)
-    k_fwd(SweepK W, MatK A, const double* __restrict__ rD, const double* u_in, double* Y,
-          double* reset, const double* __restrict__ p, const double* __restrict__ q,
-          double* __restrict__ x, double* __restrict__ r) {
+template <bool UPD, class T>
+__device__ inline void fwd_row(const FwdK<T>& W, int i, T a) {
+  T u;
+  if (UPD) {
+    W.x[i] = W.x[i] + a * W.p[i];
+    u = W.r[i] + (-a) * W.q[i];
+    W.r[i] = u;
+  } else {
+    u = W.u[i];
+  }
+  T sum = 0;
+  for (int32_t k = W.A.rb[i]; k < W.A.diag[i]; k += kSlice) {
+    const int j = W.A.colval[k];
+    sum += (W.A.nz[k] * W.Y[j]) * W.rD[j];
+  }
+  W.Y[i] = u - sum;
+}
+
+// one level: rows [row0, row0 + cnt) (forward levels are contiguous)
+template <bool UPD, class T>
+__global__ void __launch_bounds__(kBlock) k_fwd_level(FwdK<T> W, int32_t row0, int32_t cnt) {
   if (W.ks && stopped(W.ks)) return;
-  const int lane = threadIdx.x & 63;
-  const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-  const int nwaves = (int)((gridDim.x * blockDim.x) >> 6);
-  const double a = UPD ? W.ks->alpha : 0.0;
-  for (int c = wave; c < W.nchunks; c += nwaves) {
-    const int64_t pos = (int64_t)c * 64 + lane;
-    const bool active = pos < W.n;
-    const int i = active ? W.rows[pos] : 0;
-    double ui = 0.0;
-    int32_t kp = 0, k1 = 0;
-    if (active) {
-      if (UPD) {
-        x[i] = x[i] + a * p[i];
-        ui = r[i] + (-a) * q[i];
-        r[i] = ui;
-      } else {
-        ui = u_in[i];
-      }
-      if (reset) st_flag(&reset[i], __longlong_as_double((long long)kSentinel));
-      kp = A.rowptr[i];
-      k1 = A.diag[i];
-    }
-    double sum = 0.0;
-    bool done = !active;
-    unsigned spins = 0;
-    while (true) {
-      if (!done) {
-        while (kp < k1) {
-          const int j = A.colval[kp];
-          const double yj = ld_flag(&Y[j]);
-          if (is_sentinel(yj)) break;
-          sum += (A.nz[kp] * yj) * rD[j];
-          ++kp;
-        }
-        if (kp == k1) {
-          st_flag(&Y[i], ui - sum);
-          done = true;
-        }
-      }
-      if (__all(done)) break;
-      if (++spins > kSpinLimit) {
-        atomicOr(W.err, 1);
-        return;
-      }
-      __builtin_amdgcn_s_sleep(2);
-    }
+  const T a = UPD ? (T)W.ks->alpha : T(0);
+  const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < cnt) fwd_row<UPD>(W, row0 + t, a);
+}
+
+// a run of small levels [lev0, lev1) in one workgroup
+template <bool UPD, class T>
+__global__ void __launch_bounds__(kRunBlock)
+    k_fwd_run(FwdK<T> W, const int32_t* __restrict__ lev_ptr, int lev0, int lev1) {
+  if (W.ks && stopped(W.ks)) return;
+  const T a = UPD ? (T)W.ks->alpha : T(0);
+  for (int l = lev0; l < lev1; ++l) {
+    const int32_t b = lev_ptr[l + 1];
+    for (int32_t i = lev_ptr[l] + threadIdx.x; i < b; i += blockDim.x) fwd_row<UPD>(W, i, a);
+    __syncthreads();
   }
 }
 
-// Backward DILU sweep z = (D + U)^-1 y, order of backward_substitution!,
-// same scheme as k_fwd.  Reads y from Y (and resets it), publishes z in Z,
-// optionally also writes a plain copy to `out`, and accumulates up to two dot
-// products:
+// Backward sweep z = (D + U)^-1 y: order of backward_substitution!.  Reads y
+// from Y, writes z to Z and folds up to two dot products into partials:
 //   DOT 0: none, 1: r.z (CG), 2: c.z (BiCGSTAB v), 3: z.s and z.z (BiCGSTAB t),
 //   4: z.z and c.z (BiCGSTAB start, z = M r0)
-template <int DOT>
+template <class T>
+struct BwdK {
+  MatKT<T> A;
+  const int32_t* rows;  // backward-level order
+  const T* D;
+  const T* Y;
+  T* Z;
+  const T* dv;
+  double* partial;
+  const KS* ks;
+};
+
+template <int DOT, class T>
+__device__ inline void bwd_row(const BwdK<T>& W, int i, double& s0, double& s1) {
+  T sum = 0;
+  for (int32_t k = W.A.diag[i] + kSlice; k < W.A.re[i]; k += kSlice)
+    sum += W.A.nz[k] * W.Z[W.A.colval[k]];
+  const T zi = (W.Y[i] - sum) / W.D[i];
+  W.Z[i] = zi;
+  if (DOT == 1 || DOT == 2) s0 += dd(W.dv[i], zi);
+  if (DOT == 3) { s0 += dd(zi, W.dv[i]); s1 += dd(zi, zi); }
+  if (DOT == 4) { s0 += dd(zi, zi); s1 += dd(W.dv[i], zi); }
+}
+
+template <int DOT, class T>
 __global__ void __launch_bounds__(kBlock)
-    k_bwd(SweepK W, MatK A, const double* __restrict__ D, double* Y, double* Z,
-          double* __restrict__ out, const double* __restrict__ dv, int fin) {
-  __shared__ double lds[kBlock / 64];
+    k_bwd_level(BwdK<T> W, int32_t pos0, int32_t cnt, int slot) {
   if (W.ks && stopped(W.ks)) return;
-  const int lane = threadIdx.x & 63;
-  const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-  const int nwaves = (int)((gridDim.x * blockDim.x) >> 6);
+  const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   double s0 = 0.0, s1 = 0.0;
-  bool failed = false;
-  for (int c = wave; c < W.nchunks && !failed; c += nwaves) {
-    const int64_t pos = (int64_t)c * 64 + lane;
-    const bool active = pos < W.n;
-    const int i = active ? W.rows[pos] : 0;
-    double yi = 0.0;
-    int32_t kp = 0, k1 = 0;
-    if (active) {
-      yi = ld_flag(&Y[i]);
-      st_flag(&Y[i], __longlong_as_double((long long)kSentinel));
-      kp = A.diag[i] + 1;
-      k1 = A.rowptr[i + 1];
-    }
-    double sum = 0.0, zi = 0.0;
-    bool done = !active;
-    unsigned spins = 0;
-    while (true) {
-      if (!done) {
-        while (kp < k1) {
-          const double zj = ld_flag(&Z[A.colval[kp]]);
-          if (is_sentinel(zj)) break;
-          sum += A.nz[kp] * zj;
-          ++kp;
-        }
-        if (kp == k1) {
-          zi = (yi - sum) / D[i];
-          st_flag(&Z[i], zi);
-          if (out) out[i] = zi;
-          done = true;
-        }
-      }
-      if (__all(done)) break;
-      if (++spins > kSpinLimit) {
-        atomicOr(W.err, 1);
-        failed = true;
-        break;
-      }
-      __builtin_amdgcn_s_sleep(2);
-    }
-    if (DOT != 0 && active) {
-      if (DOT == 1 || DOT == 2) s0 += dv[i] * zi;
-      if (DOT == 3) { s0 += zi * dv[i]; s1 += zi * zi; }
-      if (DOT == 4) { s0 += zi * zi; s1 += dv[i] * zi; }
-    }
+  if (t < cnt) bwd_row<DOT>(W, W.rows[pos0 + t], s0, s1);
+  if (DOT != 0) store_partial<kBlock>(s0, s1, W.partial, slot + blockIdx.x);
+}
+
+template <int DOT, class T>
+__global__ void __launch_bounds__(kRunBlock)
+    k_bwd_run(BwdK<T> W, const int32_t* __restrict__ lev_ptr, int lev0, int lev1, int slot) {
+  if (W.ks && stopped(W.ks)) return;
+  double s0 = 0.0, s1 = 0.0;
+  for (int l = lev0; l < lev1; ++l) {
+    const int32_t b = lev_ptr[l + 1];
+    for (int32_t p = lev_ptr[l] + threadIdx.x; p < b; p += blockDim.x)
+      bwd_row<DOT>(W, W.rows[p], s0, s1);
+    __syncthreads();
   }
-  if (DOT == 0) return;
-  reduce_tail(s0, s1, blockIdx.x, gridDim.x, W.partial, W.done, W.ks, fin, lds);
+  if (DOT != 0) store_partial<kRunBlock>(s0, s1, W.partial, slot);
 }
 
 // ------------------------------------------------------------ host side
-struct Solver;
-template <class... P, class... Args>
-static void coop(void (*kern)(P...), const Solver& S, Args&&... a);
-
+// One Krylov solve's view of an equation in value type T: the matrix, the
+// preconditioner arrays and 12 n-vectors of workspace.
+template <class T>
 struct Solver {
   xcal_eqn* e;
   const xcal_mesh* m;
   hipStream_t s;
   int64_t n;
-  MatK A;
-  SweepK Wf, Wb;
+  MatKT<T> A;
+  const T *D, *rD, *Sj;  // DILU D and 1/D; Jacobi 1/|a_ii|
+  T* work;
   KS* ks;
   double* partial;
-  unsigned* done;
   int grid;
-  int sgrid;  // blocks of the cooperative sweep kernels
+  int* cg_batch;
+  int* bi_batch;
 };
 
-// Launch a sweep kernel cooperatively (all blocks co-resident).  Arguments are
-// converted to the kernel's exact parameter types before their addresses
-// are handed to the runtime.
-template <class... P, size_t... I>
-static void coop_impl(void (*kern)(P...), const Solver& S, std::tuple<P...>& t,
-                      std::index_sequence<I...>) {
-  void* argv[] = {(void*)&std::get<I>(t)...};
-  XCAL_HIP(hipLaunchCooperativeKernel((const void*)kern, dim3(S.sgrid), dim3(kBlock), argv, 0, S.s));
-}
-template <class... P, class... Args>
-static void coop(void (*kern)(P...), const Solver& S, Args&&... a) {
-  std::tuple<P...> t{static_cast<P>(std::forward<Args>(a))...};
-  coop_impl(kern, S, t, std::index_sequence_for<P...>{});
-}
-
-// Co-resident block count for the sweeps: bounded by occupancy (required by
-// the cooperative launch) and by about four chunks per wave, and capped at
-// 512 blocks (2048 waves, a window of ~131k rows: about one DILU level of a
-// 1M-cell mesh); XCAL_SWEEP_BLOCKS overrides.
-static int sweep_grid(xcal_ctx* c, int nchunks) {
-  if (c->coop_max_blocks == 0) {
-    int cus = 0;
-    XCAL_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
-    int occ = 1 << 30, b = 0;
-    const void* kerns[] = {(const void*)k_fwd<true>, (const void*)k_fwd<false>,
-                           (const void*)k_bwd<0>, (const void*)k_bwd<1>, (const void*)k_bwd<2>,
-                           (const void*)k_bwd<3>, (const void*)k_bwd<4>};
-    for (const void* k : kerns) {
-      XCAL_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k, kBlock, 0));
-      occ = std::min(occ, b);
-    }
-    c->coop_max_blocks = std::max(1, occ) * std::max(1, cus);
-  }
-  int want = std::min(512, (nchunks + 15) / 16);
-  if (const char* env = std::getenv("XCAL_SWEEP_BLOCKS")) want = std::min(kMaxSweepBlocks, std::atoi(env));
-  return std::max(1, std::min(want, c->coop_max_blocks));
-}
-
-static void ensure_workspace(xcal_eqn* e) {
-  const int64_t n = e->mesh->n;
+static void ensure_workspace(xcal_eqn* e, bool fp32) {
+  const xcal_mesh* m = e->mesh;
+  const int64_t n = m->n;
+  const int64_t slots = std::max<int64_t>(grid_for(n), m->sweep_slots) + 8;
+  const bool grow = (e->work.n > 0 && e->work.n < 12 * n) ||
+                    (e->kpart.n > 0 && e->kpart.n < 2 * slots) ||
+                    (fp32 && e->work32.n > 0 && e->work32.n < 12 * n) ||
+                    (fp32 && e->nz32.n > 0 && e->nz32.n < m->nzlen);
+  if (grow) XCAL_HIP(hipStreamSynchronize(m->ctx->stream));  // replaced buffers may be in use
   if (e->work.n < 12 * n) e->work.alloc(12 * n);
-  const int64_t slots = std::max<int64_t>(grid_for(n), kMaxSweepBlocks) + 8;
   if (e->kpart.n < 2 * slots) e->kpart.alloc(2 * slots);
-  if (e->kcount.n < 80) {
-    e->kcount.alloc(80);
-    XCAL_HIP(hipMemsetAsync(e->kcount.p, 0, e->kcount.bytes(), e->mesh->ctx->stream));
-  }
   if (e->kstate.n < (int64_t)sizeof(KS)) e->kstate.alloc(sizeof(KS));
   if (!e->kstate_host) XCAL_HIP(hipHostMalloc((void**)&e->kstate_host, sizeof(KS), 0));
-  if (e->yflag.n < n) {
-    e->yflag.alloc(n);
-    e->zflag.alloc(3 * n);
+  if (fp32) {
+    if (e->work32.n < 12 * n) e->work32.alloc(12 * n);
+    if (e->nz32.n < m->nzlen) e->nz32.alloc(m->nzlen);
+    if (e->pD32.n < n) {
+      e->pD32.alloc(n);
+      e->prD32.alloc(n);
+      e->pS32.alloc(n);
+    }
   }
 }
 
-static Solver make_solver(xcal_eqn* e) {
-  ensure_workspace(e);
-  Solver S;
+static Solver<double> make_solver(xcal_eqn* e) {
+  ensure_workspace(e, false);
+  Solver<double> S;
   S.e = e;
   S.m = e->mesh;
   S.s = e->mesh->ctx->stream;
   S.n = e->mesh->n;
-  S.A = MatK{S.n, S.m->rowptr.p, S.m->colval.p, S.m->diag.p, e->nzval.p};
+  S.A = matk(S.m, (const double*)e->nzval.p);
+  S.D = e->pD.p;
+  S.rD = e->prD.p;
+  S.Sj = e->pS.p;
+  S.work = e->work.p;
   S.ks = (KS*)e->kstate.p;
   S.partial = e->kpart.p;
-  S.done = e->kcount.p + 8;  // kArrive + 1 arrival counters
   S.grid = grid_for(S.n);
-  const int nchunks = (int)((S.n + 63) / 64);
-  S.sgrid = sweep_grid(S.m->ctx, nchunks);
-  S.Wf = SweepK{S.n, nchunks, S.m->lev_rows_f.p, S.done, S.partial, S.ks,
-                (int*)(e->kcount.p + 2)};
-  S.Wb = S.Wf;
-  S.Wb.rows = S.m->lev_rows_b.p;
+  S.cg_batch = &e->cg_batch;
+  S.bi_batch = &e->bi_batch;
   return S;
 }
 
-static void fill_sentinel(hipStream_t s, double* p, int64_t n) {
-  if (n > 0) hipLaunchKernelGGL(k_fill_bits, dim3(grid_for(n)), dim3(kBlock), 0, s, n, p, kSentinel);
+// FP32 copies of the matrix and the preconditioner, made from the FP64 ones
+static Solver<float> make_solver32(xcal_eqn* e) {
+  ensure_workspace(e, true);
+  const xcal_mesh* m = e->mesh;
+  hipStream_t s = m->ctx->stream;
+  const int64_t n = m->n;
+  hipLaunchKernelGGL(k_to_float, dim3(grid_for(m->nzlen)), dim3(kBlock), 0, s, m->nzlen,
+                     e->nzval.p, e->nz32.p);
+  if (e->precon == XCAL_PRECON_DILU) {
+    hipLaunchKernelGGL(k_to_float, dim3(grid_for(n)), dim3(kBlock), 0, s, n, e->pD.p, e->pD32.p);
+    hipLaunchKernelGGL(k_to_float, dim3(grid_for(n)), dim3(kBlock), 0, s, n, e->prD.p, e->prD32.p);
+  } else if (e->precon == XCAL_PRECON_JACOBI) {
+    hipLaunchKernelGGL(k_to_float, dim3(grid_for(n)), dim3(kBlock), 0, s, n, e->pS.p, e->pS32.p);
+  }
+  XCAL_HIP(hipGetLastError());
+  Solver<float> S;
+  S.e = e;
+  S.m = m;
+  S.s = s;
+  S.n = n;
+  S.A = matk(m, (const float*)e->nz32.p);
+  S.D = e->pD32.p;
+  S.rD = e->prD32.p;
+  S.Sj = e->pS32.p;
+  S.work = e->work32.p;
+  S.ks = (KS*)e->kstate.p;
+  S.partial = e->kpart.p;
+  S.grid = grid_for(n);
+  S.cg_batch = &e->cg_batch32;
+  S.bi_batch = &e->bi_batch32;
+  return S;
 }
 
-// z = M^-1 u through the sync-free sweeps (DILU), used by precond_apply
-static void dilu_apply(Solver& S, const double* u, double* z) {
-  xcal_eqn* e = S.e;
-  double* Y = e->yflag.p;
-  double* Z = e->zflag.p;
-  fill_sentinel(S.s, Y, S.n);
-  fill_sentinel(S.s, Z, S.n);
-  SweepK wf = S.Wf, wb = S.Wb;
-  wf.ks = nullptr;
-  wb.ks = nullptr;
-  coop(k_fwd<false>, S, wf, S.A, e->prD.p,
-                     u, Y, nullptr, nullptr, nullptr, nullptr, nullptr);
-  coop(k_bwd<0>, S, wb, S.A, e->pD.p, Y, Z,
-                     z, nullptr, (int)FIN_NONE);
+template <class T>
+static void finalize_launch(const Solver<T>& S, int count, int fin) {
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kRunBlock), 0, S.s, S.partial, count, S.ks, fin);
+}
+
+// forward sweep over the launch plan
+template <bool UPD, class T>
+static void sweep_fwd(const Solver<T>& S, const FwdK<T>& W) {
+  const xcal_mesh* m = S.m;
+  for (const SweepStep& st : m->plan_f) {
+    if (st.grid == 1)
+      hipLaunchKernelGGL((k_fwd_run<UPD, T>), dim3(1), dim3(kRunBlock), 0, S.s, W,
+                         m->lev_ptr_f_d.p, st.lev0, st.lev1);
+    else
+      hipLaunchKernelGGL((k_fwd_level<UPD, T>), dim3(st.grid), dim3(kBlock), 0, S.s, W, st.row0,
+                         st.row1 - st.row0);
+  }
+}
+
+// backward sweep; with DOT != 0 its partials fill slots [0, sweep_slots)
+template <int DOT, class T>
+static void sweep_bwd(const Solver<T>& S, const BwdK<T>& W) {
+  const xcal_mesh* m = S.m;
+  for (const SweepStep& st : m->plan_b) {
+    if (st.grid == 1)
+      hipLaunchKernelGGL((k_bwd_run<DOT, T>), dim3(1), dim3(kRunBlock), 0, S.s, W,
+                         m->lev_ptr_b_d.p, st.lev0, st.lev1, st.slot);
+    else
+      hipLaunchKernelGGL((k_bwd_level<DOT, T>), dim3(st.grid), dim3(kBlock), 0, S.s, W, st.row0,
+                         st.row1 - st.row0, st.slot);
+  }
+}
+
+template <class T>
+static FwdK<T> fwdk(const Solver<T>& S, const T* u, T* Y, const KS* ks) {
+  return FwdK<T>{S.A, S.rD, u, Y, nullptr, nullptr, nullptr, nullptr, ks};
+}
+template <class T>
+static BwdK<T> bwdk(const Solver<T>& S, const T* Y, T* Z, const T* dv, const KS* ks) {
+  return BwdK<T>{S.A, S.m->lev_rows_b.p, S.D, Y, Z, dv, S.partial, ks};
+}
+
+// z = M^-1 u through the level sweeps (DILU), used by precond_apply
+static void dilu_apply(Solver<double>& S, const double* u, double* z) {
+  double* Y = S.work;
+  sweep_fwd<false>(S, fwdk(S, u, Y, (const KS*)nullptr));
+  sweep_bwd<0>(S, bwdk(S, (const double*)Y, z, (const double*)nullptr, (const KS*)nullptr));
   XCAL_HIP(hipGetLastError());
 }
 
@@ -695,7 +674,8 @@ static void prof_resolve(xcal_ctx* c) {
   c->prof_pending.clear();
 }
 
-static KS read_state(Solver& S) {
+template <class T>
+static KS read_state(Solver<T>& S) {
   XCAL_HIP(hipMemcpyAsync(S.e->kstate_host, S.ks, sizeof(KS), hipMemcpyDeviceToHost, S.s));
   XCAL_HIP(hipStreamSynchronize(S.s));
   prof_resolve(S.m->ctx);
@@ -704,7 +684,8 @@ static KS read_state(Solver& S) {
   return k;
 }
 
-static void init_state(Solver& S, int itmax, double atol, double rtol) {
+template <class T>
+static void init_state(Solver<T>& S, int itmax, double atol, double rtol) {
   KS k{};
   k.atol = atol;
   k.rtol = rtol;
@@ -714,18 +695,8 @@ static void init_state(Solver& S, int itmax, double atol, double rtol) {
   XCAL_HIP(hipMemcpyAsync(S.ks, S.e->kstate_host, sizeof(KS), hipMemcpyHostToDevice, S.s));
 }
 
-static void check_sweep_error(Solver& S) {
-  int err = 0;
-  XCAL_HIP(hipMemcpyAsync(&err, S.e->kcount.p + 2, sizeof(int), hipMemcpyDeviceToHost, S.s));
-  XCAL_HIP(hipStreamSynchronize(S.s));
-  if (err) {
-    XCAL_HIP(hipMemsetAsync(S.e->kcount.p, 0, S.e->kcount.bytes(), S.s));
-    XCAL_HIP(hipStreamSynchronize(S.s));
-    throw Error(XCAL_ERR_STATE, "DILU sweep exceeded its spin limit (dependency not published)");
-  }
-}
-
-static int run_batches(Solver& S, int batch, const std::function<void()>& iteration, KS& out) {
+template <class T>
+static int run_batches(Solver<T>& S, int batch, const std::function<void()>& iteration, KS& out) {
   for (;;) {
     for (int b = 0; b < batch; ++b) iteration();
     XCAL_HIP(hipGetLastError());
@@ -734,147 +705,211 @@ static int run_batches(Solver& S, int batch, const std::function<void()>& iterat
   }
 }
 
-static int cg(Solver& S, int precon, const double* b, double* x0, int itmax, double atol,
-              double rtol, int& niter) {
-  xcal_eqn* e = S.e;
+// Krylov.jl cg! with warm start x0 (solution = x0 + correction, written to
+// x0).  With x0 == NULL the start is zero and the solution is left in
+// *xsol (a workspace vector).  Returns 1 when itmax was reached.
+template <class T>
+static int cg(Solver<T>& S, int precon, const T* b, T* x0, int itmax, double atol, double rtol,
+              int& niter, T** xsol) {
   const int64_t n = S.n;
-  double* w = e->work.p;
-  double *r = w, *x = w + n, *pa = w + 2 * n, *pb = w + 3 * n, *q = w + 4 * n, *zd = w + 5 * n;
+  T* w = S.work;
+  T *r = w, *x = w + n, *pa = w + 2 * n, *pb = w + 3 * n, *q = w + 4 * n, *zd = w + 5 * n,
+    *Y = w + 6 * n;
   init_state(S, itmax, atol, rtol);
-  hipLaunchKernelGGL(k_start, dim3(S.grid), dim3(kBlock), 0, S.s, S.A, b, x0, r, x, pa);
+  hipLaunchKernelGGL(k_start<T>, dim3(S.grid), dim3(kBlock), 0, S.s, S.A, b, (const T*)x0, r, x, pa);
   const bool dilu = precon == XCAL_PRECON_DILU;
-  double* Y = e->yflag.p;
-  double* Z = e->zflag.p;
-  const double* Sd = precon == XCAL_PRECON_JACOBI ? e->pS.p : nullptr;
+  const T* Sd = precon == XCAL_PRECON_JACOBI ? S.Sj : nullptr;
+  const int sweep_slots = S.m->sweep_slots;
   if (dilu) {
-    fill_sentinel(S.s, Y, n);
-    fill_sentinel(S.s, Z, n);
-    coop(k_fwd<false>, S, S.Wf, S.A,
-                       e->prD.p, r, Y, nullptr, nullptr, nullptr, nullptr, nullptr);
-    coop(k_bwd<1>, S, S.Wb, S.A, e->pD.p,
-                       Y, Z, nullptr, r, (int)FIN_CG_INIT);
+    sweep_fwd<false>(S, fwdk(S, (const T*)r, Y, (const KS*)S.ks));
+    sweep_bwd<1>(S, bwdk(S, (const T*)Y, zd, (const T*)r, (const KS*)S.ks));
+    finalize_launch(S, sweep_slots, FIN_CG_INIT);
   } else {
-    hipLaunchKernelGGL(k_precond_diag_start<false>, dim3(S.grid), dim3(kBlock), 0, S.s, n, Sd, r, zd,
-                       nullptr, S.partial, S.done, S.ks);
+    hipLaunchKernelGGL((k_precond_diag_start<false, T>), dim3(S.grid), dim3(kBlock), 0, S.s, n, Sd,
+                       r, zd, (const T*)nullptr, S.partial);
+    finalize_launch(S, S.grid, FIN_CG_INIT);
   }
-  const double* z = dilu ? Z : zd;
   int parity = 0;
   auto iteration = [&]() {
-    double* pold = parity ? pb : pa;
-    double* pnew = parity ? pa : pb;
+    T* pold = parity ? pb : pa;
+    T* pnew = parity ? pa : pb;
     parity ^= 1;
     xcal_ctx* c = S.m->ctx;
     timed(c, 0, [&] {
-      hipLaunchKernelGGL(k_cg_spmv, dim3(S.grid), dim3(kBlock), 0, S.s, S.A, z, pold, pnew, q,
-                         S.partial, S.done, S.ks);
+      hipLaunchKernelGGL(k_cg_spmv<T>, dim3(S.grid), dim3(kBlock), 0, S.s, S.A, (const T*)zd,
+                         (const T*)pold, pnew, q, S.partial, (const KS*)S.ks);
     });
+    finalize_launch(S, S.grid, FIN_CG_PAP);
     if (dilu) {
-      timed(c, 1, [&] {
-        coop(k_fwd<true>, S, S.Wf, S.A,
-                           e->prD.p, nullptr, Y, Z, pnew, q, x, r);
-      });
-      timed(c, 2, [&] {
-        coop(k_bwd<1>, S, S.Wb, S.A,
-                           e->pD.p, Y, Z, nullptr, r, (int)FIN_CG_ITER);
-      });
+      FwdK<T> W = fwdk(S, (const T*)nullptr, Y, (const KS*)S.ks);
+      W.p = pnew;
+      W.q = q;
+      W.x = x;
+      W.r = r;
+      timed(c, 1, [&] { sweep_fwd<true>(S, W); });
+      timed(c, 2, [&] { sweep_bwd<1>(S, bwdk(S, (const T*)Y, zd, (const T*)r, (const KS*)S.ks)); });
+      finalize_launch(S, sweep_slots, FIN_CG_ITER);
     } else {
-      hipLaunchKernelGGL(k_cg_update_diag, dim3(S.grid), dim3(kBlock), 0, S.s, n, Sd, pnew, q, x, r,
-                         zd, S.partial, S.done, S.ks);
+      hipLaunchKernelGGL(k_cg_update_diag<T>, dim3(S.grid), dim3(kBlock), 0, S.s, n, Sd,
+                         (const T*)pnew, (const T*)q, x, r, zd, S.partial, (const KS*)S.ks);
+      finalize_launch(S, S.grid, FIN_CG_ITER);
     }
   };
-  KS st;
-  {
-    st = read_state(S);
-    if (st.stop == STOP_NONE) run_batches(S, e->cg_batch, iteration, st);
-  }
-  if (dilu) check_sweep_error(S);
-  hipLaunchKernelGGL(k_add, dim3(S.grid), dim3(kBlock), 0, S.s, n, x, x0);
+  KS st = read_state(S);
+  if (st.stop == STOP_NONE) run_batches(S, *S.cg_batch, iteration, st);
+  if (x0) hipLaunchKernelGGL(k_add, dim3(S.grid), dim3(kBlock), 0, S.s, n, (const double*)x,
+                             (double*)x0);
+  if (xsol) *xsol = x;
   XCAL_HIP(hipGetLastError());
   niter = st.iter;
   // next solve: batch sized to finish in about two reads
-  e->cg_batch = std::max(4, std::min(64, niter / 2 + 1));
+  *S.cg_batch = std::max(4, std::min(64, niter / 2 + 1));
   return (st.stop == STOP_ITMAX || niter >= itmax) ? 1 : 0;
 }
 
-static int bicgstab(Solver& S, int precon, const double* b, double* x0, int itmax, double atol,
-                    double rtol, int& niter) {
-  xcal_eqn* e = S.e;
+// Krylov.jl bicgstab! (left preconditioning, shadow residual c = b); x0 as
+// in cg.
+template <class T>
+static int bicgstab(Solver<T>& S, int precon, const T* b, T* x0, int itmax, double atol,
+                    double rtol, int& niter, T** xsol) {
   const int64_t n = S.n;
-  double* w = e->work.p;
-  double *r = w, *x = w + n, *pa = w + 2 * n, *pb = w + 3 * n, *v = w + 4 * n, *sv = w + 5 * n,
-         *t = w + 6 * n, *d = w + 7 * n;
+  T* w = S.work;
+  T *r = w, *x = w + n, *pa = w + 2 * n, *pb = w + 3 * n, *v = w + 4 * n, *sv = w + 5 * n,
+    *t = w + 6 * n, *d = w + 7 * n, *Y = w + 8 * n;
   init_state(S, itmax, atol, rtol);
-  hipLaunchKernelGGL(k_start, dim3(S.grid), dim3(kBlock), 0, S.s, S.A, b, x0, r, x, pa);
+  hipLaunchKernelGGL(k_start<T>, dim3(S.grid), dim3(kBlock), 0, S.s, S.A, b, (const T*)x0, r, x, pa);
   const bool dilu = precon == XCAL_PRECON_DILU;
-  const double* Sd = precon == XCAL_PRECON_JACOBI ? e->pS.p : nullptr;
-  double* Y = e->yflag.p;
-  double* Zv = e->zflag.p;       // v = M q
-  double* Zt = e->zflag.p + n;   // t = M d
+  const T* Sd = precon == XCAL_PRECON_JACOBI ? S.Sj : nullptr;
+  const int sweep_slots = S.m->sweep_slots;
+  const KS* ks = S.ks;
   if (dilu) {
-    fill_sentinel(S.s, Y, n);
-    fill_sentinel(S.s, Zv, 2 * n);
-    // r = M r0 ; ||r||^2 and <b, r>
-    coop(k_fwd<false>, S, S.Wf, S.A,
-                       e->prD.p, r, Y, nullptr, nullptr, nullptr, nullptr, nullptr);
-    coop(k_bwd<4>, S, S.Wb, S.A, e->pD.p,
-                       Y, Zv, r, b, (int)FIN_BI_INIT);
+    // r = M r0 (in place: the backward sweep reads only Y) ; ||r||^2, <b, r>
+    sweep_fwd<false>(S, fwdk(S, (const T*)r, Y, ks));
+    sweep_bwd<4>(S, bwdk(S, (const T*)Y, r, b, ks));
+    finalize_launch(S, sweep_slots, FIN_BI_INIT);
   } else {
-    hipLaunchKernelGGL(k_precond_diag_start<true>, dim3(S.grid), dim3(kBlock), 0, S.s, n, Sd, r,
-                       nullptr, b, S.partial, S.done, S.ks);
+    hipLaunchKernelGGL((k_precond_diag_start<true, T>), dim3(S.grid), dim3(kBlock), 0, S.s, n, Sd, r,
+                       (T*)nullptr, b, S.partial);
+    finalize_launch(S, S.grid, FIN_BI_INIT);
   }
   int parity = 0;
   auto iteration = [&]() {
-    double* pold = parity ? pb : pa;
-    double* pnew = parity ? pa : pb;
+    T* pold = parity ? pb : pa;
+    T* pnew = parity ? pa : pb;
     parity ^= 1;
     if (dilu) {
-      hipLaunchKernelGGL(k_bi_spmv_p<false>, dim3(S.grid), dim3(kBlock), 0, S.s, S.A, r, pold, pnew,
-                         nullptr, d, b, S.partial, S.done, S.ks);  // d holds q
-      coop(k_fwd<false>, S, S.Wf, S.A,
-                         e->prD.p, d, Y, Zv, nullptr, nullptr, nullptr, nullptr);
-      coop(k_bwd<2>, S, S.Wb, S.A,
-                         e->pD.p, Y, Zv, v, b, (int)FIN_BI_ALPHA);
-      hipLaunchKernelGGL(k_bi_spmv_s<false>, dim3(S.grid), dim3(kBlock), 0, S.s, S.A, r, v, pnew, sv,
-                         x, nullptr, d, S.partial, S.done, S.ks);
-      coop(k_fwd<false>, S, S.Wf, S.A,
-                         e->prD.p, d, Y, Zt, nullptr, nullptr, nullptr, nullptr);
-      coop(k_bwd<3>, S, S.Wb, S.A,
-                         e->pD.p, Y, Zt, t, sv, (int)FIN_BI_OMEGA);
+      hipLaunchKernelGGL((k_bi_spmv_p<false, T>), dim3(S.grid), dim3(kBlock), 0, S.s, S.A,
+                         (const T*)r, (const T*)pold, pnew, (const T*)nullptr, d, b, S.partial,
+                         ks);  // d holds q
+      sweep_fwd<false>(S, fwdk(S, (const T*)d, Y, ks));
+      sweep_bwd<2>(S, bwdk(S, (const T*)Y, v, b, ks));
+      finalize_launch(S, sweep_slots, FIN_BI_ALPHA);
+      hipLaunchKernelGGL((k_bi_spmv_s<false, T>), dim3(S.grid), dim3(kBlock), 0, S.s, S.A,
+                         (const T*)r, (const T*)v, (const T*)pnew, sv, x, (const T*)nullptr, d,
+                         S.partial, ks);
+      sweep_fwd<false>(S, fwdk(S, (const T*)d, Y, ks));
+      sweep_bwd<3>(S, bwdk(S, (const T*)Y, t, (const T*)sv, ks));
+      finalize_launch(S, sweep_slots, FIN_BI_OMEGA);
     } else {
-      hipLaunchKernelGGL(k_bi_spmv_p<true>, dim3(S.grid), dim3(kBlock), 0, S.s, S.A, r, pold, pnew,
-                         Sd, v, b, S.partial, S.done, S.ks);
-      hipLaunchKernelGGL(k_bi_spmv_s<true>, dim3(S.grid), dim3(kBlock), 0, S.s, S.A, r, v, pnew, sv,
-                         x, Sd, t, S.partial, S.done, S.ks);
+      hipLaunchKernelGGL((k_bi_spmv_p<true, T>), dim3(S.grid), dim3(kBlock), 0, S.s, S.A,
+                         (const T*)r, (const T*)pold, pnew, Sd, v, b, S.partial, ks);
+      finalize_launch(S, S.grid, FIN_BI_ALPHA);
+      hipLaunchKernelGGL((k_bi_spmv_s<true, T>), dim3(S.grid), dim3(kBlock), 0, S.s, S.A,
+                         (const T*)r, (const T*)v, (const T*)pnew, sv, x, Sd, t, S.partial, ks);
+      finalize_launch(S, S.grid, FIN_BI_OMEGA);
     }
-    hipLaunchKernelGGL(k_bi_tail, dim3(S.grid), dim3(kBlock), 0, S.s, n, sv, t, v, x, r, pnew, b,
-                       S.partial, S.done, S.ks);
+    hipLaunchKernelGGL(k_bi_tail<T>, dim3(S.grid), dim3(kBlock), 0, S.s, n, (const T*)sv,
+                       (const T*)t, (const T*)v, x, r, pnew, b, S.partial, ks);
+    finalize_launch(S, S.grid, FIN_BI_END);
   };
   KS st = read_state(S);
-  if (st.stop == STOP_NONE) run_batches(S, e->bi_batch, iteration, st);
-  if (dilu) check_sweep_error(S);
-  hipLaunchKernelGGL(k_add, dim3(S.grid), dim3(kBlock), 0, S.s, n, x, x0);
+  if (st.stop == STOP_NONE) run_batches(S, *S.bi_batch, iteration, st);
+  if (x0) hipLaunchKernelGGL(k_add, dim3(S.grid), dim3(kBlock), 0, S.s, n, (const double*)x,
+                             (double*)x0);
+  if (xsol) *xsol = x;
   XCAL_HIP(hipGetLastError());
   niter = st.iter;
-  e->bi_batch = std::max(1, std::min(16, niter + 1));
+  *S.bi_batch = std::max(1, std::min(16, niter + 1));
   return (st.stop == STOP_ITMAX || niter >= itmax) ? 1 : 0;
+}
+
+// |b - A x|^2 and |b|^2 in FP64; r32 = (float)(b - A x)
+static void ir_residual(Solver<double>& S, const double* x, const double* b, float* r32,
+                        double& rr, double& bb) {
+  xcal_ctx* c = S.m->ctx;
+  hipLaunchKernelGGL(k_ir_residual, dim3(S.grid), dim3(kBlock), 0, S.s, S.A, x, b, r32, S.partial);
+  hipLaunchKernelGGL(k_sum2, dim3(1), dim3(kRunBlock), 0, S.s, S.partial, S.grid, c->red_result.p);
+  XCAL_HIP(hipGetLastError());
+  XCAL_HIP(hipMemcpyAsync(c->host_pinned, c->red_result.p, 2 * sizeof(double),
+                          hipMemcpyDeviceToHost, S.s));
+  XCAL_HIP(hipStreamSynchronize(S.s));
+  rr = c->host_pinned[0];
+  bb = c->host_pinned[1];
+}
+
+// Mixed precision: FP64 iterative refinement around FP32 Krylov solves.
+//   r = b - A x (FP64 SpMV) ; solve A d = r in FP32 (same Krylov method and
+//   preconditioner, to ir_inner_rtol) ; x += d (FP64).
+// Stops when |b - A x| <= atol + rtol |b - A x0| (true residual, FP64), or
+// on itmax summed inner iterations, or when a sweep no longer reduces the
+// residual (FP32 exhausted); the last two return the itmax warning.
+static int ir_solve(xcal_eqn* e, const xcal_solver_setup* st, int itmax, double* x,
+                    const double* b, int& niter, int& nouter) {
+  Solver<double> S = make_solver(e);
+  Solver<float> S32 = make_solver32(e);
+  float* r32 = e->work32.p + 11 * S.n;  // not used by the FP32 Krylov workspace (9 n)
+  const double inner_rtol = st->ir_inner_rtol > 0 ? st->ir_inner_rtol : 1e-4;
+  double rr, bb;
+  ir_residual(S, x, b, r32, rr, bb);
+  const double tol = st->atol + st->rtol * std::sqrt(rr);
+  double res = std::sqrt(rr), prev = INFINITY;
+  niter = 0;
+  nouter = 0;
+  int rc = 0;
+  while (res > tol) {
+    if (niter >= itmax || nouter >= 100 || res >= 0.5 * prev) {
+      rc = 1;
+      break;
+    }
+    int ni = 0;
+    float* d = nullptr;
+    if (st->solver == XCAL_SOLVER_CG)
+      cg(S32, st->precon, (const float*)r32, (float*)nullptr, itmax - niter, 0.0, inner_rtol, ni, &d);
+    else
+      bicgstab(S32, st->precon, (const float*)r32, (float*)nullptr, itmax - niter, 0.0, inner_rtol,
+               ni, &d);
+    niter += ni;
+    hipLaunchKernelGGL(k_ir_update, dim3(S.grid), dim3(kBlock), 0, S.s, S.n, (const float*)d, x);
+    prev = res;
+    ir_residual(S, x, b, r32, rr, bb);
+    res = std::sqrt(rr);
+    ++nouter;
+  }
+  return rc;
 }
 
 int solve_system(xcal_eqn* e, const xcal_solver_setup* st, double* x, const double* b,
                  xcal_solve_stats* stats) {
   XCAL_CHECK(st->precision == XCAL_PREC_FP64 || st->precision == XCAL_PREC_FP32_IR,
              "unknown precision mode");
+  XCAL_CHECK(st->solver == XCAL_SOLVER_CG || st->solver == XCAL_SOLVER_BICGSTAB,
+             "unknown Krylov solver");
   XCAL_CHECK(st->precon == e->precon,
              "preconditioner of the solver setup has not been updated on this equation");
-  Solver S = make_solver(e);
   const int itmax = st->itmax == 0 ? (int)(2 * e->mesh->n) : st->itmax;
-  int niter = 0, rc;
-  if (st->solver == XCAL_SOLVER_CG) rc = cg(S, st->precon, b, x, itmax, st->atol, st->rtol, niter);
-  else if (st->solver == XCAL_SOLVER_BICGSTAB)
-    rc = bicgstab(S, st->precon, b, x, itmax, st->atol, st->rtol, niter);
-  else throw Error(XCAL_ERR_UNSUPPORTED, "unknown Krylov solver");
+  int niter = 0, nouter = 1, rc;
+  if (st->precision == XCAL_PREC_FP32_IR) {
+    rc = ir_solve(e, st, itmax, x, b, niter, nouter);
+  } else {
+    Solver<double> S = make_solver(e);
+    if (st->solver == XCAL_SOLVER_CG)
+      rc = cg(S, st->precon, b, x, itmax, st->atol, st->rtol, niter, (double**)nullptr);
+    else
+      rc = bicgstab(S, st->precon, b, x, itmax, st->atol, st->rtol, niter, (double**)nullptr);
+  }
   if (stats) {
     stats->niter = niter;
-    stats->nouter = 1;
+    stats->nouter = nouter;
     stats->residual = residual(e, x, b);
   }
   return rc;
@@ -901,9 +936,8 @@ int xcal_precond_apply(xcal_eqn* e, const xcal_field* r, int32_t rc, xcal_field*
     double* zp = const_cast<double*>(cell_vec(e, z, zc, "z"));
     XCAL_CHECK(rp != zp, "r and z must not alias");
     if (e->precon == XCAL_PRECON_DILU) {
-      Solver S = make_solver(e);
+      Solver<double> S = make_solver(e);
       dilu_apply(S, rp, zp);
-      check_sweep_error(S);
     } else {
       apply_M_levels(e, rp, zp);
     }

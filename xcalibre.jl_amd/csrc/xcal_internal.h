@@ -52,7 +52,9 @@ constexpr int kBlock = 256;
 constexpr int kMaxTerms = 4;
 constexpr int kMaxSources = 2;
 constexpr int kMaxPatches = 32;
-constexpr int kRedBlocks = 1024;  // partial-sum slots of one device reduction
+constexpr int kSlice = 64;        // rows per slice of the sliced-ELL matrix storage
+constexpr int kRunBlock = 1024;   // workgroup size of the single-workgroup sweep runs
+constexpr int kSmallLevel = 2048; // levels below this many rows join a single-workgroup run
 
 inline int grid_for(int64_t n, int block = kBlock) {
   int64_t g = (n + block - 1) / block;
@@ -93,10 +95,7 @@ struct DBuf {
 struct xcal_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
-  // scratch for device reductions: partial sums + tickets + results
-  xcal::DBuf<double> red_partial;   // kRedBlocks * 8 slots
-  xcal::DBuf<unsigned> red_ticket;  // 8 slots
-  xcal::DBuf<double> red_result;    // 64 slots
+  xcal::DBuf<double> red_result;    // 64 slots: results of device reductions
   double* host_pinned = nullptr;    // 64 doubles, pinned
   // optional HIP-event timing of one pressure-solve kernel
   int prof_kernel = -1;
@@ -104,10 +103,17 @@ struct xcal_ctx {
   std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_pending;
   int64_t prof_launches = 0;
   double prof_ms = 0.0;
-  int coop_max_blocks = 0;  // co-resident blocks of the sweep kernels (0: not yet queried)
 };
 
-// Device mesh in SoA form, 0-based int32 indices, reference numbering.
+// One launch of a DILU sweep over levels [lev0, lev1): one level over `grid`
+// workgroups, or (grid == 1) a run of small levels in one workgroup.  Rows
+// [row0, row1) are device rows (forward) or positions in lev_rows_b
+// (backward); the launch writes partial sums to slots [slot, slot + grid).
+struct SweepStep {
+  int32_t lev0, lev1, row0, row1, grid, slot;
+};
+
+// Device mesh in SoA form, 0-based int32 indices, device numbering.
 struct xcal_mesh {
   xcal_ctx* ctx = nullptr;
   int dim = 3;
@@ -129,13 +135,21 @@ struct xcal_mesh {
   // device numbering: cperm/fperm map reference cell/face ids to device ids,
   // nzperm maps reference CSR slots to device slots
   std::vector<int32_t> cperm, fperm, nzperm;
-  // CSR (h_* in the reference's numbering)
+  // CSR pattern in the reference's numbering (host only)
   std::vector<int32_t> h_rowptr, h_colval;
-  xcal::DBuf<int32_t> rowptr, colval, trans;
-  // DILU level schedules
+  // device matrix storage (sliced ELL): entries of row c at
+  // rbeg[c], rbeg[c] + kSlice, ... < rend[c], in the reference's column order
+  int64_t nzlen = 0;  // storage length including slice padding
+  xcal::DBuf<int32_t> rbeg, rend, colval, trans;
+  // DILU level schedules: forward level l = device rows
+  // [lev_ptr_f[l], lev_ptr_f[l+1]); backward level l = lev_rows_b over
+  // [lev_ptr_b[l], lev_ptr_b[l+1])
   int nlev_f = 0, nlev_b = 0;
   std::vector<int32_t> lev_ptr_f, lev_ptr_b;
-  xcal::DBuf<int32_t> lev_rows_f, lev_rows_b;
+  xcal::DBuf<int32_t> lev_ptr_f_d, lev_ptr_b_d, lev_rows_b;
+  int small_level = 0;
+  std::vector<SweepStep> plan_f, plan_b;
+  int sweep_slots = 0;  // partial-sum slots used by one backward sweep
   int64_t device_bytes = 0;
 };
 
@@ -162,13 +176,12 @@ struct xcal_eqn {
   // Krylov workspace (allocated lazily)
   xcal::DBuf<double> work;    // solver vectors, 12 x n
   xcal::DBuf<double> kpart;   // reduction partial sums
-  xcal::DBuf<unsigned> kcount;  // [0] sweep chunk ticket [1] reduction ticket [2] sweep error
   xcal::DBuf<char> kstate;    // device-resident Krylov scalars
   void* kstate_host = nullptr;  // pinned copy
-  xcal::DBuf<double> yflag, zflag;  // DILU sweep outputs published through sentinels
   int cg_batch = 8, bi_batch = 2;   // iterations enqueued per state read
-  xcal::DBuf<float> work32;   // FP32 inner solve workspace
-  xcal::DBuf<float> nz32;     // FP32 copy of nzval
+  // mixed precision: FP32 copies of the matrix and preconditioner, workspace
+  xcal::DBuf<float> work32, nz32, pD32, prD32, pS32;
+  int cg_batch32 = 8, bi_batch32 = 2;
   ~xcal_eqn() {
     if (kstate_host) (void)hipHostFree(kstate_host);
   }
@@ -203,10 +216,12 @@ struct BCsK {
   BCK p[kMaxPatches];  // indexed by patch id; kind -1 = no BC on patch
 };
 
+void make_sweep_plans(xcal_mesh* m, int small);
 void upload_permuted(const xcal_mesh* m, const std::vector<int32_t>& perm, double* dev,
                      const double* host, int64_t len);
+// host[i] = dev[perm[i]] for i < len; dev holds dev_len entries (default len)
 void download_permuted(const xcal_mesh* m, const std::vector<int32_t>& perm, const double* dev,
-                       double* host, int64_t len);
+                       double* host, int64_t len, int64_t dev_len = -1);
 TermsK make_terms(const xcal_term* terms, int n);
 BCsK make_bcs(const xcal_mesh* m, const xcal_bc* bcs, int n);
 xcal_field* check_field(const xcal_field* f, const xcal_mesh* m, int kind, const char* what);

@@ -262,6 +262,7 @@ class Equation:
     def solve(self, setup: A.SolverSetupC, phi: Field, component=0):
         st = A.SolveStats()
         rc = check(lib().xcal_solve_system(self.h, C.byref(setup), phi.h, component, C.byref(st)))
+        self.last_nouter = st.nouter  # refinement sweeps (1 for FP64)
         return st.niter, st.residual, rc
 
     def __del__(self):

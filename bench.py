@@ -36,7 +36,7 @@ METRIC = "SIMPLE cell-iters/s FP64 vs mixed@1e-8; batched solves/s + p50/p99 lat
 SEED_JITTER, SEED_PERM = 0x5EED0002, 0x5EED0003
 
 
-def channel(api, mesh, iterations, precision="fp64"):
+def channel(api, mesh, iterations, precision="fp64", inner_rtol=1e-4):
     model = api.Physics(domain=mesh, fluid=api.Fluid(nu=1e-3))
     api.initialise(model.momentum.U, [1.0, 0.0, 0.0])
     api.initialise(model.momentum.p, 0.0)
@@ -49,7 +49,7 @@ def channel(api, mesh, iterations, precision="fp64"):
                                      convergence=1e-7, relax=0.7, rtol=0.1),
                    p=api.SolverSetup(solver=api.Cg(), preconditioner=api.DILU(),
                                      convergence=1e-7, relax=0.3, rtol=1e-10,
-                                     precision=precision))
+                                     precision=precision, inner_rtol=inner_rtol))
     config = api.Configuration(
         solvers=solvers, schemes=dict(U=api.Schemes(divergence=api.Upwind), p=api.Schemes()),
         runtime=api.Runtime(iterations=iterations, write_interval=-1),
@@ -84,6 +84,8 @@ def main():
                     choices=["cg_spmv", "dilu_fwd", "dilu_bwd"])
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32-ir"],
                     help="pressure solve: FP64, or FP32 inner Krylov in FP64 refinement")
+    ap.add_argument("--inner-rtol", type=float, default=1e-4,
+                    help="fp32-ir: relative tolerance of each FP32 inner solve")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-steps", type=int, default=2)
     args = ap.parse_args()
@@ -99,7 +101,7 @@ def main():
     t0 = time.perf_counter()
     mesh = x.mesh.box3d(N, N, N, jitter=0.1, seed_jitter=SEED_JITTER, seed_perm=SEED_PERM)
     t_mesh = time.perf_counter() - t0
-    model, config = channel(api, mesh, args.warmup + args.steps, args.precision)
+    model, config = channel(api, mesh, args.warmup + args.steps, args.precision, args.inner_rtol)
     setup = api.simple_setup(model, config)
     ctx = H.Context(0)
     t0 = time.perf_counter()
@@ -174,7 +176,8 @@ def main():
                                f"jittered+renumbered hex box ({n} cells)",
                    "cells": n, "nnz": dm.nnz, "dilu_levels": list(dm.levels),
                    "p_solver": "CG+DILU rtol 1e-10" if args.precision == "fp64" else
-                               "FP32 CG+DILU (inner rtol 1e-4) in FP64 refinement to 1e-10",
+                               f"FP32 CG+DILU (inner rtol {args.inner_rtol:g}) in FP64 refinement "
+                               f"to 1e-10",
                    "U_solver": "BiCGSTAB+DILU rtol 0.1",
                    "parallelism": "single GPU"},
         "roofline": roofline,

@@ -161,6 +161,20 @@ int xcal_mesh_info(const xcal_mesh* mesh, int64_t* n_cells, int64_t* nnz,
 /* CSR pattern, 0-based (rowptr n+1, colval nnz) */
 int xcal_mesh_csr_pattern(const xcal_mesh* mesh, int32_t* rowptr, int32_t* colval);
 
+/* Batched form (no single reference counterpart: it runs many independent
+ * cases of the reference's mesh/equation/run! path at once).  The cases'
+ * meshes become one device mesh whose fields, equations and SIMPLE runs
+ * cover every case; each case keeps its own Krylov iterations and stops on
+ * its own, and a case's results do not depend on which other cases share
+ * the batch.  All cases must have the same patches (the BC table is shared).
+ * Fields of a batch mesh cross the ABI as the cases' reference-numbered
+ * arrays concatenated (cells: case 0's cells, then case 1's, ...; faces
+ * likewise, each case boundary-first). */
+int xcal_mesh_create_batch(xcal_ctx* ctx, const xcal_mesh_host* const* hosts, int32_t ncase,
+                           xcal_mesh** out);
+/* ncase and the ncase+1 cell offsets of the cases (either may be NULL) */
+int xcal_mesh_cases(const xcal_mesh* mesh, int32_t* ncase, int64_t* cell_offsets);
+
 /* ------------------------------------------------------------- fields */
 /* ScalarField/VectorField/FaceScalarField/FaceVectorField/TensorField
  * (src/Fields/Fields_0_types.jl:56-203), SoA per component. */
@@ -216,6 +230,9 @@ int xcal_precond_apply(xcal_eqn* e, const xcal_field* r, int32_t rcomp, xcal_fie
 int xcal_spmv(xcal_eqn* e, const xcal_field* x, int32_t xcomp, xcal_field* y, int32_t ycomp);
 /* solve_system! (Solve_1_api.jl:268-307): Krylov solve warm-started from
  * phi, copy into phi, residual.  Returns XCAL_ITMAX when itmax was hit. */
+/* solve_system! for every case of a batch mesh: stats[ncase] */
+int xcal_solve_system_batch(xcal_eqn* e, const xcal_solver_setup* s, xcal_field* phi,
+                            int32_t component, xcal_solve_stats* stats);
 int xcal_solve_system(xcal_eqn* e, const xcal_solver_setup* s, xcal_field* phi,
                       int32_t component, xcal_solve_stats* stats);
 
@@ -286,6 +303,13 @@ int xcal_simple_get_field(const xcal_simple* s, int32_t which, double* host);
  * gets the Krylov iteration counts of the Ux, Uy, Uz and p solves. */
 int xcal_simple_run(xcal_simple* s, int32_t iterations, double* residuals, int32_t* done,
                     int32_t* krylov_iters);
+/* Batch meshes: residuals and krylov_iters laid out [case][field][iteration]
+ * (ncase*4*iterations); the run stops when every case has converged.
+ * xcal_simple_run on a batch reports the largest value over the cases. */
+int xcal_simple_run_batch(xcal_simple* s, int32_t iterations, double* residuals, int32_t* done,
+                          int32_t* krylov_iters);
+/* Batch meshes: kinematic viscosity of each case (setup.nu is the default) */
+int xcal_simple_set_case_nu(xcal_simple* s, const double* nu);
 /* Device time (ms, HIP events) spent in the last xcal_simple_run by stage:
  * [0] momentum assembly+BCs, [1] momentum solves, [2] pressure assembly,
  * [3] pressure solve, [4] gradients/interpolation/flux/corrections, [5] total */

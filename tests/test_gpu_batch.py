@@ -1,0 +1,163 @@
+"""Batched form: independent cases in one batch mesh, stepped together.
+
+A case's results must not depend on which other cases share the batch: each
+case keeps its own Krylov iterations, workgroups never straddle two cases and
+each case's reductions are summed in a fixed order over its own workgroups.
+So a case run in a batch of several must equal, bit for bit, the same case
+run as a batch of one.  Against the CPU oracle the batched solves are held
+to the tolerances of the single-mesh solver.
+"""
+import numpy as np
+import pytest
+
+from conftest import cavity_case, rel
+
+pytestmark = pytest.mark.gpu
+
+SIZES = (12, 20, 16)
+
+
+@pytest.fixture(scope="module")
+def ctx(x):
+    return x.hip.Context(0)
+
+
+def _cases(x):
+    return [x.mesh.cavity2d(n) for n in SIZES]
+
+
+def _case_fields(mesh, seed):
+    rng = np.random.default_rng(seed)
+    return dict(gamma=0.5 + rng.random(mesh.n_faces),
+                mdot=rng.standard_normal(mesh.n_faces) * mesh.face_area * 0.2,
+                src=rng.standard_normal(mesh.n_cells), phi=0.01 * rng.standard_normal(mesh.n_cells))
+
+
+def _system(x, A, dm, meshes, sym):
+    """Assemble Laplacian (+ upwind when not symmetric) on a (batch) mesh whose
+    cases are `meshes`; returns (equation, phi field, per-case fields)."""
+    H = x.hip
+    F = [_case_fields(m, 100 + SIZES.index(int(round(np.sqrt(m.n_cells))))) for m in meshes]
+    cat = lambda key: np.concatenate([f[key] for f in F])
+    g = H.Field(dm, A.FACE_SCALAR).upload(cat("gamma"))
+    md = H.Field(dm, A.FACE_SCALAR).upload(np.zeros(dm.mesh.n_faces) if sym else cat("mdot"))
+    terms = [(A.DIV_UPWIND, 1, md), (A.LAPLACIAN_LINEAR, -1, g)]
+    src = H.Field(dm, A.CELL_SCALAR).upload(cat("src"))
+    phi = H.Field(dm, A.CELL_SCALAR).upload(cat("phi"))
+    eq = H.Equation(dm, False)
+    eq.discretise(terms, [(src, 1)], phi)
+    bcs = A.bc_array([(A.BC_DIRICHLET, i, 1.0) for i in range(len(meshes[0].patch_names))])
+    eq.apply_bcs(terms, bcs, phi)
+    return eq, phi, F, terms, bcs
+
+
+def test_batch_mesh_is_block_diagonal(x, ctx):
+    ms = _cases(x)
+    dm = x.hip.DeviceMesh.batch(ctx, ms)
+    assert dm.ncase == 3 and dm.n == sum(m.n_cells for m in ms)
+    rp, cv = dm.csr_pattern()
+    off = 0
+    for m in ms:
+        r1, c1 = x.hip.DeviceMesh(ctx, m).csr_pattern()
+        assert np.array_equal(rp[off:off + m.n_cells + 1] - rp[off], r1)
+        assert np.array_equal(cv[rp[off]:rp[off + m.n_cells]] - off, c1)
+        off += m.n_cells
+
+
+def test_batch_assembly_equals_single_meshes(x, A, ctx):
+    """Assembly kernels are per cell and per face: the batch's nzval and b are
+    the cases' single-mesh results concatenated, bit for bit."""
+    ms = _cases(x)
+    dm = x.hip.DeviceMesh.batch(ctx, ms)
+    eq, _, _, _, _ = _system(x, A, dm, ms, sym=False)
+    nzb, bb = eq.download()
+    zo = co = 0
+    for m in ms:
+        d1 = x.hip.DeviceMesh(ctx, m)
+        e1, _, _, _, _ = _system(x, A, d1, [m], sym=False)
+        nz1, b1 = e1.download()
+        assert np.array_equal(nzb[zo:zo + len(nz1)], nz1)
+        assert np.array_equal(bb[co:co + m.n_cells], b1)
+        zo += len(nz1)
+        co += m.n_cells
+
+
+@pytest.mark.parametrize("solver,precon,prec", [
+    ("SOLVER_CG", "PRECON_DILU", "PREC_FP64"), ("SOLVER_CG", "PRECON_JACOBI", "PREC_FP64"),
+    ("SOLVER_BICGSTAB", "PRECON_DILU", "PREC_FP64"), ("SOLVER_BICGSTAB", "PRECON_JACOBI", "PREC_FP64"),
+    ("SOLVER_CG", "PRECON_DILU", "PREC_FP32_IR")])
+def test_batch_krylov_alone_equals_mix_and_oracle(x, A, O, ctx, solver, precon, prec):
+    """Stated tolerances: bit-identical alone vs in the batch; against the
+    oracle's FP64 solve (rtol 1e-12) relative L2 <= 1e-9 (FP64) or 1e-8
+    (FP32 inner solves in FP64 refinement to 1e-12)."""
+    ms = _cases(x)
+    sym = solver == "SOLVER_CG"
+    pre = getattr(A, precon)
+    setup = A.SolverSetupC(getattr(A, solver), pre, getattr(A, prec), 4000, 1e-7, 1.0, 0.0, 1e-12,
+                           1e-4)
+    dm = x.hip.DeviceMesh.batch(ctx, ms)
+    eq, phi, F, _, bcs = _system(x, A, dm, ms, sym)
+    eq.precond_update(pre)
+    its, outer, res, rc = eq.solve_batch(setup, phi)
+    assert rc == 0
+    sol = phi.download()
+    off = 0
+    for k, m in enumerate(ms):
+        d1 = x.hip.DeviceMesh.batch(ctx, [m])
+        e1, p1, _, _, _ = _system(x, A, d1, [m], sym)
+        e1.precond_update(pre)
+        it1, out1, res1, rc1 = e1.solve_batch(setup, p1)
+        assert np.array_equal(sol[off:off + m.n_cells], p1.download()), f"case {k}"
+        assert its[k] == it1[0] and outer[k] == out1[0] and res[k] == res1[0]
+        om = O.OracleMesh(m, A)
+        f = F[k]
+        oterms = [(A.DIV_UPWIND, 1, np.zeros(m.n_faces) if sym else f["mdot"]),
+                  (A.LAPLACIAN_LINEAR, -1, f["gamma"])]
+        onz, ob = om.discretise_scalar(oterms, [(f["src"], 1)])
+        om.apply_bcs(oterms, bcs, -1, f["phi"], onz, ob)
+        ox, oit, orc = om.krylov(getattr(A, solver), pre, onz, ob, f["phi"], itmax=4000,
+                                 atol=1e-30, rtol=1e-12)
+        e = rel(sol[off:off + m.n_cells], ox)
+        print(f"case {k} n={m.n_cells}: its {its[k]} (oracle {oit}), sweeps {outer[k]}, rel {e:.2e}")
+        assert e < (1e-9 if prec == "PREC_FP64" else 1e-8)
+        off += m.n_cells
+
+
+def _batch_setup(api, meshes, iterations):
+    model, config = cavity_case(api, meshes[0], iterations=iterations)
+    return api.simple_setup(model, config, pref=0.0)
+
+
+def test_batch_simple_alone_equals_mix_and_oracle(x, A, O, ctx):
+    """Three lid-driven cavities of different size and Reynolds number run as
+    one batch for 40 SIMPLE iterations.  Stated tolerances: each case
+    bit-identical to the same case run alone (a batch of one); U and p within
+    1e-6 relative L2 of the oracle's run of that case."""
+    api = x.api
+    ms = _cases(x)
+    nus = [0.01, 0.004, 0.0025]
+    iters = 40
+    setup = _batch_setup(api, ms, iters)
+    dm = x.hip.DeviceMesh.batch(ctx, ms)
+    sim = x.hip.Simple(dm, setup)
+    sim.set_case_nu(nus)
+    out = sim.run_batch(iters)
+    assert out["done"] == iters
+    U, p = sim.get("U"), sim.get("p")
+    off = 0
+    for k, m in enumerate(ms):
+        d1 = x.hip.DeviceMesh.batch(ctx, [m])
+        s1 = x.hip.Simple(d1, setup)
+        s1.set_case_nu([nus[k]])
+        o1 = s1.run_batch(iters)
+        sl = slice(off, off + m.n_cells)
+        assert np.array_equal(U[:, sl], s1.get("U")), f"case {k} U"
+        assert np.array_equal(p[sl], s1.get("p")), f"case {k} p"
+        assert np.array_equal(out["residuals"][k], o1["residuals"][0])
+        model, config = cavity_case(api, m, nu=nus[k], iterations=iters)
+        ref = O.OracleMesh(m, A).simple_run(api.simple_setup(model, config, pref=0.0),
+                                             model.momentum.U, model.momentum.p, iters)
+        eU, ep = rel(U[:, sl], ref["U"]), rel(p[sl], ref["p"])
+        print(f"case {k} n={m.n_cells} nu={nus[k]}: rel L2 U {eU:.2e} p {ep:.2e}")
+        assert eU < 1e-6 and ep < 1e-6
+        off += m.n_cells

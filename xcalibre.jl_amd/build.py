@@ -29,7 +29,8 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function", f"-I{INC}", f"-I{CSRC}",
 ]
-HIP_SOURCES = ["core.hip", "assemble.hip", "calculate.hip", "krylov.hip", "solver.hip", "simple.hip"]
+HIP_SOURCES = ["core.hip", "assemble.hip", "calculate.hip", "krylov.hip", "solver.hip", "batch.hip",
+               "simple.hip"]
 HEADERS = ["xcal_internal.h", "device_common.h"]
 
 

@@ -1,0 +1,755 @@
+// Batched Krylov solves: the independent systems of a batch mesh (one per
+// case) solved together, with the same CG / BiCGSTAB iterations, Jacobi /
+// DILU preconditioners and FP64 or FP32-inner-refinement modes as the
+// single-mesh solver (solver.hip), in one launch per step for all cases.
+//
+//  * Every case has its own Krylov scalars (KS) and stops on its own; the
+//    kernels of a stopped case skip their stores (sweeps skip the case).
+//  * Workgroups never straddle two cases (a block table maps each workgroup
+//    to a row range of one case), and each case's reductions are summed in a
+//    fixed order over its own workgroups, so a case's iterates do not depend
+//    on which other cases share the batch.
+//  * DILU sweeps: one workgroup per case walks that case's levels with a
+//    barrier between levels (cases of a batch are small; their levels are
+//    many and short), all cases in one launch.
+#include <cmath>
+#include <cstring>
+#include <functional>
+
+#include "solver_common.h"
+
+namespace xcal {
+
+// workgroup b covers rows [row0[b], row1[b]) of case kcase[b]
+struct BlkK {
+  const int32_t* row0;
+  const int32_t* row1;
+  const int32_t* kcase;
+};
+
+// Sums the partial pairs of segment k ([seg[k], seg[k+1])) in order and runs
+// the scalar update of case k (one thread per case).
+__global__ void __launch_bounds__(kBlock)
+    kb_finalize(const double* __restrict__ partial, const int32_t* __restrict__ seg, int ncase,
+                KS* ks, int fin) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= ncase) return;
+  KS st = ks[k];
+  if (st.stop != STOP_NONE) return;
+  double t0 = 0.0, t1 = 0.0;
+  for (int j = seg[k]; j < seg[k + 1]; ++j) {
+    t0 += partial[2 * j];
+    t1 += partial[2 * j + 1];
+  }
+  finalize(&st, fin, t0, t1);
+  ks[k] = st;
+}
+
+// per-case sums of the partial pairs -> out[2k], out[2k+1]
+__global__ void __launch_bounds__(kBlock)
+    kb_sum_cases(const double* __restrict__ partial, const int32_t* __restrict__ seg, int ncase,
+                 double* __restrict__ out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= ncase) return;
+  double t0 = 0.0, t1 = 0.0;
+  for (int j = seg[k]; j < seg[k + 1]; ++j) {
+    t0 += partial[2 * j];
+    t1 += partial[2 * j + 1];
+  }
+  out[2 * k] = t0;
+  out[2 * k + 1] = t1;
+}
+
+// r = b - A x0 (r = b when x0 is NULL) ; x = 0 ; pold = 0
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    kb_start(MatKT<T> A, const T* __restrict__ b, const T* __restrict__ x0, T* __restrict__ r,
+             T* __restrict__ x, T* __restrict__ pold) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= A.n) return;
+  if (x0) {
+    const T acc = row_sum(A, i, [&](int j) { return x0[j]; });
+    r[i] = b[i] - acc;
+  } else {
+    r[i] = b[i];
+  }
+  x[i] = 0;
+  pold[i] = 0;
+}
+
+template <bool BICG, class T>
+__global__ void __launch_bounds__(kBlock)
+    kb_precond_diag_start(BlkK B, const T* __restrict__ S, T* __restrict__ r, T* __restrict__ z,
+                          const T* __restrict__ c, double* partial) {
+  const int32_t i = B.row0[blockIdx.x] + threadIdx.x;
+  double s0 = 0.0, s1 = 0.0;
+  if (i < B.row1[blockIdx.x]) {
+    if (BICG) {
+      const T ri = S ? S[i] * r[i] : r[i];
+      r[i] = ri;
+      s0 = dd(ri, ri);
+      s1 = dd(c[i], ri);
+    } else {
+      const T zi = S ? S[i] * r[i] : r[i];
+      z[i] = zi;
+      s0 = dd(r[i], zi);
+    }
+  }
+  store_partial<kBlock>(s0, s1, partial, blockIdx.x);
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    kb_cg_spmv(BlkK B, MatKT<T> A, const T* __restrict__ z, const T* __restrict__ pold,
+               T* __restrict__ pnew, T* __restrict__ q, double* partial, const KS* ks) {
+  const KS* K = ks + B.kcase[blockIdx.x];
+  const bool live = !stopped(K);
+  const T beta = (T)K->beta;
+  const int32_t i = B.row0[blockIdx.x] + threadIdx.x;
+  double s0 = 0.0;
+  if (i < B.row1[blockIdx.x]) {
+    const T acc = row_sum(A, i, [&](int j) { return z[j] + beta * pold[j]; });
+    const T pi = z[i] + beta * pold[i];
+    if (live) {
+      pnew[i] = pi;
+      q[i] = acc;
+    }
+    s0 = dd(pi, acc);
+  }
+  store_partial<kBlock>(s0, 0.0, partial, blockIdx.x);
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    kb_cg_update_diag(BlkK B, const T* __restrict__ S, const T* __restrict__ p,
+                      const T* __restrict__ q, T* __restrict__ x, T* __restrict__ r,
+                      T* __restrict__ z, double* partial, const KS* ks) {
+  const KS* K = ks + B.kcase[blockIdx.x];
+  const bool live = !stopped(K);
+  const T a = (T)K->alpha;
+  const int32_t i = B.row0[blockIdx.x] + threadIdx.x;
+  double s0 = 0.0;
+  if (i < B.row1[blockIdx.x]) {
+    const T xi = x[i] + a * p[i];
+    const T ri = r[i] + (-a) * q[i];
+    const T zi = S ? S[i] * ri : ri;
+    if (live) {
+      x[i] = xi;
+      r[i] = ri;
+      z[i] = zi;
+    }
+    s0 = dd(ri, zi);
+  }
+  store_partial<kBlock>(s0, 0.0, partial, blockIdx.x);
+}
+
+template <bool VDIAG, class T>
+__global__ void __launch_bounds__(kBlock)
+    kb_bi_spmv_p(BlkK B, MatKT<T> A, const T* __restrict__ r, const T* __restrict__ pold,
+                 T* __restrict__ pnew, const T* __restrict__ S, T* __restrict__ qv,
+                 const T* __restrict__ c, double* partial, const KS* ks) {
+  const KS* K = ks + B.kcase[blockIdx.x];
+  const bool live = !stopped(K);
+  const T beta = (T)K->beta;
+  const int32_t i = B.row0[blockIdx.x] + threadIdx.x;
+  double s0 = 0.0;
+  if (i < B.row1[blockIdx.x]) {
+    const T acc = row_sum(A, i, [&](int j) { return r[j] + beta * pold[j]; });
+    const T pi = r[i] + beta * pold[i];
+    const T vi = VDIAG && S ? S[i] * acc : acc;
+    if (live) {
+      pnew[i] = pi;
+      qv[i] = vi;
+    }
+    if (VDIAG) s0 = dd(c[i], vi);
+  }
+  if (VDIAG) store_partial<kBlock>(s0, 0.0, partial, blockIdx.x);
+}
+
+template <bool TDIAG, class T>
+__global__ void __launch_bounds__(kBlock)
+    kb_bi_spmv_s(BlkK B, MatKT<T> A, const T* __restrict__ r, const T* __restrict__ v,
+                 const T* __restrict__ p, T* __restrict__ s, T* __restrict__ x,
+                 const T* __restrict__ S, T* __restrict__ dt, double* partial, const KS* ks) {
+  const KS* K = ks + B.kcase[blockIdx.x];
+  const bool live = !stopped(K);
+  const T a = (T)K->alpha;
+  const int32_t i = B.row0[blockIdx.x] + threadIdx.x;
+  double s0 = 0.0, s1 = 0.0;
+  if (i < B.row1[blockIdx.x]) {
+    const T acc = row_sum(A, i, [&](int j) { return r[j] + (-a) * v[j]; });
+    const T si = r[i] + (-a) * v[i];
+    const T xi = x[i] + a * p[i];
+    const T ti = TDIAG && S ? S[i] * acc : acc;
+    if (live) {
+      s[i] = si;
+      x[i] = xi;
+      dt[i] = ti;
+    }
+    if (TDIAG) {
+      s0 = dd(ti, si);
+      s1 = dd(ti, ti);
+    }
+  }
+  if (TDIAG) store_partial<kBlock>(s0, s1, partial, blockIdx.x);
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    kb_bi_tail(BlkK B, const T* __restrict__ s, const T* __restrict__ t, const T* __restrict__ v,
+               T* __restrict__ x, T* __restrict__ r, T* __restrict__ p, const T* __restrict__ c,
+               double* partial, const KS* ks) {
+  const KS* K = ks + B.kcase[blockIdx.x];
+  const bool live = !stopped(K);
+  const T w = (T)K->omega;
+  const int32_t i = B.row0[blockIdx.x] + threadIdx.x;
+  double s0 = 0.0, s1 = 0.0;
+  if (i < B.row1[blockIdx.x]) {
+    const T xi = x[i] + w * s[i];
+    const T ri = s[i] + (-w) * t[i];
+    const T pi = p[i] + (-w) * v[i];
+    if (live) {
+      x[i] = xi;
+      r[i] = ri;
+      p[i] = pi;
+    }
+    s0 = dd(c[i], ri);
+    s1 = dd(ri, ri);
+  }
+  store_partial<kBlock>(s0, s1, partial, blockIdx.x);
+}
+
+// one workgroup per case: the case's forward levels in order
+template <bool UPD, class T>
+__global__ void __launch_bounds__(kRunBlock)
+    kb_fwd(FwdK<T> W, const int32_t* __restrict__ lev_ptr, const int32_t* __restrict__ meta,
+           const int32_t* __restrict__ case_lev, const KS* ks) {
+  const int k = blockIdx.x;
+  if (ks && stopped(ks + k)) return;  // uniform over the workgroup
+  const T a = UPD ? (T)ks[k].alpha : T(0);
+  for (int u = case_lev[k]; u < case_lev[k + 1]; ++u) {
+    const int32_t r0 = lev_ptr[u], cnt = lev_ptr[u + 1] - r0, base = meta[2 * u], w = meta[2 * u + 1];
+    for (int32_t q = threadIdx.x; q < cnt; q += blockDim.x)
+      fwd_row<UPD>(W, r0 + q, (int64_t)base + q, cnt, w, a, true);
+    __syncthreads();
+  }
+}
+
+// one workgroup per case: the case's backward levels; dots into slot k
+template <int DOT, class T>
+__global__ void __launch_bounds__(kRunBlock)
+    kb_bwd(BwdK<T> W, const int32_t* __restrict__ lev_ptr, const int32_t* __restrict__ meta,
+           const int32_t* __restrict__ case_lev, const KS* ks) {
+  const int k = blockIdx.x;
+  if (ks && stopped(ks + k)) return;
+  double s0 = 0.0, s1 = 0.0;
+  for (int u = case_lev[k]; u < case_lev[k + 1]; ++u) {
+    const int32_t p0 = lev_ptr[u], cnt = lev_ptr[u + 1] - p0, base = meta[2 * u], w = meta[2 * u + 1];
+    for (int32_t q = threadIdx.x; q < cnt; q += blockDim.x)
+      bwd_row<DOT>(W, p0 + q, (int64_t)base + q, cnt, w, true, s0, s1);
+    __syncthreads();
+  }
+  if (DOT != 0) store_partial<kRunBlock>(s0, s1, W.partial, k);
+}
+
+// update_dilu_diagonal! per case: one workgroup walks the case's levels
+__global__ void __launch_bounds__(kRunBlock)
+    kb_dilu_diag(MatK A, const int32_t* __restrict__ trans, double* D,
+                 const int32_t* __restrict__ lev_ptr, const int32_t* __restrict__ case_lev) {
+  const int k = blockIdx.x;
+  for (int u = case_lev[k]; u < case_lev[k + 1]; ++u) {
+    const int32_t b = lev_ptr[u + 1];
+    for (int32_t j = lev_ptr[u] + threadIdx.x; j < b; j += blockDim.x) {
+      double Dj = D[j];
+      for (int32_t s = A.rb[j]; s < A.diag[j]; s += kSlice) {
+        const int i = A.colval[s];
+        const int kt = trans[s];
+        if (kt >= 0) Dj = Dj - (A.nz[kt] * A.nz[s]) / D[i];
+      }
+      D[j] = Dj;
+    }
+    __syncthreads();
+  }
+}
+
+// per-workgroup partials of |b - A x|^2 and |b|^2; FP32 copy of b - A x
+__global__ void __launch_bounds__(kBlock)
+    kb_residual(BlkK B, MatK A, const double* __restrict__ x, const double* __restrict__ b,
+                float* __restrict__ r32, double* partial) {
+  const int32_t i = B.row0[blockIdx.x] + threadIdx.x;
+  double s0 = 0.0, s1 = 0.0;
+  if (i < B.row1[blockIdx.x]) {
+    const double acc = row_sum(A, (int64_t)i, [&](int j) { return x[j]; });
+    const double r = b[i] - acc;
+    if (r32) r32[i] = (float)r;
+    s0 = r * r;
+    s1 = b[i] * b[i];
+  }
+  store_partial<kBlock>(s0, s1, partial, blockIdx.x);
+}
+
+// x += d on the rows of active cases
+__global__ void __launch_bounds__(kBlock)
+    kb_ir_update(BlkK B, const int32_t* __restrict__ active, const float* __restrict__ d,
+                 double* __restrict__ x) {
+  if (!active[B.kcase[blockIdx.x]]) return;
+  const int32_t i = B.row0[blockIdx.x] + threadIdx.x;
+  if (i < B.row1[blockIdx.x]) x[i] = x[i] + (double)d[i];
+}
+
+__global__ void __launch_bounds__(kBlock)
+    kb_add(int64_t n, const double* __restrict__ x, double* __restrict__ x0) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x0[i] = x[i] + x0[i];
+}
+
+__global__ void __launch_bounds__(kBlock)
+    kb_to_float(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (float)a[i];
+}
+
+// ------------------------------------------------------------ host side
+template <class T>
+struct BSolver {
+  xcal_eqn* e;
+  const xcal_mesh* m;
+  hipStream_t s;
+  int64_t n;
+  int ncase;
+  MatKT<T> A;
+  const T *Sj, *rD, *Lnz, *Unz, *Db;
+  T* work;
+  KS* ks;
+  double* partial;  // nblk slots (also used for the ncase sweep slots)
+  BlkK B;
+  int nblk;
+  int* pred;  // iterations of the last solve (first-batch size)
+};
+
+static void batch_workspace(xcal_eqn* e, bool fp32) {
+  const xcal_mesh* m = e->mesh;
+  const int64_t n = m->n;
+  const int64_t slots = std::max<int64_t>(m->nblk, m->ncase) + 8;
+  const int64_t ksb = (int64_t)sizeof(KS) * m->ncase;
+  const bool grow = (e->work.n > 0 && e->work.n < 12 * n) ||
+                    (e->kpart.n > 0 && e->kpart.n < 2 * slots) ||
+                    (e->kstate.n > 0 && e->kstate.n < ksb) ||
+                    (fp32 && e->work32.n > 0 && e->work32.n < 12 * n);
+  if (grow) XCAL_HIP(hipStreamSynchronize(m->ctx->stream));
+  if (e->work.n < 12 * n) e->work.alloc(12 * n);
+  if (e->kpart.n < 2 * slots) e->kpart.alloc(2 * slots);
+  if (e->kstate.n < ksb) {
+    e->kstate.alloc(ksb);
+    if (e->kstate_host) (void)hipHostFree(e->kstate_host);
+    e->kstate_host = nullptr;
+  }
+  if (!e->kstate_host) XCAL_HIP(hipHostMalloc((void**)&e->kstate_host, ksb, 0));
+  if (e->batch_pred.size() < 2 * 17) e->batch_pred.assign(2 * 17, 0);
+  if (e->case_active.n < m->ncase) e->case_active.alloc(m->ncase);
+  if (fp32) {
+    if (e->work32.n < 12 * n) e->work32.alloc(12 * n);
+    if (e->nz32.n < m->nzlen) e->nz32.alloc(m->nzlen);
+    if (e->pD32.n < n) {
+      e->pD32.alloc(n);
+      e->prD32.alloc(n);
+      e->pS32.alloc(n);
+    }
+  }
+}
+
+static BlkK blkk(const xcal_mesh* m) { return BlkK{m->blk_row0.p, m->blk_row1.p, m->blk_case.p}; }
+
+static BSolver<double> batch_solver(xcal_eqn* e) {
+  batch_workspace(e, false);
+  const xcal_mesh* m = e->mesh;
+  BSolver<double> S;
+  S.e = e;
+  S.m = m;
+  S.s = m->ctx->stream;
+  S.n = m->n;
+  S.ncase = m->ncase;
+  S.A = matk(m, (const double*)e->nzval.p);
+  S.Sj = e->pS.p;
+  S.rD = e->prD.p;
+  S.Lnz = e->Lnz.p;
+  S.Unz = e->Unz.p;
+  S.Db = e->Db.p;
+  S.work = e->work.p;
+  S.ks = (KS*)e->kstate.p;
+  S.partial = e->kpart.p;
+  S.B = blkk(m);
+  S.nblk = (int)m->nblk;
+  S.pred = &e->batch_pred[0];
+  return S;
+}
+
+static BSolver<float> batch_solver32(xcal_eqn* e) {
+  batch_workspace(e, true);
+  const xcal_mesh* m = e->mesh;
+  hipStream_t s = m->ctx->stream;
+  const int64_t n = m->n;
+  auto tofloat = [&](int64_t len, const double* a, float* b) {
+    if (len > 0) hipLaunchKernelGGL(kb_to_float, dim3(grid_for(len)), dim3(kBlock), 0, s, len, a, b);
+  };
+  tofloat(m->nzlen, e->nzval.p, e->nz32.p);
+  if (e->precon == XCAL_PRECON_DILU) {
+    if (e->Lnz32.n != e->Lnz.n) e->Lnz32.alloc(e->Lnz.n);
+    if (e->Unz32.n != e->Unz.n) e->Unz32.alloc(e->Unz.n);
+    if (e->Db32.n != n) e->Db32.alloc(n);
+    tofloat(n, e->prD.p, e->prD32.p);
+    tofloat(n, e->Db.p, e->Db32.p);
+    tofloat(e->Lnz.n, e->Lnz.p, e->Lnz32.p);
+    tofloat(e->Unz.n, e->Unz.p, e->Unz32.p);
+  } else if (e->precon == XCAL_PRECON_JACOBI) {
+    tofloat(n, e->pS.p, e->pS32.p);
+  }
+  XCAL_HIP(hipGetLastError());
+  BSolver<float> S;
+  S.e = e;
+  S.m = m;
+  S.s = s;
+  S.n = n;
+  S.ncase = m->ncase;
+  S.A = matk(m, (const float*)e->nz32.p);
+  S.Sj = e->pS32.p;
+  S.rD = e->prD32.p;
+  S.Lnz = e->Lnz32.p;
+  S.Unz = e->Unz32.p;
+  S.Db = e->Db32.p;
+  S.work = e->work32.p;
+  S.ks = (KS*)e->kstate.p;
+  S.partial = e->kpart.p;
+  S.B = blkk(m);
+  S.nblk = (int)m->nblk;
+  S.pred = &e->batch_pred[2];
+  return S;
+}
+
+template <class T>
+static void fin_blocks(const BSolver<T>& S, int fin) {
+  hipLaunchKernelGGL(kb_finalize, dim3(grid_for(S.ncase)), dim3(kBlock), 0, S.s, S.partial,
+                     S.m->case_blk_d.p, S.ncase, S.ks, fin);
+}
+template <class T>
+static void fin_cases(const BSolver<T>& S, int fin) {
+  hipLaunchKernelGGL(kb_finalize, dim3(grid_for(S.ncase)), dim3(kBlock), 0, S.s, S.partial,
+                     S.m->case_iota_d.p, S.ncase, S.ks, fin);
+}
+
+template <bool UPD, class T>
+static void bsweep_fwd(const BSolver<T>& S, const FwdK<T>& W, const KS* ks) {
+  hipLaunchKernelGGL((kb_fwd<UPD, T>), dim3(S.ncase), dim3(kRunBlock), 0, S.s, W,
+                     S.m->lev_ptr_f_d.p, S.m->lmeta_f_d.p, S.m->case_lev_f_d.p, ks);
+}
+template <int DOT, class T>
+static void bsweep_bwd(const BSolver<T>& S, const BwdK<T>& W, const KS* ks) {
+  hipLaunchKernelGGL((kb_bwd<DOT, T>), dim3(S.ncase), dim3(kRunBlock), 0, S.s, W,
+                     S.m->lev_ptr_b_d.p, S.m->lmeta_b_d.p, S.m->case_lev_b_d.p, ks);
+}
+template <class T>
+static FwdK<T> bfwdk(const BSolver<T>& S, const T* u, T* Y, T* Yb) {
+  return FwdK<T>{S.m->Lcol.p, S.Lnz, S.rD, u, Y, Yb, S.m->posb.p, nullptr, nullptr, nullptr,
+                 nullptr, nullptr};
+}
+template <class T>
+static BwdK<T> bbwdk(const BSolver<T>& S, const T* Yb, T* Z, const T* dv) {
+  return BwdK<T>{S.m->Ucol.p, S.Unz, S.m->lev_rows_b.p, S.Db, Yb, Z, dv, S.partial, nullptr};
+}
+
+// KS of every case; `frozen` cases start stopped (they take no update)
+template <class T>
+static void binit(BSolver<T>& S, int itmax, double atol, double rtol, const int* frozen) {
+  KS* h = (KS*)S.e->kstate_host;
+  for (int k = 0; k < S.ncase; ++k) {
+    KS q{};
+    q.atol = atol;
+    q.rtol = rtol;
+    q.itmax = itmax;
+    q.stop = frozen && frozen[k] ? STOP_SOLVED : STOP_NONE;
+    h[k] = q;
+  }
+  XCAL_HIP(hipMemcpyAsync(S.ks, h, sizeof(KS) * S.ncase, hipMemcpyHostToDevice, S.s));
+}
+
+template <class T>
+static bool bread(BSolver<T>& S) {
+  XCAL_HIP(hipMemcpyAsync(S.e->kstate_host, S.ks, sizeof(KS) * S.ncase, hipMemcpyDeviceToHost, S.s));
+  XCAL_HIP(hipStreamSynchronize(S.s));
+  const KS* h = (const KS*)S.e->kstate_host;
+  for (int k = 0; k < S.ncase; ++k)
+    if (h[k].stop == STOP_NONE) return false;
+  return true;
+}
+
+// iterations in batches until every case has stopped; the first batch is
+// sized from the previous solve (longest case)
+template <class T>
+static void brun(BSolver<T>& S, int tail, const std::function<void()>& iteration) {
+  if (bread(S)) return;
+  int batch = *S.pred > 0 ? std::max(1, (*S.pred * 9) / 10) : 8;
+  for (;;) {
+    for (int b = 0; b < batch; ++b) iteration();
+    XCAL_HIP(hipGetLastError());
+    if (bread(S)) break;
+    batch = tail;
+  }
+  const KS* h = (const KS*)S.e->kstate_host;
+  int mx = 0;
+  for (int k = 0; k < S.ncase; ++k) mx = std::max(mx, h[k].iter);
+  *S.pred = mx;
+}
+
+// x0 == NULL: zero start, solution left in *xsol
+template <class T>
+static void bcg(BSolver<T>& S, int precon, const T* b, T* x0, int itmax, double atol, double rtol,
+                const int* frozen, T** xsol) {
+  const int64_t n = S.n;
+  T* w = S.work;
+  T *r = w, *x = w + n, *pa = w + 2 * n, *pb = w + 3 * n, *q = w + 4 * n, *zd = w + 5 * n,
+    *Y = w + 6 * n, *Yb = w + 7 * n;
+  binit(S, itmax, atol, rtol, frozen);
+  hipLaunchKernelGGL(kb_start<T>, dim3(grid_for(n)), dim3(kBlock), 0, S.s, S.A, b, (const T*)x0, r,
+                     x, pa);
+  const bool dilu = precon == XCAL_PRECON_DILU;
+  const T* Sd = precon == XCAL_PRECON_JACOBI ? S.Sj : nullptr;
+  const KS* ks = S.ks;
+  if (dilu) {
+    bsweep_fwd<false>(S, bfwdk(S, (const T*)r, Y, Yb), ks);
+    bsweep_bwd<1>(S, bbwdk(S, (const T*)Yb, zd, (const T*)r), ks);
+    fin_cases(S, FIN_CG_INIT);
+  } else {
+    hipLaunchKernelGGL((kb_precond_diag_start<false, T>), dim3(S.nblk), dim3(kBlock), 0, S.s, S.B,
+                       Sd, r, zd, (const T*)nullptr, S.partial);
+    fin_blocks(S, FIN_CG_INIT);
+  }
+  int parity = 0;
+  auto iteration = [&]() {
+    T* pold = parity ? pb : pa;
+    T* pnew = parity ? pa : pb;
+    parity ^= 1;
+    hipLaunchKernelGGL(kb_cg_spmv<T>, dim3(S.nblk), dim3(kBlock), 0, S.s, S.B, S.A, (const T*)zd,
+                       (const T*)pold, pnew, q, S.partial, ks);
+    fin_blocks(S, FIN_CG_PAP);
+    if (dilu) {
+      FwdK<T> W = bfwdk(S, (const T*)nullptr, Y, Yb);
+      W.p = pnew;
+      W.q = q;
+      W.x = x;
+      W.r = r;
+      bsweep_fwd<true>(S, W, ks);
+      bsweep_bwd<1>(S, bbwdk(S, (const T*)Yb, zd, (const T*)r), ks);
+      fin_cases(S, FIN_CG_ITER);
+    } else {
+      hipLaunchKernelGGL(kb_cg_update_diag<T>, dim3(S.nblk), dim3(kBlock), 0, S.s, S.B, Sd,
+                         (const T*)pnew, (const T*)q, x, r, zd, S.partial, ks);
+      fin_blocks(S, FIN_CG_ITER);
+    }
+  };
+  brun(S, 4, iteration);
+  if (x0) hipLaunchKernelGGL(kb_add, dim3(grid_for(n)), dim3(kBlock), 0, S.s, n, (const double*)x,
+                             (double*)x0);
+  if (xsol) *xsol = x;
+  XCAL_HIP(hipGetLastError());
+}
+
+template <class T>
+static void bbicg(BSolver<T>& S, int precon, const T* b, T* x0, int itmax, double atol,
+                  double rtol, const int* frozen, T** xsol) {
+  const int64_t n = S.n;
+  T* w = S.work;
+  T *r = w, *x = w + n, *pa = w + 2 * n, *pb = w + 3 * n, *v = w + 4 * n, *sv = w + 5 * n,
+    *t = w + 6 * n, *d = w + 7 * n, *Y = w + 8 * n, *Yb = w + 9 * n;
+  binit(S, itmax, atol, rtol, frozen);
+  hipLaunchKernelGGL(kb_start<T>, dim3(grid_for(n)), dim3(kBlock), 0, S.s, S.A, b, (const T*)x0, r,
+                     x, pa);
+  const bool dilu = precon == XCAL_PRECON_DILU;
+  const T* Sd = precon == XCAL_PRECON_JACOBI ? S.Sj : nullptr;
+  const KS* ks = S.ks;
+  if (dilu) {
+    bsweep_fwd<false>(S, bfwdk(S, (const T*)r, Y, Yb), ks);
+    bsweep_bwd<4>(S, bbwdk(S, (const T*)Yb, r, b), ks);
+    fin_cases(S, FIN_BI_INIT);
+  } else {
+    hipLaunchKernelGGL((kb_precond_diag_start<true, T>), dim3(S.nblk), dim3(kBlock), 0, S.s, S.B, Sd,
+                       r, (T*)nullptr, b, S.partial);
+    fin_blocks(S, FIN_BI_INIT);
+  }
+  int parity = 0;
+  auto iteration = [&]() {
+    T* pold = parity ? pb : pa;
+    T* pnew = parity ? pa : pb;
+    parity ^= 1;
+    if (dilu) {
+      hipLaunchKernelGGL((kb_bi_spmv_p<false, T>), dim3(S.nblk), dim3(kBlock), 0, S.s, S.B, S.A,
+                         (const T*)r, (const T*)pold, pnew, (const T*)nullptr, d, b, S.partial, ks);
+      bsweep_fwd<false>(S, bfwdk(S, (const T*)d, Y, Yb), ks);
+      bsweep_bwd<2>(S, bbwdk(S, (const T*)Yb, v, b), ks);
+      fin_cases(S, FIN_BI_ALPHA);
+      hipLaunchKernelGGL((kb_bi_spmv_s<false, T>), dim3(S.nblk), dim3(kBlock), 0, S.s, S.B, S.A,
+                         (const T*)r, (const T*)v, (const T*)pnew, sv, x, (const T*)nullptr, d,
+                         S.partial, ks);
+      bsweep_fwd<false>(S, bfwdk(S, (const T*)d, Y, Yb), ks);
+      bsweep_bwd<3>(S, bbwdk(S, (const T*)Yb, t, (const T*)sv), ks);
+      fin_cases(S, FIN_BI_OMEGA);
+    } else {
+      hipLaunchKernelGGL((kb_bi_spmv_p<true, T>), dim3(S.nblk), dim3(kBlock), 0, S.s, S.B, S.A,
+                         (const T*)r, (const T*)pold, pnew, Sd, v, b, S.partial, ks);
+      fin_blocks(S, FIN_BI_ALPHA);
+      hipLaunchKernelGGL((kb_bi_spmv_s<true, T>), dim3(S.nblk), dim3(kBlock), 0, S.s, S.B, S.A,
+                         (const T*)r, (const T*)v, (const T*)pnew, sv, x, Sd, t, S.partial, ks);
+      fin_blocks(S, FIN_BI_OMEGA);
+    }
+    hipLaunchKernelGGL(kb_bi_tail<T>, dim3(S.nblk), dim3(kBlock), 0, S.s, S.B, (const T*)sv,
+                       (const T*)t, (const T*)v, x, r, pnew, b, S.partial, ks);
+    fin_blocks(S, FIN_BI_END);
+  };
+  brun(S, 2, iteration);
+  if (x0) hipLaunchKernelGGL(kb_add, dim3(grid_for(n)), dim3(kBlock), 0, S.s, n, (const double*)x,
+                             (double*)x0);
+  if (xsol) *xsol = x;
+  XCAL_HIP(hipGetLastError());
+}
+
+// per-case |b - A x|^2 and |b|^2 (host arrays of ncase); r32 optional
+static void bresidual_sums(BSolver<double>& S, const double* x, const double* b, float* r32,
+                           std::vector<double>& rr, std::vector<double>& bb) {
+  xcal_ctx* c = S.m->ctx;
+  const int K = S.ncase;
+  if (c->red_result.n < 2 * (int64_t)K) c->red_result.alloc(2 * (int64_t)K + 64);
+  if (S.e->case_host.size() < 2 * (size_t)K) {
+    S.e->case_host.resize(2 * (size_t)K);
+  }
+  hipLaunchKernelGGL(kb_residual, dim3(S.nblk), dim3(kBlock), 0, S.s, S.B, S.A, x, b, r32, S.partial);
+  hipLaunchKernelGGL(kb_sum_cases, dim3(grid_for(K)), dim3(kBlock), 0, S.s, S.partial,
+                     S.m->case_blk_d.p, K, c->red_result.p);
+  XCAL_HIP(hipGetLastError());
+  XCAL_HIP(hipMemcpyAsync(S.e->case_host.data(), c->red_result.p, 2 * sizeof(double) * K,
+                          hipMemcpyDeviceToHost, S.s));
+  XCAL_HIP(hipStreamSynchronize(S.s));
+  rr.resize(K);
+  bb.resize(K);
+  for (int k = 0; k < K; ++k) {
+    rr[k] = S.e->case_host[2 * k];
+    bb[k] = S.e->case_host[2 * k + 1];
+  }
+}
+
+// residual() of Solve_1_api.jl:441-464 per case
+static void bresidual(BSolver<double>& S, const double* x, const double* b, double* out) {
+  std::vector<double> rr, bb;
+  bresidual_sums(S, x, b, nullptr, rr, bb);
+  for (int k = 0; k < S.ncase; ++k) {
+    const double normb = std::sqrt(bb[k]);
+    const double a = std::fabs(normb);
+    const double eps_nb = std::nextafter(a, INFINITY) - a;
+    out[k] = std::sqrt(rr[k]) / (normb > eps_nb ? normb : 1.0);
+  }
+}
+
+// FP64 refinement around FP32 batched solves, per case as in ir_solve:
+// a case whose true residual met its tolerance (or stalled) is frozen and
+// takes no further update, so its result does not depend on the others.
+static void bir_solve(xcal_eqn* e, const xcal_solver_setup* st, int itmax, double* x,
+                      const double* b, std::vector<int>& niter, std::vector<int>& nouter,
+                      std::vector<int>& rcs) {
+  BSolver<double> S = batch_solver(e);
+  BSolver<float> S32 = batch_solver32(e);
+  const int K = S.ncase;
+  float* r32 = e->work32.p + 11 * S.n;
+  const double inner_rtol = st->ir_inner_rtol > 0 ? st->ir_inner_rtol : 1e-4;
+  std::vector<double> rr, bb, res(K), prev(K, INFINITY), tol(K);
+  bresidual_sums(S, x, b, r32, rr, bb);
+  std::vector<int> frozen(K, 0);
+  niter.assign(K, 0);
+  nouter.assign(K, 0);
+  rcs.assign(K, 0);
+  for (int k = 0; k < K; ++k) {
+    res[k] = std::sqrt(rr[k]);
+    tol[k] = st->atol + st->rtol * res[k];
+  }
+  for (int sweep = 0;; ++sweep) {
+    bool any = false;
+    for (int k = 0; k < K; ++k) {
+      if (frozen[k]) continue;
+      if (res[k] <= tol[k]) {
+        frozen[k] = 1;
+      } else if (niter[k] >= itmax || nouter[k] >= 100 || res[k] >= 0.5 * prev[k]) {
+        frozen[k] = 1;
+        rcs[k] = 1;
+      } else {
+        any = true;
+      }
+    }
+    if (!any) break;
+    float* d = nullptr;
+    S32.pred = &e->batch_pred[4 + std::min(sweep, 12)];
+    if (st->solver == XCAL_SOLVER_CG)
+      bcg(S32, st->precon, (const float*)r32, (float*)nullptr, itmax, 0.0, inner_rtol, frozen.data(),
+          &d);
+    else
+      bbicg(S32, st->precon, (const float*)r32, (float*)nullptr, itmax, 0.0, inner_rtol,
+            frozen.data(), &d);
+    const KS* h = (const KS*)e->kstate_host;
+    std::vector<int32_t> active(K);
+    for (int k = 0; k < K; ++k) {
+      active[k] = !frozen[k];
+      if (!frozen[k]) {
+        niter[k] += h[k].iter;
+        nouter[k] += 1;
+      }
+    }
+    XCAL_HIP(hipMemcpy(e->case_active.p, active.data(), sizeof(int32_t) * K, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(kb_ir_update, dim3(S.nblk), dim3(kBlock), 0, S.s, S.B,
+                       (const int32_t*)e->case_active.p, (const float*)d, x);
+    for (int k = 0; k < K; ++k)
+      if (!frozen[k]) prev[k] = res[k];
+    bresidual_sums(S, x, b, r32, rr, bb);
+    for (int k = 0; k < K; ++k)
+      if (!frozen[k]) res[k] = std::sqrt(rr[k]);
+  }
+}
+
+void precond_update_batch(xcal_eqn* e) {
+  const xcal_mesh* m = e->mesh;
+  hipLaunchKernelGGL(kb_dilu_diag, dim3(m->ncase), dim3(kRunBlock), 0, m->ctx->stream,
+                     matk(m, (const double*)e->nzval.p), m->trans.p, e->pD.p, m->lev_ptr_f_d.p,
+                     m->case_lev_f_d.p);
+}
+
+// solve_system! over every case of a batch mesh; per-case statistics
+int solve_system_batch(xcal_eqn* e, const xcal_solver_setup* st, double* x, const double* b,
+                       xcal_solve_stats* stats) {
+  const int K = e->mesh->ncase;
+  const int itmax = st->itmax == 0 ? (int)(2 * e->mesh->n) : st->itmax;
+  std::vector<int> niter(K, 0), nouter(K, 1), rcs(K, 0);
+  if (st->precision == XCAL_PREC_FP32_IR) {
+    bir_solve(e, st, itmax, x, b, niter, nouter, rcs);
+  } else {
+    BSolver<double> S = batch_solver(e);
+    if (st->solver == XCAL_SOLVER_CG)
+      bcg(S, st->precon, b, x, itmax, st->atol, st->rtol, (const int*)nullptr, (double**)nullptr);
+    else
+      bbicg(S, st->precon, b, x, itmax, st->atol, st->rtol, (const int*)nullptr, (double**)nullptr);
+    const KS* h = (const KS*)e->kstate_host;
+    for (int k = 0; k < K; ++k) {
+      niter[k] = h[k].iter;
+      rcs[k] = (h[k].stop == STOP_ITMAX || h[k].iter >= itmax) ? 1 : 0;
+    }
+  }
+  BSolver<double> S = batch_solver(e);
+  std::vector<double> res(K);
+  bresidual(S, x, b, res.data());
+  int rc = 0;
+  for (int k = 0; k < K; ++k) {
+    if (stats) {
+      stats[k].niter = niter[k];
+      stats[k].nouter = nouter[k];
+      stats[k].residual = res[k];
+    }
+    rc |= rcs[k];
+  }
+  return rc;
+}
+
+}  // namespace xcal

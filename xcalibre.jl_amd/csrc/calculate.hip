@@ -231,12 +231,30 @@ __global__ void __launch_bounds__(kBlock)
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double sx = 0.0, sy = 0.0, sz = 0.0;
-  for (int32_t k = rb[i]; k < re[i]; k += kSlice) {
-    const int j = colval[k];
-    const double v = nz[k];
-    sx += v * Ux[j];
-    sy += v * Uy[j];
-    sz += v * Uz[j];
+  const int32_t k1 = re[i];
+  for (int32_t k0 = rb[i]; k0 < k1; k0 += kRowAhead * kSlice) {  // loads ahead, then gathers
+    int col[kRowAhead];
+    double v[kRowAhead], ux[kRowAhead], uy[kRowAhead], uz[kRowAhead];
+#pragma unroll
+    for (int u = 0; u < kRowAhead; ++u) {
+      const int32_t k = k0 + u * kSlice;
+      col[u] = k < k1 ? colval[k] : -1;
+      v[u] = k < k1 ? nz[k] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kRowAhead; ++u) {
+      const bool ok = col[u] >= 0;
+      ux[u] = ok ? Ux[col[u]] : 0.0;
+      uy[u] = ok ? Uy[col[u]] : 0.0;
+      uz[u] = ok ? Uz[col[u]] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kRowAhead; ++u)
+      if (col[u] >= 0) {
+        sx += v[u] * ux[u];
+        sy += v[u] * uy[u];
+        sz += v[u] * uz[u];
+      }
   }
   const double D = nz[diag[i]];
   sx -= D * Ux[i];

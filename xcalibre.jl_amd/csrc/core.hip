@@ -118,6 +118,654 @@ void fill(hipStream_t s, double* p, int64_t n, double v) {
   hipLaunchKernelGGL(k_fill, dim3(grid_for(n)), dim3(kBlock), 0, s, p, n, v);
 }
 
+// ---------------------------------------------------------------- meshes
+// Host-side build of one case in device numbering: validation, CSR pattern,
+// DILU levels, renumbering, sliced-ELL slots and packed factors.
+struct HostMesh {
+  int dim = 3, n_patches = 0, nlev_f = 0, nlev_b = 0;
+  int64_t n = 0, nf = 0, nbf = 0, nnz = 0, ncf = 0, nzlen = 0, nbcell = 0;
+  std::vector<int64_t> patch_first, patch_last;
+  std::vector<double> vol, area, weight, delta, nx, ny, nz, ex, ey, ez, sfx, sfy, sfz, lapEf;
+  std::vector<int32_t> cf_ptr, cf_face, cf_nbr, cf_slot, diag, own, nei, fslot, bc_cells, bc_ptr,
+      bc_face, rbeg, rend, colval, trans, rows_b, posb, Lcol, Lslot, Ucol, Uslot, lev_ptr_f,
+      lev_ptr_b, lmeta_f, lmeta_b, cperm, fperm, nzperm, h_rowptr, h_colval;
+  std::vector<int8_t> cf_sign, bpatch;
+};
+
+static void build_host_mesh(const xcal_mesh_host* h, HostMesh& M) {
+  XCAL_CHECK(h->dim == 2 || h->dim == 3, "dim must be 2 or 3");
+  XCAL_CHECK(h->n_cells > 0 && h->n_faces > 0, "empty mesh");
+  XCAL_CHECK(h->n_cells < (1ll << 30) && h->n_faces < (1ll << 30), "mesh too large for int32 indices");
+  XCAL_CHECK(h->n_bfaces >= 0 && h->n_bfaces <= h->n_faces, "bad n_bfaces");
+  M.dim = h->dim;
+  const int64_t n = h->n_cells, nf = h->n_faces, nbf = h->n_bfaces;
+  M.n = n;
+  M.nf = nf;
+  M.nbf = nbf;
+  M.n_patches = h->n_patches;
+  XCAL_CHECK(h->n_patches <= kMaxPatches, "mesh has more than 32 patches");
+  for (int p = 0; p < h->n_patches; ++p) {
+    int64_t a = h->patch_range[2 * p] - 1, b = h->patch_range[2 * p + 1] - 1;
+    XCAL_CHECK(a >= 0 && b < nbf && a <= b + 1, "patch range outside boundary faces");
+    M.patch_first.push_back(a);
+    M.patch_last.push_back(b);
+  }
+  // cell -> internal face lists in the reference's faces_range order
+  std::vector<int32_t> cf_ptr(n + 1, 0), cf_face, cf_nbr;
+  std::vector<int8_t> cf_sign;
+  cf_face.reserve(h->n_cell_faces);
+  cf_nbr.reserve(h->n_cell_faces);
+  cf_sign.reserve(h->n_cell_faces);
+  for (int64_t c = 0; c < n; ++c) {
+    int64_t a = h->cell_faces_range[2 * c], b = h->cell_faces_range[2 * c + 1];
+    for (int64_t k = a; k <= b; ++k) {
+      XCAL_CHECK(k >= 1 && k <= h->n_cell_faces, "faces_range outside cell_faces");
+      int64_t f = h->cell_faces[k - 1] - 1, nb = h->cell_neighbours[k - 1] - 1;
+      XCAL_CHECK(f >= nbf && f < nf, "cell face is not an internal face");
+      XCAL_CHECK(nb >= 0 && nb < n, "neighbour index out of range");
+      XCAL_CHECK(h->cell_nsign[k - 1] == 1 || h->cell_nsign[k - 1] == -1, "nsign must be +-1");
+      cf_face.push_back((int32_t)f);
+      cf_nbr.push_back((int32_t)nb);
+      cf_sign.push_back((int8_t)h->cell_nsign[k - 1]);
+    }
+    cf_ptr[c + 1] = (int32_t)cf_face.size();
+  }
+  M.ncf = (int64_t)cf_face.size();
+  // CSR pattern: row i = {i} U neighbours(i), ascending, unique
+  // (sparse_matrix_connectivity + sparsecsr, ModelFramework_0_types.jl:223-242)
+  std::vector<int32_t> rowptr(n + 1, 0), colval;
+  colval.reserve(n + cf_face.size());
+  std::vector<int32_t> row;
+  for (int64_t c = 0; c < n; ++c) {
+    row.clear();
+    row.push_back((int32_t)c);
+    for (int32_t k = cf_ptr[c]; k < cf_ptr[c + 1]; ++k) row.push_back(cf_nbr[k]);
+    std::sort(row.begin(), row.end());
+    row.erase(std::unique(row.begin(), row.end()), row.end());
+    colval.insert(colval.end(), row.begin(), row.end());
+    rowptr[c + 1] = (int32_t)colval.size();
+  }
+  const int64_t nnz = (int64_t)colval.size();
+  M.nnz = nnz;
+  auto find = [&](int64_t i, int32_t j) -> int32_t {
+    const int32_t* b = colval.data() + rowptr[i];
+    const int32_t* e = colval.data() + rowptr[i + 1];
+    const int32_t* it = std::lower_bound(b, e, j);
+    return (it != e && *it == j) ? (int32_t)(it - colval.data()) : -1;
+  };
+  std::vector<int32_t> diag(n), cf_slot(cf_face.size()), trans(nnz);
+  for (int64_t c = 0; c < n; ++c) {
+    diag[c] = find(c, (int32_t)c);
+    for (int32_t k = cf_ptr[c]; k < cf_ptr[c + 1]; ++k) cf_slot[k] = find(c, cf_nbr[k]);
+    for (int32_t k = rowptr[c]; k < rowptr[c + 1]; ++k) trans[k] = find(colval[k], (int32_t)c);
+  }
+  // faces
+  std::vector<int32_t> own(nf), nei(nf);
+  std::vector<double> area(h->face_area, h->face_area + nf), weight(h->face_weight, h->face_weight + nf),
+      delta(h->face_delta, h->face_delta + nf), nx(nf), ny(nf), nz(nf), ex(nf), ey(nf), ez(nf),
+      sfx(nf), sfy(nf), sfz(nf), lapEf(nf);
+  for (int64_t f = 0; f < nf; ++f) {
+    own[f] = (int32_t)(h->face_owner[2 * f] - 1);
+    nei[f] = (int32_t)(h->face_owner[2 * f + 1] - 1);
+    XCAL_CHECK(own[f] >= 0 && own[f] < n && nei[f] >= 0 && nei[f] < n, "ownerCells out of range");
+    nx[f] = h->face_normal[3 * f];
+    ny[f] = h->face_normal[3 * f + 1];
+    nz[f] = h->face_normal[3 * f + 2];
+    ex[f] = h->face_e[3 * f];
+    ey[f] = h->face_e[3 * f + 1];
+    ez[f] = h->face_e[3 * f + 2];
+    // Sf = area*normal (flux!, green_gauss!), same rounding as the reference
+    sfx[f] = area[f] * nx[f];
+    sfy[f] = area[f] * ny[f];
+    sfz[f] = area[f] * nz[f];
+    // Laplacian{Linear} |E_f| (Discretise_1_schemes.jl:74-88); independent
+    // of the visiting side's nsign because every negation there is exact
+    double s0 = area[f] * nx[f], s1 = area[f] * ny[f], s2 = area[f] * nz[f];
+    double cfac = (s0 * s0 + s1 * s1 + s2 * s2) / (s0 * ex[f] + s1 * ey[f] + s2 * ez[f]);
+    double e0 = cfac * ex[f], e1 = cfac * ey[f], e2 = cfac * ez[f];
+    lapEf[f] = std::sqrt(e0 * e0 + e1 * e1 + e2 * e2);
+  }
+  std::vector<int32_t> fslot(nf, -1);
+  for (int64_t f = nbf; f < nf; ++f) fslot[f] = find(own[f], nei[f]);
+  // boundary faces grouped per cell, ascending face id
+  std::vector<int8_t> bpatch(nbf, -1);
+  for (int p = 0; p < h->n_patches; ++p)
+    for (int64_t f = M.patch_first[p]; f <= M.patch_last[p]; ++f)
+      if (bpatch[f] < 0) bpatch[f] = (int8_t)p;
+  std::vector<int32_t> bcount(n, 0);
+  for (int64_t f = 0; f < nbf; ++f) {
+    int64_t c = h->boundary_cellsID[f] - 1;
+    XCAL_CHECK(c >= 0 && c < n, "boundary_cellsID out of range");
+    bcount[c]++;
+  }
+  std::vector<int32_t> bc_cells, bc_ptr(1, 0), bc_face;
+  std::vector<int32_t> bstart(n, -1);
+  for (int64_t c = 0; c < n; ++c)
+    if (bcount[c] > 0) {
+      bstart[c] = (int32_t)bc_cells.size();
+      bc_cells.push_back((int32_t)c);
+      bc_ptr.push_back(bc_ptr.back() + bcount[c]);
+    }
+  bc_face.assign(nbf, 0);
+  {
+    std::vector<int32_t> cur(bc_cells.size(), 0);
+    for (int64_t f = 0; f < nbf; ++f) {
+      int64_t c = h->boundary_cellsID[f] - 1;
+      int32_t s = bstart[c];
+      bc_face[bc_ptr[s] + cur[s]++] = (int32_t)f;
+    }
+  }
+  M.nbcell = (int64_t)bc_cells.size();
+  // DILU level schedules: forward level(i) = 1 + max level of lower
+  // neighbours; backward symmetric over upper neighbours.  Rows inside a
+  // level are independent; level order preserves every i<j dependency.
+  std::vector<int32_t> flev(n, 0), blev(n, 0);
+  int maxl = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    int l = 0;
+    for (int32_t k = rowptr[i]; k < diag[i]; ++k) l = std::max(l, flev[colval[k]] + 1);
+    flev[i] = l;
+    maxl = std::max(maxl, l);
+  }
+  M.nlev_f = maxl + 1;
+  maxl = 0;
+  for (int64_t i = n - 1; i >= 0; --i) {
+    int l = 0;
+    for (int32_t k = diag[i] + 1; k < rowptr[i + 1]; ++k) l = std::max(l, blev[colval[k]] + 1);
+    blev[i] = l;
+    maxl = std::max(maxl, l);
+  }
+  M.nlev_b = maxl + 1;
+  std::vector<double> vol(h->cell_volume, h->cell_volume + n);
+
+  // ---- device numbering ------------------------------------------------
+  // Cells are stored level-major: by forward DILU level, and in Morton
+  // order of their centres inside a level.  Each forward level is then one
+  // contiguous row range, and rows that are neighbours in a level are
+  // neighbours in space, so neighbour gathers hit nearby cache lines.
+  // Internal faces follow their renumbered cells.  Every matrix row keeps
+  // its entries in the reference's (original) column order, so row sums,
+  // the DILU lower/upper split and all per-row summation orders are
+  // unchanged; fields are permuted only when they cross the C ABI.
+  std::vector<int32_t> cperm(n), cinv(n), fperm(nf), finv(nf);
+  {
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int64_t c = 0; c < n; ++c)
+      for (int a = 0; a < 3; ++a) {
+        lo[a] = std::min(lo[a], h->cell_centre[3 * c + a]);
+        hi[a] = std::max(hi[a], h->cell_centre[3 * c + a]);
+      }
+    const int naxis = h->dim == 3 ? 3 : 2;
+    const int bits = naxis == 3 ? 21 : 31;
+    auto spread = [&](uint64_t v) {  // interleave with naxis-1 zero bits
+      uint64_t r = 0;
+      for (int b = 0; b < bits; ++b) r |= ((v >> b) & 1ull) << (naxis * b);
+      return r;
+    };
+    std::vector<std::pair<std::pair<int32_t, uint64_t>, int32_t>> key(n);
+    for (int64_t c = 0; c < n; ++c) {
+      uint64_t k = 0;
+      for (int a = 0; a < naxis; ++a) {
+        double span = hi[a] - lo[a];
+        double t = span > 0 ? (h->cell_centre[3 * c + a] - lo[a]) / span : 0.0;
+        uint64_t q = (uint64_t)std::min(t * (double)((1ull << bits) - 1), (double)((1ull << bits) - 1));
+        k |= spread(q) << a;
+      }
+      key[c] = {{flev[c], k}, (int32_t)c};
+    }
+    std::sort(key.begin(), key.end());
+    for (int64_t c = 0; c < n; ++c) {
+      cinv[c] = key[c].second;
+      cperm[key[c].second] = (int32_t)c;
+    }
+    for (int64_t f = 0; f < nbf; ++f) fperm[f] = finv[f] = (int32_t)f;  // boundary faces keep ids
+    std::vector<std::pair<std::pair<int32_t, int32_t>, int32_t>> fk;
+    fk.reserve(nf - nbf);
+    for (int64_t f = nbf; f < nf; ++f) {
+      int32_t a = cperm[own[f]], b = cperm[nei[f]];
+      fk.push_back({{std::min(a, b), std::max(a, b)}, (int32_t)f});
+    }
+    std::sort(fk.begin(), fk.end());
+    for (int64_t k = 0; k < nf - nbf; ++k) {
+      finv[nbf + k] = fk[k].second;
+      fperm[fk[k].second] = (int32_t)(nbf + k);
+    }
+  }
+  // Matrix storage: sliced ELL with slices of kSlice consecutive rows, each
+  // slice column-major and as wide as its longest row.  Entry t of device
+  // row c sits at rbeg[c] + t*kSlice, so a wavefront reading entry t of 64
+  // rows reads one contiguous segment.  Padding entries are never read.
+  const int64_t nslice = (n + kSlice - 1) / kSlice;
+  std::vector<int32_t> rbeg(n), rend(n);
+  int64_t nzlen = 0;
+  for (int64_t s = 0; s < nslice; ++s) {
+    int32_t w = 0;
+    const int64_t c1 = std::min<int64_t>(n, (s + 1) * kSlice);
+    for (int64_t c = s * kSlice; c < c1; ++c) w = std::max(w, rowptr[cinv[c] + 1] - rowptr[cinv[c]]);
+    for (int64_t c = s * kSlice; c < c1; ++c) {
+      rbeg[c] = (int32_t)(nzlen + (c - s * kSlice));
+      rend[c] = rbeg[c] + kSlice * (rowptr[cinv[c] + 1] - rowptr[cinv[c]]);
+    }
+    nzlen += (int64_t)w * kSlice;
+    XCAL_CHECK(nzlen < (1ll << 31), "matrix storage exceeds int32 indices");
+  }
+  M.nzlen = nzlen;
+  std::vector<int32_t> colval_n(nzlen), nzperm(nnz), diag_n(n), trans_n(nzlen, -1);
+  for (int64_t oc = 0; oc < n; ++oc)
+    for (int32_t k = rowptr[oc]; k < rowptr[oc + 1]; ++k)
+      nzperm[k] = rbeg[cperm[oc]] + kSlice * (k - rowptr[oc]);
+  for (int64_t k = 0; k < nnz; ++k) colval_n[nzperm[k]] = cperm[colval[k]];
+  for (int64_t k = 0; k < nnz; ++k) trans_n[nzperm[k]] = trans[k] >= 0 ? nzperm[trans[k]] : -1;
+  for (int64_t oc = 0; oc < n; ++oc) diag_n[cperm[oc]] = nzperm[diag[oc]];
+  // level schedules in device numbering: forward levels are the contiguous
+  // ranges of the level-major order; backward levels list their rows
+  std::vector<int32_t> rows_b;
+  {
+    M.lev_ptr_f.assign(M.nlev_f + 1, 0);
+    for (int64_t i = 0; i < n; ++i) M.lev_ptr_f[flev[i] + 1]++;
+    for (int l = 0; l < M.nlev_f; ++l) M.lev_ptr_f[l + 1] += M.lev_ptr_f[l];
+    for (int64_t c = 0; c < n; ++c)
+      XCAL_CHECK(c >= M.lev_ptr_f[flev[cinv[c]]] && c < M.lev_ptr_f[flev[cinv[c]] + 1],
+                 "internal: level-major numbering");
+    M.lev_ptr_b.assign(M.nlev_b + 1, 0);
+    for (int64_t i = 0; i < n; ++i) M.lev_ptr_b[blev[i] + 1]++;
+    for (int l = 0; l < M.nlev_b; ++l) M.lev_ptr_b[l + 1] += M.lev_ptr_b[l];
+    rows_b.assign(n, 0);
+    std::vector<int32_t> cur(M.lev_ptr_b.begin(), M.lev_ptr_b.end() - 1);
+    for (int64_t c = 0; c < n; ++c) rows_b[cur[blev[cinv[c]]]++] = (int32_t)c;
+  }
+  // Packed triangular factors for the sweeps.  Level l of a sweep stores
+  // entry t of its q-th row at base[l] + t*cnt_l + q (column-major inside
+  // the level, as wide as the level's longest row, padding marked by
+  // column -1), in the row's column order.  The address depends only on
+  // the row's position, so a sweep thread loads all its entries at once.
+  // Backward rows are in lev_rows_b order; posb maps a device row to it.
+  std::vector<int32_t> Lcol, Lslot, Ucol, Uslot, posb(n);
+  for (int64_t p = 0; p < n; ++p) posb[rows_b[p]] = (int32_t)p;
+  auto pack = [&](const std::vector<int32_t>& ptr, int nlev, const int32_t* rows, bool lower,
+                  std::vector<int32_t>& meta, std::vector<int32_t>& col,
+                  std::vector<int32_t>& slot) {
+    meta.assign(2 * (size_t)nlev, 0);
+    auto span = [&](int32_t c, int32_t& k0, int32_t& k1) {
+      k0 = lower ? rbeg[c] : diag_n[c] + kSlice;
+      k1 = lower ? diag_n[c] : rend[c];
+    };
+    int64_t total = 0;
+    for (int l = 0; l < nlev; ++l) {
+      int32_t w = 0;
+      for (int32_t pos = ptr[l]; pos < ptr[l + 1]; ++pos) {
+        int32_t k0, k1;
+        span(rows ? rows[pos] : pos, k0, k1);
+        w = std::max(w, (k1 - k0) / kSlice);
+      }
+      meta[2 * l] = (int32_t)total;
+      meta[2 * l + 1] = w;
+      total += (int64_t)w * (ptr[l + 1] - ptr[l]);
+      XCAL_CHECK(total < (1ll << 31), "packed factor exceeds int32 indices");
+    }
+    col.assign(total, -1);
+    slot.assign(total, -1);
+    for (int l = 0; l < nlev; ++l) {
+      const int32_t cnt = ptr[l + 1] - ptr[l];
+      for (int32_t q = 0; q < cnt; ++q) {
+        const int32_t c = rows ? rows[ptr[l] + q] : ptr[l] + q;
+        int32_t k0, k1;
+        span(c, k0, k1);
+        for (int32_t k = k0, t = 0; k < k1; k += kSlice, ++t) {
+          const int64_t e = meta[2 * l] + (int64_t)t * cnt + q;
+          col[e] = colval_n[k];
+          slot[e] = k;
+        }
+      }
+    }
+  };
+  pack(M.lev_ptr_f, M.nlev_f, nullptr, true, M.lmeta_f, Lcol, Lslot);
+  pack(M.lev_ptr_b, M.nlev_b, rows_b.data(), false, M.lmeta_b, Ucol, Uslot);
+  std::vector<int32_t> cf_ptr_n(n + 1, 0), cf_face_n(cf_face.size()), cf_nbr_n(cf_face.size()),
+      cf_slot_n(cf_face.size());
+  std::vector<int8_t> cf_sign_n(cf_face.size());
+  for (int64_t c = 0; c < n; ++c) {
+    const int32_t oc = cinv[c];
+    int32_t w = cf_ptr_n[c];
+    for (int32_t k = cf_ptr[oc]; k < cf_ptr[oc + 1]; ++k, ++w) {
+      cf_face_n[w] = fperm[cf_face[k]];
+      cf_nbr_n[w] = cperm[cf_nbr[k]];
+      cf_slot_n[w] = nzperm[cf_slot[k]];
+      cf_sign_n[w] = cf_sign[k];
+    }
+    cf_ptr_n[c + 1] = w;
+  }
+  auto permf = [&](std::vector<double>& a) {
+    std::vector<double> t(nf);
+    for (int64_t f = 0; f < nf; ++f) t[fperm[f]] = a[f];
+    a.swap(t);
+  };
+  for (auto* a : {&area, &weight, &delta, &nx, &ny, &nz, &ex, &ey, &ez, &sfx, &sfy, &sfz, &lapEf})
+    permf(*a);
+  std::vector<int32_t> own_n(nf), nei_n(nf), fslot_n(nf, -1);
+  for (int64_t f = 0; f < nf; ++f) {
+    own_n[fperm[f]] = cperm[own[f]];
+    nei_n[fperm[f]] = cperm[nei[f]];
+    if (f >= nbf) fslot_n[fperm[f]] = nzperm[fslot[f]];
+  }
+  for (auto& c : bc_cells) c = cperm[c];
+  {
+    std::vector<double> t(n);
+    for (int64_t c = 0; c < n; ++c) t[cperm[c]] = vol[c];
+    vol.swap(t);
+  }
+  cf_ptr.swap(cf_ptr_n);
+  cf_face.swap(cf_face_n);
+  cf_nbr.swap(cf_nbr_n);
+  cf_slot.swap(cf_slot_n);
+  cf_sign.swap(cf_sign_n);
+  own.swap(own_n);
+  nei.swap(nei_n);
+  fslot.swap(fslot_n);
+  diag.swap(diag_n);
+  trans.swap(trans_n);
+  M.cperm = cperm;
+  M.fperm = fperm;
+  M.nzperm = nzperm;
+  // original-numbering CSR stays on the host for xcal_mesh_csr_pattern
+  M.h_rowptr = rowptr;
+  M.h_colval = colval;
+  colval.swap(colval_n);
+  M.vol.swap(vol);
+  M.area.swap(area);
+  M.weight.swap(weight);
+  M.delta.swap(delta);
+  M.nx.swap(nx);
+  M.ny.swap(ny);
+  M.nz.swap(nz);
+  M.ex.swap(ex);
+  M.ey.swap(ey);
+  M.ez.swap(ez);
+  M.sfx.swap(sfx);
+  M.sfy.swap(sfy);
+  M.sfz.swap(sfz);
+  M.lapEf.swap(lapEf);
+  M.cf_ptr.swap(cf_ptr);
+  M.cf_face.swap(cf_face);
+  M.cf_nbr.swap(cf_nbr);
+  M.cf_slot.swap(cf_slot);
+  M.cf_sign.swap(cf_sign);
+  M.diag.swap(diag);
+  M.own.swap(own);
+  M.nei.swap(nei);
+  M.fslot.swap(fslot);
+  M.bc_cells.swap(bc_cells);
+  M.bc_ptr.swap(bc_ptr);
+  M.bc_face.swap(bc_face);
+  M.bpatch.swap(bpatch);
+  M.rbeg.swap(rbeg);
+  M.rend.swap(rend);
+  M.colval.swap(colval);
+  M.trans.swap(trans);
+  M.rows_b.swap(rows_b);
+  M.posb.swap(posb);
+  M.Lcol.swap(Lcol);
+  M.Lslot.swap(Lslot);
+  M.Ucol.swap(Ucol);
+  M.Uslot.swap(Uslot);
+}
+
+// Union of independent cases.  Cells, matrix slots and packed factors are
+// concatenated case by case (each case keeps its own device numbering);
+// faces are numbered boundary-first over all cases (every case's boundary
+// faces, then every case's internal faces), as the face kernels require.
+// The reference numbering of the union is the cases' reference numberings
+// concatenated.  All cases must have the same patches.
+static void union_cases(std::vector<HostMesh>& C, HostMesh& U, std::vector<int64_t>& case_cell,
+                        std::vector<int64_t>& case_bface, std::vector<int64_t>& case_iface,
+                        std::vector<int32_t>& case_lev_f, std::vector<int32_t>& case_lev_b) {
+  const size_t K = C.size();
+  if (K == 1) {
+    U = std::move(C[0]);
+    case_cell = {0, U.n};
+    case_bface = {0, U.nbf};
+    case_iface = {U.nbf, U.nf};
+    case_lev_f = {0, U.nlev_f};
+    case_lev_b = {0, U.nlev_b};
+    return;
+  }
+  int64_t N = 0, NF = 0, NBF = 0, NNZ = 0, NZ = 0, NCF = 0, NBC = 0, NL = 0, NU = 0;
+  for (const HostMesh& M : C) {
+    XCAL_CHECK(M.dim == C[0].dim, "all cases of a batch must have the same dimension");
+    XCAL_CHECK(M.n_patches == C[0].n_patches, "all cases of a batch must have the same patches");
+    N += M.n; NF += M.nf; NBF += M.nbf; NNZ += M.nnz; NZ += M.nzlen; NCF += M.ncf;
+    NBC += M.nbcell; NL += (int64_t)M.Lcol.size(); NU += (int64_t)M.Ucol.size();
+  }
+  XCAL_CHECK(N < (1ll << 30) && NF < (1ll << 30) && NZ < (1ll << 31) && NL < (1ll << 31) &&
+                 NU < (1ll << 31),
+             "batch too large for int32 indices");
+  U = HostMesh();
+  U.dim = C[0].dim;
+  U.n_patches = C[0].n_patches;
+  U.patch_first = C[0].patch_first;
+  U.patch_last = C[0].patch_last;
+  U.n = N; U.nf = NF; U.nbf = NBF; U.nnz = NNZ; U.nzlen = NZ; U.ncf = NCF; U.nbcell = NBC;
+  for (auto* v : {&U.vol}) v->resize(N);
+  for (auto* v : {&U.area, &U.weight, &U.delta, &U.nx, &U.ny, &U.nz, &U.ex, &U.ey, &U.ez, &U.sfx,
+                  &U.sfy, &U.sfz, &U.lapEf})
+    v->resize(NF);
+  U.own.resize(NF);
+  U.nei.resize(NF);
+  U.fslot.assign(NF, -1);
+  U.bpatch.assign(NBF, -1);
+  U.cperm.resize(N);
+  U.fperm.resize(NF);
+  U.nzperm.resize(NNZ);
+  U.cf_ptr.assign(1, 0);
+  U.bc_ptr.assign(1, 0);
+  U.h_rowptr.assign(1, 0);
+  case_cell.assign(1, 0);
+  case_bface.assign(1, 0);
+  case_iface.assign(1, NBF);
+  case_lev_f.assign(1, 0);
+  case_lev_b.assign(1, 0);
+  int64_t co = 0, bo = 0, io = 0, zo = 0, ro = 0, Lo = 0, Uo = 0, fo = 0;
+  for (size_t k = 0; k < K; ++k) {
+    HostMesh& M = C[k];
+    auto face = [&](int32_t f) -> int32_t {  // case face (device) -> union face
+      return f < M.nbf ? (int32_t)(bo + f) : (int32_t)(NBF + io + (f - M.nbf));
+    };
+    auto cell = [&](int32_t c) -> int32_t { return (int32_t)(co + c); };
+    auto slot = [&](int32_t z) -> int32_t { return z < 0 ? -1 : (int32_t)(zo + z); };
+    for (int64_t c = 0; c < M.n; ++c) {
+      U.vol[co + c] = M.vol[c];
+      U.cperm[co + c] = cell(M.cperm[c]);
+      U.diag.push_back(slot(M.diag[c]));
+      U.rbeg.push_back(slot(M.rbeg[c]));
+      U.rend.push_back(slot(M.rend[c]));
+      U.rows_b.push_back(cell(M.rows_b[c]));
+      U.posb.push_back(cell(M.posb[c]));
+      for (int32_t j = M.cf_ptr[c]; j < M.cf_ptr[c + 1]; ++j) {
+        U.cf_face.push_back(face(M.cf_face[j]));
+        U.cf_nbr.push_back(cell(M.cf_nbr[j]));
+        U.cf_slot.push_back(slot(M.cf_slot[j]));
+        U.cf_sign.push_back(M.cf_sign[j]);
+      }
+      U.cf_ptr.push_back((int32_t)U.cf_face.size());
+    }
+    for (int64_t f = 0; f < M.nf; ++f) {
+      const int32_t g = face((int32_t)f);
+      U.area[g] = M.area[f]; U.weight[g] = M.weight[f]; U.delta[g] = M.delta[f];
+      U.nx[g] = M.nx[f]; U.ny[g] = M.ny[f]; U.nz[g] = M.nz[f];
+      U.ex[g] = M.ex[f]; U.ey[g] = M.ey[f]; U.ez[g] = M.ez[f];
+      U.sfx[g] = M.sfx[f]; U.sfy[g] = M.sfy[f]; U.sfz[g] = M.sfz[f]; U.lapEf[g] = M.lapEf[f];
+      U.own[g] = cell(M.own[f]);
+      U.nei[g] = cell(M.nei[f]);
+      U.fslot[g] = slot(M.fslot[f]);
+      if (f < M.nbf) U.bpatch[g] = M.bpatch[f];
+      U.fperm[fo + f] = face(M.fperm[f]);  // reference face of the union -> device face
+    }
+    for (int64_t s = 0; s < M.nbcell; ++s) {
+      U.bc_cells.push_back(cell(M.bc_cells[s]));
+      for (int32_t j = M.bc_ptr[s]; j < M.bc_ptr[s + 1]; ++j) U.bc_face.push_back(face(M.bc_face[j]));
+      U.bc_ptr.push_back((int32_t)U.bc_face.size());
+    }
+    for (int64_t z = 0; z < M.nzlen; ++z) {
+      U.colval.push_back(cell(M.colval[z]));
+      U.trans.push_back(slot(M.trans[z]));
+    }
+    for (int64_t z = 0; z < M.nnz; ++z) U.nzperm[ro + z] = slot(M.nzperm[z]);
+    for (int64_t c = 0; c < M.n; ++c) U.h_rowptr.push_back((int32_t)(ro + M.h_rowptr[c + 1]));
+    for (int32_t v : M.h_colval) U.h_colval.push_back(cell(v));
+    for (size_t e = 0; e < M.Lcol.size(); ++e) {
+      U.Lcol.push_back(M.Lcol[e] < 0 ? -1 : cell(M.Lcol[e]));
+      U.Lslot.push_back(slot(M.Lslot[e]));
+    }
+    for (size_t e = 0; e < M.Ucol.size(); ++e) {
+      U.Ucol.push_back(M.Ucol[e] < 0 ? -1 : cell(M.Ucol[e]));
+      U.Uslot.push_back(slot(M.Uslot[e]));
+    }
+    for (int l = 0; l < M.nlev_f; ++l) {
+      U.lev_ptr_f.push_back(cell(M.lev_ptr_f[l]));
+      U.lmeta_f.push_back((int32_t)(Lo + M.lmeta_f[2 * l]));
+      U.lmeta_f.push_back(M.lmeta_f[2 * l + 1]);
+    }
+    for (int l = 0; l < M.nlev_b; ++l) {
+      U.lev_ptr_b.push_back(cell(M.lev_ptr_b[l]));
+      U.lmeta_b.push_back((int32_t)(Uo + M.lmeta_b[2 * l]));
+      U.lmeta_b.push_back(M.lmeta_b[2 * l + 1]);
+    }
+    U.nlev_f += M.nlev_f;
+    U.nlev_b += M.nlev_b;
+    co += M.n; bo += M.nbf; io += M.nf - M.nbf; zo += M.nzlen; ro += M.nnz; fo += M.nf;
+    Lo += (int64_t)M.Lcol.size(); Uo += (int64_t)M.Ucol.size();
+    case_cell.push_back(co);
+    case_bface.push_back(bo);
+    case_iface.push_back(NBF + io);
+    case_lev_f.push_back(U.nlev_f);
+    case_lev_b.push_back(U.nlev_b);
+    HostMesh().cperm.swap(M.cperm);  // release the case's memory early
+    M = HostMesh();
+  }
+  U.lev_ptr_f.push_back((int32_t)N);
+  U.lev_ptr_b.push_back((int32_t)N);
+}
+
+static xcal_mesh* upload_mesh(xcal_ctx* ctx, HostMesh& M) {
+  XCAL_HIP(hipSetDevice(ctx->device));
+  auto m = std::make_unique<xcal_mesh>();
+  m->ctx = ctx;
+  m->dim = M.dim;
+  m->n = M.n;
+  m->nf = M.nf;
+  m->nbf = M.nbf;
+  m->nnz = M.nnz;
+  m->ncf = M.ncf;
+  m->nzlen = M.nzlen;
+  m->nbcell = M.nbcell;
+  m->n_patches = M.n_patches;
+  m->patch_first = M.patch_first;
+  m->patch_last = M.patch_last;
+  m->nlev_f = M.nlev_f;
+  m->nlev_b = M.nlev_b;
+  m->lev_ptr_f = M.lev_ptr_f;
+  m->lev_ptr_b = M.lev_ptr_b;
+  m->lmeta_f = M.lmeta_f;
+  m->lmeta_b = M.lmeta_b;
+  m->cperm = M.cperm;
+  m->fperm = M.fperm;
+  m->nzperm = M.nzperm;
+  m->h_rowptr = M.h_rowptr;
+  m->h_colval = M.h_colval;
+  int64_t bytes = 0;
+  upload(m->vol, M.vol, bytes);
+  upload(m->cf_ptr, M.cf_ptr, bytes);
+  upload(m->cf_face, M.cf_face, bytes);
+  upload(m->cf_nbr, M.cf_nbr, bytes);
+  upload(m->cf_slot, M.cf_slot, bytes);
+  upload(m->cf_sign, M.cf_sign, bytes);
+  upload(m->diag, M.diag, bytes);
+  upload(m->own, M.own, bytes);
+  upload(m->nei, M.nei, bytes);
+  upload(m->fslot, M.fslot, bytes);
+  upload(m->area, M.area, bytes);
+  upload(m->weight, M.weight, bytes);
+  upload(m->delta, M.delta, bytes);
+  upload(m->nx, M.nx, bytes);
+  upload(m->ny, M.ny, bytes);
+  upload(m->nz, M.nz, bytes);
+  upload(m->ex, M.ex, bytes);
+  upload(m->ey, M.ey, bytes);
+  upload(m->ez, M.ez, bytes);
+  upload(m->sfx, M.sfx, bytes);
+  upload(m->sfy, M.sfy, bytes);
+  upload(m->sfz, M.sfz, bytes);
+  upload(m->lapEf, M.lapEf, bytes);
+  upload(m->bc_cells, M.bc_cells, bytes);
+  upload(m->bc_ptr, M.bc_ptr, bytes);
+  upload(m->bc_face, M.bc_face, bytes);
+  upload(m->bface_patch, M.bpatch, bytes);
+  upload(m->rbeg, M.rbeg, bytes);
+  upload(m->rend, M.rend, bytes);
+  upload(m->colval, M.colval, bytes);
+  upload(m->trans, M.trans, bytes);
+  upload(m->lev_rows_b, M.rows_b, bytes);
+  upload(m->posb, M.posb, bytes);
+  upload(m->Lcol, M.Lcol, bytes);
+  upload(m->Lslot, M.Lslot, bytes);
+  upload(m->Ucol, M.Ucol, bytes);
+  upload(m->Uslot, M.Uslot, bytes);
+  upload(m->lmeta_f_d, M.lmeta_f, bytes);
+  upload(m->lmeta_b_d, M.lmeta_b, bytes);
+  m->device_bytes = bytes;
+  return m.release();
+}
+
+static xcal_mesh* create_mesh(xcal_ctx* ctx, const xcal_mesh_host* const* hosts, int ncase,
+                              bool batch) {
+  XCAL_CHECK(ctx && hosts && ncase >= 1, "NULL argument or empty batch");
+  std::vector<HostMesh> C(ncase);
+  for (int k = 0; k < ncase; ++k) {
+    XCAL_CHECK(hosts[k], "NULL mesh in batch");
+    build_host_mesh(hosts[k], C[k]);
+  }
+  HostMesh U;
+  std::vector<int64_t> cc, cb, ci;
+  std::vector<int32_t> lf, lb;
+  union_cases(C, U, cc, cb, ci, lf, lb);
+  xcal_mesh* m = upload_mesh(ctx, U);
+  std::unique_ptr<xcal_mesh> guard(m);
+  m->ncase = ncase;
+  m->batch = batch;
+  m->case_cell = cc;
+  m->case_bface = cb;
+  m->case_iface = ci;
+  m->case_lev_f = lf;
+  m->case_lev_b = lb;
+  // workgroup table: blocks of kBlock rows that never straddle two cases
+  std::vector<int32_t> b0, b1, bk, cblk(1, 0), iota(ncase + 1), fcase(m->nf, 0);
+  for (int k = 0; k < ncase; ++k) {
+    for (int64_t r = cc[k]; r < cc[k + 1]; r += kBlock) {
+      b0.push_back((int32_t)r);
+      b1.push_back((int32_t)std::min<int64_t>(r + kBlock, cc[k + 1]));
+      bk.push_back(k);
+    }
+    cblk.push_back((int32_t)b0.size());
+    for (int64_t f = cb[k]; f < cb[k + 1]; ++f) fcase[f] = k;
+    for (int64_t f = ci[k]; f < ci[k + 1]; ++f) fcase[f] = k;
+  }
+  for (int k = 0; k <= ncase; ++k) iota[k] = k;
+  m->nblk = (int64_t)b0.size();
+  int64_t bytes = 0;
+  upload(m->case_cell_d, cc, bytes);
+  upload(m->case_lev_f_d, lf, bytes);
+  upload(m->case_lev_b_d, lb, bytes);
+  upload(m->blk_row0, b0, bytes);
+  upload(m->blk_row1, b1, bytes);
+  upload(m->blk_case, bk, bytes);
+  upload(m->case_blk_d, cblk, bytes);
+  upload(m->case_iota_d, iota, bytes);
+  if (batch) upload(m->face_case, fcase, bytes);
+  m->device_bytes += bytes;
+  make_sweep_plans(m, 0);
+  return guard.release();
+}
+
 }  // namespace xcal
 
 using namespace xcal;
@@ -175,337 +823,25 @@ int xcal_ctx_synchronize(xcal_ctx* c) {
 int xcal_mesh_create(xcal_ctx* ctx, const xcal_mesh_host* h, xcal_mesh** out) {
   return guarded([&] {
     XCAL_CHECK(ctx && h && out, "NULL argument");
-    XCAL_CHECK(h->dim == 2 || h->dim == 3, "dim must be 2 or 3");
-    XCAL_CHECK(h->n_cells > 0 && h->n_faces > 0, "empty mesh");
-    XCAL_CHECK(h->n_cells < (1ll << 30) && h->n_faces < (1ll << 30), "mesh too large for int32 indices");
-    XCAL_CHECK(h->n_bfaces >= 0 && h->n_bfaces <= h->n_faces, "bad n_bfaces");
-    XCAL_HIP(hipSetDevice(ctx->device));
-    auto m = std::make_unique<xcal_mesh>();
-    m->ctx = ctx;
-    m->dim = h->dim;
-    const int64_t n = h->n_cells, nf = h->n_faces, nbf = h->n_bfaces;
-    m->n = n;
-    m->nf = nf;
-    m->nbf = nbf;
-    m->n_patches = h->n_patches;
-    XCAL_CHECK(h->n_patches <= kMaxPatches, "mesh has more than 32 patches");
-    for (int p = 0; p < h->n_patches; ++p) {
-      int64_t a = h->patch_range[2 * p] - 1, b = h->patch_range[2 * p + 1] - 1;
-      XCAL_CHECK(a >= 0 && b < nbf && a <= b + 1, "patch range outside boundary faces");
-      m->patch_first.push_back(a);
-      m->patch_last.push_back(b);
-    }
-    int64_t bytes = 0;
-    // cell -> internal face lists in the reference's faces_range order
-    std::vector<int32_t> cf_ptr(n + 1, 0), cf_face, cf_nbr;
-    std::vector<int8_t> cf_sign;
-    cf_face.reserve(h->n_cell_faces);
-    cf_nbr.reserve(h->n_cell_faces);
-    cf_sign.reserve(h->n_cell_faces);
-    for (int64_t c = 0; c < n; ++c) {
-      int64_t a = h->cell_faces_range[2 * c], b = h->cell_faces_range[2 * c + 1];
-      for (int64_t k = a; k <= b; ++k) {
-        XCAL_CHECK(k >= 1 && k <= h->n_cell_faces, "faces_range outside cell_faces");
-        int64_t f = h->cell_faces[k - 1] - 1, nb = h->cell_neighbours[k - 1] - 1;
-        XCAL_CHECK(f >= nbf && f < nf, "cell face is not an internal face");
-        XCAL_CHECK(nb >= 0 && nb < n, "neighbour index out of range");
-        XCAL_CHECK(h->cell_nsign[k - 1] == 1 || h->cell_nsign[k - 1] == -1, "nsign must be +-1");
-        cf_face.push_back((int32_t)f);
-        cf_nbr.push_back((int32_t)nb);
-        cf_sign.push_back((int8_t)h->cell_nsign[k - 1]);
-      }
-      cf_ptr[c + 1] = (int32_t)cf_face.size();
-    }
-    m->ncf = (int64_t)cf_face.size();
-    // CSR pattern: row i = {i} U neighbours(i), ascending, unique
-    // (sparse_matrix_connectivity + sparsecsr, ModelFramework_0_types.jl:223-242)
-    std::vector<int32_t> rowptr(n + 1, 0), colval;
-    colval.reserve(n + cf_face.size());
-    std::vector<int32_t> row;
-    for (int64_t c = 0; c < n; ++c) {
-      row.clear();
-      row.push_back((int32_t)c);
-      for (int32_t k = cf_ptr[c]; k < cf_ptr[c + 1]; ++k) row.push_back(cf_nbr[k]);
-      std::sort(row.begin(), row.end());
-      row.erase(std::unique(row.begin(), row.end()), row.end());
-      colval.insert(colval.end(), row.begin(), row.end());
-      rowptr[c + 1] = (int32_t)colval.size();
-    }
-    const int64_t nnz = (int64_t)colval.size();
-    m->nnz = nnz;
-    auto find = [&](int64_t i, int32_t j) -> int32_t {
-      const int32_t* b = colval.data() + rowptr[i];
-      const int32_t* e = colval.data() + rowptr[i + 1];
-      const int32_t* it = std::lower_bound(b, e, j);
-      return (it != e && *it == j) ? (int32_t)(it - colval.data()) : -1;
-    };
-    std::vector<int32_t> diag(n), cf_slot(cf_face.size()), trans(nnz);
-    for (int64_t c = 0; c < n; ++c) {
-      diag[c] = find(c, (int32_t)c);
-      for (int32_t k = cf_ptr[c]; k < cf_ptr[c + 1]; ++k) cf_slot[k] = find(c, cf_nbr[k]);
-      for (int32_t k = rowptr[c]; k < rowptr[c + 1]; ++k) trans[k] = find(colval[k], (int32_t)c);
-    }
-    // faces
-    std::vector<int32_t> own(nf), nei(nf);
-    std::vector<double> area(h->face_area, h->face_area + nf), weight(h->face_weight, h->face_weight + nf),
-        delta(h->face_delta, h->face_delta + nf), nx(nf), ny(nf), nz(nf), ex(nf), ey(nf), ez(nf),
-        sfx(nf), sfy(nf), sfz(nf), lapEf(nf);
-    for (int64_t f = 0; f < nf; ++f) {
-      own[f] = (int32_t)(h->face_owner[2 * f] - 1);
-      nei[f] = (int32_t)(h->face_owner[2 * f + 1] - 1);
-      XCAL_CHECK(own[f] >= 0 && own[f] < n && nei[f] >= 0 && nei[f] < n, "ownerCells out of range");
-      nx[f] = h->face_normal[3 * f];
-      ny[f] = h->face_normal[3 * f + 1];
-      nz[f] = h->face_normal[3 * f + 2];
-      ex[f] = h->face_e[3 * f];
-      ey[f] = h->face_e[3 * f + 1];
-      ez[f] = h->face_e[3 * f + 2];
-      // Sf = area*normal (flux!, green_gauss!), same rounding as the reference
-      sfx[f] = area[f] * nx[f];
-      sfy[f] = area[f] * ny[f];
-      sfz[f] = area[f] * nz[f];
-      // Laplacian{Linear} |E_f| (Discretise_1_schemes.jl:74-88); independent
-      // of the visiting side's nsign because every negation there is exact
-      double s0 = area[f] * nx[f], s1 = area[f] * ny[f], s2 = area[f] * nz[f];
-      double cfac = (s0 * s0 + s1 * s1 + s2 * s2) / (s0 * ex[f] + s1 * ey[f] + s2 * ez[f]);
-      double e0 = cfac * ex[f], e1 = cfac * ey[f], e2 = cfac * ez[f];
-      lapEf[f] = std::sqrt(e0 * e0 + e1 * e1 + e2 * e2);
-    }
-    std::vector<int32_t> fslot(nf, -1);
-    for (int64_t f = nbf; f < nf; ++f) fslot[f] = find(own[f], nei[f]);
-    // boundary faces grouped per cell, ascending face id
-    std::vector<int8_t> bpatch(nbf, -1);
-    for (int p = 0; p < h->n_patches; ++p)
-      for (int64_t f = m->patch_first[p]; f <= m->patch_last[p]; ++f)
-        if (bpatch[f] < 0) bpatch[f] = (int8_t)p;
-    std::vector<int32_t> bcount(n, 0);
-    for (int64_t f = 0; f < nbf; ++f) {
-      int64_t c = h->boundary_cellsID[f] - 1;
-      XCAL_CHECK(c >= 0 && c < n, "boundary_cellsID out of range");
-      bcount[c]++;
-    }
-    std::vector<int32_t> bc_cells, bc_ptr(1, 0), bc_face;
-    std::vector<int32_t> bstart(n, -1);
-    for (int64_t c = 0; c < n; ++c)
-      if (bcount[c] > 0) {
-        bstart[c] = (int32_t)bc_cells.size();
-        bc_cells.push_back((int32_t)c);
-        bc_ptr.push_back(bc_ptr.back() + bcount[c]);
-      }
-    bc_face.assign(nbf, 0);
-    {
-      std::vector<int32_t> cur(bc_cells.size(), 0);
-      for (int64_t f = 0; f < nbf; ++f) {
-        int64_t c = h->boundary_cellsID[f] - 1;
-        int32_t s = bstart[c];
-        bc_face[bc_ptr[s] + cur[s]++] = (int32_t)f;
-      }
-    }
-    m->nbcell = (int64_t)bc_cells.size();
-    // DILU level schedules: forward level(i) = 1 + max level of lower
-    // neighbours; backward symmetric over upper neighbours.  Rows inside a
-    // level are independent; level order preserves every i<j dependency.
-    std::vector<int32_t> flev(n, 0), blev(n, 0);
-    int maxl = 0;
-    for (int64_t i = 0; i < n; ++i) {
-      int l = 0;
-      for (int32_t k = rowptr[i]; k < diag[i]; ++k) l = std::max(l, flev[colval[k]] + 1);
-      flev[i] = l;
-      maxl = std::max(maxl, l);
-    }
-    m->nlev_f = maxl + 1;
-    maxl = 0;
-    for (int64_t i = n - 1; i >= 0; --i) {
-      int l = 0;
-      for (int32_t k = diag[i] + 1; k < rowptr[i + 1]; ++k) l = std::max(l, blev[colval[k]] + 1);
-      blev[i] = l;
-      maxl = std::max(maxl, l);
-    }
-    m->nlev_b = maxl + 1;
-    std::vector<double> vol(h->cell_volume, h->cell_volume + n);
+    *out = create_mesh(ctx, &h, 1, false);
+    return 0;
+  });
+}
 
-    // ---- device numbering ------------------------------------------------
-    // Cells are stored level-major: by forward DILU level, and in Morton
-    // order of their centres inside a level.  Each forward level is then one
-    // contiguous row range, and rows that are neighbours in a level are
-    // neighbours in space, so neighbour gathers hit nearby cache lines.
-    // Internal faces follow their renumbered cells.  Every matrix row keeps
-    // its entries in the reference's (original) column order, so row sums,
-    // the DILU lower/upper split and all per-row summation orders are
-    // unchanged; fields are permuted only when they cross the C ABI.
-    std::vector<int32_t> cperm(n), cinv(n), fperm(nf), finv(nf);
-    {
-      double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-      for (int64_t c = 0; c < n; ++c)
-        for (int a = 0; a < 3; ++a) {
-          lo[a] = std::min(lo[a], h->cell_centre[3 * c + a]);
-          hi[a] = std::max(hi[a], h->cell_centre[3 * c + a]);
-        }
-      const int naxis = h->dim == 3 ? 3 : 2;
-      const int bits = naxis == 3 ? 21 : 31;
-      auto spread = [&](uint64_t v) {  // interleave with naxis-1 zero bits
-        uint64_t r = 0;
-        for (int b = 0; b < bits; ++b) r |= ((v >> b) & 1ull) << (naxis * b);
-        return r;
-      };
-      std::vector<std::pair<std::pair<int32_t, uint64_t>, int32_t>> key(n);
-      for (int64_t c = 0; c < n; ++c) {
-        uint64_t k = 0;
-        for (int a = 0; a < naxis; ++a) {
-          double span = hi[a] - lo[a];
-          double t = span > 0 ? (h->cell_centre[3 * c + a] - lo[a]) / span : 0.0;
-          uint64_t q = (uint64_t)std::min(t * (double)((1ull << bits) - 1), (double)((1ull << bits) - 1));
-          k |= spread(q) << a;
-        }
-        key[c] = {{flev[c], k}, (int32_t)c};
-      }
-      std::sort(key.begin(), key.end());
-      for (int64_t c = 0; c < n; ++c) {
-        cinv[c] = key[c].second;
-        cperm[key[c].second] = (int32_t)c;
-      }
-      for (int64_t f = 0; f < nbf; ++f) fperm[f] = finv[f] = (int32_t)f;  // boundary faces keep ids
-      std::vector<std::pair<std::pair<int32_t, int32_t>, int32_t>> fk;
-      fk.reserve(nf - nbf);
-      for (int64_t f = nbf; f < nf; ++f) {
-        int32_t a = cperm[own[f]], b = cperm[nei[f]];
-        fk.push_back({{std::min(a, b), std::max(a, b)}, (int32_t)f});
-      }
-      std::sort(fk.begin(), fk.end());
-      for (int64_t k = 0; k < nf - nbf; ++k) {
-        finv[nbf + k] = fk[k].second;
-        fperm[fk[k].second] = (int32_t)(nbf + k);
-      }
-    }
-    // Matrix storage: sliced ELL with slices of kSlice consecutive rows, each
-    // slice column-major and as wide as its longest row.  Entry t of device
-    // row c sits at rbeg[c] + t*kSlice, so a wavefront reading entry t of 64
-    // rows reads one contiguous segment.  Padding entries are never read.
-    const int64_t nslice = (n + kSlice - 1) / kSlice;
-    std::vector<int32_t> rbeg(n), rend(n);
-    int64_t nzlen = 0;
-    for (int64_t s = 0; s < nslice; ++s) {
-      int32_t w = 0;
-      const int64_t c1 = std::min<int64_t>(n, (s + 1) * kSlice);
-      for (int64_t c = s * kSlice; c < c1; ++c) w = std::max(w, rowptr[cinv[c] + 1] - rowptr[cinv[c]]);
-      for (int64_t c = s * kSlice; c < c1; ++c) {
-        rbeg[c] = (int32_t)(nzlen + (c - s * kSlice));
-        rend[c] = rbeg[c] + kSlice * (rowptr[cinv[c] + 1] - rowptr[cinv[c]]);
-      }
-      nzlen += (int64_t)w * kSlice;
-      XCAL_CHECK(nzlen < (1ll << 31), "matrix storage exceeds int32 indices");
-    }
-    m->nzlen = nzlen;
-    std::vector<int32_t> colval_n(nzlen), nzperm(nnz), diag_n(n), trans_n(nzlen, -1);
-    for (int64_t oc = 0; oc < n; ++oc)
-      for (int32_t k = rowptr[oc]; k < rowptr[oc + 1]; ++k)
-        nzperm[k] = rbeg[cperm[oc]] + kSlice * (k - rowptr[oc]);
-    for (int64_t k = 0; k < nnz; ++k) colval_n[nzperm[k]] = cperm[colval[k]];
-    for (int64_t k = 0; k < nnz; ++k) trans_n[nzperm[k]] = trans[k] >= 0 ? nzperm[trans[k]] : -1;
-    for (int64_t oc = 0; oc < n; ++oc) diag_n[cperm[oc]] = nzperm[diag[oc]];
-    // level schedules in device numbering: forward levels are the contiguous
-    // ranges of the level-major order; backward levels list their rows
-    std::vector<int32_t> rows_b;
-    {
-      m->lev_ptr_f.assign(m->nlev_f + 1, 0);
-      for (int64_t i = 0; i < n; ++i) m->lev_ptr_f[flev[i] + 1]++;
-      for (int l = 0; l < m->nlev_f; ++l) m->lev_ptr_f[l + 1] += m->lev_ptr_f[l];
-      for (int64_t c = 0; c < n; ++c)
-        XCAL_CHECK(c >= m->lev_ptr_f[flev[cinv[c]]] && c < m->lev_ptr_f[flev[cinv[c]] + 1],
-                   "internal: level-major numbering");
-      m->lev_ptr_b.assign(m->nlev_b + 1, 0);
-      for (int64_t i = 0; i < n; ++i) m->lev_ptr_b[blev[i] + 1]++;
-      for (int l = 0; l < m->nlev_b; ++l) m->lev_ptr_b[l + 1] += m->lev_ptr_b[l];
-      rows_b.assign(n, 0);
-      std::vector<int32_t> cur(m->lev_ptr_b.begin(), m->lev_ptr_b.end() - 1);
-      for (int64_t c = 0; c < n; ++c) rows_b[cur[blev[cinv[c]]]++] = (int32_t)c;
-    }
-    std::vector<int32_t> cf_ptr_n(n + 1, 0), cf_face_n(cf_face.size()), cf_nbr_n(cf_face.size()),
-        cf_slot_n(cf_face.size());
-    std::vector<int8_t> cf_sign_n(cf_face.size());
-    for (int64_t c = 0; c < n; ++c) {
-      const int32_t oc = cinv[c];
-      int32_t w = cf_ptr_n[c];
-      for (int32_t k = cf_ptr[oc]; k < cf_ptr[oc + 1]; ++k, ++w) {
-        cf_face_n[w] = fperm[cf_face[k]];
-        cf_nbr_n[w] = cperm[cf_nbr[k]];
-        cf_slot_n[w] = nzperm[cf_slot[k]];
-        cf_sign_n[w] = cf_sign[k];
-      }
-      cf_ptr_n[c + 1] = w;
-    }
-    auto permf = [&](std::vector<double>& a) {
-      std::vector<double> t(nf);
-      for (int64_t f = 0; f < nf; ++f) t[fperm[f]] = a[f];
-      a.swap(t);
-    };
-    for (auto* a : {&area, &weight, &delta, &nx, &ny, &nz, &ex, &ey, &ez, &sfx, &sfy, &sfz, &lapEf})
-      permf(*a);
-    std::vector<int32_t> own_n(nf), nei_n(nf), fslot_n(nf, -1);
-    for (int64_t f = 0; f < nf; ++f) {
-      own_n[fperm[f]] = cperm[own[f]];
-      nei_n[fperm[f]] = cperm[nei[f]];
-      if (f >= nbf) fslot_n[fperm[f]] = nzperm[fslot[f]];
-    }
-    for (auto& c : bc_cells) c = cperm[c];
-    {
-      std::vector<double> t(n);
-      for (int64_t c = 0; c < n; ++c) t[cperm[c]] = vol[c];
-      vol.swap(t);
-    }
-    cf_ptr.swap(cf_ptr_n);
-    cf_face.swap(cf_face_n);
-    cf_nbr.swap(cf_nbr_n);
-    cf_slot.swap(cf_slot_n);
-    cf_sign.swap(cf_sign_n);
-    own.swap(own_n);
-    nei.swap(nei_n);
-    fslot.swap(fslot_n);
-    diag.swap(diag_n);
-    trans.swap(trans_n);
-    m->cperm = cperm;
-    m->fperm = fperm;
-    m->nzperm = nzperm;
-    // original-numbering CSR stays on the host for xcal_mesh_csr_pattern
-    m->h_rowptr = rowptr;
-    m->h_colval = colval;
-    colval.swap(colval_n);
+int xcal_mesh_create_batch(xcal_ctx* ctx, const xcal_mesh_host* const* hosts, int32_t ncase,
+                           xcal_mesh** out) {
+  return guarded([&] {
+    XCAL_CHECK(ctx && hosts && out, "NULL argument");
+    *out = create_mesh(ctx, hosts, ncase, true);
+    return 0;
+  });
+}
 
-    upload(m->vol, vol, bytes);
-    upload(m->cf_ptr, cf_ptr, bytes);
-    upload(m->cf_face, cf_face, bytes);
-    upload(m->cf_nbr, cf_nbr, bytes);
-    upload(m->cf_slot, cf_slot, bytes);
-    upload(m->cf_sign, cf_sign, bytes);
-    upload(m->diag, diag, bytes);
-    upload(m->own, own, bytes);
-    upload(m->nei, nei, bytes);
-    upload(m->fslot, fslot, bytes);
-    upload(m->area, area, bytes);
-    upload(m->weight, weight, bytes);
-    upload(m->delta, delta, bytes);
-    upload(m->nx, nx, bytes);
-    upload(m->ny, ny, bytes);
-    upload(m->nz, nz, bytes);
-    upload(m->ex, ex, bytes);
-    upload(m->ey, ey, bytes);
-    upload(m->ez, ez, bytes);
-    upload(m->sfx, sfx, bytes);
-    upload(m->sfy, sfy, bytes);
-    upload(m->sfz, sfz, bytes);
-    upload(m->lapEf, lapEf, bytes);
-    upload(m->bc_cells, bc_cells, bytes);
-    upload(m->bc_ptr, bc_ptr, bytes);
-    upload(m->bc_face, bc_face, bytes);
-    upload(m->bface_patch, bpatch, bytes);
-    upload(m->rbeg, rbeg, bytes);
-    upload(m->rend, rend, bytes);
-    upload(m->colval, colval, bytes);
-    upload(m->trans, trans, bytes);
-    upload(m->lev_rows_b, rows_b, bytes);
-    m->device_bytes = bytes;
-    make_sweep_plans(m.get(), 0);
-    *out = m.release();
+int xcal_mesh_cases(const xcal_mesh* m, int32_t* ncase, int64_t* cell_offsets) {
+  return guarded([&] {
+    XCAL_CHECK(m, "mesh is NULL");
+    if (ncase) *ncase = m->ncase;
+    if (cell_offsets) std::copy(m->case_cell.begin(), m->case_cell.end(), cell_offsets);
     return 0;
   });
 }

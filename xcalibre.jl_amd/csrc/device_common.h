@@ -72,6 +72,37 @@ inline MeshK meshk(const xcal_mesh* m) {
   return k;
 }
 
+// sum over row i of nz[k] * xv(colval[k]) in the row's column order.  All of
+// a row's column indices and values are loaded before any of the gathers
+// they feed, so a row costs three dependent memory round trips instead of
+// two per entry.
+constexpr int kRowAhead = 8;
+template <class T, class XV>
+__device__ inline T row_sum(const MatKT<T>& A, int64_t i, XV&& xv) {
+  const int32_t k1 = A.re[i];
+  T acc = 0;
+  for (int32_t k0 = A.rb[i]; k0 < k1; k0 += kRowAhead * kSlice) {
+    int col[kRowAhead];
+    T a[kRowAhead], x[kRowAhead];
+#pragma unroll
+    for (int u = 0; u < kRowAhead; ++u) {
+      const int32_t k = k0 + u * kSlice;
+      col[u] = -1;
+      a[u] = 0;
+      if (k < k1) {
+        col[u] = A.colval[k];
+        a[u] = A.nz[k];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kRowAhead; ++u) x[u] = col[u] >= 0 ? xv(col[u]) : T(0);
+#pragma unroll
+    for (int u = 0; u < kRowAhead; ++u)
+      if (col[u] >= 0) acc += a[u] * x[u];
+  }
+  return acc;
+}
+
 // Julia's max for Float64: NaN-propagating, +0.0 preferred over -0.0
 __device__ __host__ inline double jmax(double a, double b) {
   if (a != a || b != b) return __builtin_nan("");

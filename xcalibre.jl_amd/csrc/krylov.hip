@@ -25,9 +25,7 @@ __global__ void __launch_bounds__(kBlock)
     k_spmv(MatK A, const double* __restrict__ x, double* __restrict__ y) {
   const int64_t i = xcd_row();
   if (i >= A.n) return;
-  double acc = 0.0;
-  for (int32_t k = A.rb[i]; k < A.re[i]; k += kSlice) acc += A.nz[k] * x[A.colval[k]];
-  y[i] = acc;
+  y[i] = row_sum(A, i, [&](int j) { return x[j]; });
 }
 
 // ----------------------------------------------------------- reductions
@@ -41,8 +39,7 @@ __global__ void __launch_bounds__(kBlock)
   const int64_t i = xcd_row();
   double s0 = 0.0, s1 = 0.0;
   if (i < A.n) {
-    double acc = 0.0;
-    for (int32_t k = A.rb[i]; k < A.re[i]; k += kSlice) acc += A.nz[k] * x[A.colval[k]];
+    const double acc = row_sum(A, i, [&](int j) { return x[j]; });
     const double r = b[i] - acc;
     s0 = r * r;
     s1 = b[i] * b[i];
@@ -129,6 +126,20 @@ __global__ void __launch_bounds__(kRunBlock)
     __syncthreads();
   }
 }
+// values of the packed triangular factors: out[e] = nz[slot[e]] (0 on padding)
+__global__ void __launch_bounds__(kBlock)
+    k_gather_slots(int64_t total, const int32_t* __restrict__ slot, const double* __restrict__ nz,
+                   double* __restrict__ out) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < total) out[e] = slot[e] >= 0 ? nz[slot[e]] : 0.0;
+}
+// Db[p] = D[rows_b[p]]
+__global__ void __launch_bounds__(kBlock)
+    k_gather_rows(int64_t n, const int32_t* __restrict__ rows, const double* __restrict__ D,
+                  double* __restrict__ Db) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n) Db[p] = D[rows[p]];
+}
 __global__ void __launch_bounds__(kBlock)
     k_recip(int64_t n, const double* __restrict__ D, double* __restrict__ rD) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -193,6 +204,8 @@ void precond_update(xcal_eqn* e, int precon) {
                        e->nzval.p, e->pD.p);
     launch_check();
     const MatK A = matk(m, e->nzval.p);
+    if (m->batch) precond_update_batch(e);  // one workgroup per case
+    else
     for (const SweepStep& st : m->plan_f) {
       if (st.lev1 == 1) continue;  // level 0 rows have no lower neighbours
       if (st.grid == 1)
@@ -204,6 +217,17 @@ void precond_update(xcal_eqn* e, int precon) {
       launch_check();
     }
     hipLaunchKernelGGL(k_recip, dim3(grid_for(n)), dim3(kBlock), 0, s, n, e->pD.p, e->prD.p);
+    if (e->Lnz.n != m->Lslot.n) e->Lnz.alloc(m->Lslot.n);
+    if (e->Unz.n != m->Uslot.n) e->Unz.alloc(m->Uslot.n);
+    if (e->Db.n != n) e->Db.alloc(n);
+    if (m->Lslot.n > 0)
+      hipLaunchKernelGGL(k_gather_slots, dim3(grid_for(m->Lslot.n)), dim3(kBlock), 0, s, m->Lslot.n,
+                         m->Lslot.p, e->nzval.p, e->Lnz.p);
+    if (m->Uslot.n > 0)
+      hipLaunchKernelGGL(k_gather_slots, dim3(grid_for(m->Uslot.n)), dim3(kBlock), 0, s, m->Uslot.n,
+                         m->Uslot.p, e->nzval.p, e->Unz.p);
+    hipLaunchKernelGGL(k_gather_rows, dim3(grid_for(n)), dim3(kBlock), 0, s, n, m->lev_rows_b.p,
+                       e->pD.p, e->Db.p);
     launch_check();
   } else if (precon != XCAL_PRECON_NONE) {
     throw Error(XCAL_ERR_UNSUPPORTED, "unknown preconditioner");

@@ -4,7 +4,9 @@
 // laminar viscosity model (RANS_laminar.jl), Gauss gradients and no
 // non-orthogonal correctors.  All fields stay resident in HBM; the host only
 // sequences launches and reads back residual scalars.
+#include <algorithm>
 #include <cstring>
+#include <vector>
 
 #include "device_common.h"
 
@@ -23,6 +25,10 @@ struct xcal_simple {
   hipEvent_t ev[8]{};
   double stage_ms[6]{};
   bool initialised = false;
+  // batch meshes: viscosity per case, reference cell (device id) per case
+  std::vector<double> case_nu;
+  xcal::DBuf<double> case_nu_d;
+  xcal::DBuf<int32_t> ref_cells;
   ~xcal_simple() {
     for (xcal_field* f : {U, p, Uf, pf, gradp, mdotf, rDf, nueff, divHv, Hv, rD, prev, gradU})
       if (f) xcal_field_destroy(f);
@@ -34,6 +40,34 @@ struct xcal_simple {
 };
 
 namespace {
+
+// update_nueff! (laminar): nueff = nu, per case on a batch mesh
+__global__ void k_fill_by_case(double* __restrict__ f, const int32_t* __restrict__ face_case,
+                               const double* __restrict__ nu, int64_t nf) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nf) f[i] = nu[face_case[i]];
+}
+
+// setReference! in the reference cell of every case
+__global__ void k_set_reference_cases(const int32_t* __restrict__ diag, double* nzval, double* b,
+                                      double pref, const int32_t* __restrict__ cells, int ncase) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= ncase) return;
+  const int cell = cells[k];
+  const int c = diag[cell];
+  b[cell] = nzval[c] * pref;
+  nzval[c] += nzval[c];
+}
+
+void update_nueff(xcal_simple* S) {
+  xcal_mesh* m = S->mesh;
+  if (m->batch)
+    hipLaunchKernelGGL(k_fill_by_case, dim3(grid_for(m->nf)), dim3(kBlock), 0, m->ctx->stream,
+                       S->nueff->comp(0), m->face_case.p, S->case_nu_d.p, m->nf);
+  else
+    fill(m->ctx->stream, S->nueff->comp(0), m->nf, S->setup.nu);
+  XCAL_HIP(hipGetLastError());
+}
 
 xcal_field* mk(xcal_mesh* m, int kind) {
   xcal_field* f = nullptr;
@@ -111,6 +145,17 @@ int xcal_simple_create(xcal_mesh* m, const xcal_simple_setup* s, xcal_simple** o
     rc = xcal_eqn_create(m, 0, &S->peqn);
     if (rc) throw Error(rc, xcal_last_error());
     fill(m->ctx->stream, S->rDf->comp(0), m->nf, 1.0);  // initialise!(rDf, 1.0)
+    if (m->batch) {
+      S->case_nu.assign(m->ncase, s->nu);
+      S->case_nu_d.alloc(m->ncase);
+      XCAL_HIP(hipMemcpy(S->case_nu_d.p, S->case_nu.data(), sizeof(double) * m->ncase,
+                         hipMemcpyHostToDevice));
+      std::vector<int32_t> refs(m->ncase);
+      for (int k = 0; k < m->ncase; ++k) refs[k] = m->cperm[m->case_cell[k]];  // cell 1 of case k
+      S->ref_cells.alloc(m->ncase);
+      XCAL_HIP(hipMemcpy(S->ref_cells.p, refs.data(), sizeof(int32_t) * m->ncase,
+                         hipMemcpyHostToDevice));
+    }
     for (auto& e : S->ev) XCAL_HIP(hipEventCreate(&e));
     XCAL_HIP(hipStreamSynchronize(m->ctx->stream));
     *out = S.release();
@@ -155,11 +200,24 @@ int xcal_simple_get_field(const xcal_simple* S, int32_t which, double* host) {
   });
 }
 
-int xcal_simple_run(xcal_simple* S, int32_t iterations, double* residuals, int32_t* done,
-                    int32_t* kit) {
-  return guarded([&] {
-    XCAL_CHECK(S && residuals && done && iterations >= 0, "bad arguments");
+}  // extern "C"
+
+namespace {
+
+// The SIMPLE loop.  R[(k*4 + f)*iterations + it]: residual of field f
+// (Ux, Uy, Uz, p) of case k at iteration it; KIT likewise for Krylov
+// iterations.  A batch stops when every case has converged.
+int run_impl(xcal_simple* S, int32_t iterations, std::vector<double>& R, std::vector<int32_t>& KIT) {
+  {
     xcal_mesh* m = S->mesh;
+    const int K = m->ncase;
+    R.assign((size_t)K * 4 * iterations, 0.0);
+    KIT.assign((size_t)K * 4 * iterations, 0);
+    std::vector<xcal_solve_stats> cst(K);
+    auto solve = [&](xcal_eqn* e, const xcal_solver_setup& ss, double* x, const double* b) {
+      if (m->batch) return solve_system_batch(e, &ss, x, b, cst.data());
+      return solve_system(e, &ss, x, b, cst.data());
+    };
     hipStream_t st = m->ctx->stream;
     const xcal_simple_setup& cfg = S->setup;
     const int64_t n = m->n;
@@ -190,7 +248,7 @@ int xcal_simple_run(xcal_simple* S, int32_t iterations, double* residuals, int32
       interp_vec(S, S->U);
       flux(m, S->Uf->comp(0), S->Uf->comp(1), S->Uf->comp(2), S->mdotf->comp(0));
       grad_p(S);
-      fill(st, S->nueff->comp(0), m->nf, cfg.nu);  // update_nueff! (laminar)
+      update_nueff(S);  // update_nueff! (laminar)
       S->initialised = true;
     }
     float acc[6] = {0, 0, 0, 0, 0, 0};
@@ -202,8 +260,6 @@ int xcal_simple_run(xcal_simple* S, int32_t iterations, double* residuals, int32
       mark(0);
       discretise(Ue, UT, US, S->U, 1.0);
       mark(1);
-      double res[3] = {0.0, 0.0, 0.0};
-      int itc[3] = {0, 0, 0};
       float tA = 0.0f, tS = 0.0f;
       for (int c = 0; c < (is3d ? 3 : 2); ++c) {
         mark(2);
@@ -212,11 +268,12 @@ int xcal_simple_run(xcal_simple* S, int32_t iterations, double* residuals, int32
         relax_diagdom(Ue, S->U->comp(c), Ue->bc(c), cfg.U.relax);
         if (c == 0) precond_update(Ue, cfg.U.precon);
         mark(3);
-        xcal_solve_stats ss{};
-        solve_system(Ue, &cfg.U, S->U->comp(c), Ue->bc(c), &ss);
+        solve(Ue, cfg.U, S->U->comp(c), Ue->bc(c));
         mark(4);
-        res[c] = ss.residual;
-        itc[c] = ss.niter;
+        for (int k = 0; k < K; ++k) {
+          R[((size_t)k * 4 + c) * iterations + it] = cst[k].residual;
+          KIT[((size_t)k * 4 + c) * iterations + it] = cst[k].niter;
+        }
         float a, b;
         XCAL_HIP(hipEventSynchronize(S->ev[4]));
         XCAL_HIP(hipEventElapsedTime(&a, S->ev[2], S->ev[3]));
@@ -246,12 +303,21 @@ int xcal_simple_run(xcal_simple* S, int32_t iterations, double* residuals, int32
       // solve_equation! scalar (Solve_1_api.jl:214-231)
       discretise(pe, PT, PS, S->p, 1.0);
       apply_bcs(pe, PT, S->bp, S->p, 0);
-      if (cfg.has_pref) set_reference(pe, cfg.pref, m->cperm[0]);  // reference cell 1
+      if (cfg.has_pref) {  // reference cell 1 (of every case)
+        if (m->batch)
+          hipLaunchKernelGGL(k_set_reference_cases, dim3(grid_for(K)), dim3(kBlock), 0, st,
+                             m->diag.p, pe->nzval.p, pe->bc(0), cfg.pref, S->ref_cells.p, K);
+        else
+          set_reference(pe, cfg.pref, m->cperm[0]);
+      }
       precond_update(pe, cfg.p.precon);
       mark(2);
-      xcal_solve_stats ps{};
-      solve_system(pe, &cfg.p, S->p->comp(0), pe->bc(0), &ps);
+      solve(pe, cfg.p, S->p->comp(0), pe->bc(0));
       mark(3);
+      for (int k = 0; k < K; ++k) {
+        R[((size_t)k * 4 + 3) * iterations + it] = cst[k].residual;
+        KIT[((size_t)k * 4 + 3) * iterations + it] = cst[k].niter;
+      }
       relax_explicit(m, n, S->p->comp(0), S->prev->comp(0), cfg.p.relax);
       grad_p(S);
       correct_mass_flux(pe, S->p->comp(0), S->rDf->comp(0), -1, S->bp, S->mdotf->comp(0));
@@ -261,7 +327,7 @@ int xcal_simple_run(xcal_simple* S, int32_t iterations, double* residuals, int32
       // turbulence! laminar (RANS_laminar.jl:73-79): tensor gradient of U
       interp_vec(S, S->U);
       green_gauss_tensor(m, S->Uf->comp(0), S->Uf->comp(1), S->Uf->comp(2), S->gradU->comp(0));
-      fill(st, S->nueff->comp(0), m->nf, cfg.nu);  // update_nueff!
+      update_nueff(S);  // update_nueff!
       mark(4);
       {
         float a, b, c, d;
@@ -274,19 +340,14 @@ int xcal_simple_run(xcal_simple* S, int32_t iterations, double* residuals, int32
         acc[2] += b;
         acc[3] += c;
       }
-      residuals[it] = res[0];
-      residuals[iterations + it] = res[1];
-      residuals[2 * iterations + it] = res[2];
-      residuals[3 * iterations + it] = ps.residual;
-      if (kit) {
-        kit[it] = itc[0];
-        kit[iterations + it] = itc[1];
-        kit[2 * iterations + it] = itc[2];
-        kit[3 * iterations + it] = ps.niter;
+      bool conv = true;
+      for (int k = 0; k < K; ++k) {
+        const double* r = &R[(size_t)k * 4 * iterations];
+        const bool zc = is3d ? r[2 * iterations + it] <= cfg.U.convergence : true;
+        conv = conv && r[it] <= cfg.U.convergence && r[iterations + it] <= cfg.U.convergence &&
+               zc && r[3 * iterations + it] <= cfg.p.convergence;
       }
-      const bool zc = is3d ? res[2] <= cfg.U.convergence : true;
-      if (res[0] <= cfg.U.convergence && res[1] <= cfg.U.convergence && zc &&
-          ps.residual <= cfg.p.convergence) {
+      if (conv) {
         ++it;
         break;
       }
@@ -297,10 +358,66 @@ int xcal_simple_run(xcal_simple* S, int32_t iterations, double* residuals, int32
     XCAL_HIP(hipEventElapsedTime(&tot, S->ev[6], S->ev[7]));
     for (int k = 0; k < 5; ++k) S->stage_ms[k] = acc[k];
     S->stage_ms[5] = tot;
+    return it;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int xcal_simple_run(xcal_simple* S, int32_t iterations, double* residuals, int32_t* done,
+                    int32_t* kit) {
+  return guarded([&] {
+    XCAL_CHECK(S && residuals && done && iterations >= 0, "bad arguments");
+    std::vector<double> R;
+    std::vector<int32_t> KIT;
+    const int it = run_impl(S, iterations, R, KIT);
+    // one case: R holds [Ux..., Uy..., Uz..., p...]; a batch reports the
+    // largest residual and Krylov count over its cases
+    const int K = S->mesh->ncase;
+    for (int f = 0; f < 4; ++f)
+      for (int i = 0; i < iterations; ++i) {
+        double r = 0.0;
+        int32_t q = 0;
+        for (int k = 0; k < K; ++k) {
+          r = std::max(r, R[((size_t)k * 4 + f) * iterations + i]);
+          q = std::max(q, KIT[((size_t)k * 4 + f) * iterations + i]);
+        }
+        residuals[(size_t)f * iterations + i] = r;
+        if (kit) kit[(size_t)f * iterations + i] = q;
+      }
     *done = it;
     return 0;
   });
 }
+
+int xcal_simple_run_batch(xcal_simple* S, int32_t iterations, double* residuals, int32_t* done,
+                          int32_t* kit) {
+  return guarded([&] {
+    XCAL_CHECK(S && residuals && done && iterations >= 0, "bad arguments");
+    std::vector<double> R;
+    std::vector<int32_t> KIT;
+    *done = run_impl(S, iterations, R, KIT);
+    std::copy(R.begin(), R.end(), residuals);
+    if (kit) std::copy(KIT.begin(), KIT.end(), kit);
+    return 0;
+  });
+}
+
+int xcal_simple_set_case_nu(xcal_simple* S, const double* nu) {
+  return guarded([&] {
+    XCAL_CHECK(S && nu, "NULL argument");
+    xcal_mesh* m = S->mesh;
+    XCAL_CHECK(m->batch, "per-case viscosity applies to batch meshes");
+    for (int k = 0; k < m->ncase; ++k) XCAL_CHECK(nu[k] > 0.0, "viscosity must be positive");
+    S->case_nu.assign(nu, nu + m->ncase);
+    XCAL_HIP(hipMemcpy(S->case_nu_d.p, nu, sizeof(double) * m->ncase, hipMemcpyHostToDevice));
+    S->initialised = false;
+    return 0;
+  });
+}
+
 
 int xcal_simple_stage_times(const xcal_simple* S, double* ms6) {
   return guarded([&] {

@@ -28,122 +28,17 @@
 #include <cstring>
 #include <functional>
 
-#include "device_common.h"
+#include "solver_common.h"
 
 namespace xcal {
-
-enum : int { STOP_NONE = 0, STOP_SOLVED = 1, STOP_ITMAX = 2, STOP_BREAKDOWN = 3,
-             STOP_ZERO_RHS = 4 };
-
-struct KS {
-  double gamma, pAp, alpha, beta, rNorm, eps, pNorm2, rho, next_rho, omega, atol, rtol;
-  int iter, itmax, stop, err;
-};
-
-enum Fin : int { FIN_NONE = 0, FIN_CG_INIT, FIN_CG_PAP, FIN_CG_ITER, FIN_BI_INIT, FIN_BI_ALPHA,
-                 FIN_BI_OMEGA, FIN_BI_END };
-
-__device__ inline void finalize(KS* ks, int fin, double d0, double d1) {
-  const double epsm = 2.220446049250313e-16;
-  switch (fin) {
-    case FIN_CG_INIT: {
-      ks->gamma = d0;
-      ks->iter = 0;
-      ks->beta = 0.0;
-      if (d0 == 0.0) { ks->stop = STOP_ZERO_RHS; return; }
-      ks->rNorm = sqrt(d0);
-      ks->pNorm2 = d0;
-      ks->eps = ks->atol + ks->rtol * ks->rNorm;
-      if (ks->rNorm <= ks->eps) ks->stop = STOP_SOLVED;
-      else if (ks->iter >= ks->itmax) ks->stop = STOP_ITMAX;
-      return;
-    }
-    case FIN_CG_PAP: {
-      ks->pAp = d0;
-      if (d0 <= epsm * ks->pNorm2 && fabs(d0) <= epsm * ks->pNorm2) ks->stop = STOP_BREAKDOWN;
-      else ks->alpha = ks->gamma / d0;
-      return;
-    }
-    case FIN_CG_ITER: {
-      const double gn = d0;
-      ks->rNorm = sqrt(gn);
-      const bool solved = ks->rNorm <= ks->eps || (ks->rNorm + 1.0 <= 1.0);
-      if (!solved) {
-        const double beta = gn / ks->gamma;
-        ks->beta = beta;
-        ks->pNorm2 = gn + beta * beta * ks->pNorm2;
-        ks->gamma = gn;
-      }
-      ks->iter += 1;
-      if (solved) ks->stop = STOP_SOLVED;
-      else if (ks->iter >= ks->itmax) ks->stop = STOP_ITMAX;
-      return;
-    }
-    case FIN_BI_INIT: {
-      ks->iter = 0;
-      ks->beta = 0.0;
-      ks->rNorm = sqrt(d0);
-      if (ks->rNorm == 0.0) { ks->stop = STOP_ZERO_RHS; return; }
-      ks->eps = ks->atol + ks->rtol * ks->rNorm;
-      ks->next_rho = d1;
-      if (d1 == 0.0) { ks->stop = STOP_ZERO_RHS; return; }
-      if (ks->rNorm <= ks->eps) ks->stop = STOP_SOLVED;
-      else if (ks->iter >= ks->itmax) ks->stop = STOP_ITMAX;
-      return;
-    }
-    case FIN_BI_ALPHA: {
-      ks->iter += 1;
-      ks->rho = ks->next_rho;
-      ks->alpha = ks->rho / d0;
-      return;
-    }
-    case FIN_BI_OMEGA: {
-      ks->omega = d0 / d1;
-      return;
-    }
-    case FIN_BI_END: {
-      ks->next_rho = d0;
-      ks->beta = (ks->next_rho / ks->rho) * (ks->alpha / ks->omega);
-      ks->rNorm = sqrt(d1);
-      const bool solved = ks->rNorm <= ks->eps || (ks->rNorm + 1.0 <= 1.0);
-      const bool tired = ks->iter >= ks->itmax;
-      const bool brk = ks->alpha == 0.0 || ks->alpha != ks->alpha;
-      if (solved) ks->stop = STOP_SOLVED;
-      else if (tired) ks->stop = STOP_ITMAX;
-      else if (brk) ks->stop = STOP_BREAKDOWN;
-      return;
-    }
-    default:
-      return;
-  }
-}
-
-
-// ---------------------------------------------------------- reductions
-// The kernels below are templates on the value type T of the matrix and the
-// vectors: double for the FP64 solve, float for the inner solves of the
-// mixed-precision (FP32 inner, FP64 refinement) mode.  Dot products are
-// accumulated in double in both cases; the Krylov scalars are double.
-__device__ inline bool stopped(const KS* ks) { return ks->stop != STOP_NONE; }
-
-// workgroup sums of s0, s1 -> partial[2 slot], partial[2 slot + 1]
-template <int NT>
-__device__ inline void store_partial(double s0, double s1, double* partial, int slot) {
-  __shared__ double lds[NT / 64];
-  s0 = block_sum(s0, lds);
-  s1 = block_sum(s1, lds);
-  if (threadIdx.x == 0) {
-    partial[2 * slot] = s0;
-    partial[2 * slot + 1] = s1;
-  }
-}
 
 // Sums `count` partial pairs in slot order (fixed per-thread strides, fixed
 // tree) and runs the scalar update `fin`.
 __global__ void __launch_bounds__(kRunBlock)
     k_finalize(const double* __restrict__ partial, int count, KS* ks, int fin) {
   __shared__ double lds[kRunBlock / 64];
-  if (stopped(ks)) return;
+  KS k;
+  if (threadIdx.x == 0) k = *ks;  // loaded while the partials load
   double t0 = 0.0, t1 = 0.0;
   for (int b = threadIdx.x; b < count; b += blockDim.x) {
     t0 += partial[2 * b];
@@ -151,10 +46,12 @@ __global__ void __launch_bounds__(kRunBlock)
   }
   t0 = block_sum(t0, lds);
   t1 = block_sum(t1, lds);
-  if (threadIdx.x == 0) finalize(ks, fin, t0, t1);
+  if (threadIdx.x == 0 && k.stop == STOP_NONE) {
+    finalize(&k, fin, t0, t1);
+    *ks = k;
+  }
 }
 
-__device__ inline double dd(double a, double b) { return a * b; }  // products of dots in double
 
 // --------------------------------------------------------- static kernels
 // r = b - A x0 (r = b when x0 is NULL: zero start) ; x = 0 ; pold = 0
@@ -165,8 +62,7 @@ __global__ void __launch_bounds__(kBlock)
   const int64_t i = xcd_row();
   if (i >= A.n) return;
   if (x0) {
-    T acc = 0;
-    for (int32_t k = A.rb[i]; k < A.re[i]; k += kSlice) acc += A.nz[k] * x0[A.colval[k]];
+    const T acc = row_sum(A, i, [&](int j) { return x0[j]; });
     r[i] = b[i] - acc;
   } else {
     r[i] = b[i];
@@ -203,19 +99,17 @@ template <class T>
 __global__ void __launch_bounds__(kBlock)
     k_cg_spmv(MatKT<T> A, const T* __restrict__ z, const T* __restrict__ pold,
               T* __restrict__ pnew, T* __restrict__ q, double* partial, const KS* ks) {
-  if (stopped(ks)) return;
+  const bool live = !stopped(ks);
   const T beta = (T)ks->beta;
   const int64_t i = xcd_row();
   double s0 = 0.0;
   if (i < A.n) {
-    T acc = 0;
-    for (int32_t k = A.rb[i]; k < A.re[i]; k += kSlice) {
-      const int j = A.colval[k];
-      acc += A.nz[k] * (z[j] + beta * pold[j]);
-    }
+    const T acc = row_sum(A, i, [&](int j) { return z[j] + beta * pold[j]; });
     const T pi = z[i] + beta * pold[i];
-    pnew[i] = pi;
-    q[i] = acc;
+    if (live) {
+      pnew[i] = pi;
+      q[i] = acc;
+    }
     s0 = dd(pi, acc);
   }
   store_partial<kBlock>(s0, 0.0, partial, blockIdx.x);
@@ -227,16 +121,19 @@ __global__ void __launch_bounds__(kBlock)
     k_cg_update_diag(int64_t n, const T* __restrict__ S, const T* __restrict__ p,
                      const T* __restrict__ q, T* __restrict__ x, T* __restrict__ r,
                      T* __restrict__ z, double* partial, const KS* ks) {
-  if (stopped(ks)) return;
+  const bool live = !stopped(ks);
   const T a = (T)ks->alpha;
   const int64_t i = xcd_row();
   double s0 = 0.0;
   if (i < n) {
-    x[i] = x[i] + a * p[i];
+    const T xi = x[i] + a * p[i];
     const T ri = r[i] + (-a) * q[i];
-    r[i] = ri;
     const T zi = S ? S[i] * ri : ri;
-    z[i] = zi;
+    if (live) {
+      x[i] = xi;
+      r[i] = ri;
+      z[i] = zi;
+    }
     s0 = dd(ri, zi);
   }
   store_partial<kBlock>(s0, 0.0, partial, blockIdx.x);
@@ -249,24 +146,19 @@ __global__ void __launch_bounds__(kBlock)
     k_bi_spmv_p(MatKT<T> A, const T* __restrict__ r, const T* __restrict__ pold,
                 T* __restrict__ pnew, const T* __restrict__ S, T* __restrict__ qv,
                 const T* __restrict__ c, double* partial, const KS* ks) {
-  if (stopped(ks)) return;
+  const bool live = !stopped(ks);
   const T beta = (T)ks->beta;
   const int64_t i = xcd_row();
   double s0 = 0.0;
   if (i < A.n) {
-    T acc = 0;
-    for (int32_t k = A.rb[i]; k < A.re[i]; k += kSlice) {
-      const int j = A.colval[k];
-      acc += A.nz[k] * (r[j] + beta * pold[j]);
-    }
-    pnew[i] = r[i] + beta * pold[i];
-    if (VDIAG) {
-      const T vi = S ? S[i] * acc : acc;
+    const T acc = row_sum(A, i, [&](int j) { return r[j] + beta * pold[j]; });
+    const T pi = r[i] + beta * pold[i];
+    const T vi = VDIAG && S ? S[i] * acc : acc;
+    if (live) {
+      pnew[i] = pi;
       qv[i] = vi;
-      s0 = dd(c[i], vi);
-    } else {
-      qv[i] = acc;
     }
+    if (VDIAG) s0 = dd(c[i], vi);
   }
   if (VDIAG) store_partial<kBlock>(s0, 0.0, partial, blockIdx.x);
 }
@@ -278,26 +170,23 @@ __global__ void __launch_bounds__(kBlock)
     k_bi_spmv_s(MatKT<T> A, const T* __restrict__ r, const T* __restrict__ v,
                 const T* __restrict__ p, T* __restrict__ s, T* __restrict__ x,
                 const T* __restrict__ S, T* __restrict__ dt, double* partial, const KS* ks) {
-  if (stopped(ks)) return;
+  const bool live = !stopped(ks);
   const T a = (T)ks->alpha;
   const int64_t i = xcd_row();
   double s0 = 0.0, s1 = 0.0;
   if (i < A.n) {
-    T acc = 0;
-    for (int32_t k = A.rb[i]; k < A.re[i]; k += kSlice) {
-      const int j = A.colval[k];
-      acc += A.nz[k] * (r[j] + (-a) * v[j]);
-    }
+    const T acc = row_sum(A, i, [&](int j) { return r[j] + (-a) * v[j]; });
     const T si = r[i] + (-a) * v[i];
-    s[i] = si;
-    x[i] = x[i] + a * p[i];
-    if (TDIAG) {
-      const T ti = S ? S[i] * acc : acc;
+    const T xi = x[i] + a * p[i];
+    const T ti = TDIAG && S ? S[i] * acc : acc;
+    if (live) {
+      s[i] = si;
+      x[i] = xi;
       dt[i] = ti;
+    }
+    if (TDIAG) {
       s0 = dd(ti, si);
       s1 = dd(ti, ti);
-    } else {
-      dt[i] = acc;
     }
   }
   if (TDIAG) store_partial<kBlock>(s0, s1, partial, blockIdx.x);
@@ -309,15 +198,19 @@ __global__ void __launch_bounds__(kBlock)
     k_bi_tail(int64_t n, const T* __restrict__ s, const T* __restrict__ t, const T* __restrict__ v,
               T* __restrict__ x, T* __restrict__ r, T* __restrict__ p, const T* __restrict__ c,
               double* partial, const KS* ks) {
-  if (stopped(ks)) return;
+  const bool live = !stopped(ks);
   const T w = (T)ks->omega;
   const int64_t i = xcd_row();
   double s0 = 0.0, s1 = 0.0;
   if (i < n) {
-    x[i] = x[i] + w * s[i];
+    const T xi = x[i] + w * s[i];
     const T ri = s[i] + (-w) * t[i];
-    r[i] = ri;
-    p[i] = p[i] + (-w) * v[i];
+    const T pi = p[i] + (-w) * v[i];
+    if (live) {
+      x[i] = xi;
+      r[i] = ri;
+      p[i] = pi;
+    }
     s0 = dd(c[i], ri);
     s1 = dd(ri, ri);
   }
@@ -340,8 +233,7 @@ __global__ void __launch_bounds__(kBlock)
   const int64_t i = xcd_row();
   double s0 = 0.0, s1 = 0.0;
   if (i < A.n) {
-    double acc = 0.0;
-    for (int32_t k = A.rb[i]; k < A.re[i]; k += kSlice) acc += A.nz[k] * x[A.colval[k]];
+    const double acc = row_sum(A, i, [&](int j) { return x[j]; });
     const double r = b[i] - acc;
     r32[i] = (float)r;
     s0 = r * r;
@@ -379,110 +271,51 @@ __global__ void __launch_bounds__(kBlock)
   if (i < n) b[i] = (float)a[i];
 }
 
-// ------------------------------------------------------------ DILU sweeps
-// Forward sweep y = (I + L D^-1)^-1 u: row order and per-row summation order
-// of forward_substitution!.  UPD fuses the CG update of the row's own x and
-// r first (u = r after r -= a q).
-template <class T>
-struct FwdK {
-  MatKT<T> A;
-  const T* rD;
-  const T* u;
-  T* Y;
-  const T* p;
-  const T* q;
-  T* x;
-  T* r;
-  const KS* ks;  // NULL outside a Krylov solve
-};
-
-template <bool UPD, class T>
-__device__ inline void fwd_row(const FwdK<T>& W, int i, T a) {
-  T u;
-  if (UPD) {
-    W.x[i] = W.x[i] + a * W.p[i];
-    u = W.r[i] + (-a) * W.q[i];
-    W.r[i] = u;
-  } else {
-    u = W.u[i];
-  }
-  T sum = 0;
-  for (int32_t k = W.A.rb[i]; k < W.A.diag[i]; k += kSlice) {
-    const int j = W.A.colval[k];
-    sum += (W.A.nz[k] * W.Y[j]) * W.rD[j];
-  }
-  W.Y[i] = u - sum;
-}
-
 // one level: rows [row0, row0 + cnt) (forward levels are contiguous)
 template <bool UPD, class T>
-__global__ void __launch_bounds__(kBlock) k_fwd_level(FwdK<T> W, int32_t row0, int32_t cnt) {
-  if (W.ks && stopped(W.ks)) return;
+__global__ void __launch_bounds__(kBlock)
+    k_fwd_level(FwdK<T> W, int32_t row0, int32_t cnt, int32_t base, int32_t w) {
+  const bool live = !W.ks || !stopped(W.ks);
   const T a = UPD ? (T)W.ks->alpha : T(0);
   const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < cnt) fwd_row<UPD>(W, row0 + t, a);
+  if (t < cnt) fwd_row<UPD>(W, row0 + t, (int64_t)base + t, cnt, w, a, live);
 }
 
 // a run of small levels [lev0, lev1) in one workgroup
 template <bool UPD, class T>
 __global__ void __launch_bounds__(kRunBlock)
-    k_fwd_run(FwdK<T> W, const int32_t* __restrict__ lev_ptr, int lev0, int lev1) {
-  if (W.ks && stopped(W.ks)) return;
+    k_fwd_run(FwdK<T> W, const int32_t* __restrict__ lev_ptr, const int32_t* __restrict__ meta,
+              int lev0, int lev1) {
+  if (W.ks && stopped(W.ks)) return;  // uniform over the workgroup
   const T a = UPD ? (T)W.ks->alpha : T(0);
   for (int l = lev0; l < lev1; ++l) {
-    const int32_t b = lev_ptr[l + 1];
-    for (int32_t i = lev_ptr[l] + threadIdx.x; i < b; i += blockDim.x) fwd_row<UPD>(W, i, a);
+    const int32_t r0 = lev_ptr[l], cnt = lev_ptr[l + 1] - r0, base = meta[2 * l], w = meta[2 * l + 1];
+    for (int32_t q = threadIdx.x; q < cnt; q += blockDim.x)
+      fwd_row<UPD>(W, r0 + q, (int64_t)base + q, cnt, w, a, true);
     __syncthreads();
   }
 }
 
-// Backward sweep z = (D + U)^-1 y: order of backward_substitution!.  Reads y
-// from Y, writes z to Z and folds up to two dot products into partials:
-//   DOT 0: none, 1: r.z (CG), 2: c.z (BiCGSTAB v), 3: z.s and z.z (BiCGSTAB t),
-//   4: z.z and c.z (BiCGSTAB start, z = M r0)
-template <class T>
-struct BwdK {
-  MatKT<T> A;
-  const int32_t* rows;  // backward-level order
-  const T* D;
-  const T* Y;
-  T* Z;
-  const T* dv;
-  double* partial;
-  const KS* ks;
-};
-
-template <int DOT, class T>
-__device__ inline void bwd_row(const BwdK<T>& W, int i, double& s0, double& s1) {
-  T sum = 0;
-  for (int32_t k = W.A.diag[i] + kSlice; k < W.A.re[i]; k += kSlice)
-    sum += W.A.nz[k] * W.Z[W.A.colval[k]];
-  const T zi = (W.Y[i] - sum) / W.D[i];
-  W.Z[i] = zi;
-  if (DOT == 1 || DOT == 2) s0 += dd(W.dv[i], zi);
-  if (DOT == 3) { s0 += dd(zi, W.dv[i]); s1 += dd(zi, zi); }
-  if (DOT == 4) { s0 += dd(zi, zi); s1 += dd(W.dv[i], zi); }
-}
-
 template <int DOT, class T>
 __global__ void __launch_bounds__(kBlock)
-    k_bwd_level(BwdK<T> W, int32_t pos0, int32_t cnt, int slot) {
-  if (W.ks && stopped(W.ks)) return;
+    k_bwd_level(BwdK<T> W, int32_t pos0, int32_t cnt, int32_t base, int32_t w, int slot) {
+  const bool live = !W.ks || !stopped(W.ks);
   const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   double s0 = 0.0, s1 = 0.0;
-  if (t < cnt) bwd_row<DOT>(W, W.rows[pos0 + t], s0, s1);
+  if (t < cnt) bwd_row<DOT>(W, pos0 + t, (int64_t)base + t, cnt, w, live, s0, s1);
   if (DOT != 0) store_partial<kBlock>(s0, s1, W.partial, slot + blockIdx.x);
 }
 
 template <int DOT, class T>
 __global__ void __launch_bounds__(kRunBlock)
-    k_bwd_run(BwdK<T> W, const int32_t* __restrict__ lev_ptr, int lev0, int lev1, int slot) {
+    k_bwd_run(BwdK<T> W, const int32_t* __restrict__ lev_ptr, const int32_t* __restrict__ meta,
+              int lev0, int lev1, int slot) {
   if (W.ks && stopped(W.ks)) return;
   double s0 = 0.0, s1 = 0.0;
   for (int l = lev0; l < lev1; ++l) {
-    const int32_t b = lev_ptr[l + 1];
-    for (int32_t p = lev_ptr[l] + threadIdx.x; p < b; p += blockDim.x)
-      bwd_row<DOT>(W, W.rows[p], s0, s1);
+    const int32_t p0 = lev_ptr[l], cnt = lev_ptr[l + 1] - p0, base = meta[2 * l], w = meta[2 * l + 1];
+    for (int32_t q = threadIdx.x; q < cnt; q += blockDim.x)
+      bwd_row<DOT>(W, p0 + q, (int64_t)base + q, cnt, w, true, s0, s1);
     __syncthreads();
   }
   if (DOT != 0) store_partial<kRunBlock>(s0, s1, W.partial, slot);
@@ -499,6 +332,7 @@ struct Solver {
   int64_t n;
   MatKT<T> A;
   const T *D, *rD, *Sj;  // DILU D and 1/D; Jacobi 1/|a_ii|
+  const T *Lnz, *Unz, *Db;  // packed DILU factors, D in backward-level order
   T* work;
   KS* ks;
   double* partial;
@@ -542,6 +376,9 @@ static Solver<double> make_solver(xcal_eqn* e) {
   S.D = e->pD.p;
   S.rD = e->prD.p;
   S.Sj = e->pS.p;
+  S.Lnz = e->Lnz.p;
+  S.Unz = e->Unz.p;
+  S.Db = e->Db.p;
   S.work = e->work.p;
   S.ks = (KS*)e->kstate.p;
   S.partial = e->kpart.p;
@@ -560,8 +397,18 @@ static Solver<float> make_solver32(xcal_eqn* e) {
   hipLaunchKernelGGL(k_to_float, dim3(grid_for(m->nzlen)), dim3(kBlock), 0, s, m->nzlen,
                      e->nzval.p, e->nz32.p);
   if (e->precon == XCAL_PRECON_DILU) {
+    if (e->Lnz32.n != e->Lnz.n) e->Lnz32.alloc(e->Lnz.n);
+    if (e->Unz32.n != e->Unz.n) e->Unz32.alloc(e->Unz.n);
+    if (e->Db32.n != n) e->Db32.alloc(n);
     hipLaunchKernelGGL(k_to_float, dim3(grid_for(n)), dim3(kBlock), 0, s, n, e->pD.p, e->pD32.p);
     hipLaunchKernelGGL(k_to_float, dim3(grid_for(n)), dim3(kBlock), 0, s, n, e->prD.p, e->prD32.p);
+    hipLaunchKernelGGL(k_to_float, dim3(grid_for(n)), dim3(kBlock), 0, s, n, e->Db.p, e->Db32.p);
+    if (e->Lnz.n > 0)
+      hipLaunchKernelGGL(k_to_float, dim3(grid_for(e->Lnz.n)), dim3(kBlock), 0, s, e->Lnz.n,
+                         e->Lnz.p, e->Lnz32.p);
+    if (e->Unz.n > 0)
+      hipLaunchKernelGGL(k_to_float, dim3(grid_for(e->Unz.n)), dim3(kBlock), 0, s, e->Unz.n,
+                         e->Unz.p, e->Unz32.p);
   } else if (e->precon == XCAL_PRECON_JACOBI) {
     hipLaunchKernelGGL(k_to_float, dim3(grid_for(n)), dim3(kBlock), 0, s, n, e->pS.p, e->pS32.p);
   }
@@ -575,12 +422,15 @@ static Solver<float> make_solver32(xcal_eqn* e) {
   S.D = e->pD32.p;
   S.rD = e->prD32.p;
   S.Sj = e->pS32.p;
+  S.Lnz = e->Lnz32.p;
+  S.Unz = e->Unz32.p;
+  S.Db = e->Db32.p;
   S.work = e->work32.p;
   S.ks = (KS*)e->kstate.p;
   S.partial = e->kpart.p;
   S.grid = grid_for(n);
-  S.cg_batch = &e->cg_batch32;
-  S.bi_batch = &e->bi_batch32;
+  S.cg_batch = &e->ir_pred[0][0];
+  S.bi_batch = &e->ir_pred[1][0];
   return S;
 }
 
@@ -596,10 +446,10 @@ static void sweep_fwd(const Solver<T>& S, const FwdK<T>& W) {
   for (const SweepStep& st : m->plan_f) {
     if (st.grid == 1)
       hipLaunchKernelGGL((k_fwd_run<UPD, T>), dim3(1), dim3(kRunBlock), 0, S.s, W,
-                         m->lev_ptr_f_d.p, st.lev0, st.lev1);
+                         m->lev_ptr_f_d.p, m->lmeta_f_d.p, st.lev0, st.lev1);
     else
       hipLaunchKernelGGL((k_fwd_level<UPD, T>), dim3(st.grid), dim3(kBlock), 0, S.s, W, st.row0,
-                         st.row1 - st.row0);
+                         st.row1 - st.row0, m->lmeta_f[2 * st.lev0], m->lmeta_f[2 * st.lev0 + 1]);
   }
 }
 
@@ -610,27 +460,31 @@ static void sweep_bwd(const Solver<T>& S, const BwdK<T>& W) {
   for (const SweepStep& st : m->plan_b) {
     if (st.grid == 1)
       hipLaunchKernelGGL((k_bwd_run<DOT, T>), dim3(1), dim3(kRunBlock), 0, S.s, W,
-                         m->lev_ptr_b_d.p, st.lev0, st.lev1, st.slot);
+                         m->lev_ptr_b_d.p, m->lmeta_b_d.p, st.lev0, st.lev1, st.slot);
     else
       hipLaunchKernelGGL((k_bwd_level<DOT, T>), dim3(st.grid), dim3(kBlock), 0, S.s, W, st.row0,
-                         st.row1 - st.row0, st.slot);
+                         st.row1 - st.row0, m->lmeta_b[2 * st.lev0], m->lmeta_b[2 * st.lev0 + 1],
+                         st.slot);
   }
 }
 
+// Y, Yb: two n-vectors of workspace (forward output in device and in
+// backward-level order)
 template <class T>
-static FwdK<T> fwdk(const Solver<T>& S, const T* u, T* Y, const KS* ks) {
-  return FwdK<T>{S.A, S.rD, u, Y, nullptr, nullptr, nullptr, nullptr, ks};
+static FwdK<T> fwdk(const Solver<T>& S, const T* u, T* Y, T* Yb, const KS* ks) {
+  return FwdK<T>{S.m->Lcol.p, S.Lnz, S.rD, u, Y, Yb, S.m->posb.p, nullptr, nullptr, nullptr,
+                 nullptr, ks};
 }
 template <class T>
-static BwdK<T> bwdk(const Solver<T>& S, const T* Y, T* Z, const T* dv, const KS* ks) {
-  return BwdK<T>{S.A, S.m->lev_rows_b.p, S.D, Y, Z, dv, S.partial, ks};
+static BwdK<T> bwdk(const Solver<T>& S, const T* Yb, T* Z, const T* dv, const KS* ks) {
+  return BwdK<T>{S.m->Ucol.p, S.Unz, S.m->lev_rows_b.p, S.Db, Yb, Z, dv, S.partial, ks};
 }
 
 // z = M^-1 u through the level sweeps (DILU), used by precond_apply
 static void dilu_apply(Solver<double>& S, const double* u, double* z) {
-  double* Y = S.work;
-  sweep_fwd<false>(S, fwdk(S, u, Y, (const KS*)nullptr));
-  sweep_bwd<0>(S, bwdk(S, (const double*)Y, z, (const double*)nullptr, (const KS*)nullptr));
+  double *Y = S.work, *Yb = S.work + S.n;
+  sweep_fwd<false>(S, fwdk(S, u, Y, Yb, (const KS*)nullptr));
+  sweep_bwd<0>(S, bwdk(S, (const double*)Yb, z, (const double*)nullptr, (const KS*)nullptr));
   XCAL_HIP(hipGetLastError());
 }
 
@@ -695,14 +549,24 @@ static void init_state(Solver<T>& S, int itmax, double atol, double rtol) {
   XCAL_HIP(hipMemcpyAsync(S.ks, S.e->kstate_host, sizeof(KS), hipMemcpyHostToDevice, S.s));
 }
 
+// Enqueues iterations in batches and reads the device state after each.
+// Iterations enqueued after the solve has stopped still load their data
+// (only their stores are predicated off), so the first batch is sized from
+// the previous solve of the same equation (`*pred` iterations; 0 = none) and
+// later batches are short.
 template <class T>
-static int run_batches(Solver<T>& S, int batch, const std::function<void()>& iteration, KS& out) {
+static int run_batches(Solver<T>& S, int* pred, int tail, const std::function<void()>& iteration,
+                       KS& out) {
+  int batch = *pred > 0 ? std::max(1, (*pred * 9) / 10) : 8;
   for (;;) {
     for (int b = 0; b < batch; ++b) iteration();
     XCAL_HIP(hipGetLastError());
     out = read_state(S);
-    if (out.stop != STOP_NONE) return out.stop;
+    if (out.stop != STOP_NONE) break;
+    batch = tail;
   }
+  *pred = out.iter;
+  return out.stop;
 }
 
 // Krylov.jl cg! with warm start x0 (solution = x0 + correction, written to
@@ -714,15 +578,15 @@ static int cg(Solver<T>& S, int precon, const T* b, T* x0, int itmax, double ato
   const int64_t n = S.n;
   T* w = S.work;
   T *r = w, *x = w + n, *pa = w + 2 * n, *pb = w + 3 * n, *q = w + 4 * n, *zd = w + 5 * n,
-    *Y = w + 6 * n;
+    *Y = w + 6 * n, *Yb = w + 7 * n;
   init_state(S, itmax, atol, rtol);
   hipLaunchKernelGGL(k_start<T>, dim3(S.grid), dim3(kBlock), 0, S.s, S.A, b, (const T*)x0, r, x, pa);
   const bool dilu = precon == XCAL_PRECON_DILU;
   const T* Sd = precon == XCAL_PRECON_JACOBI ? S.Sj : nullptr;
   const int sweep_slots = S.m->sweep_slots;
   if (dilu) {
-    sweep_fwd<false>(S, fwdk(S, (const T*)r, Y, (const KS*)S.ks));
-    sweep_bwd<1>(S, bwdk(S, (const T*)Y, zd, (const T*)r, (const KS*)S.ks));
+    sweep_fwd<false>(S, fwdk(S, (const T*)r, Y, Yb, (const KS*)S.ks));
+    sweep_bwd<1>(S, bwdk(S, (const T*)Yb, zd, (const T*)r, (const KS*)S.ks));
     finalize_launch(S, sweep_slots, FIN_CG_INIT);
   } else {
     hipLaunchKernelGGL((k_precond_diag_start<false, T>), dim3(S.grid), dim3(kBlock), 0, S.s, n, Sd,
@@ -741,13 +605,13 @@ static int cg(Solver<T>& S, int precon, const T* b, T* x0, int itmax, double ato
     });
     finalize_launch(S, S.grid, FIN_CG_PAP);
     if (dilu) {
-      FwdK<T> W = fwdk(S, (const T*)nullptr, Y, (const KS*)S.ks);
+      FwdK<T> W = fwdk(S, (const T*)nullptr, Y, Yb, (const KS*)S.ks);
       W.p = pnew;
       W.q = q;
       W.x = x;
       W.r = r;
       timed(c, 1, [&] { sweep_fwd<true>(S, W); });
-      timed(c, 2, [&] { sweep_bwd<1>(S, bwdk(S, (const T*)Y, zd, (const T*)r, (const KS*)S.ks)); });
+      timed(c, 2, [&] { sweep_bwd<1>(S, bwdk(S, (const T*)Yb, zd, (const T*)r, (const KS*)S.ks)); });
       finalize_launch(S, sweep_slots, FIN_CG_ITER);
     } else {
       hipLaunchKernelGGL(k_cg_update_diag<T>, dim3(S.grid), dim3(kBlock), 0, S.s, n, Sd,
@@ -756,14 +620,12 @@ static int cg(Solver<T>& S, int precon, const T* b, T* x0, int itmax, double ato
     }
   };
   KS st = read_state(S);
-  if (st.stop == STOP_NONE) run_batches(S, *S.cg_batch, iteration, st);
+  if (st.stop == STOP_NONE) run_batches(S, S.cg_batch, 4, iteration, st);
   if (x0) hipLaunchKernelGGL(k_add, dim3(S.grid), dim3(kBlock), 0, S.s, n, (const double*)x,
                              (double*)x0);
   if (xsol) *xsol = x;
   XCAL_HIP(hipGetLastError());
   niter = st.iter;
-  // next solve: batch sized to finish in about two reads
-  *S.cg_batch = std::max(4, std::min(64, niter / 2 + 1));
   return (st.stop == STOP_ITMAX || niter >= itmax) ? 1 : 0;
 }
 
@@ -775,7 +637,7 @@ static int bicgstab(Solver<T>& S, int precon, const T* b, T* x0, int itmax, doub
   const int64_t n = S.n;
   T* w = S.work;
   T *r = w, *x = w + n, *pa = w + 2 * n, *pb = w + 3 * n, *v = w + 4 * n, *sv = w + 5 * n,
-    *t = w + 6 * n, *d = w + 7 * n, *Y = w + 8 * n;
+    *t = w + 6 * n, *d = w + 7 * n, *Y = w + 8 * n, *Yb = w + 9 * n;
   init_state(S, itmax, atol, rtol);
   hipLaunchKernelGGL(k_start<T>, dim3(S.grid), dim3(kBlock), 0, S.s, S.A, b, (const T*)x0, r, x, pa);
   const bool dilu = precon == XCAL_PRECON_DILU;
@@ -784,8 +646,8 @@ static int bicgstab(Solver<T>& S, int precon, const T* b, T* x0, int itmax, doub
   const KS* ks = S.ks;
   if (dilu) {
     // r = M r0 (in place: the backward sweep reads only Y) ; ||r||^2, <b, r>
-    sweep_fwd<false>(S, fwdk(S, (const T*)r, Y, ks));
-    sweep_bwd<4>(S, bwdk(S, (const T*)Y, r, b, ks));
+    sweep_fwd<false>(S, fwdk(S, (const T*)r, Y, Yb, ks));
+    sweep_bwd<4>(S, bwdk(S, (const T*)Yb, r, b, ks));
     finalize_launch(S, sweep_slots, FIN_BI_INIT);
   } else {
     hipLaunchKernelGGL((k_precond_diag_start<true, T>), dim3(S.grid), dim3(kBlock), 0, S.s, n, Sd, r,
@@ -801,14 +663,14 @@ static int bicgstab(Solver<T>& S, int precon, const T* b, T* x0, int itmax, doub
       hipLaunchKernelGGL((k_bi_spmv_p<false, T>), dim3(S.grid), dim3(kBlock), 0, S.s, S.A,
                          (const T*)r, (const T*)pold, pnew, (const T*)nullptr, d, b, S.partial,
                          ks);  // d holds q
-      sweep_fwd<false>(S, fwdk(S, (const T*)d, Y, ks));
-      sweep_bwd<2>(S, bwdk(S, (const T*)Y, v, b, ks));
+      sweep_fwd<false>(S, fwdk(S, (const T*)d, Y, Yb, ks));
+      sweep_bwd<2>(S, bwdk(S, (const T*)Yb, v, b, ks));
       finalize_launch(S, sweep_slots, FIN_BI_ALPHA);
       hipLaunchKernelGGL((k_bi_spmv_s<false, T>), dim3(S.grid), dim3(kBlock), 0, S.s, S.A,
                          (const T*)r, (const T*)v, (const T*)pnew, sv, x, (const T*)nullptr, d,
                          S.partial, ks);
-      sweep_fwd<false>(S, fwdk(S, (const T*)d, Y, ks));
-      sweep_bwd<3>(S, bwdk(S, (const T*)Y, t, (const T*)sv, ks));
+      sweep_fwd<false>(S, fwdk(S, (const T*)d, Y, Yb, ks));
+      sweep_bwd<3>(S, bwdk(S, (const T*)Yb, t, (const T*)sv, ks));
       finalize_launch(S, sweep_slots, FIN_BI_OMEGA);
     } else {
       hipLaunchKernelGGL((k_bi_spmv_p<true, T>), dim3(S.grid), dim3(kBlock), 0, S.s, S.A,
@@ -823,13 +685,12 @@ static int bicgstab(Solver<T>& S, int precon, const T* b, T* x0, int itmax, doub
     finalize_launch(S, S.grid, FIN_BI_END);
   };
   KS st = read_state(S);
-  if (st.stop == STOP_NONE) run_batches(S, *S.bi_batch, iteration, st);
+  if (st.stop == STOP_NONE) run_batches(S, S.bi_batch, 2, iteration, st);
   if (x0) hipLaunchKernelGGL(k_add, dim3(S.grid), dim3(kBlock), 0, S.s, n, (const double*)x,
                              (double*)x0);
   if (xsol) *xsol = x;
   XCAL_HIP(hipGetLastError());
   niter = st.iter;
-  *S.bi_batch = std::max(1, std::min(16, niter + 1));
   return (st.stop == STOP_ITMAX || niter >= itmax) ? 1 : 0;
 }
 
@@ -857,7 +718,7 @@ static int ir_solve(xcal_eqn* e, const xcal_solver_setup* st, int itmax, double*
                     const double* b, int& niter, int& nouter) {
   Solver<double> S = make_solver(e);
   Solver<float> S32 = make_solver32(e);
-  float* r32 = e->work32.p + 11 * S.n;  // not used by the FP32 Krylov workspace (9 n)
+  float* r32 = e->work32.p + 11 * S.n;  // not used by the FP32 Krylov workspace (10 n)
   const double inner_rtol = st->ir_inner_rtol > 0 ? st->ir_inner_rtol : 1e-4;
   double rr, bb;
   ir_residual(S, x, b, r32, rr, bb);
@@ -873,6 +734,8 @@ static int ir_solve(xcal_eqn* e, const xcal_solver_setup* st, int itmax, double*
     }
     int ni = 0;
     float* d = nullptr;
+    S32.cg_batch = &e->ir_pred[0][std::min(nouter, 15)];
+    S32.bi_batch = &e->ir_pred[1][std::min(nouter, 15)];
     if (st->solver == XCAL_SOLVER_CG)
       cg(S32, st->precon, (const float*)r32, (float*)nullptr, itmax - niter, 0.0, inner_rtol, ni, &d);
     else
@@ -896,6 +759,19 @@ int solve_system(xcal_eqn* e, const xcal_solver_setup* st, double* x, const doub
              "unknown Krylov solver");
   XCAL_CHECK(st->precon == e->precon,
              "preconditioner of the solver setup has not been updated on this equation");
+  if (e->mesh->batch) {  // batch mesh: the worst case is reported
+    std::vector<xcal_solve_stats> cs(e->mesh->ncase);
+    const int rc = solve_system_batch(e, st, x, b, cs.data());
+    if (stats) {
+      *stats = cs[0];
+      for (const auto& c : cs) {
+        stats->niter = std::max(stats->niter, c.niter);
+        stats->nouter = std::max(stats->nouter, c.nouter);
+        stats->residual = std::max(stats->residual, c.residual);
+      }
+    }
+    return rc;
+  }
   const int itmax = st->itmax == 0 ? (int)(2 * e->mesh->n) : st->itmax;
   int niter = 0, nouter = 1, rc;
   if (st->precision == XCAL_PREC_FP32_IR) {
@@ -966,6 +842,20 @@ int xcal_kernel_timing_read(xcal_ctx* c, int64_t* launches, double* total_ms) {
     *launches = c->prof_launches;
     *total_ms = c->prof_ms;
     return 0;
+  });
+}
+
+int xcal_solve_system_batch(xcal_eqn* e, const xcal_solver_setup* s, xcal_field* phi,
+                            int32_t comp, xcal_solve_stats* stats) {
+  return guarded([&] {
+    XCAL_CHECK(e && s && stats, "NULL argument");
+    check_field(phi, e->mesh, e->vector ? XCAL_CELL_VECTOR : XCAL_CELL_SCALAR, "phi");
+    XCAL_CHECK(comp >= 0 && comp < (e->vector ? 3 : 1), "component out of range");
+    XCAL_CHECK(s->precon == e->precon,
+               "preconditioner of the solver setup has not been updated on this equation");
+    XCAL_CHECK(s->solver == XCAL_SOLVER_CG || s->solver == XCAL_SOLVER_BICGSTAB,
+               "unknown Krylov solver");
+    return solve_system_batch(e, s, phi->comp(comp), e->bc(e->vector ? comp : 0), stats);
   });
 }
 

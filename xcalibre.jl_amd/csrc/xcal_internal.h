@@ -54,7 +54,7 @@ constexpr int kMaxSources = 2;
 constexpr int kMaxPatches = 32;
 constexpr int kSlice = 64;        // rows per slice of the sliced-ELL matrix storage
 constexpr int kRunBlock = 1024;   // workgroup size of the single-workgroup sweep runs
-constexpr int kSmallLevel = 2048; // levels below this many rows join a single-workgroup run
+constexpr int kSmallLevel = 256;  // levels below this many rows join a single-workgroup run
 
 inline int grid_for(int64_t n, int block = kBlock) {
   int64_t g = (n + block - 1) / block;
@@ -146,7 +146,27 @@ struct xcal_mesh {
   // [lev_ptr_b[l], lev_ptr_b[l+1])
   int nlev_f = 0, nlev_b = 0;
   std::vector<int32_t> lev_ptr_f, lev_ptr_b;
-  xcal::DBuf<int32_t> lev_ptr_f_d, lev_ptr_b_d, lev_rows_b;
+  xcal::DBuf<int32_t> lev_ptr_f_d, lev_ptr_b_d, lev_rows_b, posb;
+  // packed lower (forward) / upper (backward) factors: per level l,
+  // lmeta[2l] = base, lmeta[2l+1] = width; entry t of the level's q-th row at
+  // base + t*cnt + q; *col = device column (-1: padding), *slot = nz slot
+  std::vector<int32_t> lmeta_f, lmeta_b;
+  xcal::DBuf<int32_t> lmeta_f_d, lmeta_b_d, Lcol, Lslot, Ucol, Uslot;
+  // cases of a batch (one case for a plain mesh): cells [case_cell[k],
+  // case_cell[k+1]), boundary faces [case_bface[k], case_bface[k+1]),
+  // internal faces [case_iface[k], case_iface[k+1]), forward/backward levels
+  // [case_lev_*[k], case_lev_*[k+1]) of the level arrays
+  int ncase = 1;
+  bool batch = false;  // created as a batch (of any size): batched solvers
+  std::vector<int64_t> case_cell, case_bface, case_iface;
+  std::vector<int32_t> case_lev_f, case_lev_b;
+  xcal::DBuf<int64_t> case_cell_d;
+  xcal::DBuf<int32_t> case_lev_f_d, case_lev_b_d;
+  // batched solves: workgroup b covers rows [blk_row0[b], blk_row1[b]) of
+  // case blk_case[b]; case k owns workgroups [case_blk[k], case_blk[k+1]);
+  // case_iota = 0..ncase; face_case = case of each face (batches only)
+  int64_t nblk = 0;
+  xcal::DBuf<int32_t> blk_row0, blk_row1, blk_case, case_blk_d, case_iota_d, face_case;
   int small_level = 0;
   std::vector<SweepStep> plan_f, plan_b;
   int sweep_slots = 0;  // partial-sum slots used by one backward sweep
@@ -173,15 +193,20 @@ struct xcal_eqn {
   xcal::DBuf<double> pS;   // Jacobi 1/|a_ii|
   xcal::DBuf<double> pD;   // DILU D
   xcal::DBuf<double> prD;  // DILU 1/D
+  xcal::DBuf<double> Lnz, Unz, Db;  // packed factor values; D in backward-level order
   // Krylov workspace (allocated lazily)
   xcal::DBuf<double> work;    // solver vectors, 12 x n
   xcal::DBuf<double> kpart;   // reduction partial sums
   xcal::DBuf<char> kstate;    // device-resident Krylov scalars
   void* kstate_host = nullptr;  // pinned copy
-  int cg_batch = 8, bi_batch = 2;   // iterations enqueued per state read
+  int cg_batch = 0, bi_batch = 0;   // iterations of the last solve (first-batch size)
   // mixed precision: FP32 copies of the matrix and preconditioner, workspace
-  xcal::DBuf<float> work32, nz32, pD32, prD32, pS32;
-  int cg_batch32 = 8, bi_batch32 = 2;
+  xcal::DBuf<float> work32, nz32, pD32, prD32, pS32, Lnz32, Unz32, Db32;
+  // batched solves: first-batch sizes, per-case flags, host staging
+  std::vector<int> batch_pred;
+  xcal::DBuf<int32_t> case_active;
+  std::vector<double> case_host;
+  int ir_pred[2][16] = {};  // FP32 inner solves: iterations of the last solve per sweep index
   ~xcal_eqn() {
     if (kstate_host) (void)hipHostFree(kstate_host);
   }
@@ -260,6 +285,10 @@ void apply_M_levels(const xcal_eqn* e, const double* r, double* z);
 void precond_update(xcal_eqn* e, int precon);
 int solve_system(xcal_eqn* e, const xcal_solver_setup* s, double* x, const double* b,
                  xcal_solve_stats* st);
+// batch meshes: every case solved independently; st has one entry per case
+int solve_system_batch(xcal_eqn* e, const xcal_solver_setup* s, double* x, const double* b,
+                       xcal_solve_stats* st);
+void precond_update_batch(xcal_eqn* e);
 double residual(xcal_eqn* e, const double* x, const double* b);
 
 }  // namespace xcal

@@ -42,6 +42,13 @@ def lib():
             "xcal_mesh_info": [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), _pi32, _pi32,
                                C.POINTER(C.c_int64)],
             "xcal_mesh_csr_pattern": [_vp, _pi32, _pi32],
+            "xcal_mesh_create_batch": [_vp, C.POINTER(C.POINTER(A.MeshHost)), C.c_int32,
+                                       C.POINTER(_vp)],
+            "xcal_mesh_cases": [_vp, _pi32, C.POINTER(C.c_int64)],
+            "xcal_solve_system_batch": [_vp, C.POINTER(A.SolverSetupC), _vp, C.c_int32,
+                                        C.POINTER(A.SolveStats)],
+            "xcal_simple_run_batch": [_vp, C.c_int32, _pd, _pi32, _pi32],
+            "xcal_simple_set_case_nu": [_vp, _pd],
             "xcal_field_create": [_vp, C.c_int32, C.POINTER(_vp)],
             "xcal_field_destroy": [_vp],
             "xcal_field_upload": [_vp, C.c_int32, _pd],
@@ -128,13 +135,35 @@ class Context:
             self.h = None
 
 
+class BatchMeshInfo:
+    """Host view of a batch: the cases' meshes and their concatenated sizes."""
+
+    def __init__(self, meshes):
+        self.cases = list(meshes)
+        self.n_cells = sum(m.n_cells for m in self.cases)
+        self.n_faces = sum(m.n_faces for m in self.cases)
+        self.patch_names = list(self.cases[0].patch_names)
+        self.cell_offsets = np.cumsum([0] + [m.n_cells for m in self.cases])
+        self.face_offsets = np.cumsum([0] + [m.n_faces for m in self.cases])
+
+
 class DeviceMesh:
-    def __init__(self, ctx: Context, mesh):
+    """xcal_mesh.  DeviceMesh(ctx, mesh) uploads one mesh; DeviceMesh.batch(ctx,
+    meshes) uploads independent cases as one batch mesh (xcal_mesh_create_batch)."""
+
+    def __init__(self, ctx: Context, mesh, _batch=None):
         self.ctx = ctx
         self.mesh = mesh
-        host = A.mesh_host(mesh)
         self.h = _vp()
-        check(lib().xcal_mesh_create(ctx.h, C.byref(host), C.byref(self.h)))
+        if _batch is None:
+            host = A.mesh_host(mesh)
+            check(lib().xcal_mesh_create(ctx.h, C.byref(host), C.byref(self.h)))
+            self.ncase = 1
+        else:
+            hosts = [A.mesh_host(m) for m in _batch]
+            arr = (C.POINTER(A.MeshHost) * len(hosts))(*[C.pointer(h) for h in hosts])
+            check(lib().xcal_mesh_create_batch(ctx.h, arr, len(hosts), C.byref(self.h)))
+            self.ncase = len(hosts)
         n, nnz, bytes_ = C.c_int64(), C.c_int64(), C.c_int64()
         fl, bl = C.c_int32(), C.c_int32()
         check(lib().xcal_mesh_info(self.h, C.byref(n), C.byref(nnz), C.byref(fl), C.byref(bl),
@@ -142,6 +171,10 @@ class DeviceMesh:
         self.n, self.nnz = n.value, nnz.value
         self.levels = (fl.value, bl.value)
         self.device_bytes = bytes_.value
+
+    @classmethod
+    def batch(cls, ctx: Context, meshes):
+        return cls(ctx, BatchMeshInfo(meshes), _batch=list(meshes))
 
     def csr_pattern(self):
         rp = np.zeros(self.n + 1, np.int32)
@@ -265,6 +298,14 @@ class Equation:
         self.last_nouter = st.nouter  # refinement sweeps (1 for FP64)
         return st.niter, st.residual, rc
 
+    def solve_batch(self, setup: A.SolverSetupC, phi: Field, component=0):
+        """Per-case (niter, nouter, residual) arrays and the status of a batch solve."""
+        K = self.dmesh.ncase
+        st = (A.SolveStats * K)()
+        rc = check(lib().xcal_solve_system_batch(self.h, C.byref(setup), phi.h, component, st))
+        return (np.array([s.niter for s in st]), np.array([s.nouter for s in st]),
+                np.array([s.residual for s in st]), rc)
+
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:
             _lib.xcal_eqn_destroy(self.h)
@@ -304,6 +345,22 @@ class Simple:
         d = done.value
         return dict(done=d, residuals=res.reshape(4, iterations)[:, :d],
                     krylov_iters=kit.reshape(4, iterations)[:, :d])
+
+    def run_batch(self, iterations: int):
+        """Batch meshes: residuals and Krylov counts per case, (ncase, 4, done)."""
+        K = self.dmesh.ncase
+        res = np.zeros(K * 4 * iterations)
+        kit = np.zeros(K * 4 * iterations, np.int32)
+        done = C.c_int32()
+        check(lib().xcal_simple_run_batch(self.h, iterations, _pd_of(res), C.byref(done),
+                                          kit.ctypes.data_as(_pi32)))
+        d = done.value
+        return dict(done=d, residuals=res.reshape(K, 4, iterations)[:, :, :d],
+                    krylov_iters=kit.reshape(K, 4, iterations)[:, :, :d])
+
+    def set_case_nu(self, nu):
+        nu = np.ascontiguousarray(np.asarray(nu, dtype=np.float64).reshape(self.dmesh.ncase))
+        check(lib().xcal_simple_set_case_nu(self.h, _pd_of(nu)))
 
     def stage_times(self):
         t = np.zeros(6)

@@ -27,37 +27,49 @@ struct BlkK {
   const int32_t* kcase;
 };
 
-// Sums the partial pairs of segment k ([seg[k], seg[k+1])) in order and runs
-// the scalar update of case k (one thread per case).
-__global__ void __launch_bounds__(kBlock)
-    kb_finalize(const double* __restrict__ partial, const int32_t* __restrict__ seg, int ncase,
-                KS* ks, int fin) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= ncase) return;
-  KS st = ks[k];
-  if (st.stop != STOP_NONE) return;
-  double t0 = 0.0, t1 = 0.0;
-  for (int j = seg[k]; j < seg[k + 1]; ++j) {
+// Sum of segment k ([seg[k], seg[k+1])) of the partial pairs by one
+// wavefront: fixed per-lane strides, then a fixed shuffle tree (the result
+// depends only on the segment).
+__device__ inline void wave_sum_segment(const double* __restrict__ partial, int32_t j0, int32_t j1,
+                                        double& t0, double& t1) {
+  t0 = 0.0;
+  t1 = 0.0;
+  for (int32_t j = j0 + (int32_t)threadIdx.x; j < j1; j += 64) {
     t0 += partial[2 * j];
     t1 += partial[2 * j + 1];
   }
-  finalize(&st, fin, t0, t1);
-  ks[k] = st;
+  for (int o = 32; o > 0; o >>= 1) {
+    t0 += __shfl_down(t0, o, 64);
+    t1 += __shfl_down(t1, o, 64);
+  }
+}
+
+// one wavefront per case: sum its partials and run its scalar update
+__global__ void __launch_bounds__(64)
+    kb_finalize(const double* __restrict__ partial, const int32_t* __restrict__ seg, int ncase,
+                KS* ks, int fin) {
+  const int k = blockIdx.x;
+  KS st;
+  if (threadIdx.x == 0) st = ks[k];  // loaded while the partials load
+  double t0, t1;
+  wave_sum_segment(partial, seg[k], seg[k + 1], t0, t1);
+  if (threadIdx.x == 0 && st.stop == STOP_NONE) {
+    finalize(&st, fin, t0, t1);
+    ks[k] = st;
+  }
 }
 
 // per-case sums of the partial pairs -> out[2k], out[2k+1]
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(64)
     kb_sum_cases(const double* __restrict__ partial, const int32_t* __restrict__ seg, int ncase,
                  double* __restrict__ out) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= ncase) return;
-  double t0 = 0.0, t1 = 0.0;
-  for (int j = seg[k]; j < seg[k + 1]; ++j) {
-    t0 += partial[2 * j];
-    t1 += partial[2 * j + 1];
+  const int k = blockIdx.x;
+  double t0, t1;
+  wave_sum_segment(partial, seg[k], seg[k + 1], t0, t1);
+  if (threadIdx.x == 0) {
+    out[2 * k] = t0;
+    out[2 * k + 1] = t1;
   }
-  out[2 * k] = t0;
-  out[2 * k + 1] = t1;
 }
 
 // r = b - A x0 (r = b when x0 is NULL) ; x = 0 ; pold = 0
@@ -428,13 +440,13 @@ static BSolver<float> batch_solver32(xcal_eqn* e) {
 
 template <class T>
 static void fin_blocks(const BSolver<T>& S, int fin) {
-  hipLaunchKernelGGL(kb_finalize, dim3(grid_for(S.ncase)), dim3(kBlock), 0, S.s, S.partial,
-                     S.m->case_blk_d.p, S.ncase, S.ks, fin);
+  hipLaunchKernelGGL(kb_finalize, dim3(S.ncase), dim3(64), 0, S.s, S.partial, S.m->case_blk_d.p,
+                     S.ncase, S.ks, fin);
 }
 template <class T>
 static void fin_cases(const BSolver<T>& S, int fin) {
-  hipLaunchKernelGGL(kb_finalize, dim3(grid_for(S.ncase)), dim3(kBlock), 0, S.s, S.partial,
-                     S.m->case_iota_d.p, S.ncase, S.ks, fin);
+  hipLaunchKernelGGL(kb_finalize, dim3(S.ncase), dim3(64), 0, S.s, S.partial, S.m->case_iota_d.p,
+                     S.ncase, S.ks, fin);
 }
 
 template <bool UPD, class T>
@@ -476,6 +488,7 @@ template <class T>
 static bool bread(BSolver<T>& S) {
   XCAL_HIP(hipMemcpyAsync(S.e->kstate_host, S.ks, sizeof(KS) * S.ncase, hipMemcpyDeviceToHost, S.s));
   XCAL_HIP(hipStreamSynchronize(S.s));
+  prof_resolve(S.m->ctx);
   const KS* h = (const KS*)S.e->kstate_host;
   for (int k = 0; k < S.ncase; ++k)
     if (h[k].stop == STOP_NONE) return false;
@@ -524,12 +537,15 @@ static void bcg(BSolver<T>& S, int precon, const T* b, T* x0, int itmax, double 
     fin_blocks(S, FIN_CG_INIT);
   }
   int parity = 0;
+  xcal_ctx* c = S.m->ctx;
   auto iteration = [&]() {
     T* pold = parity ? pb : pa;
     T* pnew = parity ? pa : pb;
     parity ^= 1;
-    hipLaunchKernelGGL(kb_cg_spmv<T>, dim3(S.nblk), dim3(kBlock), 0, S.s, S.B, S.A, (const T*)zd,
-                       (const T*)pold, pnew, q, S.partial, ks);
+    timed(c, 0, [&] {
+      hipLaunchKernelGGL(kb_cg_spmv<T>, dim3(S.nblk), dim3(kBlock), 0, S.s, S.B, S.A, (const T*)zd,
+                         (const T*)pold, pnew, q, S.partial, ks);
+    });
     fin_blocks(S, FIN_CG_PAP);
     if (dilu) {
       FwdK<T> W = bfwdk(S, (const T*)nullptr, Y, Yb);
@@ -537,8 +553,8 @@ static void bcg(BSolver<T>& S, int precon, const T* b, T* x0, int itmax, double 
       W.q = q;
       W.x = x;
       W.r = r;
-      bsweep_fwd<true>(S, W, ks);
-      bsweep_bwd<1>(S, bbwdk(S, (const T*)Yb, zd, (const T*)r), ks);
+      timed(c, 1, [&] { bsweep_fwd<true>(S, W, ks); });
+      timed(c, 2, [&] { bsweep_bwd<1>(S, bbwdk(S, (const T*)Yb, zd, (const T*)r), ks); });
       fin_cases(S, FIN_CG_ITER);
     } else {
       hipLaunchKernelGGL(kb_cg_update_diag<T>, dim3(S.nblk), dim3(kBlock), 0, S.s, S.B, Sd,
@@ -621,8 +637,8 @@ static void bresidual_sums(BSolver<double>& S, const double* x, const double* b,
     S.e->case_host.resize(2 * (size_t)K);
   }
   hipLaunchKernelGGL(kb_residual, dim3(S.nblk), dim3(kBlock), 0, S.s, S.B, S.A, x, b, r32, S.partial);
-  hipLaunchKernelGGL(kb_sum_cases, dim3(grid_for(K)), dim3(kBlock), 0, S.s, S.partial,
-                     S.m->case_blk_d.p, K, c->red_result.p);
+  hipLaunchKernelGGL(kb_sum_cases, dim3(K), dim3(64), 0, S.s, S.partial, S.m->case_blk_d.p, K,
+                     c->red_result.p);
   XCAL_HIP(hipGetLastError());
   XCAL_HIP(hipMemcpyAsync(S.e->case_host.data(), c->red_result.p, 2 * sizeof(double) * K,
                           hipMemcpyDeviceToHost, S.s));
@@ -657,7 +673,8 @@ static void bir_solve(xcal_eqn* e, const xcal_solver_setup* st, int itmax, doubl
   BSolver<float> S32 = batch_solver32(e);
   const int K = S.ncase;
   float* r32 = e->work32.p + 11 * S.n;
-  const double inner_rtol = st->ir_inner_rtol > 0 ? st->ir_inner_rtol : 1e-4;
+  // an inner solve need not be tighter than the outer tolerance
+  const double inner_rtol = std::max(st->ir_inner_rtol > 0 ? st->ir_inner_rtol : 1e-4, st->rtol);
   std::vector<double> rr, bb, res(K), prev(K, INFINITY), tol(K);
   bresidual_sums(S, x, b, r32, rr, bb);
   std::vector<int> frozen(K, 0);
@@ -668,12 +685,16 @@ static void bir_solve(xcal_eqn* e, const xcal_solver_setup* st, int itmax, doubl
     res[k] = std::sqrt(rr[k]);
     tol[k] = st->atol + st->rtol * res[k];
   }
+  const bool single = st->rtol >= kIrSingleSweep;  // one FP32 solve (see ir_solve)
   for (int sweep = 0;; ++sweep) {
     bool any = false;
     for (int k = 0; k < K; ++k) {
       if (frozen[k]) continue;
       if (res[k] <= tol[k]) {
         frozen[k] = 1;
+      } else if (single && sweep == 1) {
+        frozen[k] = 1;
+        rcs[k] = niter[k] >= itmax ? 1 : 0;
       } else if (niter[k] >= itmax || nouter[k] >= 100 || res[k] >= 0.5 * prev[k]) {
         frozen[k] = 1;
         rcs[k] = 1;
@@ -684,11 +705,12 @@ static void bir_solve(xcal_eqn* e, const xcal_solver_setup* st, int itmax, doubl
     if (!any) break;
     float* d = nullptr;
     S32.pred = &e->batch_pred[4 + std::min(sweep, 12)];
+    const double ia = single ? st->atol : 0.0;
     if (st->solver == XCAL_SOLVER_CG)
-      bcg(S32, st->precon, (const float*)r32, (float*)nullptr, itmax, 0.0, inner_rtol, frozen.data(),
+      bcg(S32, st->precon, (const float*)r32, (float*)nullptr, itmax, ia, inner_rtol, frozen.data(),
           &d);
     else
-      bbicg(S32, st->precon, (const float*)r32, (float*)nullptr, itmax, 0.0, inner_rtol,
+      bbicg(S32, st->precon, (const float*)r32, (float*)nullptr, itmax, ia, inner_rtol,
             frozen.data(), &d);
     const KS* h = (const KS*)e->kstate_host;
     std::vector<int32_t> active(K);

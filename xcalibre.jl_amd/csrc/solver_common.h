@@ -6,6 +6,9 @@
 
 namespace xcal {
 
+// mixed precision: outer tolerances at least this loose take one FP32 solve
+constexpr double kIrSingleSweep = 1e-3;
+
 enum : int { STOP_NONE = 0, STOP_SOLVED = 1, STOP_ITMAX = 2, STOP_BREAKDOWN = 3,
              STOP_ZERO_RHS = 4 };
 
@@ -243,6 +246,46 @@ __device__ inline void bwd_row(const BwdK<T>& W, int32_t pos, int64_t e0, int32_
   if (DOT == 1 || DOT == 2) s0 += dd(W.dv[i], zi);
   if (DOT == 3) { s0 += dd(zi, W.dv[i]); s1 += dd(zi, zi); }
   if (DOT == 4) { s0 += dd(zi, zi); s1 += dd(W.dv[i], zi); }
+}
+
+// ----------------------------------------------------- kernel timing hook
+inline hipEvent_t prof_event(xcal_ctx* c) {
+  if (c->prof_pool.empty()) {
+    hipEvent_t ev;
+    XCAL_HIP(hipEventCreate(&ev));
+    return ev;
+  }
+  hipEvent_t ev = c->prof_pool.back();
+  c->prof_pool.pop_back();
+  return ev;
+}
+
+// Launch `fn` on the context stream, bracketed by events when kernel `k` is
+// the one being timed.
+template <class Fn>
+inline void timed(xcal_ctx* c, int k, Fn&& fn) {
+  if (c->prof_kernel != k) {
+    fn();
+    return;
+  }
+  hipEvent_t a = prof_event(c), b = prof_event(c);
+  XCAL_HIP(hipEventRecord(a, c->stream));
+  fn();
+  XCAL_HIP(hipEventRecord(b, c->stream));
+  c->prof_pending.emplace_back(a, b);
+}
+
+// after a stream synchronisation: fold completed event pairs into the totals
+inline void prof_resolve(xcal_ctx* c) {
+  for (auto& pr : c->prof_pending) {
+    float ms = 0.0f;
+    XCAL_HIP(hipEventElapsedTime(&ms, pr.first, pr.second));
+    c->prof_ms += ms;
+    c->prof_launches += 1;
+    c->prof_pool.push_back(pr.first);
+    c->prof_pool.push_back(pr.second);
+  }
+  c->prof_pending.clear();
 }
 
 }  // namespace xcal

@@ -35,6 +35,91 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "SIMPLE cell-iters/s FP64 vs mixed@1e-8; batched solves/s + p50/p99 latency"
 SEED_JITTER, SEED_PERM = 0x5EED0002, 0x5EED0003
 SEED_BATCH = 0x5EED0004
+SEED_SERVE = 0x5EED0005
+
+
+def step_case(api, x, ratio, cells, iterations, U_in=10.0, nu=0.05):
+    """2D backward-facing step from the block mesher: step height `ratio` of
+    the outlet height 1, inlet length 1, outlet length 4, about `cells` cells.
+    Laminar SIMPLE with inlet U = (U_in, 0, 0), outlet p = 0, no-slip walls."""
+    ny = max(4, int(round(np.sqrt(cells / (5.0 - ratio)))))
+    ny_low = max(1, int(round(ratio * ny)))
+    ny_up = max(1, ny - ny_low)
+    mesh = x.mesh.backward_facing_step_2d(ratio, 1.0, 1.0, 4.0, nx_in=ny, nx_out=4 * ny,
+                                          ny_low=ny_low, ny_up=ny_up)
+    model = api.Physics(domain=mesh, fluid=api.Fluid(nu=nu))
+    BCs = api.assign(region=mesh,
+                     U=[api.Dirichlet("inlet", [U_in, 0.0, 0.0]), api.Extrapolated("outlet"),
+                        api.Wall("walls", [0, 0, 0]), api.Wall("top", [0, 0, 0])],
+                     p=[api.Extrapolated("inlet"), api.Dirichlet("outlet", 0.0),
+                        api.Extrapolated("walls"), api.Extrapolated("top")])
+    solvers = dict(U=api.SolverSetup(solver=api.Bicgstab(), preconditioner=api.DILU(),
+                                     convergence=1e-6, relax=0.7),
+                   p=api.SolverSetup(solver=api.Cg(), preconditioner=api.DILU(),
+                                     convergence=1e-6, relax=0.3))
+    config = api.Configuration(
+        solvers=solvers, schemes=dict(U=api.Schemes(divergence=api.Upwind), p=api.Schemes()),
+        runtime=api.Runtime(iterations=iterations, write_interval=-1),
+        hardware=api.Hardware(backend=api.HIPBackend(0)), boundaries=BCs)
+    return mesh, model, config
+
+
+def serve_cases(nreq, seed=SEED_SERVE):
+    """Step-height ratio ~ U[0.3, 0.6], cells log-uniform in [4k, 250k]."""
+    rng = np.random.default_rng(seed)
+    ratio = rng.uniform(0.3, 0.6, nreq)
+    cells = np.exp(rng.uniform(np.log(4e3), np.log(2.5e5), nreq))
+    return ratio, cells
+
+
+def run_serve(args, x):
+    """Serving form: Poisson arrivals at a fixed rate, continuous admission."""
+    api, H = x.api, x.hip
+    ratio, cells = serve_cases(args.requests)
+    t0 = time.perf_counter()
+    cases = [step_case(api, x, r, c, args.max_iterations) for r, c in zip(ratio, cells)]
+    setups = [api.simple_setup(model, config) for _, model, config in cases]
+    t_mesh = time.perf_counter() - t0
+    rng = np.random.default_rng(SEED_SERVE + 1)
+    gaps = rng.exponential(1.0 / args.rate, args.requests)
+    srv = H.Server(0, slots=args.slots, hbm_budget=int(args.hbm_gb * (1 << 30)),
+                   max_iterations=args.max_iterations)
+    ids = []
+    t0 = time.perf_counter()
+    for (mesh, model, _), setup, gap in zip(cases, setups, gaps):
+        if ids:
+            time.sleep(gap)  # exponential inter-arrival gap
+        ids.append(srv.submit(mesh, setup, model.momentum.U, model.momentum.p))
+        print(f"submitted {len(ids)}/{args.requests} at {time.perf_counter() - t0:.2f} s",
+              file=sys.stderr, flush=True)
+    while not srv.wait(0, 30.0):
+        print(f"waiting ... {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    wall = time.perf_counter() - t0
+    st = [srv.status(i) for i in ids]
+    lat = np.array([s["t_end"] - s["t_submit"] for s in st])
+    its = np.array([s["iterations"] for s in st])
+    ncell = np.array([s["cells"] for s in st])
+    conv = int(np.sum(its < args.max_iterations))
+    srv.close()
+    line = {
+        "metric": METRIC, "value": round(float(np.sum(ncell * its)) / wall, 1),
+        "unit": "cell-iters/s", "n_gpus": 1, "steps": int(np.sum(its)), "warmup": 0,
+        "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded step ratios, sizes and arrival times)",
+        "config": {"workload": f"serving {args.requests} 2D backward-facing steps, step ratio "
+                               f"U[0.3,0.6], cells log-uniform [4k,250k], laminar (nu "
+                               f"0.05, U 10), Poisson arrivals at {args.rate}/s, residual 1e-6",
+                   "slots": args.slots, "hbm_budget_gb": args.hbm_gb,
+                   "max_iterations": args.max_iterations, "parallelism": "single GPU"},
+        "roofline": None, "cpu_baseline": None,
+        "detail": {"mesh_build_s": round(t_mesh, 2), "makespan_s": round(wall, 2),
+                   "latency_p50_s": round(float(np.percentile(lat, 50)), 3),
+                   "latency_p99_s": round(float(np.percentile(lat, 99)), 3),
+                   "requests_per_s": round(args.requests / wall, 3),
+                   "converged": conv, "iterations": its.tolist(), "cells": ncell.tolist()},
+    }
+    print(json.dumps(line))
 
 
 def cavity(api, mesh, iterations, precision="fp64"):
@@ -188,8 +273,15 @@ def main():
     ap.add_argument("--inner-rtol", type=float, default=1e-4,
                     help="fp32-ir: relative tolerance of each FP32 inner solve")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--mode", default="channel", choices=["channel", "batch"],
-                    help="channel: the 1M-cell FP64 channel; batch: batched cavities")
+    ap.add_argument("--mode", default="channel", choices=["channel", "batch", "serve"],
+                    help="channel: the 1M-cell FP64 channel; batch: batched cavities; "
+                         "serve: Poisson-arrival serving of step cases")
+    ap.add_argument("--requests", type=int, default=16, help="serve mode: number of requests")
+    ap.add_argument("--rate", type=float, default=2.0, help="serve mode: arrivals per second")
+    ap.add_argument("--slots", type=int, default=4, help="serve mode: concurrent slots")
+    ap.add_argument("--hbm-gb", type=float, default=64.0, help="serve mode: HBM budget (GiB)")
+    ap.add_argument("--max-iterations", type=int, default=1000,
+                    help="serve mode: SIMPLE iterations per request at most")
     ap.add_argument("--cases", type=int, default=256, help="batch mode: number of cases")
     ap.add_argument("--cpu-steps", type=int, default=2)
     args = ap.parse_args()
@@ -202,6 +294,9 @@ def main():
     x = ge.load_package()
     if args.mode == "batch":
         run_batch(args, x)
+        return
+    if args.mode == "serve":
+        run_serve(args, x)
         return
     api, H = x.api, x.hip
     N = args.cells

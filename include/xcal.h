@@ -315,6 +315,42 @@ int xcal_simple_set_case_nu(xcal_simple* s, const double* nu);
  * [3] pressure solve, [4] gradients/interpolation/flux/corrections, [5] total */
 int xcal_simple_stage_times(const xcal_simple* s, double* ms6);
 
+/* Serving (no single reference counterpart: it runs many independent
+ * run!(model, config) cases).  Requests are built on the host at submission,
+ * wait in a FIFO queue and are admitted to one of `slots` slots (each its own
+ * HIP stream, so in-flight cases overlap on the device) when their device
+ * footprint fits the HBM budget left by the cases in flight.  A case stays
+ * resident until it converges (the run!/simple! convergence test) or reaches
+ * max_iterations; its U and p are then kept on the host until released.
+ * A case's result does not depend on which other cases are in flight. */
+typedef struct xcal_server xcal_server;
+typedef struct {
+  int32_t device;
+  int32_t slots;          /* 1..16 */
+  int64_t hbm_budget;     /* bytes of device memory the cases may occupy */
+  int32_t max_iterations; /* SIMPLE iterations per case at most */
+} xcal_server_config;
+enum { XCAL_REQ_QUEUED = 0, XCAL_REQ_RUNNING = 1, XCAL_REQ_DONE = 2, XCAL_REQ_FAILED = 3 };
+typedef struct {
+  int32_t state;
+  int32_t iterations;   /* SIMPLE iterations performed */
+  int64_t cells;
+  int64_t footprint;    /* admitted device bytes (estimate) */
+  double t_submit, t_start, t_end; /* seconds since server creation */
+} xcal_request_status;
+int xcal_server_create(const xcal_server_config* cfg, xcal_server** out);
+/* waits for the cases in flight; queued cases are dropped */
+int xcal_server_destroy(xcal_server* s);
+/* U0 (3*n_cells, x/y/z blocks) and p0 (n_cells) may be NULL (zero start) */
+int xcal_server_submit(xcal_server* s, const xcal_mesh_host* mesh, const xcal_simple_setup* setup,
+                       const double* U0, const double* p0, int64_t* id);
+int xcal_server_poll(xcal_server* s, int64_t id, xcal_request_status* status);
+/* id > 0: wait for that request; id <= 0: until nothing is queued or running.
+ * timeout_s <= 0 waits without limit; returns 2 on timeout. */
+int xcal_server_wait(xcal_server* s, int64_t id, double timeout_s);
+int xcal_server_fetch(xcal_server* s, int64_t id, int32_t which, double* host);
+int xcal_server_release(xcal_server* s, int64_t id);
+
 /* Kernel timing with HIP events on the context's stream (no reference
  * counterpart; measurement support).  While enabled, every launch of the
  * selected pressure-solve kernel is bracketed by an event pair:

@@ -75,6 +75,20 @@ class SimpleSetupC(C.Structure):
                 ("bc_U", C.POINTER(BC)), ("bc_p", C.POINTER(BC)), ("n_bcs", C.c_int32)]
 
 
+class ServerConfigC(C.Structure):
+    _fields_ = [("device", C.c_int32), ("slots", C.c_int32), ("hbm_budget", C.c_int64),
+                ("max_iterations", C.c_int32)]
+
+
+class RequestStatusC(C.Structure):
+    _fields_ = [("state", C.c_int32), ("iterations", C.c_int32), ("cells", C.c_int64),
+                ("footprint", C.c_int64), ("t_submit", C.c_double), ("t_start", C.c_double),
+                ("t_end", C.c_double)]
+
+
+REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED = 0, 1, 2, 3
+
+
 class SolveStats(C.Structure):
     _fields_ = [("niter", C.c_int32), ("nouter", C.c_int32), ("residual", C.c_double)]
 

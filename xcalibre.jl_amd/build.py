@@ -30,7 +30,7 @@ HIP_FLAGS = [
     "-Wall", "-Wno-unused-function", f"-I{INC}", f"-I{CSRC}",
 ]
 HIP_SOURCES = ["core.hip", "assemble.hip", "calculate.hip", "krylov.hip", "solver.hip", "batch.hip",
-               "simple.hip"]
+               "simple.hip", "serve.hip"]
 HEADERS = ["xcal_internal.h", "device_common.h"]
 
 
@@ -63,7 +63,8 @@ def build_hip(jobs: int = 8) -> Path:
         objs = list(ex.map(compile_one, HIP_SOURCES))
     out = PKG / "libxcalhip.so"
     tmp = out.with_suffix(".so.tmp")
-    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *map(str, objs), "-o", str(tmp)])
+    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *map(str, objs), "-o", str(tmp),
+          "-lpthread"])
     os.replace(tmp, out)
     return out
 

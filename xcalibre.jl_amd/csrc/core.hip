@@ -716,20 +716,55 @@ static xcal_mesh* upload_mesh(xcal_ctx* ctx, HostMesh& M) {
   return m.release();
 }
 
-static xcal_mesh* create_mesh(xcal_ctx* ctx, const xcal_mesh_host* const* hosts, int ncase,
-                              bool batch) {
-  XCAL_CHECK(ctx && hosts && ncase >= 1, "NULL argument or empty batch");
+// A mesh (or batch) built on the host, ready to upload.
+struct PreparedMesh {
+  HostMesh U;
+  std::vector<int64_t> cc, cb, ci;
+  std::vector<int32_t> lf, lb;
+  int ncase = 1;
+  bool batch = false;
+};
+
+std::shared_ptr<PreparedMesh> prepare_mesh(const xcal_mesh_host* const* hosts, int ncase, bool batch) {
+  XCAL_CHECK(hosts && ncase >= 1, "NULL argument or empty batch");
   std::vector<HostMesh> C(ncase);
   for (int k = 0; k < ncase; ++k) {
     XCAL_CHECK(hosts[k], "NULL mesh in batch");
     build_host_mesh(hosts[k], C[k]);
   }
-  HostMesh U;
-  std::vector<int64_t> cc, cb, ci;
-  std::vector<int32_t> lf, lb;
-  union_cases(C, U, cc, cb, ci, lf, lb);
-  xcal_mesh* m = upload_mesh(ctx, U);
+  auto P = std::make_shared<PreparedMesh>();
+  union_cases(C, P->U, P->cc, P->cb, P->ci, P->lf, P->lb);
+  P->ncase = ncase;
+  P->batch = batch;
+  return P;
+}
+
+int64_t prepared_cells(const PreparedMesh& P) { return P.U.n; }
+
+// Device bytes of a prepared mesh plus a steady SIMPLE run on it (fields,
+// the U and p equations with DILU, FP64 and FP32 Krylov workspaces).
+int64_t prepared_footprint(const PreparedMesh& P) {
+  const HostMesh& M = P.U;
+  int64_t mesh = 0;
+  mesh += 8 * (int64_t)(M.vol.size() + 13 * M.area.size());
+  mesh += 4 * (int64_t)(M.cf_ptr.size() + M.cf_face.size() * 3 + M.diag.size() + 3 * M.own.size() +
+                        M.bc_cells.size() + M.bc_ptr.size() + M.bc_face.size() + 2 * M.rbeg.size() +
+                        2 * M.colval.size() + 2 * M.rows_b.size() + 2 * M.Lcol.size() +
+                        2 * M.Ucol.size());
+  const int64_t n = M.n, nf = M.nf, nz = M.nzlen;
+  const int64_t fields = 8 * (22 * n + 7 * nf);
+  const int64_t eqns = 8 * (3 * nz + 4 * n) + 8 * 2 * (nz + 3 * n);
+  const int64_t krylov = 2 * 8 * 12 * n + 4 * (12 * n + nz + 3 * n);
+  return mesh + fields + eqns + krylov;
+}
+
+xcal_mesh* upload_prepared(xcal_ctx* ctx, PreparedMesh& P) {
+  xcal_mesh* m = upload_mesh(ctx, P.U);
   std::unique_ptr<xcal_mesh> guard(m);
+  const int ncase = P.ncase;
+  const bool batch = P.batch;
+  const std::vector<int64_t>&cc = P.cc, &cb = P.cb, &ci = P.ci;
+  const std::vector<int32_t>&lf = P.lf, &lb = P.lb;
   m->ncase = ncase;
   m->batch = batch;
   m->case_cell = cc;
@@ -764,6 +799,13 @@ static xcal_mesh* create_mesh(xcal_ctx* ctx, const xcal_mesh_host* const* hosts,
   m->device_bytes += bytes;
   make_sweep_plans(m, 0);
   return guard.release();
+}
+
+static xcal_mesh* create_mesh(xcal_ctx* ctx, const xcal_mesh_host* const* hosts, int ncase,
+                              bool batch) {
+  XCAL_CHECK(ctx, "NULL context");
+  auto P = prepare_mesh(hosts, ncase, batch);
+  return upload_prepared(ctx, *P);
 }
 
 }  // namespace xcal

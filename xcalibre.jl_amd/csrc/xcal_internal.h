@@ -242,6 +242,12 @@ struct BCsK {
 };
 
 void make_sweep_plans(xcal_mesh* m, int small);
+// meshes built on the host first (server admission), uploaded later
+struct PreparedMesh;
+std::shared_ptr<PreparedMesh> prepare_mesh(const xcal_mesh_host* const* hosts, int ncase, bool batch);
+xcal_mesh* upload_prepared(xcal_ctx* ctx, PreparedMesh& P);
+int64_t prepared_footprint(const PreparedMesh& P);
+int64_t prepared_cells(const PreparedMesh& P);
 void upload_permuted(const xcal_mesh* m, const std::vector<int32_t>& perm, double* dev,
                      const double* host, int64_t len);
 // host[i] = dev[perm[i]] for i < len; dev holds dev_len entries (default len)

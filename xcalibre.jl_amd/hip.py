@@ -49,6 +49,14 @@ def lib():
                                         C.POINTER(A.SolveStats)],
             "xcal_simple_run_batch": [_vp, C.c_int32, _pd, _pi32, _pi32],
             "xcal_simple_set_case_nu": [_vp, _pd],
+            "xcal_server_create": [C.POINTER(A.ServerConfigC), C.POINTER(_vp)],
+            "xcal_server_destroy": [_vp],
+            "xcal_server_submit": [_vp, C.POINTER(A.MeshHost), C.POINTER(A.SimpleSetupC), _pd, _pd,
+                                   C.POINTER(C.c_int64)],
+            "xcal_server_poll": [_vp, C.c_int64, C.POINTER(A.RequestStatusC)],
+            "xcal_server_wait": [_vp, C.c_int64, C.c_double],
+            "xcal_server_fetch": [_vp, C.c_int64, C.c_int32, _pd],
+            "xcal_server_release": [_vp, C.c_int64],
             "xcal_field_create": [_vp, C.c_int32, C.POINTER(_vp)],
             "xcal_field_destroy": [_vp],
             "xcal_field_upload": [_vp, C.c_int32, _pd],
@@ -372,3 +380,56 @@ class Simple:
         if getattr(self, "h", None) and _lib is not None:
             _lib.xcal_simple_destroy(self.h)
             self.h = None
+
+
+class Server:
+    """Serving engine (xcal_server_*): continuous admission of independent
+    steady SIMPLE cases into `slots` concurrent slots under an HBM budget."""
+
+    def __init__(self, device=0, slots=4, hbm_budget=64 << 30, max_iterations=1000):
+        cfg = A.ServerConfigC(device, slots, int(hbm_budget), max_iterations)
+        self.h = _vp()
+        check(lib().xcal_server_create(C.byref(cfg), C.byref(self.h)))
+        self._cells = {}
+
+    def submit(self, mesh, setup: A.SimpleSetupC, U0=None, p0=None) -> int:
+        host = A.mesh_host(mesh)
+        u = None if U0 is None else np.ascontiguousarray(np.asarray(U0, np.float64).reshape(-1))
+        p = None if p0 is None else np.ascontiguousarray(np.asarray(p0, np.float64).reshape(-1))
+        rid = C.c_int64()
+        check(lib().xcal_server_submit(self.h, C.byref(host), C.byref(setup),
+                                       None if u is None else _pd_of(u),
+                                       None if p is None else _pd_of(p), C.byref(rid)))
+        self._cells[rid.value] = mesh.n_cells
+        return rid.value
+
+    def status(self, rid: int) -> dict:
+        st = A.RequestStatusC()
+        check(lib().xcal_server_poll(self.h, rid, C.byref(st)))
+        return dict(state=st.state, iterations=st.iterations, cells=st.cells,
+                    footprint=st.footprint, t_submit=st.t_submit, t_start=st.t_start,
+                    t_end=st.t_end)
+
+    def wait(self, rid: int = 0, timeout: float = 0.0) -> bool:
+        """True when finished, False on timeout."""
+        return check(lib().xcal_server_wait(self.h, rid, float(timeout))) == 0
+
+    def fetch(self, rid: int):
+        n = self._cells[rid]
+        U = np.zeros(3 * n)
+        p = np.zeros(n)
+        check(lib().xcal_server_fetch(self.h, rid, 0, _pd_of(U)))
+        check(lib().xcal_server_fetch(self.h, rid, 1, _pd_of(p)))
+        return U.reshape(3, n), p
+
+    def release(self, rid: int):
+        check(lib().xcal_server_release(self.h, rid))
+        self._cells.pop(rid, None)
+
+    def close(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.xcal_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()

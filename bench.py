@@ -38,30 +38,67 @@ SEED_BATCH = 0x5EED0004
 SEED_SERVE = 0x5EED0005
 
 
-def step_case(api, x, ratio, cells, iterations, U_in=10.0, nu=0.05):
+def step_case(api, x, ratio, cells, iterations, U_in=10.0, nu=None, rans=True, Tu=0.05,
+              nut_ratio=100.0, precon="DILU"):
     """2D backward-facing step from the block mesher: step height `ratio` of
     the outlet height 1, inlet length 1, outlet length 4, about `cells` cells.
-    Laminar SIMPLE with inlet U = (U_in, 0, 0), outlet p = 0, no-slip walls."""
+    Inlet U = (U_in, 0, 0), outlet p = 0, walls.  rans: k-omega with wall
+    functions, inlet k = 1.5 (Tu U_in)^2 and omega = k / (nut_ratio nu), every
+    field started from its inlet value (nu 1e-5 by default); otherwise laminar
+    from rest (nu 0.05 by default).  All solves BiCGSTAB (CG for p) with the
+    `precon` preconditioner, upwind convection, residual target 1e-6."""
+    nu = nu if nu is not None else (1e-5 if rans else 0.05)
     ny = max(4, int(round(np.sqrt(cells / (5.0 - ratio)))))
     ny_low = max(1, int(round(ratio * ny)))
     ny_up = max(1, ny - ny_low)
     mesh = x.mesh.backward_facing_step_2d(ratio, 1.0, 1.0, 4.0, nx_in=ny, nx_out=4 * ny,
                                           ny_low=ny_low, ny_up=ny_up)
-    model = api.Physics(domain=mesh, fluid=api.Fluid(nu=nu))
-    BCs = api.assign(region=mesh,
-                     U=[api.Dirichlet("inlet", [U_in, 0.0, 0.0]), api.Extrapolated("outlet"),
-                        api.Wall("walls", [0, 0, 0]), api.Wall("top", [0, 0, 0])],
-                     p=[api.Extrapolated("inlet"), api.Dirichlet("outlet", 0.0),
-                        api.Extrapolated("walls"), api.Extrapolated("top")])
-    solvers = dict(U=api.SolverSetup(solver=api.Bicgstab(), preconditioner=api.DILU(),
-                                     convergence=1e-6, relax=0.7),
-                   p=api.SolverSetup(solver=api.Cg(), preconditioner=api.DILU(),
-                                     convergence=1e-6, relax=0.3))
+    model = api.Physics(domain=mesh, fluid=api.Fluid(nu=nu),
+                        turbulence=api.RANS(api.KOmega) if rans else api.RANS(api.Laminar))
+    bc = dict(U=[api.Dirichlet("inlet", [U_in, 0.0, 0.0]), api.Extrapolated("outlet"),
+                 api.Wall("walls", [0, 0, 0]), api.Wall("top", [0, 0, 0])])
+
+    def S(solver, relax, rtol=None):
+        tol = {} if rtol is None else dict(rtol=rtol, atol=1e-10)
+        return api.SolverSetup(solver=solver, preconditioner=getattr(api, precon)(), convergence=1e-6,
+                               relax=relax, **tol)
+    if rans:
+        k_in = 1.5 * (Tu * U_in) ** 2
+        w_in = k_in / (nut_ratio * nu)
+        for f, v in ((model.momentum.U, [U_in, 0.0, 0.0]), (model.turbulence.k, k_in),
+                     (model.turbulence.omega, w_in), (model.turbulence.nut, k_in / w_in)):
+            api.initialise(f, v)
+        bc.update(
+            p=[api.Neumann("inlet", 0.0), api.Dirichlet("outlet", 0.0), api.Wall("walls"),
+               api.Wall("top")],
+            k=[api.Dirichlet("inlet", k_in), api.Extrapolated("outlet"),
+               api.KWallFunction("walls"), api.KWallFunction("top")],
+            omega=[api.Dirichlet("inlet", w_in), api.Extrapolated("outlet"),
+                   api.OmegaWallFunction("walls"), api.OmegaWallFunction("top")],
+            nut=[api.Dirichlet("inlet", k_in / w_in), api.Extrapolated("outlet"),
+                 api.NutWallFunction("walls"), api.NutWallFunction("top")])
+        solvers = dict(U=S(api.Bicgstab(), 0.7, 1e-2), p=S(api.Cg(), 0.3, 1e-3),
+                       k=S(api.Bicgstab(), 0.7, 1e-2), omega=S(api.Bicgstab(), 0.7, 1e-2))
+    else:
+        bc.update(p=[api.Extrapolated("inlet"), api.Dirichlet("outlet", 0.0),
+                     api.Extrapolated("walls"), api.Extrapolated("top")])
+        solvers = dict(U=S(api.Bicgstab(), 0.7), p=S(api.Cg(), 0.3))
     config = api.Configuration(
-        solvers=solvers, schemes=dict(U=api.Schemes(divergence=api.Upwind), p=api.Schemes()),
+        solvers=solvers, schemes={f: api.Schemes(divergence=api.Upwind) for f in solvers},
         runtime=api.Runtime(iterations=iterations, write_interval=-1),
-        hardware=api.Hardware(backend=api.HIPBackend(0)), boundaries=BCs)
+        hardware=api.Hardware(backend=api.HIPBackend(0)), boundaries=api.assign(region=mesh, **bc))
     return mesh, model, config
+
+
+def submit_case(srv, api, mesh, model, config):
+    """Submit one step case to a Server (turbulent cases carry k, omega, nut)."""
+    setup = api.simple_setup(model, config)
+    t = model.turbulence
+    if getattr(t, "k", None) is None:
+        return srv.submit(mesh, setup, model.momentum.U, model.momentum.p)
+    return srv.submit(mesh, setup, model.momentum.U, model.momentum.p,
+                      turbulence=api.turbulence_setup(model, config), k0=t.k, omega0=t.omega,
+                      nut0=t.nut)
 
 
 def serve_cases(nreq, seed=SEED_SERVE):
@@ -77,8 +114,8 @@ def run_serve(args, x):
     api, H = x.api, x.hip
     ratio, cells = serve_cases(args.requests)
     t0 = time.perf_counter()
-    cases = [step_case(api, x, r, c, args.max_iterations) for r, c in zip(ratio, cells)]
-    setups = [api.simple_setup(model, config) for _, model, config in cases]
+    cases = [step_case(api, x, r, c, args.max_iterations, rans=not args.laminar, precon=args.precon)
+             for r, c in zip(ratio, cells)]
     t_mesh = time.perf_counter() - t0
     rng = np.random.default_rng(SEED_SERVE + 1)
     gaps = rng.exponential(1.0 / args.rate, args.requests)
@@ -86,20 +123,23 @@ def run_serve(args, x):
                    max_iterations=args.max_iterations)
     ids = []
     t0 = time.perf_counter()
-    for (mesh, model, _), setup, gap in zip(cases, setups, gaps):
+    for (mesh, model, config), gap in zip(cases, gaps):
         if ids:
             time.sleep(gap)  # exponential inter-arrival gap
-        ids.append(srv.submit(mesh, setup, model.momentum.U, model.momentum.p))
+        ids.append(submit_case(srv, api, mesh, model, config))
         print(f"submitted {len(ids)}/{args.requests} at {time.perf_counter() - t0:.2f} s",
               file=sys.stderr, flush=True)
     while not srv.wait(0, 30.0):
-        print(f"waiting ... {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+        done = sum(srv.status(i)["state"] >= x._abi.REQ_DONE for i in ids)
+        print(f"{done}/{len(ids)} finished at {time.perf_counter() - t0:.1f} s", file=sys.stderr,
+              flush=True)
     wall = time.perf_counter() - t0
     st = [srv.status(i) for i in ids]
     lat = np.array([s["t_end"] - s["t_submit"] for s in st])
     its = np.array([s["iterations"] for s in st])
     ncell = np.array([s["cells"] for s in st])
-    conv = int(np.sum(its < args.max_iterations))
+    ok = np.array([s["state"] == x._abi.REQ_DONE for s in st])
+    conv = int(np.sum(ok & (its < args.max_iterations)))
     srv.close()
     line = {
         "metric": METRIC, "value": round(float(np.sum(ncell * its)) / wall, 1),
@@ -108,16 +148,18 @@ def run_serve(args, x):
         "dtype": "f64",
         "data": "synthetic (seeded step ratios, sizes and arrival times)",
         "config": {"workload": f"serving {args.requests} 2D backward-facing steps, step ratio "
-                               f"U[0.3,0.6], cells log-uniform [4k,250k], laminar (nu "
-                               f"0.05, U 10), Poisson arrivals at {args.rate}/s, residual 1e-6",
-                   "slots": args.slots, "hbm_budget_gb": args.hbm_gb,
+                               f"U[0.3,0.6], cells log-uniform [4k,250k], "
+                               + ("laminar (nu 0.05" if args.laminar else "k-omega RANS (nu 1e-5")
+                               + f", U 10), Poisson arrivals at {args.rate}/s, residual 1e-6",
+                   "slots": args.slots, "hbm_budget_gb": args.hbm_gb, "precon": args.precon,
                    "max_iterations": args.max_iterations, "parallelism": "single GPU"},
         "roofline": None, "cpu_baseline": None,
         "detail": {"mesh_build_s": round(t_mesh, 2), "makespan_s": round(wall, 2),
                    "latency_p50_s": round(float(np.percentile(lat, 50)), 3),
                    "latency_p99_s": round(float(np.percentile(lat, 99)), 3),
                    "requests_per_s": round(args.requests / wall, 3),
-                   "converged": conv, "iterations": its.tolist(), "cells": ncell.tolist()},
+                   "converged": conv, "failed": int(np.sum(~ok)), "iterations": its.tolist(), "cells": ncell.tolist(),
+                   "run_s": [round(s["t_end"] - s["t_start"], 3) for s in st]},
     }
     print(json.dumps(line))
 
@@ -282,6 +324,10 @@ def main():
     ap.add_argument("--hbm-gb", type=float, default=64.0, help="serve mode: HBM budget (GiB)")
     ap.add_argument("--max-iterations", type=int, default=1000,
                     help="serve mode: SIMPLE iterations per request at most")
+    ap.add_argument("--precon", default="DILU", choices=["DILU", "Jacobi"],
+                    help="serve mode: preconditioner of every solve")
+    ap.add_argument("--laminar", action="store_true",
+                    help="serve mode: laminar cases instead of k-omega RANS")
     ap.add_argument("--cases", type=int, default=256, help="batch mode: number of cases")
     ap.add_argument("--cpu-steps", type=int, default=2)
     args = ap.parse_args()

@@ -109,7 +109,15 @@ enum {
   XCAL_BC_ZEROGRADIENT = 2,
   XCAL_BC_EXTRAPOLATED = 3,
   XCAL_BC_WALL = 4,
-  XCAL_BC_SYMMETRY = 5
+  XCAL_BC_SYMMETRY = 5,
+  /* wall functions of the k-omega model (Discretise_0_types.jl:38-160) with
+   * the reference's default coefficients kappa 0.41, beta1 0.075, cmu 0.09,
+   * B 5.2, E 9.8.  Assembly: KWallFunction as Zerogradient, OmegaWallFunction
+   * as Extrapolated (Discretise_3_boundary_conditions.jl); face values are
+   * zero-gradient (zerogradient_interpolation.jl:1). */
+  XCAL_BC_KWALL = 6,
+  XCAL_BC_OMEGAWALL = 7,
+  XCAL_BC_NUTWALL = 8
 };
 typedef struct {
   int32_t kind;
@@ -289,11 +297,30 @@ typedef struct {
   int32_t n_bcs;
 } xcal_simple_setup;
 
-enum { XCAL_SIMPLE_U = 0, XCAL_SIMPLE_P = 1, XCAL_SIMPLE_MDOTF = 2 };
+enum { XCAL_SIMPLE_U = 0, XCAL_SIMPLE_P = 1, XCAL_SIMPLE_MDOTF = 2, XCAL_SIMPLE_K = 3,
+       XCAL_SIMPLE_OMEGA = 4, XCAL_SIMPLE_NUT = 5 };
+
+/* RANS{KOmega} (RANS_kOmega.jl:41-247): k and omega transport with
+ * production from the strain rate, wall functions, bound! and nut = k/omega;
+ * the momentum Laplacian then uses nueff = nu + nutf (update_nueff!,
+ * Solvers_0_functions.jl:5-36).  Turbulence residuals join the convergence
+ * test of the SIMPLE loop (Solvers_1_SIMPLE.jl:226-233). */
+enum { XCAL_TURB_LAMINAR = 0, XCAL_TURB_KOMEGA = 1 };
+typedef struct {
+  int32_t model;
+  double beta_star, alpha1, beta1, sigma_k, sigma_omega; /* 0.09, 0.52, 0.072, 0.5, 0.5 */
+  int32_t div_scheme;   /* XCAL_DIV_* of the k and omega convection terms */
+  xcal_solver_setup k, omega;
+  const xcal_bc* bc_k;     /* n_patches entries each */
+  const xcal_bc* bc_omega;
+  const xcal_bc* bc_nut;
+} xcal_turbulence_setup;
 
 int xcal_simple_create(xcal_mesh* mesh, const xcal_simple_setup* setup, xcal_simple** out);
 int xcal_simple_destroy(xcal_simple* s);
-/* U: 3*n_cells (x block, y block, z block); p: n_cells */
+/* turbulence model of a SIMPLE run (laminar unless set) */
+int xcal_simple_set_turbulence(xcal_simple* s, const xcal_turbulence_setup* t);
+/* U: 3*n_cells (x block, y block, z block); p, k, omega, nut: n_cells */
 int xcal_simple_set_field(xcal_simple* s, int32_t which, const double* host);
 int xcal_simple_get_field(const xcal_simple* s, int32_t which, double* host);
 /* Run up to `iterations` SIMPLE iterations (stops early on convergence as
@@ -321,7 +348,7 @@ int xcal_simple_stage_times(const xcal_simple* s, double* ms6);
  * HIP stream, so in-flight cases overlap on the device) when their device
  * footprint fits the HBM budget left by the cases in flight.  A case stays
  * resident until it converges (the run!/simple! convergence test) or reaches
- * max_iterations; its U and p are then kept on the host until released.
+ * max_iterations; its fields are then kept on the host until released.
  * A case's result does not depend on which other cases are in flight. */
 typedef struct xcal_server xcal_server;
 typedef struct {
@@ -341,13 +368,17 @@ typedef struct {
 int xcal_server_create(const xcal_server_config* cfg, xcal_server** out);
 /* waits for the cases in flight; queued cases are dropped */
 int xcal_server_destroy(xcal_server* s);
-/* U0 (3*n_cells, x/y/z blocks) and p0 (n_cells) may be NULL (zero start) */
+/* turb: NULL or XCAL_TURB_LAMINAR for a laminar case.  Initial fields may be
+ * NULL (zero start): U0 3*n_cells (x/y/z blocks), p0, k0, omega0, nut0 n_cells
+ * (the last three only with a turbulence model). */
 int xcal_server_submit(xcal_server* s, const xcal_mesh_host* mesh, const xcal_simple_setup* setup,
-                       const double* U0, const double* p0, int64_t* id);
+                       const xcal_turbulence_setup* turb, const double* U0, const double* p0,
+                       const double* k0, const double* omega0, const double* nut0, int64_t* id);
 int xcal_server_poll(xcal_server* s, int64_t id, xcal_request_status* status);
 /* id > 0: wait for that request; id <= 0: until nothing is queued or running.
  * timeout_s <= 0 waits without limit; returns 2 on timeout. */
 int xcal_server_wait(xcal_server* s, int64_t id, double timeout_s);
+/* which: XCAL_SIMPLE_U, _P, or (turbulent cases) _K, _OMEGA, _NUT */
 int xcal_server_fetch(xcal_server* s, int64_t id, int32_t which, double* host);
 int xcal_server_release(xcal_server* s, int64_t id);
 

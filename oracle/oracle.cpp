@@ -218,12 +218,15 @@ void bc_coeffs(const orc_mesh& m, const xcal_bc& bc, const orc_term& t, int64_t 
       }
       return;
     case XCAL_BC_ZEROGRADIENT:  // zerogradient.jl:25-76
+    case XCAL_BC_KWALL:         // KWallFunction, Discretise_3_boundary_conditions.jl:11-95
+    case XCAL_BC_NUTWALL:
       if (divLUL) ap_out = sign * J;
       else if (divBU) {
         double ap = sign * J;
         ap_out = ap - J;
       }
       return;
+    case XCAL_BC_OMEGAWALL:       // OmegaWallFunction, same file :27-123
     case XCAL_BC_EXTRAPOLATED:  // extrapolated.jl:21-72
       if (lap) {
         double flux = ((-J) * area) / delta;
@@ -793,7 +796,10 @@ int orc_correct_boundaries(const orc_mesh* mp, const xcal_bc* bcs, int32_t nbc,
             }
             break;
           case XCAL_BC_ZEROGRADIENT:
-          case XCAL_BC_EXTRAPOLATED:  // zerogradient_interpolation.jl
+          case XCAL_BC_KWALL:
+          case XCAL_BC_OMEGAWALL:
+          case XCAL_BC_NUTWALL:
+          case XCAL_BC_EXTRAPOLATED:  // zerogradient_interpolation.jl:1
             fx[fID] = phix[c];
             if (vec) {
               fy[fID] = phiy[c];
@@ -967,9 +973,151 @@ int orc_correct_mass_flux(const orc_mesh* mp, const double* pnz, const double* p
   });
 }
 
-int orc_simple_run(const orc_mesh* mp, const xcal_simple_setup* S, double* Ux, double* Uy,
-                   double* Uz, double* p, int32_t iterations, double* residuals, int32_t* done,
-                   int32_t* kit, double* mdot_out) {
+
+// ------------------------------------------------------------- k-omega
+// turbulence!(KOmegaModel) (RANS_kOmega.jl:175-235) with the wall functions
+// of RANS_functions.jl; rho = 1.  Wall-function face loops run in ascending
+// boundary-face order (patch order), so the last wall face of a cell wins.
+namespace {
+
+int patch_of(const orc_mesh& m, int64_t f) {
+  for (size_t p = 0; p < m.pfirst.size(); ++p)
+    if (f >= m.pfirst[p] && f <= m.plast[p]) return (int)p;
+  return -1;
+}
+
+struct Wall {  // KWallFunction/OmegaWallFunction/NutWallFunction defaults
+  double kappa = 0.41, beta1 = 0.075, cmu = 0.09, E = 9.8, cmu25, yPlusLam, nu;
+  explicit Wall(double nu_) : nu(nu_) {
+    cmu25 = std::pow(cmu, 0.25);
+    double yL = 11.0;  // y_plus_laminar (RANS_functions.jl:40-43)
+    for (int i = 0; i < 10; ++i) yL = std::log(std::max(yL * E, 1.0)) / kappa;
+    yPlusLam = yL;
+  }
+  double yplus(double k, double y) const { return ((cmu25 * y) * std::sqrt(k)) / nu; }
+  double nutw(double yp) const {
+    return jmax(nu * ((yp * kappa) / std::log(jmax(E * yp, 1.0 + 1e-4)) - 1.0), 0.0);
+  }
+};
+
+// bound! (RANS_functions.jl:2-38), from the values before the call
+void bound(const orc_mesh& m, double* v, double* tmp) {
+  const int64_t n = m.n;
+  std::copy(v, v + n, tmp);
+  const double mzero = std::numeric_limits<double>::epsilon();
+  for (int64_t i = 0; i < n; ++i) {
+    double sum = 0.0;
+    int cnt = 0;
+    for (int64_t k = m.cf_first[i]; k <= m.cf_last[i]; ++k) {
+      sum += jmax(tmp[m.cell_nbr[k]], mzero);
+      ++cnt;
+    }
+    const double average = sum / (double)cnt;
+    const double a = std::signbit(tmp[i]) ? average : std::copysign(0.0, average);
+    v[i] = jmax(jmax(tmp[i], a), mzero);
+  }
+}
+
+}  // namespace
+
+static bool komega_step(const orc_mesh& m, const xcal_simple_setup& S,
+                        const xcal_turbulence_setup& T, const double* Ux, const double* Uy,
+                        const double* Uz, const double* mdotf, double* const* gU, double* k,
+                        double* w, double* nut, double* nutf, double* Pk, double* Pw, double* Dk,
+                        double* Dw, double* mk, double* mw, double* tmp, double* A, double* b) {
+  const int64_t n = m.n, nf = m.nf;
+  const int nb = S.n_bcs;
+  const Wall W(S.nu);
+  auto chk = [](int rc) {
+    if (rc < 0) throw std::runtime_error(g_err);
+  };
+  // magnitude2!(Pk, S, 2) with S = (gradU + gradU')/2, then Pw, Pk (:193-197)
+  for (int64_t i = 0; i < n; ++i) {
+    double sum = 0.0;
+    for (int j = 0; j < 3; ++j)
+      for (int q = 0; q < 3; ++q) {
+        const double sj = 0.5 * (gU[3 * j + q][i] + gU[3 * q + j][i]);
+        sum += sj * sj;
+      }
+    const double P = sum * 2.0;
+    Pw[i] = (1.0 * T.alpha1) * P;
+    Pk[i] = (1.0 * nut[i]) * P;
+  }
+  // correct_production! (RANS_functions.jl:62-124), Uw = 0
+  for (int64_t f = 0; f < m.nbf; ++f) {
+    const int p = patch_of(m, f);
+    if (p < 0 || T.bc_k[p].kind != XCAL_BC_KWALL) continue;
+    const int64_t c = m.bcell[f];
+    const double delta = m.delta[f];
+    const double uStar = W.cmu25 * std::sqrt(k[c]);
+    const double dUdy = uStar / (W.kappa * delta);
+    const double yp = W.yplus(k[c], delta);
+    const double nutw = W.nutw(yp);
+    const V3 nrm = m.fn[f];
+    const double dx = Ux[c] - 0.0, dy = Uy[c] - 0.0, dz = Uz[c] - 0.0;
+    const double dn = dx * nrm.x + dy * nrm.y + dz * nrm.z;
+    const double gx = (dx - dn * nrm.x) / delta, gy = (dy - dn * nrm.y) / delta,
+                 gz = (dz - dn * nrm.z) / delta;
+    const double mag = std::sqrt(gx * gx + gy * gy + gz * gz);
+    Pk[c] = yp > W.yPlusLam ? ((W.nu + nutw) * mag) * dUdy : 0.0;
+  }
+  for (int64_t i = 0; i < n; ++i) {
+    Dw[i] = (1.0 * T.beta1) * w[i];
+    Dk[i] = (1.0 * T.beta_star) * w[i];
+  }
+  for (int64_t f = 0; f < nf; ++f) {
+    mw[f] = 1.0 * (S.nu + T.sigma_omega * nutf[f]);
+    mk[f] = 1.0 * (S.nu + T.sigma_k * nutf[f]);
+  }
+  auto transport = [&](double* phi, double* D, double* mu, double* P, const xcal_bc* bc,
+                       const xcal_solver_setup& ss, bool constrain) {
+    orc_term t[4] = {{XCAL_TIME_STEADY, 1, nullptr, 1.0},
+                     {T.div_scheme, 1, mdotf, 0.0},
+                     {XCAL_LAPLACIAN_LINEAR, -1, mu, 0.0},
+                     {XCAL_SI, 1, D, 0.0}};
+    orc_source src[1] = {{P, nullptr, nullptr, 1}};
+    chk(orc_discretise_scalar(&m, t, 4, src, 1, phi, 1.0, A, b));
+    chk(orc_apply_bcs(&m, t, 4, bc, nb, -1, phi, nullptr, nullptr, A, b));
+    orc_relax_implicit_diagdom(&m, A, b, phi, ss.relax);
+    if (constrain) {  // constrain! (RANS_functions.jl:262-356)
+      for (int64_t f = 0; f < m.nbf; ++f) {
+        const int p = patch_of(m, f);
+        if (p < 0 || bc[p].kind != XCAL_BC_OMEGAWALL) continue;
+        const int64_t c = m.bcell[f];
+        const double y = m.delta[f];
+        const double wvis = (6.0 * W.nu) / (W.beta1 * (y * y));
+        const double wlog = std::sqrt(k[c]) / ((W.cmu25 * W.kappa) * y);
+        const double wc = W.yplus(k[c], y) > W.yPlusLam ? wlog : wvis;
+        for (int64_t q = m.rowptr[c]; q < m.rowptr[c + 1]; ++q) A[q] = 0.0;
+        A[spindex(m, c, c)] = 1.0;
+        b[c] = wc;
+      }
+    }
+    int32_t it = 0;
+    chk(orc_krylov(&m, ss.solver, ss.precon, A, b, phi, ss.itmax, ss.atol, ss.rtol, &it));
+    const double r = orc_residual(&m, A, b, phi);
+    bound(m, phi, tmp);
+    return r;
+  };
+  const double rw = transport(w, Dw, mw, Pw, T.bc_omega, T.omega, true);
+  const double rk = transport(k, Dk, mk, Pk, T.bc_k, T.k, false);
+  for (int64_t i = 0; i < n; ++i) nut[i] = k[i] / w[i];
+  orc_interpolate(&m, nut, nutf);
+  chk(orc_correct_boundaries(&m, T.bc_nut, nb, nut, nullptr, nullptr, nutf, nullptr, nullptr));
+  for (int64_t f = 0; f < m.nbf; ++f) {  // correct_eddy_viscosity! (RANS_functions.jl:126-200)
+    const int p = patch_of(m, f);
+    if (p < 0 || T.bc_nut[p].kind != XCAL_BC_NUTWALL) continue;
+    const int64_t c = m.bcell[f];
+    const double yp = W.yplus(k[c], m.delta[f]);
+    nutf[f] = yp > W.yPlusLam ? W.nutw(yp) : 0.0;
+  }
+  return rk < T.k.convergence && rw < T.omega.convergence;
+}
+
+int orc_simple_run_turb(const orc_mesh* mp, const xcal_simple_setup* S,
+                        const xcal_turbulence_setup* T, double* Ux, double* Uy, double* Uz,
+                        double* p, double* kf_, double* wf_, double* nut_, int32_t iterations,
+                        double* residuals, int32_t* done, int32_t* kit, double* mdot_out) {
   return guard([&] {
     const orc_mesh& m = *mp;
     const int64_t n = m.n, nf = m.nf, nnz = m.rowptr[n];
@@ -1003,11 +1151,18 @@ int orc_simple_run(const orc_mesh* mp, const xcal_simple_setup* S, double* Ux, d
       orc_interpolate(&m, z, Ufz.data());
       chk(orc_correct_boundaries(&m, S->bc_U, nb, x, y, z, Ufx.data(), Ufy.data(), Ufz.data()));
     };
+    // k-omega state (RANS_kOmega.jl): nutf starts at zero
+    const bool turb = T && T->model == XCAL_TURB_KOMEGA;
+    std::vector<double> nutf(nf, 0.0), Pk(n), Pw(n), Dk(n), Dw(n), mk(nf), mw(nf), tmp(n);
+    const double nu = S->nu;
+    auto update_nueff = [&] {  // update_nueff! (Solvers_0_functions.jl:5-36)
+      for (int64_t f = 0; f < nf; ++f) nueff[f] = turb ? nu + nutf[f] : nu;
+    };
     // initial calculations (Solvers_1_SIMPLE.jl:150-156)
     interp_vec(Ux, Uy, Uz);
     orc_flux(&m, Ufx.data(), Ufy.data(), Ufz.data(), mdotf.data());
     grad_p();
-    for (auto& v : nueff) v = S->nu;
+    update_nueff();
     const bool is3d = m.dim == 3;
     int it = 0;
     double* U[3] = {Ux, Uy, Uz};
@@ -1061,10 +1216,15 @@ int orc_simple_run(const orc_mesh* mp, const xcal_simple_setup* S, double* Ux, d
         Uy[i] = Hy[i] - gpy[i] * r;
         Uz[i] = Hz[i] - gpz[i] * r;
       }
-      // turbulence! laminar (RANS_laminar.jl:73-79): tensor gradient of U
+      // turbulence!: grad!(gradU) (RANS_laminar.jl:73-79, RANS_kOmega.jl:192)
       interp_vec(Ux, Uy, Uz);
       orc_green_gauss_tensor(&m, Ufx.data(), Ufy.data(), Ufz.data(), gU);
-      for (auto& v : nueff) v = S->nu;  // update_nueff!
+      bool turb_conv = true;
+      if (turb)
+        turb_conv = komega_step(m, *S, *T, Ux, Uy, Uz, mdotf.data(), gU, kf_, wf_, nut_,
+                                nutf.data(), Pk.data(), Pw.data(), Dk.data(), Dw.data(), mk.data(),
+                                mw.data(), tmp.data(), pA.data(), pb.data());
+      update_nueff();
       residuals[it] = res[0];
       residuals[iterations + it] = res[1];
       residuals[2 * iterations + it] = res[2];
@@ -1077,7 +1237,7 @@ int orc_simple_run(const orc_mesh* mp, const xcal_simple_setup* S, double* Ux, d
       }
       bool zc = is3d ? res[2] <= S->U.convergence : true;
       if (res[0] <= S->U.convergence && res[1] <= S->U.convergence && zc &&
-          rp <= S->p.convergence) {
+          rp <= S->p.convergence && turb_conv) {
         ++it;
         break;
       }
@@ -1088,4 +1248,11 @@ int orc_simple_run(const orc_mesh* mp, const xcal_simple_setup* S, double* Ux, d
   });
 }
 
+}
+
+int orc_simple_run(const orc_mesh* mp, const xcal_simple_setup* S, double* Ux, double* Uy,
+                   double* Uz, double* p, int32_t iterations, double* residuals, int32_t* done,
+                   int32_t* kit, double* mdot_out) {
+  return orc_simple_run_turb(mp, S, nullptr, Ux, Uy, Uz, p, nullptr, nullptr, nullptr, iterations,
+                             residuals, done, kit, mdot_out);
 }  // extern "C"

@@ -120,6 +120,12 @@ int orc_correct_mass_flux(const orc_mesh* m, const double* p_nzval, const double
 int orc_simple_run(const orc_mesh* m, const xcal_simple_setup* setup, double* Ux, double* Uy,
                    double* Uz, double* p, int32_t iterations, double* residuals,
                    int32_t* done, int32_t* krylov_iters, double* mdotf);
+/* the same with the k-omega model (RANS_kOmega.jl:175-235) when t is given;
+ * k, omega, nut (n_cells each) are in/out */
+int orc_simple_run_turb(const orc_mesh* m, const xcal_simple_setup* setup,
+                        const xcal_turbulence_setup* t, double* Ux, double* Uy, double* Uz,
+                        double* p, double* k, double* omega, double* nut, int32_t iterations,
+                        double* residuals, int32_t* done, int32_t* krylov_iters, double* mdotf);
 
 #ifdef __cplusplus
 }

@@ -100,6 +100,8 @@ def lib():
                                             C.c_int32, _pd]
         L.orc_simple_run.argtypes = [C.c_void_p, C.c_void_p, _pd, _pd, _pd, _pd, C.c_int32, _pd,
                                      _pi32, _pi32, _pd]
+        L.orc_simple_run_turb.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, _pd, _pd, _pd, _pd,
+                                          _pd, _pd, _pd, C.c_int32, _pd, _pi32, _pi32, _pd]
         L.orc_set_threads.argtypes = [C.c_int32]
         _lib = L
     return _lib
@@ -274,18 +276,28 @@ class OracleMesh:
         _chk(lib().orc_correct_mass_flux(self.h, _p(p_nzval), _p(p), _p(pflux), psign, bcs,
                                          len(bcs), _p(mdot)))
 
-    def simple_run(self, setup_c, U, p, iterations):
-        """setup_c: _abi.SimpleSetupC; U: (3, n) array (modified copy returned)."""
+    def simple_run(self, setup_c, U, p, iterations, turb=None, k=None, omega=None, nut=None):
+        """setup_c: _abi.SimpleSetupC; U: (3, n) array (modified copy returned).
+        turb: _abi.TurbulenceSetupC for the k-omega model, with initial k, omega, nut."""
         Ux, Uy, Uz = (np.array(U[i], dtype=np.float64, copy=True) for i in range(3))
         pp = np.array(p, dtype=np.float64, copy=True)
         res = np.zeros(4 * iterations)
         kit = np.zeros(4 * iterations, np.int32)
         done = C.c_int32()
         mdot = np.zeros(self.mesh.n_faces)
-        _chk(lib().orc_simple_run(self.h, C.byref(setup_c), _p(Ux), _p(Uy), _p(Uz), _p(pp),
-                                  iterations, _p(res), C.byref(done),
-                                  kit.ctypes.data_as(_pi32), _p(mdot)))
+        out = {}
+        if turb is None:
+            _chk(lib().orc_simple_run(self.h, C.byref(setup_c), _p(Ux), _p(Uy), _p(Uz), _p(pp),
+                                      iterations, _p(res), C.byref(done),
+                                      kit.ctypes.data_as(_pi32), _p(mdot)))
+        else:
+            kk, ww, nn = (np.array(v, dtype=np.float64, copy=True) for v in (k, omega, nut))
+            _chk(lib().orc_simple_run_turb(self.h, C.byref(setup_c), C.byref(turb), _p(Ux),
+                                           _p(Uy), _p(Uz), _p(pp), _p(kk), _p(ww), _p(nn),
+                                           iterations, _p(res), C.byref(done),
+                                           kit.ctypes.data_as(_pi32), _p(mdot)))
+            out = dict(k=kk, omega=ww, nut=nn)
         d = done.value
         return dict(U=np.stack([Ux, Uy, Uz]), p=pp, mdotf=mdot, done=d,
                     residuals=res.reshape(4, iterations)[:, :d],
-                    krylov_iters=kit.reshape(4, iterations)[:, :d])
+                    krylov_iters=kit.reshape(4, iterations)[:, :d], **out)

@@ -10,6 +10,7 @@ TIME_STEADY, TIME_EULER, LAPLACIAN_LINEAR, DIV_LINEAR, DIV_UPWIND, DIV_LUST, \
     DIV_BOUNDED_UPWIND, SI = range(8)
 # boundary condition kinds
 BC_DIRICHLET, BC_NEUMANN, BC_ZEROGRADIENT, BC_EXTRAPOLATED, BC_WALL, BC_SYMMETRY = range(6)
+BC_KWALL, BC_OMEGAWALL, BC_NUTWALL = 6, 7, 8
 # solvers / preconditioners / precision
 SOLVER_CG, SOLVER_BICGSTAB = 0, 1
 PRECON_NONE, PRECON_JACOBI, PRECON_DILU = 0, 1, 2
@@ -87,6 +88,17 @@ class RequestStatusC(C.Structure):
 
 
 REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED = 0, 1, 2, 3
+
+
+class TurbulenceSetupC(C.Structure):
+    _fields_ = [("model", C.c_int32), ("beta_star", C.c_double), ("alpha1", C.c_double),
+                ("beta1", C.c_double), ("sigma_k", C.c_double), ("sigma_omega", C.c_double),
+                ("div_scheme", C.c_int32), ("k", SolverSetupC), ("omega", SolverSetupC),
+                ("bc_k", C.POINTER(BC)), ("bc_omega", C.POINTER(BC)), ("bc_nut", C.POINTER(BC))]
+
+
+TURB_LAMINAR, TURB_KOMEGA = 0, 1
+SIMPLE_U, SIMPLE_P, SIMPLE_MDOTF, SIMPLE_K, SIMPLE_OMEGA, SIMPLE_NUT = 0, 1, 2, 3, 4, 5
 
 
 class SolveStats(C.Structure):

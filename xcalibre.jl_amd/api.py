@@ -145,10 +145,29 @@ class Fluid:
     nu: float = 0.0
 
 
-def RANS(kind=Laminar):
-    if kind is not Laminar:
-        raise NotImplementedError("only RANS{Laminar} is on the device path")
-    return kind()
+@dataclass
+class KOmega:
+    """RANS{KOmega} coefficients (RANS_kOmega.jl:41-45); k, omega, nut are
+    cell fields allocated by Physics."""
+    beta_star: float = 0.09
+    alpha1: float = 0.52
+    beta1: float = 0.072
+    sigma_k: float = 0.5
+    sigma_omega: float = 0.5
+    k: np.ndarray = None
+    omega: np.ndarray = None
+    nut: np.ndarray = None
+
+
+def RANS(kind=Laminar, **coeffs):
+    """RANS{Laminar}() or RANS{KOmega}(; β⁺, α1, β1, σk, σω)."""
+    if kind is Laminar:
+        return Laminar()
+    if kind is KOmega:
+        names = {"β⁺": "beta_star", "α1": "alpha1", "β1": "beta1", "σk": "sigma_k",
+                 "σω": "sigma_omega"}
+        return KOmega(**{names.get(k, k): float(v) for k, v in coeffs.items()})
+    raise NotImplementedError(f"RANS{{{kind}}} is not on the device path")
 
 
 def Energy(kind=Isothermal):
@@ -173,6 +192,9 @@ class Physics:
     def __post_init__(self):
         n = self.domain.n_cells
         self.momentum = Momentum(np.zeros((3, n)), np.zeros(n))
+        if isinstance(self.turbulence, KOmega):
+            t = self.turbulence
+            t.k, t.omega, t.nut = np.zeros(n), np.zeros(n), np.zeros(n)
 
 
 def initialise(field_values: np.ndarray, value):
@@ -200,6 +222,10 @@ def Zerogradient(name): return _BC(name, 0.0, A.BC_ZEROGRADIENT)
 def Extrapolated(name): return _BC(name, 0.0, A.BC_EXTRAPOLATED)
 def Wall(name, value=0.0): return _BC(name, value, A.BC_WALL)
 def Symmetry(name): return _BC(name, 0.0, A.BC_SYMMETRY)
+# wall functions with the reference's default coefficients (Discretise_0_types.jl:63,108,154)
+def KWallFunction(name): return _BC(name, 0.0, A.BC_KWALL)
+def OmegaWallFunction(name): return _BC(name, 0.0, A.BC_OMEGAWALL)
+def NutWallFunction(name): return _BC(name, 0.0, A.BC_NUTWALL)
 
 
 def assign(region, **fields):
@@ -227,8 +253,8 @@ def assign(region, **fields):
 
 def simple_setup(model: Physics, config: Configuration, pref=None) -> A.SimpleSetupC:
     """xcal_simple_setup for run!(model, config; pref)."""
-    if type(model.turbulence) is not Laminar:
-        raise NotImplementedError("only laminar SIMPLE is on the device path")
+    if not isinstance(model.turbulence, (Laminar, KOmega)):
+        raise NotImplementedError("laminar and k-omega SIMPLE are on the device path")
     sU, sp = config.schemes["U"], config.schemes["p"]
     for s in (sU, sp):
         if s.time is not SteadyState:
@@ -245,6 +271,25 @@ def simple_setup(model: Physics, config: Configuration, pref=None) -> A.SimpleSe
                            bU, bp, len(config.boundaries["U"]))
     setup._keep = (bU, bp)
     return setup
+
+
+def turbulence_setup(model: Physics, config: Configuration):
+    """xcal_turbulence_setup of RANS{KOmega} (None for laminar)."""
+    t = model.turbulence
+    if not isinstance(t, KOmega):
+        return None
+    sk = config.schemes.get("k", Schemes())
+    div = _DIV.get(sk.divergence)
+    if div is None:
+        raise NotImplementedError(f"divergence scheme {sk.divergence!r}")
+    bk = A.bc_array(config.boundaries["k"])
+    bw = A.bc_array(config.boundaries["omega"])
+    bn = A.bc_array(config.boundaries["nut"])
+    ts = A.TurbulenceSetupC(A.TURB_KOMEGA, t.beta_star, t.alpha1, t.beta1, t.sigma_k,
+                            t.sigma_omega, div, config.solvers["k"].c(),
+                            config.solvers["omega"].c(), bk, bw, bn)
+    ts._keep = (bk, bw, bn)
+    return ts
 
 
 _ctx_cache = {}
@@ -270,8 +315,18 @@ def run(model: Physics, config: Configuration, pref=None, ncorrectors: int = 0,
     sim = hip.Simple(dmesh, simple_setup(model, config, pref))
     sim.set_U(model.momentum.U)
     sim.set_p(model.momentum.p)
+    ts = turbulence_setup(model, config)
+    if ts is not None:
+        sim.set_turbulence(ts)
+        t = model.turbulence
+        sim.set_scalar("k", t.k)
+        sim.set_scalar("omega", t.omega)
+        sim.set_scalar("nut", t.nut)
     out = sim.run(config.runtime.iterations)
     model.momentum.U[:] = sim.get("U")
     model.momentum.p[:] = sim.get("p")
+    if ts is not None:
+        for name in ("k", "omega", "nut"):
+            getattr(model.turbulence, name)[:] = sim.get(name)
     r = out["residuals"]
     return dict(Ux=r[0], Uy=r[1], Uz=r[2], p=r[3])

@@ -49,6 +49,9 @@ __global__ void __launch_bounds__(kBlock)
       break;
     case XCAL_BC_ZEROGRADIENT:
     case XCAL_BC_EXTRAPOLATED:
+    case XCAL_BC_KWALL:
+    case XCAL_BC_OMEGAWALL:
+    case XCAL_BC_NUTWALL:
       fx[f] = px[c];
       if (vec) { fy[f] = py[c]; fz[f] = pz[c]; }
       break;

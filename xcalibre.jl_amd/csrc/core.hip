@@ -41,7 +41,7 @@ BCsK make_bcs(const xcal_mesh* m, const xcal_bc* bcs, int n) {
   for (int p = 0; p < kMaxPatches; ++p) b.p[p].kind = -1;
   for (int i = 0; i < n; ++i) {
     XCAL_CHECK(bcs[i].patch >= 0 && bcs[i].patch < m->n_patches, "BC patch index out of range");
-    XCAL_CHECK(bcs[i].kind >= XCAL_BC_DIRICHLET && bcs[i].kind <= XCAL_BC_SYMMETRY,
+    XCAL_CHECK(bcs[i].kind >= XCAL_BC_DIRICHLET && bcs[i].kind <= XCAL_BC_NUTWALL,
                "unknown boundary condition kind");
     // the reference applies the first BC whose range holds the face
     if (b.p[bcs[i].patch].kind != -1) continue;
@@ -743,7 +743,7 @@ int64_t prepared_cells(const PreparedMesh& P) { return P.U.n; }
 
 // Device bytes of a prepared mesh plus a steady SIMPLE run on it (fields,
 // the U and p equations with DILU, FP64 and FP32 Krylov workspaces).
-int64_t prepared_footprint(const PreparedMesh& P) {
+int64_t prepared_footprint(const PreparedMesh& P, bool turb) {
   const HostMesh& M = P.U;
   int64_t mesh = 0;
   mesh += 8 * (int64_t)(M.vol.size() + 13 * M.area.size());
@@ -755,7 +755,8 @@ int64_t prepared_footprint(const PreparedMesh& P) {
   const int64_t fields = 8 * (22 * n + 7 * nf);
   const int64_t eqns = 8 * (3 * nz + 4 * n) + 8 * 2 * (nz + 3 * n);
   const int64_t krylov = 2 * 8 * 12 * n + 4 * (12 * n + nz + 3 * n);
-  return mesh + fields + eqns + krylov;
+  const int64_t turbulence = turb ? 8 * (8 * n + 3 * nf) : 0;
+  return mesh + fields + eqns + krylov + turbulence;
 }
 
 xcal_mesh* upload_prepared(xcal_ctx* ctx, PreparedMesh& P) {

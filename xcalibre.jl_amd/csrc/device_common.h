@@ -219,9 +219,12 @@ __device__ inline void bc_coeffs(const MeshK& m, const BCK& bc, const TermK& t, 
       }
       return;
     case XCAL_BC_ZEROGRADIENT:  // zerogradient.jl:25-76
+    case XCAL_BC_KWALL:         // Discretise_3_boundary_conditions.jl:11-95
+    case XCAL_BC_NUTWALL:
       if (divLUL) ap_out = sign * J;
       else if (divBU) ap_out = sign * J - J;
       return;
+    case XCAL_BC_OMEGAWALL:       // Discretise_3_boundary_conditions.jl:27-123
     case XCAL_BC_EXTRAPOLATED: {  // extrapolated.jl:21-72
       if (lap) {
         double flux = ((-J) * m.area[f]) / m.delta[f];

@@ -29,8 +29,11 @@ struct Request {
   std::shared_ptr<PreparedMesh> mesh;
   xcal_simple_setup setup{};
   std::vector<xcal_bc> bcU, bcp;
-  std::vector<double> U0, p0;  // initial fields (empty: zero)
-  std::vector<double> U, p;    // results
+  bool turb = false;
+  xcal_turbulence_setup tset{};
+  std::vector<xcal_bc> bcK, bcW, bcN;
+  // initial fields indexed by XCAL_SIMPLE_* (empty: zero start), then results
+  std::vector<double> f0[6], f[6];
   int64_t footprint = 0, cells = 0;
   int32_t state = XCAL_REQ_QUEUED, iterations = 0;
   double t_submit = 0, t_start = 0, t_end = 0;
@@ -66,19 +69,28 @@ void run_request(xcal_ctx* ctx, const xcal_server_config& cfg, Request& r) {
   int rc = xcal_simple_create(m, &s, &sim);
   if (rc) throw Error(rc, xcal_last_error());
   std::unique_ptr<xcal_simple, int (*)(xcal_simple*)> sg(sim, xcal_simple_destroy);
-  if (!r.U0.empty() && (rc = xcal_simple_set_field(sim, XCAL_SIMPLE_U, r.U0.data())))
-    throw Error(rc, xcal_last_error());
-  if (!r.p0.empty() && (rc = xcal_simple_set_field(sim, XCAL_SIMPLE_P, r.p0.data())))
-    throw Error(rc, xcal_last_error());
+  simple_stop_nonfinite(sim, true);
+  if (r.turb) {
+    xcal_turbulence_setup t = r.tset;
+    t.bc_k = r.bcK.data();
+    t.bc_omega = r.bcW.data();
+    t.bc_nut = r.bcN.data();
+    if ((rc = xcal_simple_set_turbulence(sim, &t))) throw Error(rc, xcal_last_error());
+  }
+  for (int w = 0; w < 6; ++w)
+    if (!r.f0[w].empty() && (rc = xcal_simple_set_field(sim, w, r.f0[w].data())))
+      throw Error(rc, xcal_last_error());
   std::vector<double> res(4 * (size_t)cfg.max_iterations);
   int32_t done = 0;
   if ((rc = xcal_simple_run(sim, cfg.max_iterations, res.data(), &done, nullptr)))
     throw Error(rc, xcal_last_error());
   r.iterations = done;
-  r.U.resize(3 * (size_t)r.cells);
-  r.p.resize((size_t)r.cells);
-  if ((rc = xcal_simple_get_field(sim, XCAL_SIMPLE_U, r.U.data()))) throw Error(rc, xcal_last_error());
-  if ((rc = xcal_simple_get_field(sim, XCAL_SIMPLE_P, r.p.data()))) throw Error(rc, xcal_last_error());
+  if (simple_diverged(sim)) throw Error(XCAL_ERR_STATE, "diverged: non-finite residual");
+  for (int w : {XCAL_SIMPLE_U, XCAL_SIMPLE_P, XCAL_SIMPLE_K, XCAL_SIMPLE_OMEGA, XCAL_SIMPLE_NUT}) {
+    if (w >= XCAL_SIMPLE_K && !r.turb) break;
+    r.f[w].resize((w == XCAL_SIMPLE_U ? 3 : 1) * (size_t)r.cells);
+    if ((rc = xcal_simple_get_field(sim, w, r.f[w].data()))) throw Error(rc, xcal_last_error());
+  }
 }
 
 void worker(xcal_server* S) {
@@ -159,21 +171,35 @@ int xcal_server_destroy(xcal_server* S) {
 }
 
 int xcal_server_submit(xcal_server* S, const xcal_mesh_host* mesh, const xcal_simple_setup* setup,
-                       const double* U0, const double* p0, int64_t* id) {
+                       const xcal_turbulence_setup* turb, const double* U0, const double* p0,
+                       const double* k0, const double* omega0, const double* nut0, int64_t* id) {
   return guarded([&] {
     XCAL_CHECK(S && mesh && setup && id, "NULL argument");
     XCAL_CHECK(setup->n_bcs == mesh->n_patches, "one boundary condition per patch is required");
+    const bool rans = turb && turb->model != XCAL_TURB_LAMINAR;
+    XCAL_CHECK(!rans || (turb->bc_k && turb->bc_omega && turb->bc_nut),
+               "k, omega and nut need boundary conditions");
+    XCAL_CHECK(rans || !(k0 || omega0 || nut0), "k, omega and nut fields need a turbulence model");
     auto r = std::make_shared<Request>();
     r->t_submit = S->now();
     r->mesh = prepare_mesh(&mesh, 1, false);  // host-side build happens at submission
     r->cells = prepared_cells(*r->mesh);
-    r->footprint = prepared_footprint(*r->mesh);
+    r->footprint = prepared_footprint(*r->mesh, rans);
     XCAL_CHECK(r->footprint <= S->cfg.hbm_budget, "case does not fit the server's HBM budget");
     r->setup = *setup;
     r->bcU.assign(setup->bc_U, setup->bc_U + setup->n_bcs);
     r->bcp.assign(setup->bc_p, setup->bc_p + setup->n_bcs);
-    if (U0) r->U0.assign(U0, U0 + 3 * r->cells);
-    if (p0) r->p0.assign(p0, p0 + r->cells);
+    if (rans) {
+      const int np = mesh->n_patches;
+      r->turb = true;
+      r->tset = *turb;
+      r->bcK.assign(turb->bc_k, turb->bc_k + np);
+      r->bcW.assign(turb->bc_omega, turb->bc_omega + np);
+      r->bcN.assign(turb->bc_nut, turb->bc_nut + np);
+    }
+    const double* init[6] = {U0, p0, nullptr, k0, omega0, nut0};
+    for (int w = 0; w < 6; ++w)
+      if (init[w]) r->f0[w].assign(init[w], init[w] + (w == XCAL_SIMPLE_U ? 3 : 1) * r->cells);
     {
       std::lock_guard<std::mutex> lk(S->mu);
       XCAL_CHECK(!S->stop, "server is shutting down");
@@ -232,9 +258,9 @@ int xcal_server_fetch(xcal_server* S, int64_t id, int32_t which, double* host) {
     const Request& r = *it->second;
     if (r.state == XCAL_REQ_FAILED) throw Error(XCAL_ERR_STATE, "request failed: " + r.error);
     XCAL_CHECK(r.state == XCAL_REQ_DONE, "request has not finished");
-    XCAL_CHECK(which == XCAL_SIMPLE_U || which == XCAL_SIMPLE_P, "which: U or p");
-    const std::vector<double>& v = which == XCAL_SIMPLE_U ? r.U : r.p;
-    std::copy(v.begin(), v.end(), host);
+    XCAL_CHECK(which >= 0 && which < 6 && which != XCAL_SIMPLE_MDOTF, "which: U, p, k, omega or nut");
+    XCAL_CHECK(!r.f[which].empty(), "the request has no such field (laminar case)");
+    std::copy(r.f[which].begin(), r.f[which].end(), host);
     return 0;
   });
 }

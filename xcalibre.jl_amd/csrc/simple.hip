@@ -25,12 +25,24 @@ struct xcal_simple {
   hipEvent_t ev[8]{};
   double stage_ms[6]{};
   bool initialised = false;
+  // serving policy: end the run at the first non-finite residual
+  bool stop_nonfinite = false, diverged = false;
+  // k-omega model (RANS_kOmega.jl), off unless set
+  bool turb = false;
+  xcal_turbulence_setup tset{};
+  std::vector<xcal_bc> bcK, bcW, bcN;
+  BCsK bK{}, bW{}, bN{};
+  xcal_field *k = nullptr, *omega = nullptr, *nut = nullptr, *nutf = nullptr, *Pk = nullptr,
+             *Pw = nullptr, *Dk = nullptr, *Dw = nullptr, *mk = nullptr, *mw = nullptr,
+             *tmp = nullptr;
+  double res_k = 0.0, res_w = 0.0;
   // batch meshes: viscosity per case, reference cell (device id) per case
   std::vector<double> case_nu;
   xcal::DBuf<double> case_nu_d;
   xcal::DBuf<int32_t> ref_cells;
   ~xcal_simple() {
-    for (xcal_field* f : {U, p, Uf, pf, gradp, mdotf, rDf, nueff, divHv, Hv, rD, prev, gradU})
+    for (xcal_field* f : {U, p, Uf, pf, gradp, mdotf, rDf, nueff, divHv, Hv, rD, prev, gradU, k,
+                          omega, nut, nutf, Pk, Pw, Dk, Dw, mk, mw, tmp})
       if (f) xcal_field_destroy(f);
     if (Ueqn) xcal_eqn_destroy(Ueqn);
     if (peqn) xcal_eqn_destroy(peqn);
@@ -61,7 +73,9 @@ __global__ void k_set_reference_cases(const int32_t* __restrict__ diag, double* 
 
 void update_nueff(xcal_simple* S) {
   xcal_mesh* m = S->mesh;
-  if (m->batch)
+  if (S->turb)
+    turb_nueff(m, S->setup.nu, S->nutf->comp(0), S->nueff->comp(0));  // nueff = nu + nutf
+  else if (m->batch)
     hipLaunchKernelGGL(k_fill_by_case, dim3(grid_for(m->nf)), dim3(kBlock), 0, m->ctx->stream,
                        S->nueff->comp(0), m->face_case.p, S->case_nu_d.p, m->nf);
   else
@@ -111,6 +125,11 @@ void interp_vec(xcal_simple* S, const xcal_field* V) {
 }
 
 }  // namespace
+
+namespace xcal {
+void simple_stop_nonfinite(xcal_simple* S, bool on) { S->stop_nonfinite = on; }
+bool simple_diverged(const xcal_simple* S) { return S->diverged; }
+}  // namespace xcal
 
 extern "C" {
 
@@ -173,8 +192,13 @@ int xcal_simple_destroy(xcal_simple* s) {
 int xcal_simple_set_field(xcal_simple* S, int32_t which, const double* host) {
   return guarded([&] {
     XCAL_CHECK(S && host, "NULL argument");
-    xcal_field* f = which == XCAL_SIMPLE_U ? S->U : (which == XCAL_SIMPLE_P ? S->p : nullptr);
-    XCAL_CHECK(f, "only U and p can be set");
+    xcal_field* f = which == XCAL_SIMPLE_U       ? S->U
+                    : which == XCAL_SIMPLE_P     ? S->p
+                    : which == XCAL_SIMPLE_K     ? S->k
+                    : which == XCAL_SIMPLE_OMEGA ? S->omega
+                    : which == XCAL_SIMPLE_NUT   ? S->nut
+                                                 : nullptr;
+    XCAL_CHECK(f, "only U, p and (with a turbulence model) k, omega, nut can be set");
     for (int c = 0; c < f->ncomp; ++c) {
       int rc = xcal_field_upload(f, c, host + (int64_t)c * f->len);
       if (rc) throw Error(rc, xcal_last_error());
@@ -187,11 +211,14 @@ int xcal_simple_set_field(xcal_simple* S, int32_t which, const double* host) {
 int xcal_simple_get_field(const xcal_simple* S, int32_t which, double* host) {
   return guarded([&] {
     XCAL_CHECK(S && host, "NULL argument");
-    const xcal_field* f = which == XCAL_SIMPLE_U ? S->U
-                          : which == XCAL_SIMPLE_P ? S->p
+    const xcal_field* f = which == XCAL_SIMPLE_U       ? S->U
+                          : which == XCAL_SIMPLE_P     ? S->p
                           : which == XCAL_SIMPLE_MDOTF ? S->mdotf
+                          : which == XCAL_SIMPLE_K     ? S->k
+                          : which == XCAL_SIMPLE_OMEGA ? S->omega
+                          : which == XCAL_SIMPLE_NUT   ? S->nut
                                                        : nullptr;
-    XCAL_CHECK(f, "unknown field selector");
+    XCAL_CHECK(f, "unknown field selector (or no turbulence model)");
     for (int c = 0; c < f->ncomp; ++c) {
       int rc = xcal_field_download(f, c, host + (int64_t)c * f->len);
       if (rc) throw Error(rc, xcal_last_error());
@@ -203,6 +230,51 @@ int xcal_simple_get_field(const xcal_simple* S, int32_t which, double* host) {
 }  // extern "C"
 
 namespace {
+
+// turbulence!(KOmegaModel) (RANS_kOmega.jl:175-235), after the velocity
+// correction; gradU and Uf are current.  Returns whether k and omega met
+// their convergence criteria.
+bool komega_step(xcal_simple* S) {
+  xcal_mesh* m = S->mesh;
+  const xcal_turbulence_setup& T = S->tset;
+  const double nu = S->setup.nu;
+  xcal_eqn* e = S->peqn;  // k and omega share the pressure equation's storage
+  turb_production(m, S->gradU->comp(0), S->nut->comp(0), T.alpha1, S->Pk->comp(0), S->Pw->comp(0));
+  turb_wall_production(m, S->bK, nu, S->U->comp(0), S->U->comp(1), S->U->comp(2), S->k->comp(0),
+                       S->Pk->comp(0));
+  turb_coeffs(m, S->omega->comp(0), S->nutf->comp(0), nu, T.beta1, T.beta_star, T.sigma_omega,
+              T.sigma_k, S->Dw->comp(0), S->Dk->comp(0), S->mw->comp(0), S->mk->comp(0));
+  auto transport = [&](xcal_field* phi, xcal_field* D, xcal_field* mu, xcal_field* P,
+                       const BCsK& bc, const xcal_solver_setup& ss, bool constrain) {
+    xcal_term t[4] = {{XCAL_TIME_STEADY, 1, nullptr, 1.0},
+                      {T.div_scheme, 1, S->mdotf, 0.0},
+                      {XCAL_LAPLACIAN_LINEAR, -1, mu, 0.0},
+                      {XCAL_SI, 1, D, 0.0}};
+    TermsK TT = make_terms(t, 4);
+    SourcesK SS{};
+    SS.n = 1;
+    SS.sign[0] = 1;
+    SS.x[0] = P->comp(0);
+    discretise(e, TT, SS, phi, 1.0);
+    apply_bcs(e, TT, bc, phi, 0);
+    relax_diagdom(e, phi->comp(0), e->bc(0), ss.relax);
+    if (constrain) turb_omega_constrain(e, bc, nu, S->k->comp(0));
+    precond_update(e, ss.precon);
+    xcal_solve_stats st{};
+    solve_system(e, &ss, phi->comp(0), e->bc(0), &st);
+    turb_bound(m, phi->comp(0), S->tmp->comp(0));
+    return st.residual;
+  };
+  S->res_w = transport(S->omega, S->Dw, S->mw, S->Pw, S->bW, T.omega, true);
+  S->res_k = transport(S->k, S->Dk, S->mk, S->Pk, S->bK, T.k, false);
+  turb_nut(m, S->k->comp(0), S->omega->comp(0), S->nut->comp(0));
+  interpolate(m, S->nut->comp(0), S->nutf->comp(0));
+  const double* ph[1] = {S->nut->comp(0)};
+  double* fc[1] = {S->nutf->comp(0)};
+  correct_boundaries(m, S->bN, ph, fc, 1);
+  turb_nut_wall(m, S->bN, nu, S->k->comp(0), S->nutf->comp(0));
+  return S->res_k < T.k.convergence && S->res_w < T.omega.convergence;
+}
 
 // The SIMPLE loop.  R[(k*4 + f)*iterations + it]: residual of field f
 // (Ux, Uy, Uz, p) of case k at iteration it; KIT likewise for Krylov
@@ -324,9 +396,10 @@ int run_impl(xcal_simple* S, int32_t iterations, std::vector<double>& R, std::ve
       correct_velocity(m, S->U->comp(0), S->U->comp(1), S->U->comp(2), S->Hv->comp(0),
                        S->Hv->comp(1), S->Hv->comp(2), S->gradp->comp(0), S->gradp->comp(1),
                        S->gradp->comp(2), S->rD->comp(0));
-      // turbulence! laminar (RANS_laminar.jl:73-79): tensor gradient of U
+      // turbulence!: grad!(gradU) (RANS_laminar.jl:73-79, RANS_kOmega.jl:192)
       interp_vec(S, S->U);
       green_gauss_tensor(m, S->Uf->comp(0), S->Uf->comp(1), S->Uf->comp(2), S->gradU->comp(0));
+      const bool turb_conv = S->turb ? komega_step(S) : true;
       update_nueff(S);  // update_nueff!
       mark(4);
       {
@@ -340,7 +413,15 @@ int run_impl(xcal_simple* S, int32_t iterations, std::vector<double>& R, std::ve
         acc[2] += b;
         acc[3] += c;
       }
-      bool conv = true;
+      bool conv = turb_conv;
+      if (S->stop_nonfinite) {
+        for (int k = 0; k < K * 4; ++k)
+          S->diverged = S->diverged || !std::isfinite(R[(size_t)k * iterations + it]);
+        if (S->diverged) {
+          ++it;
+          break;
+        }
+      }
       for (int k = 0; k < K; ++k) {
         const double* r = &R[(size_t)k * 4 * iterations];
         const bool zc = is3d ? r[2 * iterations + it] <= cfg.U.convergence : true;
@@ -401,6 +482,56 @@ int xcal_simple_run_batch(xcal_simple* S, int32_t iterations, double* residuals,
     *done = run_impl(S, iterations, R, KIT);
     std::copy(R.begin(), R.end(), residuals);
     if (kit) std::copy(KIT.begin(), KIT.end(), kit);
+    return 0;
+  });
+}
+
+int xcal_simple_set_turbulence(xcal_simple* S, const xcal_turbulence_setup* t) {
+  return guarded([&] {
+    XCAL_CHECK(S && t, "NULL argument");
+    xcal_mesh* m = S->mesh;
+    if (t->model == XCAL_TURB_LAMINAR) {
+      S->turb = false;
+      S->initialised = false;
+      return 0;
+    }
+    XCAL_CHECK(t->model == XCAL_TURB_KOMEGA, "unknown turbulence model");
+    XCAL_CHECK(!m->batch, "turbulence models run on single meshes");
+    XCAL_CHECK(t->bc_k && t->bc_omega && t->bc_nut, "k, omega and nut need boundary conditions");
+    XCAL_CHECK(t->div_scheme >= XCAL_DIV_LINEAR && t->div_scheme <= XCAL_DIV_BOUNDED_UPWIND,
+               "k/omega divergence scheme must be a Divergence type");
+    for (const xcal_solver_setup* ss : {&t->k, &t->omega}) {
+      XCAL_CHECK(ss->solver == XCAL_SOLVER_CG || ss->solver == XCAL_SOLVER_BICGSTAB, "unknown solver");
+      XCAL_CHECK(ss->precon >= XCAL_PRECON_NONE && ss->precon <= XCAL_PRECON_DILU,
+                 "unknown preconditioner");
+      XCAL_CHECK(ss->relax > 0.0, "relaxation factor must be positive");
+    }
+    const int np = m->n_patches;
+    S->tset = *t;
+    S->bcK.assign(t->bc_k, t->bc_k + np);
+    S->bcW.assign(t->bc_omega, t->bc_omega + np);
+    S->bcN.assign(t->bc_nut, t->bc_nut + np);
+    S->tset.bc_k = S->bcK.data();
+    S->tset.bc_omega = S->bcW.data();
+    S->tset.bc_nut = S->bcN.data();
+    S->bK = make_bcs(m, S->bcK.data(), np);
+    S->bW = make_bcs(m, S->bcW.data(), np);
+    S->bN = make_bcs(m, S->bcN.data(), np);
+    if (!S->k) {
+      S->k = mk(m, XCAL_CELL_SCALAR);
+      S->omega = mk(m, XCAL_CELL_SCALAR);
+      S->nut = mk(m, XCAL_CELL_SCALAR);
+      S->nutf = mk(m, XCAL_FACE_SCALAR);
+      S->Pk = mk(m, XCAL_CELL_SCALAR);
+      S->Pw = mk(m, XCAL_CELL_SCALAR);
+      S->Dk = mk(m, XCAL_CELL_SCALAR);
+      S->Dw = mk(m, XCAL_CELL_SCALAR);
+      S->mk = mk(m, XCAL_FACE_SCALAR);
+      S->mw = mk(m, XCAL_FACE_SCALAR);
+      S->tmp = mk(m, XCAL_CELL_SCALAR);
+    }
+    S->turb = true;
+    S->initialised = false;
     return 0;
   });
 }

@@ -246,7 +246,14 @@ void make_sweep_plans(xcal_mesh* m, int small);
 struct PreparedMesh;
 std::shared_ptr<PreparedMesh> prepare_mesh(const xcal_mesh_host* const* hosts, int ncase, bool batch);
 xcal_mesh* upload_prepared(xcal_ctx* ctx, PreparedMesh& P);
-int64_t prepared_footprint(const PreparedMesh& P);
+// Serving policy of a SIMPLE object: stop a run at the first non-finite
+// residual (the reference's loop would carry on); simple_diverged reports it.
+void simple_stop_nonfinite(xcal_simple* S, bool on);
+bool simple_diverged(const xcal_simple* S);
+
+// device bytes a single-case SIMPLE run on P occupies (estimate); turb adds
+// the k-omega fields
+int64_t prepared_footprint(const PreparedMesh& P, bool turb = false);
 int64_t prepared_cells(const PreparedMesh& P);
 void upload_permuted(const xcal_mesh* m, const std::vector<int32_t>& perm, double* dev,
                      const double* host, int64_t len);
@@ -285,6 +292,19 @@ void correct_mass_flux(const xcal_eqn* pe, const double* p, const double* pflux,
 void correct_velocity(const xcal_mesh* m, double* Ux, double* Uy, double* Uz, const double* Hx,
                       const double* Hy, const double* Hz, const double* gx, const double* gy,
                       const double* gz, const double* rD);
+// k-omega model
+void turb_production(const xcal_mesh* m, const double* gradU, const double* nut, double alpha1,
+                     double* Pk, double* Pw);
+void turb_wall_production(const xcal_mesh* m, const BCsK& bk, double nu, const double* Ux,
+                          const double* Uy, const double* Uz, const double* k, double* Pk);
+void turb_omega_constrain(xcal_eqn* e, const BCsK& bw, double nu, const double* k);
+void turb_coeffs(const xcal_mesh* m, const double* omega, const double* nutf, double nu,
+                 double beta1, double beta_star, double sigma_w, double sigma_k, double* Dw,
+                 double* Dk, double* mw, double* mk);
+void turb_bound(const xcal_mesh* m, double* v, double* tmp);
+void turb_nut(const xcal_mesh* m, const double* k, const double* omega, double* nut);
+void turb_nut_wall(const xcal_mesh* m, const BCsK& bn, double nu, const double* k, double* nutf);
+void turb_nueff(const xcal_mesh* m, double nu, const double* nutf, double* nueff);
 // linear algebra
 void spmv(const xcal_eqn* e, const double* x, double* y);
 void apply_M_levels(const xcal_eqn* e, const double* r, double* z);

@@ -49,9 +49,11 @@ def lib():
                                         C.POINTER(A.SolveStats)],
             "xcal_simple_run_batch": [_vp, C.c_int32, _pd, _pi32, _pi32],
             "xcal_simple_set_case_nu": [_vp, _pd],
+            "xcal_simple_set_turbulence": [_vp, C.POINTER(A.TurbulenceSetupC)],
             "xcal_server_create": [C.POINTER(A.ServerConfigC), C.POINTER(_vp)],
             "xcal_server_destroy": [_vp],
-            "xcal_server_submit": [_vp, C.POINTER(A.MeshHost), C.POINTER(A.SimpleSetupC), _pd, _pd,
+            "xcal_server_submit": [_vp, C.POINTER(A.MeshHost), C.POINTER(A.SimpleSetupC),
+                                   C.POINTER(A.TurbulenceSetupC), _pd, _pd, _pd, _pd, _pd,
                                    C.POINTER(C.c_int64)],
             "xcal_server_poll": [_vp, C.c_int64, C.POINTER(A.RequestStatusC)],
             "xcal_server_wait": [_vp, C.c_int64, C.c_double],
@@ -337,8 +339,17 @@ class Simple:
         p = np.ascontiguousarray(np.asarray(p, dtype=np.float64).reshape(self.dmesh.n))
         check(lib().xcal_simple_set_field(self.h, 1, _pd_of(p)))
 
+    def set_turbulence(self, tsetup: A.TurbulenceSetupC):
+        self._tsetup = tsetup  # keeps BC arrays alive
+        check(lib().xcal_simple_set_turbulence(self.h, C.byref(tsetup)))
+
+    def set_scalar(self, which: str, values):
+        sel = {"p": 1, "k": 3, "omega": 4, "nut": 5}[which]
+        v = np.ascontiguousarray(np.asarray(values, dtype=np.float64).reshape(self.dmesh.n))
+        check(lib().xcal_simple_set_field(self.h, sel, _pd_of(v)))
+
     def get(self, which: str) -> np.ndarray:
-        sel = {"U": 0, "p": 1, "mdotf": 2}[which]
+        sel = {"U": 0, "p": 1, "mdotf": 2, "k": 3, "omega": 4, "nut": 5}[which]
         n = self.dmesh.mesh.n_faces if which == "mdotf" else self.dmesh.n
         out = np.zeros((3 if which == "U" else 1) * n)
         check(lib().xcal_simple_get_field(self.h, sel, _pd_of(out)))
@@ -392,15 +403,20 @@ class Server:
         check(lib().xcal_server_create(C.byref(cfg), C.byref(self.h)))
         self._cells = {}
 
-    def submit(self, mesh, setup: A.SimpleSetupC, U0=None, p0=None) -> int:
+    def submit(self, mesh, setup: A.SimpleSetupC, U0=None, p0=None, turbulence=None, k0=None,
+               omega0=None, nut0=None) -> int:
+        """turbulence: None (laminar) or an A.TurbulenceSetupC; the initial
+        fields may be None (zero start)."""
         host = A.mesh_host(mesh)
-        u = None if U0 is None else np.ascontiguousarray(np.asarray(U0, np.float64).reshape(-1))
-        p = None if p0 is None else np.ascontiguousarray(np.asarray(p0, np.float64).reshape(-1))
+        arrs = [None if v is None else np.ascontiguousarray(np.asarray(v, np.float64).reshape(-1))
+                for v in (U0, p0, k0, omega0, nut0)]
         rid = C.c_int64()
         check(lib().xcal_server_submit(self.h, C.byref(host), C.byref(setup),
-                                       None if u is None else _pd_of(u),
-                                       None if p is None else _pd_of(p), C.byref(rid)))
-        self._cells[rid.value] = mesh.n_cells
+                                       None if turbulence is None else C.byref(turbulence),
+                                       *[None if a is None else _pd_of(a) for a in arrs],
+                                       C.byref(rid)))
+        self._cells[rid.value] = (mesh.n_cells, turbulence is not None
+                                  and turbulence.model != A.TURB_LAMINAR)
         return rid.value
 
     def status(self, rid: int) -> dict:
@@ -414,13 +430,19 @@ class Server:
         """True when finished, False on timeout."""
         return check(lib().xcal_server_wait(self.h, rid, float(timeout))) == 0
 
-    def fetch(self, rid: int):
-        n = self._cells[rid]
-        U = np.zeros(3 * n)
-        p = np.zeros(n)
-        check(lib().xcal_server_fetch(self.h, rid, 0, _pd_of(U)))
-        check(lib().xcal_server_fetch(self.h, rid, 1, _pd_of(p)))
-        return U.reshape(3, n), p
+    def fetch(self, rid: int) -> dict:
+        """Fields of a finished request: U (3, n), p, and k, omega, nut for
+        turbulent cases."""
+        n, turb = self._cells[rid]
+        out = {}
+        names = [("U", A.SIMPLE_U), ("p", A.SIMPLE_P)]
+        if turb:
+            names += [("k", A.SIMPLE_K), ("omega", A.SIMPLE_OMEGA), ("nut", A.SIMPLE_NUT)]
+        for name, w in names:
+            v = np.zeros((3 if name == "U" else 1) * n)
+            check(lib().xcal_server_fetch(self.h, rid, w, _pd_of(v)))
+            out[name] = v.reshape(3, n) if name == "U" else v
+        return out
 
     def release(self, rid: int):
         check(lib().xcal_server_release(self.h, rid))

@@ -140,7 +140,26 @@ def run_serve(args, x):
     ncell = np.array([s["cells"] for s in st])
     ok = np.array([s["state"] == x._abi.REQ_DONE for s in st])
     conv = int(np.sum(ok & (its < args.max_iterations)))
+    mix = {i: srv.fetch(i) for i, good in zip(ids, ok) if good}
     srv.close()
+    # each request again, alone on a one-slot server: its run time without
+    # the mix, and whether its fields are the same as in the mix
+    alone_s, same = [], True
+    if not args.no_alone:
+        srv1 = H.Server(0, slots=1, hbm_budget=int(args.hbm_gb * (1 << 30)),
+                        max_iterations=args.max_iterations)
+        for k, ((mesh, model, config), rid) in enumerate(zip(cases, ids)):
+            j = submit_case(srv1, api, mesh, model, config)
+            srv1.wait(j, 0.0)
+            s1 = srv1.status(j)
+            alone_s.append(s1["t_end"] - s1["t_start"])
+            same = same and s1["iterations"] == st[k]["iterations"] and s1["state"] == st[k]["state"]
+            if rid in mix:
+                got = srv1.fetch(j)
+                same = same and all(np.array_equal(got[f], mix[rid][f]) for f in got)
+            srv1.release(j)
+            log(f"alone {k + 1}/{len(ids)}: {alone_s[-1]:.2f} s")
+        srv1.close()
     line = {
         "metric": METRIC, "value": round(float(np.sum(ncell * its)) / wall, 1),
         "unit": "cell-iters/s", "n_gpus": 1, "steps": int(np.sum(its)), "warmup": 0,
@@ -158,6 +177,9 @@ def run_serve(args, x):
                    "latency_p50_s": round(float(np.percentile(lat, 50)), 3),
                    "latency_p99_s": round(float(np.percentile(lat, 99)), 3),
                    "requests_per_s": round(args.requests / wall, 3),
+                   "alone_p50_s": round(float(np.percentile(alone_s, 50)), 3) if alone_s else None,
+                   "alone_p99_s": round(float(np.percentile(alone_s, 99)), 3) if alone_s else None,
+                   "identical_alone_vs_mix": same if alone_s else None,
                    "converged": conv, "failed": int(np.sum(~ok)), "iterations": its.tolist(), "cells": ncell.tolist(),
                    "run_s": [round(s["t_end"] - s["t_start"], 3) for s in st]},
     }
@@ -251,7 +273,9 @@ def run_batch(args, x):
                    "p_solver": "CG+DILU rtol 0.1", "U_solver": "BiCGSTAB+DILU rtol 0.1",
                    "parallelism": "single GPU, one batch mesh"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": hbm_traffic(args.roofline_kernel,
+                                            f"batch{args.cases}_{args.precision}"),
                      "kernel": args.roofline_kernel, "bytes_per_launch": nbytes,
                      "launches": launches, "avg_us": round(avg_s * 1e6, 2)},
         "cpu_baseline": None,
@@ -285,6 +309,38 @@ def channel(api, mesh, iterations, precision="fp64", inner_rtol=1e-4):
     return model, config
 
 
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def run_calib(args):
+    """HBM counter calibration: a plain 256 MiB device-to-device copy, three
+    times.  rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this command give the
+    counter reading of a kernel that moves a known number of bytes."""
+    import torch
+    nbytes = 256 << 20
+    a = torch.ones(nbytes, dtype=torch.uint8, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration": "copy", "bytes_read": nbytes, "bytes_written": nbytes,
+                      "launches": 3}))
+
+
+def hbm_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` on `workload` from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, calibrated against a
+    copy kernel), or None when no summary covers this pair."""
+    import re
+    path = ROOT / "profiles" / f"hbm_traffic_{kernel}_{workload}.md"
+    if not path.exists():
+        return None
+    m = re.search(r"traffic_bytes_per_launch\s*=\s*([0-9.eE+]+)", path.read_text())
+    return float(m.group(1)) if m else None
+
+
 def kernel_bytes(name, n, nnz, vb=8):
     """Algorithmic HBM bytes of one timed unit (FP64 values, int32 indices;
     every array element counted once, gathers included once, slice padding
@@ -315,7 +371,7 @@ def main():
     ap.add_argument("--inner-rtol", type=float, default=1e-4,
                     help="fp32-ir: relative tolerance of each FP32 inner solve")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--mode", default="channel", choices=["channel", "batch", "serve"],
+    ap.add_argument("--mode", default="channel", choices=["channel", "batch", "serve", "calib"],
                     help="channel: the 1M-cell FP64 channel; batch: batched cavities; "
                          "serve: Poisson-arrival serving of step cases")
     ap.add_argument("--requests", type=int, default=16, help="serve mode: number of requests")
@@ -324,8 +380,11 @@ def main():
     ap.add_argument("--hbm-gb", type=float, default=64.0, help="serve mode: HBM budget (GiB)")
     ap.add_argument("--max-iterations", type=int, default=1000,
                     help="serve mode: SIMPLE iterations per request at most")
-    ap.add_argument("--precon", default="DILU", choices=["DILU", "Jacobi"],
-                    help="serve mode: preconditioner of every solve")
+    ap.add_argument("--precon", default="Jacobi", choices=["DILU", "Jacobi"],
+                    help="serve mode: preconditioner of every solve (Jacobi, as the "
+                         "reference's k-omega step example)")
+    ap.add_argument("--no-alone", action="store_true",
+                    help="serve mode: skip re-running each request alone")
     ap.add_argument("--laminar", action="store_true",
                     help="serve mode: laminar cases instead of k-omega RANS")
     ap.add_argument("--cases", type=int, default=256, help="batch mode: number of cases")
@@ -344,11 +403,15 @@ def main():
     if args.mode == "serve":
         run_serve(args, x)
         return
+    if args.mode == "calib":
+        run_calib(args)
+        return
     api, H = x.api, x.hip
     N = args.cells
     t0 = time.perf_counter()
     mesh = x.mesh.box3d(N, N, N, jitter=0.1, seed_jitter=SEED_JITTER, seed_perm=SEED_PERM)
     t_mesh = time.perf_counter() - t0
+    log(f"mesh built: {mesh.n_cells} cells in {t_mesh:.1f} s")
     model, config = channel(api, mesh, args.warmup + args.steps, args.precision, args.inner_rtol)
     setup = api.simple_setup(model, config)
     ctx = H.Context(0)
@@ -359,8 +422,10 @@ def main():
     sim.set_p(model.momentum.p)
     ctx.synchronize()
     t_upload = time.perf_counter() - t0
+    log(f"uploaded in {t_upload:.1f} s")
     if args.warmup:
         sim.run(args.warmup)
+        log(f"{args.warmup} warmup steps done")
     ctx.time_kernel(args.roofline_kernel, True)
     ctx.synchronize()
     t0 = time.perf_counter()
@@ -372,6 +437,7 @@ def main():
     if out["done"] != args.steps:
         raise RuntimeError(f"SIMPLE stopped after {out['done']} of {args.steps} steps")
     wall = t1 - t0
+    log(f"{args.steps} timed steps in {wall:.2f} s")
     n = mesh.n_cells
     value = n * args.steps / wall
     stages = sim.stage_times()
@@ -379,8 +445,9 @@ def main():
     nbytes = kernel_bytes(args.roofline_kernel, n, dm.nnz)
     avg_s = kms / 1e3 / max(launches, 1)
     achieved = nbytes / avg_s / 1e9
+    traffic = hbm_traffic(args.roofline_kernel, f"channel_{N}cubed_{args.precision}")
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": args.roofline_kernel, "bytes_per_launch": nbytes,
                 "launches": launches, "avg_us": round(avg_s * 1e6, 2)}
 

@@ -242,7 +242,7 @@ __global__ void __launch_bounds__(kRunBlock)
   for (int u = case_lev[k]; u < case_lev[k + 1]; ++u) {
     const int32_t r0 = lev_ptr[u], cnt = lev_ptr[u + 1] - r0, base = meta[2 * u], w = meta[2 * u + 1];
     for (int32_t q = threadIdx.x; q < cnt; q += blockDim.x)
-      fwd_row<UPD>(W, r0 + q, (int64_t)base + q, cnt, w, a, true);
+      fwd_row<UPD, 0>(W, r0 + q, (int64_t)base + q, cnt, w, a, true);
     __syncthreads();
   }
 }
@@ -258,7 +258,7 @@ __global__ void __launch_bounds__(kRunBlock)
   for (int u = case_lev[k]; u < case_lev[k + 1]; ++u) {
     const int32_t p0 = lev_ptr[u], cnt = lev_ptr[u + 1] - p0, base = meta[2 * u], w = meta[2 * u + 1];
     for (int32_t q = threadIdx.x; q < cnt; q += blockDim.x)
-      bwd_row<DOT>(W, p0 + q, (int64_t)base + q, cnt, w, true, s0, s1);
+      bwd_row<DOT, 0>(W, p0 + q, (int64_t)base + q, cnt, w, true, s0, s1);
     __syncthreads();
   }
   if (DOT != 0) store_partial<kRunBlock>(s0, s1, W.partial, k);

@@ -79,7 +79,7 @@ static std::vector<SweepStep> plan_levels(const std::vector<int32_t>& ptr, int n
     st.row0 = ptr[l];
     if (cnt >= small) {
       st.lev1 = l + 1;
-      st.grid = grid_for(cnt);
+      st.grid = level_grid(cnt);
     } else {
       int e = l;
       while (e < nlev && ptr[e + 1] - ptr[e] < small) ++e;
@@ -93,6 +93,34 @@ static std::vector<SweepStep> plan_levels(const std::vector<int32_t>& ptr, int n
     l = st.lev1;
   }
   return plan;
+}
+
+// Row blocks of kBlock rows for the gathering kernels: each forward level
+// [a, b) is cut at 64-aligned eighths; eighth x of every level goes to XCD x
+// (the dispatcher deals block b to XCD b%8).  Shorter XCD lists are padded
+// with empty blocks.
+std::vector<int32_t> xcd_row_blocks(const std::vector<int32_t>& lev_ptr, int nlev) {
+  std::vector<std::vector<int32_t>> per(8);
+  for (int l = 0; l < nlev; ++l) {
+    const int64_t a = lev_ptr[l], cnt = lev_ptr[l + 1] - a;
+    for (int x = 0; x < 8; ++x) {
+      const int64_t c0 = a + (cnt * x / 8) / 64 * 64;
+      const int64_t c1 = x == 7 ? a + cnt : a + (cnt * (x + 1) / 8) / 64 * 64;
+      for (int64_t r = c0; r < c1; r += kBlock) {
+        per[x].push_back((int32_t)r);
+        per[x].push_back((int32_t)std::min<int64_t>(r + kBlock, c1));
+      }
+    }
+  }
+  size_t nb = 1;
+  for (const auto& v : per) nb = std::max(nb, v.size() / 2);
+  std::vector<int32_t> out(16 * nb, 0);
+  for (int x = 0; x < 8; ++x)
+    for (size_t k = 0; k < per[x].size() / 2; ++k) {
+      out[2 * (8 * k + x)] = per[x][2 * k];
+      out[2 * (8 * k + x) + 1] = per[x][2 * k + 1];
+    }
+  return out;
 }
 
 void make_sweep_plans(xcal_mesh* m, int small) {
@@ -109,6 +137,9 @@ void make_sweep_plans(xcal_mesh* m, int small) {
   if (m->lev_ptr_f_d.n == 0) {
     upload(m->lev_ptr_f_d, m->lev_ptr_f, bytes);
     upload(m->lev_ptr_b_d, m->lev_ptr_b, bytes);
+    m->xblk = xcd_row_blocks(m->lev_ptr_f, m->nlev_f);
+    m->nxblk = (int)(m->xblk.size() / 2);
+    upload(m->xblk_d, m->xblk, bytes);
     m->device_bytes += bytes;
   }
 }
@@ -288,6 +319,7 @@ static void build_host_mesh(const xcal_mesh_host* h, HostMesh& M) {
   // the DILU lower/upper split and all per-row summation orders are
   // unchanged; fields are permuted only when they cross the C ABI.
   std::vector<int32_t> cperm(n), cinv(n), fperm(nf), finv(nf);
+  std::vector<uint64_t> dmort(n);  // Morton key of each device row
   {
     double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
     for (int64_t c = 0; c < n; ++c)
@@ -317,6 +349,7 @@ static void build_host_mesh(const xcal_mesh_host* h, HostMesh& M) {
     for (int64_t c = 0; c < n; ++c) {
       cinv[c] = key[c].second;
       cperm[key[c].second] = (int32_t)c;
+      dmort[c] = key[c].first.second;
     }
     for (int64_t f = 0; f < nbf; ++f) fperm[f] = finv[f] = (int32_t)f;  // boundary faces keep ids
     std::vector<std::pair<std::pair<int32_t, int32_t>, int32_t>> fk;
@@ -373,6 +406,11 @@ static void build_host_mesh(const xcal_mesh_host* h, HostMesh& M) {
     rows_b.assign(n, 0);
     std::vector<int32_t> cur(M.lev_ptr_b.begin(), M.lev_ptr_b.end() - 1);
     for (int64_t c = 0; c < n; ++c) rows_b[cur[blev[cinv[c]]]++] = (int32_t)c;
+    // Morton order inside each backward level, so that an eighth of a level
+    // is one region of space (as for the forward levels)
+    for (int l = 0; l < M.nlev_b; ++l)
+      std::stable_sort(rows_b.begin() + M.lev_ptr_b[l], rows_b.begin() + M.lev_ptr_b[l + 1],
+                       [&](int32_t a, int32_t b) { return dmort[a] < dmort[b]; });
   }
   // Packed triangular factors for the sweeps.  Level l of a sweep stores
   // entry t of its q-th row at base[l] + t*cnt_l + q (column-major inside
@@ -830,6 +868,8 @@ int xcal_ctx_create(int32_t device, xcal_ctx** out) {
     try {
       XCAL_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
       c->red_result.alloc(64);
+      c->nostop.alloc(32);
+      XCAL_HIP(hipMemset(c->nostop.p, 0, sizeof(double) * 32));
       XCAL_HIP(hipHostMalloc((void**)&c->host_pinned, sizeof(double) * 64, hipHostMallocDefault));
     } catch (...) {
       if (c->stream) (void)hipStreamDestroy(c->stream);

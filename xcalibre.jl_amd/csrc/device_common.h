@@ -29,11 +29,20 @@ struct MatKT {
   int64_t n;
   const int32_t *rb, *re, *colval, *diag;
   const T* nz;
+  const int32_t* blk;  // row blocks in dispatch order (xcal_mesh::xblk)
 };
 using MatK = MatKT<double>;
 template <class T>
 inline MatKT<T> matk(const xcal_mesh* m, const T* nz) {
-  return MatKT<T>{m->n, m->rbeg.p, m->rend.p, m->colval.p, m->diag.p, nz};
+  return MatKT<T>{m->n, m->rbeg.p, m->rend.p, m->colval.p, m->diag.p, nz, m->xblk_d.p};
+}
+// Row of this work-item in a launch over A's row blocks (grid = nxblk);
+// valid while row < end.
+template <class T>
+__device__ inline int64_t block_row(const MatKT<T>& A, int64_t& end) {
+  const int32_t* b = A.blk + 2 * blockIdx.x;
+  end = b[1];
+  return (int64_t)b[0] + threadIdx.x;
 }
 
 inline MeshK meshk(const xcal_mesh* m) {
@@ -299,6 +308,17 @@ __device__ inline int64_t xcd_block() {
   return x * q + (x < r ? x : r) + l;
 }
 __device__ inline int64_t xcd_row() { return xcd_block() * blockDim.x + threadIdx.x; }
+
+// Position of this work-item in a level launch of cnt rows (grid =
+// level_grid(cnt)): workgroups on XCD x (blockIdx % 8) take eighth x of the
+// level, whose rows are one Morton range, so their gathers share that XCD's
+// L2.  Valid while the position is below `end`.
+__device__ inline int32_t level_pos(int32_t cnt, int32_t& end) {
+  const int x = blockIdx.x & 7, l = blockIdx.x >> 3;
+  const int64_t c0 = (int64_t)cnt * x / 8 / 64 * 64;
+  end = (int32_t)(x == 7 ? cnt : (int64_t)cnt * (x + 1) / 8 / 64 * 64);
+  return (int32_t)(c0 + (int64_t)l * blockDim.x + threadIdx.x);
+}
 
 // Deterministic block-level sum (wave shuffles + LDS), result in thread 0.
 __device__ inline double block_sum(double v, double* lds) {

@@ -23,8 +23,9 @@ namespace xcal {
 // segment.
 __global__ void __launch_bounds__(kBlock)
     k_spmv(MatK A, const double* __restrict__ x, double* __restrict__ y) {
-  const int64_t i = xcd_row();
-  if (i >= A.n) return;
+  int64_t end;
+  const int64_t i = block_row(A, end);
+  if (i >= end) return;
   y[i] = row_sum(A, i, [&](int j) { return x[j]; });
 }
 
@@ -36,9 +37,10 @@ __global__ void __launch_bounds__(kBlock)
     k_residual_sums(MatK A, const double* __restrict__ x, const double* __restrict__ b,
                     double* __restrict__ partial) {
   __shared__ double lds[kBlock / 64];
-  const int64_t i = xcd_row();
+  int64_t end;
+  const int64_t i = block_row(A, end);
   double s0 = 0.0, s1 = 0.0;
-  if (i < A.n) {
+  if (i < end) {
     const double acc = row_sum(A, i, [&](int j) { return x[j]; });
     const double r = b[i] - acc;
     s0 = r * r;
@@ -114,8 +116,9 @@ __device__ inline void dilu_diag_row(const MatK& A, const int32_t* __restrict__ 
 __global__ void __launch_bounds__(kBlock)
     k_dilu_diag_level(MatK A, const int32_t* __restrict__ trans, double* D, int32_t row0,
                       int32_t cnt) {
-  const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < cnt) dilu_diag_row(A, trans, D, row0 + t);
+  int32_t end;
+  const int32_t t = level_pos(cnt, end);
+  if (t < end) dilu_diag_row(A, trans, D, row0 + t);
 }
 __global__ void __launch_bounds__(kRunBlock)
     k_dilu_diag_run(MatK A, const int32_t* __restrict__ trans, double* D,
@@ -156,7 +159,7 @@ static void launch_check() { XCAL_HIP(hipGetLastError()); }
 
 void spmv(const xcal_eqn* e, const double* x, double* y) {
   const xcal_mesh* m = e->mesh;
-  hipLaunchKernelGGL(k_spmv, dim3(grid_for(m->n)), dim3(kBlock), 0, m->ctx->stream,
+  hipLaunchKernelGGL(k_spmv, dim3(m->nxblk), dim3(kBlock), 0, m->ctx->stream,
                      matk(m, e->nzval.p), x, y);
   launch_check();
 }
@@ -164,7 +167,7 @@ void spmv(const xcal_eqn* e, const double* x, double* y) {
 double residual(xcal_eqn* e, const double* x, const double* b) {
   const xcal_mesh* m = e->mesh;
   xcal_ctx* c = m->ctx;
-  const int grid = grid_for(m->n);
+  const int grid = m->nxblk;
   if (e->kpart.n < 2 * (int64_t)grid) {
     XCAL_HIP(hipStreamSynchronize(c->stream));  // the old buffer may still be in use
     e->kpart.alloc(2 * (int64_t)grid);

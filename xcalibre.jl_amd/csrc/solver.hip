@@ -59,8 +59,9 @@ template <class T>
 __global__ void __launch_bounds__(kBlock)
     k_start(MatKT<T> A, const T* __restrict__ b, const T* __restrict__ x0, T* __restrict__ r,
             T* __restrict__ x, T* __restrict__ pold) {
-  const int64_t i = xcd_row();
-  if (i >= A.n) return;
+  int64_t end;
+  const int64_t i = block_row(A, end);
+  if (i >= end) return;
   if (x0) {
     const T acc = row_sum(A, i, [&](int j) { return x0[j]; });
     r[i] = b[i] - acc;
@@ -101,9 +102,10 @@ __global__ void __launch_bounds__(kBlock)
               T* __restrict__ pnew, T* __restrict__ q, double* partial, const KS* ks) {
   const bool live = !stopped(ks);
   const T beta = (T)ks->beta;
-  const int64_t i = xcd_row();
+  int64_t end;
+  const int64_t i = block_row(A, end);
   double s0 = 0.0;
-  if (i < A.n) {
+  if (i < end) {
     const T acc = row_sum(A, i, [&](int j) { return z[j] + beta * pold[j]; });
     const T pi = z[i] + beta * pold[i];
     if (live) {
@@ -148,9 +150,10 @@ __global__ void __launch_bounds__(kBlock)
                 const T* __restrict__ c, double* partial, const KS* ks) {
   const bool live = !stopped(ks);
   const T beta = (T)ks->beta;
-  const int64_t i = xcd_row();
+  int64_t end;
+  const int64_t i = block_row(A, end);
   double s0 = 0.0;
-  if (i < A.n) {
+  if (i < end) {
     const T acc = row_sum(A, i, [&](int j) { return r[j] + beta * pold[j]; });
     const T pi = r[i] + beta * pold[i];
     const T vi = VDIAG && S ? S[i] * acc : acc;
@@ -172,9 +175,10 @@ __global__ void __launch_bounds__(kBlock)
                 const T* __restrict__ S, T* __restrict__ dt, double* partial, const KS* ks) {
   const bool live = !stopped(ks);
   const T a = (T)ks->alpha;
-  const int64_t i = xcd_row();
+  int64_t end;
+  const int64_t i = block_row(A, end);
   double s0 = 0.0, s1 = 0.0;
-  if (i < A.n) {
+  if (i < end) {
     const T acc = row_sum(A, i, [&](int j) { return r[j] + (-a) * v[j]; });
     const T si = r[i] + (-a) * v[i];
     const T xi = x[i] + a * p[i];
@@ -230,9 +234,10 @@ __global__ void __launch_bounds__(kBlock)
 __global__ void __launch_bounds__(kBlock)
     k_ir_residual(MatK A, const double* __restrict__ x, const double* __restrict__ b,
                   float* __restrict__ r32, double* partial) {
-  const int64_t i = xcd_row();
+  int64_t end;
+  const int64_t i = block_row(A, end);
   double s0 = 0.0, s1 = 0.0;
-  if (i < A.n) {
+  if (i < end) {
     const double acc = row_sum(A, i, [&](int j) { return x[j]; });
     const double r = b[i] - acc;
     r32[i] = (float)r;
@@ -272,13 +277,12 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // one level: rows [row0, row0 + cnt) (forward levels are contiguous)
-template <bool UPD, class T>
+template <bool UPD, int WK, class T>
 __global__ void __launch_bounds__(kBlock)
     k_fwd_level(FwdK<T> W, int32_t row0, int32_t cnt, int32_t base, int32_t w) {
-  const bool live = !W.ks || !stopped(W.ks);
-  const T a = UPD ? (T)W.ks->alpha : T(0);
-  const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < cnt) fwd_row<UPD>(W, row0 + t, (int64_t)base + t, cnt, w, a, live);
+  int32_t end;
+  const int32_t t = level_pos(cnt, end);
+  if (t < end) fwd_row<UPD, WK, T, true>(W, row0 + t, (int64_t)base + t, cnt, w, T(0), true);
 }
 
 // a run of small levels [lev0, lev1) in one workgroup
@@ -286,23 +290,23 @@ template <bool UPD, class T>
 __global__ void __launch_bounds__(kRunBlock)
     k_fwd_run(FwdK<T> W, const int32_t* __restrict__ lev_ptr, const int32_t* __restrict__ meta,
               int lev0, int lev1) {
-  if (W.ks && stopped(W.ks)) return;  // uniform over the workgroup
+  if (stopped(W.ks)) return;  // uniform over the workgroup
   const T a = UPD ? (T)W.ks->alpha : T(0);
   for (int l = lev0; l < lev1; ++l) {
     const int32_t r0 = lev_ptr[l], cnt = lev_ptr[l + 1] - r0, base = meta[2 * l], w = meta[2 * l + 1];
     for (int32_t q = threadIdx.x; q < cnt; q += blockDim.x)
-      fwd_row<UPD>(W, r0 + q, (int64_t)base + q, cnt, w, a, true);
+      fwd_row<UPD, 0>(W, r0 + q, (int64_t)base + q, cnt, w, a, true);
     __syncthreads();
   }
 }
 
-template <int DOT, class T>
+template <int DOT, int WK, class T>
 __global__ void __launch_bounds__(kBlock)
     k_bwd_level(BwdK<T> W, int32_t pos0, int32_t cnt, int32_t base, int32_t w, int slot) {
-  const bool live = !W.ks || !stopped(W.ks);
-  const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  int32_t end;
+  const int32_t t = level_pos(cnt, end);
   double s0 = 0.0, s1 = 0.0;
-  if (t < cnt) bwd_row<DOT>(W, pos0 + t, (int64_t)base + t, cnt, w, live, s0, s1);
+  if (t < end) bwd_row<DOT, WK, T, true>(W, pos0 + t, (int64_t)base + t, cnt, w, true, s0, s1);
   if (DOT != 0) store_partial<kBlock>(s0, s1, W.partial, slot + blockIdx.x);
 }
 
@@ -310,12 +314,12 @@ template <int DOT, class T>
 __global__ void __launch_bounds__(kRunBlock)
     k_bwd_run(BwdK<T> W, const int32_t* __restrict__ lev_ptr, const int32_t* __restrict__ meta,
               int lev0, int lev1, int slot) {
-  if (W.ks && stopped(W.ks)) return;
+  if (stopped(W.ks)) return;
   double s0 = 0.0, s1 = 0.0;
   for (int l = lev0; l < lev1; ++l) {
     const int32_t p0 = lev_ptr[l], cnt = lev_ptr[l + 1] - p0, base = meta[2 * l], w = meta[2 * l + 1];
     for (int32_t q = threadIdx.x; q < cnt; q += blockDim.x)
-      bwd_row<DOT>(W, p0 + q, (int64_t)base + q, cnt, w, true, s0, s1);
+      bwd_row<DOT, 0>(W, p0 + q, (int64_t)base + q, cnt, w, true, s0, s1);
     __syncthreads();
   }
   if (DOT != 0) store_partial<kRunBlock>(s0, s1, W.partial, slot);
@@ -336,7 +340,8 @@ struct Solver {
   T* work;
   KS* ks;
   double* partial;
-  int grid;
+  int grid;   // elementwise kernels: grid_for(n) blocks in XCD order
+  int gridA;  // gathering kernels: the mesh's row blocks (xcal_mesh::xblk)
   int* cg_batch;
   int* bi_batch;
 };
@@ -344,7 +349,8 @@ struct Solver {
 static void ensure_workspace(xcal_eqn* e, bool fp32) {
   const xcal_mesh* m = e->mesh;
   const int64_t n = m->n;
-  const int64_t slots = std::max<int64_t>(grid_for(n), m->sweep_slots) + 8;
+  const int64_t slots =
+      std::max<int64_t>(std::max<int64_t>(grid_for(n), m->nxblk), m->sweep_slots) + 8;
   const bool grow = (e->work.n > 0 && e->work.n < 12 * n) ||
                     (e->kpart.n > 0 && e->kpart.n < 2 * slots) ||
                     (fp32 && e->work32.n > 0 && e->work32.n < 12 * n) ||
@@ -383,6 +389,7 @@ static Solver<double> make_solver(xcal_eqn* e) {
   S.ks = (KS*)e->kstate.p;
   S.partial = e->kpart.p;
   S.grid = grid_for(S.n);
+  S.gridA = S.m->nxblk;
   S.cg_batch = &e->cg_batch;
   S.bi_batch = &e->bi_batch;
   return S;
@@ -429,6 +436,7 @@ static Solver<float> make_solver32(xcal_eqn* e) {
   S.ks = (KS*)e->kstate.p;
   S.partial = e->kpart.p;
   S.grid = grid_for(n);
+  S.gridA = m->nxblk;
   S.cg_batch = &e->ir_pred[0][0];
   S.bi_batch = &e->ir_pred[1][0];
   return S;
@@ -447,9 +455,19 @@ static void sweep_fwd(const Solver<T>& S, const FwdK<T>& W) {
     if (st.grid == 1)
       hipLaunchKernelGGL((k_fwd_run<UPD, T>), dim3(1), dim3(kRunBlock), 0, S.s, W,
                          m->lev_ptr_f_d.p, m->lmeta_f_d.p, st.lev0, st.lev1);
-    else
-      hipLaunchKernelGGL((k_fwd_level<UPD, T>), dim3(st.grid), dim3(kBlock), 0, S.s, W, st.row0,
-                         st.row1 - st.row0, m->lmeta_f[2 * st.lev0], m->lmeta_f[2 * st.lev0 + 1]);
+    else {
+      const int w = m->lmeta_f[2 * st.lev0 + 1];
+      auto go = [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3(st.grid), dim3(kBlock), 0, S.s, W, st.row0,
+                           st.row1 - st.row0, m->lmeta_f[2 * st.lev0], w);
+      };
+      switch (entry_bound(w)) {
+        case 2: go(k_fwd_level<UPD, 2, T>); break;
+        case 4: go(k_fwd_level<UPD, 4, T>); break;
+        case 8: go(k_fwd_level<UPD, 8, T>); break;
+        default: go(k_fwd_level<UPD, 0, T>);
+      }
+    }
   }
 }
 
@@ -461,10 +479,19 @@ static void sweep_bwd(const Solver<T>& S, const BwdK<T>& W) {
     if (st.grid == 1)
       hipLaunchKernelGGL((k_bwd_run<DOT, T>), dim3(1), dim3(kRunBlock), 0, S.s, W,
                          m->lev_ptr_b_d.p, m->lmeta_b_d.p, st.lev0, st.lev1, st.slot);
-    else
-      hipLaunchKernelGGL((k_bwd_level<DOT, T>), dim3(st.grid), dim3(kBlock), 0, S.s, W, st.row0,
-                         st.row1 - st.row0, m->lmeta_b[2 * st.lev0], m->lmeta_b[2 * st.lev0 + 1],
-                         st.slot);
+    else {
+      const int w = m->lmeta_b[2 * st.lev0 + 1];
+      auto go = [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3(st.grid), dim3(kBlock), 0, S.s, W, st.row0,
+                           st.row1 - st.row0, m->lmeta_b[2 * st.lev0], w, st.slot);
+      };
+      switch (entry_bound(w)) {
+        case 2: go(k_bwd_level<DOT, 2, T>); break;
+        case 4: go(k_bwd_level<DOT, 4, T>); break;
+        case 8: go(k_bwd_level<DOT, 8, T>); break;
+        default: go(k_bwd_level<DOT, 0, T>);
+      }
+    }
   }
 }
 
@@ -483,8 +510,9 @@ static BwdK<T> bwdk(const Solver<T>& S, const T* Yb, T* Z, const T* dv, const KS
 // z = M^-1 u through the level sweeps (DILU), used by precond_apply
 static void dilu_apply(Solver<double>& S, const double* u, double* z) {
   double *Y = S.work, *Yb = S.work + S.n;
-  sweep_fwd<false>(S, fwdk(S, u, Y, Yb, (const KS*)nullptr));
-  sweep_bwd<0>(S, bwdk(S, (const double*)Yb, z, (const double*)nullptr, (const KS*)nullptr));
+  const KS* live = (const KS*)S.m->ctx->nostop.p;  // a state that never stops
+  sweep_fwd<false>(S, fwdk(S, u, Y, Yb, live));
+  sweep_bwd<0>(S, bwdk(S, (const double*)Yb, z, (const double*)nullptr, live));
   XCAL_HIP(hipGetLastError());
 }
 
@@ -540,7 +568,7 @@ static int cg(Solver<T>& S, int precon, const T* b, T* x0, int itmax, double ato
   T *r = w, *x = w + n, *pa = w + 2 * n, *pb = w + 3 * n, *q = w + 4 * n, *zd = w + 5 * n,
     *Y = w + 6 * n, *Yb = w + 7 * n;
   init_state(S, itmax, atol, rtol);
-  hipLaunchKernelGGL(k_start<T>, dim3(S.grid), dim3(kBlock), 0, S.s, S.A, b, (const T*)x0, r, x, pa);
+  hipLaunchKernelGGL(k_start<T>, dim3(S.gridA), dim3(kBlock), 0, S.s, S.A, b, (const T*)x0, r, x, pa);
   const bool dilu = precon == XCAL_PRECON_DILU;
   const T* Sd = precon == XCAL_PRECON_JACOBI ? S.Sj : nullptr;
   const int sweep_slots = S.m->sweep_slots;
@@ -560,10 +588,10 @@ static int cg(Solver<T>& S, int precon, const T* b, T* x0, int itmax, double ato
     parity ^= 1;
     xcal_ctx* c = S.m->ctx;
     timed(c, 0, [&] {
-      hipLaunchKernelGGL(k_cg_spmv<T>, dim3(S.grid), dim3(kBlock), 0, S.s, S.A, (const T*)zd,
+      hipLaunchKernelGGL(k_cg_spmv<T>, dim3(S.gridA), dim3(kBlock), 0, S.s, S.A, (const T*)zd,
                          (const T*)pold, pnew, q, S.partial, (const KS*)S.ks);
     });
-    finalize_launch(S, S.grid, FIN_CG_PAP);
+    finalize_launch(S, S.gridA, FIN_CG_PAP);
     if (dilu) {
       FwdK<T> W = fwdk(S, (const T*)nullptr, Y, Yb, (const KS*)S.ks);
       W.p = pnew;
@@ -599,7 +627,7 @@ static int bicgstab(Solver<T>& S, int precon, const T* b, T* x0, int itmax, doub
   T *r = w, *x = w + n, *pa = w + 2 * n, *pb = w + 3 * n, *v = w + 4 * n, *sv = w + 5 * n,
     *t = w + 6 * n, *d = w + 7 * n, *Y = w + 8 * n, *Yb = w + 9 * n;
   init_state(S, itmax, atol, rtol);
-  hipLaunchKernelGGL(k_start<T>, dim3(S.grid), dim3(kBlock), 0, S.s, S.A, b, (const T*)x0, r, x, pa);
+  hipLaunchKernelGGL(k_start<T>, dim3(S.gridA), dim3(kBlock), 0, S.s, S.A, b, (const T*)x0, r, x, pa);
   const bool dilu = precon == XCAL_PRECON_DILU;
   const T* Sd = precon == XCAL_PRECON_JACOBI ? S.Sj : nullptr;
   const int sweep_slots = S.m->sweep_slots;
@@ -620,25 +648,25 @@ static int bicgstab(Solver<T>& S, int precon, const T* b, T* x0, int itmax, doub
     T* pnew = parity ? pa : pb;
     parity ^= 1;
     if (dilu) {
-      hipLaunchKernelGGL((k_bi_spmv_p<false, T>), dim3(S.grid), dim3(kBlock), 0, S.s, S.A,
+      hipLaunchKernelGGL((k_bi_spmv_p<false, T>), dim3(S.gridA), dim3(kBlock), 0, S.s, S.A,
                          (const T*)r, (const T*)pold, pnew, (const T*)nullptr, d, b, S.partial,
                          ks);  // d holds q
       sweep_fwd<false>(S, fwdk(S, (const T*)d, Y, Yb, ks));
       sweep_bwd<2>(S, bwdk(S, (const T*)Yb, v, b, ks));
       finalize_launch(S, sweep_slots, FIN_BI_ALPHA);
-      hipLaunchKernelGGL((k_bi_spmv_s<false, T>), dim3(S.grid), dim3(kBlock), 0, S.s, S.A,
+      hipLaunchKernelGGL((k_bi_spmv_s<false, T>), dim3(S.gridA), dim3(kBlock), 0, S.s, S.A,
                          (const T*)r, (const T*)v, (const T*)pnew, sv, x, (const T*)nullptr, d,
                          S.partial, ks);
       sweep_fwd<false>(S, fwdk(S, (const T*)d, Y, Yb, ks));
       sweep_bwd<3>(S, bwdk(S, (const T*)Yb, t, (const T*)sv, ks));
       finalize_launch(S, sweep_slots, FIN_BI_OMEGA);
     } else {
-      hipLaunchKernelGGL((k_bi_spmv_p<true, T>), dim3(S.grid), dim3(kBlock), 0, S.s, S.A,
+      hipLaunchKernelGGL((k_bi_spmv_p<true, T>), dim3(S.gridA), dim3(kBlock), 0, S.s, S.A,
                          (const T*)r, (const T*)pold, pnew, Sd, v, b, S.partial, ks);
-      finalize_launch(S, S.grid, FIN_BI_ALPHA);
-      hipLaunchKernelGGL((k_bi_spmv_s<true, T>), dim3(S.grid), dim3(kBlock), 0, S.s, S.A,
+      finalize_launch(S, S.gridA, FIN_BI_ALPHA);
+      hipLaunchKernelGGL((k_bi_spmv_s<true, T>), dim3(S.gridA), dim3(kBlock), 0, S.s, S.A,
                          (const T*)r, (const T*)v, (const T*)pnew, sv, x, Sd, t, S.partial, ks);
-      finalize_launch(S, S.grid, FIN_BI_OMEGA);
+      finalize_launch(S, S.gridA, FIN_BI_OMEGA);
     }
     hipLaunchKernelGGL(k_bi_tail<T>, dim3(S.grid), dim3(kBlock), 0, S.s, n, (const T*)sv,
                        (const T*)t, (const T*)v, x, r, pnew, b, S.partial, ks);
@@ -658,8 +686,8 @@ static int bicgstab(Solver<T>& S, int precon, const T* b, T* x0, int itmax, doub
 static void ir_residual(Solver<double>& S, const double* x, const double* b, float* r32,
                         double& rr, double& bb) {
   xcal_ctx* c = S.m->ctx;
-  hipLaunchKernelGGL(k_ir_residual, dim3(S.grid), dim3(kBlock), 0, S.s, S.A, x, b, r32, S.partial);
-  hipLaunchKernelGGL(k_sum2, dim3(1), dim3(kRunBlock), 0, S.s, S.partial, S.grid, c->red_result.p);
+  hipLaunchKernelGGL(k_ir_residual, dim3(S.gridA), dim3(kBlock), 0, S.s, S.A, x, b, r32, S.partial);
+  hipLaunchKernelGGL(k_sum2, dim3(1), dim3(kRunBlock), 0, S.s, S.partial, S.gridA, c->red_result.p);
   XCAL_HIP(hipGetLastError());
   XCAL_HIP(hipMemcpyAsync(c->host_pinned, c->red_result.p, 2 * sizeof(double),
                           hipMemcpyDeviceToHost, S.s));

@@ -16,6 +16,7 @@ struct KS {
   double gamma, pAp, alpha, beta, rNorm, eps, pNorm2, rho, next_rho, omega, atol, rtol;
   int iter, itmax, stop, err;
 };
+static_assert(sizeof(KS) <= 32 * sizeof(double), "the context's zeroed state holds one KS");
 
 enum Fin : int { FIN_NONE = 0, FIN_CG_INIT, FIN_CG_PAP, FIN_CG_ITER, FIN_BI_INIT, FIN_BI_ALPHA,
                  FIN_BI_OMEGA, FIN_BI_END };
@@ -148,47 +149,109 @@ struct FwdK {
   const KS* ks;  // NULL outside a Krylov solve
 };
 
-template <bool UPD, class T>
-__device__ inline void fwd_row(const FwdK<T>& W, int32_t i, int64_t e0, int32_t cnt, int w, T a,
-                               bool live) {
-  T u;
-  if (UPD) {
-    const T xi = W.x[i] + a * W.p[i];
-    u = W.r[i] + (-a) * W.q[i];
-    if (live) {
-      W.x[i] = xi;
-      W.r[i] = u;
-    }
-  } else {
-    u = W.u[i];
+// Loads of one row's entries.  WK > 0: the level's entry count w satisfies
+// 1 <= w <= WK and all WK loads are issued at once, branch-free (entries past
+// w re-read entry w-1 and are masked with column -1).
+template <int WK, class T>
+__device__ inline void load_entries(const int32_t* __restrict__ col_of, const T* __restrict__ val,
+                                    int64_t e0, int32_t cnt, int w, int (&col)[WK],
+                                    T (&av)[WK]) {
+#pragma unroll
+  for (int k = 0; k < WK; ++k) {
+    const int64_t e = e0 + (int64_t)(k < w ? k : w - 1) * cnt;
+    const int c = col_of[e];
+    col[k] = k < w ? c : -1;
+    av[k] = val[e];
   }
-  const int32_t pb = W.posb[i];
+}
+
+// Sum over a row's entries for a level with more than 8 entries per row (or
+// none): kAhead entries at a time.
+template <class T, class G>
+__device__ inline T entry_sum_loop(const int32_t* __restrict__ col_of, const T* __restrict__ val,
+                                   int64_t e0, int32_t cnt, int w, G&& term) {
   T sum = 0;
   for (int t0 = 0; t0 < w; t0 += kAhead) {
     int col[kAhead];
-    T av[kAhead], yv[kAhead], rv[kAhead];
+    T av[kAhead], tv[kAhead];
 #pragma unroll
     for (int k = 0; k < kAhead; ++k) {
       col[k] = -1;
       av[k] = 0;
       if (t0 + k < w) {
         const int64_t e = e0 + (int64_t)(t0 + k) * cnt;
-        col[k] = W.Lcol[e];
-        av[k] = W.Lnz[e];
+        col[k] = col_of[e];
+        av[k] = val[e];
       }
     }
 #pragma unroll
-    for (int k = 0; k < kAhead; ++k) {
-      yv[k] = 0;
-      rv[k] = 0;
-      if (col[k] >= 0) {
-        yv[k] = W.Y[col[k]];
-        rv[k] = W.rD[col[k]];
-      }
-    }
+    for (int k = 0; k < kAhead; ++k) tv[k] = col[k] >= 0 ? term(av[k], col[k]) : T(0);
 #pragma unroll
     for (int k = 0; k < kAhead; ++k)
-      if (col[k] >= 0) sum += (av[k] * yv[k]) * rv[k];
+      if (col[k] >= 0) sum += tv[k];
+  }
+  return sum;
+}
+
+// A row's memory work comes in two round trips: (1) everything addressed by
+// the row's position (own values, packed entries), (2) the gathers addressed
+// by the entries.  Scheduling barriers keep arithmetic that needs the stop
+// flag or alpha from being hoisted between them, where it would add a third.
+// READ: the stop flag and alpha are read from W.ks after the row's first
+// round trip has been issued (a and live are then ignored).
+template <bool UPD, int WK, class T, bool READ = false>
+__device__ inline void fwd_row(const FwdK<T>& W, int32_t i, int64_t e0, int32_t cnt, int w, T a,
+                               bool live) {
+  T xo = 0, po = 0, ro = 0, qo = 0, uo = 0;
+  if (UPD) {
+    xo = W.x[i];
+    po = W.p[i];
+    ro = W.r[i];
+    qo = W.q[i];
+  } else {
+    uo = W.u[i];
+  }
+  const int32_t pb = W.posb[i];
+  T sum = 0;
+  if constexpr (WK > 0) {
+    int col[WK];
+    T av[WK], yv[WK], rv[WK];
+    load_entries<WK>(W.Lcol, W.Lnz, e0, cnt, w, col, av);
+    __builtin_amdgcn_sched_barrier(0);
+    if (READ) {
+      live = !stopped(W.ks);
+      a = UPD ? (T)W.ks->alpha : T(0);
+    }
+#pragma unroll
+    for (int k = 0; k < WK; ++k) {
+      const int c = col[k] >= 0 ? col[k] : 0;
+      yv[k] = W.Y[c];
+      rv[k] = W.rD[c];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int k = 0; k < WK; ++k) {
+      const T t = (av[k] * yv[k]) * rv[k];
+      if (col[k] >= 0) sum += t;
+    }
+  } else {
+    if (READ) {
+      live = !stopped(W.ks);
+      a = UPD ? (T)W.ks->alpha : T(0);
+    }
+    sum = entry_sum_loop(W.Lcol, W.Lnz, e0, cnt, w,
+                         [&](T av, int c) { return (av * W.Y[c]) * W.rD[c]; });
+  }
+  // x and r are stored back unconditionally (their old values once the
+  // solve has stopped) so their loads are not sunk under the predicate
+  T u;
+  if (UPD) {
+    const T xi = xo + a * po;
+    u = ro + (-a) * qo;
+    W.x[i] = live ? xi : xo;
+    W.r[i] = live ? u : ro;
+  } else {
+    u = uo;
   }
   const T y = u - sum;
   if (live) {
@@ -216,36 +279,45 @@ struct BwdK {
   const KS* ks;
 };
 
-template <int DOT, class T>
+template <int DOT, int WK, class T, bool READ = false>
 __device__ inline void bwd_row(const BwdK<T>& W, int32_t pos, int64_t e0, int32_t cnt, int w,
                                bool live, double& s0, double& s1) {
   const int32_t i = W.rows[pos];
   const T yb = W.Yb[pos], d = W.Db[pos];
-  T sum = 0;
-  for (int t0 = 0; t0 < w; t0 += kAhead) {
-    int col[kAhead];
-    T av[kAhead], zv[kAhead];
+  T sum = 0, dvi = 0;
+  if constexpr (WK > 0) {
+    int col[WK];
+    T av[WK], zv[WK];
+    load_entries<WK>(W.Ucol, W.Unz, e0, cnt, w, col, av);
+    __builtin_amdgcn_sched_barrier(0);
+    if (READ) live = !stopped(W.ks);
+    // the dot operand is read before Z is written (it may alias Z), in the
+    // same round trip as the gathers
+    if (DOT != 0) dvi = W.dv[i];
 #pragma unroll
-    for (int k = 0; k < kAhead; ++k) {
-      col[k] = -1;
-      av[k] = 0;
-      if (t0 + k < w) {
-        const int64_t e = e0 + (int64_t)(t0 + k) * cnt;
-        col[k] = W.Ucol[e];
-        av[k] = W.Unz[e];
-      }
+    for (int k = 0; k < WK; ++k) zv[k] = W.Z[col[k] >= 0 ? col[k] : 0];
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int k = 0; k < WK; ++k) {
+      const T t = av[k] * zv[k];
+      if (col[k] >= 0) sum += t;
     }
-#pragma unroll
-    for (int k = 0; k < kAhead; ++k) zv[k] = col[k] >= 0 ? W.Z[col[k]] : T(0);
-#pragma unroll
-    for (int k = 0; k < kAhead; ++k)
-      if (col[k] >= 0) sum += av[k] * zv[k];
+  } else {
+    if (READ) live = !stopped(W.ks);
+    if (DOT != 0) dvi = W.dv[i];
+    sum = entry_sum_loop(W.Ucol, W.Unz, e0, cnt, w, [&](T av, int c) { return av * W.Z[c]; });
   }
   const T zi = (yb - sum) / d;
   if (live) W.Z[i] = zi;
-  if (DOT == 1 || DOT == 2) s0 += dd(W.dv[i], zi);
-  if (DOT == 3) { s0 += dd(zi, W.dv[i]); s1 += dd(zi, zi); }
-  if (DOT == 4) { s0 += dd(zi, zi); s1 += dd(W.dv[i], zi); }
+  if (DOT == 1 || DOT == 2) s0 += dd(dvi, zi);
+  if (DOT == 3) { s0 += dd(zi, dvi); s1 += dd(zi, zi); }
+  if (DOT == 4) { s0 += dd(zi, zi); s1 += dd(dvi, zi); }
+}
+
+// Compile-time entry bound for a level whose rows have w entries: 0 selects
+// the general loop.
+inline int entry_bound(int w) {
+  return w <= 0 ? 0 : w <= 2 ? 2 : w <= 4 ? 4 : w <= 8 ? 8 : 0;
 }
 
 // ----------------------------------------------------- kernel timing hook

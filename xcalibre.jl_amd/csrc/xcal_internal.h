@@ -56,6 +56,15 @@ constexpr int kSlice = 64;        // rows per slice of the sliced-ELL matrix sto
 constexpr int kRunBlock = 1024;   // workgroup size of the single-workgroup sweep runs
 constexpr int kSmallLevel = 256;  // levels below this many rows join a single-workgroup run
 
+// Cut c_x (x = 0..8) of a level of cnt rows into eighths at 64-row
+// boundaries; eighth x is worked by the workgroups on XCD x.
+inline int64_t eighth_cut(int64_t cnt, int x) { return x >= 8 ? cnt : cnt * x / 8 / 64 * 64; }
+// Workgroups of a level launch: 8 x (blocks of the longest eighth).
+inline int level_grid(int64_t cnt, int block = 256) {
+  int64_t m = 0;
+  for (int x = 0; x < 8; ++x) m = std::max(m, eighth_cut(cnt, x + 1) - eighth_cut(cnt, x));
+  return (int)(8 * ((m + block - 1) / block));
+}
 inline int grid_for(int64_t n, int block = kBlock) {
   int64_t g = (n + block - 1) / block;
   return (int)(g < 1 ? 1 : g);
@@ -96,6 +105,8 @@ struct xcal_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   xcal::DBuf<double> red_result;    // 64 slots: results of device reductions
+  xcal::DBuf<double> nostop;        // zeroed Krylov state (stop flag clear) for sweeps
+                                    // that run outside a Krylov solve
   double* host_pinned = nullptr;    // 64 doubles, pinned
   // optional HIP-event timing of one pressure-solve kernel
   int prof_kernel = -1;
@@ -146,6 +157,14 @@ struct xcal_mesh {
   // [lev_ptr_b[l], lev_ptr_b[l+1])
   int nlev_f = 0, nlev_b = 0;
   std::vector<int32_t> lev_ptr_f, lev_ptr_b;
+  // Row blocks of the gathering row kernels (SpMV and friends), in dispatch
+  // order: block b = rows [xblk[2b], xblk[2b+1]) runs on XCD b%8, which gets
+  // the (b%8)-th 64-aligned eighth of every forward level.  Rows are Morton
+  // ordered within a level, so an XCD's rows -- and the neighbours they
+  // gather -- sit in one eighth of the domain and share its L2.
+  std::vector<int32_t> xblk;
+  xcal::DBuf<int32_t> xblk_d;
+  int nxblk = 0;
   xcal::DBuf<int32_t> lev_ptr_f_d, lev_ptr_b_d, lev_rows_b, posb;
   // packed lower (forward) / upper (backward) factors: per level l,
   // lmeta[2l] = base, lmeta[2l+1] = width; entry t of the level's q-th row at

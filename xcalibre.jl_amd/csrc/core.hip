@@ -748,6 +748,11 @@ static xcal_mesh* upload_mesh(xcal_ctx* ctx, HostMesh& M) {
   upload(m->Lslot, M.Lslot, bytes);
   upload(m->Ucol, M.Ucol, bytes);
   upload(m->Uslot, M.Uslot, bytes);
+  {
+    std::vector<int32_t> upos(M.Ucol.size());
+    for (size_t e = 0; e < upos.size(); ++e) upos[e] = M.Ucol[e] < 0 ? -1 : M.posb[M.Ucol[e]];
+    upload(m->Upos, upos, bytes);
+  }
   upload(m->lmeta_f_d, M.lmeta_f, bytes);
   upload(m->lmeta_b_d, M.lmeta_b, bytes);
   m->device_bytes = bytes;
@@ -788,7 +793,7 @@ int64_t prepared_footprint(const PreparedMesh& P, bool turb) {
   mesh += 4 * (int64_t)(M.cf_ptr.size() + M.cf_face.size() * 3 + M.diag.size() + 3 * M.own.size() +
                         M.bc_cells.size() + M.bc_ptr.size() + M.bc_face.size() + 2 * M.rbeg.size() +
                         2 * M.colval.size() + 2 * M.rows_b.size() + 2 * M.Lcol.size() +
-                        2 * M.Ucol.size());
+                        3 * M.Ucol.size());
   const int64_t n = M.n, nf = M.nf, nz = M.nzlen;
   const int64_t fields = 8 * (22 * n + 7 * nf);
   const int64_t eqns = 8 * (3 * nz + 4 * n) + 8 * 2 * (nz + 3 * n);
@@ -837,6 +842,36 @@ xcal_mesh* upload_prepared(xcal_ctx* ctx, PreparedMesh& P) {
   if (batch) upload(m->face_case, fcase, bytes);
   m->device_bytes += bytes;
   make_sweep_plans(m, 0);
+  // pipelined batch sweeps: every case within their limits
+  // dep(e): the row (forward) or position (backward) that packed entry e
+  // depends on, -1 on padding; every one must lie within the LDS ring.
+  auto pipe_bound = [&](const std::vector<int32_t>& ptr, const std::vector<int32_t>& meta,
+                        const std::vector<int32_t>& cl, auto&& dep) {
+    int maxw = 0;
+    for (int k = 0; k < ncase; ++k) {
+      if (cl[k + 1] - cl[k] > kPipeLev) return -1;
+      for (int l = cl[k]; l < cl[k + 1]; ++l) {
+        const int32_t cnt = ptr[l + 1] - ptr[l], w = meta[2 * l + 1];
+        if (cnt > kPipeBlock) return -1;
+        maxw = std::max(maxw, w);
+        for (int t = 0; t < w; ++t)
+          for (int32_t q = 0; q < cnt; ++q) {
+            const int32_t d = dep((int64_t)meta[2 * l] + (int64_t)t * cnt + q);
+            if (d >= 0 && d < ptr[l + 1] - kPipeRing) return -1;
+          }
+      }
+    }
+    const int wk = entry_bound(maxw);
+    return wk > 0 ? wk : -1;
+  };
+  const char* nopipe = std::getenv("XCAL_NO_PIPE");
+  if (batch && !(nopipe && *nopipe == '1')) {
+    const HostMesh& M = P.U;
+    m->pipe_wf = pipe_bound(m->lev_ptr_f, m->lmeta_f, lf, [&](int64_t e) { return M.Lcol[e]; });
+    m->pipe_wb = pipe_bound(m->lev_ptr_b, m->lmeta_b, lb, [&](int64_t e) {
+      return M.Ucol[e] < 0 ? -1 : M.posb[M.Ucol[e]];
+    });
+  }
   return guard.release();
 }
 
@@ -950,6 +985,15 @@ int xcal_mesh_info(const xcal_mesh* m, int64_t* n_cells, int64_t* nnz, int32_t* 
     if (fl) *fl = m->nlev_f;
     if (bl) *bl = m->nlev_b;
     if (bytes) *bytes = m->device_bytes;
+    return 0;
+  });
+}
+
+int xcal_mesh_sweep_form(const xcal_mesh* m, int32_t* fwd, int32_t* bwd) {
+  return guarded([&] {
+    XCAL_CHECK(m && fwd && bwd, "NULL argument");
+    *fwd = m->pipe_wf;
+    *bwd = m->pipe_wb;
     return 0;
   });
 }

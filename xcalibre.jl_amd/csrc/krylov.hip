@@ -129,6 +129,13 @@ __global__ void __launch_bounds__(kRunBlock)
     __syncthreads();
   }
 }
+// out[e] = v[col[e]] (0 on padding)
+__global__ void __launch_bounds__(kBlock)
+    k_gather_cols(int64_t total, const int32_t* __restrict__ col, const double* __restrict__ v,
+                  double* __restrict__ out) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < total) out[e] = col[e] >= 0 ? v[col[e]] : 0.0;
+}
 // values of the packed triangular factors: out[e] = nz[slot[e]] (0 on padding)
 __global__ void __launch_bounds__(kBlock)
     k_gather_slots(int64_t total, const int32_t* __restrict__ slot, const double* __restrict__ nz,
@@ -220,6 +227,12 @@ void precond_update(xcal_eqn* e, int precon) {
       launch_check();
     }
     hipLaunchKernelGGL(k_recip, dim3(grid_for(n)), dim3(kBlock), 0, s, n, e->pD.p, e->prD.p);
+    if (m->pipe_wf > 0) {  // 1/D per packed lower entry, for the pipelined sweeps
+      if (e->Lrd.n != m->Lcol.n) e->Lrd.alloc(m->Lcol.n);
+      if (m->Lcol.n > 0)
+        hipLaunchKernelGGL(k_gather_cols, dim3(grid_for(m->Lcol.n)), dim3(kBlock), 0, s, m->Lcol.n,
+                           m->Lcol.p, e->prD.p, e->Lrd.p);
+    }
     if (e->Lnz.n != m->Lslot.n) e->Lnz.alloc(m->Lslot.n);
     if (e->Unz.n != m->Uslot.n) e->Unz.alloc(m->Uslot.n);
     if (e->Db.n != n) e->Db.alloc(n);

@@ -314,11 +314,6 @@ __device__ inline void bwd_row(const BwdK<T>& W, int32_t pos, int64_t e0, int32_
   if (DOT == 4) { s0 += dd(zi, zi); s1 += dd(dvi, zi); }
 }
 
-// Compile-time entry bound for a level whose rows have w entries: 0 selects
-// the general loop.
-inline int entry_bound(int w) {
-  return w <= 0 ? 0 : w <= 2 ? 2 : w <= 4 ? 4 : w <= 8 ? 8 : 0;
-}
 
 // ----------------------------------------------------- kernel timing hook
 inline hipEvent_t prof_event(xcal_ctx* c) {

@@ -56,6 +56,17 @@ constexpr int kSlice = 64;        // rows per slice of the sliced-ELL matrix sto
 constexpr int kRunBlock = 1024;   // workgroup size of the single-workgroup sweep runs
 constexpr int kSmallLevel = 256;  // levels below this many rows join a single-workgroup run
 
+// Compile-time entry bound for a level whose rows have w entries: 0 selects
+// the general loop.
+inline int entry_bound(int w) {
+  return w <= 0 ? 0 : w <= 2 ? 2 : w <= 4 ? 4 : w <= 8 ? 8 : 0;
+}
+// Limits of the pipelined one-workgroup-per-case sweeps of batch meshes:
+// rows per level (one per thread) and levels per case (staged in LDS).
+constexpr int kPipeBlock = 256;
+constexpr int kPipeLev = 2048;
+constexpr int kPipeRing = 4096;  // LDS ring of recent sweep outputs
+
 // Cut c_x (x = 0..8) of a level of cnt rows into eighths at 64-row
 // boundaries; eighth x is worked by the workgroups on XCD x.
 inline int64_t eighth_cut(int64_t cnt, int x) { return x >= 8 ? cnt : cnt * x / 8 / 64 * 64; }
@@ -162,6 +173,12 @@ struct xcal_mesh {
   // the (b%8)-th 64-aligned eighth of every forward level.  Rows are Morton
   // ordered within a level, so an XCD's rows -- and the neighbours they
   // gather -- sit in one eighth of the domain and share its L2.
+  // Backward-level position of each packed upper entry's column (-1 on
+  // padding): where the pipelined backward sweep finds it in its LDS ring.
+  xcal::DBuf<int32_t> Upos;
+  // Batch meshes: entry bound of the pipelined one-workgroup-per-case
+  // sweeps (forward, backward), or -1 when a case exceeds their limits.
+  int pipe_wf = -1, pipe_wb = -1;
   std::vector<int32_t> xblk;
   xcal::DBuf<int32_t> xblk_d;
   int nxblk = 0;
@@ -213,6 +230,8 @@ struct xcal_eqn {
   xcal::DBuf<double> pD;   // DILU D
   xcal::DBuf<double> prD;  // DILU 1/D
   xcal::DBuf<double> Lnz, Unz, Db;  // packed factor values; D in backward-level order
+  xcal::DBuf<double> Lrd;  // 1/D of each packed lower entry's column (0 on padding)
+  xcal::DBuf<double> Lprod;  // a_ij a_ji of each packed lower entry (DILU diagonal update)
   // Krylov workspace (allocated lazily)
   xcal::DBuf<double> work;    // solver vectors, 12 x n
   xcal::DBuf<double> kpart;   // reduction partial sums
@@ -220,7 +239,7 @@ struct xcal_eqn {
   void* kstate_host = nullptr;  // pinned copy
   int cg_batch = 0, bi_batch = 0;   // iterations of the last solve (first-batch size)
   // mixed precision: FP32 copies of the matrix and preconditioner, workspace
-  xcal::DBuf<float> work32, nz32, pD32, prD32, pS32, Lnz32, Unz32, Db32;
+  xcal::DBuf<float> work32, nz32, pD32, prD32, pS32, Lnz32, Unz32, Db32, Lrd32;
   // batched solves: first-batch sizes, per-case flags, host staging
   std::vector<int> batch_pred;
   xcal::DBuf<int32_t> case_active;

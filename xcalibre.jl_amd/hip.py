@@ -45,6 +45,7 @@ def lib():
             "xcal_mesh_create_batch": [_vp, C.POINTER(C.POINTER(A.MeshHost)), C.c_int32,
                                        C.POINTER(_vp)],
             "xcal_mesh_cases": [_vp, _pi32, C.POINTER(C.c_int64)],
+            "xcal_mesh_sweep_form": [_vp, _pi32, _pi32],
             "xcal_solve_system_batch": [_vp, C.POINTER(A.SolverSetupC), _vp, C.c_int32,
                                         C.POINTER(A.SolveStats)],
             "xcal_simple_run_batch": [_vp, C.c_int32, _pd, _pi32, _pi32],
@@ -185,6 +186,13 @@ class DeviceMesh:
     @classmethod
     def batch(cls, ctx: Context, meshes):
         return cls(ctx, BatchMeshInfo(meshes), _batch=list(meshes))
+
+    def sweep_form(self):
+        """(forward, backward) entry bounds of the pipelined batch sweeps, -1
+        where the sweeps step level by level."""
+        f, b = C.c_int32(), C.c_int32()
+        check(lib().xcal_mesh_sweep_form(self.h, C.byref(f), C.byref(b)))
+        return f.value, b.value
 
     def csr_pattern(self):
         rp = np.zeros(self.n + 1, np.int32)

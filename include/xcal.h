@@ -166,10 +166,12 @@ int xcal_mesh_destroy(xcal_mesh* mesh);
 /* n_cells, nnz, number of forward/backward DILU levels, device bytes */
 int xcal_mesh_info(const xcal_mesh* mesh, int64_t* n_cells, int64_t* nnz,
                    int32_t* fwd_levels, int32_t* bwd_levels, int64_t* device_bytes);
-/* Batch meshes: entries per row of the pipelined per-case DILU sweeps
- * (forward, backward), or -1 where a case exceeds their limits (more than 256
- * rows in a level, 2048 levels or 8 entries per row) and the sweeps step
- * level by level.  No reference counterpart (reports a device code path). */
+/* Entries per row of the pipelined one-workgroup DILU sweeps (forward,
+ * backward), or -1 where the sweeps step level by level.  Batch meshes: the
+ * per-case sweeps, -1 if any case exceeds their limits (more than 256 rows
+ * in a level, 2048 levels, 8 entries per row, or a dependency more than 4096
+ * rows back).  Single meshes: the widest pipelined run of small levels.  No
+ * reference counterpart (reports a device code path). */
 int xcal_mesh_sweep_form(const xcal_mesh* mesh, int32_t* fwd, int32_t* bwd);
 /* CSR pattern, 0-based (rowptr n+1, colval nnz) */
 int xcal_mesh_csr_pattern(const xcal_mesh* mesh, int32_t* rowptr, int32_t* colval);

@@ -17,6 +17,7 @@
 #include <functional>
 
 #include "solver_common.h"
+#include "sweep_pipe.h"
 
 namespace xcal {
 
@@ -265,22 +266,10 @@ __global__ void __launch_bounds__(kRunBlock)
 }
 
 // ------------------------------------------- pipelined per-case sweeps
-// The same sweeps as kb_fwd/kb_bwd (one workgroup per case, the same row to
-// thread assignment and summation orders, hence the same results), with the
-// memory latency taken off the level-to-level chain:
-//  * the case's level table is staged in LDS once;
-//  * each row's own loads -- its values, its packed entries and, per entry,
-//    1/D of the column (forward) or the column's backward position
-//    (backward) -- are issued P levels ahead into registers;
-//  * the outputs of the last kPipeRing rows (positions) live in an LDS ring,
-//    from which a row reads the values it depends on.
-// The mesh build admits a batch only if every case fits: <= kPipeBlock rows
-// per level, <= kPipeLev levels, <= WK entries per row and every dependency
-// within the ring.  The code is branch-free apart from the level loop, so the
-// compiler can count outstanding loads exactly and wait only for the level at
-// hand: threads past a level's last row repeat that row (same inputs, same
-// values stored, dot products masked), and entries past w load entry 0 of
-// the packed arrays and are masked.
+// One workgroup per case walks the case's levels with the pipelined bodies of
+// sweep_pipe.h (same results as kb_fwd/kb_bwd/kb_dilu_diag).  Within a case
+// nothing depends on rows before it, so the ring starts empty.
+
 // CG update of the pipelined path: x += a p ; r -= a q per case (the part of
 // the fused forward row that does not depend on other rows)
 template <class T>
@@ -297,266 +286,45 @@ __global__ void __launch_bounds__(kBlock)
   r[i] = ri;
 }
 
-template <class T, int WK>
-struct FRec {
-  int32_t i, pb;
-  T u;
-  int32_t col[WK];
-  T av[WK], rv[WK];
-};
-
-__device__ inline int stage_levels(const int32_t* __restrict__ lev_ptr,
-                                   const int32_t* __restrict__ meta, int u0, int nl,
-                                   int32_t* s_r0, int32_t* s_base, int32_t* s_w) {
-  for (int j = threadIdx.x; j <= nl; j += blockDim.x) {
-    s_r0[j] = lev_ptr[u0 + j];
-    if (j < nl) {
-      s_base[j] = meta[2 * (u0 + j)];
-      s_w[j] = meta[2 * (u0 + j) + 1];
-    }
-  }
-  __syncthreads();
-  return s_r0[0];
-}
-
-template <bool UPD, class T, int WK, int P>
+// UPD: x and r were updated by kb_cg_xr; the sweep's input is u = r
+template <bool UPD, bool FAR, class T, int WK>
 __global__ void __launch_bounds__(kPipeBlock)
     kb_fwd_pipe(FwdK<T> W, const T* __restrict__ Lrd, const int32_t* __restrict__ lev_ptr,
                 const int32_t* __restrict__ meta, const int32_t* __restrict__ case_lev,
                 const KS* ks) {
-  __shared__ T ring[kPipeRing];
-  __shared__ int32_t s_r0[kPipeLev + 1], s_base[kPipeLev], s_w[kPipeLev];
+  __shared__ PipeLDS<T> L;
   const int k = blockIdx.x;
   if (ks && stopped(ks + k)) return;  // uniform over the workgroup
-  const int u0 = case_lev[k], nl = case_lev[k + 1] - u0;
-  stage_levels(lev_ptr, meta, u0, nl, s_r0, s_base, s_w);
-  // UPD: the CG update x += a p, r -= a q was done by kb_cg_xr; u = r
-  const T* __restrict__ uin = UPD ? W.r : W.u;
-  auto fetch = [&](FRec<T, WK>& R, int l) {
-    const int lc = l < nl ? l : nl - 1;
-    const int32_t r0 = s_r0[lc], cnt = s_r0[lc + 1] - r0;
-    const int32_t q = threadIdx.x < cnt ? (int32_t)threadIdx.x : cnt - 1;
-    const int w = s_w[lc];
-    const int64_t e0 = (int64_t)s_base[lc] + q;
-    const int32_t i = r0 + q;
-    R.i = i;
-    R.pb = W.posb[i];
-    R.u = uin[i];
-#pragma unroll
-    for (int t = 0; t < WK; ++t) {
-      const int64_t e = t < w ? e0 + (int64_t)t * cnt : 0;
-      const int c = W.Lcol[e];
-      R.col[t] = t < w ? c : -1;
-      R.av[t] = W.Lnz[e];
-      R.rv[t] = Lrd[e];
-    }
-  };
-  auto process = [&](const FRec<T, WK>& R) {
-    T sum = 0;
-#pragma unroll
-    for (int t = 0; t < WK; ++t) {
-      const int c = R.col[t];
-      const T yv = ring[(c >= 0 ? c : 0) & (kPipeRing - 1)];
-      const T term = (R.av[t] * yv) * R.rv[t];
-      sum = c >= 0 ? sum + term : sum;
-    }
-    const T y = R.u - sum;
-    ring[R.i & (kPipeRing - 1)] = y;  // later levels read y here, not from Y
-    W.Yb[R.pb] = y;
-  };
-  // full groups of P levels without conditionals, then the last nl % P
-  FRec<T, WK> buf[P];
-#pragma unroll
-  for (int s = 0; s < P; ++s) fetch(buf[s], s);
-  const int nfull = nl - nl % P;
-  for (int l0 = 0; l0 < nfull; l0 += P) {
-#pragma unroll
-    for (int s = 0; s < P; ++s) {
-      process(buf[s]);
-      fetch(buf[s], l0 + s + P);
-      __syncthreads();
-    }
-  }
-#pragma unroll
-  for (int s = 0; s < P; ++s)
-    if (nfull + s < nl) {  // uniform
-      process(buf[s]);
-      __syncthreads();
-    }
+  const int u0 = case_lev[k];
+  fwd_pipe<false, FAR, T, WK, pipe_depth<WK>()>(W, UPD ? W.r : W.u, Lrd, lev_ptr, meta, u0,
+                                           case_lev[k + 1] - u0, false, L);
 }
 
-template <class T, int WK>
-struct BRec {
-  int32_t pos, i;
-  bool own;  // the thread's own row (not a repeat of the level's last row)
-  T yb, d, dvi;
-  int32_t up[WK];
-  T av[WK];
-};
-
-// The dot operand dv[i] is gathered Q levels before its row is processed
-// (its row index i arrives with the record, P levels ahead).
-template <int DOT, class T, int WK, int P>
+template <int DOT, bool FAR, class T, int WK>
 __global__ void __launch_bounds__(kPipeBlock)
     kb_bwd_pipe(BwdK<T> W, const int32_t* __restrict__ Upos, const int32_t* __restrict__ lev_ptr,
                 const int32_t* __restrict__ meta, const int32_t* __restrict__ case_lev,
                 const KS* ks) {
-  constexpr int Q = P / 2 > 0 ? P / 2 : 1;
-  __shared__ T ring[kPipeRing];
-  __shared__ int32_t s_r0[kPipeLev + 1], s_base[kPipeLev], s_w[kPipeLev];
+  __shared__ PipeLDS<T> L;
   const int k = blockIdx.x;
   if (ks && stopped(ks + k)) return;
-  const int u0 = case_lev[k], nl = case_lev[k + 1] - u0;
-  stage_levels(lev_ptr, meta, u0, nl, s_r0, s_base, s_w);
-  auto fetch = [&](BRec<T, WK>& R, int l) {
-    const int lc = l < nl ? l : nl - 1;
-    const int32_t p0 = s_r0[lc], cnt = s_r0[lc + 1] - p0;
-    R.own = (int32_t)threadIdx.x < cnt;
-    const int32_t q = R.own ? (int32_t)threadIdx.x : cnt - 1;
-    const int w = s_w[lc];
-    const int64_t e0 = (int64_t)s_base[lc] + q;
-    const int32_t pos = p0 + q;
-    R.pos = pos;
-    R.i = W.rows[pos];
-    R.yb = W.Yb[pos];
-    R.d = W.Db[pos];
-#pragma unroll
-    for (int t = 0; t < WK; ++t) {
-      const int64_t e = t < w ? e0 + (int64_t)t * cnt : 0;
-      const int up = Upos[e];  // -1 on padding
-      R.up[t] = t < w ? up : -1;
-      R.av[t] = W.Unz[e];
-    }
-  };
-  auto gather_dv = [&](BRec<T, WK>& R) {
-    if (DOT != 0) R.dvi = W.dv[R.i];
-  };
+  const int u0 = case_lev[k];
   double s0 = 0.0, s1 = 0.0;
-  auto process = [&](const BRec<T, WK>& R) {
-    T sum = 0;
-#pragma unroll
-    for (int t = 0; t < WK; ++t) {
-      const int up = R.up[t];
-      const T zv = ring[(up >= 0 ? up : 0) & (kPipeRing - 1)];
-      const T term = R.av[t] * zv;
-      sum = up >= 0 ? sum + term : sum;
-    }
-    const T zi = (R.yb - sum) / R.d;
-    ring[R.pos & (kPipeRing - 1)] = zi;
-    W.Z[R.i] = zi;
-    const T dvi = DOT != 0 ? R.dvi : T(0);
-    double a0 = 0.0, a1 = 0.0;
-    if (DOT == 1 || DOT == 2) a0 = dd(dvi, zi);
-    if (DOT == 3) { a0 = dd(zi, dvi); a1 = dd(zi, zi); }
-    if (DOT == 4) { a0 = dd(zi, zi); a1 = dd(dvi, zi); }
-    s0 = R.own ? s0 + a0 : s0;
-    s1 = R.own ? s1 + a1 : s1;
-  };
-  // full groups of P levels without conditionals, then the last nl % P
-  BRec<T, WK> buf[P];
-#pragma unroll
-  for (int s = 0; s < P; ++s) fetch(buf[s], s);
-#pragma unroll
-  for (int s = 0; s < Q && s < P; ++s) gather_dv(buf[s]);
-  const int nfull = nl - nl % P;
-  for (int l0 = 0; l0 < nfull; l0 += P) {
-#pragma unroll
-    for (int s = 0; s < P; ++s) {
-      if (Q < P) gather_dv(buf[(s + Q) % P]);  // level l0 + s + Q (a repeat past the end)
-      process(buf[s]);
-      fetch(buf[s], l0 + s + P);
-      __syncthreads();
-    }
-  }
-#pragma unroll
-  for (int s = 0; s < P; ++s)
-    if (nfull + s < nl) {  // uniform
-      if (s + Q < P && nfull + s + Q < nl) gather_dv(buf[s + Q < P ? s + Q : 0]);
-      process(buf[s]);
-      __syncthreads();
-    }
+  bwd_pipe<DOT, FAR, T, WK, pipe_depth<WK>()>(W, Upos, lev_ptr, meta, u0, case_lev[k + 1] - u0, false,
+                                         L, s0, s1);
   if (DOT != 0) store_partial<kPipeBlock>(s0, s1, W.partial, k);
 }
 
-// update_dilu_diagonal! per case, pipelined like kb_fwd_pipe: row j's D_j
-// starts from a_jj and takes D_j -= (a_ij a_ji) / D_i over its lower entries in
-// column order; prod holds a_ij a_ji per packed entry, D_i comes from the
-// ring.
-template <int WK, int P>
+template <bool FAR, int WK>
 __global__ void __launch_bounds__(kPipeBlock)
     kb_dilu_diag_pipe(double* D, const double* __restrict__ prod, const int32_t* __restrict__ Lcol,
                       const int32_t* __restrict__ lev_ptr, const int32_t* __restrict__ meta,
                       const int32_t* __restrict__ case_lev) {
-  __shared__ double ring[kPipeRing];
-  __shared__ int32_t s_r0[kPipeLev + 1], s_base[kPipeLev], s_w[kPipeLev];
+  __shared__ PipeLDS<double> L;
   const int k = blockIdx.x;
-  const int u0 = case_lev[k], nl = case_lev[k + 1] - u0;
-  stage_levels(lev_ptr, meta, u0, nl, s_r0, s_base, s_w);
-  struct Rec {
-    int32_t j;
-    double d0;
-    int32_t col[WK];
-    double pr[WK];
-  };
-  auto fetch = [&](Rec& R, int l) {
-    const int lc = l < nl ? l : nl - 1;
-    const int32_t r0 = s_r0[lc], cnt = s_r0[lc + 1] - r0;
-    const int32_t q = threadIdx.x < cnt ? (int32_t)threadIdx.x : cnt - 1;
-    const int w = s_w[lc];
-    const int64_t e0 = (int64_t)s_base[lc] + q;
-    R.j = r0 + q;
-    R.d0 = D[R.j];
-#pragma unroll
-    for (int t = 0; t < WK; ++t) {
-      const int64_t e = t < w ? e0 + (int64_t)t * cnt : 0;
-      const int c = Lcol[e];
-      R.col[t] = t < w ? c : -1;
-      R.pr[t] = prod[e];
-    }
-  };
-  auto process = [&](const Rec& R) {
-    double Dj = R.d0;
-#pragma unroll
-    for (int t = 0; t < WK; ++t) {
-      const int c = R.col[t];
-      const double di = ring[(c >= 0 ? c : 0) & (kPipeRing - 1)];
-      const double nd = Dj - R.pr[t] / di;
-      Dj = c >= 0 ? nd : Dj;
-    }
-    ring[R.j & (kPipeRing - 1)] = Dj;
-    D[R.j] = Dj;
-  };
-  Rec buf[P];
-#pragma unroll
-  for (int s = 0; s < P; ++s) fetch(buf[s], s);
-  const int nfull = nl - nl % P;
-  for (int l0 = 0; l0 < nfull; l0 += P) {
-#pragma unroll
-    for (int s = 0; s < P; ++s) {
-      process(buf[s]);
-      fetch(buf[s], l0 + s + P);
-      __syncthreads();
-    }
-  }
-#pragma unroll
-  for (int s = 0; s < P; ++s)
-    if (nfull + s < nl) {  // uniform
-      process(buf[s]);
-      __syncthreads();
-    }
-}
-
-// prod[e] = a_ij a_ji of packed lower entry e (nz[trans[slot]] nz[slot]; 0 on
-// padding)
-__global__ void __launch_bounds__(kBlock)
-    kb_lower_products(int64_t total, const int32_t* __restrict__ slot,
-                      const int32_t* __restrict__ trans, const double* __restrict__ nz,
-                      double* __restrict__ prod) {
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= total) return;
-  const int32_t k = slot[e];
-  const int32_t kt = k >= 0 ? trans[k] : -1;
-  prod[e] = kt >= 0 ? nz[kt] * nz[k] : 0.0;
+  const int u0 = case_lev[k];
+  diag_pipe<FAR, WK, pipe_depth<WK>()>(D, prod, Lcol, lev_ptr, meta, u0, case_lev[k + 1] - u0,
+                                       false, L);
 }
 
 // update_dilu_diagonal! per case: one workgroup walks the case's levels
@@ -761,14 +529,13 @@ static void bsweep_fwd(const BSolver<T>& S, const FwdK<T>& W, const KS* ks) {
   if (UPD && m->pipe_wf > 0)
     hipLaunchKernelGGL(kb_cg_xr<T>, dim3(S.nblk), dim3(kBlock), 0, S.s, S.B, W.x, W.p, W.r, W.q,
                        ks);
-  switch (m->pipe_wf) {
-    case 2: pipe(kb_fwd_pipe<UPD, T, 2, 8>); return;
-    case 4: pipe(kb_fwd_pipe<UPD, T, 4, 6>); return;
-    case 8: pipe(kb_fwd_pipe<UPD, T, 8, 4>); return;
-    default:
-      hipLaunchKernelGGL((kb_fwd<UPD, T>), dim3(S.ncase), dim3(kRunBlock), 0, S.s, W,
-                         m->lev_ptr_f_d.p, m->lmeta_f_d.p, m->case_lev_f_d.p, ks);
-  }
+  if (m->pipe_wf > 0)
+    with_pipe_form(m->pipe_wf, m->pipe_far_f, [&](auto WK, auto FAR) {
+      pipe(kb_fwd_pipe<UPD, decltype(FAR)::value, T, decltype(WK)::value>);
+    });
+  else
+    hipLaunchKernelGGL((kb_fwd<UPD, T>), dim3(S.ncase), dim3(kRunBlock), 0, S.s, W,
+                       m->lev_ptr_f_d.p, m->lmeta_f_d.p, m->case_lev_f_d.p, ks);
 }
 template <int DOT, class T>
 static void bsweep_bwd(const BSolver<T>& S, const BwdK<T>& W, const KS* ks) {
@@ -777,14 +544,13 @@ static void bsweep_bwd(const BSolver<T>& S, const BwdK<T>& W, const KS* ks) {
     hipLaunchKernelGGL(kern, dim3(S.ncase), dim3(kPipeBlock), 0, S.s, W, m->Upos.p,
                        m->lev_ptr_b_d.p, m->lmeta_b_d.p, m->case_lev_b_d.p, ks);
   };
-  switch (m->pipe_wb) {
-    case 2: pipe(kb_bwd_pipe<DOT, T, 2, 8>); return;
-    case 4: pipe(kb_bwd_pipe<DOT, T, 4, 6>); return;
-    case 8: pipe(kb_bwd_pipe<DOT, T, 8, 4>); return;
-    default:
-      hipLaunchKernelGGL((kb_bwd<DOT, T>), dim3(S.ncase), dim3(kRunBlock), 0, S.s, W,
-                         m->lev_ptr_b_d.p, m->lmeta_b_d.p, m->case_lev_b_d.p, ks);
-  }
+  if (m->pipe_wb > 0)
+    with_pipe_form(m->pipe_wb, m->pipe_far_b, [&](auto WK, auto FAR) {
+      pipe(kb_bwd_pipe<DOT, decltype(FAR)::value, T, decltype(WK)::value>);
+    });
+  else
+    hipLaunchKernelGGL((kb_bwd<DOT, T>), dim3(S.ncase), dim3(kRunBlock), 0, S.s, W,
+                       m->lev_ptr_b_d.p, m->lmeta_b_d.p, m->case_lev_b_d.p, ks);
 }
 template <class T>
 static FwdK<T> bfwdk(const BSolver<T>& S, const T* u, T* Y, T* Yb) {
@@ -1065,18 +831,17 @@ void precond_update_batch(xcal_eqn* e) {
   if (m->pipe_wf > 0) {
     const int64_t nl = m->Lslot.n;
     if (e->Lprod.n != nl) e->Lprod.alloc(nl);
-    hipLaunchKernelGGL(kb_lower_products, dim3(grid_for(nl)), dim3(kBlock), 0, s, nl, m->Lslot.p,
+    hipLaunchKernelGGL(k_lower_products, dim3(grid_for(nl)), dim3(kBlock), 0, s, nl, m->Lslot.p,
                        m->trans.p, e->nzval.p, e->Lprod.p);
     auto pipe = [&](auto kern) {
       hipLaunchKernelGGL(kern, dim3(m->ncase), dim3(kPipeBlock), 0, s, e->pD.p,
                          (const double*)e->Lprod.p, m->Lcol.p, m->lev_ptr_f_d.p, m->lmeta_f_d.p,
                          m->case_lev_f_d.p);
     };
-    switch (m->pipe_wf) {
-      case 2: pipe(kb_dilu_diag_pipe<2, 8>); return;
-      case 4: pipe(kb_dilu_diag_pipe<4, 6>); return;
-      default: pipe(kb_dilu_diag_pipe<8, 4>); return;
-    }
+    with_pipe_form(m->pipe_wf, m->pipe_far_f, [&](auto WK, auto FAR) {
+      pipe(kb_dilu_diag_pipe<decltype(FAR)::value, decltype(WK)::value>);
+    });
+    return;
   }
   hipLaunchKernelGGL(kb_dilu_diag, dim3(m->ncase), dim3(kRunBlock), 0, s,
                      matk(m, (const double*)e->nzval.p), m->trans.p, e->pD.p, m->lev_ptr_f_d.p,

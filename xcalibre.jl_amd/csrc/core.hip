@@ -82,7 +82,7 @@ static std::vector<SweepStep> plan_levels(const std::vector<int32_t>& ptr, int n
       st.grid = level_grid(cnt);
     } else {
       int e = l;
-      while (e < nlev && ptr[e + 1] - ptr[e] < small) ++e;
+      while (e < nlev && e - l < kPipeLev && ptr[e + 1] - ptr[e] < small) ++e;
       st.lev1 = e;
       st.grid = 1;
     }
@@ -96,13 +96,23 @@ static std::vector<SweepStep> plan_levels(const std::vector<int32_t>& ptr, int n
 }
 
 // Row blocks of kBlock rows for the gathering kernels: each forward level
-// [a, b) is cut at 64-aligned eighths; eighth x of every level goes to XCD x
-// (the dispatcher deals block b to XCD b%8).  Shorter XCD lists are padded
-// with empty blocks.
+// [a, b) of at least 8 kBlock rows is cut at 64-aligned eighths, and eighth x
+// of every such level goes to XCD x (the dispatcher deals block b to XCD
+// b%8).  A maximal stretch of consecutive smaller levels is cut the same way
+// as one range: neighbouring small levels are neighbouring bands of space.
+// Shorter XCD lists are padded with empty blocks.
 std::vector<int32_t> xcd_row_blocks(const std::vector<int32_t>& lev_ptr, int nlev) {
   std::vector<std::vector<int32_t>> per(8);
-  for (int l = 0; l < nlev; ++l) {
-    const int64_t a = lev_ptr[l], cnt = lev_ptr[l + 1] - a;
+  std::vector<std::pair<int64_t, int64_t>> ranges;
+  for (int l = 0; l < nlev;) {
+    int e = l + 1;
+    if (lev_ptr[l + 1] - lev_ptr[l] < 8 * kBlock)
+      while (e < nlev && lev_ptr[e + 1] - lev_ptr[e] < 8 * kBlock) ++e;
+    ranges.push_back({lev_ptr[l], lev_ptr[e]});
+    l = e;
+  }
+  for (const auto& rg : ranges) {
+    const int64_t a = rg.first, cnt = rg.second - rg.first;
     for (int x = 0; x < 8; ++x) {
       const int64_t c0 = a + (cnt * x / 8) / 64 * 64;
       const int64_t c1 = x == 7 ? a + cnt : a + (cnt * (x + 1) / 8) / 64 * 64;
@@ -846,8 +856,9 @@ xcal_mesh* upload_prepared(xcal_ctx* ctx, PreparedMesh& P) {
   // dep(e): the row (forward) or position (backward) that packed entry e
   // depends on, -1 on padding; every one must lie within the LDS ring.
   auto pipe_bound = [&](const std::vector<int32_t>& ptr, const std::vector<int32_t>& meta,
-                        const std::vector<int32_t>& cl, auto&& dep) {
+                        const std::vector<int32_t>& cl, bool& far, auto&& dep) {
     int maxw = 0;
+    far = false;
     for (int k = 0; k < ncase; ++k) {
       if (cl[k + 1] - cl[k] > kPipeLev) return -1;
       for (int l = cl[k]; l < cl[k + 1]; ++l) {
@@ -857,7 +868,7 @@ xcal_mesh* upload_prepared(xcal_ctx* ctx, PreparedMesh& P) {
         for (int t = 0; t < w; ++t)
           for (int32_t q = 0; q < cnt; ++q) {
             const int32_t d = dep((int64_t)meta[2 * l] + (int64_t)t * cnt + q);
-            if (d >= 0 && d < ptr[l + 1] - kPipeRing) return -1;
+            far = far || (d >= 0 && d < ptr[l + 1] - kPipeRing);
           }
       }
     }
@@ -865,10 +876,48 @@ xcal_mesh* upload_prepared(xcal_ctx* ctx, PreparedMesh& P) {
     return wk > 0 ? wk : -1;
   };
   const char* nopipe = std::getenv("XCAL_NO_PIPE");
+  if (!batch && !(nopipe && *nopipe == '1')) {
+    // single meshes: the runs of small levels (dependencies before a run
+    // reach its ring through the preload)
+    const HostMesh& M = P.U;
+    auto run_bound = [&](SweepStep& st, const std::vector<int32_t>& ptr,
+                         const std::vector<int32_t>& meta, size_t npacked, auto&& dep) {
+      st.pipe = 0;
+      st.far = false;
+      if (npacked == 0) return;  // masked entries read entry 0 of the packed arrays
+      if (st.lev1 - st.lev0 < kPipeMinLevels) return;
+      int maxw = 0;
+      bool far = false;
+      for (int l = st.lev0; l < st.lev1; ++l) {
+        const int32_t cnt = ptr[l + 1] - ptr[l], w = meta[2 * l + 1];
+        if (cnt > kPipeBlock) return;
+        maxw = std::max(maxw, w);
+        for (int t = 0; t < w; ++t)
+          for (int32_t q = 0; q < cnt; ++q) {
+            const int32_t d = dep((int64_t)meta[2 * l] + (int64_t)t * cnt + q);
+            far = far || (d >= 0 && d < ptr[l + 1] - kPipeRing);
+          }
+      }
+      st.pipe = entry_bound(std::max(maxw, 1));
+      st.far = far;
+    };
+    for (SweepStep& st : m->plan_f)
+      if (st.grid == 1) {
+        run_bound(st, m->lev_ptr_f, m->lmeta_f, M.Lcol.size(),
+                  [&](int64_t e) { return M.Lcol[e]; });
+        m->pipe_runs = m->pipe_runs || st.pipe > 0;
+      }
+    for (SweepStep& st : m->plan_b)
+      if (st.grid == 1)
+        run_bound(st, m->lev_ptr_b, m->lmeta_b, M.Ucol.size(), [&](int64_t e) {
+          return M.Ucol[e] < 0 ? -1 : M.posb[M.Ucol[e]];
+        });
+  }
   if (batch && !(nopipe && *nopipe == '1')) {
     const HostMesh& M = P.U;
-    m->pipe_wf = pipe_bound(m->lev_ptr_f, m->lmeta_f, lf, [&](int64_t e) { return M.Lcol[e]; });
-    m->pipe_wb = pipe_bound(m->lev_ptr_b, m->lmeta_b, lb, [&](int64_t e) {
+    m->pipe_wf = pipe_bound(m->lev_ptr_f, m->lmeta_f, lf, m->pipe_far_f,
+                            [&](int64_t e) { return M.Lcol[e]; });
+    m->pipe_wb = pipe_bound(m->lev_ptr_b, m->lmeta_b, lb, m->pipe_far_b, [&](int64_t e) {
       return M.Ucol[e] < 0 ? -1 : M.posb[M.Ucol[e]];
     });
   }
@@ -994,6 +1043,13 @@ int xcal_mesh_sweep_form(const xcal_mesh* m, int32_t* fwd, int32_t* bwd) {
     XCAL_CHECK(m && fwd && bwd, "NULL argument");
     *fwd = m->pipe_wf;
     *bwd = m->pipe_wb;
+    if (!m->batch) {  // single meshes: the widest pipelined run, -1 if none
+      *fwd = *bwd = -1;
+      for (const SweepStep& st : m->plan_f)
+        if (st.pipe > 0) *fwd = std::max(*fwd, st.pipe);
+      for (const SweepStep& st : m->plan_b)
+        if (st.pipe > 0) *bwd = std::max(*bwd, st.pipe);
+    }
     return 0;
   });
 }

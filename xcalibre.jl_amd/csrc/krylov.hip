@@ -13,6 +13,7 @@
 #include <cmath>
 
 #include "device_common.h"
+#include "sweep_pipe.h"
 
 namespace xcal {
 
@@ -129,6 +130,16 @@ __global__ void __launch_bounds__(kRunBlock)
     __syncthreads();
   }
 }
+// update_dilu_diagonal! over a run of small levels, pipelined (sweep_pipe.h)
+template <bool FAR, int WK>
+__global__ void __launch_bounds__(kPipeBlock)
+    k_dilu_diag_run_pipe(double* D, const double* __restrict__ prod,
+                         const int32_t* __restrict__ Lcol, const int32_t* __restrict__ lev_ptr,
+                         const int32_t* __restrict__ meta, int lev0, int lev1) {
+  __shared__ PipeLDS<double> L;
+  diag_pipe<FAR, WK, pipe_depth<WK>()>(D, prod, Lcol, lev_ptr, meta, lev0, lev1 - lev0, true, L);
+}
+
 // out[e] = v[col[e]] (0 on padding)
 __global__ void __launch_bounds__(kBlock)
     k_gather_cols(int64_t total, const int32_t* __restrict__ col, const double* __restrict__ v,
@@ -214,11 +225,25 @@ void precond_update(xcal_eqn* e, int precon) {
                        e->nzval.p, e->pD.p);
     launch_check();
     const MatK A = matk(m, e->nzval.p);
+    if (!m->batch && m->pipe_runs) {  // a_ij a_ji per packed lower entry
+      if (e->Lprod.n != m->Lslot.n) e->Lprod.alloc(m->Lslot.n);
+      hipLaunchKernelGGL(k_lower_products, dim3(grid_for(m->Lslot.n)), dim3(kBlock), 0, s,
+                         m->Lslot.n, m->Lslot.p, m->trans.p, e->nzval.p, e->Lprod.p);
+    }
     if (m->batch) precond_update_batch(e);  // one workgroup per case
     else
     for (const SweepStep& st : m->plan_f) {
       if (st.lev1 == 1) continue;  // level 0 rows have no lower neighbours
-      if (st.grid == 1)
+      if (st.grid == 1 && st.pipe > 0) {
+        auto pipe = [&](auto kern) {
+          hipLaunchKernelGGL(kern, dim3(1), dim3(kPipeBlock), 0, s, e->pD.p,
+                             (const double*)e->Lprod.p, m->Lcol.p, m->lev_ptr_f_d.p,
+                             m->lmeta_f_d.p, st.lev0, st.lev1);
+        };
+        with_pipe_form(st.pipe, st.far, [&](auto WK, auto FAR) {
+          pipe(k_dilu_diag_run_pipe<decltype(FAR)::value, decltype(WK)::value>);
+        });
+      } else if (st.grid == 1)
         hipLaunchKernelGGL(k_dilu_diag_run, dim3(1), dim3(kRunBlock), 0, s, A, m->trans.p, e->pD.p,
                            m->lev_ptr_f_d.p, st.lev0, st.lev1);
       else
@@ -227,7 +252,7 @@ void precond_update(xcal_eqn* e, int precon) {
       launch_check();
     }
     hipLaunchKernelGGL(k_recip, dim3(grid_for(n)), dim3(kBlock), 0, s, n, e->pD.p, e->prD.p);
-    if (m->pipe_wf > 0) {  // 1/D per packed lower entry, for the pipelined sweeps
+    if (m->pipe_wf > 0 || m->pipe_runs) {  // 1/D per packed lower entry, for the pipelined sweeps
       if (e->Lrd.n != m->Lcol.n) e->Lrd.alloc(m->Lcol.n);
       if (m->Lcol.n > 0)
         hipLaunchKernelGGL(k_gather_cols, dim3(grid_for(m->Lcol.n)), dim3(kBlock), 0, s, m->Lcol.n,

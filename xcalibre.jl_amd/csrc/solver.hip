@@ -29,6 +29,7 @@
 #include <functional>
 
 #include "solver_common.h"
+#include "sweep_pipe.h"
 
 namespace xcal {
 
@@ -325,6 +326,44 @@ __global__ void __launch_bounds__(kRunBlock)
   if (DOT != 0) store_partial<kRunBlock>(s0, s1, W.partial, slot);
 }
 
+// Pipelined runs (sweep_pipe.h): one workgroup per run of small levels,
+// with the ring preloaded from the launches before it.  UPD runs have had
+// x += a p, r -= a q done by k_cg_xr over their rows; their input is u = r.
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_cg_xr(int32_t row0, int32_t row1, T* __restrict__ x, const T* __restrict__ p,
+            T* __restrict__ r, const T* __restrict__ q, const KS* ks) {
+  const int32_t i = row0 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (stopped(ks) || i >= row1) return;
+  const T a = (T)ks->alpha;
+  const T xi = x[i] + a * p[i];
+  const T ri = r[i] + (-a) * q[i];
+  x[i] = xi;
+  r[i] = ri;
+}
+
+template <bool UPD, bool FAR, class T, int WK>
+__global__ void __launch_bounds__(kPipeBlock)
+    k_fwd_run_pipe(FwdK<T> W, const T* __restrict__ Lrd, const int32_t* __restrict__ lev_ptr,
+                   const int32_t* __restrict__ meta, int lev0, int lev1) {
+  __shared__ PipeLDS<T> L;
+  if (stopped(W.ks)) return;  // uniform over the workgroup
+  fwd_pipe<true, FAR, T, WK, pipe_depth<WK>()>(W, UPD ? W.r : W.u, Lrd, lev_ptr, meta, lev0,
+                                          lev1 - lev0, true, L);
+}
+
+template <int DOT, bool FAR, class T, int WK>
+__global__ void __launch_bounds__(kPipeBlock)
+    k_bwd_run_pipe(BwdK<T> W, const int32_t* __restrict__ Upos, const int32_t* __restrict__ lev_ptr,
+                   const int32_t* __restrict__ meta, int lev0, int lev1, int slot) {
+  __shared__ PipeLDS<T> L;
+  if (stopped(W.ks)) return;
+  double s0 = 0.0, s1 = 0.0;
+  bwd_pipe<DOT, FAR, T, WK, pipe_depth<WK>()>(W, Upos, lev_ptr, meta, lev0, lev1 - lev0, true, L, s0,
+                                         s1);
+  if (DOT != 0) store_partial<kPipeBlock>(s0, s1, W.partial, slot);
+}
+
 // ------------------------------------------------------------ host side
 // One Krylov solve's view of an equation in value type T: the matrix, the
 // preconditioner arrays and 12 n-vectors of workspace.
@@ -337,6 +376,7 @@ struct Solver {
   MatKT<T> A;
   const T *D, *rD, *Sj;  // DILU D and 1/D; Jacobi 1/|a_ii|
   const T *Lnz, *Unz, *Db;  // packed DILU factors, D in backward-level order
+  const T* Lrd;             // 1/D per packed lower entry (pipelined runs)
   T* work;
   KS* ks;
   double* partial;
@@ -385,6 +425,7 @@ static Solver<double> make_solver(xcal_eqn* e) {
   S.Lnz = e->Lnz.p;
   S.Unz = e->Unz.p;
   S.Db = e->Db.p;
+  S.Lrd = e->Lrd.p;
   S.work = e->work.p;
   S.ks = (KS*)e->kstate.p;
   S.partial = e->kpart.p;
@@ -416,6 +457,11 @@ static Solver<float> make_solver32(xcal_eqn* e) {
     if (e->Unz.n > 0)
       hipLaunchKernelGGL(k_to_float, dim3(grid_for(e->Unz.n)), dim3(kBlock), 0, s, e->Unz.n,
                          e->Unz.p, e->Unz32.p);
+    if (e->Lrd.n > 0) {
+      if (e->Lrd32.n != e->Lrd.n) e->Lrd32.alloc(e->Lrd.n);
+      hipLaunchKernelGGL(k_to_float, dim3(grid_for(e->Lrd.n)), dim3(kBlock), 0, s, e->Lrd.n,
+                         e->Lrd.p, e->Lrd32.p);
+    }
   } else if (e->precon == XCAL_PRECON_JACOBI) {
     hipLaunchKernelGGL(k_to_float, dim3(grid_for(n)), dim3(kBlock), 0, s, n, e->pS.p, e->pS32.p);
   }
@@ -432,6 +478,7 @@ static Solver<float> make_solver32(xcal_eqn* e) {
   S.Lnz = e->Lnz32.p;
   S.Unz = e->Unz32.p;
   S.Db = e->Db32.p;
+  S.Lrd = e->Lrd32.p;
   S.work = e->work32.p;
   S.ks = (KS*)e->kstate.p;
   S.partial = e->kpart.p;
@@ -452,7 +499,18 @@ template <bool UPD, class T>
 static void sweep_fwd(const Solver<T>& S, const FwdK<T>& W) {
   const xcal_mesh* m = S.m;
   for (const SweepStep& st : m->plan_f) {
-    if (st.grid == 1)
+    if (st.grid == 1 && st.pipe > 0) {
+      if (UPD)
+        hipLaunchKernelGGL(k_cg_xr<T>, dim3(grid_for(st.row1 - st.row0)), dim3(kBlock), 0, S.s,
+                           st.row0, st.row1, W.x, W.p, W.r, W.q, W.ks);
+      auto pipe = [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3(1), dim3(kPipeBlock), 0, S.s, W, S.Lrd, m->lev_ptr_f_d.p,
+                           m->lmeta_f_d.p, st.lev0, st.lev1);
+      };
+      with_pipe_form(st.pipe, st.far, [&](auto WK, auto FAR) {
+        pipe(k_fwd_run_pipe<UPD, decltype(FAR)::value, T, decltype(WK)::value>);
+      });
+    } else if (st.grid == 1)
       hipLaunchKernelGGL((k_fwd_run<UPD, T>), dim3(1), dim3(kRunBlock), 0, S.s, W,
                          m->lev_ptr_f_d.p, m->lmeta_f_d.p, st.lev0, st.lev1);
     else {
@@ -476,7 +534,15 @@ template <int DOT, class T>
 static void sweep_bwd(const Solver<T>& S, const BwdK<T>& W) {
   const xcal_mesh* m = S.m;
   for (const SweepStep& st : m->plan_b) {
-    if (st.grid == 1)
+    if (st.grid == 1 && st.pipe > 0) {
+      auto pipe = [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3(1), dim3(kPipeBlock), 0, S.s, W, m->Upos.p,
+                           m->lev_ptr_b_d.p, m->lmeta_b_d.p, st.lev0, st.lev1, st.slot);
+      };
+      with_pipe_form(st.pipe, st.far, [&](auto WK, auto FAR) {
+        pipe(k_bwd_run_pipe<DOT, decltype(FAR)::value, T, decltype(WK)::value>);
+      });
+    } else if (st.grid == 1)
       hipLaunchKernelGGL((k_bwd_run<DOT, T>), dim3(1), dim3(kRunBlock), 0, S.s, W,
                          m->lev_ptr_b_d.p, m->lmeta_b_d.p, st.lev0, st.lev1, st.slot);
     else {

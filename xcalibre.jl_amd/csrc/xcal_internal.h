@@ -66,6 +66,9 @@ inline int entry_bound(int w) {
 constexpr int kPipeBlock = 256;
 constexpr int kPipeLev = 2048;
 constexpr int kPipeRing = 4096;  // LDS ring of recent sweep outputs
+// Runs shorter than this keep the level-stepping form: the pipeline's set-up
+// (level table, ring preload, P levels of prefetch) outweighs its gain.
+constexpr int kPipeMinLevels = 16;
 
 // Cut c_x (x = 0..8) of a level of cnt rows into eighths at 64-row
 // boundaries; eighth x is worked by the workgroups on XCD x.
@@ -133,6 +136,8 @@ struct xcal_ctx {
 // (backward); the launch writes partial sums to slots [slot, slot + grid).
 struct SweepStep {
   int32_t lev0, lev1, row0, row1, grid, slot;
+  int32_t pipe = 0;  // runs: entry bound of the pipelined form, 0 = level stepping
+  bool far = false;  // some dependency of the run lies before its LDS ring
 };
 
 // Device mesh in SoA form, 0-based int32 indices, device numbering.
@@ -179,6 +184,9 @@ struct xcal_mesh {
   // Batch meshes: entry bound of the pipelined one-workgroup-per-case
   // sweeps (forward, backward), or -1 when a case exceeds their limits.
   int pipe_wf = -1, pipe_wb = -1;
+  bool pipe_far_f = false, pipe_far_b = false;  // some dependency before the ring
+  // Single meshes: some forward run is pipelined (needs Lrd and Lprod)
+  bool pipe_runs = false;
   std::vector<int32_t> xblk;
   xcal::DBuf<int32_t> xblk_d;
   int nxblk = 0;

@@ -98,9 +98,9 @@ __device__ inline T row_sum(const MatKT<T>& A, int64_t i, XV&& xv) {
       const int32_t k = k0 + u * kSlice;
       col[u] = -1;
       a[u] = 0;
-      if (k < k1) {
-        col[u] = A.colval[k];
-        a[u] = A.nz[k];
+      if (k < k1) {  // streamed once: keep them out of the way of the gathers
+        col[u] = __builtin_nontemporal_load(&A.colval[k]);
+        a[u] = __builtin_nontemporal_load(&A.nz[k]);
       }
     }
 #pragma unroll

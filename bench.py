@@ -14,8 +14,16 @@ roofline  = algorithmic bytes of the dominant pressure-solve kernel per launch
             / its mean launch time from HIP events on the library's stream
 cpu_baseline = the CPU oracle (restatement of the reference's FP64 path)
             timed on the host's cores over a bounded sample of the same run.
+detail.mixed_precision = the same channel with FP32 inner Krylov solves in
+            FP64 iterative refinement, and its per-field error against FP64;
+detail.batched = 256 batched 2D cavities (FP64 and FP32-IR pressure), the
+            form that --mode batch reports in full.  --quick skips both.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+Other modes: --mode batch (batched cavities), --mode serve (k-omega serving
+with Poisson arrivals: p50/p99 time to converged), --mode calib (a plain
+256 MiB copy for HBM counter calibration).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--quick]
 """
 from __future__ import annotations
 
@@ -215,13 +223,19 @@ def batch_cases(ncase, seed=SEED_BATCH):
 
 
 def run_batch(args, x):
+    print(json.dumps(batch_line(args, x, args.precision)))
+
+
+def batch_line(args, x, precision, steps=None, warmup=None):
     """Batched form: independent 2D cavities stepped together on one batch mesh."""
     api, H = x.api, x.hip
+    steps = args.steps if steps is None else steps
+    warmup = args.warmup if warmup is None else warmup
     sizes, Re = batch_cases(args.cases)
     t0 = time.perf_counter()
     meshes = [x.mesh.cavity2d(int(m)) for m in sizes]
     t_mesh = time.perf_counter() - t0
-    model, config = cavity(api, meshes[0], args.warmup + args.steps, args.precision)
+    model, config = cavity(api, meshes[0], warmup + steps, precision)
     setup = api.simple_setup(model, config, pref=0.0)
     ctx = H.Context(0)
     t0 = time.perf_counter()
@@ -230,65 +244,65 @@ def run_batch(args, x):
     sim.set_case_nu(1.0 / Re)
     ctx.synchronize()
     t_upload = time.perf_counter() - t0
-    if args.warmup:
-        sim.run_batch(args.warmup)
+    if warmup:
+        sim.run_batch(warmup)
     ctx.time_kernel(args.roofline_kernel, True)
     ctx.synchronize()
     t0 = time.perf_counter()
-    out = sim.run_batch(args.steps)
+    out = sim.run_batch(steps)
     ctx.synchronize()
     t1 = time.perf_counter()
     launches, kms = ctx.kernel_time()
     ctx.time_kernel(args.roofline_kernel, False)
-    if out["done"] != args.steps:
-        raise RuntimeError(f"batch stopped after {out['done']} of {args.steps} steps")
+    if out["done"] != steps:
+        raise RuntimeError(f"batch stopped after {out['done']} of {steps} steps")
     wall = t1 - t0
     n = dm.n
-    vb = 4 if args.precision == "fp32-ir" else 8
+    vb = 4 if precision == "fp32-ir" else 8
     nbytes = kernel_bytes(args.roofline_kernel, n, dm.nnz, vb)
     avg_s = kms / 1e3 / max(launches, 1)
     achieved = nbytes / avg_s / 1e9
     err = None
-    if args.precision != "fp64":
-        m64, c64 = cavity(api, meshes[0], args.warmup + args.steps)
+    if precision != "fp64":
+        m64, c64 = cavity(api, meshes[0], warmup + steps)
         s64 = H.Simple(dm, api.simple_setup(m64, c64, pref=0.0))
         s64.set_case_nu(1.0 / Re)
-        s64.run_batch(args.warmup + args.steps)
+        s64.run_batch(warmup + steps)
 
         def rel(a, b):
             return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
         err = {f: rel(sim.get(f), s64.get(f)) for f in ("U", "p", "mdotf")}
     kit = out["krylov_iters"]
     line = {
-        "metric": METRIC, "value": round(n * args.steps / wall, 1), "unit": "cell-iters/s",
-        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": round(1e3 * wall / args.steps, 3), "higher_is_better": True,
+        "metric": METRIC, "value": round(n * steps / wall, 1), "unit": "cell-iters/s",
+        "n_gpus": 1, "steps": steps, "warmup": warmup,
+        "ms_per_step": round(1e3 * wall / steps, 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f64 (pressure Krylov f32 in f64 refinement)",
+        "dtype": "f64" if precision == "fp64" else "f64 (pressure Krylov f32 in f64 refinement)",
         "data": "synthetic (seeded cavity sizes and Reynolds numbers)",
         "config": {"workload": f"{args.cases} batched 2D lid-driven cavities, m x m quads with m "
                                f"log-uniform in [32,256], Re uniform in [50,400], SIMPLE "
-                               f"{args.precision.upper()} pressure",
+                               f"{precision.upper()} pressure",
                    "cases": args.cases, "cells": n, "nnz": dm.nnz,
                    "p_solver": "CG+DILU rtol 0.1", "U_solver": "BiCGSTAB+DILU rtol 0.1",
                    "parallelism": "single GPU, one batch mesh"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": hbm_traffic(args.roofline_kernel,
-                                            f"batch{args.cases}_{args.precision}"),
+                                            f"batch{args.cases}_{precision}"),
                      "kernel": args.roofline_kernel, "bytes_per_launch": nbytes,
                      "launches": launches, "avg_us": round(avg_s * 1e6, 2)},
         "cpu_baseline": None,
         "detail": {"mesh_build_s": round(t_mesh, 2), "upload_s": round(t_upload, 2),
-                   "case_iters_per_s": round(args.cases * args.steps / wall, 1),
+                   "case_iters_per_s": round(args.cases * steps / wall, 1),
                    "device_ms": {k: round(v, 2) for k, v in sim.stage_times().items()},
                    "p_iters_max_mean": float(np.mean(kit[:, 3].max(axis=0))),
                    "error_vs_fp64": err},
     }
-    print(json.dumps(line))
+    return line
 
 
-def channel(api, mesh, iterations, precision="fp64", inner_rtol=1e-4):
+def channel(api, mesh, iterations, precision="fp64", inner_rtol=1e-6):
     model = api.Physics(domain=mesh, fluid=api.Fluid(nu=1e-3))
     api.initialise(model.momentum.U, [1.0, 0.0, 0.0])
     api.initialise(model.momentum.p, 0.0)
@@ -368,9 +382,11 @@ def main():
                     choices=["cg_spmv", "dilu_fwd", "dilu_bwd"])
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32-ir"],
                     help="pressure solve: FP64, or FP32 inner Krylov in FP64 refinement")
-    ap.add_argument("--inner-rtol", type=float, default=1e-4,
+    ap.add_argument("--inner-rtol", type=float, default=1e-6,
                     help="fp32-ir: relative tolerance of each FP32 inner solve")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--quick", action="store_true",
+                    help="channel mode: skip the mixed-precision and batched summaries")
     ap.add_argument("--mode", default="channel", choices=["channel", "batch", "serve", "calib"],
                     help="channel: the 1M-cell FP64 channel; batch: batched cavities; "
                          "serve: Poisson-arrival serving of step cases")
@@ -465,6 +481,42 @@ def main():
             return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
         err = {f: rel(sim.get(f), sim64.get(f)) for f in ("U", "p", "mdotf")}
 
+    mixed = batched = None
+    if args.precision == "fp64" and not args.quick:
+        # the same channel with FP32 inner solves in FP64 refinement, and the
+        # batched form: the other two parts of the headline metric
+        mm, mc = channel(api, mesh, args.warmup + args.steps, "fp32-ir", args.inner_rtol)
+        simm = H.Simple(dm, api.simple_setup(mm, mc))
+        simm.set_U(mm.momentum.U)
+        simm.set_p(mm.momentum.p)
+        if args.warmup:
+            simm.run(args.warmup)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        om = simm.run(args.steps)
+        ctx.synchronize()
+        wm = time.perf_counter() - t0
+
+        def rel(a, b):
+            return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+        mixed = {"value": round(n * om["done"] / wm, 1), "unit": "cell-iters/s",
+                 "ms_per_step": round(1e3 * wm / max(om["done"], 1), 3),
+                 "p_solver": f"FP32 CG+DILU (inner rtol {args.inner_rtol:g}) in FP64 refinement "
+                             f"to 1e-10",
+                 "p_inner_iters_mean": float(np.mean(om["krylov_iters"][3])),
+                 "error_vs_fp64": {f: rel(simm.get(f), sim.get(f)) for f in ("U", "p", "mdotf")}}
+        log(f"mixed precision: {mixed['value']:.4g} cell-iters/s")
+        del simm
+        batched = {}
+        for prec in ("fp64", "fp32-ir"):
+            bl = batch_line(args, x, prec, steps=10, warmup=2)
+            batched[prec] = {"value": bl["value"], "unit": bl["unit"],
+                             "case_iters_per_s": bl["detail"]["case_iters_per_s"],
+                             "ms_per_step": bl["ms_per_step"], "cells": bl["config"]["cells"],
+                             "roofline_frac": bl["roofline"]["frac"]}
+            log(f"batch {prec}: {bl['value']:.4g} cell-iters/s")
+        batched["workload"] = bl["config"]["workload"].split(", SIMPLE")[0]
+
     cpu = None
     if not args.no_cpu and args.precision == "fp64":
         O = ge.load_oracle()
@@ -501,7 +553,7 @@ def main():
                    "device_ms": {k: round(v, 2) for k, v in stages.items()},
                    "p_cg_iters_mean": float(np.mean(kit[3])),
                    "U_bicgstab_iters_mean": float(np.mean(kit[:3])),
-                   "error_vs_fp64": err},
+                   "error_vs_fp64": err, "mixed_precision": mixed, "batched": batched},
     }
     print(json.dumps(line))
 

@@ -139,7 +139,7 @@ typedef struct {
   double relax;
   double atol;
   double rtol;
-  double ir_inner_rtol; /* FP32_IR: relative tolerance of each inner solve */
+  double ir_inner_rtol; /* FP32_IR: relative tolerance of each inner solve (0: 1e-6) */
 } xcal_solver_setup;
 
 typedef struct {

@@ -80,7 +80,7 @@ class SolverSetup:
     atol: Optional[float] = None
     rtol: float = 0.1
     precision: str = "fp64"      # "fp64" or "fp32-ir" (FP32 inner Krylov, FP64 refinement)
-    inner_rtol: float = 1e-4     # fp32-ir: relative tolerance of each inner solve
+    inner_rtol: float = 1e-6     # fp32-ir: relative tolerance of each inner solve
 
     def __post_init__(self):
         if self.atol is None:

@@ -774,7 +774,7 @@ static int ir_solve(xcal_eqn* e, const xcal_solver_setup* st, int itmax, double*
   Solver<float> S32 = make_solver32(e);
   float* r32 = e->work32.p + 11 * S.n;  // not used by the FP32 Krylov workspace (10 n)
   // an inner solve need not be tighter than the outer tolerance
-  const double inner_rtol = std::max(st->ir_inner_rtol > 0 ? st->ir_inner_rtol : 1e-4, st->rtol);
+  const double inner_rtol = std::max(st->ir_inner_rtol > 0 ? st->ir_inner_rtol : 1e-6, st->rtol);
   double rr, bb;
   ir_residual(S, x, b, r32, rr, bb);
   const double tol = st->atol + st->rtol * std::sqrt(rr);

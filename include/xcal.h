@@ -117,7 +117,11 @@ enum {
    * zero-gradient (zerogradient_interpolation.jl:1). */
   XCAL_BC_KWALL = 6,
   XCAL_BC_OMEGAWALL = 7,
-  XCAL_BC_NUTWALL = 8
+  XCAL_BC_NUTWALL = 8,
+  /* Empty (empty.jl:46-106): no coefficient from any operator, face values
+   * zero (empty_interpolation.jl:1-16).  Outlet (outlet.jl) acts as
+   * XCAL_BC_ZEROGRADIENT in pressure-based solvers and is passed as such. */
+  XCAL_BC_EMPTY = 9
 };
 typedef struct {
   int32_t kind;
@@ -264,6 +268,14 @@ int xcal_green_gauss(xcal_field* grad, const xcal_field* face);
  * (Calculate_0_gradient.jl:105-117) */
 int xcal_grad(xcal_field* grad, xcal_field* face, const xcal_field* cell, const xcal_bc* bcs,
               int32_t nbcs);
+/* grad! with Grad{Midpoint} (Calculate_0_gradient.jl:121-250):
+ * interpolate_midpoint! (face = (P + N)/2), correct_boundaries!,
+ * green_gauss!, then twice correct_interpolation! (internal faces:
+ * face = (P + N)/2 + (grad_P + grad_N)/2 . (r_f - (r_P + r_N)/2)) and
+ * green_gauss!.  Scalar cell field -> vector gradient, or vector cell field
+ * -> tensor gradient; `face` holds the final face values. */
+int xcal_grad_midpoint(xcal_field* grad, xcal_field* face, const xcal_field* cell,
+                       const xcal_bc* bcs, int32_t nbcs);
 /* div! for a FaceScalarField (Calculate_0_divergence.jl:99-171) */
 int xcal_div(xcal_field* cell, const xcal_field* face);
 /* flux! (Solvers_0_functions.jl:41-64): mdot_f = psi_f . S_f */
@@ -290,7 +302,7 @@ int xcal_correct_velocity(xcal_field* U, const xcal_field* Hv, const xcal_field*
 
 /* ------------------------------------------------------ SIMPLE driver */
 /* The steady incompressible SIMPLE loop of Solvers_1_SIMPLE.jl:103-263 with
- * laminar viscosity, Gauss gradients and no non-orthogonal correctors,
+ * Gauss or Midpoint gradients and no non-orthogonal correctors,
  * entirely on the device (run!/simple!, Solvers_3_solver_dispatch.jl:120-135). */
 typedef struct {
   double nu;
@@ -302,7 +314,10 @@ typedef struct {
   const xcal_bc* bc_U;  /* n_patches entries */
   const xcal_bc* bc_p;  /* n_patches entries */
   int32_t n_bcs;
+  int32_t grad_p;       /* XCAL_GRAD_* of grad!(∇p) (schemes.p.gradient)   */
+  int32_t grad_U;       /* XCAL_GRAD_* of grad!(∇U) (schemes.U.gradient)   */
 } xcal_simple_setup;
+enum { XCAL_GRAD_GAUSS = 0, XCAL_GRAD_MIDPOINT = 1 };
 
 enum { XCAL_SIMPLE_U = 0, XCAL_SIMPLE_P = 1, XCAL_SIMPLE_MDOTF = 2, XCAL_SIMPLE_K = 3,
        XCAL_SIMPLE_OMEGA = 4, XCAL_SIMPLE_NUT = 5 };

@@ -217,6 +217,8 @@ void bc_coeffs(const orc_mesh& m, const xcal_bc& bc, const orc_term& t, int64_t 
         bp_out = ((-value) * ap) * delta;
       }
       return;
+    case XCAL_BC_EMPTY:  // empty.jl:46-106: (0, 0) for every operator
+      return;
     case XCAL_BC_ZEROGRADIENT:  // zerogradient.jl:25-76
     case XCAL_BC_KWALL:         // KWallFunction, Discretise_3_boundary_conditions.jl:11-95
     case XCAL_BC_NUTWALL:
@@ -806,6 +808,13 @@ int orc_correct_boundaries(const orc_mesh* mp, const xcal_bc* bcs, int32_t nbc,
               fz[fID] = phiz[c];
             }
             break;
+          case XCAL_BC_EMPTY:  // empty_interpolation.jl:1-16
+            fx[fID] = 0.0;
+            if (vec) {
+              fy[fID] = 0.0;
+              fz[fID] = 0.0;
+            }
+            break;
           case XCAL_BC_SYMMETRY:  // symmetry_interpolation.jl
             if (vec) {
               V3 n = m.fn[fID];
@@ -888,6 +897,45 @@ void orc_green_gauss_tensor(const orc_mesh* mp, const double* fx, const double* 
     for (int p = 0; p < 3; ++p)
       for (int q = 0; q < 3; ++q) T[3 * p + q][c] += (P[p] * S[q]) / V;
   }
+}
+
+int orc_grad_midpoint(const orc_mesh* mp, const xcal_bc* bcs, int32_t nbc, int32_t ncomp,
+                      const double* const* phi, double* const* phif, double* const* grad) {
+  return guard([&] {
+    const orc_mesh& m = *mp;
+    if (ncomp != 1 && ncomp != 3) throw std::runtime_error("grad!: scalar or vector field");
+    for (int c = 0; c < ncomp; ++c)  // interpolate_midpoint! (:121-193)
+      for (int64_t i = 0; i < m.nf; ++i) {
+        const double p1 = phi[c][m.own1[i]];
+        phif[c][i] = 0.5 * (p1 + phi[c][m.own2[i]]);
+      }
+    int rc = ncomp == 1 ? orc_correct_boundaries(mp, bcs, nbc, phi[0], nullptr, nullptr, phif[0],
+                                                 nullptr, nullptr)
+                        : orc_correct_boundaries(mp, bcs, nbc, phi[0], phi[1], phi[2], phif[0],
+                                                 phif[1], phif[2]);
+    if (rc < 0) throw std::runtime_error(g_err);
+    auto gauss = [&] {
+      if (ncomp == 1) orc_green_gauss(mp, phif[0], grad[0], grad[1], grad[2]);
+      else orc_green_gauss_tensor(mp, phif[0], phif[1], phif[2], grad);
+    };
+    gauss();
+    for (int it = 0; it < 2; ++it) {  // correct_interpolation! (:195-249) + green_gauss!
+      for (int64_t i = m.nbf; i < m.nf; ++i) {
+        const int64_t c1 = m.own1[i], c2 = m.own2[i];
+        const V3 rf = m.fc[i], rP = m.cc[c1], rN = m.cc[c2];
+        const double R[3] = {rf.x - 0.5 * (rP.x + rN.x), rf.y - 0.5 * (rP.y + rN.y),
+                             rf.z - 0.5 * (rP.z + rN.z)};
+        for (int p = 0; p < ncomp; ++p) {
+          const double phip = 0.5 * (phi[p][c1] + phi[p][c2]);
+          double g[3];
+          for (int q = 0; q < 3; ++q) g[q] = 0.5 * (grad[3 * p + q][c1] + grad[3 * p + q][c2]);
+          phif[p][i] = phip + ((g[0] * R[0] + g[1] * R[1]) + g[2] * R[2]);
+        }
+      }
+      gauss();
+    }
+    return 0;
+  });
 }
 
 void orc_div(const orc_mesh* mp, const double* psif, double* div) {
@@ -1140,7 +1188,14 @@ int orc_simple_run_turb(const orc_mesh* mp, const xcal_simple_setup* S,
     auto chk = [](int rc) {
       if (rc < 0) throw std::runtime_error(g_err);
     };
-    auto grad_p = [&] {  // grad! (Calculate_0_gradient.jl:105-109)
+    auto grad_p = [&] {  // grad! (Calculate_0_gradient.jl:105-109, Midpoint :235-244)
+      if (S->grad_p == XCAL_GRAD_MIDPOINT) {
+        const double* ph[1] = {p};
+        double* fc[1] = {pf.data()};
+        double* g[3] = {gpx.data(), gpy.data(), gpz.data()};
+        chk(orc_grad_midpoint(&m, S->bc_p, nb, 1, ph, fc, g));
+        return;
+      }
       orc_interpolate(&m, p, pf.data());
       chk(orc_correct_boundaries(&m, S->bc_p, nb, p, nullptr, nullptr, pf.data(), nullptr, nullptr));
       orc_green_gauss(&m, pf.data(), gpx.data(), gpy.data(), gpz.data());
@@ -1217,8 +1272,14 @@ int orc_simple_run_turb(const orc_mesh* mp, const xcal_simple_setup* S,
         Uz[i] = Hz[i] - gpz[i] * r;
       }
       // turbulence!: grad!(gradU) (RANS_laminar.jl:73-79, RANS_kOmega.jl:192)
-      interp_vec(Ux, Uy, Uz);
-      orc_green_gauss_tensor(&m, Ufx.data(), Ufy.data(), Ufz.data(), gU);
+      if (S->grad_U == XCAL_GRAD_MIDPOINT) {
+        const double* ph[3] = {Ux, Uy, Uz};
+        double* fc[3] = {Ufx.data(), Ufy.data(), Ufz.data()};
+        chk(orc_grad_midpoint(&m, S->bc_U, nb, 3, ph, fc, gU));
+      } else {
+        interp_vec(Ux, Uy, Uz);
+        orc_green_gauss_tensor(&m, Ufx.data(), Ufy.data(), Ufz.data(), gU);
+      }
       bool turb_conv = true;
       if (turb)
         turb_conv = komega_step(m, *S, *T, Ux, Uy, Uz, mdotf.data(), gU, kf_, wf_, nut_,

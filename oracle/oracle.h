@@ -99,6 +99,11 @@ void orc_green_gauss(const orc_mesh* m, const double* phif, double* gx, double* 
  * t: 9 arrays xx,xy,xz,yx,yy,yz,zx,zy,zz */
 void orc_green_gauss_tensor(const orc_mesh* m, const double* fx, const double* fy,
                             const double* fz, double* const* t);
+/* grad! with Grad{Midpoint} (Calculate_0_gradient.jl:121-250): phi holds 1
+ * (scalar) or 3 (vector) cell arrays, phif as many face arrays (the final
+ * face values), grad 3 or 9 arrays (tensor row p = component p). */
+int orc_grad_midpoint(const orc_mesh* m, const xcal_bc* bcs, int32_t nbc, int32_t ncomp,
+                      const double* const* phi, double* const* phif, double* const* grad);
 /* div! (Calculate_0_divergence.jl:99-171) */
 void orc_div(const orc_mesh* m, const double* psif, double* div);
 /* flux! (Solvers_0_functions.jl:41-64) */

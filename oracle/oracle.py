@@ -90,6 +90,8 @@ def lib():
         L.orc_green_gauss.restype = None
         L.orc_green_gauss_tensor.argtypes = [C.c_void_p, _pd, _pd, _pd, C.POINTER(_pd)]
         L.orc_green_gauss_tensor.restype = None
+        L.orc_grad_midpoint.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32,
+                                        C.POINTER(_pd), C.POINTER(_pd), C.POINTER(_pd)]
         L.orc_div.argtypes = [C.c_void_p, _pd, _pd]
         L.orc_div.restype = None
         L.orc_flux.argtypes = [C.c_void_p, _pd, _pd, _pd, _pd]
@@ -255,6 +257,18 @@ class OracleMesh:
         ptrs = (_pd * 9)(*[_p(t) for t in T])
         lib().orc_green_gauss_tensor(self.h, _p(fx), _p(fy), _p(fz), ptrs)
         return T
+
+    def grad_midpoint(self, bcs, phi):
+        """grad! with Grad{Midpoint}: phi a cell array (scalar) or three
+        (vector); returns (face values, gradient components)."""
+        ph = [np.ascontiguousarray(phi, dtype=np.float64)] if np.ndim(phi[0]) == 0 else \
+            [np.ascontiguousarray(c, dtype=np.float64) for c in phi]
+        nc = len(ph)
+        f = [np.zeros(self.mesh.n_faces) for _ in range(nc)]
+        g = [np.zeros(self.n) for _ in range(3 * nc)]
+        arr = lambda xs: (_pd * len(xs))(*[_p(v) for v in xs])
+        _chk(lib().orc_grad_midpoint(self.h, bcs, len(bcs), nc, arr(ph), arr(f), arr(g)))
+        return f, g
 
     def div(self, psif):
         d = np.zeros(self.n)

@@ -411,3 +411,100 @@ def test_channel_simple_fp32_pressure_matches_fp64_oracle(x, A, O, meshes):
     print(f"box12 fp32-ir: rel L2 U {eU:.3e} p {ep:.3e} mdot {em:.3e}")
     assert out["done"] == ref["done"]
     assert eU < 1e-8 and ep < 1e-8 and em < 1e-8
+
+
+@pytest.mark.parametrize("name", ["step", "box12", "cavity_hex"])
+def test_empty_bc_bit_exact(x, A, O, ctx, meshes, name):
+    """Empty (empty.jl, empty_interpolation.jl) on the last patch of a scalar
+    and a vector equation: assembly with BCs and correct_boundaries!
+    bit-exact against the oracle."""
+    mesh = meshes(name)
+    rng = _rng(7)
+    F = _fields(mesh, rng)
+    H = x.hip
+    dm = H.DeviceMesh(ctx, mesh)
+    om = O.OracleMesh(mesh, A)
+    last = len(mesh.patch_names) - 1
+    sbl = [(A.BC_EMPTY, i, 0.0) if i == last else b for i, b in enumerate(_scalar_bcs(A, mesh))]
+    vbl = [(A.BC_EMPTY, i, [0.0] * 3) if i == last else b
+           for i, b in enumerate(_vector_bcs(A, mesh))]
+    sb, vb = A.bc_array(sbl), A.bc_array(vbl)
+    g = H.Field(dm, A.FACE_SCALAR).upload(F["gamma"])
+    md = H.Field(dm, A.FACE_SCALAR).upload(F["mdot"])
+    phi = H.Field(dm, A.CELL_SCALAR).upload(F["phi"])
+    dterms = [(A.DIV_UPWIND, 1, md), (A.LAPLACIAN_LINEAR, -1, g)]
+    oterms = [(A.DIV_UPWIND, 1, F["mdot"]), (A.LAPLACIAN_LINEAR, -1, F["gamma"])]
+    eq = H.Equation(dm, False)
+    eq.discretise(dterms, [], phi)
+    onz, ob = om.discretise_scalar(oterms, [], F["phi"])
+    eq.apply_bcs(dterms, sb, phi)
+    om.apply_bcs(oterms, sb, -1, F["phi"], onz, ob)
+    nz, b = eq.download()
+    assert np.array_equal(nz, onz) and np.array_equal(b, ob)
+    U = H.Field(dm, A.CELL_VECTOR).upload(F["U"])
+    Uh = [np.ascontiguousarray(F["U"][c]) for c in range(3)]
+    veq = H.Equation(dm, True)
+    veq.discretise(dterms, [], U)
+    onz0, obv = om.discretise_vector(oterms, [], F["U"])
+    for c in range(3):
+        veq.update_equation()
+        veq.apply_bcs(dterms, vb, U, c)
+        onz = onz0.copy()
+        obc = obv[c].copy()
+        om.apply_bcs(oterms, vb, c, Uh, onz, obc)
+        nz, b = veq.download(0, c)
+        assert np.array_equal(nz, onz) and np.array_equal(b, obc), f"component {c}"
+    pf = H.Field(dm, A.FACE_SCALAR)
+    grad = H.Field(dm, A.CELL_VECTOR)
+    H.check(H.lib().xcal_grad(grad.h, pf.h, phi.h, sb, len(sb)))
+    opf = om.interpolate(F["phi"])
+    om.correct_boundaries(sb, F["phi"], opf)
+    assert np.array_equal(pf.download(), opf)
+    assert np.array_equal(grad.download(), np.stack(om.green_gauss(opf)))
+
+
+@pytest.mark.parametrize("name", ["cavity40", "box12", "step", "cavity_hex", "trig40"])
+def test_midpoint_gradient_bit_exact(x, A, O, ctx, meshes, name):
+    """grad! with Grad{Midpoint}: scalar -> vector and vector -> tensor, the
+    final face values and the gradient bit-exact against the oracle."""
+    mesh = meshes(name)
+    rng = _rng(9)
+    H = x.hip
+    dm = H.DeviceMesh(ctx, mesh)
+    om = O.OracleMesh(mesh, A)
+    n = mesh.n_cells
+    phi_h = rng.standard_normal(n)
+    U_h = rng.standard_normal((3, n))
+    sb = A.bc_array(_scalar_bcs(A, mesh))
+    vb = A.bc_array(_vector_bcs(A, mesh))
+    phi = H.Field(dm, A.CELL_SCALAR).upload(phi_h)
+    pf = H.Field(dm, A.FACE_SCALAR)
+    grad = H.Field(dm, A.CELL_VECTOR)
+    H.check(H.lib().xcal_grad_midpoint(grad.h, pf.h, phi.h, sb, len(sb)))
+    of, og = om.grad_midpoint(sb, phi_h)
+    assert np.array_equal(pf.download(), of[0])
+    assert np.array_equal(grad.download(), np.stack(og))
+    U = H.Field(dm, A.CELL_VECTOR).upload(U_h)
+    Uf = H.Field(dm, A.FACE_VECTOR)
+    gU = H.Field(dm, A.CELL_TENSOR)
+    H.check(H.lib().xcal_grad_midpoint(gU.h, Uf.h, U.h, vb, len(vb)))
+    of, og = om.grad_midpoint(vb, [U_h[c] for c in range(3)])
+    assert np.array_equal(Uf.download(), np.stack(of))
+    assert np.array_equal(gU.download(), np.stack(og))
+
+
+def test_simple_midpoint_gradients_match_oracle(x, A, O, meshes):
+    """SIMPLE with Grad{Midpoint} for p and U on the jittered box channel:
+    fields after 10 iterations within 1e-8 relative L2 of the oracle."""
+    api = x.api
+    mesh = meshes("box12")
+    model, config = channel_case(api, mesh, iterations=10)
+    for f in ("U", "p"):
+        config.schemes[f].gradient = api.Midpoint
+    setup = api.simple_setup(model, config)
+    assert setup.grad_p == A.GRAD_MIDPOINT and setup.grad_U == A.GRAD_MIDPOINT
+    ref = O.OracleMesh(mesh, A).simple_run(setup, model.momentum.U, model.momentum.p, 10)
+    api.run(model, config)
+    eU, ep = rel(model.momentum.U, ref["U"]), rel(model.momentum.p, ref["p"])
+    print(f"midpoint SIMPLE box12: U {eU:.2e} p {ep:.2e}")
+    assert eU < 1e-8 and ep < 1e-8

@@ -251,3 +251,66 @@ def test_channel_simple_mass_conservation(x, A, O, meshes):
         imbalance.append(abs(outflow + inflow))
         assert np.abs(out["U"]).max() < 3.0
     assert imbalance[1] < 2e-3 and imbalance[1] < imbalance[0] / 3
+
+
+@pytest.mark.parametrize("name", ["step", "box12"])
+def test_empty_bc_closed_form(A, O, meshes, name):
+    """Empty gives no coefficient from any operator (empty.jl:46-106): its
+    assembly equals Zerogradient's once the flux through its faces is zero
+    (Zerogradient keeps the convective a_p = F).  Its face values are zero
+    (empty_interpolation.jl)."""
+    m = meshes(name)
+    om = O.OracleMesh(m, A)
+    rng = np.random.default_rng(11)
+    nf = m.n_faces
+    gamma = 0.5 + rng.random(nf)
+    mdot = rng.standard_normal(nf) * m.face_area
+    phi = rng.standard_normal(m.n_cells)
+    last = len(m.patch_names) - 1
+    pr = np.asarray(m.arrays["patch_range"]).reshape(-1, 2) - 1 if "patch_range" in m.arrays else None
+    lo, hi = int(pr[last][0]), int(pr[last][1])
+    base = [(A.BC_DIRICHLET, i, 1.0 + i) for i in range(last)]
+    empty = A.bc_array(base + [(A.BC_EMPTY, last, 0.0)])
+    zg = A.bc_array(base + [(A.BC_ZEROGRADIENT, last, 0.0)])
+    md0 = mdot.copy()
+    md0[lo:hi + 1] = 0.0
+    for bcs, md in ((empty, mdot), (zg, md0)):
+        terms = [(A.DIV_UPWIND, 1, md), (A.LAPLACIAN_LINEAR, -1, gamma)]
+        nz, b = om.discretise_scalar(terms, [], phi)
+        om.apply_bcs(terms, bcs, -1, phi, nz, b)
+        if bcs is empty:
+            nz_e, b_e = nz, b
+        else:
+            # the interior faces of the two runs see different fluxes only on
+            # boundary faces, which interior coefficients never read
+            assert np.array_equal(nz_e, nz) and np.array_equal(b_e, b)
+    pf = om.interpolate(phi)
+    om.correct_boundaries(empty, phi, pf)
+    assert np.all(pf[lo:hi + 1] == 0.0)
+
+
+def test_midpoint_gradient_exact_for_linear_fields(A, O, meshes):
+    """Grad{Midpoint} (Calculate_0_gradient.jl:235-244): on a uniform mesh the
+    face midpoint of the two cell centres is the face centre, so a linear
+    field's face values, and hence its Green-Gauss gradient in cells whose
+    faces are all internal, are exact; the corrected face values equal the
+    field at the face centres."""
+    m = meshes("cavity16")
+    om = O.OracleMesh(m, A)
+    a = np.array([0.7, -1.3, 0.0])
+    cc = m.cell_centre.reshape(-1, 3)
+    phi = cc @ a
+    bcs = A.bc_array([(A.BC_EXTRAPOLATED, i, 0.0) for i in range(len(m.patch_names))])
+    f, g = om.grad_midpoint(bcs, phi)
+    fc = m.face_centre.reshape(-1, 3)
+    nbf = m.n_bfaces
+    assert np.allclose(f[0][nbf:], (fc @ a)[nbf:], rtol=0, atol=1e-13)
+    interior = np.setdiff1d(np.arange(m.n_cells), m.boundary_cellsID - 1)
+    assert np.allclose(g[0][interior], a[0], atol=1e-12)
+    assert np.allclose(g[1][interior], a[1], atol=1e-12)
+    # vector field: each component's gradient row
+    U = [cc @ a, cc @ (2 * a), cc @ (-a)]
+    fv, gv = om.grad_midpoint(bcs, U)
+    for p, s in enumerate((1.0, 2.0, -1.0)):
+        assert np.allclose(gv[3 * p][interior], s * a[0], atol=1e-12)
+        assert np.allclose(gv[3 * p + 1][interior], s * a[1], atol=1e-12)

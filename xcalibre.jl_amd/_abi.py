@@ -10,7 +10,7 @@ TIME_STEADY, TIME_EULER, LAPLACIAN_LINEAR, DIV_LINEAR, DIV_UPWIND, DIV_LUST, \
     DIV_BOUNDED_UPWIND, SI = range(8)
 # boundary condition kinds
 BC_DIRICHLET, BC_NEUMANN, BC_ZEROGRADIENT, BC_EXTRAPOLATED, BC_WALL, BC_SYMMETRY = range(6)
-BC_KWALL, BC_OMEGAWALL, BC_NUTWALL = 6, 7, 8
+BC_KWALL, BC_OMEGAWALL, BC_NUTWALL, BC_EMPTY = 6, 7, 8, 9
 # solvers / preconditioners / precision
 SOLVER_CG, SOLVER_BICGSTAB = 0, 1
 PRECON_NONE, PRECON_JACOBI, PRECON_DILU = 0, 1, 2
@@ -73,7 +73,11 @@ class SolverSetupC(C.Structure):
 class SimpleSetupC(C.Structure):
     _fields_ = [("nu", C.c_double), ("div_scheme", C.c_int32), ("has_pref", C.c_int32),
                 ("pref", C.c_double), ("U", SolverSetupC), ("p", SolverSetupC),
-                ("bc_U", C.POINTER(BC)), ("bc_p", C.POINTER(BC)), ("n_bcs", C.c_int32)]
+                ("bc_U", C.POINTER(BC)), ("bc_p", C.POINTER(BC)), ("n_bcs", C.c_int32),
+                ("grad_p", C.c_int32), ("grad_U", C.c_int32)]
+
+
+GRAD_GAUSS, GRAD_MIDPOINT = 0, 1
 
 
 class ServerConfigC(C.Structure):

@@ -39,6 +39,11 @@ from . import hip
 # ------------------------------------------------------------- schemes
 class SteadyState: ...
 class Euler: ...
+class CrankNicolson: ...  # assembles as Euler (Discretise_1_schemes.jl:44-58)
+
+
+# operator type of each time scheme (xcal_term.type)
+TIME = {SteadyState: A.TIME_STEADY, Euler: A.TIME_EULER, CrankNicolson: A.TIME_EULER}
 class Linear: ...
 class Upwind: ...
 class LUST: ...
@@ -49,6 +54,7 @@ class Midpoint: ...
 
 _DIV = {Linear: A.DIV_LINEAR, Upwind: A.DIV_UPWIND, LUST: A.DIV_LUST,
         BoundedUpwind: A.DIV_BOUNDED_UPWIND}
+_GRAD = {Gauss: A.GRAD_GAUSS, Midpoint: A.GRAD_MIDPOINT}
 
 
 @dataclass
@@ -222,6 +228,9 @@ def Zerogradient(name): return _BC(name, 0.0, A.BC_ZEROGRADIENT)
 def Extrapolated(name): return _BC(name, 0.0, A.BC_EXTRAPOLATED)
 def Wall(name, value=0.0): return _BC(name, value, A.BC_WALL)
 def Symmetry(name): return _BC(name, 0.0, A.BC_SYMMETRY)
+def Empty(name): return _BC(name, 0.0, A.BC_EMPTY)
+# Outlet (outlet.jl): zero gradient in the pressure-based solvers
+def Outlet(name): return _BC(name, 0.0, A.BC_ZEROGRADIENT)
 # wall functions with the reference's default coefficients (Discretise_0_types.jl:63,108,154)
 def KWallFunction(name): return _BC(name, 0.0, A.BC_KWALL)
 def OmegaWallFunction(name): return _BC(name, 0.0, A.BC_OMEGAWALL)
@@ -259,8 +268,9 @@ def simple_setup(model: Physics, config: Configuration, pref=None) -> A.SimpleSe
     for s in (sU, sp):
         if s.time is not SteadyState:
             raise NotImplementedError("SIMPLE is a steady solver (Time{SteadyState})")
-        if s.gradient is not Gauss or s.limiter is not None or s.laplacian is not Linear:
-            raise NotImplementedError("device path: Gauss gradient, Linear laplacian, no limiter")
+        if s.gradient not in _GRAD or s.limiter is not None or s.laplacian is not Linear:
+            raise NotImplementedError("device path: Gauss or Midpoint gradient, Linear "
+                                      "laplacian, no limiter")
     div = _DIV.get(sU.divergence)
     if div is None:
         raise NotImplementedError(f"divergence scheme {sU.divergence!r}")
@@ -268,7 +278,8 @@ def simple_setup(model: Physics, config: Configuration, pref=None) -> A.SimpleSe
     bp = A.bc_array(config.boundaries["p"])
     setup = A.SimpleSetupC(float(model.fluid.nu), div, int(pref is not None),
                            float(pref or 0.0), config.solvers["U"].c(), config.solvers["p"].c(),
-                           bU, bp, len(config.boundaries["U"]))
+                           bU, bp, len(config.boundaries["U"]), _GRAD[sp.gradient],
+                           _GRAD[sU.gradient])
     setup._keep = (bU, bp)
     return setup
 

@@ -22,6 +22,46 @@ __global__ void __launch_bounds__(kBlock)
   phif[f] = wf * phi[own[f]] + omw * phi[nei[f]];
 }
 
+// interpolate_midpoint! (Calculate_0_gradient.jl:121-193): every face, the
+// two cells' mean (a boundary face's two cells are the same cell)
+__global__ void __launch_bounds__(kBlock)
+    k_interp_midpoint(int64_t nf, const int32_t* __restrict__ own, const int32_t* __restrict__ nei,
+                      const double* __restrict__ phi, double* __restrict__ phif) {
+  const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nf) return;
+  const double p1 = phi[own[f]];
+  phif[f] = 0.5 * (p1 + phi[nei[f]]);
+}
+
+// correct_interpolation! (Calculate_0_gradient.jl:195-249) over internal
+// faces: face = (P + N)/2 + g . Ri with g = (grad_P + grad_N)/2 and
+// Ri = r_f - (r_P + r_N)/2, for each of the field's ncomp components; grad
+// holds 3 ncomp components of n values (row p of the tensor: component 3p+q)
+__global__ void __launch_bounds__(kBlock)
+    k_correct_interp(int64_t nbf, int64_t nf, int64_t n, int ncomp, const int32_t* __restrict__ own,
+                     const int32_t* __restrict__ nei, const double* __restrict__ rix,
+                     const double* __restrict__ riy, const double* __restrict__ riz,
+                     const double* __restrict__ px, const double* __restrict__ py,
+                     const double* __restrict__ pz, const double* __restrict__ grad,
+                     double* __restrict__ fx, double* __restrict__ fy, double* __restrict__ fz) {
+  const int64_t f = nbf + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nf) return;
+  const int64_t c1 = own[f], c2 = nei[f];
+  const double R[3] = {rix[f], riy[f], riz[f]};
+  const double* P[3] = {px, py, pz};
+  double* F[3] = {fx, fy, fz};
+  for (int p = 0; p < ncomp; ++p) {
+    const double phip = 0.5 * (P[p][c1] + P[p][c2]);
+    double g[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const double* gq = grad + (int64_t)(3 * p + q) * n;
+      g[q] = 0.5 * (gq[c1] + gq[c2]);
+    }
+    F[p][f] = phip + ((g[0] * R[0] + g[1] * R[1]) + g[2] * R[2]);
+  }
+}
+
 // boundary_interpolation! (1_boundary_interpolation.jl and the
 // *_interpolation.jl files), one work-item per boundary face
 __global__ void __launch_bounds__(kBlock)
@@ -54,6 +94,10 @@ __global__ void __launch_bounds__(kBlock)
     case XCAL_BC_NUTWALL:
       fx[f] = px[c];
       if (vec) { fy[f] = py[c]; fz[f] = pz[c]; }
+      break;
+    case XCAL_BC_EMPTY:  // empty_interpolation.jl:1-16
+      fx[f] = 0.0;
+      if (vec) { fy[f] = 0.0; fz[f] = 0.0; }
       break;
     case XCAL_BC_SYMMETRY:
       if (vec) {
@@ -349,6 +393,32 @@ void green_gauss_tensor(const xcal_mesh* m, const double* fx, const double* fy, 
   LAUNCH(k_green_gauss_tensor_bnd, m->nbcell, mk, fx, fy, fz, t0);
 }
 
+// grad! with Grad{Midpoint} (Calculate_0_gradient.jl:235-244) of a scalar
+// (ncomp 1, grad = 3 components) or vector (ncomp 3, grad = 9 components)
+void grad_midpoint(const xcal_mesh* m, const BCsK& b, const double* const* phi,
+                   double* const* phif, int ncomp, double* grad) {
+  for (int c = 0; c < ncomp; ++c)
+    LAUNCH(k_interp_midpoint, m->nf, m->nf, m->own.p, m->nei.p, phi[c], phif[c]);
+  correct_boundaries(m, b, phi, phif, ncomp);
+  const int64_t n = m->n;
+  auto gauss = [&] {
+    if (ncomp == 1) green_gauss(m, phif[0], grad, grad + n, grad + 2 * n);
+    else green_gauss_tensor(m, phif[0], phif[1], phif[2], grad);
+  };
+  gauss();
+  const double* px = phi[0];
+  const double* py = ncomp == 3 ? phi[1] : px;
+  const double* pz = ncomp == 3 ? phi[2] : px;
+  double* fx = phif[0];
+  double* fy = ncomp == 3 ? phif[1] : fx;
+  double* fz = ncomp == 3 ? phif[2] : fx;
+  for (int it = 0; it < 2; ++it) {
+    LAUNCH(k_correct_interp, m->nf - m->nbf, m->nbf, m->nf, n, ncomp, m->own.p, m->nei.p,
+           m->rix.p, m->riy.p, m->riz.p, px, py, pz, (const double*)grad, fx, fy, fz);
+    gauss();
+  }
+}
+
 void divergence(const xcal_mesh* m, const double* psif, double* div) {
   MeshK mk = meshk(m);
   LAUNCH(k_div, m->n, mk, psif, div);
@@ -455,6 +525,33 @@ int xcal_grad(xcal_field* grad, xcal_field* face, const xcal_field* cell, const 
   rc = xcal_correct_boundaries(face, cell, bcs, nbcs);
   if (rc) return rc;
   return xcal_green_gauss(grad, face);
+}
+
+int xcal_grad_midpoint(xcal_field* grad, xcal_field* face, const xcal_field* cell,
+                       const xcal_bc* bcs, int32_t nbcs) {
+  return guarded([&] {
+    XCAL_CHECK(grad && face && cell && (bcs || nbcs == 0), "NULL argument");
+    xcal_mesh* m = cell->mesh;
+    XCAL_CHECK(grad->mesh == m && face->mesh == m, "fields must share a mesh");
+    const bool scalar = cell->kind == XCAL_CELL_SCALAR;
+    XCAL_CHECK((scalar && face->kind == XCAL_FACE_SCALAR && grad->kind == XCAL_CELL_VECTOR) ||
+                   (cell->kind == XCAL_CELL_VECTOR && face->kind == XCAL_FACE_VECTOR &&
+                    grad->kind == XCAL_CELL_TENSOR),
+               "grad!: ScalarField -> VectorField or VectorField -> TensorField");
+    XCAL_CHECK(nbcs == m->n_patches, "one boundary condition per patch is required");
+    for (int i = 0; i < nbcs && !scalar; ++i)
+      XCAL_CHECK(bcs[i].kind != XCAL_BC_NEUMANN, "Neumann is not defined for vector fields");
+    const BCsK b = make_bcs(m, bcs, nbcs);
+    const int nc = scalar ? 1 : 3;
+    const double* ph[3] = {cell->comp(0), nullptr, nullptr};
+    double* fc[3] = {face->comp(0), nullptr, nullptr};
+    for (int c = 1; c < nc; ++c) {
+      ph[c] = cell->comp(c);
+      fc[c] = face->comp(c);
+    }
+    grad_midpoint(m, b, ph, fc, nc, grad->comp(0));
+    return 0;
+  });
 }
 
 int xcal_div(xcal_field* cell, const xcal_field* face) {

@@ -41,7 +41,7 @@ BCsK make_bcs(const xcal_mesh* m, const xcal_bc* bcs, int n) {
   for (int p = 0; p < kMaxPatches; ++p) b.p[p].kind = -1;
   for (int i = 0; i < n; ++i) {
     XCAL_CHECK(bcs[i].patch >= 0 && bcs[i].patch < m->n_patches, "BC patch index out of range");
-    XCAL_CHECK(bcs[i].kind >= XCAL_BC_DIRICHLET && bcs[i].kind <= XCAL_BC_NUTWALL,
+    XCAL_CHECK(bcs[i].kind >= XCAL_BC_DIRICHLET && bcs[i].kind <= XCAL_BC_EMPTY,
                "unknown boundary condition kind");
     // the reference applies the first BC whose range holds the face
     if (b.p[bcs[i].patch].kind != -1) continue;
@@ -166,7 +166,8 @@ struct HostMesh {
   int dim = 3, n_patches = 0, nlev_f = 0, nlev_b = 0;
   int64_t n = 0, nf = 0, nbf = 0, nnz = 0, ncf = 0, nzlen = 0, nbcell = 0;
   std::vector<int64_t> patch_first, patch_last;
-  std::vector<double> vol, area, weight, delta, nx, ny, nz, ex, ey, ez, sfx, sfy, sfz, lapEf;
+  std::vector<double> vol, area, weight, delta, nx, ny, nz, ex, ey, ez, sfx, sfy, sfz, lapEf, rix,
+      riy, riz;
   std::vector<int32_t> cf_ptr, cf_face, cf_nbr, cf_slot, diag, own, nei, fslot, bc_cells, bc_ptr,
       bc_face, rbeg, rend, colval, trans, rows_b, posb, Lcol, Lslot, Ucol, Uslot, lev_ptr_f,
       lev_ptr_b, lmeta_f, lmeta_b, cperm, fperm, nzperm, h_rowptr, h_colval;
@@ -244,7 +245,7 @@ static void build_host_mesh(const xcal_mesh_host* h, HostMesh& M) {
   std::vector<int32_t> own(nf), nei(nf);
   std::vector<double> area(h->face_area, h->face_area + nf), weight(h->face_weight, h->face_weight + nf),
       delta(h->face_delta, h->face_delta + nf), nx(nf), ny(nf), nz(nf), ex(nf), ey(nf), ez(nf),
-      sfx(nf), sfy(nf), sfz(nf), lapEf(nf);
+      sfx(nf), sfy(nf), sfz(nf), lapEf(nf), rix(nf, 0.0), riy(nf, 0.0), riz(nf, 0.0);
   for (int64_t f = 0; f < nf; ++f) {
     own[f] = (int32_t)(h->face_owner[2 * f] - 1);
     nei[f] = (int32_t)(h->face_owner[2 * f + 1] - 1);
@@ -265,6 +266,16 @@ static void build_host_mesh(const xcal_mesh_host* h, HostMesh& M) {
     double cfac = (s0 * s0 + s1 * s1 + s2 * s2) / (s0 * ex[f] + s1 * ey[f] + s2 * ez[f]);
     double e0 = cfac * ex[f], e1 = cfac * ey[f], e2 = cfac * ez[f];
     lapEf[f] = std::sqrt(e0 * e0 + e1 * e1 + e2 * e2);
+    // correct_interpolation! (Calculate_0_gradient.jl:224-243): Ri = rf -
+    // 0.5 (rP + rN) of an internal face
+    if (f >= nbf) {
+      const double* rf = h->face_centre + 3 * f;
+      const double* rP = h->cell_centre + 3 * (int64_t)own[f];
+      const double* rN = h->cell_centre + 3 * (int64_t)nei[f];
+      rix[f] = rf[0] - 0.5 * (rP[0] + rN[0]);
+      riy[f] = rf[1] - 0.5 * (rP[1] + rN[1]);
+      riz[f] = rf[2] - 0.5 * (rP[2] + rN[2]);
+    }
   }
   std::vector<int32_t> fslot(nf, -1);
   for (int64_t f = nbf; f < nf; ++f) fslot[f] = find(own[f], nei[f]);
@@ -488,7 +499,8 @@ static void build_host_mesh(const xcal_mesh_host* h, HostMesh& M) {
     for (int64_t f = 0; f < nf; ++f) t[fperm[f]] = a[f];
     a.swap(t);
   };
-  for (auto* a : {&area, &weight, &delta, &nx, &ny, &nz, &ex, &ey, &ez, &sfx, &sfy, &sfz, &lapEf})
+  for (auto* a : {&area, &weight, &delta, &nx, &ny, &nz, &ex, &ey, &ez, &sfx, &sfy, &sfz, &lapEf,
+                  &rix, &riy, &riz})
     permf(*a);
   std::vector<int32_t> own_n(nf), nei_n(nf), fslot_n(nf, -1);
   for (int64_t f = 0; f < nf; ++f) {
@@ -533,6 +545,9 @@ static void build_host_mesh(const xcal_mesh_host* h, HostMesh& M) {
   M.sfy.swap(sfy);
   M.sfz.swap(sfz);
   M.lapEf.swap(lapEf);
+  M.rix.swap(rix);
+  M.riy.swap(riy);
+  M.riz.swap(riz);
   M.cf_ptr.swap(cf_ptr);
   M.cf_face.swap(cf_face);
   M.cf_nbr.swap(cf_nbr);
@@ -595,7 +610,7 @@ static void union_cases(std::vector<HostMesh>& C, HostMesh& U, std::vector<int64
   U.n = N; U.nf = NF; U.nbf = NBF; U.nnz = NNZ; U.nzlen = NZ; U.ncf = NCF; U.nbcell = NBC;
   for (auto* v : {&U.vol}) v->resize(N);
   for (auto* v : {&U.area, &U.weight, &U.delta, &U.nx, &U.ny, &U.nz, &U.ex, &U.ey, &U.ez, &U.sfx,
-                  &U.sfy, &U.sfz, &U.lapEf})
+                  &U.sfy, &U.sfz, &U.lapEf, &U.rix, &U.riy, &U.riz})
     v->resize(NF);
   U.own.resize(NF);
   U.nei.resize(NF);
@@ -642,6 +657,7 @@ static void union_cases(std::vector<HostMesh>& C, HostMesh& U, std::vector<int64
       U.nx[g] = M.nx[f]; U.ny[g] = M.ny[f]; U.nz[g] = M.nz[f];
       U.ex[g] = M.ex[f]; U.ey[g] = M.ey[f]; U.ez[g] = M.ez[f];
       U.sfx[g] = M.sfx[f]; U.sfy[g] = M.sfy[f]; U.sfz[g] = M.sfz[f]; U.lapEf[g] = M.lapEf[f];
+      U.rix[g] = M.rix[f]; U.riy[g] = M.riy[f]; U.riz[g] = M.riz[f];
       U.own[g] = cell(M.own[f]);
       U.nei[g] = cell(M.nei[f]);
       U.fslot[g] = slot(M.fslot[f]);
@@ -744,6 +760,9 @@ static xcal_mesh* upload_mesh(xcal_ctx* ctx, HostMesh& M) {
   upload(m->sfy, M.sfy, bytes);
   upload(m->sfz, M.sfz, bytes);
   upload(m->lapEf, M.lapEf, bytes);
+  upload(m->rix, M.rix, bytes);
+  upload(m->riy, M.riy, bytes);
+  upload(m->riz, M.riz, bytes);
   upload(m->bc_cells, M.bc_cells, bytes);
   upload(m->bc_ptr, M.bc_ptr, bytes);
   upload(m->bc_face, M.bc_face, bytes);
@@ -799,7 +818,7 @@ int64_t prepared_cells(const PreparedMesh& P) { return P.U.n; }
 int64_t prepared_footprint(const PreparedMesh& P, bool turb) {
   const HostMesh& M = P.U;
   int64_t mesh = 0;
-  mesh += 8 * (int64_t)(M.vol.size() + 13 * M.area.size());
+  mesh += 8 * (int64_t)(M.vol.size() + 16 * M.area.size());
   mesh += 4 * (int64_t)(M.cf_ptr.size() + M.cf_face.size() * 3 + M.diag.size() + 3 * M.own.size() +
                         M.bc_cells.size() + M.bc_ptr.size() + M.bc_face.size() + 2 * M.rbeg.size() +
                         2 * M.colval.size() + 2 * M.rows_b.size() + 2 * M.Lcol.size() +

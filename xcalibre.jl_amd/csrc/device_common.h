@@ -227,6 +227,8 @@ __device__ inline void bc_coeffs(const MeshK& m, const BCK& bc, const TermK& t, 
         bp_out = ((-value) * ap) * m.delta[f];
       }
       return;
+    case XCAL_BC_EMPTY:  // empty.jl:46-106: (0, 0) for every operator
+      return;
     case XCAL_BC_ZEROGRADIENT:  // zerogradient.jl:25-76
     case XCAL_BC_KWALL:         // Discretise_3_boundary_conditions.jl:11-95
     case XCAL_BC_NUTWALL:

@@ -103,14 +103,20 @@ void check_setup(const xcal_mesh* m, const xcal_simple_setup* s) {
   }
   for (int i = 0; i < s->n_bcs; ++i)
     XCAL_CHECK(s->bc_U[i].kind != XCAL_BC_NEUMANN, "Neumann is not defined for vector fields");
+  for (int g : {s->grad_p, s->grad_U})
+    XCAL_CHECK(g == XCAL_GRAD_GAUSS || g == XCAL_GRAD_MIDPOINT, "gradient scheme: Gauss or Midpoint");
 }
 
 // grad!(∇p, pf, p, BCs.p) (Calculate_0_gradient.jl:105-109)
 void grad_p(xcal_simple* S) {
   xcal_mesh* m = S->mesh;
-  interpolate(m, S->p->comp(0), S->pf->comp(0));
   const double* ph[1] = {S->p->comp(0)};
   double* fc[1] = {S->pf->comp(0)};
+  if (S->setup.grad_p == XCAL_GRAD_MIDPOINT) {  // Grad{Midpoint} (:235-244)
+    grad_midpoint(m, S->bp, ph, fc, 1, S->gradp->comp(0));
+    return;
+  }
+  interpolate(m, S->p->comp(0), S->pf->comp(0));
   correct_boundaries(m, S->bp, ph, fc, 1);
   green_gauss(m, S->pf->comp(0), S->gradp->comp(0), S->gradp->comp(1), S->gradp->comp(2));
 }
@@ -397,8 +403,14 @@ int run_impl(xcal_simple* S, int32_t iterations, std::vector<double>& R, std::ve
                        S->Hv->comp(1), S->Hv->comp(2), S->gradp->comp(0), S->gradp->comp(1),
                        S->gradp->comp(2), S->rD->comp(0));
       // turbulence!: grad!(gradU) (RANS_laminar.jl:73-79, RANS_kOmega.jl:192)
-      interp_vec(S, S->U);
-      green_gauss_tensor(m, S->Uf->comp(0), S->Uf->comp(1), S->Uf->comp(2), S->gradU->comp(0));
+      if (cfg.grad_U == XCAL_GRAD_MIDPOINT) {  // Grad{Midpoint} of U
+        const double* ph[3] = {S->U->comp(0), S->U->comp(1), S->U->comp(2)};
+        double* fc[3] = {S->Uf->comp(0), S->Uf->comp(1), S->Uf->comp(2)};
+        grad_midpoint(m, S->bU, ph, fc, 3, S->gradU->comp(0));
+      } else {
+        interp_vec(S, S->U);
+        green_gauss_tensor(m, S->Uf->comp(0), S->Uf->comp(1), S->Uf->comp(2), S->gradU->comp(0));
+      }
       const bool turb_conv = S->turb ? komega_step(S) : true;
       update_nueff(S);  // update_nueff!
       mark(4);

@@ -155,6 +155,7 @@ struct xcal_mesh {
   // faces
   xcal::DBuf<int32_t> own, nei, fslot;  // fslot: nz index of (owner, neighbour)
   xcal::DBuf<double> area, weight, delta, nx, ny, nz, ex, ey, ez, sfx, sfy, sfz, lapEf;
+  xcal::DBuf<double> rix, riy, riz;  // Ri = rf - (rP + rN)/2 of internal faces (Midpoint)
   // boundary faces grouped by cell (ascending face id within a cell)
   int64_t nbcell = 0;
   xcal::DBuf<int32_t> bc_cells, bc_ptr, bc_face;
@@ -322,6 +323,8 @@ void interpolate(const xcal_mesh* m, const double* phi, double* phif);
 void correct_boundaries(const xcal_mesh* m, const BCsK& b, const double* const* phi,
                         double* const* phif, int ncomp);
 void green_gauss(const xcal_mesh* m, const double* phif, double* gx, double* gy, double* gz);
+void grad_midpoint(const xcal_mesh* m, const BCsK& b, const double* const* phi,
+                   double* const* phif, int ncomp, double* grad);
 void green_gauss_tensor(const xcal_mesh* m, const double* fx, const double* fy, const double* fz,
                         double* t0);
 void divergence(const xcal_mesh* m, const double* psif, double* div);

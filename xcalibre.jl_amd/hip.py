@@ -82,6 +82,7 @@ def lib():
             "xcal_correct_boundaries": [_vp, _vp, _vp, C.c_int32],
             "xcal_green_gauss": [_vp, _vp],
             "xcal_grad": [_vp, _vp, _vp, _vp, C.c_int32],
+            "xcal_grad_midpoint": [_vp, _vp, _vp, _vp, C.c_int32],
             "xcal_div": [_vp, _vp],
             "xcal_flux": [_vp, _vp],
             "xcal_relax_explicit": [_vp, _vp, C.c_double],

@@ -111,7 +111,7 @@ __global__ void __launch_bounds__(kBlock)
   store_partial<kBlock>(s0, s1, partial, blockIdx.x);
 }
 
-template <class T>
+template <int G, class T>
 __global__ void __launch_bounds__(kBlock)
     kb_cg_spmv(BlkK B, MatKT<T> A, const T* __restrict__ z, const T* __restrict__ pold,
                T* __restrict__ pnew, T* __restrict__ q, double* partial, const KS* ks) {
@@ -121,8 +121,9 @@ __global__ void __launch_bounds__(kBlock)
   const int32_t i = B.row0[blockIdx.x] + threadIdx.x;
   double s0 = 0.0;
   if (i < B.row1[blockIdx.x]) {
-    const T acc = row_sum(A, i, [&](int j) { return z[j] + beta * pold[j]; });
-    const T pi = z[i] + beta * pold[i];
+    const T zi = z[i], poi = pold[i];  // issued with the row's first loads
+    const T acc = row_sum<G>(A, i, [&](int j) { return z[j] + beta * pold[j]; });
+    const T pi = zi + beta * poi;
     if (live) {
       pnew[i] = pi;
       q[i] = acc;
@@ -636,8 +637,10 @@ static void bcg(BSolver<T>& S, int precon, const T* b, T* x0, int itmax, double 
     T* pnew = parity ? pa : pb;
     parity ^= 1;
     timed(c, 0, [&] {
-      hipLaunchKernelGGL(kb_cg_spmv<T>, dim3(S.nblk), dim3(kBlock), 0, S.s, S.B, S.A, (const T*)zd,
-                         (const T*)pold, pnew, q, S.partial, ks);
+      with_row_group(S.A.rw, [&](auto G) {
+        hipLaunchKernelGGL((kb_cg_spmv<decltype(G)::value, T>), dim3(S.nblk), dim3(kBlock), 0, S.s,
+                           S.B, S.A, (const T*)zd, (const T*)pold, pnew, q, S.partial, ks);
+      });
     });
     fin_blocks(S, FIN_CG_PAP);
     if (dilu) {

@@ -278,30 +278,34 @@ __global__ void __launch_bounds__(kBlock)
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double sx = 0.0, sy = 0.0, sz = 0.0;
+  // kG entries at a time, every load issued unconditionally (slots past the
+  // row's last entry re-read it and are left out of the sums)
+  constexpr int kG = 8;
   const int32_t k1 = re[i];
-  for (int32_t k0 = rb[i]; k0 < k1; k0 += kRowAhead * kSlice) {  // loads ahead, then gathers
-    int col[kRowAhead];
-    double v[kRowAhead], ux[kRowAhead], uy[kRowAhead], uz[kRowAhead];
+  const int32_t klast = k1 - kSlice;
+  for (int32_t k0 = rb[i]; k0 < k1; k0 += kG * kSlice) {
+    int col[kG];
+    double v[kG], ux[kG], uy[kG], uz[kG];
 #pragma unroll
-    for (int u = 0; u < kRowAhead; ++u) {
-      const int32_t k = k0 + u * kSlice;
-      col[u] = k < k1 ? colval[k] : -1;
-      v[u] = k < k1 ? nz[k] : 0.0;
+    for (int u = 0; u < kG; ++u) {
+      const int32_t k = min(k0 + u * kSlice, klast);
+      col[u] = colval[k];
+      v[u] = nz[k];
     }
 #pragma unroll
-    for (int u = 0; u < kRowAhead; ++u) {
-      const bool ok = col[u] >= 0;
-      ux[u] = ok ? Ux[col[u]] : 0.0;
-      uy[u] = ok ? Uy[col[u]] : 0.0;
-      uz[u] = ok ? Uz[col[u]] : 0.0;
+    for (int u = 0; u < kG; ++u) {
+      ux[u] = Ux[col[u]];
+      uy[u] = Uy[col[u]];
+      uz[u] = Uz[col[u]];
     }
 #pragma unroll
-    for (int u = 0; u < kRowAhead; ++u)
-      if (col[u] >= 0) {
-        sx += v[u] * ux[u];
-        sy += v[u] * uy[u];
-        sz += v[u] * uz[u];
-      }
+    for (int u = 0; u < kG; ++u) {
+      const bool in = k0 + u * kSlice < k1;
+      const double tx = v[u] * ux[u], ty = v[u] * uy[u], tz = v[u] * uz[u];
+      sx = in ? sx + tx : sx;
+      sy = in ? sy + ty : sy;
+      sz = in ? sz + tz : sz;
+    }
   }
   const double D = nz[diag[i]];
   sx -= D * Ux[i];

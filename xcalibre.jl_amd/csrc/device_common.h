@@ -30,11 +30,12 @@ struct MatKT {
   const int32_t *rb, *re, *colval, *diag;
   const T* nz;
   const int32_t* blk;  // row blocks in dispatch order (xcal_mesh::xblk)
+  int32_t rw;          // most entries in one row (selects row_sum's group size)
 };
 using MatK = MatKT<double>;
 template <class T>
 inline MatKT<T> matk(const xcal_mesh* m, const T* nz) {
-  return MatKT<T>{m->n, m->rbeg.p, m->rend.p, m->colval.p, m->diag.p, nz, m->xblk_d.p};
+  return MatKT<T>{m->n, m->rbeg.p, m->rend.p, m->colval.p, m->diag.p, nz, m->xblk_d.p, m->roww};
 }
 // Row of this work-item in a launch over A's row blocks (grid = nxblk);
 // valid while row < end.
@@ -81,35 +82,60 @@ inline MeshK meshk(const xcal_mesh* m) {
   return k;
 }
 
-// sum over row i of nz[k] * xv(colval[k]) in the row's column order.  All of
-// a row's column indices and values are loaded before any of the gathers
-// they feed, so a row costs three dependent memory round trips instead of
-// two per entry.
-constexpr int kRowAhead = 8;
-template <class T, class XV>
-__device__ inline T row_sum(const MatKT<T>& A, int64_t i, XV&& xv) {
+// sum over row i of nz[k] * xv(colval[k]) in the row's column order, G
+// entries at a time.  All loads of a group are issued unconditionally: slots
+// past the row's last entry re-read that entry (every row holds at least its
+// diagonal) and are left out of the sum.  With no branch between them, the
+// compiler issues the group's column and value loads together, then all its
+// gathers, and waits for each gathered value only where it is added; a
+// conditional load per entry made it wait for every load after each entry.
+template <int G, class T, class XV>
+__device__ inline T row_sum_g(const MatKT<T>& A, int64_t i, XV&& xv) {
   const int32_t k1 = A.re[i];
+  const int32_t klast = k1 - kSlice;
   T acc = 0;
-  for (int32_t k0 = A.rb[i]; k0 < k1; k0 += kRowAhead * kSlice) {
-    int col[kRowAhead];
-    T a[kRowAhead], x[kRowAhead];
+  for (int32_t k0 = A.rb[i]; k0 < k1; k0 += G * kSlice) {
+    int col[G];
+    T a[G], x[G];
 #pragma unroll
-    for (int u = 0; u < kRowAhead; ++u) {
-      const int32_t k = k0 + u * kSlice;
-      col[u] = -1;
-      a[u] = 0;
-      if (k < k1) {  // streamed once: keep them out of the way of the gathers
-        col[u] = __builtin_nontemporal_load(&A.colval[k]);
-        a[u] = __builtin_nontemporal_load(&A.nz[k]);
-      }
+    for (int u = 0; u < G; ++u) {
+      const int32_t k = min(k0 + u * kSlice, klast);
+      // streamed once: keep them out of the way of the gathers
+      col[u] = __builtin_nontemporal_load(&A.colval[k]);
+      a[u] = __builtin_nontemporal_load(&A.nz[k]);
     }
 #pragma unroll
-    for (int u = 0; u < kRowAhead; ++u) x[u] = col[u] >= 0 ? xv(col[u]) : T(0);
+    for (int u = 0; u < G; ++u) x[u] = xv(col[u]);
 #pragma unroll
-    for (int u = 0; u < kRowAhead; ++u)
-      if (col[u] >= 0) acc += a[u] * x[u];
+    for (int u = 0; u < G; ++u) {
+      const T t = a[u] * x[u];
+      acc = k0 + u * kSlice < k1 ? acc + t : acc;
+    }
   }
   return acc;
+}
+// The group is as wide as the mesh's widest row, up to 8 (quads and tets 5,
+// hexahedra 7), so a row's loads are issued in one group with few re-reads.
+// A.rw is uniform over the launch: the choice is a scalar branch.
+// G > 0 fixes the group at compile time (a kernel instantiated per group
+// width keeps the registers, and so the occupancy, of its own width).
+template <int G = 0, class T, class XV>
+__device__ inline T row_sum(const MatKT<T>& A, int64_t i, XV&& xv) {
+  if constexpr (G > 0) {
+    return row_sum_g<G>(A, i, xv);
+  } else {
+    if (A.rw <= 5) return row_sum_g<5>(A, i, xv);
+    if (A.rw <= 7) return row_sum_g<7>(A, i, xv);
+    return row_sum_g<8>(A, i, xv);
+  }
+}
+// Host side: calls f(std::integral_constant<int, G>) with row_sum's group
+// width for rows of at most rw entries.
+template <class F>
+inline void with_row_group(int rw, F&& f) {
+  if (rw <= 5) f(std::integral_constant<int, 5>{});
+  else if (rw <= 7) f(std::integral_constant<int, 7>{});
+  else f(std::integral_constant<int, 8>{});
 }
 
 // Julia's max for Float64: NaN-propagating, +0.0 preferred over -0.0

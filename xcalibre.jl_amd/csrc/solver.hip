@@ -97,7 +97,7 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // CG: pnew = z + beta pold (on the fly for neighbours) ; q = A pnew ; p.q
-template <class T>
+template <int G, class T>
 __global__ void __launch_bounds__(kBlock)
     k_cg_spmv(MatKT<T> A, const T* __restrict__ z, const T* __restrict__ pold,
               T* __restrict__ pnew, T* __restrict__ q, double* partial, const KS* ks) {
@@ -107,8 +107,9 @@ __global__ void __launch_bounds__(kBlock)
   const int64_t i = block_row(A, end);
   double s0 = 0.0;
   if (i < end) {
-    const T acc = row_sum(A, i, [&](int j) { return z[j] + beta * pold[j]; });
-    const T pi = z[i] + beta * pold[i];
+    const T zi = z[i], poi = pold[i];  // issued with the row's first loads
+    const T acc = row_sum<G>(A, i, [&](int j) { return z[j] + beta * pold[j]; });
+    const T pi = zi + beta * poi;
     if (live) {
       pnew[i] = pi;
       q[i] = acc;
@@ -654,8 +655,10 @@ static int cg(Solver<T>& S, int precon, const T* b, T* x0, int itmax, double ato
     parity ^= 1;
     xcal_ctx* c = S.m->ctx;
     timed(c, 0, [&] {
-      hipLaunchKernelGGL(k_cg_spmv<T>, dim3(S.gridA), dim3(kBlock), 0, S.s, S.A, (const T*)zd,
-                         (const T*)pold, pnew, q, S.partial, (const KS*)S.ks);
+      with_row_group(S.A.rw, [&](auto G) {
+        hipLaunchKernelGGL((k_cg_spmv<decltype(G)::value, T>), dim3(S.gridA), dim3(kBlock), 0, S.s,
+                           S.A, (const T*)zd, (const T*)pold, pnew, q, S.partial, (const KS*)S.ks);
+      });
     });
     finalize_launch(S, S.gridA, FIN_CG_PAP);
     if (dilu) {

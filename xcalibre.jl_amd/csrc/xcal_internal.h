@@ -168,6 +168,7 @@ struct xcal_mesh {
   // device matrix storage (sliced ELL): entries of row c at
   // rbeg[c], rbeg[c] + kSlice, ... < rend[c], in the reference's column order
   int64_t nzlen = 0;  // storage length including slice padding
+  int32_t roww = 0;   // most entries in one row
   xcal::DBuf<int32_t> rbeg, rend, colval, trans;
   // DILU level schedules: forward level l = device rows
   // [lev_ptr_f[l], lev_ptr_f[l+1]); backward level l = lev_rows_b over

@@ -291,7 +291,7 @@ def batch_line(args, x, precision, steps=None, warmup=None):
                      "traffic": hbm_traffic(args.roofline_kernel,
                                             f"batch{args.cases}_{precision}"),
                      "kernel": args.roofline_kernel, "bytes_per_launch": nbytes,
-                     "launches": launches, "avg_us": round(avg_s * 1e6, 2)},
+                     "timed_launches": launches, "avg_us": round(avg_s * 1e6, 2)},
         "cpu_baseline": None,
         "detail": {"mesh_build_s": round(t_mesh, 2), "upload_s": round(t_upload, 2),
                    "case_iters_per_s": round(args.cases * steps / wall, 1),
@@ -361,8 +361,8 @@ def kernel_bytes(name, n, nnz, vb=8):
     not counted).  cg_spmv is one launch; dilu_fwd/dilu_bwd are one whole
     level-synchronous sweep (one launch per large level)."""
     nlow = (nnz - n) // 2
-    if name == "cg_spmv":   # row begin/end; colval+nz; z and pold read; pnew and q written
-        return 8 * n + (4 + vb) * nnz + vb * 4 * n
+    if name == "cg_spmv":   # row begin/end; colval+nz; p read (gathers included once); q written
+        return 8 * n + (4 + vb) * nnz + vb * 2 * n
     if name == "dilu_fwd":  # row begin, diag; lower colval+nz; rD and y gathered; y written;
         # p, q read; x, r read+written
         return 8 * n + 12 * nlow + 8 * n * (1 + 1 + 1 + 2 + 4)
@@ -465,7 +465,7 @@ def main():
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": args.roofline_kernel, "bytes_per_launch": nbytes,
-                "launches": launches, "avg_us": round(avg_s * 1e6, 2)}
+                "timed_launches": launches, "avg_us": round(avg_s * 1e6, 2)}
 
     err = None
     if args.precision != "fp64":

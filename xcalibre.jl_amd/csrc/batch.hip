@@ -111,23 +111,32 @@ __global__ void __launch_bounds__(kBlock)
   store_partial<kBlock>(s0, s1, partial, blockIdx.x);
 }
 
-template <int G, class T>
+// p = z + beta p per case, over the rows each workgroup of kb_cg_spmv reads
+// as its own
+template <class T>
 __global__ void __launch_bounds__(kBlock)
-    kb_cg_spmv(BlkK B, MatKT<T> A, const T* __restrict__ z, const T* __restrict__ pold,
-               T* __restrict__ pnew, T* __restrict__ q, double* partial, const KS* ks) {
+    kb_cg_p(BlkK B, const T* __restrict__ z, const T* __restrict__ pold, T* __restrict__ pnew,
+            const KS* ks) {
   const KS* K = ks + B.kcase[blockIdx.x];
   const bool live = !stopped(K);
   const T beta = (T)K->beta;
   const int32_t i = B.row0[blockIdx.x] + threadIdx.x;
+  if (i < B.row1[blockIdx.x] && live) pnew[i] = z[i] + beta * pold[i];
+}
+
+// q = A p ; p.q per case (one gather per entry; p formed by kb_cg_p)
+template <int G, class T>
+__global__ void __launch_bounds__(kBlock)
+    kb_cg_spmv(BlkK B, MatKT<T> A, const T* __restrict__ p, T* __restrict__ q, double* partial,
+               const KS* ks) {
+  const KS* K = ks + B.kcase[blockIdx.x];
+  const bool live = !stopped(K);
+  const int32_t i = B.row0[blockIdx.x] + threadIdx.x;
   double s0 = 0.0;
   if (i < B.row1[blockIdx.x]) {
-    const T zi = z[i], poi = pold[i];  // issued with the row's first loads
-    const T acc = row_sum<G>(A, i, [&](int j) { return z[j] + beta * pold[j]; });
-    const T pi = zi + beta * poi;
-    if (live) {
-      pnew[i] = pi;
-      q[i] = acc;
-    }
+    const T pi = p[i];  // issued with the row's first loads
+    const T acc = row_sum<G>(A, i, [&](int j) { return p[j]; });
+    if (live) q[i] = acc;
     s0 = dd(pi, acc);
   }
   store_partial<kBlock>(s0, 0.0, partial, blockIdx.x);
@@ -636,10 +645,12 @@ static void bcg(BSolver<T>& S, int precon, const T* b, T* x0, int itmax, double 
     T* pold = parity ? pb : pa;
     T* pnew = parity ? pa : pb;
     parity ^= 1;
+    hipLaunchKernelGGL(kb_cg_p<T>, dim3(S.nblk), dim3(kBlock), 0, S.s, S.B, (const T*)zd,
+                       (const T*)pold, pnew, ks);
     timed(c, 0, [&] {
       with_row_group(S.A.rw, [&](auto G) {
         hipLaunchKernelGGL((kb_cg_spmv<decltype(G)::value, T>), dim3(S.nblk), dim3(kBlock), 0, S.s,
-                           S.B, S.A, (const T*)zd, (const T*)pold, pnew, q, S.partial, ks);
+                           S.B, S.A, (const T*)pnew, q, S.partial, ks);
       });
     });
     fin_blocks(S, FIN_CG_PAP);

@@ -96,24 +96,34 @@ __global__ void __launch_bounds__(kBlock)
   store_partial<kBlock>(s0, s1, partial, blockIdx.x);
 }
 
-// CG: pnew = z + beta pold (on the fly for neighbours) ; q = A pnew ; p.q
-template <int G, class T>
+// CG: p = z + beta p, over the rows of A's row blocks (the rows each
+// workgroup of k_cg_spmv reads as its own).
+template <class T>
 __global__ void __launch_bounds__(kBlock)
-    k_cg_spmv(MatKT<T> A, const T* __restrict__ z, const T* __restrict__ pold,
-              T* __restrict__ pnew, T* __restrict__ q, double* partial, const KS* ks) {
+    k_cg_p(MatKT<T> A, const T* __restrict__ z, const T* __restrict__ pold, T* __restrict__ pnew,
+           const KS* ks) {
   const bool live = !stopped(ks);
   const T beta = (T)ks->beta;
   int64_t end;
   const int64_t i = block_row(A, end);
+  if (i < end && live) pnew[i] = z[i] + beta * pold[i];
+}
+
+// CG: q = A p ; p.q.  p is formed by k_cg_p beforehand, so each entry
+// gathers one value (forming p on the fly for every neighbour doubles the
+// gathers, and the gathers bound this kernel).
+template <int G, class T>
+__global__ void __launch_bounds__(kBlock)
+    k_cg_spmv(MatKT<T> A, const T* __restrict__ p, T* __restrict__ q, double* partial,
+              const KS* ks) {
+  const bool live = !stopped(ks);
+  int64_t end;
+  const int64_t i = block_row(A, end);
   double s0 = 0.0;
   if (i < end) {
-    const T zi = z[i], poi = pold[i];  // issued with the row's first loads
-    const T acc = row_sum<G>(A, i, [&](int j) { return z[j] + beta * pold[j]; });
-    const T pi = zi + beta * poi;
-    if (live) {
-      pnew[i] = pi;
-      q[i] = acc;
-    }
+    const T pi = p[i];  // issued with the row's first loads
+    const T acc = row_sum<G>(A, i, [&](int j) { return p[j]; });
+    if (live) q[i] = acc;
     s0 = dd(pi, acc);
   }
   store_partial<kBlock>(s0, 0.0, partial, blockIdx.x);
@@ -654,10 +664,12 @@ static int cg(Solver<T>& S, int precon, const T* b, T* x0, int itmax, double ato
     T* pnew = parity ? pa : pb;
     parity ^= 1;
     xcal_ctx* c = S.m->ctx;
+    hipLaunchKernelGGL(k_cg_p<T>, dim3(S.gridA), dim3(kBlock), 0, S.s, S.A, (const T*)zd,
+                       (const T*)pold, pnew, (const KS*)S.ks);
     timed(c, 0, [&] {
       with_row_group(S.A.rw, [&](auto G) {
         hipLaunchKernelGGL((k_cg_spmv<decltype(G)::value, T>), dim3(S.gridA), dim3(kBlock), 0, S.s,
-                           S.A, (const T*)zd, (const T*)pold, pnew, q, S.partial, (const KS*)S.ks);
+                           S.A, (const T*)pnew, q, S.partial, (const KS*)S.ks);
       });
     });
     finalize_launch(S, S.gridA, FIN_CG_PAP);
@@ -891,6 +903,7 @@ int xcal_kernel_timing_enable(xcal_ctx* c, int32_t kernel, int32_t enable) {
     XCAL_HIP(hipStreamSynchronize(c->stream));
     prof_resolve(c);
     c->prof_kernel = enable ? kernel : -1;
+    c->prof_seq = 0;
     c->prof_launches = 0;
     c->prof_ms = 0.0;
     return 0;

@@ -316,10 +316,13 @@ __device__ inline void bwd_row(const BwdK<T>& W, int32_t pos, int64_t e0, int32_
 
 
 // ----------------------------------------------------- kernel timing hook
+// Timing-only events: without the system-scope fence, recording one does not
+// write back and invalidate the caches between the kernels it separates.
+// Elapsed times are read only after the stream has been synchronised.
 inline hipEvent_t prof_event(xcal_ctx* c) {
   if (c->prof_pool.empty()) {
     hipEvent_t ev;
-    XCAL_HIP(hipEventCreate(&ev));
+    XCAL_HIP(hipEventCreateWithFlags(&ev, hipEventDisableSystemFence));
     return ev;
   }
   hipEvent_t ev = c->prof_pool.back();
@@ -328,10 +331,12 @@ inline hipEvent_t prof_event(xcal_ctx* c) {
 }
 
 // Launch `fn` on the context stream, bracketed by events when kernel `k` is
-// the one being timed.
+// the one being timed: every kProfEvery-th launch of it, starting with the
+// first, so that the events cost the timed work little.
+constexpr int64_t kProfEvery = 8;
 template <class Fn>
 inline void timed(xcal_ctx* c, int k, Fn&& fn) {
-  if (c->prof_kernel != k) {
+  if (c->prof_kernel != k || c->prof_seq++ % kProfEvery != 0) {
     fn();
     return;
   }

@@ -127,6 +127,7 @@ struct xcal_ctx {
   std::vector<hipEvent_t> prof_pool;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_pending;
   int64_t prof_launches = 0;
+  int64_t prof_seq = 0;  // launches of the timed kernel since timing was enabled
   double prof_ms = 0.0;
 };
 

@@ -329,8 +329,9 @@ def log(msg):
 
 def run_calib(args):
     """HBM counter calibration: a plain 256 MiB device-to-device copy, three
-    times.  rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this command give the
-    counter reading of a kernel that moves a known number of bytes."""
+    times.  rocprofv3 --pmc of the TCC_EA0_RDREQ_* counters and of WRITE_SIZE
+    on this command give the counter readings of a kernel that moves a known
+    number of bytes."""
     import torch
     nbytes = 256 << 20
     a = torch.ones(nbytes, dtype=torch.uint8, device="cuda")
@@ -345,8 +346,9 @@ def run_calib(args):
 
 def hbm_traffic(kernel, workload):
     """HBM bytes per launch of `kernel` on `workload` from the committed
-    rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, calibrated against a
-    copy kernel), or None when no summary covers this pair."""
+    rocprofv3 --pmc summary (L2-to-memory read requests by size, plus
+    WRITE_SIZE, calibrated against a copy kernel), or None when no summary
+    covers this pair."""
     import re
     path = ROOT / "profiles" / f"hbm_traffic_{kernel}_{workload}.md"
     if not path.exists():

@@ -74,6 +74,8 @@ def meshes(x, fixture_dir):
                 cache[name] = M.box3d(12, 10, 8, jitter=0.1)
             elif name == "box20":
                 cache[name] = M.box3d(20, 20, 20, jitter=0.1)
+            elif name == "box70":  # 343 000 cells: above the CG split threshold
+                cache[name] = M.box3d(70, 70, 70, jitter=0.1)
             elif name == "box12_natural":
                 cache[name] = M.box3d(12, 10, 8, jitter=0.0, seed_perm=0)
             elif name == "step":

@@ -166,7 +166,7 @@ __global__ void __launch_bounds__(kBlock)
   store_partial<kBlock>(s0, 0.0, partial, blockIdx.x);
 }
 
-template <bool VDIAG, class T>
+template <int G, bool VDIAG, class T>
 __global__ void __launch_bounds__(kBlock)
     kb_bi_spmv_p(BlkK B, MatKT<T> A, const T* __restrict__ r, const T* __restrict__ pold,
                  T* __restrict__ pnew, const T* __restrict__ S, T* __restrict__ qv,
@@ -177,7 +177,7 @@ __global__ void __launch_bounds__(kBlock)
   const int32_t i = B.row0[blockIdx.x] + threadIdx.x;
   double s0 = 0.0;
   if (i < B.row1[blockIdx.x]) {
-    const T acc = row_sum(A, i, [&](int j) { return r[j] + beta * pold[j]; });
+    const T acc = row_sum<G>(A, i, [&](int j) { return r[j] + beta * pold[j]; });
     const T pi = r[i] + beta * pold[i];
     const T vi = VDIAG && S ? S[i] * acc : acc;
     if (live) {
@@ -189,7 +189,7 @@ __global__ void __launch_bounds__(kBlock)
   if (VDIAG) store_partial<kBlock>(s0, 0.0, partial, blockIdx.x);
 }
 
-template <bool TDIAG, class T>
+template <int G, bool TDIAG, class T>
 __global__ void __launch_bounds__(kBlock)
     kb_bi_spmv_s(BlkK B, MatKT<T> A, const T* __restrict__ r, const T* __restrict__ v,
                  const T* __restrict__ p, T* __restrict__ s, T* __restrict__ x,
@@ -200,7 +200,7 @@ __global__ void __launch_bounds__(kBlock)
   const int32_t i = B.row0[blockIdx.x] + threadIdx.x;
   double s0 = 0.0, s1 = 0.0;
   if (i < B.row1[blockIdx.x]) {
-    const T acc = row_sum(A, i, [&](int j) { return r[j] + (-a) * v[j]; });
+    const T acc = row_sum<G>(A, i, [&](int j) { return r[j] + (-a) * v[j]; });
     const T si = r[i] + (-a) * v[i];
     const T xi = x[i] + a * p[i];
     const T ti = TDIAG && S ? S[i] * acc : acc;
@@ -704,23 +704,39 @@ static void bbicg(BSolver<T>& S, int precon, const T* b, T* x0, int itmax, doubl
     T* pnew = parity ? pa : pb;
     parity ^= 1;
     if (dilu) {
-      hipLaunchKernelGGL((kb_bi_spmv_p<false, T>), dim3(S.nblk), dim3(kBlock), 0, S.s, S.B, S.A,
-                         (const T*)r, (const T*)pold, pnew, (const T*)nullptr, d, b, S.partial, ks);
+      with_row_group(S.A.rw, [&](auto G) {
+        constexpr int g = decltype(G)::value;
+        hipLaunchKernelGGL((kb_bi_spmv_p<g, false, T>), dim3(S.nblk), dim3(kBlock), 0,
+                           S.s, S.B, S.A,
+                           (const T*)r, (const T*)pold, pnew, (const T*)nullptr, d, b, S.partial, ks);
+      });
       bsweep_fwd<false>(S, bfwdk(S, (const T*)d, Y, Yb), ks);
       bsweep_bwd<2>(S, bbwdk(S, (const T*)Yb, v, b), ks);
       fin_cases(S, FIN_BI_ALPHA);
-      hipLaunchKernelGGL((kb_bi_spmv_s<false, T>), dim3(S.nblk), dim3(kBlock), 0, S.s, S.B, S.A,
-                         (const T*)r, (const T*)v, (const T*)pnew, sv, x, (const T*)nullptr, d,
-                         S.partial, ks);
+      with_row_group(S.A.rw, [&](auto G) {
+        constexpr int g = decltype(G)::value;
+        hipLaunchKernelGGL((kb_bi_spmv_s<g, false, T>), dim3(S.nblk), dim3(kBlock), 0,
+                           S.s, S.B, S.A,
+                           (const T*)r, (const T*)v, (const T*)pnew, sv, x, (const T*)nullptr, d,
+                           S.partial, ks);
+      });
       bsweep_fwd<false>(S, bfwdk(S, (const T*)d, Y, Yb), ks);
       bsweep_bwd<3>(S, bbwdk(S, (const T*)Yb, t, (const T*)sv), ks);
       fin_cases(S, FIN_BI_OMEGA);
     } else {
-      hipLaunchKernelGGL((kb_bi_spmv_p<true, T>), dim3(S.nblk), dim3(kBlock), 0, S.s, S.B, S.A,
-                         (const T*)r, (const T*)pold, pnew, Sd, v, b, S.partial, ks);
+      with_row_group(S.A.rw, [&](auto G) {
+        constexpr int g = decltype(G)::value;
+        hipLaunchKernelGGL((kb_bi_spmv_p<g, true, T>), dim3(S.nblk), dim3(kBlock), 0,
+                           S.s, S.B, S.A,
+                           (const T*)r, (const T*)pold, pnew, Sd, v, b, S.partial, ks);
+      });
       fin_blocks(S, FIN_BI_ALPHA);
-      hipLaunchKernelGGL((kb_bi_spmv_s<true, T>), dim3(S.nblk), dim3(kBlock), 0, S.s, S.B, S.A,
-                         (const T*)r, (const T*)v, (const T*)pnew, sv, x, Sd, t, S.partial, ks);
+      with_row_group(S.A.rw, [&](auto G) {
+        constexpr int g = decltype(G)::value;
+        hipLaunchKernelGGL((kb_bi_spmv_s<g, true, T>), dim3(S.nblk), dim3(kBlock), 0,
+                           S.s, S.B, S.A,
+                           (const T*)r, (const T*)v, (const T*)pnew, sv, x, Sd, t, S.partial, ks);
+      });
       fin_blocks(S, FIN_BI_OMEGA);
     }
     hipLaunchKernelGGL(kb_bi_tail<T>, dim3(S.nblk), dim3(kBlock), 0, S.s, S.B, (const T*)sv,

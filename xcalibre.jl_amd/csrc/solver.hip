@@ -109,6 +109,31 @@ __global__ void __launch_bounds__(kBlock)
   if (i < end && live) pnew[i] = z[i] + beta * pold[i];
 }
 
+// CG in one launch for meshes below kSplitRows rows: p = z + beta p on the
+// fly for every neighbour ; q = A p ; p.q.  There the launch costs more than
+// the second gather per entry.
+template <int G, class T>
+__global__ void __launch_bounds__(kBlock)
+    k_cg_pspmv(MatKT<T> A, const T* __restrict__ z, const T* __restrict__ pold,
+               T* __restrict__ pnew, T* __restrict__ q, double* partial, const KS* ks) {
+  const bool live = !stopped(ks);
+  const T beta = (T)ks->beta;
+  int64_t end;
+  const int64_t i = block_row(A, end);
+  double s0 = 0.0;
+  if (i < end) {
+    const T zi = z[i], poi = pold[i];  // issued with the row's first loads
+    const T acc = row_sum<G>(A, i, [&](int j) { return z[j] + beta * pold[j]; });
+    const T pi = zi + beta * poi;
+    if (live) {
+      pnew[i] = pi;
+      q[i] = acc;
+    }
+    s0 = dd(pi, acc);
+  }
+  store_partial<kBlock>(s0, 0.0, partial, blockIdx.x);
+}
+
 // CG: q = A p ; p.q.  p is formed by k_cg_p beforehand, so each entry
 // gathers one value (forming p on the fly for every neighbour doubles the
 // gathers, and the gathers bound this kernel).
@@ -155,7 +180,7 @@ __global__ void __launch_bounds__(kBlock)
 
 // BiCGSTAB: pnew = r + beta pold (on the fly) ; q = A pnew ; v = S q (diag
 // preconditioner, fused when VDIAG) ; c.v
-template <bool VDIAG, class T>
+template <int G, bool VDIAG, class T>
 __global__ void __launch_bounds__(kBlock)
     k_bi_spmv_p(MatKT<T> A, const T* __restrict__ r, const T* __restrict__ pold,
                 T* __restrict__ pnew, const T* __restrict__ S, T* __restrict__ qv,
@@ -166,7 +191,7 @@ __global__ void __launch_bounds__(kBlock)
   const int64_t i = block_row(A, end);
   double s0 = 0.0;
   if (i < end) {
-    const T acc = row_sum(A, i, [&](int j) { return r[j] + beta * pold[j]; });
+    const T acc = row_sum<G>(A, i, [&](int j) { return r[j] + beta * pold[j]; });
     const T pi = r[i] + beta * pold[i];
     const T vi = VDIAG && S ? S[i] * acc : acc;
     if (live) {
@@ -180,7 +205,7 @@ __global__ void __launch_bounds__(kBlock)
 
 // BiCGSTAB: s = r - a v (on the fly) ; d = A s ; x += a p ;
 // t = S d and the dots t.s, t.t when TDIAG
-template <bool TDIAG, class T>
+template <int G, bool TDIAG, class T>
 __global__ void __launch_bounds__(kBlock)
     k_bi_spmv_s(MatKT<T> A, const T* __restrict__ r, const T* __restrict__ v,
                 const T* __restrict__ p, T* __restrict__ s, T* __restrict__ x,
@@ -191,7 +216,7 @@ __global__ void __launch_bounds__(kBlock)
   const int64_t i = block_row(A, end);
   double s0 = 0.0, s1 = 0.0;
   if (i < end) {
-    const T acc = row_sum(A, i, [&](int j) { return r[j] + (-a) * v[j]; });
+    const T acc = row_sum<G>(A, i, [&](int j) { return r[j] + (-a) * v[j]; });
     const T si = r[i] + (-a) * v[i];
     const T xi = x[i] + a * p[i];
     const T ti = TDIAG && S ? S[i] * acc : acc;
@@ -664,12 +689,19 @@ static int cg(Solver<T>& S, int precon, const T* b, T* x0, int itmax, double ato
     T* pnew = parity ? pa : pb;
     parity ^= 1;
     xcal_ctx* c = S.m->ctx;
-    hipLaunchKernelGGL(k_cg_p<T>, dim3(S.gridA), dim3(kBlock), 0, S.s, S.A, (const T*)zd,
-                       (const T*)pold, pnew, (const KS*)S.ks);
+    const bool split = n >= kSplitRows;
+    if (split)
+      hipLaunchKernelGGL(k_cg_p<T>, dim3(S.gridA), dim3(kBlock), 0, S.s, S.A, (const T*)zd,
+                         (const T*)pold, pnew, (const KS*)S.ks);
     timed(c, 0, [&] {
       with_row_group(S.A.rw, [&](auto G) {
-        hipLaunchKernelGGL((k_cg_spmv<decltype(G)::value, T>), dim3(S.gridA), dim3(kBlock), 0, S.s,
-                           S.A, (const T*)pnew, q, S.partial, (const KS*)S.ks);
+        constexpr int g = decltype(G)::value;
+        if (split)
+          hipLaunchKernelGGL((k_cg_spmv<g, T>), dim3(S.gridA), dim3(kBlock), 0, S.s, S.A,
+                             (const T*)pnew, q, S.partial, (const KS*)S.ks);
+        else
+          hipLaunchKernelGGL((k_cg_pspmv<g, T>), dim3(S.gridA), dim3(kBlock), 0, S.s, S.A,
+                             (const T*)zd, (const T*)pold, pnew, q, S.partial, (const KS*)S.ks);
       });
     });
     finalize_launch(S, S.gridA, FIN_CG_PAP);
@@ -729,24 +761,40 @@ static int bicgstab(Solver<T>& S, int precon, const T* b, T* x0, int itmax, doub
     T* pnew = parity ? pa : pb;
     parity ^= 1;
     if (dilu) {
-      hipLaunchKernelGGL((k_bi_spmv_p<false, T>), dim3(S.gridA), dim3(kBlock), 0, S.s, S.A,
-                         (const T*)r, (const T*)pold, pnew, (const T*)nullptr, d, b, S.partial,
-                         ks);  // d holds q
+      with_row_group(S.A.rw, [&](auto G) {  // d holds q
+        constexpr int g = decltype(G)::value;
+        hipLaunchKernelGGL((k_bi_spmv_p<g, false, T>), dim3(S.gridA), dim3(kBlock), 0,
+                           S.s, S.A,
+                           (const T*)r, (const T*)pold, pnew, (const T*)nullptr, d, b, S.partial,
+                           ks);
+      });
       sweep_fwd<false>(S, fwdk(S, (const T*)d, Y, Yb, ks));
       sweep_bwd<2>(S, bwdk(S, (const T*)Yb, v, b, ks));
       finalize_launch(S, sweep_slots, FIN_BI_ALPHA);
-      hipLaunchKernelGGL((k_bi_spmv_s<false, T>), dim3(S.gridA), dim3(kBlock), 0, S.s, S.A,
-                         (const T*)r, (const T*)v, (const T*)pnew, sv, x, (const T*)nullptr, d,
-                         S.partial, ks);
+      with_row_group(S.A.rw, [&](auto G) {
+        constexpr int g = decltype(G)::value;
+        hipLaunchKernelGGL((k_bi_spmv_s<g, false, T>), dim3(S.gridA), dim3(kBlock), 0,
+                           S.s, S.A,
+                           (const T*)r, (const T*)v, (const T*)pnew, sv, x, (const T*)nullptr, d,
+                           S.partial, ks);
+      });
       sweep_fwd<false>(S, fwdk(S, (const T*)d, Y, Yb, ks));
       sweep_bwd<3>(S, bwdk(S, (const T*)Yb, t, (const T*)sv, ks));
       finalize_launch(S, sweep_slots, FIN_BI_OMEGA);
     } else {
-      hipLaunchKernelGGL((k_bi_spmv_p<true, T>), dim3(S.gridA), dim3(kBlock), 0, S.s, S.A,
-                         (const T*)r, (const T*)pold, pnew, Sd, v, b, S.partial, ks);
+      with_row_group(S.A.rw, [&](auto G) {
+        constexpr int g = decltype(G)::value;
+        hipLaunchKernelGGL((k_bi_spmv_p<g, true, T>), dim3(S.gridA), dim3(kBlock), 0,
+                           S.s, S.A,
+                           (const T*)r, (const T*)pold, pnew, Sd, v, b, S.partial, ks);
+      });
       finalize_launch(S, S.gridA, FIN_BI_ALPHA);
-      hipLaunchKernelGGL((k_bi_spmv_s<true, T>), dim3(S.gridA), dim3(kBlock), 0, S.s, S.A,
-                         (const T*)r, (const T*)v, (const T*)pnew, sv, x, Sd, t, S.partial, ks);
+      with_row_group(S.A.rw, [&](auto G) {
+        constexpr int g = decltype(G)::value;
+        hipLaunchKernelGGL((k_bi_spmv_s<g, true, T>), dim3(S.gridA), dim3(kBlock), 0,
+                           S.s, S.A,
+                           (const T*)r, (const T*)v, (const T*)pnew, sv, x, Sd, t, S.partial, ks);
+      });
       finalize_launch(S, S.gridA, FIN_BI_OMEGA);
     }
     hipLaunchKernelGGL(k_bi_tail<T>, dim3(S.grid), dim3(kBlock), 0, S.s, n, (const T*)sv,

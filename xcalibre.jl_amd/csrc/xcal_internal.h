@@ -55,6 +55,10 @@ constexpr int kMaxPatches = 32;
 constexpr int kSlice = 64;        // rows per slice of the sliced-ELL matrix storage
 constexpr int kRunBlock = 1024;   // workgroup size of the single-workgroup sweep runs
 constexpr int kSmallLevel = 256;  // levels below this many rows join a single-workgroup run
+// Meshes with at least this many rows form CG's p = z + beta p in a streamed
+// pass before a single-gather SpMV; smaller ones form it on the fly inside the
+// SpMV, saving a launch.
+constexpr int64_t kSplitRows = 1 << 18;
 
 // Compile-time entry bound for a level whose rows have w entries: 0 selects
 // the general loop.

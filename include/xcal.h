@@ -221,6 +221,11 @@ int xcal_eqn_destroy(xcal_eqn* e);
  * prev: the field (scalar or vector) used by Time terms. */
 int xcal_discretise(xcal_eqn* e, const xcal_term* terms, int32_t nterms,
                     const xcal_source* sources, int32_t nsources, const xcal_field* prev);
+/* discretise! with config.runtime.dt (Discretise_1_schemes.jl:25-42,
+ * Time{Euler}: a_P += V/dt, b += prev V/dt); dt > 0. */
+int xcal_discretise_dt(xcal_eqn* e, const xcal_term* terms, int32_t nterms,
+                       const xcal_source* sources, int32_t nsources, const xcal_field* prev,
+                       double dt);
 /* update_equation! (Discretise_2_generated_distretisation.jl:305-330): A <- A0 */
 int xcal_update_equation(xcal_eqn* e);
 /* apply_boundary_conditions! (Discretise_5_apply_bcs.jl:5-123) for the
@@ -316,6 +321,9 @@ typedef struct {
   int32_t n_bcs;
   int32_t grad_p;       /* XCAL_GRAD_* of grad!(∇p) (schemes.p.gradient)   */
   int32_t grad_U;       /* XCAL_GRAD_* of grad!(∇U) (schemes.U.gradient)   */
+  int32_t time_U;       /* XCAL_TIME_STEADY or _EULER: Time{schemes.U.time}(U)
+                           of the momentum equation (Solvers_1_SIMPLE.jl:68-74) */
+  double dt;            /* runtime.dt of Time{Euler} (> 0 when time_U is Euler) */
 } xcal_simple_setup;
 enum { XCAL_GRAD_GAUSS = 0, XCAL_GRAD_MIDPOINT = 1 };
 
@@ -352,6 +360,15 @@ int xcal_simple_get_field(const xcal_simple* s, int32_t which, double* host);
  * gets the Krylov iteration counts of the Ux, Uy, Uz and p solves. */
 int xcal_simple_run(xcal_simple* s, int32_t iterations, double* residuals, int32_t* done,
                     int32_t* krylov_iters);
+/* Transient incompressible PISO (Solvers_2_PISO.jl:40-211; run! of a
+ * Transient model, Solvers_3_solver_dispatch.jl:186-190, calls it with
+ * inner_loops = 2): `steps` time steps of setup.dt, each one momentum solve
+ * then `inner_loops` pressure correctors, the last one unrelaxed.  Laminar
+ * models; every step runs (no convergence stop).  residuals, done and
+ * krylov_iters as for xcal_simple_run, per time step (p: the last
+ * corrector's solve). */
+int xcal_piso_run(xcal_simple* s, int32_t steps, int32_t inner_loops, double* residuals,
+                  int32_t* done, int32_t* krylov_iters);
 /* Batch meshes: residuals and krylov_iters laid out [case][field][iteration]
  * (ncase*4*iterations); the run stops when every case has converged.
  * xcal_simple_run on a batch reports the largest value over the cases. */

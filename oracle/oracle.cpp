@@ -1162,10 +1162,17 @@ static bool komega_step(const orc_mesh& m, const xcal_simple_setup& S,
   return rk < T.k.convergence && rw < T.omega.convergence;
 }
 
-int orc_simple_run_turb(const orc_mesh* mp, const xcal_simple_setup* S,
-                        const xcal_turbulence_setup* T, double* Ux, double* Uy, double* Uz,
-                        double* p, double* kf_, double* wf_, double* nut_, int32_t iterations,
-                        double* residuals, int32_t* done, int32_t* kit, double* mdot_out) {
+}  // extern "C"
+
+namespace {
+
+// The SIMPLE loop (inner == 0; Solvers_1_SIMPLE.jl:103-263) or PISO's (inner
+// >= 1 pressure correctors per time step, the last unrelaxed, every step
+// taken; Solvers_2_PISO.jl:107-175).
+int run_core(const orc_mesh* mp, const xcal_simple_setup* S, const xcal_turbulence_setup* T,
+             double* Ux, double* Uy, double* Uz, double* p, double* kf_, double* wf_,
+             double* nut_, int32_t iterations, double* residuals, int32_t* done, int32_t* kit,
+             double* mdot_out, int inner) {
   return guard([&] {
     const orc_mesh& m = *mp;
     const int64_t n = m.n, nf = m.nf, nnz = m.rowptr[n];
@@ -1179,7 +1186,13 @@ int orc_simple_run_turb(const orc_mesh* mp, const xcal_simple_setup* S,
     std::vector<std::vector<double>> gradU(9, std::vector<double>(n));
     double* gU[9];
     for (int k = 0; k < 9; ++k) gU[k] = gradU[k].data();
-    orc_term Uterms[3] = {{XCAL_TIME_STEADY, 1, nullptr, 1.0},
+    if (S->time_U != XCAL_TIME_STEADY && S->time_U != XCAL_TIME_EULER)
+      throw std::runtime_error("momentum time scheme: SteadyState or Euler");
+    if (S->time_U == XCAL_TIME_EULER && !(S->dt > 0.0))
+      throw std::runtime_error("Time{Euler} needs runtime.dt > 0");
+    const double dt = S->time_U == XCAL_TIME_EULER ? S->dt : 1.0;
+    const int ncorr = inner > 0 ? inner : 1;
+    orc_term Uterms[3] = {{S->time_U, 1, nullptr, 1.0},
                           {S->div_scheme, 1, mdotf.data(), 0.0},
                           {XCAL_LAPLACIAN_LINEAR, -1, nueff.data(), 0.0}};
     orc_source Usrc[1] = {{gpx.data(), gpy.data(), gpz.data(), -1}};
@@ -1224,7 +1237,7 @@ int orc_simple_run_turb(const orc_mesh* mp, const xcal_simple_setup* S,
     double* bU[3] = {bx.data(), by.data(), bz.data()};
     for (it = 0; it < iterations; ++it) {
       // solve_equation! vector (Solve_1_api.jl:233-266)
-      chk(orc_discretise_vector(&m, Uterms, 3, Usrc, 1, Ux, Uy, Uz, 1.0, A0.data(), bx.data(),
+      chk(orc_discretise_vector(&m, Uterms, 3, Usrc, 1, Ux, Uy, Uz, dt, A0.data(), bx.data(),
                                 by.data(), bz.data()));
       double res[3] = {0.0, 0.0, 0.0};
       int itc[3] = {0, 0, 0};
@@ -1247,29 +1260,34 @@ int orc_simple_run_turb(const orc_mesh* mp, const xcal_simple_setup* S,
         by[i] -= (-1.0 * gpy[i]) * V;
         bz[i] -= (-1.0 * gpz[i]) * V;
       }
-      orc_H(&m, A.data(), bx.data(), by.data(), bz.data(), Ux, Uy, Uz, Hx.data(), Hy.data(),
-            Hz.data());
-      interp_vec(Hx.data(), Hy.data(), Hz.data());
-      orc_flux(&m, Ufx.data(), Ufy.data(), Ufz.data(), mdotf.data());
-      orc_div(&m, mdotf.data(), divHv.data());
-      // pressure equation (Solve_1_api.jl:214-231)
-      std::copy(p, p + n, prev.begin());
-      chk(orc_discretise_scalar(&m, pterms, 1, psrc, 1, p, 1.0, pA.data(), pb.data()));
-      chk(orc_apply_bcs(&m, pterms, 1, S->bc_p, nb, -1, p, nullptr, nullptr, pA.data(), pb.data()));
-      if (S->has_pref) orc_set_reference(&m, pA.data(), pb.data(), S->pref, 0);
+      double rp = 0.0;
       int32_t pit = 0;
-      chk(orc_krylov(&m, S->p.solver, S->p.precon, pA.data(), pb.data(), p, S->p.itmax,
-                     S->p.atol, S->p.rtol, &pit));
-      double rp = orc_residual(&m, pA.data(), pb.data(), p);
-      for (int64_t i = 0; i < n; ++i)  // explicit_relaxation!
-        p[i] = prev[i] + S->p.relax * (p[i] - prev[i]);
-      grad_p();
-      chk(orc_correct_mass_flux(&m, pA.data(), p, rDf.data(), -1, S->bc_p, nb, mdotf.data()));
-      for (int64_t i = 0; i < n; ++i) {  // correct_velocity!
-        double r = rD[i];
-        Ux[i] = Hx[i] - gpx[i] * r;
-        Uy[i] = Hy[i] - gpy[i] * r;
-        Uz[i] = Hz[i] - gpz[i] * r;
+      for (int ic = 0; ic < ncorr; ++ic) {  // PISO's corrector loop (Solvers_2_PISO.jl:123-175)
+        orc_H(&m, A.data(), bx.data(), by.data(), bz.data(), Ux, Uy, Uz, Hx.data(), Hy.data(),
+              Hz.data());
+        interp_vec(Hx.data(), Hy.data(), Hz.data());
+        orc_flux(&m, Ufx.data(), Ufy.data(), Ufz.data(), mdotf.data());
+        orc_div(&m, mdotf.data(), divHv.data());
+        // pressure equation (Solve_1_api.jl:214-231)
+        std::copy(p, p + n, prev.begin());
+        chk(orc_discretise_scalar(&m, pterms, 1, psrc, 1, p, 1.0, pA.data(), pb.data()));
+        chk(orc_apply_bcs(&m, pterms, 1, S->bc_p, nb, -1, p, nullptr, nullptr, pA.data(),
+                          pb.data()));
+        if (S->has_pref) orc_set_reference(&m, pA.data(), pb.data(), S->pref, 0);
+        chk(orc_krylov(&m, S->p.solver, S->p.precon, pA.data(), pb.data(), p, S->p.itmax,
+                       S->p.atol, S->p.rtol, &pit));
+        rp = orc_residual(&m, pA.data(), pb.data(), p);
+        // explicit_relaxation!: PISO's last corrector uses 1.0 (:141-145)
+        const double prelax = inner > 0 && ic == ncorr - 1 ? 1.0 : S->p.relax;
+        for (int64_t i = 0; i < n; ++i) p[i] = prev[i] + prelax * (p[i] - prev[i]);
+        grad_p();
+        chk(orc_correct_mass_flux(&m, pA.data(), p, rDf.data(), -1, S->bc_p, nb, mdotf.data()));
+        for (int64_t i = 0; i < n; ++i) {  // correct_velocity!
+          double r = rD[i];
+          Ux[i] = Hx[i] - gpx[i] * r;
+          Uy[i] = Hy[i] - gpy[i] * r;
+          Uz[i] = Hz[i] - gpz[i] * r;
+        }
       }
       // turbulence!: grad!(gradU) (RANS_laminar.jl:73-79, RANS_kOmega.jl:192)
       if (S->grad_U == XCAL_GRAD_MIDPOINT) {
@@ -1297,7 +1315,7 @@ int orc_simple_run_turb(const orc_mesh* mp, const xcal_simple_setup* S,
         kit[3 * iterations + it] = pit;
       }
       bool zc = is3d ? res[2] <= S->U.convergence : true;
-      if (res[0] <= S->U.convergence && res[1] <= S->U.convergence && zc &&
+      if (inner == 0 && res[0] <= S->U.convergence && res[1] <= S->U.convergence && zc &&
           rp <= S->p.convergence && turb_conv) {
         ++it;
         break;
@@ -1309,11 +1327,34 @@ int orc_simple_run_turb(const orc_mesh* mp, const xcal_simple_setup* S,
   });
 }
 
+}  // namespace
+
+extern "C" {
+
+int orc_simple_run_turb(const orc_mesh* mp, const xcal_simple_setup* S,
+                        const xcal_turbulence_setup* T, double* Ux, double* Uy, double* Uz,
+                        double* p, double* kf_, double* wf_, double* nut_, int32_t iterations,
+                        double* residuals, int32_t* done, int32_t* kit, double* mdot_out) {
+  return run_core(mp, S, T, Ux, Uy, Uz, p, kf_, wf_, nut_, iterations, residuals, done, kit,
+                  mdot_out, 0);
 }
 
 int orc_simple_run(const orc_mesh* mp, const xcal_simple_setup* S, double* Ux, double* Uy,
                    double* Uz, double* p, int32_t iterations, double* residuals, int32_t* done,
                    int32_t* kit, double* mdot_out) {
-  return orc_simple_run_turb(mp, S, nullptr, Ux, Uy, Uz, p, nullptr, nullptr, nullptr, iterations,
-                             residuals, done, kit, mdot_out);
+  return run_core(mp, S, nullptr, Ux, Uy, Uz, p, nullptr, nullptr, nullptr, iterations,
+                  residuals, done, kit, mdot_out, 0);
+}
+
+int orc_piso_run(const orc_mesh* mp, const xcal_simple_setup* S, double* Ux, double* Uy,
+                 double* Uz, double* p, int32_t steps, int32_t inner_loops, double* residuals,
+                 int32_t* done, int32_t* kit, double* mdot_out) {
+  if (inner_loops < 1) {
+    g_err = "PISO needs at least one pressure corrector";
+    return -1;
+  }
+  return run_core(mp, S, nullptr, Ux, Uy, Uz, p, nullptr, nullptr, nullptr, steps, residuals,
+                  done, kit, mdot_out, inner_loops);
+}
+
 }  // extern "C"

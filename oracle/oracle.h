@@ -125,6 +125,12 @@ int orc_correct_mass_flux(const orc_mesh* m, const double* p_nzval, const double
 int orc_simple_run(const orc_mesh* m, const xcal_simple_setup* setup, double* Ux, double* Uy,
                    double* Uz, double* p, int32_t iterations, double* residuals,
                    int32_t* done, int32_t* krylov_iters, double* mdotf);
+/* Transient PISO run (Solvers_2_PISO.jl:40-211, laminar): `steps` time
+ * steps of setup->dt with `inner_loops` pressure correctors each, the last
+ * one unrelaxed; every step is taken.  Arguments as for orc_simple_run. */
+int orc_piso_run(const orc_mesh* m, const xcal_simple_setup* setup, double* Ux, double* Uy,
+                 double* Uz, double* p, int32_t steps, int32_t inner_loops, double* residuals,
+                 int32_t* done, int32_t* krylov_iters, double* mdotf);
 /* the same with the k-omega model (RANS_kOmega.jl:175-235) when t is given;
  * k, omega, nut (n_cells each) are in/out */
 int orc_simple_run_turb(const orc_mesh* m, const xcal_simple_setup* setup,

@@ -104,6 +104,8 @@ def lib():
                                      _pi32, _pi32, _pd]
         L.orc_simple_run_turb.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, _pd, _pd, _pd, _pd,
                                           _pd, _pd, _pd, C.c_int32, _pd, _pi32, _pi32, _pd]
+        L.orc_piso_run.argtypes = [C.c_void_p, C.c_void_p, _pd, _pd, _pd, _pd, C.c_int32,
+                                   C.c_int32, _pd, _pi32, _pi32, _pd]
         L.orc_set_threads.argtypes = [C.c_int32]
         _lib = L
     return _lib
@@ -289,6 +291,23 @@ class OracleMesh:
     def correct_mass_flux(self, p_nzval, p, pflux, psign, bcs, mdot):
         _chk(lib().orc_correct_mass_flux(self.h, _p(p_nzval), _p(p), _p(pflux), psign, bcs,
                                          len(bcs), _p(mdot)))
+
+    def piso_run(self, setup_c, U, p, steps, inner_loops=2):
+        """Transient PISO (Solvers_2_PISO.jl:40-211): `steps` time steps of
+        setup_c.dt with `inner_loops` pressure correctors each."""
+        Ux, Uy, Uz = (np.array(U[i], dtype=np.float64, copy=True) for i in range(3))
+        pp = np.array(p, dtype=np.float64, copy=True)
+        res = np.zeros(4 * steps)
+        kit = np.zeros(4 * steps, np.int32)
+        done = C.c_int32()
+        mdot = np.zeros(self.mesh.n_faces)
+        _chk(lib().orc_piso_run(self.h, C.byref(setup_c), _p(Ux), _p(Uy), _p(Uz), _p(pp), steps,
+                                inner_loops, _p(res), C.byref(done), kit.ctypes.data_as(_pi32),
+                                _p(mdot)))
+        d = done.value
+        return dict(U=np.stack([Ux, Uy, Uz]), p=pp, mdotf=mdot, done=d,
+                    residuals=res.reshape(4, steps)[:, :d],
+                    krylov_iters=kit.reshape(4, steps)[:, :d])
 
     def simple_run(self, setup_c, U, p, iterations, turb=None, k=None, omega=None, nut=None):
         """setup_c: _abi.SimpleSetupC; U: (3, n) array (modified copy returned).

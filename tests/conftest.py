@@ -113,6 +113,41 @@ def cavity_case(api, mesh, nu=0.01, iterations=200, precon="DILU", rtol_p=0.1,
     return model, config
 
 
+def transient_bfs_case(api, mesh, steps, dt=0.005, precon="Jacobi", rtol=1e-3):
+    """The reference's test/0_TEST_CASES/2d_incompressible_transient_laminar_BFS.jl
+    (U 0.5, nu 1e-3, Euler, dt 0.005, Jacobi, relax 1, rtol 1e-3), with CG in
+    place of GMRES for the pressure."""
+    P = getattr(api, precon)
+    u = 0.5
+    model = api.Physics(domain=mesh, fluid=api.Fluid(nu=1e-3), time=api.Transient())
+    BCs = api.assign(region=mesh,
+                     U=[api.Dirichlet("inlet", [u, 0, 0]), api.Extrapolated("outlet"),
+                        api.Wall("wall", [0, 0, 0]), api.Dirichlet("top", [0, 0, 0])],
+                     p=[api.Extrapolated("inlet"), api.Dirichlet("outlet", 0.0),
+                        api.Extrapolated("wall"), api.Extrapolated("top")])
+
+    def S(s):
+        return api.SolverSetup(solver=s, preconditioner=P(), convergence=1e-7, relax=1.0,
+                               rtol=rtol)
+    config = api.Configuration(
+        solvers=dict(U=S(api.Bicgstab()), p=S(api.Cg())),
+        schemes=dict(U=api.Schemes(time=api.Euler), p=api.Schemes()),
+        runtime=api.Runtime(iterations=steps, write_interval=-1, time_step=dt),
+        hardware=api.Hardware(backend=api.HIPBackend(0)), boundaries=BCs)
+    api.initialise(model.momentum.U, [u, 0, 0])
+    return model, config
+
+
+def patch_face_average(mesh, U, patch):
+    """boundary_average(patch, U, BCs.U, config) (Postprocess_functions.jl:157-181)
+    for a zero-gradient patch (Extrapolated): face values are the owner
+    cells' values, averaged without weights."""
+    k = mesh.patch_index(patch)
+    f0, f1 = mesh.patch_range[2 * k] - 1, mesh.patch_range[2 * k + 1]
+    cells = mesh.boundary_cellsID[f0:f1] - 1
+    return np.asarray(U)[:, cells].mean(axis=1)
+
+
 def channel_case(api, mesh, nu=1e-3, iterations=10, precon="DILU", rtol_p=1e-10,
                  div=None, precision_p="fp64"):
     P = getattr(api, precon)

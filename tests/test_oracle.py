@@ -7,7 +7,8 @@ import json
 import numpy as np
 import pytest
 
-from conftest import GOLDEN, cavity_case, channel_case, rel
+from conftest import (GOLDEN, cavity_case, channel_case, patch_face_average, rel,
+                      transient_bfs_case)
 
 
 def dense(om, nz):
@@ -314,3 +315,75 @@ def test_midpoint_gradient_exact_for_linear_fields(A, O, meshes):
     for p, s in enumerate((1.0, 2.0, -1.0)):
         assert np.allclose(gv[3 * p][interior], s * a[0], atol=1e-12)
         assert np.allclose(gv[3 * p + 1][interior], s * a[1], atol=1e-12)
+
+
+@pytest.mark.parametrize("name", ["cavity16", "box12"])
+def test_euler_time_term_closed_form(A, O, meshes, name):
+    """Time{Euler} (Discretise_1_schemes.jl:25-42) adds V/dt to the diagonal
+    and prev V/dt to b, and nothing off the diagonal: the assembly with it
+    minus the assembly with Time{SteadyState} is exactly that, per
+    component of a vector equation."""
+    m = meshes(name)
+    om = O.OracleMesh(m, A)
+    rng = np.random.default_rng(5)
+    n, nf = m.n_cells, m.n_faces
+    mdot, gam = rng.standard_normal(nf) * m.face_area, 0.5 + rng.random(nf)
+    prev = rng.standard_normal((3, n))
+    src = rng.standard_normal((3, n))
+    dt = 0.0137
+    out = {}
+    for kind in (A.TIME_STEADY, A.TIME_EULER):
+        terms = [(kind, 1, 1.0), (A.DIV_UPWIND, 1, mdot), (A.LAPLACIAN_LINEAR, -1, gam)]
+        out[kind] = om.discretise_vector(terms, [([src[0], src[1], src[2]], -1)], prev, dt)
+    (nzS, bS), (nzE, bE) = out[A.TIME_STEADY], out[A.TIME_EULER]
+    rp, cv = om.csr()
+    diag = np.array([rp[i] + np.searchsorted(cv[rp[i]:rp[i + 1]], i) for i in range(n)])
+    off = np.ones(len(nzS), bool)
+    off[diag] = False
+    assert np.array_equal(nzS[off], nzE[off])
+    V = m.cell_volume
+    assert np.allclose(nzE[diag] - nzS[diag], V / dt, rtol=1e-12, atol=0)
+    for c in range(3):
+        assert np.allclose(bE[c] - bS[c], prev[c] * V / dt, rtol=1e-9, atol=1e-12 * np.abs(bE[c]).max())
+
+
+def test_piso_single_corrector_is_simple_with_time_term(x, A, O, meshes):
+    """One PISO time step with one corrector (Solvers_2_PISO.jl:107-175) is
+    one SIMPLE iteration (Solvers_1_SIMPLE.jl:164-258) of the same
+    Time{Euler} momentum equation with the pressure unrelaxed: the same
+    calls in the same order.  So the two loops agree bit for bit, step by
+    step, which pins the PISO loop to the SIMPLE loop the physical checks
+    above pin."""
+    api = x.api
+    m = meshes("cavity16")
+    model, config = cavity_case(api, m, iterations=12)
+    config.schemes["U"] = api.Schemes(divergence=api.Upwind, time=api.Euler)
+    config.runtime.time_step = 0.05
+    config.solvers["p"].relax = 1.0
+    for s in config.solvers.values():
+        s.convergence = 0.0  # SIMPLE takes every iteration, as PISO does
+    setup = api.simple_setup(model, config)
+    om = O.OracleMesh(m, A)
+    a = om.simple_run(setup, model.momentum.U, model.momentum.p, 12)
+    b = om.piso_run(setup, model.momentum.U, model.momentum.p, 12, inner_loops=1)
+    assert a["done"] == b["done"] == 12
+    for k in ("U", "p", "mdotf", "residuals", "krylov_iters"):
+        assert np.array_equal(a[k], b[k]), k
+    c = om.piso_run(setup, model.momentum.U, model.momentum.p, 12, inner_loops=2)
+    assert not np.array_equal(a["p"], c["p"])  # the second corrector does work
+
+
+def test_piso_transient_bfs_reference_check(x, A, O, meshes):
+    """The reference's transient laminar BFS test
+    (2d_incompressible_transient_laminar_BFS.jl): after 1000 PISO steps the
+    outlet's average velocity is half the inlet's, within 0.1 U."""
+    api = x.api
+    m = meshes("bfs10")
+    model, config = transient_bfs_case(api, m, 1000)
+    setup = api.simple_setup(model, config)
+    out = O.OracleMesh(m, A).piso_run(setup, model.momentum.U, model.momentum.p, 1000, 2)
+    assert out["done"] == 1000
+    inlet = np.array([0.5, 0.0, 0.0])
+    outlet = patch_face_average(m, out["U"], "outlet")
+    assert np.linalg.norm(outlet - 0.5 * inlet) <= 0.1 * 0.5, outlet
+    assert np.all(np.isfinite(out["U"])) and np.all(np.isfinite(out["p"]))

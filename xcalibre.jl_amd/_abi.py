@@ -74,7 +74,8 @@ class SimpleSetupC(C.Structure):
     _fields_ = [("nu", C.c_double), ("div_scheme", C.c_int32), ("has_pref", C.c_int32),
                 ("pref", C.c_double), ("U", SolverSetupC), ("p", SolverSetupC),
                 ("bc_U", C.POINTER(BC)), ("bc_p", C.POINTER(BC)), ("n_bcs", C.c_int32),
-                ("grad_p", C.c_int32), ("grad_U", C.c_int32)]
+                ("grad_p", C.c_int32), ("grad_U", C.c_int32), ("time_U", C.c_int32),
+                ("dt", C.c_double)]
 
 
 GRAD_GAUSS, GRAD_MIDPOINT = 0, 1

@@ -140,6 +140,7 @@ class Configuration:
 
 # ------------------------------------------------------------- physics
 class Steady: ...
+class Transient: ...  # run! dispatches to PISO (Solvers_3_solver_dispatch.jl:186-190)
 class Incompressible: ...
 class Laminar: ...
 class Isothermal: ...
@@ -265,9 +266,9 @@ def simple_setup(model: Physics, config: Configuration, pref=None) -> A.SimpleSe
     if not isinstance(model.turbulence, (Laminar, KOmega)):
         raise NotImplementedError("laminar and k-omega SIMPLE are on the device path")
     sU, sp = config.schemes["U"], config.schemes["p"]
+    if sU.time not in TIME:
+        raise NotImplementedError(f"time scheme {sU.time!r}")
     for s in (sU, sp):
-        if s.time is not SteadyState:
-            raise NotImplementedError("SIMPLE is a steady solver (Time{SteadyState})")
         if s.gradient not in _GRAD or s.limiter is not None or s.laplacian is not Linear:
             raise NotImplementedError("device path: Gauss or Midpoint gradient, Linear "
                                       "laplacian, no limiter")
@@ -279,7 +280,7 @@ def simple_setup(model: Physics, config: Configuration, pref=None) -> A.SimpleSe
     setup = A.SimpleSetupC(float(model.fluid.nu), div, int(pref is not None),
                            float(pref or 0.0), config.solvers["U"].c(), config.solvers["p"].c(),
                            bU, bp, len(config.boundaries["U"]), _GRAD[sp.gradient],
-                           _GRAD[sU.gradient])
+                           _GRAD[sU.gradient], TIME[sU.time], float(config.runtime.time_step))
     setup._keep = (bU, bp)
     return setup
 
@@ -314,9 +315,12 @@ def _context(device: int) -> hip.Context:
 
 def run(model: Physics, config: Configuration, pref=None, ncorrectors: int = 0,
         inner_loops: int = 0):
-    """run!(model, config; pref, ncorrectors) for steady incompressible
-    laminar SIMPLE.  Returns the residual histories (Ux, Uy, Uz, p) and
-    leaves the converged fields in model.momentum."""
+    """run!(model, config; pref, ncorrectors) for incompressible flow:
+    SIMPLE for a Steady model (laminar or k-omega), PISO with 2 pressure
+    correctors per time step for a Transient one (laminar), as run! calls
+    piso!(model, config; pref) (Solvers_3_solver_dispatch.jl:186-190).
+    Returns the residual histories (Ux, Uy, Uz, p) and leaves the final
+    fields in model.momentum."""
     if ncorrectors != 0:
         raise NotImplementedError("non-orthogonal correctors are not on the device path")
     if not isinstance(config.hardware.backend, HIPBackend):
@@ -333,7 +337,12 @@ def run(model: Physics, config: Configuration, pref=None, ncorrectors: int = 0,
         sim.set_scalar("k", t.k)
         sim.set_scalar("omega", t.omega)
         sim.set_scalar("nut", t.nut)
-    out = sim.run(config.runtime.iterations)
+    if isinstance(model.time, Transient):
+        if ts is not None:
+            raise NotImplementedError("transient k-omega is not on the device path")
+        out = sim.run_piso(config.runtime.iterations, 2)
+    else:
+        out = sim.run(config.runtime.iterations)
     model.momentum.U[:] = sim.get("U")
     model.momentum.p[:] = sim.get("p")
     if ts is not None:

@@ -245,8 +245,25 @@ int xcal_discretise(xcal_eqn* e, const xcal_term* terms, int32_t nterms, const x
     for (int i = 0; i < nterms; ++i) needs_prev |= terms[i].type == XCAL_TIME_EULER;
     if (prev) check_field(prev, e->mesh, e->vector ? XCAL_CELL_VECTOR : XCAL_CELL_SCALAR, "prev");
     XCAL_CHECK(!needs_prev || prev, "Time{Euler} needs the previous field");
-    XCAL_CHECK(!needs_prev, "Time{Euler} needs runtime.dt: use the transient entry points");
+    XCAL_CHECK(!needs_prev, "Time{Euler} needs runtime.dt: use xcal_discretise_dt");
     discretise(e, t, s, prev, 1.0);
+    return 0;
+  });
+}
+
+int xcal_discretise_dt(xcal_eqn* e, const xcal_term* terms, int32_t nterms, const xcal_source* src,
+                       int32_t nsrc, const xcal_field* prev, double dt) {
+  return guarded([&] {
+    XCAL_CHECK(e, "equation is NULL");
+    XCAL_CHECK(dt > 0.0, "dt must be positive");
+    check_terms(e, terms, nterms);
+    TermsK t = make_terms(terms, nterms);
+    SourcesK s = make_sources(e, src, nsrc);
+    bool needs_prev = false;
+    for (int i = 0; i < nterms; ++i) needs_prev |= terms[i].type == XCAL_TIME_EULER;
+    if (prev) check_field(prev, e->mesh, e->vector ? XCAL_CELL_VECTOR : XCAL_CELL_SCALAR, "prev");
+    XCAL_CHECK(!needs_prev || prev, "Time{Euler} needs the previous field");
+    discretise(e, t, s, prev, dt);
     return 0;
   });
 }

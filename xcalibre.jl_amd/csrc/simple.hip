@@ -1,9 +1,11 @@
-// Steady incompressible SIMPLE driver on the device: the loop of
+// Incompressible pressure-velocity drivers on the device: the steady loop of
 // SIMPLE (Solvers_1_SIMPLE.jl:103-263) behind simple!/run!
-// (Solvers_1_SIMPLE.jl:28-42, Solvers_3_solver_dispatch.jl:120-135) with a
-// laminar viscosity model (RANS_laminar.jl), Gauss gradients and no
-// non-orthogonal correctors.  All fields stay resident in HBM; the host only
-// sequences launches and reads back residual scalars.
+// (Solvers_1_SIMPLE.jl:28-42, Solvers_3_solver_dispatch.jl:120-135), and the
+// transient loop of PISO (Solvers_2_PISO.jl:40-211) behind run! of a
+// Transient model (Solvers_3_solver_dispatch.jl:186-190).  Laminar or
+// k-omega (SIMPLE) viscosity, Gauss or Midpoint gradients, no non-orthogonal
+// correctors.  All fields stay resident in HBM; the host only sequences
+// launches and reads back residual scalars.
 #include <algorithm>
 #include <cstring>
 #include <vector>
@@ -105,6 +107,9 @@ void check_setup(const xcal_mesh* m, const xcal_simple_setup* s) {
     XCAL_CHECK(s->bc_U[i].kind != XCAL_BC_NEUMANN, "Neumann is not defined for vector fields");
   for (int g : {s->grad_p, s->grad_U})
     XCAL_CHECK(g == XCAL_GRAD_GAUSS || g == XCAL_GRAD_MIDPOINT, "gradient scheme: Gauss or Midpoint");
+  XCAL_CHECK(s->time_U == XCAL_TIME_STEADY || s->time_U == XCAL_TIME_EULER,
+             "momentum time scheme: SteadyState or Euler");
+  XCAL_CHECK(s->time_U != XCAL_TIME_EULER || s->dt > 0.0, "Time{Euler} needs runtime.dt > 0");
 }
 
 // grad!(∇p, pf, p, BCs.p) (Calculate_0_gradient.jl:105-109)
@@ -282,10 +287,13 @@ bool komega_step(xcal_simple* S) {
   return S->res_k < T.k.convergence && S->res_w < T.omega.convergence;
 }
 
-// The SIMPLE loop.  R[(k*4 + f)*iterations + it]: residual of field f
-// (Ux, Uy, Uz, p) of case k at iteration it; KIT likewise for Krylov
-// iterations.  A batch stops when every case has converged.
-int run_impl(xcal_simple* S, int32_t iterations, std::vector<double>& R, std::vector<int32_t>& KIT) {
+// The SIMPLE loop (inner == 0), or PISO's (inner >= 1 pressure correctors
+// per time step, Solvers_2_PISO.jl:107-211).  R[(k*4 + f)*iterations + it]:
+// residual of field f (Ux, Uy, Uz, p) of case k at iteration it; KIT
+// likewise for Krylov iterations.  SIMPLE stops when every case has
+// converged; PISO runs every step.
+int run_impl(xcal_simple* S, int32_t iterations, std::vector<double>& R, std::vector<int32_t>& KIT,
+             int inner) {
   {
     xcal_mesh* m = S->mesh;
     const int K = m->ncase;
@@ -301,7 +309,7 @@ int run_impl(xcal_simple* S, int32_t iterations, std::vector<double>& R, std::ve
     const int64_t n = m->n;
     XCAL_HIP(hipSetDevice(m->ctx->device));
     // operator definitions (Solvers_1_SIMPLE.jl:68-78)
-    xcal_term Ut[3] = {{XCAL_TIME_STEADY, 1, nullptr, 1.0},
+    xcal_term Ut[3] = {{cfg.time_U, 1, nullptr, 1.0},
                        {cfg.div_scheme, 1, S->mdotf, 0.0},
                        {XCAL_LAPLACIAN_LINEAR, -1, S->nueff, 0.0}};
     xcal_term pt[1] = {{XCAL_LAPLACIAN_LINEAR, -1, S->rDf, 0.0}};
@@ -319,6 +327,8 @@ int run_impl(xcal_simple* S, int32_t iterations, std::vector<double>& R, std::ve
     xcal_eqn* Ue = S->Ueqn;
     xcal_eqn* pe = S->peqn;
     const bool is3d = m->dim == 3;
+    const double dt = cfg.time_U == XCAL_TIME_EULER ? cfg.dt : 1.0;
+    const int ncorr = inner > 0 ? inner : 1;
     std::memset(S->stage_ms, 0, sizeof(S->stage_ms));
 
     // initial calculations (Solvers_1_SIMPLE.jl:150-156)
@@ -336,7 +346,7 @@ int run_impl(xcal_simple* S, int32_t iterations, std::vector<double>& R, std::ve
     for (it = 0; it < iterations; ++it) {
       // solve_equation! vector (Solve_1_api.jl:233-266)
       mark(0);
-      discretise(Ue, UT, US, S->U, 1.0);
+      discretise(Ue, UT, US, S->U, dt);
       mark(1);
       float tA = 0.0f, tS = 0.0f;
       for (int c = 0; c < (is3d ? 3 : 2); ++c) {
@@ -365,43 +375,61 @@ int run_impl(xcal_simple* S, int32_t iterations, std::vector<double>& R, std::ve
         acc[0] += a + tA;
         acc[1] += tS;
       }
-      // pressure correction (Solvers_1_SIMPLE.jl:170-185)
+      // pressure correction (Solvers_1_SIMPLE.jl:170-185; PISO :117-121 then
+      // `inner` correctors :123-175)
       mark(0);
       inverse_diagonal(Ue, S->rD->comp(0));
       interpolate(m, S->rD->comp(0), S->rDf->comp(0));
       remove_pressure_source(Ue, S->gradp->comp(0), S->gradp->comp(1), S->gradp->comp(2), -1);
-      Hop(Ue, S->U->comp(0), S->U->comp(1), S->U->comp(2), S->Hv->comp(0), S->Hv->comp(1),
-          S->Hv->comp(2));
-      interp_vec(S, S->Hv);
-      flux(m, S->Uf->comp(0), S->Uf->comp(1), S->Uf->comp(2), S->mdotf->comp(0));
-      divergence(m, S->mdotf->comp(0), S->divHv->comp(0));
-      XCAL_HIP(hipMemcpyAsync(S->prev->comp(0), S->p->comp(0), sizeof(double) * n,
-                              hipMemcpyDeviceToDevice, st));
-      mark(1);
-      // solve_equation! scalar (Solve_1_api.jl:214-231)
-      discretise(pe, PT, PS, S->p, 1.0);
-      apply_bcs(pe, PT, S->bp, S->p, 0);
-      if (cfg.has_pref) {  // reference cell 1 (of every case)
-        if (m->batch)
-          hipLaunchKernelGGL(k_set_reference_cases, dim3(grid_for(K)), dim3(kBlock), 0, st,
-                             m->diag.p, pe->nzval.p, pe->bc(0), cfg.pref, S->ref_cells.p, K);
-        else
-          set_reference(pe, cfg.pref, m->cperm[0]);
+      for (int ic = 0; ic < ncorr; ++ic) {
+        if (ic > 0) mark(0);
+        Hop(Ue, S->U->comp(0), S->U->comp(1), S->U->comp(2), S->Hv->comp(0), S->Hv->comp(1),
+            S->Hv->comp(2));
+        interp_vec(S, S->Hv);
+        flux(m, S->Uf->comp(0), S->Uf->comp(1), S->Uf->comp(2), S->mdotf->comp(0));
+        divergence(m, S->mdotf->comp(0), S->divHv->comp(0));
+        XCAL_HIP(hipMemcpyAsync(S->prev->comp(0), S->p->comp(0), sizeof(double) * n,
+                                hipMemcpyDeviceToDevice, st));
+        mark(1);
+        // solve_equation! scalar (Solve_1_api.jl:214-231)
+        discretise(pe, PT, PS, S->p, 1.0);
+        apply_bcs(pe, PT, S->bp, S->p, 0);
+        if (cfg.has_pref) {  // reference cell 1 (of every case)
+          if (m->batch)
+            hipLaunchKernelGGL(k_set_reference_cases, dim3(grid_for(K)), dim3(kBlock), 0, st,
+                               m->diag.p, pe->nzval.p, pe->bc(0), cfg.pref, S->ref_cells.p, K);
+          else
+            set_reference(pe, cfg.pref, m->cperm[0]);
+        }
+        precond_update(pe, cfg.p.precon);
+        mark(2);
+        solve(pe, cfg.p, S->p->comp(0), pe->bc(0));
+        mark(3);
+        for (int k = 0; k < K; ++k) {
+          R[((size_t)k * 4 + 3) * iterations + it] = cst[k].residual;
+          KIT[((size_t)k * 4 + 3) * iterations + it] = cst[k].niter;
+        }
+        // explicit_relaxation!: PISO's last corrector is unrelaxed
+        const double prelax = inner > 0 && ic == ncorr - 1 ? 1.0 : cfg.p.relax;
+        relax_explicit(m, n, S->p->comp(0), S->prev->comp(0), prelax);
+        grad_p(S);
+        correct_mass_flux(pe, S->p->comp(0), S->rDf->comp(0), -1, S->bp, S->mdotf->comp(0));
+        correct_velocity(m, S->U->comp(0), S->U->comp(1), S->U->comp(2), S->Hv->comp(0),
+                         S->Hv->comp(1), S->Hv->comp(2), S->gradp->comp(0), S->gradp->comp(1),
+                         S->gradp->comp(2), S->rD->comp(0));
+        if (ic < ncorr - 1) {  // stage times of the correctors before the last
+          mark(4);
+          float a, b, c, d;
+          XCAL_HIP(hipEventSynchronize(S->ev[4]));
+          XCAL_HIP(hipEventElapsedTime(&a, S->ev[0], S->ev[1]));
+          XCAL_HIP(hipEventElapsedTime(&b, S->ev[1], S->ev[2]));
+          XCAL_HIP(hipEventElapsedTime(&c, S->ev[2], S->ev[3]));
+          XCAL_HIP(hipEventElapsedTime(&d, S->ev[3], S->ev[4]));
+          acc[4] += a + d;
+          acc[2] += b;
+          acc[3] += c;
+        }
       }
-      precond_update(pe, cfg.p.precon);
-      mark(2);
-      solve(pe, cfg.p, S->p->comp(0), pe->bc(0));
-      mark(3);
-      for (int k = 0; k < K; ++k) {
-        R[((size_t)k * 4 + 3) * iterations + it] = cst[k].residual;
-        KIT[((size_t)k * 4 + 3) * iterations + it] = cst[k].niter;
-      }
-      relax_explicit(m, n, S->p->comp(0), S->prev->comp(0), cfg.p.relax);
-      grad_p(S);
-      correct_mass_flux(pe, S->p->comp(0), S->rDf->comp(0), -1, S->bp, S->mdotf->comp(0));
-      correct_velocity(m, S->U->comp(0), S->U->comp(1), S->U->comp(2), S->Hv->comp(0),
-                       S->Hv->comp(1), S->Hv->comp(2), S->gradp->comp(0), S->gradp->comp(1),
-                       S->gradp->comp(2), S->rD->comp(0));
       // turbulence!: grad!(gradU) (RANS_laminar.jl:73-79, RANS_kOmega.jl:192)
       if (cfg.grad_U == XCAL_GRAD_MIDPOINT) {  // Grad{Midpoint} of U
         const double* ph[3] = {S->U->comp(0), S->U->comp(1), S->U->comp(2)};
@@ -440,6 +468,7 @@ int run_impl(xcal_simple* S, int32_t iterations, std::vector<double>& R, std::ve
         conv = conv && r[it] <= cfg.U.convergence && r[iterations + it] <= cfg.U.convergence &&
                zc && r[3 * iterations + it] <= cfg.p.convergence;
       }
+      if (inner > 0) conv = false;  // a transient run takes every time step
       if (conv) {
         ++it;
         break;
@@ -455,6 +484,24 @@ int run_impl(xcal_simple* S, int32_t iterations, std::vector<double>& R, std::ve
   }
 }
 
+// One case: R holds [Ux..., Uy..., Uz..., p...]; a batch reports the largest
+// residual and Krylov count over its cases.
+void report(const xcal_simple* S, int32_t iterations, const std::vector<double>& R,
+            const std::vector<int32_t>& KIT, double* residuals, int32_t* kit) {
+  const int K = S->mesh->ncase;
+  for (int f = 0; f < 4; ++f)
+    for (int i = 0; i < iterations; ++i) {
+      double r = 0.0;
+      int32_t q = 0;
+      for (int k = 0; k < K; ++k) {
+        r = std::max(r, R[((size_t)k * 4 + f) * iterations + i]);
+        q = std::max(q, KIT[((size_t)k * 4 + f) * iterations + i]);
+      }
+      residuals[(size_t)f * iterations + i] = r;
+      if (kit) kit[(size_t)f * iterations + i] = q;
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -465,21 +512,23 @@ int xcal_simple_run(xcal_simple* S, int32_t iterations, double* residuals, int32
     XCAL_CHECK(S && residuals && done && iterations >= 0, "bad arguments");
     std::vector<double> R;
     std::vector<int32_t> KIT;
-    const int it = run_impl(S, iterations, R, KIT);
-    // one case: R holds [Ux..., Uy..., Uz..., p...]; a batch reports the
-    // largest residual and Krylov count over its cases
-    const int K = S->mesh->ncase;
-    for (int f = 0; f < 4; ++f)
-      for (int i = 0; i < iterations; ++i) {
-        double r = 0.0;
-        int32_t q = 0;
-        for (int k = 0; k < K; ++k) {
-          r = std::max(r, R[((size_t)k * 4 + f) * iterations + i]);
-          q = std::max(q, KIT[((size_t)k * 4 + f) * iterations + i]);
-        }
-        residuals[(size_t)f * iterations + i] = r;
-        if (kit) kit[(size_t)f * iterations + i] = q;
-      }
+    const int it = run_impl(S, iterations, R, KIT, 0);
+    report(S, iterations, R, KIT, residuals, kit);
+    *done = it;
+    return 0;
+  });
+}
+
+int xcal_piso_run(xcal_simple* S, int32_t steps, int32_t inner_loops, double* residuals,
+                  int32_t* done, int32_t* kit) {
+  return guarded([&] {
+    XCAL_CHECK(S && residuals && done && steps >= 0, "bad arguments");
+    XCAL_CHECK(inner_loops >= 1, "PISO needs at least one pressure corrector");
+    XCAL_CHECK(!S->turb, "PISO runs laminar models");
+    std::vector<double> R;
+    std::vector<int32_t> KIT;
+    const int it = run_impl(S, steps, R, KIT, inner_loops);
+    report(S, steps, R, KIT, residuals, kit);
     *done = it;
     return 0;
   });
@@ -491,7 +540,7 @@ int xcal_simple_run_batch(xcal_simple* S, int32_t iterations, double* residuals,
     XCAL_CHECK(S && residuals && done && iterations >= 0, "bad arguments");
     std::vector<double> R;
     std::vector<int32_t> KIT;
-    *done = run_impl(S, iterations, R, KIT);
+    *done = run_impl(S, iterations, R, KIT, 0);
     std::copy(R.begin(), R.end(), residuals);
     if (kit) std::copy(KIT.begin(), KIT.end(), kit);
     return 0;

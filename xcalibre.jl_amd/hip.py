@@ -68,6 +68,7 @@ def lib():
             "xcal_eqn_create": [_vp, C.c_int32, C.POINTER(_vp)],
             "xcal_eqn_destroy": [_vp],
             "xcal_discretise": [_vp, _vp, C.c_int32, _vp, C.c_int32, _vp],
+            "xcal_discretise_dt": [_vp, _vp, C.c_int32, _vp, C.c_int32, _vp, C.c_double],
             "xcal_update_equation": [_vp],
             "xcal_apply_bcs": [_vp, _vp, C.c_int32, _vp, C.c_int32, _vp, C.c_int32],
             "xcal_relax_implicit_diagdom": [_vp, _vp, C.c_int32, C.c_double],
@@ -96,6 +97,7 @@ def lib():
             "xcal_simple_set_field": [_vp, C.c_int32, _pd],
             "xcal_simple_get_field": [_vp, C.c_int32, _pd],
             "xcal_simple_run": [_vp, C.c_int32, _pd, _pi32, _pi32],
+            "xcal_piso_run": [_vp, C.c_int32, C.c_int32, _pd, _pi32, _pi32],
             "xcal_simple_stage_times": [_vp, _pd],
             "xcal_kernel_timing_enable": [_vp, C.c_int32, C.c_int32],
             "xcal_kernel_timing_read": [_vp, C.POINTER(C.c_int64), _pd],
@@ -279,9 +281,15 @@ class Equation:
         self.h = _vp()
         check(lib().xcal_eqn_create(dmesh.h, int(vector), C.byref(self.h)))
 
-    def discretise(self, terms, srcs, prev=None):
-        check(lib().xcal_discretise(self.h, terms_c(terms), len(terms), sources_c(srcs), len(srcs),
-                                    prev.h if prev is not None else None))
+    def discretise(self, terms, srcs, prev=None, dt=None):
+        """discretise!(eqn, prev, config); dt = config.runtime.dt for Time{Euler}."""
+        ph = prev.h if prev is not None else None
+        if dt is None:
+            check(lib().xcal_discretise(self.h, terms_c(terms), len(terms), sources_c(srcs),
+                                        len(srcs), ph))
+        else:
+            check(lib().xcal_discretise_dt(self.h, terms_c(terms), len(terms), sources_c(srcs),
+                                           len(srcs), ph, float(dt)))
 
     def update_equation(self):
         check(lib().xcal_update_equation(self.h))
@@ -373,6 +381,17 @@ class Simple:
         d = done.value
         return dict(done=d, residuals=res.reshape(4, iterations)[:, :d],
                     krylov_iters=kit.reshape(4, iterations)[:, :d])
+
+    def run_piso(self, steps: int, inner_loops: int = 2):
+        """Transient PISO time steps (setup dt); residuals per step."""
+        res = np.zeros(4 * steps)
+        kit = np.zeros(4 * steps, np.int32)
+        done = C.c_int32()
+        check(lib().xcal_piso_run(self.h, steps, inner_loops, _pd_of(res), C.byref(done),
+                                  kit.ctypes.data_as(_pi32)))
+        d = done.value
+        return dict(done=d, residuals=res.reshape(4, steps)[:, :d],
+                    krylov_iters=kit.reshape(4, steps)[:, :d])
 
     def run_batch(self, iterations: int):
         """Batch meshes: residuals and Krylov counts per case, (ncase, 4, done)."""

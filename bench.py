@@ -20,8 +20,9 @@ detail.batched = 256 batched 2D cavities (FP64 and FP32-IR pressure), the
             form that --mode batch reports in full.  --quick skips both.
 
 Other modes: --mode batch (batched cavities), --mode serve (k-omega serving
-with Poisson arrivals: p50/p99 time to converged), --mode calib (a plain
-256 MiB copy for HBM counter calibration).
+with Poisson arrivals: p50/p99 time to converged), --mode piso (the channel as
+a transient PISO case: cell-steps/s), --mode calib (a plain 256 MiB copy for
+HBM counter calibration).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--quick]
 """
@@ -323,6 +324,81 @@ def channel(api, mesh, iterations, precision="fp64", inner_rtol=1e-6):
     return model, config
 
 
+def transient_channel(api, mesh, steps, dt):
+    """The channel above as a transient PISO case: Time{Euler} with step dt,
+    U BiCGSTAB + DILU (rtol 1e-3, unrelaxed), p CG + DILU (rtol 1e-6), two
+    pressure correctors per step."""
+    model, config = channel(api, mesh, steps)
+    model.time = api.Transient()
+    config.schemes["U"] = api.Schemes(divergence=api.Upwind, time=api.Euler)
+    config.runtime.time_step = dt
+    config.solvers["U"] = api.SolverSetup(solver=api.Bicgstab(), preconditioner=api.DILU(),
+                                          convergence=1e-7, relax=1.0, rtol=1e-3)
+    config.solvers["p"] = api.SolverSetup(solver=api.Cg(), preconditioner=api.DILU(),
+                                          convergence=1e-7, relax=1.0, rtol=1e-6)
+    return model, config
+
+
+def run_piso(args, x):
+    """Transient PISO on the 1M-cell channel: cell-steps per second, with the
+    CG SpMV roofline as in the channel line."""
+    api, H = x.api, x.hip
+    N = args.cells
+    t0 = time.perf_counter()
+    mesh = x.mesh.box3d(N, N, N, jitter=0.1, seed_jitter=SEED_JITTER, seed_perm=SEED_PERM)
+    t_mesh = time.perf_counter() - t0
+    dt = 0.005
+    model, config = transient_channel(api, mesh, args.warmup + args.steps, dt)
+    setup = api.simple_setup(model, config)
+    ctx = H.Context(0)
+    t0 = time.perf_counter()
+    dm = H.DeviceMesh(ctx, mesh)
+    sim = H.Simple(dm, setup)
+    sim.set_U(model.momentum.U)
+    sim.set_p(model.momentum.p)
+    ctx.synchronize()
+    t_upload = time.perf_counter() - t0
+    if args.warmup:
+        sim.run_piso(args.warmup, 2)
+    ctx.time_kernel(args.roofline_kernel, True)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    out = sim.run_piso(args.steps, 2)
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    launches, kms = ctx.kernel_time()
+    ctx.time_kernel(args.roofline_kernel, False)
+    wall = t1 - t0
+    n = mesh.n_cells
+    nbytes = kernel_bytes(args.roofline_kernel, n, dm.nnz)
+    avg_s = kms / 1e3 / max(launches, 1)
+    achieved = nbytes / avg_s / 1e9
+    kit = out["krylov_iters"]
+    line = {
+        "metric": "PISO cell-steps/s (transient channel, 2 pressure correctors per step)",
+        "value": round(n * args.steps / wall, 1), "unit": "cell-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(1e3 * wall / args.steps, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded jittered, renumbered hex box)",
+        "config": {"workload": f"3D laminar channel PISO FP64, {N}x{N}x{N} jittered+renumbered "
+                               f"hex box ({n} cells), Euler dt {dt}",
+                   "cells": n, "nnz": dm.nnz, "inner_loops": 2,
+                   "p_solver": "CG+DILU rtol 1e-6", "U_solver": "BiCGSTAB+DILU rtol 1e-3",
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": hbm_traffic(args.roofline_kernel, f"channel_{N}cubed_fp64"),
+                     "kernel": args.roofline_kernel, "bytes_per_launch": nbytes,
+                     "timed_launches": launches, "avg_us": round(avg_s * 1e6, 2)},
+        "cpu_baseline": None,
+        "detail": {"mesh_build_s": round(t_mesh, 2), "upload_s": round(t_upload, 2),
+                   "p_cg_iters_mean_last_corrector": float(np.mean(kit[3])),
+                   "U_bicgstab_iters_mean": float(np.mean(kit[0]))},
+    }
+    print(json.dumps(line))
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -389,9 +465,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--quick", action="store_true",
                     help="channel mode: skip the mixed-precision and batched summaries")
-    ap.add_argument("--mode", default="channel", choices=["channel", "batch", "serve", "calib"],
+    ap.add_argument("--mode", default="channel",
+                    choices=["channel", "batch", "serve", "piso", "calib"],
                     help="channel: the 1M-cell FP64 channel; batch: batched cavities; "
-                         "serve: Poisson-arrival serving of step cases")
+                         "serve: Poisson-arrival serving of step cases; piso: the channel "
+                         "as a transient PISO case")
     ap.add_argument("--requests", type=int, default=16, help="serve mode: number of requests")
     ap.add_argument("--rate", type=float, default=2.0, help="serve mode: arrivals per second")
     ap.add_argument("--slots", type=int, default=4, help="serve mode: concurrent slots")
@@ -423,6 +501,9 @@ def main():
         return
     if args.mode == "calib":
         run_calib(args)
+        return
+    if args.mode == "piso":
+        run_piso(args, x)
         return
     api, H = x.api, x.hip
     N = args.cells

@@ -723,6 +723,8 @@ static xcal_mesh* upload_mesh(xcal_ctx* ctx, HostMesh& M) {
   m->nzlen = M.nzlen;
   for (size_t c = 0; c < M.rbeg.size(); ++c)
     m->roww = std::max(m->roww, (M.rend[c] - M.rbeg[c]) / kSlice);
+  const char* nofuse = std::getenv("XCAL_NO_FUSE");
+  m->fuse = !(nofuse && *nofuse == '1');
   m->nbcell = M.nbcell;
   m->n_patches = M.n_patches;
   m->patch_first = M.patch_first;

@@ -33,26 +33,28 @@
 
 namespace xcal {
 
-// Sums `count` partial pairs in slot order (fixed per-thread strides, fixed
-// tree) and runs the scalar update `fin`.
-__global__ void __launch_bounds__(kRunBlock)
+// Sums `count` partial pairs in slot order (reduce_partials) and runs the
+// scalar update `fin`.
+__global__ void __launch_bounds__(kBlock)
     k_finalize(const double* __restrict__ partial, int count, KS* ks, int fin) {
-  __shared__ double lds[kRunBlock / 64];
+  __shared__ double lds[kBlock / 64];
   KS k;
   if (threadIdx.x == 0) k = *ks;  // loaded while the partials load
-  double t0 = 0.0, t1 = 0.0;
-  for (int b = threadIdx.x; b < count; b += blockDim.x) {
-    t0 += partial[2 * b];
-    t1 += partial[2 * b + 1];
-  }
-  t0 = block_sum(t0, lds);
-  t1 = block_sum(t1, lds);
+  double t0, t1;
+  reduce_partials(partial, count, t0, t1, lds);
   if (threadIdx.x == 0 && k.stop == STOP_NONE) {
     finalize(&k, fin, t0, t1);
     *ks = k;
   }
 }
 
+// The finalize of a fused chain's last kernel, from its state `in` into `out`.
+__global__ void __launch_bounds__(kBlock)
+    k_finalize_to(const double* __restrict__ partial, int count, const KS* in, KS* out, int fin) {
+  __shared__ double lds[kBlock / 64];
+  const KS k = kstate(KStep{in, out, partial, count, fin}, lds);
+  (void)k;
+}
 
 // --------------------------------------------------------- static kernels
 // r = b - A x0 (r = b when x0 is NULL: zero start) ; x = 0 ; pold = 0
@@ -115,9 +117,11 @@ __global__ void __launch_bounds__(kBlock)
 template <int G, class T>
 __global__ void __launch_bounds__(kBlock)
     k_cg_pspmv(MatKT<T> A, const T* __restrict__ z, const T* __restrict__ pold,
-               T* __restrict__ pnew, T* __restrict__ q, double* partial, const KS* ks) {
-  const bool live = !stopped(ks);
-  const T beta = (T)ks->beta;
+               T* __restrict__ pnew, T* __restrict__ q, double* partial, KStep K) {
+  __shared__ double lds[kBlock / 64];
+  const KS k = kstate(K, lds);
+  const bool live = k.stop == STOP_NONE;
+  const T beta = (T)k.beta;
   int64_t end;
   const int64_t i = block_row(A, end);
   double s0 = 0.0;
@@ -159,9 +163,11 @@ template <class T>
 __global__ void __launch_bounds__(kBlock)
     k_cg_update_diag(int64_t n, const T* __restrict__ S, const T* __restrict__ p,
                      const T* __restrict__ q, T* __restrict__ x, T* __restrict__ r,
-                     T* __restrict__ z, double* partial, const KS* ks) {
-  const bool live = !stopped(ks);
-  const T a = (T)ks->alpha;
+                     T* __restrict__ z, double* partial, KStep K) {
+  __shared__ double lds[kBlock / 64];
+  const KS k = kstate(K, lds);
+  const bool live = k.stop == STOP_NONE;
+  const T a = (T)k.alpha;
   const int64_t i = xcd_row();
   double s0 = 0.0;
   if (i < n) {
@@ -184,9 +190,11 @@ template <int G, bool VDIAG, class T>
 __global__ void __launch_bounds__(kBlock)
     k_bi_spmv_p(MatKT<T> A, const T* __restrict__ r, const T* __restrict__ pold,
                 T* __restrict__ pnew, const T* __restrict__ S, T* __restrict__ qv,
-                const T* __restrict__ c, double* partial, const KS* ks) {
-  const bool live = !stopped(ks);
-  const T beta = (T)ks->beta;
+                const T* __restrict__ c, double* partial, KStep K) {
+  __shared__ double lds[kBlock / 64];
+  const KS k = kstate(K, lds);
+  const bool live = k.stop == STOP_NONE;
+  const T beta = (T)k.beta;
   int64_t end;
   const int64_t i = block_row(A, end);
   double s0 = 0.0;
@@ -209,9 +217,11 @@ template <int G, bool TDIAG, class T>
 __global__ void __launch_bounds__(kBlock)
     k_bi_spmv_s(MatKT<T> A, const T* __restrict__ r, const T* __restrict__ v,
                 const T* __restrict__ p, T* __restrict__ s, T* __restrict__ x,
-                const T* __restrict__ S, T* __restrict__ dt, double* partial, const KS* ks) {
-  const bool live = !stopped(ks);
-  const T a = (T)ks->alpha;
+                const T* __restrict__ S, T* __restrict__ dt, double* partial, KStep K) {
+  __shared__ double lds[kBlock / 64];
+  const KS k = kstate(K, lds);
+  const bool live = k.stop == STOP_NONE;
+  const T a = (T)k.alpha;
   int64_t end;
   const int64_t i = block_row(A, end);
   double s0 = 0.0, s1 = 0.0;
@@ -238,9 +248,11 @@ template <class T>
 __global__ void __launch_bounds__(kBlock)
     k_bi_tail(int64_t n, const T* __restrict__ s, const T* __restrict__ t, const T* __restrict__ v,
               T* __restrict__ x, T* __restrict__ r, T* __restrict__ p, const T* __restrict__ c,
-              double* partial, const KS* ks) {
-  const bool live = !stopped(ks);
-  const T w = (T)ks->omega;
+              double* partial, KStep K) {
+  __shared__ double lds[kBlock / 64];
+  const KS k = kstate(K, lds);
+  const bool live = k.stop == STOP_NONE;
+  const T w = (T)k.omega;
   const int64_t i = xcd_row();
   double s0 = 0.0, s1 = 0.0;
   if (i < n) {
@@ -416,6 +428,8 @@ struct Solver {
   T* work;
   KS* ks;
   double* partial;
+  KS* fks;        // two states for fused finalizes
+  double* fpart;  // two partial areas of kFuseBlocks pairs for fused finalizes
   int grid;   // elementwise kernels: grid_for(n) blocks in XCD order
   int gridA;  // gathering kernels: the mesh's row blocks (xcal_mesh::xblk)
   int* cg_batch;
@@ -435,6 +449,8 @@ static void ensure_workspace(xcal_eqn* e, bool fp32) {
   if (e->work.n < 12 * n) e->work.alloc(12 * n);
   if (e->kpart.n < 2 * slots) e->kpart.alloc(2 * slots);
   if (e->kstate.n < (int64_t)sizeof(KS)) e->kstate.alloc(sizeof(KS));
+  if (e->fpart.n < 4 * kFuseBlocks) e->fpart.alloc(4 * kFuseBlocks);
+  if (e->fstate.n < (int64_t)(2 * sizeof(KS))) e->fstate.alloc(2 * sizeof(KS));
   if (!e->kstate_host) XCAL_HIP(hipHostMalloc((void**)&e->kstate_host, sizeof(KS), 0));
   if (fp32) {
     if (e->work32.n < 12 * n) e->work32.alloc(12 * n);
@@ -465,6 +481,8 @@ static Solver<double> make_solver(xcal_eqn* e) {
   S.work = e->work.p;
   S.ks = (KS*)e->kstate.p;
   S.partial = e->kpart.p;
+  S.fks = (KS*)e->fstate.p;
+  S.fpart = e->fpart.p;
   S.grid = grid_for(S.n);
   S.gridA = S.m->nxblk;
   S.cg_batch = &e->cg_batch;
@@ -518,6 +536,8 @@ static Solver<float> make_solver32(xcal_eqn* e) {
   S.work = e->work32.p;
   S.ks = (KS*)e->kstate.p;
   S.partial = e->kpart.p;
+  S.fks = (KS*)e->fstate.p;
+  S.fpart = e->fpart.p;
   S.grid = grid_for(n);
   S.gridA = m->nxblk;
   S.cg_batch = &e->ir_pred[0][0];
@@ -527,8 +547,57 @@ static Solver<float> make_solver32(xcal_eqn* e) {
 
 template <class T>
 static void finalize_launch(const Solver<T>& S, int count, int fin) {
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kRunBlock), 0, S.s, S.partial, count, S.ks, fin);
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, S.s, S.partial, count, S.ks, fin);
 }
+
+// Fused finalizes (kFuseBlocks).  Each kernel of the chain reads the state
+// `cur` and the pending partials, writes its own partials to the other area
+// and the updated state to the other slot; flush() runs the last pending
+// finalize into S.ks before the host reads the state.  Off, every kernel
+// reads S.ks and writes S.partial, with a one-workgroup finalize between.
+template <class T>
+struct Fuse {
+  const Solver<T>& S;
+  bool on;
+  KS* cur;
+  int kslot = 0, pslot = 0;
+  const double* pend_p = nullptr;
+  int pend_n = 0, pend_fin = FIN_NONE;
+  Fuse(const Solver<T>& s, bool fuse) : S(s), on(fuse), cur(s.ks) {}
+  // partial area and state step of the next kernel of the chain
+  double* part() {
+    if (!on) return S.partial;
+    double* p = S.fpart + 2 * kFuseBlocks * pslot;
+    pslot ^= 1;
+    return p;
+  }
+  KStep step() {
+    if (!on) return kstep_plain(S.ks);
+    KS* out = S.fks + kslot;
+    kslot ^= 1;
+    const KStep K{cur, out, pend_p, pend_n, pend_fin};
+    cur = out;
+    pend_fin = FIN_NONE;
+    return K;
+  }
+  // the kernel just launched left `count` partials at p for update `fin`
+  void done(const double* p, int count, int fin) {
+    if (!on) {
+      finalize_launch(S, count, fin);
+      return;
+    }
+    pend_p = p;
+    pend_n = count;
+    pend_fin = fin;
+  }
+  void flush() {
+    if (!on || (pend_fin == FIN_NONE && cur == S.ks)) return;
+    hipLaunchKernelGGL(k_finalize_to, dim3(1), dim3(kBlock), 0, S.s, pend_p, pend_n,
+                       (const KS*)cur, S.ks, pend_fin);
+    cur = S.ks;
+    pend_fin = FIN_NONE;
+  }
+};
 
 // forward sweep over the launch plan
 template <bool UPD, class T>
@@ -646,10 +715,11 @@ static void init_state(Solver<T>& S, int itmax, double atol, double rtol) {
 // later batches are short.
 template <class T>
 static int run_batches(Solver<T>& S, int* pred, int tail, const std::function<void()>& iteration,
-                       KS& out) {
+                       KS& out, const std::function<void()>& flush = nullptr) {
   int batch = *pred > 0 ? std::max(1, (*pred * 9) / 10) : 8;
   for (;;) {
     for (int b = 0; b < batch; ++b) iteration();
+    if (flush) flush();
     XCAL_HIP(hipGetLastError());
     out = read_state(S);
     if (out.stop != STOP_NONE) break;
@@ -684,15 +754,18 @@ static int cg(Solver<T>& S, int precon, const T* b, T* x0, int itmax, double ato
     finalize_launch(S, S.grid, FIN_CG_INIT);
   }
   int parity = 0;
+  const bool split = n >= kSplitRows;
+  Fuse<T> F(S, S.m->fuse && !dilu && !split && S.gridA <= kFuseBlocks && S.grid <= kFuseBlocks);
   auto iteration = [&]() {
     T* pold = parity ? pb : pa;
     T* pnew = parity ? pa : pb;
     parity ^= 1;
     xcal_ctx* c = S.m->ctx;
-    const bool split = n >= kSplitRows;
     if (split)
       hipLaunchKernelGGL(k_cg_p<T>, dim3(S.gridA), dim3(kBlock), 0, S.s, S.A, (const T*)zd,
                          (const T*)pold, pnew, (const KS*)S.ks);
+    double* pp = F.part();
+    const KStep K = F.step();
     timed(c, 0, [&] {
       with_row_group(S.A.rw, [&](auto G) {
         constexpr int g = decltype(G)::value;
@@ -701,10 +774,10 @@ static int cg(Solver<T>& S, int precon, const T* b, T* x0, int itmax, double ato
                              (const T*)pnew, q, S.partial, (const KS*)S.ks);
         else
           hipLaunchKernelGGL((k_cg_pspmv<g, T>), dim3(S.gridA), dim3(kBlock), 0, S.s, S.A,
-                             (const T*)zd, (const T*)pold, pnew, q, S.partial, (const KS*)S.ks);
+                             (const T*)zd, (const T*)pold, pnew, q, pp, K);
       });
     });
-    finalize_launch(S, S.gridA, FIN_CG_PAP);
+    F.done(pp, S.gridA, FIN_CG_PAP);
     if (dilu) {
       FwdK<T> W = fwdk(S, (const T*)nullptr, Y, Yb, (const KS*)S.ks);
       W.p = pnew;
@@ -715,13 +788,15 @@ static int cg(Solver<T>& S, int precon, const T* b, T* x0, int itmax, double ato
       timed(c, 2, [&] { sweep_bwd<1>(S, bwdk(S, (const T*)Yb, zd, (const T*)r, (const KS*)S.ks)); });
       finalize_launch(S, sweep_slots, FIN_CG_ITER);
     } else {
+      double* pu = F.part();
+      const KStep Ku = F.step();
       hipLaunchKernelGGL(k_cg_update_diag<T>, dim3(S.grid), dim3(kBlock), 0, S.s, n, Sd,
-                         (const T*)pnew, (const T*)q, x, r, zd, S.partial, (const KS*)S.ks);
-      finalize_launch(S, S.grid, FIN_CG_ITER);
+                         (const T*)pnew, (const T*)q, x, r, zd, pu, Ku);
+      F.done(pu, S.grid, FIN_CG_ITER);
     }
   };
   KS st = read_state(S);
-  if (st.stop == STOP_NONE) run_batches(S, S.cg_batch, 4, iteration, st);
+  if (st.stop == STOP_NONE) run_batches(S, S.cg_batch, 4, iteration, st, [&] { F.flush(); });
   if (x0) hipLaunchKernelGGL(k_add, dim3(S.grid), dim3(kBlock), 0, S.s, n, (const double*)x,
                              (double*)x0);
   if (xsol) *xsol = x;
@@ -756,6 +831,7 @@ static int bicgstab(Solver<T>& S, int precon, const T* b, T* x0, int itmax, doub
     finalize_launch(S, S.grid, FIN_BI_INIT);
   }
   int parity = 0;
+  Fuse<T> F(S, S.m->fuse && !dilu && S.gridA <= kFuseBlocks && S.grid <= kFuseBlocks);
   auto iteration = [&]() {
     T* pold = parity ? pb : pa;
     T* pnew = parity ? pa : pb;
@@ -766,7 +842,7 @@ static int bicgstab(Solver<T>& S, int precon, const T* b, T* x0, int itmax, doub
         hipLaunchKernelGGL((k_bi_spmv_p<g, false, T>), dim3(S.gridA), dim3(kBlock), 0,
                            S.s, S.A,
                            (const T*)r, (const T*)pold, pnew, (const T*)nullptr, d, b, S.partial,
-                           ks);
+                           kstep_plain(ks));
       });
       sweep_fwd<false>(S, fwdk(S, (const T*)d, Y, Yb, ks));
       sweep_bwd<2>(S, bwdk(S, (const T*)Yb, v, b, ks));
@@ -776,33 +852,39 @@ static int bicgstab(Solver<T>& S, int precon, const T* b, T* x0, int itmax, doub
         hipLaunchKernelGGL((k_bi_spmv_s<g, false, T>), dim3(S.gridA), dim3(kBlock), 0,
                            S.s, S.A,
                            (const T*)r, (const T*)v, (const T*)pnew, sv, x, (const T*)nullptr, d,
-                           S.partial, ks);
+                           S.partial, kstep_plain(ks));
       });
       sweep_fwd<false>(S, fwdk(S, (const T*)d, Y, Yb, ks));
       sweep_bwd<3>(S, bwdk(S, (const T*)Yb, t, (const T*)sv, ks));
       finalize_launch(S, sweep_slots, FIN_BI_OMEGA);
     } else {
+      double* pp = F.part();
+      const KStep Kp = F.step();
       with_row_group(S.A.rw, [&](auto G) {
         constexpr int g = decltype(G)::value;
         hipLaunchKernelGGL((k_bi_spmv_p<g, true, T>), dim3(S.gridA), dim3(kBlock), 0,
                            S.s, S.A,
-                           (const T*)r, (const T*)pold, pnew, Sd, v, b, S.partial, ks);
+                           (const T*)r, (const T*)pold, pnew, Sd, v, b, pp, Kp);
       });
-      finalize_launch(S, S.gridA, FIN_BI_ALPHA);
+      F.done(pp, S.gridA, FIN_BI_ALPHA);
+      double* ps = F.part();
+      const KStep Ks = F.step();
       with_row_group(S.A.rw, [&](auto G) {
         constexpr int g = decltype(G)::value;
         hipLaunchKernelGGL((k_bi_spmv_s<g, true, T>), dim3(S.gridA), dim3(kBlock), 0,
                            S.s, S.A,
-                           (const T*)r, (const T*)v, (const T*)pnew, sv, x, Sd, t, S.partial, ks);
+                           (const T*)r, (const T*)v, (const T*)pnew, sv, x, Sd, t, ps, Ks);
       });
-      finalize_launch(S, S.gridA, FIN_BI_OMEGA);
+      F.done(ps, S.gridA, FIN_BI_OMEGA);
     }
+    double* pt = F.part();
+    const KStep Kt = F.step();
     hipLaunchKernelGGL(k_bi_tail<T>, dim3(S.grid), dim3(kBlock), 0, S.s, n, (const T*)sv,
-                       (const T*)t, (const T*)v, x, r, pnew, b, S.partial, ks);
-    finalize_launch(S, S.grid, FIN_BI_END);
+                       (const T*)t, (const T*)v, x, r, pnew, b, pt, Kt);
+    F.done(pt, S.grid, FIN_BI_END);
   };
   KS st = read_state(S);
-  if (st.stop == STOP_NONE) run_batches(S, S.bi_batch, 2, iteration, st);
+  if (st.stop == STOP_NONE) run_batches(S, S.bi_batch, 2, iteration, st, [&] { F.flush(); });
   if (x0) hipLaunchKernelGGL(k_add, dim3(S.grid), dim3(kBlock), 0, S.s, n, (const double*)x,
                              (double*)x0);
   if (xsol) *xsol = x;

@@ -106,6 +106,54 @@ __device__ inline bool stopped(const KS* ks) { return ks->stop != STOP_NONE; }
 // Kernels of an iteration read the stop flag early but use it only to
 // predicate their stores, so their loads do not wait behind it.
 
+// Sum of `count` partial pairs in slot order, the same in every kernel that
+// reduces them: thread t < kBlock adds slots t, t + kBlock, ... in turn,
+// then block_sum; every thread receives both sums.  lds: >= kBlock / 64.
+__device__ inline void reduce_partials(const double* __restrict__ partial, int count, double& t0,
+                                       double& t1, double* lds) {
+  double a = 0.0, b = 0.0;
+  if ((int)threadIdx.x < kBlock)
+    for (int s = threadIdx.x; s < count; s += kBlock) {
+      a += partial[2 * s];
+      b += partial[2 * s + 1];
+    }
+  a = block_sum(a, lds);
+  b = block_sum(b, lds);
+  if (threadIdx.x == 0) {
+    lds[0] = a;
+    lds[1] = b;
+  }
+  __syncthreads();
+  t0 = lds[0];
+  t1 = lds[1];
+  __syncthreads();
+}
+
+// The Krylov state a kernel of an iteration works with.  Unfused (fin ==
+// FIN_NONE, out == nullptr) it is `in`, as a one-workgroup finalize left it.
+// Fused, the kernel first does that finalize itself: every workgroup reduces
+// the previous kernel's partials in the same order and applies the same
+// scalar update, so all hold the same state, and workgroup 0 stores it to
+// `out` (never `in`, which the other workgroups are still reading).
+struct KStep {
+  const KS* in;
+  KS* out;
+  const double* pin;
+  int cin;
+  int fin;
+};
+inline KStep kstep_plain(const KS* ks) { return KStep{ks, nullptr, nullptr, 0, FIN_NONE}; }
+__device__ inline KS kstate(const KStep& K, double* lds) {
+  KS k = *K.in;
+  if (K.fin != FIN_NONE) {
+    double t0, t1;
+    reduce_partials(K.pin, K.cin, t0, t1, lds);
+    if (k.stop == STOP_NONE) finalize(&k, K.fin, t0, t1);
+  }
+  if (K.out && blockIdx.x == 0 && threadIdx.x == 0) *K.out = k;
+  return k;
+}
+
 // workgroup sums of s0, s1 -> partial[2 slot], partial[2 slot + 1]
 template <int NT>
 __device__ inline void store_partial(double s0, double s1, double* partial, int slot) {

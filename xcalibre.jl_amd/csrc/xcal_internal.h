@@ -59,6 +59,10 @@ constexpr int kSmallLevel = 256;  // levels below this many rows join a single-w
 // pass before a single-gather SpMV; smaller ones form it on the fly inside the
 // SpMV, saving a launch.
 constexpr int64_t kSplitRows = 1 << 18;
+// Krylov solves with a diagonal (or no) preconditioner on meshes whose
+// launches have at most this many workgroups fold each one-workgroup
+// finalize into the next kernel of the iteration.
+constexpr int kFuseBlocks = 256;
 
 // Compile-time entry bound for a level whose rows have w entries: 0 selects
 // the general loop.
@@ -174,6 +178,7 @@ struct xcal_mesh {
   // rbeg[c], rbeg[c] + kSlice, ... < rend[c], in the reference's column order
   int64_t nzlen = 0;  // storage length including slice padding
   int32_t roww = 0;   // most entries in one row
+  bool fuse = true;   // fused Krylov finalizes allowed (XCAL_NO_FUSE=1 at creation: off)
   xcal::DBuf<int32_t> rbeg, rend, colval, trans;
   // DILU level schedules: forward level l = device rows
   // [lev_ptr_f[l], lev_ptr_f[l+1]); backward level l = lev_rows_b over
@@ -251,6 +256,8 @@ struct xcal_eqn {
   xcal::DBuf<double> work;    // solver vectors, 12 x n
   xcal::DBuf<double> kpart;   // reduction partial sums
   xcal::DBuf<char> kstate;    // device-resident Krylov scalars
+  xcal::DBuf<double> fpart;   // fused finalizes: two partial areas
+  xcal::DBuf<char> fstate;    // fused finalizes: two more Krylov states
   void* kstate_host = nullptr;  // pinned copy
   int cg_batch = 0, bi_batch = 0;   // iterations of the last solve (first-batch size)
   // mixed precision: FP32 copies of the matrix and preconditioner, workspace

@@ -129,6 +129,68 @@ __device__ inline T row_sum(const MatKT<T>& A, int64_t i, XV&& xv) {
     return row_sum_g<8>(A, i, xv);
   }
 }
+// A row sum over entries nz * f(U[col], W[col]) in two phases: row_pair_load
+// issues the row's first G entries and both gathers, before the caller has
+// the scalars f needs (a Krylov state formed in the same kernel);
+// row_pair_sum then adds in the row's order -- the same arithmetic as
+// row_sum<G> with xv(j) = f(U[j], W[j]) -- and sums any entries past the
+// first group itself.
+template <int G, class T>
+struct RowPair {
+  int32_t k0, k1;
+  T a[G], u[G], w[G];
+};
+template <int G, class T>
+__device__ inline RowPair<G, T> row_pair_load(const MatKT<T>& A, int64_t i,
+                                              const T* __restrict__ U, const T* __restrict__ W) {
+  RowPair<G, T> R;
+  R.k0 = A.rb[i];
+  R.k1 = A.re[i];
+  const int32_t klast = R.k1 - kSlice;
+  int col[G];
+#pragma unroll
+  for (int u = 0; u < G; ++u) {
+    const int32_t k = min(R.k0 + u * kSlice, klast);
+    col[u] = __builtin_nontemporal_load(&A.colval[k]);
+    R.a[u] = __builtin_nontemporal_load(&A.nz[k]);
+  }
+#pragma unroll
+  for (int u = 0; u < G; ++u) {
+    R.u[u] = U[col[u]];
+    R.w[u] = W[col[u]];
+  }
+  return R;
+}
+template <int G, class T, class F>
+__device__ inline T row_pair_sum(const MatKT<T>& A, const RowPair<G, T>& R,
+                                 const T* __restrict__ U, const T* __restrict__ W, F&& f) {
+  T acc = 0;
+#pragma unroll
+  for (int u = 0; u < G; ++u) {
+    const T t = R.a[u] * f(R.u[u], R.w[u]);
+    acc = R.k0 + u * kSlice < R.k1 ? acc + t : acc;
+  }
+  const int32_t klast = R.k1 - kSlice;
+  for (int32_t k0 = R.k0 + G * kSlice; k0 < R.k1; k0 += G * kSlice) {  // rows wider than G
+    int col[G];
+    T a[G], x[G];
+#pragma unroll
+    for (int u = 0; u < G; ++u) {
+      const int32_t k = min(k0 + u * kSlice, klast);
+      col[u] = A.colval[k];
+      a[u] = A.nz[k];
+    }
+#pragma unroll
+    for (int u = 0; u < G; ++u) x[u] = f(U[col[u]], W[col[u]]);
+#pragma unroll
+    for (int u = 0; u < G; ++u) {
+      const T t = a[u] * x[u];
+      acc = k0 + u * kSlice < R.k1 ? acc + t : acc;
+    }
+  }
+  return acc;
+}
+
 // Host side: calls f(std::integral_constant<int, G>) with row_sum's group
 // width for rows of at most rw entries.
 template <class F>

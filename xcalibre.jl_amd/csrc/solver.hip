@@ -119,15 +119,19 @@ __global__ void __launch_bounds__(kBlock)
     k_cg_pspmv(MatKT<T> A, const T* __restrict__ z, const T* __restrict__ pold,
                T* __restrict__ pnew, T* __restrict__ q, double* partial, KStep K) {
   __shared__ double lds[kBlock / 64];
+  int64_t end;
+  const int64_t i = block_row(A, end);
+  const bool in = i < end;
+  const int64_t ic = in ? i : 0;
+  // the row's loads go out before the state is formed (a fused finalize)
+  const RowPair<G, T> R = row_pair_load<G>(A, ic, z, pold);
+  const T zi = z[ic], poi = pold[ic];
   const KS k = kstate(K, lds);
   const bool live = k.stop == STOP_NONE;
   const T beta = (T)k.beta;
-  int64_t end;
-  const int64_t i = block_row(A, end);
   double s0 = 0.0;
-  if (i < end) {
-    const T zi = z[i], poi = pold[i];  // issued with the row's first loads
-    const T acc = row_sum<G>(A, i, [&](int j) { return z[j] + beta * pold[j]; });
+  if (in) {
+    const T acc = row_pair_sum(A, R, z, pold, [&](T u, T w) { return u + beta * w; });
     const T pi = zi + beta * poi;
     if (live) {
       pnew[i] = pi;
@@ -165,15 +169,19 @@ __global__ void __launch_bounds__(kBlock)
                      const T* __restrict__ q, T* __restrict__ x, T* __restrict__ r,
                      T* __restrict__ z, double* partial, KStep K) {
   __shared__ double lds[kBlock / 64];
+  const int64_t i = xcd_row();
+  const bool in = i < n;
+  const int64_t c = in ? i : 0;
+  // the row's loads go out before the state is formed (a fused finalize)
+  const T xo = x[c], po = p[c], ro = r[c], qo = q[c], so = S ? S[c] : T(1);
   const KS k = kstate(K, lds);
   const bool live = k.stop == STOP_NONE;
   const T a = (T)k.alpha;
-  const int64_t i = xcd_row();
   double s0 = 0.0;
-  if (i < n) {
-    const T xi = x[i] + a * p[i];
-    const T ri = r[i] + (-a) * q[i];
-    const T zi = S ? S[i] * ri : ri;
+  if (in) {
+    const T xi = xo + a * po;
+    const T ri = ro + (-a) * qo;
+    const T zi = S ? so * ri : ri;
     if (live) {
       x[i] = xi;
       r[i] = ri;
@@ -192,15 +200,20 @@ __global__ void __launch_bounds__(kBlock)
                 T* __restrict__ pnew, const T* __restrict__ S, T* __restrict__ qv,
                 const T* __restrict__ c, double* partial, KStep K) {
   __shared__ double lds[kBlock / 64];
+  int64_t end;
+  const int64_t i = block_row(A, end);
+  const bool in = i < end;
+  const int64_t ic = in ? i : 0;
+  // the row's loads go out before the state is formed (a fused finalize)
+  const RowPair<G, T> R = row_pair_load<G>(A, ic, r, pold);
+  const T ri = r[ic], poi = pold[ic];
   const KS k = kstate(K, lds);
   const bool live = k.stop == STOP_NONE;
   const T beta = (T)k.beta;
-  int64_t end;
-  const int64_t i = block_row(A, end);
   double s0 = 0.0;
-  if (i < end) {
-    const T acc = row_sum<G>(A, i, [&](int j) { return r[j] + beta * pold[j]; });
-    const T pi = r[i] + beta * pold[i];
+  if (in) {
+    const T acc = row_pair_sum(A, R, r, pold, [&](T u, T w) { return u + beta * w; });
+    const T pi = ri + beta * poi;
     const T vi = VDIAG && S ? S[i] * acc : acc;
     if (live) {
       pnew[i] = pi;
@@ -219,16 +232,21 @@ __global__ void __launch_bounds__(kBlock)
                 const T* __restrict__ p, T* __restrict__ s, T* __restrict__ x,
                 const T* __restrict__ S, T* __restrict__ dt, double* partial, KStep K) {
   __shared__ double lds[kBlock / 64];
+  int64_t end;
+  const int64_t i = block_row(A, end);
+  const bool in = i < end;
+  const int64_t ic = in ? i : 0;
+  // the row's loads go out before the state is formed (a fused finalize)
+  const RowPair<G, T> R = row_pair_load<G>(A, ic, r, v);
+  const T ri = r[ic], vi = v[ic], xo = x[ic], po = p[ic];
   const KS k = kstate(K, lds);
   const bool live = k.stop == STOP_NONE;
   const T a = (T)k.alpha;
-  int64_t end;
-  const int64_t i = block_row(A, end);
   double s0 = 0.0, s1 = 0.0;
-  if (i < end) {
-    const T acc = row_sum<G>(A, i, [&](int j) { return r[j] + (-a) * v[j]; });
-    const T si = r[i] + (-a) * v[i];
-    const T xi = x[i] + a * p[i];
+  if (in) {
+    const T acc = row_pair_sum(A, R, r, v, [&](T u, T w) { return u + (-a) * w; });
+    const T si = ri + (-a) * vi;
+    const T xi = xo + a * po;
     const T ti = TDIAG && S ? S[i] * acc : acc;
     if (live) {
       s[i] = si;
@@ -250,21 +268,25 @@ __global__ void __launch_bounds__(kBlock)
               T* __restrict__ x, T* __restrict__ r, T* __restrict__ p, const T* __restrict__ c,
               double* partial, KStep K) {
   __shared__ double lds[kBlock / 64];
+  const int64_t i = xcd_row();
+  const bool in = i < n;
+  const int64_t j = in ? i : 0;
+  // the row's loads go out before the state is formed (a fused finalize)
+  const T xo = x[j], so = s[j], to = t[j], po = p[j], vo = v[j], co = c[j];
   const KS k = kstate(K, lds);
   const bool live = k.stop == STOP_NONE;
   const T w = (T)k.omega;
-  const int64_t i = xcd_row();
   double s0 = 0.0, s1 = 0.0;
-  if (i < n) {
-    const T xi = x[i] + w * s[i];
-    const T ri = s[i] + (-w) * t[i];
-    const T pi = p[i] + (-w) * v[i];
+  if (in) {
+    const T xi = xo + w * so;
+    const T ri = so + (-w) * to;
+    const T pi = po + (-w) * vo;
     if (live) {
       x[i] = xi;
       r[i] = ri;
       p[i] = pi;
     }
-    s0 = dd(c[i], ri);
+    s0 = dd(co, ri);
     s1 = dd(ri, ri);
   }
   store_partial<kBlock>(s0, s1, partial, blockIdx.x);

@@ -111,11 +111,23 @@ __device__ inline bool stopped(const KS* ks) { return ks->stop != STOP_NONE; }
 // then block_sum; every thread receives both sums.  lds: >= kBlock / 64.
 __device__ inline void reduce_partials(const double* __restrict__ partial, int count, double& t0,
                                        double& t1, double* lds) {
+  constexpr int U = 8;  // slots loaded together, then added in slot order
   double a = 0.0, b = 0.0;
   if ((int)threadIdx.x < kBlock)
-    for (int s = threadIdx.x; s < count; s += kBlock) {
-      a += partial[2 * s];
-      b += partial[2 * s + 1];
+    for (int s0 = threadIdx.x; s0 < count; s0 += U * kBlock) {
+      double pa[U], pb[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int s = s0 + u * kBlock < count ? s0 + u * kBlock : 0;
+        pa[u] = partial[2 * s];
+        pb[u] = partial[2 * s + 1];
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (s0 + u * kBlock < count) {
+          a += pa[u];
+          b += pb[u];
+        }
     }
   a = block_sum(a, lds);
   b = block_sum(b, lds);

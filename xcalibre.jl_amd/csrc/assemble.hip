@@ -22,20 +22,45 @@ __global__ void __launch_bounds__(kBlock)
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m.n) return;
   double ac_sum = 0.0;
-  const int32_t k0 = m.cf_ptr[i], k1 = m.cf_ptr[i + 1];
-  for (int32_t k = k0; k < k1; ++k) {
-    const int f = m.cf_face[k];
-    const double ns = (double)m.cf_sign[k];
-    double AC = 0.0, AN = 0.0;  // _scheme! (:173-197)
-    for (int t = 0; t < T.n; ++t) {
-      double ac, an;
-      scheme(m, T.t[t], f, ns, ac, an);
-      AC += ac;
-      AN += an;
-    }
-    ac_sum += AC;
-    nzval[m.cf_slot[k]] = AN;
-  }
+  // each face's values for every term are loaded for a group of faces at
+  // once (a term without a flux field reads the weights and takes its
+  // constant), then the faces are summed in list order
+  struct Rec {
+    double ns, w, lapEf, delta, J[kMaxTerms];
+    int32_t slot;
+  };
+  cell_faces(
+      m.cf_ptr[i], m.cf_ptr[i + 1],
+      [&](int32_t k) {
+        const int f = m.cf_face[k];
+        Rec r;
+        r.ns = (double)m.cf_sign[k];
+        r.slot = m.cf_slot[k];
+        r.w = m.weight[f];
+        r.lapEf = m.lapEf[f];
+        r.delta = m.delta[f];
+#pragma unroll
+        for (int t = 0; t < kMaxTerms; ++t) {
+          const double* src = t < T.n && T.t[t].flux ? T.t[t].flux : m.weight;
+          r.J[t] = src[f];
+        }
+        return r;
+      },
+      [&](const Rec& r, bool ok) {
+        double AC = 0.0, AN = 0.0;  // _scheme! (:173-197)
+#pragma unroll
+        for (int t = 0; t < kMaxTerms; ++t) {
+          if (t < T.n) {
+            const double J = T.t[t].flux ? r.J[t] : T.t[t].flux_const;
+            double ac, an;
+            scheme_vals(T.t[t], J, r.ns, r.w, r.lapEf, r.delta, ac, an);
+            AC += ac;
+            AN += an;
+          }
+        }
+        ac_sum = ok ? ac_sum + AC : ac_sum;
+        if (ok) nzval[r.slot] = AN;
+      });
   double AC = 0.0, BX = 0.0, BY = 0.0, BZ = 0.0;  // _scheme_source! (:200-252)
   for (int t = 0; t < T.n; ++t) {
     double ac, bx;

@@ -122,14 +122,21 @@ __global__ void __launch_bounds__(kBlock)
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m.n) return;
   double rx = 0.0, ry = 0.0, rz = 0.0;
-  for (int32_t k = m.cf_ptr[i]; k < m.cf_ptr[i + 1]; ++k) {
-    const int f = m.cf_face[k];
-    const double ns = (double)m.cf_sign[k];
-    const double v = phif[f];
-    rx += v * (m.sfx[f] * ns);
-    ry += v * (m.sfy[f] * ns);
-    rz += v * (m.sfz[f] * ns);
-  }
+  struct Rec {
+    double v, ns, sx, sy, sz;
+  };
+  cell_faces(
+      m.cf_ptr[i], m.cf_ptr[i + 1],
+      [&](int32_t k) {
+        const int f = m.cf_face[k];
+        return Rec{phif[f], (double)m.cf_sign[k], m.sfx[f], m.sfy[f], m.sfz[f]};
+      },
+      [&](const Rec& r, bool ok) {
+        const double tx = r.v * (r.sx * r.ns), ty = r.v * (r.sy * r.ns), tz = r.v * (r.sz * r.ns);
+        rx = ok ? rx + tx : rx;
+        ry = ok ? ry + ty : ry;
+        rz = ok ? rz + tz : rz;
+      });
   const double V = m.vol[i];
   gx[i] = rx / V;
   gy[i] = ry / V;
@@ -165,16 +172,25 @@ __global__ void __launch_bounds__(kBlock)
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m.n) return;
   double r[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  for (int32_t k = m.cf_ptr[i]; k < m.cf_ptr[i + 1]; ++k) {
-    const int f = m.cf_face[k];
-    const double ns = (double)m.cf_sign[k];
-    const double S[3] = {m.sfx[f] * ns, m.sfy[f] * ns, m.sfz[f] * ns};
-    const double P[3] = {fx[f], fy[f], fz[f]};
+  struct Rec {
+    double S[3], P[3];
+  };
+  cell_faces(
+      m.cf_ptr[i], m.cf_ptr[i + 1],
+      [&](int32_t k) {
+        const int f = m.cf_face[k];
+        const double ns = (double)m.cf_sign[k];
+        return Rec{{m.sfx[f] * ns, m.sfy[f] * ns, m.sfz[f] * ns}, {fx[f], fy[f], fz[f]}};
+      },
+      [&](const Rec& e, bool ok) {
 #pragma unroll
-    for (int p = 0; p < 3; ++p)
+        for (int p = 0; p < 3; ++p)
 #pragma unroll
-      for (int q = 0; q < 3; ++q) r[3 * p + q] += P[p] * S[q];
-  }
+          for (int q = 0; q < 3; ++q) {
+            const double t = e.P[p] * e.S[q];
+            r[3 * p + q] = ok ? r[3 * p + q] + t : r[3 * p + q];
+          }
+      });
   const double V = m.vol[i];
 #pragma unroll
   for (int k = 0; k < 9; ++k) t0[(int64_t)k * m.n + i] = r[k] / V;
@@ -210,8 +226,10 @@ __global__ void __launch_bounds__(kBlock)
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m.n) return;
   double red = 0.0;
-  for (int32_t k = m.cf_ptr[i]; k < m.cf_ptr[i + 1]; ++k)
-    red += psif[m.cf_face[k]] * (double)m.cf_sign[k];
+  cell_faces(
+      m.cf_ptr[i], m.cf_ptr[i + 1],
+      [&](int32_t k) { return psif[m.cf_face[k]] * (double)m.cf_sign[k]; },
+      [&](double t, bool ok) { red = ok ? red + t : red; });
   div[i] = red / m.vol[i];
 }
 __global__ void __launch_bounds__(kBlock)

@@ -211,34 +211,35 @@ __device__ inline double term_flux(const TermK& t, int64_t i) {
   return t.flux ? t.flux[i] : t.flux_const;
 }
 
-// scheme! (Discretise_1_schemes.jl) for one internal face visit
-__device__ inline void scheme(const MeshK& m, const TermK& t, int f, double ns, double& ac,
-                              double& an) {
+// scheme! (Discretise_1_schemes.jl) for one internal face visit, from the
+// face's values: the term's flux J, the weight w, |E_f| (lapEf) and delta.
+__device__ inline void scheme_vals(const TermK& t, double J, double ns, double w0, double lapEf,
+                                   double delta, double& ac, double& an) {
   const double sign = (double)t.sign;
   switch (t.type) {
     case XCAL_LAPLACIAN_LINEAR: {  // :68-104
-      double ap = (sign * (term_flux(t, f) * m.lapEf[f])) / m.delta[f];
+      double ap = (sign * (J * lapEf)) / delta;
       ac = -ap;
       an = ap;
       return;
     }
     case XCAL_DIV_LINEAR: {  // :114-129
-      double w = 0.5 + ns * (m.weight[f] - 0.5);
-      double ap = sign * (term_flux(t, f) * ns);
+      double w = 0.5 + ns * (w0 - 0.5);
+      double ap = sign * (J * ns);
       ac = ap * w;
       an = ap * (1.0 - w);
       return;
     }
     case XCAL_DIV_UPWIND: {  // :137-147
-      double ap = sign * (term_flux(t, f) * ns);
+      double ap = sign * (J * ns);
       ac = jmax(ap, 0.0);
       an = -jmax(-ap, 0.0);
       return;
     }
     case XCAL_DIV_LUST: {  // :155-175
-      double w = m.weight[f];
+      double w = w0;
       if (ns < 0.0) w = 1.0 - w;
-      double ap = sign * (term_flux(t, f) * ns);
+      double ap = sign * (J * ns);
       double acL = ap * w, anL = ap * (1.0 - w);
       double acU = jmax(ap, 0.0), anU = -jmax(-ap, 0.0);
       ac = 0.75 * acL + 0.25 * acU;
@@ -246,7 +247,7 @@ __device__ inline void scheme(const MeshK& m, const TermK& t, int f, double ns, 
       return;
     }
     case XCAL_DIV_BOUNDED_UPWIND: {  // :183-195
-      double ap = sign * (term_flux(t, f) * ns);
+      double ap = sign * (J * ns);
       ac = jmax(-ap, 0.0);
       an = -jmax(-ap, 0.0);
       return;
@@ -255,6 +256,30 @@ __device__ inline void scheme(const MeshK& m, const TermK& t, int f, double ns, 
       ac = 0.0;
       an = 0.0;
       return;
+  }
+}
+__device__ inline void scheme(const MeshK& m, const TermK& t, int f, double ns, double& ac,
+                              double& an) {
+  scheme_vals(t, term_flux(t, f), ns, m.weight[f], m.lapEf[f], m.delta[f], ac, an);
+}
+
+// Visits a cell's internal faces k0 <= k < k1 (its cf list) kFaceGroup at a
+// time: load(k) forms the face's record (loads addressed by k and by the
+// face id) for the whole group, slots past k1 repeating the last face, before
+// use(record, valid) runs on each in list order.  With no branch between the
+// loads, the compiler issues a group's loads together and waits for each
+// only where it is used; a loop over dependent loads paid two round trips
+// per face.  Six faces cover a hexahedron.
+constexpr int kFaceGroup = 6;
+template <class Load, class Use>
+__device__ inline void cell_faces(int32_t k0, int32_t k1, Load&& load, Use&& use) {
+  for (int32_t kb = k0; kb < k1; kb += kFaceGroup) {
+    using R = decltype(load(k0));
+    R rec[kFaceGroup];
+#pragma unroll
+    for (int u = 0; u < kFaceGroup; ++u) rec[u] = load(min(kb + u, k1 - 1));
+#pragma unroll
+    for (int u = 0; u < kFaceGroup; ++u) use(rec[u], kb + u < k1);
   }
 }
 

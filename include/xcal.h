@@ -304,11 +304,17 @@ int xcal_correct_mass_flux(xcal_field* mdot, const xcal_eqn* peqn, const xcal_fi
 /* correct_velocity! (Solvers_0_functions.jl:126-152) */
 int xcal_correct_velocity(xcal_field* U, const xcal_field* Hv, const xcal_field* gradp,
                           const xcal_field* rD);
+/* nonorthogonal_face_correction (Solvers_1_SIMPLE.jl:267-321): adds
+ * (flux gradf).T_hat of every internal face to b of its owner and subtracts
+ * it from b of its neighbour (gradp: the pressure gradient, flux: rDf) */
+int xcal_nonorthogonal_correction(xcal_eqn* p_eqn, const xcal_field* gradp, const xcal_field* p,
+                                  const xcal_field* flux);
 
 /* ------------------------------------------------------ SIMPLE driver */
 /* The steady incompressible SIMPLE loop of Solvers_1_SIMPLE.jl:103-263 with
- * Gauss or Midpoint gradients and no non-orthogonal correctors,
- * entirely on the device (run!/simple!, Solvers_3_solver_dispatch.jl:120-135). */
+ * Gauss or Midpoint gradients and setup.ncorrectors non-orthogonal
+ * correctors, entirely on the device (run!/simple!,
+ * Solvers_3_solver_dispatch.jl:120-135). */
 typedef struct {
   double nu;
   int32_t div_scheme;   /* XCAL_DIV_* for the momentum convection term */
@@ -324,6 +330,9 @@ typedef struct {
   int32_t time_U;       /* XCAL_TIME_STEADY or _EULER: Time{schemes.U.time}(U)
                            of the momentum equation (Solvers_1_SIMPLE.jl:68-74) */
   double dt;            /* runtime.dt of Time{Euler} (> 0 when time_U is Euler) */
+  int32_t ncorrectors;  /* run!(...; ncorrectors): non-orthogonal correctors of
+                           the pressure (Solvers_1_SIMPLE.jl:194-206,
+                           Solvers_2_PISO.jl:148-165) */
 } xcal_simple_setup;
 enum { XCAL_GRAD_GAUSS = 0, XCAL_GRAD_MIDPOINT = 1 };
 

@@ -1021,6 +1021,44 @@ int orc_correct_mass_flux(const orc_mesh* mp, const double* pnz, const double* p
   });
 }
 
+int orc_nonorth_correction(const orc_mesh* mp, const double* gx, const double* gy,
+                           const double* gz, const double* p, const double* flux, double* b) {
+  return guard([&] {
+    const orc_mesh& m = *mp;
+    std::vector<double> corr(m.nf, 0.0);
+    for (int64_t f = m.nbf; f < m.nf; ++f) {  // _nonorthogonal_face_correction :285-321
+      const int64_t c1 = m.own1[f], c2 = m.own2[f];
+      const V3 xf = m.fc[f], xC = m.cc[c1], xN = m.cc[c2], nr = m.fn[f], e = m.fe[f];
+      const double area = m.area[f];
+      // correction_weight (:323-336)
+      const V3 c1_f{xf.x - xC.x, xf.y - xC.y, xf.z - xC.z};
+      const V3 c1_c2{xN.x - xC.x, xN.y - xC.y, xN.z - xC.z};
+      const double q = dot(c1_f, c1_c2) / dot(c1_c2, c1_c2);
+      const V3 f_prime{xC.x - q * (xC.x - xN.x), xC.y - q * (xC.y - xN.y),
+                       xC.z - q * (xC.z - xN.z)};
+      const double w = norm(V3{xN.x - f_prime.x, xN.y - f_prime.y, xN.z - f_prime.z}) /
+                       norm(c1_c2);
+      const V3 gi{w * gx[c1] + (1.0 - w) * gx[c2], w * gy[c1] + (1.0 - w) * gy[c2],
+                  w * gz[c1] + (1.0 - w) * gz[c2]};
+      const double s = (p[c2] - p[c1]) / m.delta[f] - dot(gi, e);
+      const V3 gf{gi.x + s * e.x, gi.y + s * e.y, gi.z + s * e.z};
+      const V3 dPN = c1_c2;
+      const double nn = norm(nr);
+      const double fac = (nn * nn) / dot(dPN, nr);
+      const V3 T{area * nr.x - (dPN.x * fac) * area, area * nr.y - (dPN.y * fac) * area,
+                 area * nr.z - (dPN.z * fac) * area};
+      const double J = flux[f];
+      corr[f] = dot(V3{J * gf.x, J * gf.y, J * gf.z}, T);  // flux*gradf.T_hat
+    }
+    // b[owner] += c, b[neighbour] -= c; the reference's atomics have no fixed
+    // order, this restatement takes each cell's face-list order
+    for (int64_t i = 0; i < m.n; ++i)
+      for (int64_t fi = m.cf_first[i]; fi <= m.cf_last[i]; ++fi)
+        b[i] += (double)m.cell_nsign[fi] * corr[m.cell_faces[fi]];
+    return 0;
+  });
+}
+
 
 // ------------------------------------------------------------- k-omega
 // turbulence!(KOmegaModel) (RANS_kOmega.jl:175-235) with the wall functions
@@ -1190,6 +1228,7 @@ int run_core(const orc_mesh* mp, const xcal_simple_setup* S, const xcal_turbulen
       throw std::runtime_error("momentum time scheme: SteadyState or Euler");
     if (S->time_U == XCAL_TIME_EULER && !(S->dt > 0.0))
       throw std::runtime_error("Time{Euler} needs runtime.dt > 0");
+    if (S->ncorrectors < 0) throw std::runtime_error("ncorrectors must be >= 0");
     const double dt = S->time_U == XCAL_TIME_EULER ? S->dt : 1.0;
     const int ncorr = inner > 0 ? inner : 1;
     orc_term Uterms[3] = {{S->time_U, 1, nullptr, 1.0},
@@ -1281,6 +1320,23 @@ int run_core(const orc_mesh* mp, const xcal_simple_setup* S, const xcal_turbulen
         const double prelax = inner > 0 && ic == ncorr - 1 ? 1.0 : S->p.relax;
         for (int64_t i = 0; i < n; ++i) p[i] = prev[i] + prelax * (p[i] - prev[i]);
         grad_p();
+        // non-orthogonal correctors (Solvers_1_SIMPLE.jl:194-206; PISO :148-165
+        // also sets the reference, updates the preconditioner and leaves its
+        // last corrector unrelaxed)
+        for (int jc = 0; jc < S->ncorrectors; ++jc) {
+          chk(orc_discretise_scalar(&m, pterms, 1, psrc, 1, p, 1.0, pA.data(), pb.data()));
+          chk(orc_apply_bcs(&m, pterms, 1, S->bc_p, nb, -1, p, nullptr, nullptr, pA.data(),
+                            pb.data()));
+          if (inner > 0 && S->has_pref) orc_set_reference(&m, pA.data(), pb.data(), S->pref, 0);
+          chk(orc_nonorth_correction(&m, gpx.data(), gpy.data(), gpz.data(), p, rDf.data(),
+                                     pb.data()));
+          chk(orc_krylov(&m, S->p.solver, S->p.precon, pA.data(), pb.data(), p, S->p.itmax,
+                         S->p.atol, S->p.rtol, &pit));
+          rp = orc_residual(&m, pA.data(), pb.data(), p);
+          const double nrelax = inner > 0 && jc == S->ncorrectors - 1 ? 1.0 : S->p.relax;
+          for (int64_t i = 0; i < n; ++i) p[i] = prev[i] + nrelax * (p[i] - prev[i]);
+          grad_p();
+        }
         chk(orc_correct_mass_flux(&m, pA.data(), p, rDf.data(), -1, S->bc_p, nb, mdotf.data()));
         for (int64_t i = 0; i < n; ++i) {  // correct_velocity!
           double r = rD[i];

@@ -118,6 +118,12 @@ int orc_correct_mass_flux(const orc_mesh* m, const double* p_nzval, const double
                           const double* pflux, int32_t psign, const xcal_bc* bcs,
                           int32_t nbc, double* mdot);
 
+/* nonorthogonal_face_correction (Solvers_1_SIMPLE.jl:267-336): b of each
+ * internal face's owner += (flux gradf).T_hat, of its neighbour -= (added
+ * per cell in face-list order; the reference's atomics fix no order) */
+int orc_nonorth_correction(const orc_mesh* m, const double* gx, const double* gy,
+                           const double* gz, const double* p, const double* flux, double* b);
+
 /* Whole steady SIMPLE run (Solvers_1_SIMPLE.jl:45-263, laminar).
  * U: 3 arrays (x,y,z) in/out, p in/out.  residuals: 4*iterations
  * [Ux.., Uy.., Uz.., p..]; krylov_iters: 4*iterations (may be NULL);

@@ -106,6 +106,7 @@ def lib():
                                           _pd, _pd, _pd, C.c_int32, _pd, _pi32, _pi32, _pd]
         L.orc_piso_run.argtypes = [C.c_void_p, C.c_void_p, _pd, _pd, _pd, _pd, C.c_int32,
                                    C.c_int32, _pd, _pi32, _pi32, _pd]
+        L.orc_nonorth_correction.argtypes = [C.c_void_p, _pd, _pd, _pd, _pd, _pd, _pd]
         L.orc_set_threads.argtypes = [C.c_int32]
         _lib = L
     return _lib
@@ -291,6 +292,18 @@ class OracleMesh:
     def correct_mass_flux(self, p_nzval, p, pflux, psign, bcs, mdot):
         _chk(lib().orc_correct_mass_flux(self.h, _p(p_nzval), _p(p), _p(pflux), psign, bcs,
                                          len(bcs), _p(mdot)))
+
+    def nonorth_correction(self, grad, p, flux, b):
+        """nonorthogonal_face_correction: b (modified copy returned) plus the
+        non-orthogonal face corrections of gradient `grad` (3, n) and field p
+        with face coefficients `flux`."""
+        b = np.array(b, dtype=np.float64, copy=True)
+        g = [np.ascontiguousarray(grad[c], dtype=np.float64) for c in range(3)]
+        _chk(lib().orc_nonorth_correction(self.h, _p(g[0]), _p(g[1]), _p(g[2]),
+                                          _p(np.ascontiguousarray(p, dtype=np.float64)),
+                                          _p(np.ascontiguousarray(flux, dtype=np.float64)),
+                                          _p(b)))
+        return b
 
     def piso_run(self, setup_c, U, p, steps, inner_loops=2):
         """Transient PISO (Solvers_2_PISO.jl:40-211): `steps` time steps of

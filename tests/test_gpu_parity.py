@@ -511,3 +511,45 @@ def test_simple_midpoint_gradients_match_oracle(x, A, O, meshes):
     eU, ep = rel(model.momentum.U, ref["U"]), rel(model.momentum.p, ref["p"])
     print(f"midpoint SIMPLE box12: U {eU:.2e} p {ep:.2e}")
     assert eU < 1e-8 and ep < 1e-8
+
+
+@pytest.mark.parametrize("name", ["box12", "step", "cavity_hex", "trig40"])
+def test_nonorth_correction_bit_exact(x, A, O, ctx, meshes, name):
+    """nonorthogonal_face_correction on the device against the oracle, bit
+    for bit (both add each cell's corrections in its face-list order)."""
+    mesh = meshes(name)
+    rng = _rng(13)
+    H = x.hip
+    dm = H.DeviceMesh(ctx, mesh)
+    om = O.OracleMesh(mesh, A)
+    n, nf = mesh.n_cells, mesh.n_faces
+    F = _fields(mesh, rng)
+    g = rng.standard_normal((3, n))
+    eq = H.Equation(dm, False)
+    terms = [(A.LAPLACIAN_LINEAR, -1, H.Field(dm, A.FACE_SCALAR).upload(F["gamma"]))]
+    src = H.Field(dm, A.CELL_SCALAR).upload(F["src"])
+    phi = H.Field(dm, A.CELL_SCALAR).upload(F["phi"])
+    eq.discretise(terms, [(src, -1)], phi)
+    _, b0 = eq.download()
+    gf = H.Field(dm, A.CELL_VECTOR).upload(g)
+    fl = H.Field(dm, A.FACE_SCALAR).upload(F["gamma"])
+    H.check(H.lib().xcal_nonorthogonal_correction(eq.h, gf.h, phi.h, fl.h))
+    _, b1 = eq.download()
+    assert np.array_equal(b1, om.nonorth_correction(g, F["phi"], F["gamma"], b0))
+
+
+@pytest.mark.parametrize("nc", [1, 2])
+def test_simple_nonorth_correctors_match_oracle(x, A, O, meshes, nc):
+    """run!(...; ncorrectors) on the jittered box channel: 5 SIMPLE iterations
+    with 1 or 2 non-orthogonal correctors, U and p within 1e-8 of the
+    oracle."""
+    api = x.api
+    mesh = meshes("box12")
+    model, config = channel_case(api, mesh, iterations=5)
+    setup = api.simple_setup(model, config, ncorrectors=nc)
+    ref = O.OracleMesh(mesh, A).simple_run(setup, model.momentum.U, model.momentum.p, 5)
+    res = api.run(model, config, ncorrectors=nc)
+    assert len(res["p"]) == ref["done"]
+    eU, ep = rel(model.momentum.U, ref["U"]), rel(model.momentum.p, ref["p"])
+    print(f"ncorrectors={nc}: U {eU:.2e} p {ep:.2e}")
+    assert eU < 1e-8 and ep < 1e-8

@@ -124,3 +124,19 @@ def test_piso_transient_bfs_reference_case(x, A, O, meshes):
     assert len(res["p"]) == 1000
     outlet = patch_face_average(mesh, model.momentum.U, "outlet")
     assert np.linalg.norm(outlet - np.array([0.25, 0.0, 0.0])) <= 0.05, outlet
+
+
+def test_piso_nonorth_correctors_match_oracle(x, A, O, meshes):
+    """PISO with one non-orthogonal corrector per pressure corrector
+    (Solvers_2_PISO.jl:148-165) on the jittered box channel as a transient
+    case: 5 steps, U and p within 1e-8 of the oracle."""
+    import bench
+    api = x.api
+    mesh = meshes("box12")
+    model, config = bench.transient_channel(api, mesh, 5, 0.05)
+    setup = api.simple_setup(model, config, ncorrectors=1)
+    ref = O.OracleMesh(mesh, A).piso_run(setup, model.momentum.U, model.momentum.p, 5, 2)
+    api.run(model, config, ncorrectors=1)
+    eU, ep = rel(model.momentum.U, ref["U"]), rel(model.momentum.p, ref["p"])
+    print(f"PISO ncorrectors=1: U {eU:.2e} p {ep:.2e}")
+    assert eU < 1e-8 and ep < 1e-8

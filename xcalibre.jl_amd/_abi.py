@@ -75,7 +75,7 @@ class SimpleSetupC(C.Structure):
                 ("pref", C.c_double), ("U", SolverSetupC), ("p", SolverSetupC),
                 ("bc_U", C.POINTER(BC)), ("bc_p", C.POINTER(BC)), ("n_bcs", C.c_int32),
                 ("grad_p", C.c_int32), ("grad_U", C.c_int32), ("time_U", C.c_int32),
-                ("dt", C.c_double)]
+                ("dt", C.c_double), ("ncorrectors", C.c_int32)]
 
 
 GRAD_GAUSS, GRAD_MIDPOINT = 0, 1

@@ -261,8 +261,9 @@ def assign(region, **fields):
     return out
 
 
-def simple_setup(model: Physics, config: Configuration, pref=None) -> A.SimpleSetupC:
-    """xcal_simple_setup for run!(model, config; pref)."""
+def simple_setup(model: Physics, config: Configuration, pref=None,
+                 ncorrectors: int = 0) -> A.SimpleSetupC:
+    """xcal_simple_setup for run!(model, config; pref, ncorrectors)."""
     if not isinstance(model.turbulence, (Laminar, KOmega)):
         raise NotImplementedError("laminar and k-omega SIMPLE are on the device path")
     sU, sp = config.schemes["U"], config.schemes["p"]
@@ -280,7 +281,8 @@ def simple_setup(model: Physics, config: Configuration, pref=None) -> A.SimpleSe
     setup = A.SimpleSetupC(float(model.fluid.nu), div, int(pref is not None),
                            float(pref or 0.0), config.solvers["U"].c(), config.solvers["p"].c(),
                            bU, bp, len(config.boundaries["U"]), _GRAD[sp.gradient],
-                           _GRAD[sU.gradient], TIME[sU.time], float(config.runtime.time_step))
+                           _GRAD[sU.gradient], TIME[sU.time], float(config.runtime.time_step),
+                           int(ncorrectors))
     setup._keep = (bU, bp)
     return setup
 
@@ -321,13 +323,13 @@ def run(model: Physics, config: Configuration, pref=None, ncorrectors: int = 0,
     piso!(model, config; pref) (Solvers_3_solver_dispatch.jl:186-190).
     Returns the residual histories (Ux, Uy, Uz, p) and leaves the final
     fields in model.momentum."""
-    if ncorrectors != 0:
-        raise NotImplementedError("non-orthogonal correctors are not on the device path")
+    if ncorrectors < 0:
+        raise ValueError("ncorrectors must be >= 0")
     if not isinstance(config.hardware.backend, HIPBackend):
         raise TypeError("the device path runs on HIPBackend()")
     ctx = _context(config.hardware.backend.device)
     dmesh = hip.DeviceMesh(ctx, model.domain)
-    sim = hip.Simple(dmesh, simple_setup(model, config, pref))
+    sim = hip.Simple(dmesh, simple_setup(model, config, pref, ncorrectors))
     sim.set_U(model.momentum.U)
     sim.set_p(model.momentum.p)
     ts = turbulence_setup(model, config)

@@ -482,6 +482,55 @@ void correct_mass_flux(const xcal_eqn* pe, const double* p, const double* pflux,
   LAUNCH(k_correct_boundary_mass_flux, m->nbf, mk, b, p, pflux, (double)psign, mdot);
 }
 
+// nonorthogonal_face_correction (Solvers_1_SIMPLE.jl:267-321), per internal
+// face: gradf = gradi + ((p_N - p_P)/delta - gradi.e) e with gradi the
+// weighted cell gradients, and the correction (flux gradf).T_hat, in the
+// reference's operation order (T_hat and the weight precomputed per face).
+__global__ void __launch_bounds__(kBlock)
+    k_nonorth_face(MeshK m, const double* __restrict__ ncw, const double* __restrict__ ntx,
+                   const double* __restrict__ nty, const double* __restrict__ ntz,
+                   const double* __restrict__ gx, const double* __restrict__ gy,
+                   const double* __restrict__ gz, const double* __restrict__ p,
+                   const double* __restrict__ flux, double* __restrict__ corr) {
+  const int64_t f = m.nbf + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= m.nf) return;
+  const int c1 = m.own[f], c2 = m.nei[f];
+  const double w = ncw[f], w1 = 1.0 - w;
+  const double gix = w * gx[c1] + w1 * gx[c2];
+  const double giy = w * gy[c1] + w1 * gy[c2];
+  const double giz = w * gz[c1] + w1 * gz[c2];
+  const double ex = m.ex[f], ey = m.ey[f], ez = m.ez[f];
+  const double s = (p[c2] - p[c1]) / m.delta[f] - (gix * ex + giy * ey + giz * ez);
+  const double J = flux[f];
+  const double fx = J * (gix + s * ex), fy = J * (giy + s * ey), fz = J * (giz + s * ez);
+  corr[f] = fx * ntx[f] + fy * nty[f] + fz * ntz[f];
+}
+
+// b[owner] += correction, b[neighbour] -= correction.  The reference adds
+// with atomics in no fixed order; here each cell adds its faces' corrections
+// in its face-list order.
+__global__ void __launch_bounds__(kBlock)
+    k_nonorth_cell(MeshK m, const double* __restrict__ corr, double* __restrict__ b) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m.n) return;
+  double acc = b[i];
+  cell_faces(
+      m.cf_ptr[i], m.cf_ptr[i + 1],
+      [&](int32_t k) { return (double)m.cf_sign[k] * corr[m.cf_face[k]]; },
+      [&](double t, bool ok) { acc = ok ? acc + t : acc; });
+  b[i] = acc;
+}
+
+void nonorthogonal_correction(xcal_eqn* pe, const double* gx, const double* gy, const double* gz,
+                              const double* p, const double* flux) {
+  const xcal_mesh* m = pe->mesh;
+  if (pe->ncflux.n < m->nf) pe->ncflux.alloc(m->nf);
+  MeshK mk = meshk(m);
+  LAUNCH(k_nonorth_face, m->nf - m->nbf, mk, m->ncw.p, m->ntx.p, m->nty.p, m->ntz.p, gx, gy, gz,
+         p, flux, pe->ncflux.p);
+  LAUNCH(k_nonorth_cell, m->n, mk, pe->ncflux.p, pe->bc(0));
+}
+
 void correct_velocity(const xcal_mesh* m, double* Ux, double* Uy, double* Uz, const double* Hx,
                       const double* Hy, const double* Hz, const double* gx, const double* gy,
                       const double* gz, const double* rD) {
@@ -643,6 +692,19 @@ int xcal_correct_mass_flux(xcal_field* mdot, const xcal_eqn* pe, const xcal_fiel
     check_field(pflux, pe->mesh, XCAL_FACE_SCALAR, "rDf");
     BCsK b = make_bcs(pe->mesh, bcs, nbcs);
     correct_mass_flux(pe, p->comp(0), pflux->comp(0), psign, b, mdot->comp(0));
+    return 0;
+  });
+}
+
+int xcal_nonorthogonal_correction(xcal_eqn* pe, const xcal_field* gradp, const xcal_field* p,
+                                  const xcal_field* flux) {
+  return guarded([&] {
+    XCAL_CHECK(pe && !pe->vector, "the correction applies to a scalar (pressure) equation");
+    check_field(gradp, pe->mesh, XCAL_CELL_VECTOR, "grad");
+    check_field(p, pe->mesh, XCAL_CELL_SCALAR, "p");
+    check_field(flux, pe->mesh, XCAL_FACE_SCALAR, "flux");
+    nonorthogonal_correction(pe, gradp->comp(0), gradp->comp(1), gradp->comp(2), p->comp(0),
+                             flux->comp(0));
     return 0;
   });
 }

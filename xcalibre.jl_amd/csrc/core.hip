@@ -161,13 +161,20 @@ void fill(hipStream_t s, double* p, int64_t n, double v) {
 
 // ---------------------------------------------------------------- meshes
 // Host-side build of one case in device numbering: validation, CSR pattern,
+// Julia's dot and norm of 3-vectors (StaticArrays): (a1 b1 + a2 b2) + a3 b3,
+// and the square root of the same sum of squares
+static inline double dot3(const double* a, const double* b) {
+  return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+static inline double norm3(const double* a) { return std::sqrt(dot3(a, a)); }
+
 // DILU levels, renumbering, sliced-ELL slots and packed factors.
 struct HostMesh {
   int dim = 3, n_patches = 0, nlev_f = 0, nlev_b = 0;
   int64_t n = 0, nf = 0, nbf = 0, nnz = 0, ncf = 0, nzlen = 0, nbcell = 0;
   std::vector<int64_t> patch_first, patch_last;
   std::vector<double> vol, area, weight, delta, nx, ny, nz, ex, ey, ez, sfx, sfy, sfz, lapEf, rix,
-      riy, riz;
+      riy, riz, ncw, ntx, nty, ntz;
   std::vector<int32_t> cf_ptr, cf_face, cf_nbr, cf_slot, diag, own, nei, fslot, bc_cells, bc_ptr,
       bc_face, rbeg, rend, colval, trans, rows_b, posb, Lcol, Lslot, Ucol, Uslot, lev_ptr_f,
       lev_ptr_b, lmeta_f, lmeta_b, cperm, fperm, nzperm, h_rowptr, h_colval;
@@ -245,7 +252,8 @@ static void build_host_mesh(const xcal_mesh_host* h, HostMesh& M) {
   std::vector<int32_t> own(nf), nei(nf);
   std::vector<double> area(h->face_area, h->face_area + nf), weight(h->face_weight, h->face_weight + nf),
       delta(h->face_delta, h->face_delta + nf), nx(nf), ny(nf), nz(nf), ex(nf), ey(nf), ez(nf),
-      sfx(nf), sfy(nf), sfz(nf), lapEf(nf), rix(nf, 0.0), riy(nf, 0.0), riz(nf, 0.0);
+      sfx(nf), sfy(nf), sfz(nf), lapEf(nf), rix(nf, 0.0), riy(nf, 0.0), riz(nf, 0.0),
+      ncw(nf, 0.0), ntx(nf, 0.0), nty(nf, 0.0), ntz(nf, 0.0);
   for (int64_t f = 0; f < nf; ++f) {
     own[f] = (int32_t)(h->face_owner[2 * f] - 1);
     nei[f] = (int32_t)(h->face_owner[2 * f + 1] - 1);
@@ -275,6 +283,25 @@ static void build_host_mesh(const xcal_mesh_host* h, HostMesh& M) {
       rix[f] = rf[0] - 0.5 * (rP[0] + rN[0]);
       riy[f] = rf[1] - 0.5 * (rP[1] + rN[1]);
       riz[f] = rf[2] - 0.5 * (rP[2] + rN[2]);
+      // nonorthogonal_face_correction (Solvers_1_SIMPLE.jl:285-336): the
+      // correction weight and T_hat = Sf - Ef, Ef = dPN |n|^2 / (dPN.n) A
+      const double* n3 = h->face_normal + 3 * f;
+      double c1f[3], c1c2[3], fp[3], d2[3];
+      for (int a = 0; a < 3; ++a) {
+        c1f[a] = rf[a] - rP[a];
+        c1c2[a] = rN[a] - rP[a];
+      }
+      const double q = dot3(c1f, c1c2) / dot3(c1c2, c1c2);
+      for (int a = 0; a < 3; ++a) {
+        fp[a] = rP[a] - q * (rP[a] - rN[a]);
+        d2[a] = rN[a] - fp[a];
+      }
+      ncw[f] = norm3(d2) / norm3(c1c2);
+      const double nn = norm3(n3);
+      const double fac = (nn * nn) / dot3(c1c2, n3);
+      ntx[f] = area[f] * n3[0] - (c1c2[0] * fac) * area[f];
+      nty[f] = area[f] * n3[1] - (c1c2[1] * fac) * area[f];
+      ntz[f] = area[f] * n3[2] - (c1c2[2] * fac) * area[f];
     }
   }
   std::vector<int32_t> fslot(nf, -1);
@@ -500,7 +527,7 @@ static void build_host_mesh(const xcal_mesh_host* h, HostMesh& M) {
     a.swap(t);
   };
   for (auto* a : {&area, &weight, &delta, &nx, &ny, &nz, &ex, &ey, &ez, &sfx, &sfy, &sfz, &lapEf,
-                  &rix, &riy, &riz})
+                  &rix, &riy, &riz, &ncw, &ntx, &nty, &ntz})
     permf(*a);
   std::vector<int32_t> own_n(nf), nei_n(nf), fslot_n(nf, -1);
   for (int64_t f = 0; f < nf; ++f) {
@@ -548,6 +575,10 @@ static void build_host_mesh(const xcal_mesh_host* h, HostMesh& M) {
   M.rix.swap(rix);
   M.riy.swap(riy);
   M.riz.swap(riz);
+  M.ncw.swap(ncw);
+  M.ntx.swap(ntx);
+  M.nty.swap(nty);
+  M.ntz.swap(ntz);
   M.cf_ptr.swap(cf_ptr);
   M.cf_face.swap(cf_face);
   M.cf_nbr.swap(cf_nbr);
@@ -610,7 +641,8 @@ static void union_cases(std::vector<HostMesh>& C, HostMesh& U, std::vector<int64
   U.n = N; U.nf = NF; U.nbf = NBF; U.nnz = NNZ; U.nzlen = NZ; U.ncf = NCF; U.nbcell = NBC;
   for (auto* v : {&U.vol}) v->resize(N);
   for (auto* v : {&U.area, &U.weight, &U.delta, &U.nx, &U.ny, &U.nz, &U.ex, &U.ey, &U.ez, &U.sfx,
-                  &U.sfy, &U.sfz, &U.lapEf, &U.rix, &U.riy, &U.riz})
+                  &U.sfy, &U.sfz, &U.lapEf, &U.rix, &U.riy, &U.riz, &U.ncw, &U.ntx, &U.nty,
+                  &U.ntz})
     v->resize(NF);
   U.own.resize(NF);
   U.nei.resize(NF);
@@ -658,6 +690,7 @@ static void union_cases(std::vector<HostMesh>& C, HostMesh& U, std::vector<int64
       U.ex[g] = M.ex[f]; U.ey[g] = M.ey[f]; U.ez[g] = M.ez[f];
       U.sfx[g] = M.sfx[f]; U.sfy[g] = M.sfy[f]; U.sfz[g] = M.sfz[f]; U.lapEf[g] = M.lapEf[f];
       U.rix[g] = M.rix[f]; U.riy[g] = M.riy[f]; U.riz[g] = M.riz[f];
+      U.ncw[g] = M.ncw[f]; U.ntx[g] = M.ntx[f]; U.nty[g] = M.nty[f]; U.ntz[g] = M.ntz[f];
       U.own[g] = cell(M.own[f]);
       U.nei[g] = cell(M.nei[f]);
       U.fslot[g] = slot(M.fslot[f]);
@@ -767,6 +800,10 @@ static xcal_mesh* upload_mesh(xcal_ctx* ctx, HostMesh& M) {
   upload(m->rix, M.rix, bytes);
   upload(m->riy, M.riy, bytes);
   upload(m->riz, M.riz, bytes);
+  upload(m->ncw, M.ncw, bytes);
+  upload(m->ntx, M.ntx, bytes);
+  upload(m->nty, M.nty, bytes);
+  upload(m->ntz, M.ntz, bytes);
   upload(m->bc_cells, M.bc_cells, bytes);
   upload(m->bc_ptr, M.bc_ptr, bytes);
   upload(m->bc_face, M.bc_face, bytes);
@@ -822,7 +859,7 @@ int64_t prepared_cells(const PreparedMesh& P) { return P.U.n; }
 int64_t prepared_footprint(const PreparedMesh& P, bool turb) {
   const HostMesh& M = P.U;
   int64_t mesh = 0;
-  mesh += 8 * (int64_t)(M.vol.size() + 16 * M.area.size());
+  mesh += 8 * (int64_t)(M.vol.size() + 20 * M.area.size());
   mesh += 4 * (int64_t)(M.cf_ptr.size() + M.cf_face.size() * 3 + M.diag.size() + 3 * M.own.size() +
                         M.bc_cells.size() + M.bc_ptr.size() + M.bc_face.size() + 2 * M.rbeg.size() +
                         2 * M.colval.size() + 2 * M.rows_b.size() + 2 * M.Lcol.size() +

@@ -3,9 +3,9 @@
 // (Solvers_1_SIMPLE.jl:28-42, Solvers_3_solver_dispatch.jl:120-135), and the
 // transient loop of PISO (Solvers_2_PISO.jl:40-211) behind run! of a
 // Transient model (Solvers_3_solver_dispatch.jl:186-190).  Laminar or
-// k-omega (SIMPLE) viscosity, Gauss or Midpoint gradients, no non-orthogonal
-// correctors.  All fields stay resident in HBM; the host only sequences
-// launches and reads back residual scalars.
+// k-omega (SIMPLE) viscosity, Gauss or Midpoint gradients, and `ncorrectors`
+// non-orthogonal correctors of the pressure.  All fields stay resident in
+// HBM; the host only sequences launches and reads back residual scalars.
 #include <algorithm>
 #include <cstring>
 #include <vector>
@@ -110,6 +110,7 @@ void check_setup(const xcal_mesh* m, const xcal_simple_setup* s) {
   XCAL_CHECK(s->time_U == XCAL_TIME_STEADY || s->time_U == XCAL_TIME_EULER,
              "momentum time scheme: SteadyState or Euler");
   XCAL_CHECK(s->time_U != XCAL_TIME_EULER || s->dt > 0.0, "Time{Euler} needs runtime.dt > 0");
+  XCAL_CHECK(s->ncorrectors >= 0, "ncorrectors must be >= 0");
 }
 
 // grad!(∇p, pf, p, BCs.p) (Calculate_0_gradient.jl:105-109)
@@ -413,6 +414,32 @@ int run_impl(xcal_simple* S, int32_t iterations, std::vector<double>& R, std::ve
         const double prelax = inner > 0 && ic == ncorr - 1 ? 1.0 : cfg.p.relax;
         relax_explicit(m, n, S->p->comp(0), S->prev->comp(0), prelax);
         grad_p(S);
+        // non-orthogonal correctors (Solvers_1_SIMPLE.jl:194-206; PISO
+        // :148-165 also sets the reference, updates the preconditioner and
+        // leaves its last corrector unrelaxed)
+        for (int jc = 0; jc < cfg.ncorrectors; ++jc) {
+          discretise(pe, PT, PS, S->p, 1.0);
+          apply_bcs(pe, PT, S->bp, S->p, 0);
+          if (inner > 0 && cfg.has_pref) {
+            if (m->batch)
+              hipLaunchKernelGGL(k_set_reference_cases, dim3(grid_for(K)), dim3(kBlock), 0, st,
+                                 m->diag.p, pe->nzval.p, pe->bc(0), cfg.pref, S->ref_cells.p, K);
+            else
+              set_reference(pe, cfg.pref, m->cperm[0]);
+          }
+          nonorthogonal_correction(pe, S->gradp->comp(0), S->gradp->comp(1), S->gradp->comp(2),
+                                   S->p->comp(0), S->rDf->comp(0));
+          if (inner > 0) precond_update(pe, cfg.p.precon);
+          solve(pe, cfg.p, S->p->comp(0), pe->bc(0));
+          for (int k = 0; k < K; ++k) {
+            R[((size_t)k * 4 + 3) * iterations + it] = cst[k].residual;
+            KIT[((size_t)k * 4 + 3) * iterations + it] = cst[k].niter;
+          }
+          const double nrelax =
+              inner > 0 && jc == cfg.ncorrectors - 1 ? 1.0 : cfg.p.relax;
+          relax_explicit(m, n, S->p->comp(0), S->prev->comp(0), nrelax);
+          grad_p(S);
+        }
         correct_mass_flux(pe, S->p->comp(0), S->rDf->comp(0), -1, S->bp, S->mdotf->comp(0));
         correct_velocity(m, S->U->comp(0), S->U->comp(1), S->U->comp(2), S->Hv->comp(0),
                          S->Hv->comp(1), S->Hv->comp(2), S->gradp->comp(0), S->gradp->comp(1),

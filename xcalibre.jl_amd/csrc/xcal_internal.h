@@ -165,6 +165,7 @@ struct xcal_mesh {
   xcal::DBuf<int32_t> own, nei, fslot;  // fslot: nz index of (owner, neighbour)
   xcal::DBuf<double> area, weight, delta, nx, ny, nz, ex, ey, ez, sfx, sfy, sfz, lapEf;
   xcal::DBuf<double> rix, riy, riz;  // Ri = rf - (rP + rN)/2 of internal faces (Midpoint)
+  xcal::DBuf<double> ncw, ntx, nty, ntz;  // non-orthogonal correction: weight, T_hat
   // boundary faces grouped by cell (ascending face id within a cell)
   int64_t nbcell = 0;
   xcal::DBuf<int32_t> bc_cells, bc_ptr, bc_face;
@@ -256,6 +257,7 @@ struct xcal_eqn {
   xcal::DBuf<double> work;    // solver vectors, 12 x n
   xcal::DBuf<double> kpart;   // reduction partial sums
   xcal::DBuf<char> kstate;    // device-resident Krylov scalars
+  xcal::DBuf<double> ncflux;  // non-orthogonal correction per internal face
   xcal::DBuf<double> fpart;   // fused finalizes: two partial areas
   xcal::DBuf<char> fstate;    // fused finalizes: two more Krylov states
   void* kstate_host = nullptr;  // pinned copy
@@ -349,6 +351,8 @@ void remove_pressure_source(xcal_eqn* e, const double* gx, const double* gy, con
                             int sign);
 void Hop(const xcal_eqn* e, const double* Ux, const double* Uy, const double* Uz, double* Hx,
          double* Hy, double* Hz);
+void nonorthogonal_correction(xcal_eqn* pe, const double* gx, const double* gy, const double* gz,
+                              const double* p, const double* flux);
 void correct_mass_flux(const xcal_eqn* pe, const double* p, const double* pflux, int psign,
                        const BCsK& b, double* mdot);
 void correct_velocity(const xcal_mesh* m, double* Ux, double* Uy, double* Uz, const double* Hx,

@@ -92,6 +92,7 @@ def lib():
             "xcal_H": [_vp, _vp, _vp],
             "xcal_correct_mass_flux": [_vp, _vp, _vp, _vp, C.c_int32, _vp, C.c_int32],
             "xcal_correct_velocity": [_vp, _vp, _vp, _vp],
+            "xcal_nonorthogonal_correction": [_vp, _vp, _vp, _vp],
             "xcal_simple_create": [_vp, C.POINTER(A.SimpleSetupC), C.POINTER(_vp)],
             "xcal_simple_destroy": [_vp],
             "xcal_simple_set_field": [_vp, C.c_int32, _pd],

@@ -23,8 +23,10 @@ __global__ void __launch_bounds__(kBlock)
   if (i >= m.n) return;
   double ac_sum = 0.0;
   // each face's values for every term are loaded for a group of faces at
-  // once (a term without a flux field reads the weights and takes its
-  // constant), then the faces are summed in list order
+  // once, then the faces are summed in list order.  Only Laplacian and
+  // Divergence terms read a face flux; a term without a face flux field
+  // (a constant, or the cell field of a Time or Si term, which has no face
+  // contribution) reads the weights in its place and never uses them.
   struct Rec {
     double ns, w, lapEf, delta, J[kMaxTerms];
     int32_t slot;
@@ -41,7 +43,8 @@ __global__ void __launch_bounds__(kBlock)
         r.delta = m.delta[f];
 #pragma unroll
         for (int t = 0; t < kMaxTerms; ++t) {
-          const double* src = t < T.n && T.t[t].flux ? T.t[t].flux : m.weight;
+          const bool face_flux = t < T.n && T.t[t].flux && face_term(T.t[t].type);
+          const double* src = face_flux ? T.t[t].flux : m.weight;
           r.J[t] = src[f];
         }
         return r;

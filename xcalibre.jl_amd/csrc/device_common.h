@@ -211,6 +211,12 @@ __device__ inline double term_flux(const TermK& t, int64_t i) {
   return t.flux ? t.flux[i] : t.flux_const;
 }
 
+// Terms with a face contribution, whose flux is a face field
+__device__ __host__ inline bool face_term(int type) {
+  return type == XCAL_LAPLACIAN_LINEAR || type == XCAL_DIV_LINEAR || type == XCAL_DIV_UPWIND ||
+         type == XCAL_DIV_LUST || type == XCAL_DIV_BOUNDED_UPWIND;
+}
+
 // scheme! (Discretise_1_schemes.jl) for one internal face visit, from the
 // face's values: the term's flux J, the weight w, |E_f| (lapEf) and delta.
 __device__ inline void scheme_vals(const TermK& t, double J, double ns, double w0, double lapEf,

@@ -12,7 +12,7 @@ from conftest import ROOT
 def declared(header):
     text = (ROOT / "include" / header).read_text()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(xcal[m]?_[a-z0-9_]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(xcal[mw]?_[a-z0-9_]+)\s*\(", text)))
 
 
 def exported(so):
@@ -22,7 +22,8 @@ def exported(so):
 
 
 @pytest.mark.parametrize("header,so", [("xcal.h", "libxcalhip.so"),
-                                       ("xcal_mesh.h", "libxcal_mesh.so")])
+                                       ("xcal_mesh.h", "libxcal_mesh.so"),
+                                       ("xcal_io.h", "libxcal_mesh.so")])
 def test_library_exports_every_declared_symbol(x, header, so):
     path = ROOT / "xcalibre.jl_amd" / so
     C.CDLL(str(path))  # loads on a host without a GPU

@@ -1,6 +1,6 @@
 """Loading of the in-tree native libraries.
 
-``libxcal_mesh.so`` (host mesh builders) and ``libxcalhip.so`` (the gfx950
+``libxcal_mesh.so`` (host mesh builders and result writers) and ``libxcalhip.so`` (the gfx950
 HIP library behind the C ABI in include/xcal.h) are built in-tree by
 ``build.py``.  Nothing here falls back to Python or CPU code: a missing
 library is an error.
@@ -54,5 +54,16 @@ def mesh_lib():
         lib.xcalm_patch_name.restype = _cp
         lib.xcalm_free.argtypes = [_vp]
         lib.xcalm_free.restype = None
+        lib.xcalw_last_error.restype = _cp
+        lib.xcalw_format_double.argtypes = [_dbl, _cp]
+        lib.xcalw_format_double.restype = _i32
+        lib.xcalw_vtk2d_init.argtypes = [_i64, _pdbl, _i64, _pi64, _pi64, C.POINTER(_vp)]
+        lib.xcalw_vtu3d_init.argtypes = [_i64, _pdbl, _i64, _pi64, _pi64, _pdbl, _i64, _pi64,
+                                         _pi64, _pi64, _pdbl, C.POINTER(_vp)]
+        lib.xcalw_write_results.argtypes = [_vp, _i64, _dbl, _cp, _cp, _i32, _vp]
+        lib.xcalw_result_name.argtypes = [_vp, _i64, _dbl, _cp, _cp, _i32]
+        lib.xcalw_result_name.restype = _i32
+        lib.xcalw_free.argtypes = [_vp]
+        lib.xcalw_free.restype = None
         _mesh = lib
     return _mesh

@@ -34,6 +34,7 @@ import numpy as np
 
 from . import _abi as A
 from . import hip
+from .writers import VTK, initialise_writer, write_results
 
 
 # ------------------------------------------------------------- schemes
@@ -315,18 +316,51 @@ def _context(device: int) -> hip.Context:
     return _ctx_cache[device]
 
 
+def save_output(model: Physics, outputWriter, iteration: int, time, config: Configuration,
+                directory=None):
+    """save_output(model, outputWriter, iteration, time, config): U and p,
+    plus k, omega and nut for k-omega (RANS_laminar.jl:91-110,
+    RANS_kOmega.jl:235-260).  Returns the path written."""
+    args = [("U", model.momentum.U), ("p", model.momentum.p)]
+    if isinstance(model.turbulence, KOmega):
+        t = model.turbulence
+        args += [("k", t.k), ("omega", t.omega), ("nut", t.nut)]
+    return write_results(iteration, time, model.domain, outputWriter, config.boundaries, *args,
+                         directory=directory)
+
+
+def _converged(R: np.ndarray, config: Configuration, is3d: bool) -> bool:
+    """The SIMPLE stop test on the last iteration's residuals
+    (Solvers_1_SIMPLE.jl:225-229)."""
+    u, p = config.solvers["U"].convergence, config.solvers["p"].convergence
+    return bool(R[0, -1] <= u and R[1, -1] <= u and (not is3d or R[2, -1] <= u)
+                and R[3, -1] <= p)
+
+
 def run(model: Physics, config: Configuration, pref=None, ncorrectors: int = 0,
-        inner_loops: int = 0):
-    """run!(model, config; pref, ncorrectors) for incompressible flow:
-    SIMPLE for a Steady model (laminar or k-omega), PISO with 2 pressure
-    correctors per time step for a Transient one (laminar), as run! calls
-    piso!(model, config; pref) (Solvers_3_solver_dispatch.jl:186-190).
+        inner_loops: int = 0, output=VTK(), directory=None):
+    """run!(model, config; output, pref, ncorrectors) for incompressible
+    flow: SIMPLE for a Steady model (laminar or k-omega), PISO with 2
+    pressure correctors per time step for a Transient one (laminar), as run!
+    calls piso!(model, config; pref) (Solvers_3_solver_dispatch.jl:186-190).
     Returns the residual histories (Ux, Uy, Uz, p) and leaves the final
-    fields in model.momentum."""
+    fields in model.momentum.
+
+    Results are written as the reference writes them: with
+    config.runtime.write_interval w >= 0, after every iteration that is a
+    multiple of w, and (SIMPLE) at the iteration that converges
+    (Solvers_1_SIMPLE.jl:234-258, Solvers_2_PISO.jl:205-208); w < 0 writes
+    nothing.  The device loop runs the iterations between two writes in one
+    call and stays resident across calls, so the iterates are those of one
+    uninterrupted run.  `directory` (default: the working directory) is
+    where the files go."""
     if ncorrectors < 0:
         raise ValueError("ncorrectors must be >= 0")
     if not isinstance(config.hardware.backend, HIPBackend):
         raise TypeError("the device path runs on HIPBackend()")
+    wi = config.runtime.write_interval
+    if wi == 0:
+        raise ZeroDivisionError("write_interval 0: iteration % write_interval is undefined")
     ctx = _context(config.hardware.backend.device)
     dmesh = hip.DeviceMesh(ctx, model.domain)
     sim = hip.Simple(dmesh, simple_setup(model, config, pref, ncorrectors))
@@ -339,16 +373,37 @@ def run(model: Physics, config: Configuration, pref=None, ncorrectors: int = 0,
         sim.set_scalar("k", t.k)
         sim.set_scalar("omega", t.omega)
         sim.set_scalar("nut", t.nut)
-    if isinstance(model.time, Transient):
+    transient = isinstance(model.time, Transient)
+    if transient and ts is not None:
+        raise NotImplementedError("transient k-omega is not on the device path")
+    writer = initialise_writer(output, model.domain) if wi > 0 else None
+
+    def fetch():
+        model.momentum.U[:] = sim.get("U")
+        model.momentum.p[:] = sim.get("p")
         if ts is not None:
-            raise NotImplementedError("transient k-omega is not on the device path")
-        out = sim.run_piso(config.runtime.iterations, 2)
-    else:
-        out = sim.run(config.runtime.iterations)
-    model.momentum.U[:] = sim.get("U")
-    model.momentum.p[:] = sim.get("p")
-    if ts is not None:
-        for name in ("k", "omega", "nut"):
-            getattr(model.turbulence, name)[:] = sim.get(name)
-    r = out["residuals"]
+            for name in ("k", "omega", "nut"):
+                getattr(model.turbulence, name)[:] = sim.get(name)
+
+    iterations = config.runtime.iterations
+    step = iterations if writer is None else wi
+    parts, it, time = [], 0, 0.0
+    while it < iterations:
+        chunk = min(step - it % step, iterations - it)
+        out = sim.run_piso(chunk, 2) if transient else sim.run(chunk)
+        parts.append(out["residuals"])
+        done = out["done"]
+        if transient:
+            for _ in range(done):
+                time += config.runtime.time_step
+        it += done
+        stop = not transient and done > 0 and (
+            done < chunk or _converged(out["residuals"], config, model.domain.dim == 3))
+        if writer is not None and (stop or it % wi == 0):
+            fetch()
+            save_output(model, writer, it, time if transient else it, config, directory)
+        if stop or done < chunk:
+            break
+    fetch()
+    r = np.concatenate(parts, axis=1) if parts else np.zeros((4, 0))
     return dict(Ux=r[0], Uy=r[1], Uz=r[2], p=r[3])

@@ -1,6 +1,6 @@
 """Build the in-tree native libraries of the package.
 
-* ``libxcal_mesh.so`` -- host mesh builders (g++).
+* ``libxcal_mesh.so`` -- host mesh builders and VTK result writers (g++).
 * ``libxcalhip.so``   -- the gfx950 HIP library behind include/xcal.h (hipcc).
 
 All HIP code is compiled for gfx950 with FMA contraction off (the assembly
@@ -45,7 +45,7 @@ def build_mesh() -> Path:
     out = PKG / "libxcal_mesh.so"
     tmp = out.with_suffix(".so.tmp")
     _run(["g++", "-O2", "-ffp-contract=off", "-fPIC", "-shared", "-std=c++17", f"-I{INC}",
-          str(CSRC / "mesh" / "meshgen.cpp"), "-o", str(tmp)])
+          str(CSRC / "mesh" / "meshgen.cpp"), str(CSRC / "io" / "writers.cpp"), "-o", str(tmp)])
     os.replace(tmp, out)
     return out
 

@@ -1,6 +1,7 @@
 """Result output during device runs.
 
-run(model, config) writes VTK files where the reference's run! does: after
+run(model, config) writes VTK files (or an OpenFOAM case with
+output=OpenFOAM()) where the reference's run! does: after
 every iteration that is a multiple of write_interval, and at the SIMPLE
 iteration that converges (Solvers_1_SIMPLE.jl:234-258,
 Solvers_2_PISO.jl:205-208).  The iterations between two writes run as one
@@ -118,3 +119,26 @@ def test_3d_run_writes_vtu(x, meshes, tmp_path):
     for name, m in (("iteration_3.vtu", at3), ("iteration_6.vtu", ref)):
         d = vtu_cell_data(tmp_path / name)
         assert np.array_equal(d["U"], m.momentum.U) and np.array_equal(d["p"], m.momentum.p)
+
+
+def foam_internal(path):
+    body = path.read_text().split("(\n", 1)[1].split("\n);\n")[0].split("\n")
+    v = np.array([[float(t) for t in ln.strip("()").split()] for ln in body])
+    return v.T if v.shape[1] == 3 else v[:, 0]
+
+
+def test_3d_run_writes_openfoam_case(x, meshes, tmp_path):
+    api = x.api
+    mesh = meshes("box12")
+
+    def make():
+        return channel_case(api, mesh, iterations=6)
+    at3, _ = run_case(api, make, -1, 3)
+    model, config = make()
+    api.run(model, interval(config, 3, 6), output=api.OpenFOAM(), directory=tmp_path)
+    assert sorted(p.name for p in tmp_path.iterdir()) == ["3", "6", "XCALibre.foam", "constant"]
+    back = x.mesh.FOAM3D_mesh(tmp_path / "constant" / "polyMesh")
+    assert np.array_equal(back.face_nodes, mesh.face_nodes)
+    for d, m in (("3", at3), ("6", model)):
+        assert np.array_equal(foam_internal(tmp_path / d / "U"), m.momentum.U)
+        assert np.array_equal(foam_internal(tmp_path / d / "p"), m.momentum.p)

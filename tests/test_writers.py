@@ -189,3 +189,86 @@ def test_result_names_and_errors(W, meshes, tmp_path):
                         directory=tmp_path / "missing")
     with pytest.raises(ValueError, match="writer and mesh differ"):
         W.write_results(1, 1, m3, w2, None)
+
+
+def foam_list(path):
+    """The entries between the outer ( and ) of an OpenFOAM list file."""
+    L = path.read_text().split("\n")
+    i = L.index("(")
+    n = int(L[i - 1])
+    j = len(L) - 1 - L[::-1].index(")")
+    assert j - i - 1 == n
+    return L[i + 1:j]
+
+
+@pytest.mark.parametrize("name", ["cavity_hex", "box12"])
+def test_foam_case_reads_back(W, x, meshes, tmp_path, name):
+    """initialise_writer(OpenFOAM(), mesh) (OpenFOAM_writer.jl:1-185): the
+    written polyMesh, read back by FOAM3D_mesh, gives the same mesh --
+    topology exactly, points to the 6 digits of the reference's %g."""
+    m = meshes(name)
+    W.initialise_writer(W.OpenFOAM(), m, directory=tmp_path / "case")
+    assert (tmp_path / "case" / "XCALibre.foam").read_bytes() == b""
+    pm = tmp_path / "case" / "constant" / "polyMesh"
+    faces = foam_list(pm / "faces")
+    assert faces[0].endswith(" )") and faces[0].split("(")[0] == "4"
+    owner = foam_list(pm / "owner")
+    assert len(owner) == m.n_faces and len(foam_list(pm / "neighbour")) == m.n_ifaces
+    assert f'note        "nPoints: {m.n_nodes} nCells: {m.n_cells} nFaces: {m.n_faces} ' \
+           f'nInternalFaces: {m.n_ifaces}";' in (pm / "owner").read_text()
+    r = x.mesh.FOAM3D_mesh(pm)
+    assert (r.n_cells, r.n_faces, r.n_bfaces, r.n_nodes) == (m.n_cells, m.n_faces, m.n_bfaces,
+                                                             m.n_nodes)
+    assert r.patch_names == m.patch_names
+    for k in ("face_nodes_range", "face_nodes", "face_owner", "patch_range", "cell_faces",
+              "cell_nodes", "cell_nodes_range", "cell_neighbours", "boundary_cellsID"):
+        assert np.array_equal(getattr(r, k), getattr(m, k)), k
+    assert np.allclose(r.node_coords, m.node_coords, rtol=1e-5, atol=0)
+    txt = [ln.strip("()").split() for ln in foam_list(pm / "points")]
+    assert [[f"{v:g}" for v in row] for row in m.node_coords.reshape(-1, 3)] == txt
+
+
+def test_foam_fields_and_boundary_entries(W, x, meshes, tmp_path):
+    """write_results for the OpenFOAM writer (OpenFOAM_writer.jl:191-273) and
+    the boundary entries of _foam_boundary_entry (:277-359)."""
+    api = x.api
+    m = meshes("box12")
+    names = m.patch_names
+    assert len(names) == 3
+    BCs = api.assign(region=m, U=[api.Wall(names[0], [1, 0, 0]), api.Dirichlet(names[1], [0, 2.5, 0]),
+                                  api.Empty(names[2])],
+                     p=[api.Dirichlet(names[0], 1e-7), api.Wall(names[1]), api.Symmetry(names[2])],
+                     k=[api.KWallFunction(names[0]), api.Neumann(names[1], 0.0),
+                        api.Extrapolated(names[2])])
+    w = W.initialise_writer(W.OpenFOAM(), m, directory=tmp_path)
+    rng = np.random.default_rng(3)
+    U, p, k = rng.standard_normal((3, m.n_cells)), rng.standard_normal(m.n_cells), rng.random(m.n_cells)
+    assert W.write_results(20, 20, m, w, BCs, ("U", U), ("p", p), ("k", k)) == f"{tmp_path}/20"
+    assert W.write_results(4, 0.125, m, w, BCs, ("p", p)) == f"{tmp_path}/0.12500000"
+    d = tmp_path / "20"
+    Ut = (d / "U").read_text()
+    assert Ut.startswith("FoamFile\n{\n    version     2.0;\n    format      ascii;\n"
+                         "    class       volVectorField;\n    location    \"20\";\n"
+                         "    object      U;\n}\n\ninternalField   nonuniform List<vector>\n")
+    body = Ut.split("(\n", 1)[1].split("\n);\n")[0].split("\n")
+    assert np.array_equal(np.array([[float(t) for t in ln.strip("()").split()] for ln in body]), U.T)
+    assert Ut.endswith(
+        "boundaryField\n{\n"
+        f"\t{names[0]}\n\t{{\n\t\ttype fixedValue;\n\t\tvalue uniform (1.0 0.0 0.0);\n\t}}\n\n"
+        f"\t{names[1]}\n\t{{\n\t\ttype fixedValue;\n\t\tvalue uniform (0.0 2.5 0.0);\n\t}}\n\n"
+        f"\t{names[2]}\n\t{{\n\t\ttype empty;\n\t}}\n\n}}\n")
+    pt = (d / "p").read_text()
+    assert "class       volScalarField;" in pt and "internalField   nonuniform List<scalar>\n" in pt
+    body = pt.split("(\n", 1)[1].split("\n);\n")[0].split("\n")
+    assert np.array_equal(np.array([float(t) for t in body]), p)
+    zg = "\t{\n\t\ttype zeroGradient;\n\t}\n\n"
+    assert pt.endswith("boundaryField\n{\n"
+                       f"\t{names[0]}\n\t{{\n\t\ttype fixedValue;\n\t\tvalue uniform 1.0e-7;\n\t}}\n\n"
+                       f"\t{names[1]}\n{zg}\t{names[2]}\n{zg}}}\n")
+    assert (d / "k").read_text().endswith(
+        "boundaryField\n{\n" + "".join(f"\t{n}\n{zg}" for n in names) + "}\n")
+    assert 'location    "4";' in (tmp_path / "0.12500000" / "p").read_text()
+    with pytest.raises(TypeError, match="ScalarField or VectorField"):
+        W.write_results(1, 1, m, w, BCs, ("U", np.zeros((9, m.n_cells))))
+    with pytest.raises(RuntimeError, match="only be used for 3D"):
+        W.initialise_writer(W.OpenFOAM(), meshes("quad40"), directory=tmp_path / "2d")

@@ -63,6 +63,11 @@ def mesh_lib():
         lib.xcalw_write_results.argtypes = [_vp, _i64, _dbl, _cp, _cp, _i32, _vp]
         lib.xcalw_result_name.argtypes = [_vp, _i64, _dbl, _cp, _cp, _i32]
         lib.xcalw_result_name.restype = _i32
+        lib.xcalw_foam_init.argtypes = [_cp, _i32, _i64, _pdbl, _i64, _i64, _i64, _pi64, _pi64,
+                                        _pi64, _i32, C.POINTER(_cp), _pi64, C.POINTER(_vp)]
+        lib.xcalw_foam_write_results.argtypes = [_vp, _i64, _dbl, _i32, _vp]
+        lib.xcalw_foam_time_dir.argtypes = [_i64, _dbl, _cp, _i32]
+        lib.xcalw_foam_time_dir.restype = _i32
         lib.xcalw_free.argtypes = [_vp]
         lib.xcalw_free.restype = None
         _mesh = lib

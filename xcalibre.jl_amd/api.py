@@ -34,7 +34,7 @@ import numpy as np
 
 from . import _abi as A
 from . import hip
-from .writers import VTK, initialise_writer, write_results
+from .writers import VTK, OpenFOAM, initialise_writer, write_results  # noqa: F401
 
 
 # ------------------------------------------------------------- schemes
@@ -353,7 +353,8 @@ def run(model: Physics, config: Configuration, pref=None, ncorrectors: int = 0,
     nothing.  The device loop runs the iterations between two writes in one
     call and stays resident across calls, so the iterates are those of one
     uninterrupted run.  `directory` (default: the working directory) is
-    where the files go."""
+    where the files go; with output=OpenFOAM() it is the case directory,
+    whose constant/polyMesh is written at the start of the run."""
     if ncorrectors < 0:
         raise ValueError("ncorrectors must be >= 0")
     if not isinstance(config.hardware.backend, HIPBackend):
@@ -376,7 +377,12 @@ def run(model: Physics, config: Configuration, pref=None, ncorrectors: int = 0,
     transient = isinstance(model.time, Transient)
     if transient and ts is not None:
         raise NotImplementedError("transient k-omega is not on the device path")
-    writer = initialise_writer(output, model.domain) if wi > 0 else None
+    if isinstance(output, OpenFOAM):  # writes constant/polyMesh now, as run! does
+        writer = initialise_writer(output, model.domain, directory)
+    else:
+        writer = initialise_writer(output, model.domain) if wi > 0 else None
+    if wi < 0:
+        writer = None
 
     def fetch():
         model.momentum.U[:] = sim.get("U")

@@ -1,12 +1,16 @@
-// VTK result writers (include/xcal_io.h), host C++.
+// Result writers (include/xcal_io.h), host C++.
 //
 // File layouts follow the reference's writers byte for byte:
 //   2D legacy ASCII  -- src/IOFormats/VTK/VTK_writer.jl:6-124
 //   3D XML polyhedra -- src/IOFormats/VTK/VTK_writer_3D.jl:16-224
-// The whole file is formatted into one buffer and written with one fwrite.
+//   OpenFOAM case    -- src/IOFormats/OpenFOAM/OpenFOAM_writer.jl:1-359
+// Each file is formatted into one buffer and written with one fwrite.
 #include "xcal_io.h"
 
+#include "xcal.h"
+
 #include <charconv>
+#include <filesystem>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -16,9 +20,12 @@
 
 struct xcalw_writer {
   int dim = 0;
+  bool foam = false;
   int64_t n_cells = 0;
   std::string header;  // 2D: everything up to and including "CELL_DATA n\n"
   std::string footer;  // 3D only
+  std::string dir;                  // OpenFOAM: the case directory
+  std::vector<std::string> patches; // OpenFOAM: patch names
 };
 
 namespace {
@@ -318,6 +325,182 @@ void write_field(Buf& b, int dim, int64_t n, const xcalw_field& f) {
   if (dim == 3) b.t("     </DataArray>\n");
 }
 
+
+std::string join_path(const std::string& dir, const std::string& rel) {
+  return dir.empty() ? rel : dir + "/" + rel;
+}
+
+void write_file(const std::string& path, const std::string& text) {
+  FILE* fp = std::fopen(path.c_str(), "wb");
+  need(fp != nullptr, "cannot open " + path + " for writing");
+  bool ok = std::fwrite(text.data(), 1, text.size(), fp) == text.size();
+  ok = (std::fclose(fp) == 0) && ok;
+  need(ok, "write to " + path + " failed");
+}
+
+void make_dirs(const std::string& path) {
+  std::error_code ec;
+  std::filesystem::create_directories(path, ec);
+  need(!ec, "cannot create directory " + path + ": " + ec.message());
+}
+
+// The FoamFile header the reference prints with println of a triple-quoted
+// string: the block, then a blank line.
+std::string foam_header(const char* cls, const std::string& location, const char* object,
+                        const std::string& note = "") {
+  std::string h = "FoamFile\n{\n    version     2.0;\n    format      ascii;\n    class       ";
+  h += cls;
+  h += ";\n";
+  if (!note.empty()) h += "    note        \"" + note + "\";\n";
+  h += "    location    " + location + ";\n    object      " + object + ";\n}\n\n";
+  return h;
+}
+
+// OpenFOAM_writer.jl:1-185
+void build_foam(xcalw_writer* w, const std::string& dir, int64_t n_nodes, const double* X,
+                int64_t n_cells, int64_t n_faces, int64_t n_bf, const int64_t* fnr,
+                const int64_t* fn, const int64_t* fo, int32_t n_patches,
+                const char* const* names, const int64_t* pr) {
+  need(n_bf >= 0 && n_bf <= n_faces, "bad boundary face count");
+  for (int64_t f = 0; f < n_faces; ++f) {
+    need(fnr[2 * f] >= 1 && fnr[2 * f + 1] >= fnr[2 * f], "bad face_nodes_range");
+    need(fo[2 * f] >= 1 && fo[2 * f] <= n_cells && fo[2 * f + 1] >= 1 &&
+             fo[2 * f + 1] <= n_cells, "face owner out of range");
+  }
+  for (int32_t i = 0; i < n_patches; ++i)
+    need(names[i] && pr[2 * i] >= 1 && pr[2 * i + 1] <= n_bf, "bad patch");
+  w->dir = dir;
+  for (int32_t i = 0; i < n_patches; ++i) w->patches.emplace_back(names[i]);
+  if (!dir.empty()) make_dirs(dir);
+  // touch("XCALibre.foam")
+  {
+    const std::string t = join_path(dir, "XCALibre.foam");
+    FILE* fp = std::fopen(t.c_str(), "ab");
+    need(fp != nullptr, "cannot create " + t);
+    std::fclose(fp);
+  }
+  const std::string pm = join_path(dir, "constant/polyMesh");
+  make_dirs(pm);
+  const int64_t n_if = n_faces - n_bf;
+  const std::string loc = "\"constant/polyMesh\"";
+  {
+    Buf b;
+    b.s.reserve((size_t)n_nodes * 40 + 512);
+    b.t(foam_header("vectorField", loc, "points")).i(n_nodes).t("\n(\n");
+    char line[128];
+    for (int64_t i = 0; i < n_nodes; ++i) {
+      const int k = std::snprintf(line, sizeof line, "(%g %g %g)\n", X[3 * i], X[3 * i + 1],
+                                  X[3 * i + 2]);
+      b.s.append(line, (size_t)k);
+    }
+    b.t(")\n");
+    write_file(pm + "/points", b.s);
+  }
+  // internal faces first, then the boundary faces (OpenFOAM numbering)
+  auto face_order = [&](auto&& emit) {
+    for (int64_t f = n_bf; f < n_faces; ++f) emit(f);
+    for (int64_t f = 0; f < n_bf; ++f) emit(f);
+  };
+  {
+    Buf b;
+    b.s.reserve((size_t)n_faces * 40 + 512);
+    b.t(foam_header("faceList", loc, "faces")).i(n_faces).t("\n(\n");
+    face_order([&](int64_t f) {
+      b.i(fnr[2 * f + 1] - fnr[2 * f] + 1).t("(");
+      for (int64_t k = fnr[2 * f]; k <= fnr[2 * f + 1]; ++k) b.i(fn[k - 1] - 1).t(" ");
+      b.t(")\n");
+    });
+    b.t(")\n");
+    write_file(pm + "/faces", b.s);
+  }
+  const std::string note = "nPoints: " + std::to_string(n_nodes) + " nCells: " +
+                           std::to_string(n_cells) + " nFaces: " + std::to_string(n_faces) +
+                           " nInternalFaces: " + std::to_string(n_if);
+  {
+    Buf b;
+    b.t(foam_header("labelList", loc, "owner", note)).i(n_faces).t("\n(\n");
+    face_order([&](int64_t f) { b.i(fo[2 * f] - 1).t("\n"); });
+    b.t(")\n");
+    write_file(pm + "/owner", b.s);
+  }
+  {
+    Buf b;
+    b.t(foam_header("labelList", loc, "neighbour", note)).i(n_if).t("\n(\n");
+    for (int64_t f = n_bf; f < n_faces; ++f) b.i(fo[2 * f + 1] - 1).t("\n");
+    b.t(")\n");
+    write_file(pm + "/neighbour", b.s);
+  }
+  {
+    Buf b;
+    b.t(foam_header("polyBoundaryMesh", loc, "boundary")).i(n_patches).t("\n(\n");
+    for (int32_t i = 0; i < n_patches; ++i) {
+      b.t(names[i]).t("\n{\n    type            patch;\n    nFaces          ");
+      b.i(pr[2 * i + 1] - pr[2 * i] + 1).t(";\n    startFace       ");
+      b.i(pr[2 * i] + n_if - 1).t(";\n}\n");
+    }
+    b.t(")\n");
+    write_file(pm + "/boundary", b.s);
+  }
+}
+
+std::string foam_time_dir(int64_t iteration, double time) {
+  if ((double)iteration == time) return std::to_string(iteration);
+  char b[64];
+  std::snprintf(b, sizeof b, "%.8f", time);
+  return b;
+}
+
+// _foam_boundary_entry (OpenFOAM_writer.jl:277-359)
+void foam_bc_entry(Buf& b, const xcalw_foam_bc& bc, int ncomp) {
+  const bool fixed = bc.kind == XCAL_BC_DIRICHLET || (bc.kind == XCAL_BC_WALL && ncomp == 3);
+  if (bc.kind == XCAL_BC_EMPTY) {
+    b.t("\t{\n\t\ttype empty;\n\t}\n");
+  } else if (fixed) {
+    b.t("\t{\n\t\ttype fixedValue;\n\t\tvalue uniform ");
+    if (ncomp == 1)
+      b.d(bc.value[0]);
+    else
+      b.t("(").d(bc.value[0]).t(" ").d(bc.value[1]).t(" ").d(bc.value[2]).t(")");
+    b.t(";\n\t}\n");
+  } else {
+    b.t("\t{\n\t\ttype zeroGradient;\n\t}\n");
+  }
+}
+
+// OpenFOAM_writer.jl:191-273
+void write_foam_field(const xcalw_writer* w, const std::string& tdir, int64_t iteration,
+                      const xcalw_foam_field& f) {
+  need(f.label && f.values, "field without label or values");
+  need(f.ncomp == 1 || f.ncomp == 3,
+       "Input data should be a ScalarField or VectorField e.g. (\"U\", U)");
+  need(f.n_bcs >= 0 && (f.n_bcs == 0 || f.bcs), "bad boundary conditions");
+  const int64_t n = w->n_cells;
+  Buf b;
+  b.s.reserve((size_t)n * 26 * f.ncomp + 1024);
+  b.t(foam_header(f.ncomp == 1 ? "volScalarField" : "volVectorField",
+                  "\"" + std::to_string(iteration) + "\"", f.label));
+  b.t(f.ncomp == 1 ? "internalField   nonuniform List<scalar>\n"
+                   : "internalField   nonuniform List<vector>\n");
+  b.i(n).t("\n(\n");
+  const double* v = f.values;
+  for (int64_t i = 0; i < n; ++i) {
+    if (f.ncomp == 1)
+      b.d(v[i]).t("\n");
+    else
+      b.t("(").d(v[i]).t(" ").d(v[n + i]).t(" ").d(v[2 * n + i]).t(")\n");
+  }
+  b.t(");\n").t("boundaryField\n").t("{\n");
+  for (int32_t k = 0; k < f.n_bcs; ++k) {
+    const xcalw_foam_bc& bc = f.bcs[k];
+    need(bc.patch >= 0 && bc.patch < (int32_t)w->patches.size(), "bad patch index");
+    b.t("\t").t(w->patches[bc.patch]).t("\n");
+    foam_bc_entry(b, bc, f.ncomp);
+    b.t("\n");
+  }
+  b.t("}\n");
+  write_file(tdir + "/" + f.label, b.s);
+}
+
 }  // namespace
 
 extern "C" {
@@ -383,7 +566,7 @@ int xcalw_write_results(const xcalw_writer* w, int64_t iteration, double time,
                         const char* suffix, const char* directory, int32_t n_fields,
                         const xcalw_field* fields) {
   return guarded([&] {
-    need(w && n_fields >= 0 && (n_fields == 0 || fields), "bad arguments");
+    need(w && !w->foam && n_fields >= 0 && (n_fields == 0 || fields), "bad arguments");
     std::string path = result_name(w, iteration, time, suffix);
     if (directory && *directory) path = std::string(directory) + "/" + path;
     Buf b;
@@ -398,6 +581,54 @@ int xcalw_write_results(const xcalw_writer* w, int64_t iteration, double time,
     ok = ok && std::fwrite(w->footer.data(), 1, w->footer.size(), fp) == w->footer.size();
     ok = (std::fclose(fp) == 0) && ok;
     need(ok, "write to " + path + " failed");
+  });
+}
+
+int xcalw_foam_init(const char* directory, int32_t dim, int64_t n_nodes,
+                    const double* node_coords, int64_t n_cells, int64_t n_faces,
+                    int64_t n_bfaces, const int64_t* face_nodes_range,
+                    const int64_t* face_nodes, const int64_t* face_owner, int32_t n_patches,
+                    const char* const* patch_names, const int64_t* patch_range,
+                    xcalw_writer** out) {
+  return guarded([&] {
+    need(dim == 3,
+         "The OpenFOAM format can only be used for 3D simulations. Use `output=VTK()` "
+         "instead.");
+    need(out && node_coords && face_nodes_range && face_nodes && face_owner && n_nodes >= 0 &&
+             n_cells >= 0 && n_faces >= 0 && n_patches >= 0 &&
+             (n_patches == 0 || (patch_names && patch_range)),
+         "bad arguments");
+    auto* w = new xcalw_writer;
+    try {
+      w->dim = 3;
+      w->foam = true;
+      w->n_cells = n_cells;
+      build_foam(w, directory ? directory : "", n_nodes, node_coords, n_cells, n_faces,
+                 n_bfaces, face_nodes_range, face_nodes, face_owner, n_patches, patch_names,
+                 patch_range);
+    } catch (...) {
+      delete w;
+      throw;
+    }
+    *out = w;
+  });
+}
+
+int32_t xcalw_foam_time_dir(int64_t iteration, double time, char* buf, int32_t cap) {
+  if (!buf) return -1;
+  const std::string d = foam_time_dir(iteration, time);
+  if ((int64_t)d.size() + 1 > cap) return -1;
+  std::memcpy(buf, d.c_str(), d.size() + 1);
+  return (int32_t)d.size();
+}
+
+int xcalw_foam_write_results(const xcalw_writer* w, int64_t iteration, double time,
+                             int32_t n_fields, const xcalw_foam_field* fields) {
+  return guarded([&] {
+    need(w && w->foam && n_fields >= 0 && (n_fields == 0 || fields), "bad arguments");
+    const std::string tdir = join_path(w->dir, foam_time_dir(iteration, time));
+    make_dirs(tdir);
+    for (int32_t i = 0; i < n_fields; ++i) write_foam_field(w, tdir, iteration, fields[i]);
   });
 }
 

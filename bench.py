@@ -436,11 +436,13 @@ def hbm_traffic(kernel, workload):
 def kernel_bytes(name, n, nnz, vb=8):
     """Algorithmic HBM bytes of one timed unit (FP64 values, int32 indices;
     every array element counted once, gathers included once, slice padding
-    not counted).  cg_spmv is one launch; dilu_fwd/dilu_bwd are one whole
-    level-synchronous sweep (one launch per large level)."""
+    not counted).  cg_spmv is one launch, counted as a CSR SpMV whatever the
+    storage: nnz (value + index) + (n + 1) row pointers + x once + y;
+    dilu_fwd/dilu_bwd are one whole level-synchronous sweep (one launch per
+    large level)."""
     nlow = (nnz - n) // 2
-    if name == "cg_spmv":   # row begin/end; colval+nz; p read (gathers included once); q written
-        return 8 * n + (4 + vb) * nnz + vb * 2 * n
+    if name == "cg_spmv":   # colval+nz; n+1 row pointers; p read (gathers once); q written
+        return (4 + vb) * nnz + 4 * (n + 1) + vb * 2 * n
     if name == "dilu_fwd":  # row begin, diag; lower colval+nz; rD and y gathered; y written;
         # p, q read; x, r read+written
         return 8 * n + 12 * nlow + 8 * n * (1 + 1 + 1 + 2 + 4)

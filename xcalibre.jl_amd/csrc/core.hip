@@ -176,7 +176,7 @@ struct HostMesh {
   std::vector<double> vol, area, weight, delta, nx, ny, nz, ex, ey, ez, sfx, sfy, sfz, lapEf, rix,
       riy, riz, ncw, ntx, nty, ntz;
   std::vector<int32_t> cf_ptr, cf_face, cf_nbr, cf_slot, diag, own, nei, fslot, bc_cells, bc_ptr,
-      bc_face, rbeg, rend, colval, trans, rows_b, posb, Lcol, Lslot, Ucol, Uslot, lev_ptr_f,
+      bc_face, rbeg, rend, sinfo, colval, trans, rows_b, posb, Lcol, Lslot, Ucol, Uslot, lev_ptr_f,
       lev_ptr_b, lmeta_f, lmeta_b, cperm, fperm, nzperm, h_rowptr, h_colval;
   std::vector<int8_t> cf_sign, bpatch;
 };
@@ -417,7 +417,7 @@ static void build_host_mesh(const xcal_mesh_host* h, HostMesh& M) {
   // row c sits at rbeg[c] + t*kSlice, so a wavefront reading entry t of 64
   // rows reads one contiguous segment.  Padding entries are never read.
   const int64_t nslice = (n + kSlice - 1) / kSlice;
-  std::vector<int32_t> rbeg(n), rend(n);
+  std::vector<int32_t> rbeg(n), rend(n), sinfo(2 * nslice);
   int64_t nzlen = 0;
   for (int64_t s = 0; s < nslice; ++s) {
     int32_t w = 0;
@@ -427,6 +427,8 @@ static void build_host_mesh(const xcal_mesh_host* h, HostMesh& M) {
       rbeg[c] = (int32_t)(nzlen + (c - s * kSlice));
       rend[c] = rbeg[c] + kSlice * (rowptr[cinv[c] + 1] - rowptr[cinv[c]]);
     }
+    sinfo[2 * s] = (int32_t)nzlen;
+    sinfo[2 * s + 1] = w;
     nzlen += (int64_t)w * kSlice;
     XCAL_CHECK(nzlen < (1ll << 31), "matrix storage exceeds int32 indices");
   }
@@ -594,6 +596,7 @@ static void build_host_mesh(const xcal_mesh_host* h, HostMesh& M) {
   M.bpatch.swap(bpatch);
   M.rbeg.swap(rbeg);
   M.rend.swap(rend);
+  M.sinfo.swap(sinfo);
   M.colval.swap(colval);
   M.trans.swap(trans);
   M.rows_b.swap(rows_b);
@@ -810,6 +813,7 @@ static xcal_mesh* upload_mesh(xcal_ctx* ctx, HostMesh& M) {
   upload(m->bface_patch, M.bpatch, bytes);
   upload(m->rbeg, M.rbeg, bytes);
   upload(m->rend, M.rend, bytes);
+  upload(m->sinfo, M.sinfo, bytes);  // empty for a batch union (batch kernels use rbeg/rend)
   upload(m->colval, M.colval, bytes);
   upload(m->trans, M.trans, bytes);
   upload(m->lev_rows_b, M.rows_b, bytes);
@@ -862,6 +866,7 @@ int64_t prepared_footprint(const PreparedMesh& P, bool turb) {
   mesh += 8 * (int64_t)(M.vol.size() + 20 * M.area.size());
   mesh += 4 * (int64_t)(M.cf_ptr.size() + M.cf_face.size() * 3 + M.diag.size() + 3 * M.own.size() +
                         M.bc_cells.size() + M.bc_ptr.size() + M.bc_face.size() + 2 * M.rbeg.size() +
+                        M.sinfo.size() +
                         2 * M.colval.size() + 2 * M.rows_b.size() + 2 * M.Lcol.size() +
                         3 * M.Ucol.size());
   const int64_t n = M.n, nf = M.nf, nz = M.nzlen;

@@ -31,11 +31,13 @@ struct MatKT {
   const T* nz;
   const int32_t* blk;  // row blocks in dispatch order (xcal_mesh::xblk)
   int32_t rw;          // most entries in one row (selects row_sum's group size)
+  const int32_t* sinfo = nullptr;  // per slice: first slot, width (single meshes)
 };
 using MatK = MatKT<double>;
 template <class T>
 inline MatKT<T> matk(const xcal_mesh* m, const T* nz) {
-  return MatKT<T>{m->n, m->rbeg.p, m->rend.p, m->colval.p, m->diag.p, nz, m->xblk_d.p, m->roww};
+  return MatKT<T>{m->n, m->rbeg.p, m->rend.p, m->colval.p, m->diag.p, nz, m->xblk_d.p, m->roww,
+                  m->sinfo.p};
 }
 // Row of this work-item in a launch over A's row blocks (grid = nxblk);
 // valid while row < end.
@@ -129,6 +131,35 @@ __device__ inline T row_sum(const MatKT<T>& A, int64_t i, XV&& xv) {
     return row_sum_g<8>(A, i, xv);
   }
 }
+// row_sum_g over a whole slice width: row i's entries are found from its
+// slice's first slot and width (one small table, cached, instead of the
+// row's own begin/end), and the slice's padding -- column 0, value 0 --
+// is summed like an entry.  a*x of a padding slot is +-0, so the sum is the
+// same as row_sum_g's (up to the sign of an all-zero row's 0) for finite x.
+// Needs the slice width <= G (rw <= G) and zeroed padding in nz.
+template <int G, class T, class XV>
+__device__ inline T row_sum_slice(const MatKT<T>& A, int64_t i, XV&& xv) {
+  const int2 si = reinterpret_cast<const int2*>(A.sinfo)[i / kSlice];
+  const int32_t k0 = si.x + (int32_t)(i % kSlice), w = si.y;
+  int col[G];
+  T a[G], x[G];
+#pragma unroll
+  for (int u = 0; u < G; ++u) {
+    const int32_t k = k0 + min(u, w - 1) * kSlice;
+    col[u] = __builtin_nontemporal_load(&A.colval[k]);
+    a[u] = __builtin_nontemporal_load(&A.nz[k]);
+  }
+#pragma unroll
+  for (int u = 0; u < G; ++u) x[u] = xv(col[u]);
+  T acc = 0;
+#pragma unroll
+  for (int u = 0; u < G; ++u) {
+    const T t = a[u] * x[u];
+    acc = u < w ? acc + t : acc;
+  }
+  return acc;
+}
+
 // A row sum over entries nz * f(U[col], W[col]) in two phases: row_pair_load
 // issues the row's first G entries and both gathers, before the caller has
 // the scalars f needs (a Krylov state formed in the same kernel);

@@ -144,7 +144,8 @@ __global__ void __launch_bounds__(kBlock)
 
 // CG: q = A p ; p.q.  p is formed by k_cg_p beforehand, so each entry
 // gathers one value (forming p on the fly for every neighbour doubles the
-// gathers, and the gathers bound this kernel).
+// gathers, and the gathers bound this kernel).  Rows are addressed through
+// their slice (row_sum_slice): two dependent round trips instead of three.
 template <int G, class T>
 __global__ void __launch_bounds__(kBlock)
     k_cg_spmv(MatKT<T> A, const T* __restrict__ p, T* __restrict__ q, double* partial,
@@ -155,7 +156,9 @@ __global__ void __launch_bounds__(kBlock)
   double s0 = 0.0;
   if (i < end) {
     const T pi = p[i];  // issued with the row's first loads
-    const T acc = row_sum<G>(A, i, [&](int j) { return p[j]; });
+    const auto pj = [&](int j) { return p[j]; };
+    // A.rw and A.sinfo are uniform over the launch: a scalar branch
+    const T acc = A.sinfo && A.rw <= G ? row_sum_slice<G>(A, i, pj) : row_sum<G>(A, i, pj);
     if (live) q[i] = acc;
     s0 = dd(pi, acc);
   }

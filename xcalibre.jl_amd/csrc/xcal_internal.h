@@ -181,6 +181,7 @@ struct xcal_mesh {
   int32_t roww = 0;   // most entries in one row
   bool fuse = true;   // fused Krylov finalizes allowed (XCAL_NO_FUSE=1 at creation: off)
   xcal::DBuf<int32_t> rbeg, rend, colval, trans;
+  xcal::DBuf<int32_t> sinfo;  // per slice of kSlice rows: first slot, width (entries)
   // DILU level schedules: forward level l = device rows
   // [lev_ptr_f[l], lev_ptr_f[l+1]); backward level l = lev_rows_b over
   // [lev_ptr_b[l], lev_ptr_b[l+1])

@@ -81,7 +81,12 @@ enum {
   XCAL_DIV_UPWIND = 4,        /* Divergence{Upwind}         :137-151 */
   XCAL_DIV_LUST = 5,          /* Divergence{LUST}           :155-179 */
   XCAL_DIV_BOUNDED_UPWIND = 6,/* Divergence{BoundedUpwind}  :183-199 */
-  XCAL_SI = 7                 /* Si (implicit source)       :204-218 */
+  XCAL_SI = 7,                /* Si (implicit source)       :204-218 */
+  /* Time{CrankNicolson} (:44-62): assembled as Time{Euler}.  An equation
+   * whose first term is Crank-Nicolson takes the explicit step
+   * x = 2 x - x_old after each of its Krylov solves (solve_system!,
+   * Solve_1_api.jl:291-296). */
+  XCAL_TIME_CRANK_NICOLSON = 8
 };
 
 /* One operator term: Operator(flux, phi, sign, type).  flux is a face scalar
@@ -144,6 +149,12 @@ typedef struct {
   double atol;
   double rtol;
   double ir_inner_rtol; /* FP32_IR: relative tolerance of each inner solve (0: 1e-6) */
+  /* SolverSetup.smoother = JacobiSmoother(; loops, omega)
+   * (Smoothers_jacobi.jl:36-51): `smoother_loops` weighted-Jacobi sweeps of
+   * the field before the Krylov solve (apply_smoother!, :54-110, called by
+   * solve_system!, Solve_1_api.jl:285).  0 loops: no smoother (nothing). */
+  int32_t smoother_loops;
+  double smoother_omega;
 } xcal_solver_setup;
 
 typedef struct {
@@ -327,9 +338,10 @@ typedef struct {
   int32_t n_bcs;
   int32_t grad_p;       /* XCAL_GRAD_* of grad!(∇p) (schemes.p.gradient)   */
   int32_t grad_U;       /* XCAL_GRAD_* of grad!(∇U) (schemes.U.gradient)   */
-  int32_t time_U;       /* XCAL_TIME_STEADY or _EULER: Time{schemes.U.time}(U)
-                           of the momentum equation (Solvers_1_SIMPLE.jl:68-74) */
-  double dt;            /* runtime.dt of Time{Euler} (> 0 when time_U is Euler) */
+  int32_t time_U;       /* XCAL_TIME_STEADY, _EULER or _CRANK_NICOLSON:
+                           Time{schemes.U.time}(U) of the momentum equation
+                           (Solvers_1_SIMPLE.jl:68-74) */
+  double dt;            /* runtime.dt of Time{Euler/CrankNicolson} (> 0 then) */
   int32_t ncorrectors;  /* run!(...; ncorrectors): non-orthogonal correctors of
                            the pressure (Solvers_1_SIMPLE.jl:194-206,
                            Solvers_2_PISO.jl:148-165) */
@@ -369,6 +381,11 @@ int xcal_simple_get_field(const xcal_simple* s, int32_t which, double* host);
  * gets the Krylov iteration counts of the Ux, Uy, Uz and p solves. */
 int xcal_simple_run(xcal_simple* s, int32_t iterations, double* residuals, int32_t* done,
                     int32_t* krylov_iters);
+/* Whether the last xcal_simple_run stopped on the convergence test of
+ * Solvers_1_SIMPLE.jl:226-233 (every residual below its SolverSetup's
+ * convergence, and the turbulence model's state.converged): 1, else 0
+ * (iteration limit or the end of the requested iterations). */
+int xcal_simple_converged(const xcal_simple* s, int32_t* converged);
 /* Transient incompressible PISO (Solvers_2_PISO.jl:40-211; run! of a
  * Transient model, Solvers_3_solver_dispatch.jl:186-190, calls it with
  * inner_loops = 2): `steps` time steps of setup.dt, each one momentum solve

@@ -96,6 +96,7 @@ inline void scheme(const orc_mesh& m, const orc_term& t, int64_t fID, double ns,
   switch (t.type) {
     case XCAL_TIME_STEADY:
     case XCAL_TIME_EULER:
+    case XCAL_TIME_CRANK_NICOLSON:
     case XCAL_SI:
       ac = 0.0;
       an = 0.0;
@@ -151,7 +152,8 @@ inline void scheme(const orc_mesh& m, const orc_term& t, int64_t fID, double ns,
 inline void scheme_source(const orc_mesh& m, const orc_term& t, int64_t cID, const double* prev,
                           double dt, double& ac, double& b) {
   switch (t.type) {
-    case XCAL_TIME_EULER: {  // :30-42
+    case XCAL_TIME_EULER:             // :30-42
+    case XCAL_TIME_CRANK_NICOLSON: {  // :52-62, the same coefficients
       double vol_rdt = (term_flux(t, cID) * m.vol[cID]) / dt;
       ac = vol_rdt;
       b = prev[cID] * vol_rdt;
@@ -183,7 +185,9 @@ void bc_coeffs(const orc_mesh& m, const xcal_bc& bc, const orc_term& t, int64_t 
   };
   ap_out = 0.0;
   bp_out = 0.0;
-  if (t.type == XCAL_TIME_STEADY || t.type == XCAL_TIME_EULER || t.type == XCAL_SI) return;
+  if (t.type == XCAL_TIME_STEADY || t.type == XCAL_TIME_EULER ||
+      t.type == XCAL_TIME_CRANK_NICOLSON || t.type == XCAL_SI)
+    return;
   const bool lap = t.type == XCAL_LAPLACIAN_LINEAR;
   const bool divLUL = t.type == XCAL_DIV_LINEAR || t.type == XCAL_DIV_UPWIND ||
                       t.type == XCAL_DIV_LUST;
@@ -753,6 +757,44 @@ int orc_krylov(const orc_mesh* mp, int32_t solver, int32_t precon, const double*
   });
 }
 
+void orc_smooth_jacobi(const orc_mesh* mp, const double* nzval, const double* b, double* x,
+                       int32_t loops, double omega) {
+  const orc_mesh& m = *mp;
+  const int64_t n = m.n;
+  std::vector<double> xt(n);  // smoother.x_temp
+  for (int32_t l = 0; l < loops; ++l) {
+    OMP_FOR
+    for (int64_t i = 0; i < n; ++i) {  // multiply_row (Smoothers_jacobi.jl:97-110)
+      double sum = 0.0;
+      for (int64_t k = m.rowptr[i]; k < m.rowptr[i + 1]; ++k) sum += nzval[k] * x[m.colval[k]];
+      const double a_ii = nzval[m.diag[i]];
+      const double xi = x[i];
+      sum -= a_ii * xi;
+      const double rD = 1.0 / a_ii;
+      xt[i] = (omega * rD) * (b[i] - sum) + (1.0 - omega) * xi;
+    }
+    std::copy(xt.begin(), xt.end(), x);
+  }
+}
+
+int orc_solve_system(const orc_mesh* mp, const xcal_solver_setup* ss, int32_t cn,
+                     const double* nzval, const double* b, double* x, int32_t* niter) {
+  if (ss->smoother_loops < 0) {
+    g_err = "smoother loops must be >= 0";
+    return -1;
+  }
+  if (ss->smoother_loops > 0)
+    orc_smooth_jacobi(mp, nzval, b, x, ss->smoother_loops, ss->smoother_omega);
+  std::vector<double> xold;
+  if (cn) xold.assign(x, x + mp->n);
+  const int rc = orc_krylov(mp, ss->solver, ss->precon, nzval, b, x, ss->itmax, ss->atol,
+                            ss->rtol, niter);
+  if (rc < 0) return rc;
+  if (cn)
+    for (int64_t i = 0; i < mp->n; ++i) x[i] = 2.0 * x[i] - xold[i];
+  return rc;
+}
+
 void orc_interpolate(const orc_mesh* mp, const double* phi, double* phif) {
   const orc_mesh& m = *mp;
   const int64_t n = m.nf;
@@ -1106,18 +1148,11 @@ void bound(const orc_mesh& m, double* v, double* tmp) {
 
 }  // namespace
 
-static bool komega_step(const orc_mesh& m, const xcal_simple_setup& S,
-                        const xcal_turbulence_setup& T, const double* Ux, const double* Uy,
-                        const double* Uz, const double* mdotf, double* const* gU, double* k,
-                        double* w, double* nut, double* nutf, double* Pk, double* Pw, double* Dk,
-                        double* Dw, double* mk, double* mw, double* tmp, double* A, double* b) {
-  const int64_t n = m.n, nf = m.nf;
-  const int nb = S.n_bcs;
-  const Wall W(S.nu);
-  auto chk = [](int rc) {
-    if (rc < 0) throw std::runtime_error(g_err);
-  };
-  // magnitude2!(Pk, S, 2) with S = (gradU + gradU')/2, then Pw, Pk (:193-197)
+// magnitude2!(Pk, S, 2) with S = (gradU + gradU')/2 (RANS_tensor_algebra.jl:68),
+// then Pw = alpha1 * 2S:S and Pk = nut * 2S:S (RANS_kOmega.jl:193-197)
+void orc_kw_production(const orc_mesh* mp, const double* const* gU, const double* nut,
+                       double alpha1, double* Pk, double* Pw) {
+  const int64_t n = mp->n;
   for (int64_t i = 0; i < n; ++i) {
     double sum = 0.0;
     for (int j = 0; j < 3; ++j)
@@ -1126,35 +1161,137 @@ static bool komega_step(const orc_mesh& m, const xcal_simple_setup& S,
         sum += sj * sj;
       }
     const double P = sum * 2.0;
-    Pw[i] = (1.0 * T.alpha1) * P;
+    Pw[i] = (1.0 * alpha1) * P;
     Pk[i] = (1.0 * nut[i]) * P;
   }
-  // correct_production! (RANS_functions.jl:62-124), Uw = 0
-  for (int64_t f = 0; f < m.nbf; ++f) {
-    const int p = patch_of(m, f);
-    if (p < 0 || T.bc_k[p].kind != XCAL_BC_KWALL) continue;
-    const int64_t c = m.bcell[f];
-    const double delta = m.delta[f];
-    const double uStar = W.cmu25 * std::sqrt(k[c]);
-    const double dUdy = uStar / (W.kappa * delta);
-    const double yp = W.yplus(k[c], delta);
-    const double nutw = W.nutw(yp);
-    const V3 nrm = m.fn[f];
-    const double dx = Ux[c] - 0.0, dy = Uy[c] - 0.0, dz = Uz[c] - 0.0;
-    const double dn = dx * nrm.x + dy * nrm.y + dz * nrm.z;
-    const double gx = (dx - dn * nrm.x) / delta, gy = (dy - dn * nrm.y) / delta,
-                 gz = (dz - dn * nrm.z) / delta;
-    const double mag = std::sqrt(gx * gx + gy * gy + gz * gz);
-    Pk[c] = yp > W.yPlusLam ? ((W.nu + nutw) * mag) * dUdy : 0.0;
-  }
+}
+
+// correct_production! (RANS_functions.jl:62-124) on KWallFunction patches,
+// wall velocity 0: the cell's Pk becomes the log-law production
+int orc_kw_wall_production(const orc_mesh* mp, const xcal_bc* bc_k, int32_t nbc, double nu,
+                           const double* Ux, const double* Uy, const double* Uz,
+                           const double* k, double* Pk) {
+  return guard([&] {
+    const orc_mesh& m = *mp;
+    if (nbc != (int32_t)m.pfirst.size()) throw std::runtime_error("one BC per patch");
+    const Wall W(nu);
+    for (int64_t f = 0; f < m.nbf; ++f) {
+      const int p = patch_of(m, f);
+      if (p < 0 || bc_k[p].kind != XCAL_BC_KWALL) continue;
+      const int64_t c = m.bcell[f];
+      const double delta = m.delta[f];
+      const double uStar = W.cmu25 * std::sqrt(k[c]);
+      const double dUdy = uStar / (W.kappa * delta);
+      const double yp = W.yplus(k[c], delta);
+      const double nutw = W.nutw(yp);
+      const V3 nrm = m.fn[f];
+      const double dx = Ux[c] - 0.0, dy = Uy[c] - 0.0, dz = Uz[c] - 0.0;
+      const double dn = dx * nrm.x + dy * nrm.y + dz * nrm.z;
+      const double gx = (dx - dn * nrm.x) / delta, gy = (dy - dn * nrm.y) / delta,
+                   gz = (dz - dn * nrm.z) / delta;
+      const double mag = std::sqrt(gx * gx + gy * gy + gz * gz);
+      Pk[c] = yp > W.yPlusLam ? ((W.nu + nutw) * mag) * dUdy : 0.0;
+    }
+    return 0;
+  });
+}
+
+// The remaining coefficients of turbulence!(KOmegaModel) (RANS_kOmega.jl:198-206):
+// implicit destruction Dw = beta1 omega and Dk = beta* omega, face
+// diffusivities mueffw = nu + sigma_omega nutf and mueffk = nu + sigma_k nutf
+void orc_kw_coeffs(const orc_mesh* mp, const double* w, const double* nutf, double nu,
+                   double beta1, double beta_star, double sigma_omega, double sigma_k, double* Dw,
+                   double* Dk, double* mw, double* mk) {
+  const int64_t n = mp->n, nf = mp->nf;
   for (int64_t i = 0; i < n; ++i) {
-    Dw[i] = (1.0 * T.beta1) * w[i];
-    Dk[i] = (1.0 * T.beta_star) * w[i];
+    Dw[i] = (1.0 * beta1) * w[i];
+    Dk[i] = (1.0 * beta_star) * w[i];
   }
   for (int64_t f = 0; f < nf; ++f) {
-    mw[f] = 1.0 * (S.nu + T.sigma_omega * nutf[f]);
-    mk[f] = 1.0 * (S.nu + T.sigma_k * nutf[f]);
+    mw[f] = 1.0 * (nu + sigma_omega * nutf[f]);
+    mk[f] = 1.0 * (nu + sigma_k * nutf[f]);
   }
+}
+
+// constrain_equation!/constrain! (RANS_functions.jl:262-356): the row of each
+// OmegaWallFunction face's cell becomes an identity row with b = omega_wall,
+// the viscous-sublayer value 6 nu / (beta1 y^2) below y+_lam and the log-law
+// value sqrt(k) / (cmu^1/4 kappa y) above (last face of a cell wins)
+int orc_kw_constrain(const orc_mesh* mp, const xcal_bc* bc_w, int32_t nbc, double nu,
+                     const double* k, double* A, double* b) {
+  return guard([&] {
+    const orc_mesh& m = *mp;
+    if (nbc != (int32_t)m.pfirst.size()) throw std::runtime_error("one BC per patch");
+    const Wall W(nu);
+    for (int64_t f = 0; f < m.nbf; ++f) {
+      const int p = patch_of(m, f);
+      if (p < 0 || bc_w[p].kind != XCAL_BC_OMEGAWALL) continue;
+      const int64_t c = m.bcell[f];
+      const double y = m.delta[f];
+      const double wvis = (6.0 * W.nu) / (W.beta1 * (y * y));
+      const double wlog = std::sqrt(k[c]) / ((W.cmu25 * W.kappa) * y);
+      const double wc = W.yplus(k[c], y) > W.yPlusLam ? wlog : wvis;
+      for (int64_t q = m.rowptr[c]; q < m.rowptr[c + 1]; ++q) A[q] = 0.0;
+      A[spindex(m, c, c)] = 1.0;
+      b[c] = wc;
+    }
+    return 0;
+  });
+}
+
+void orc_bound(const orc_mesh* mp, double* v) {
+  std::vector<double> tmp(mp->n);
+  bound(*mp, v, tmp.data());
+}
+
+// nut = k / omega, its face values (interpolate!, correct_boundaries!) and
+// correct_eddy_viscosity!'s wall value on NutWallFunction faces
+// (RANS_functions.jl:126-200): nu (y+ kappa / log(E y+) - 1) above y+_lam, 0 below
+int orc_kw_nut(const orc_mesh* mp, const xcal_bc* bc_nut, int32_t nbc, double nu,
+               const double* k, const double* w, double* nut, double* nutf) {
+  return guard([&] {
+    const orc_mesh& m = *mp;
+    if (nbc != (int32_t)m.pfirst.size()) throw std::runtime_error("one BC per patch");
+    const Wall W(nu);
+    for (int64_t i = 0; i < m.n; ++i) nut[i] = k[i] / w[i];
+    orc_interpolate(&m, nut, nutf);
+    const int rc = orc_correct_boundaries(&m, bc_nut, nbc, nut, nullptr, nullptr, nutf, nullptr,
+                                          nullptr);
+    if (rc < 0) throw std::runtime_error(g_err);
+    for (int64_t f = 0; f < m.nbf; ++f) {
+      const int p = patch_of(m, f);
+      if (p < 0 || bc_nut[p].kind != XCAL_BC_NUTWALL) continue;
+      const int64_t c = m.bcell[f];
+      const double yp = W.yplus(k[c], m.delta[f]);
+      nutf[f] = yp > W.yPlusLam ? W.nutw(yp) : 0.0;
+    }
+    return 0;
+  });
+}
+
+void orc_kw_wall_constants(double nu, double* out4) {
+  const Wall W(nu);
+  out4[0] = W.cmu25;
+  out4[1] = W.yPlusLam;
+  out4[2] = W.kappa;
+  out4[3] = W.E;
+}
+
+}  // extern "C"
+
+static bool komega_step(const orc_mesh& m, const xcal_simple_setup& S,
+                        const xcal_turbulence_setup& T, const double* Ux, const double* Uy,
+                        const double* Uz, const double* mdotf, double* const* gU, double* k,
+                        double* w, double* nut, double* nutf, double* Pk, double* Pw, double* Dk,
+                        double* Dw, double* mk, double* mw, double* tmp, double* A, double* b) {
+  const int nb = S.n_bcs;
+  auto chk = [](int rc) {
+    if (rc < 0) throw std::runtime_error(g_err);
+  };
+  orc_kw_production(&m, gU, nut, T.alpha1, Pk, Pw);
+  chk(orc_kw_wall_production(&m, T.bc_k, nb, S.nu, Ux, Uy, Uz, k, Pk));
+  orc_kw_coeffs(&m, w, nutf, S.nu, T.beta1, T.beta_star, T.sigma_omega, T.sigma_k, Dw, Dk, mw,
+                mk);
   auto transport = [&](double* phi, double* D, double* mu, double* P, const xcal_bc* bc,
                        const xcal_solver_setup& ss, bool constrain) {
     orc_term t[4] = {{XCAL_TIME_STEADY, 1, nullptr, 1.0},
@@ -1165,42 +1302,20 @@ static bool komega_step(const orc_mesh& m, const xcal_simple_setup& S,
     chk(orc_discretise_scalar(&m, t, 4, src, 1, phi, 1.0, A, b));
     chk(orc_apply_bcs(&m, t, 4, bc, nb, -1, phi, nullptr, nullptr, A, b));
     orc_relax_implicit_diagdom(&m, A, b, phi, ss.relax);
-    if (constrain) {  // constrain! (RANS_functions.jl:262-356)
-      for (int64_t f = 0; f < m.nbf; ++f) {
-        const int p = patch_of(m, f);
-        if (p < 0 || bc[p].kind != XCAL_BC_OMEGAWALL) continue;
-        const int64_t c = m.bcell[f];
-        const double y = m.delta[f];
-        const double wvis = (6.0 * W.nu) / (W.beta1 * (y * y));
-        const double wlog = std::sqrt(k[c]) / ((W.cmu25 * W.kappa) * y);
-        const double wc = W.yplus(k[c], y) > W.yPlusLam ? wlog : wvis;
-        for (int64_t q = m.rowptr[c]; q < m.rowptr[c + 1]; ++q) A[q] = 0.0;
-        A[spindex(m, c, c)] = 1.0;
-        b[c] = wc;
-      }
-    }
+    if (constrain) chk(orc_kw_constrain(&m, bc, nb, S.nu, k, A, b));
     int32_t it = 0;
-    chk(orc_krylov(&m, ss.solver, ss.precon, A, b, phi, ss.itmax, ss.atol, ss.rtol, &it));
+    chk(orc_solve_system(&m, &ss, 0, A, b, phi, &it));
     const double r = orc_residual(&m, A, b, phi);
     bound(m, phi, tmp);
     return r;
   };
   const double rw = transport(w, Dw, mw, Pw, T.bc_omega, T.omega, true);
   const double rk = transport(k, Dk, mk, Pk, T.bc_k, T.k, false);
-  for (int64_t i = 0; i < n; ++i) nut[i] = k[i] / w[i];
-  orc_interpolate(&m, nut, nutf);
-  chk(orc_correct_boundaries(&m, T.bc_nut, nb, nut, nullptr, nullptr, nutf, nullptr, nullptr));
-  for (int64_t f = 0; f < m.nbf; ++f) {  // correct_eddy_viscosity! (RANS_functions.jl:126-200)
-    const int p = patch_of(m, f);
-    if (p < 0 || T.bc_nut[p].kind != XCAL_BC_NUTWALL) continue;
-    const int64_t c = m.bcell[f];
-    const double yp = W.yplus(k[c], m.delta[f]);
-    nutf[f] = yp > W.yPlusLam ? W.nutw(yp) : 0.0;
-  }
+  chk(orc_kw_nut(&m, T.bc_nut, nb, S.nu, k, w, nut, nutf));
   return rk < T.k.convergence && rw < T.omega.convergence;
 }
 
-}  // extern "C"
+
 
 namespace {
 
@@ -1224,12 +1339,16 @@ int run_core(const orc_mesh* mp, const xcal_simple_setup* S, const xcal_turbulen
     std::vector<std::vector<double>> gradU(9, std::vector<double>(n));
     double* gU[9];
     for (int k = 0; k < 9; ++k) gU[k] = gradU[k].data();
-    if (S->time_U != XCAL_TIME_STEADY && S->time_U != XCAL_TIME_EULER)
-      throw std::runtime_error("momentum time scheme: SteadyState or Euler");
-    if (S->time_U == XCAL_TIME_EULER && !(S->dt > 0.0))
-      throw std::runtime_error("Time{Euler} needs runtime.dt > 0");
+    if (S->time_U != XCAL_TIME_STEADY && S->time_U != XCAL_TIME_EULER &&
+        S->time_U != XCAL_TIME_CRANK_NICOLSON)
+      throw std::runtime_error("momentum time scheme: SteadyState, Euler or CrankNicolson");
+    if (S->time_U != XCAL_TIME_STEADY && !(S->dt > 0.0))
+      throw std::runtime_error("Time{Euler} and Time{CrankNicolson} need runtime.dt > 0");
     if (S->ncorrectors < 0) throw std::runtime_error("ncorrectors must be >= 0");
-    const double dt = S->time_U == XCAL_TIME_EULER ? S->dt : 1.0;
+    const double dt = S->time_U != XCAL_TIME_STEADY ? S->dt : 1.0;
+    // solve_system!'s Crank-Nicolson step applies to the momentum equation,
+    // whose first term is Time{schemes.U.time} (Solve_1_api.jl:291-296)
+    const int32_t Ucn = S->time_U == XCAL_TIME_CRANK_NICOLSON;
     const int ncorr = inner > 0 ? inner : 1;
     orc_term Uterms[3] = {{S->time_U, 1, nullptr, 1.0},
                           {S->div_scheme, 1, mdotf.data(), 0.0},
@@ -1285,8 +1404,7 @@ int run_core(const orc_mesh* mp, const xcal_simple_setup* S, const xcal_turbulen
         chk(orc_apply_bcs(&m, Uterms, 3, S->bc_U, nb, c, Ux, Uy, Uz, A.data(), bU[c]));
         orc_relax_implicit_diagdom(&m, A.data(), bU[c], U[c], S->U.relax);
         int32_t ni = 0;
-        chk(orc_krylov(&m, S->U.solver, S->U.precon, A.data(), bU[c], U[c], S->U.itmax,
-                       S->U.atol, S->U.rtol, &ni));
+        chk(orc_solve_system(&m, &S->U, Ucn, A.data(), bU[c], U[c], &ni));
         itc[c] = ni;
         res[c] = orc_residual(&m, A.data(), bU[c], U[c]);
       }
@@ -1313,8 +1431,7 @@ int run_core(const orc_mesh* mp, const xcal_simple_setup* S, const xcal_turbulen
         chk(orc_apply_bcs(&m, pterms, 1, S->bc_p, nb, -1, p, nullptr, nullptr, pA.data(),
                           pb.data()));
         if (S->has_pref) orc_set_reference(&m, pA.data(), pb.data(), S->pref, 0);
-        chk(orc_krylov(&m, S->p.solver, S->p.precon, pA.data(), pb.data(), p, S->p.itmax,
-                       S->p.atol, S->p.rtol, &pit));
+        chk(orc_solve_system(&m, &S->p, 0, pA.data(), pb.data(), p, &pit));
         rp = orc_residual(&m, pA.data(), pb.data(), p);
         // explicit_relaxation!: PISO's last corrector uses 1.0 (:141-145)
         const double prelax = inner > 0 && ic == ncorr - 1 ? 1.0 : S->p.relax;
@@ -1330,8 +1447,7 @@ int run_core(const orc_mesh* mp, const xcal_simple_setup* S, const xcal_turbulen
           if (inner > 0 && S->has_pref) orc_set_reference(&m, pA.data(), pb.data(), S->pref, 0);
           chk(orc_nonorth_correction(&m, gpx.data(), gpy.data(), gpz.data(), p, rDf.data(),
                                      pb.data()));
-          chk(orc_krylov(&m, S->p.solver, S->p.precon, pA.data(), pb.data(), p, S->p.itmax,
-                         S->p.atol, S->p.rtol, &pit));
+          chk(orc_solve_system(&m, &S->p, 0, pA.data(), pb.data(), p, &pit));
           rp = orc_residual(&m, pA.data(), pb.data(), p);
           const double nrelax = inner > 0 && jc == S->ncorrectors - 1 ? 1.0 : S->p.relax;
           for (int64_t i = 0; i < n; ++i) p[i] = prev[i] + nrelax * (p[i] - prev[i]);

@@ -85,6 +85,16 @@ int orc_krylov(const orc_mesh* m, int32_t solver, int32_t precon, const double* 
                const double* b, double* x, int32_t itmax, double atol, double rtol,
                int32_t* niter);
 
+/* apply_smoother!(JacobiSmoother) (Smoothers_jacobi.jl:54-110): `loops`
+ * weighted-Jacobi sweeps of x, each reading the previous sweep's values */
+void orc_smooth_jacobi(const orc_mesh* m, const double* nzval, const double* b, double* x,
+                       int32_t loops, double omega);
+/* solve_system! (Solve_1_api.jl:268-307) around the Krylov call: the setup's
+ * smoother on x, the Krylov solve warm-started from it, and with cn != 0
+ * (the equation's first term is Time{CrankNicolson}) x = 2 x - x_old. */
+int orc_solve_system(const orc_mesh* m, const xcal_solver_setup* setup, int32_t cn,
+                     const double* nzval, const double* b, double* x, int32_t* niter);
+
 /* interpolate! scalar (Calculate_2_interpolation.jl:40-76) */
 void orc_interpolate(const orc_mesh* m, const double* phi, double* phif);
 /* correct_boundaries! (1_boundary_interpolation.jl); vector: component 0..2
@@ -123,6 +133,32 @@ int orc_correct_mass_flux(const orc_mesh* m, const double* p_nzval, const double
  * per cell in face-list order; the reference's atomics fix no order) */
 int orc_nonorth_correction(const orc_mesh* m, const double* gx, const double* gy,
                            const double* gz, const double* p, const double* flux, double* b);
+
+/* ---- pieces of turbulence!(KOmegaModel) (RANS_kOmega.jl:175-235) ---- */
+/* magnitude2! of S = (gradU + gradU')/2 times 2 (RANS_tensor_algebra.jl:68):
+ * Pw = alpha1 * 2 S:S, Pk = nut * 2 S:S.  gU: 9 arrays, gU[3*j+q] = dU_j/dx_q */
+void orc_kw_production(const orc_mesh* m, const double* const* gU, const double* nut,
+                       double alpha1, double* Pk, double* Pw);
+/* correct_production! (RANS_functions.jl:62-124), wall velocity 0 */
+int orc_kw_wall_production(const orc_mesh* m, const xcal_bc* bc_k, int32_t nbc, double nu,
+                           const double* Ux, const double* Uy, const double* Uz,
+                           const double* k, double* Pk);
+/* Dw = beta1 omega, Dk = beta* omega (cells); mueff of omega and k (faces) */
+void orc_kw_coeffs(const orc_mesh* m, const double* omega, const double* nutf, double nu,
+                   double beta1, double beta_star, double sigma_omega, double sigma_k, double* Dw,
+                   double* Dk, double* mw, double* mk);
+/* constrain_equation!/constrain! (RANS_functions.jl:262-356) */
+int orc_kw_constrain(const orc_mesh* m, const xcal_bc* bc_omega, int32_t nbc, double nu,
+                     const double* k, double* nzval, double* b);
+/* bound! (RANS_functions.jl:2-38) in place */
+void orc_bound(const orc_mesh* m, double* v);
+/* nut = k/omega, nutf by interpolate!/correct_boundaries!, then
+ * correct_eddy_viscosity! on NutWallFunction faces (RANS_functions.jl:126-200) */
+int orc_kw_nut(const orc_mesh* m, const xcal_bc* bc_nut, int32_t nbc, double nu, const double* k,
+               const double* omega, double* nut, double* nutf);
+/* wall-function constants for viscosity nu: cmu^1/4, y+_lam (10 fixed-point
+ * iterations of the log law, RANS_functions.jl:40-43), kappa, E */
+void orc_kw_wall_constants(double nu, double* out4);
 
 /* Whole steady SIMPLE run (Solvers_1_SIMPLE.jl:45-263, laminar).
  * U: 3 arrays (x,y,z) in/out, p in/out.  residuals: 4*iterations

@@ -108,6 +108,20 @@ def lib():
                                    C.c_int32, _pd, _pi32, _pi32, _pd]
         L.orc_nonorth_correction.argtypes = [C.c_void_p, _pd, _pd, _pd, _pd, _pd, _pd]
         L.orc_set_threads.argtypes = [C.c_int32]
+        L.orc_smooth_jacobi.argtypes = [C.c_void_p, _pd, _pd, _pd, C.c_int32, C.c_double]
+        L.orc_solve_system.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, _pd, _pd, _pd,
+                                       C.POINTER(C.c_int32)]
+        L.orc_kw_production.argtypes = [C.c_void_p, C.POINTER(_pd), _pd, C.c_double, _pd, _pd]
+        L.orc_kw_wall_production.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_double, _pd,
+                                             _pd, _pd, _pd, _pd]
+        L.orc_kw_coeffs.argtypes = [C.c_void_p, _pd, _pd, C.c_double, C.c_double, C.c_double,
+                                    C.c_double, C.c_double, _pd, _pd, _pd, _pd]
+        L.orc_kw_constrain.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_double, _pd, _pd,
+                                       _pd]
+        L.orc_bound.argtypes = [C.c_void_p, _pd]
+        L.orc_kw_nut.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_double, _pd, _pd, _pd,
+                                 _pd]
+        L.orc_kw_wall_constants.argtypes = [C.c_double, _pd]
         _lib = L
     return _lib
 
@@ -134,6 +148,7 @@ class OracleMesh:
 
     def __init__(self, mesh, abi):
         self.mesh = mesh
+        self.abi = abi
         self._host = abi.mesh_host(mesh)
         h = C.c_void_p()
         _chk(lib().orc_mesh_create(C.byref(self._host), C.byref(h)))
@@ -236,6 +251,77 @@ class OracleMesh:
         rc = _chk(lib().orc_krylov(self.h, solver, precon, _p(nzval), _p(b), _p(x), itmax, atol,
                                    rtol, C.byref(it)))
         return x, it.value, rc
+
+    def smooth_jacobi(self, nzval, b, x0, loops, omega):
+        """apply_smoother!(JacobiSmoother(loops, omega)) on a copy of x0."""
+        x = np.array(x0, dtype=np.float64, copy=True)
+        lib().orc_smooth_jacobi(self.h, _p(nzval), _p(np.ascontiguousarray(b, dtype=np.float64)),
+                                _p(x), int(loops), float(omega))
+        return x
+
+    def solve_system(self, setup_c, nzval, b, x0, cn=False):
+        """solve_system!: smoother, Krylov solve, Crank-Nicolson step (cn)."""
+        x = np.array(x0, dtype=np.float64, copy=True)
+        it = C.c_int32()
+        rc = _chk(lib().orc_solve_system(self.h, C.byref(setup_c), int(bool(cn)), _p(nzval),
+                                         _p(np.ascontiguousarray(b, dtype=np.float64)), _p(x),
+                                         C.byref(it)))
+        return x, it.value, rc
+
+    # ---- k-omega pieces
+    def kw_production(self, gU, nut, alpha1):
+        g = [np.ascontiguousarray(gU[i], dtype=np.float64) for i in range(9)]
+        arr = (_pd * 9)(*[_p(a) for a in g])
+        Pk, Pw = np.zeros(self.n), np.zeros(self.n)
+        lib().orc_kw_production(self.h, arr, _p(np.ascontiguousarray(nut, dtype=np.float64)),
+                                float(alpha1), _p(Pk), _p(Pw))
+        return Pk, Pw
+
+    def kw_wall_production(self, bc_k, nu, U, k, Pk):
+        bc = self.abi.bc_array(bc_k)
+        out = np.array(Pk, dtype=np.float64, copy=True)
+        U = [np.ascontiguousarray(U[i], dtype=np.float64) for i in range(3)]
+        _chk(lib().orc_kw_wall_production(self.h, bc, len(bc_k), float(nu), _p(U[0]), _p(U[1]),
+                                          _p(U[2]), _p(np.ascontiguousarray(k, dtype=np.float64)),
+                                          _p(out)))
+        return out
+
+    def kw_coeffs(self, omega, nutf, nu, beta1, beta_star, sigma_omega, sigma_k):
+        Dw, Dk = np.zeros(self.n), np.zeros(self.n)
+        mw, mk = np.zeros(self.mesh.n_faces), np.zeros(self.mesh.n_faces)
+        lib().orc_kw_coeffs(self.h, _p(np.ascontiguousarray(omega, dtype=np.float64)),
+                            _p(np.ascontiguousarray(nutf, dtype=np.float64)), float(nu),
+                            float(beta1), float(beta_star), float(sigma_omega), float(sigma_k),
+                            _p(Dw), _p(Dk), _p(mw), _p(mk))
+        return Dw, Dk, mw, mk
+
+    def kw_constrain(self, bc_omega, nu, k, nzval, b):
+        bc = self.abi.bc_array(bc_omega)
+        A = np.array(nzval, dtype=np.float64, copy=True)
+        bb = np.array(b, dtype=np.float64, copy=True)
+        _chk(lib().orc_kw_constrain(self.h, bc, len(bc_omega), float(nu),
+                                    _p(np.ascontiguousarray(k, dtype=np.float64)), _p(A), _p(bb)))
+        return A, bb
+
+    def bound(self, v):
+        out = np.array(v, dtype=np.float64, copy=True)
+        lib().orc_bound(self.h, _p(out))
+        return out
+
+    def kw_nut(self, bc_nut, nu, k, omega):
+        bc = self.abi.bc_array(bc_nut)
+        nut, nutf = np.zeros(self.n), np.zeros(self.mesh.n_faces)
+        _chk(lib().orc_kw_nut(self.h, bc, len(bc_nut), float(nu),
+                              _p(np.ascontiguousarray(k, dtype=np.float64)),
+                              _p(np.ascontiguousarray(omega, dtype=np.float64)), _p(nut),
+                              _p(nutf)))
+        return nut, nutf
+
+    @staticmethod
+    def kw_wall_constants(nu):
+        out = np.zeros(4)
+        lib().orc_kw_wall_constants(float(nu), _p(out))
+        return dict(cmu25=out[0], yplus_lam=out[1], kappa=out[2], E=out[3])
 
     def interpolate(self, phi):
         f = np.zeros(self.mesh.n_faces)

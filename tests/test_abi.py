@@ -35,7 +35,7 @@ def test_library_exports_every_declared_symbol(x, header, so):
 
 def test_hip_library_abi_version_and_errors(x):
     L = x.hip.lib()
-    assert L.xcal_abi_version() == 1
+    assert L.xcal_abi_version() == 2
     # argument checks run before any device call
     assert L.xcal_ctx_create(0, None) == x._abi.OK - 1
     assert b"NULL" in L.xcal_last_error()

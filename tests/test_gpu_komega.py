@@ -9,39 +9,9 @@ omega and nut within 1e-6 relative L2 of the oracle."""
 import numpy as np
 import pytest
 
-from conftest import rel
+from conftest import komega_bfs_case, rel
 
 pytestmark = pytest.mark.gpu
-
-
-def komega_bfs_case(api, mesh, iterations, precon="Jacobi"):
-    nu, u, k_in, w_in = 1e-3, 1.5, 1.0, 1000.0
-    model = api.Physics(domain=mesh, fluid=api.Fluid(nu=nu), turbulence=api.RANS(api.KOmega))
-    BCs = api.assign(
-        region=mesh,
-        U=[api.Dirichlet("inlet", [u, 0, 0]), api.Extrapolated("outlet"),
-           api.Wall("wall", [0, 0, 0]), api.Wall("top", [0, 0, 0])],
-        p=[api.Neumann("inlet", 0.0), api.Dirichlet("outlet", 0.0), api.Wall("wall"),
-           api.Wall("top")],
-        k=[api.Dirichlet("inlet", k_in), api.Extrapolated("outlet"), api.KWallFunction("wall"),
-           api.KWallFunction("top")],
-        omega=[api.Dirichlet("inlet", w_in), api.Extrapolated("outlet"),
-               api.OmegaWallFunction("wall"), api.OmegaWallFunction("top")],
-        nut=[api.Dirichlet("inlet", k_in / w_in), api.Extrapolated("outlet"),
-             api.NutWallFunction("wall"), api.NutWallFunction("top")])
-    P = getattr(api, precon)
-
-    def S(s, r, rt):
-        return api.SolverSetup(solver=s, preconditioner=P(), convergence=1e-7, relax=r, rtol=rt,
-                               atol=1e-10)
-    solvers = dict(U=S(api.Bicgstab(), 0.7, 1e-2), p=S(api.Cg(), 0.3, 1e-3),
-                   k=S(api.Bicgstab(), 0.7, 1e-2), omega=S(api.Bicgstab(), 0.7, 1e-2))
-    schemes = {f: api.Schemes(divergence=api.Upwind) for f in ("U", "p", "k", "omega")}
-    config = api.Configuration(
-        solvers=solvers, schemes=schemes,
-        runtime=api.Runtime(iterations=iterations, write_interval=-1),
-        hardware=api.Hardware(backend=api.HIPBackend(0)), boundaries=BCs)
-    return model, config
 
 
 @pytest.mark.parametrize("precon", ["Jacobi", "DILU"])

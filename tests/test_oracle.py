@@ -7,7 +7,8 @@ import json
 import numpy as np
 import pytest
 
-from conftest import (GOLDEN, cavity_case, channel_case, patch_face_average, rel,
+from conftest import (GOLDEN, bfs_verification_profile, cavity_case, channel_case,
+                      docs_bfs_case, patch_face_average, rel, steady_bfs_case,
                       transient_bfs_case)
 
 
@@ -411,3 +412,152 @@ def test_nonorth_correction_exact_for_linear_fields(A, O, meshes):
     r1 = (Aphi - om.nonorth_correction(grad, phi, np.ones(nf), b))[interior]
     assert np.abs(r0).max() > 1e-2 * scale
     assert np.abs(r1).max() < 1e-12 * scale
+
+
+def _chain5(A, O, meshes):
+    m = meshes("chain5")
+    om = O.OracleMesh(m, A)
+    rp, cv = om.csr()
+    assert [list(cv[rp[i]:rp[i + 1]]) for i in range(5)] == \
+        [[0, 1], [0, 1, 2], [1, 2, 3], [2, 3, 4], [3, 4]]
+    return om
+
+
+def test_jacobi_smoother_golden(A, O, meshes):
+    """The system of test_smoothers.jl:32-46 (300/200 diagonal, -100
+    off-diagonals, b = [1e5, 0, 0, 0, 2e4], solution [460, 380, 300, 220,
+    140]) with JacobiSmoother(loops, 1) from zero.  The reference asserts
+    1e-10 after 50 loops, which no Jacobi iteration reaches on this matrix:
+    its iteration matrix I - D^-1 A has spectral radius 0.816, so 50 loops
+    leave an error of 2.5e-2 in norm.  Pinned here: the error after L loops
+    is the iteration matrix's power applied to the start error (to rounding)
+    for L = 50, and 200 loops reach the dense solution within 1e-10."""
+    om = _chain5(A, O, meshes)
+    v = np.array([300, -100, -100, 200, -100, -100, 200, -100, -100, 200, -100, -100, 300.0])
+    b = np.array([200 * 500, 0, 0, 0, 200 * 100.0])
+    M = dense(om, v)
+    xs = np.linalg.solve(M, b)
+    assert np.allclose(xs, [460, 380, 300, 220, 140], rtol=1e-14)
+    G = np.eye(5) - np.linalg.solve(np.diag(np.diag(M)), M)
+    rho = np.abs(np.linalg.eigvals(G)).max()
+    assert 0.81 < rho < 0.82
+    e50 = np.linalg.matrix_power(G, 50) @ (np.zeros(5) - xs)
+    x50 = om.smooth_jacobi(v, b, np.zeros(5), 50, 1.0)
+    assert np.allclose(x50 - xs, e50, rtol=1e-8, atol=1e-10)
+    assert 1e-2 < np.linalg.norm(x50 - xs) < 5e-2
+    x200 = om.smooth_jacobi(v, b, np.zeros(5), 200, 1.0)
+    assert np.allclose(x200, xs, rtol=0, atol=1e-10)
+
+
+def test_jacobi_smoother_weighted_sweeps(A, O, meshes):
+    """Two sweeps of weight 2/3 equal the matrix form
+    x <- omega D^-1 (b - (A - D) x) + (1 - omega) x, every row reading the
+    previous sweep (a Gauss-Seidel order or a dropped weight differs)."""
+    om = _chain5(A, O, meshes)
+    rng = np.random.default_rng(5)
+    v = np.array([300, -100, -100, 200, -100, -100, 200, -100, -100, 200, -100, -100, 300.0])
+    v = v * rng.uniform(0.9, 1.1, v.size)
+    M = dense(om, v)
+    D = np.diag(np.diag(M))
+    b = rng.standard_normal(5)
+    x0 = rng.standard_normal(5)
+    w = 2 / 3
+    ref = x0.copy()
+    for _ in range(2):
+        ref = w * np.linalg.solve(D, b - (M - D) @ ref) + (1 - w) * ref
+    assert np.allclose(om.smooth_jacobi(v, b, x0, 2, w), ref, rtol=1e-13)
+
+
+def test_smoother_constructors(x):
+    """JacobiSmoother(mesh): 5 loops, omega 1; JacobiSmoother(domain, loops):
+    omega 2/3 (test_smoothers.jl:16-30)."""
+    api = x.api
+    m = x.mesh.cavity2d(4)
+    s = api.JacobiSmoother(m)
+    assert (s.loops, s.omega) == (5, 1.0)
+    s = api.JacobiSmoother(domain=m, loops=5)
+    assert s.loops == 5 and s.omega == pytest.approx(2 / 3)
+    c = api.SolverSetup(solver=api.Cg(), preconditioner=api.Jacobi(), convergence=1e-7,
+                        relax=1.0, smoother=s).c()
+    assert c.smoother_loops == 5 and c.smoother_omega == pytest.approx(2 / 3)
+    assert api.SolverSetup(solver=api.Cg(), preconditioner=api.Jacobi(), convergence=1e-7,
+                           relax=1.0).c().smoother_loops == 0
+
+
+def test_crank_nicolson_step_closed_form(A, O, meshes):
+    """solve_system! with Time{CrankNicolson} first: x = 2 x* - x_old, x* the
+    Krylov solution and x_old the field it started from -- after the
+    smoother when one is set."""
+    m = meshes("cavity16")
+    om = O.OracleMesh(m, A)
+    n = m.n_cells
+    rng = np.random.default_rng(7)
+    terms = [(A.TIME_EULER, 1, 1.0), (A.LAPLACIAN_LINEAR, -1, 0.1)]
+    prev = rng.standard_normal(n)
+    nz, b = om.discretise_scalar(terms, [], prev, dt=0.01)
+    x0 = rng.standard_normal(n)
+    xs = np.linalg.solve(dense(om, nz), b)
+    setup = A.SolverSetupC(A.SOLVER_CG, A.PRECON_DILU, A.PREC_FP64, 1000, 1e-7, 1.0, 1e-300,
+                           1e-15, 0.0, 0, 0.0)
+    x, _, _ = om.solve_system(setup, nz, b, x0, cn=True)
+    assert np.allclose(x, 2 * xs - x0, rtol=0, atol=1e-11 * np.abs(xs).max())
+    x_euler, _, _ = om.solve_system(setup, nz, b, x0, cn=False)
+    assert np.allclose(x_euler, xs, rtol=0, atol=1e-11 * np.abs(xs).max())
+    setup.smoother_loops, setup.smoother_omega = 3, 2 / 3
+    x, _, _ = om.solve_system(setup, nz, b, x0, cn=True)
+    xsm = om.smooth_jacobi(nz, b, x0, 3, 2 / 3)
+    assert np.allclose(x, 2 * xs - xsm, rtol=0, atol=1e-11 * np.abs(xs).max())
+
+
+def test_crank_nicolson_assembles_as_euler(A, O, meshes):
+    """Time{CrankNicolson}'s coefficients are Time{Euler}'s
+    (Discretise_1_schemes.jl:44-62)."""
+    m = meshes("cavity16")
+    om = O.OracleMesh(m, A)
+    prev = np.linspace(0.0, 1.0, m.n_cells)
+    e = om.discretise_scalar([(A.TIME_EULER, 1, 1.0), (A.LAPLACIAN_LINEAR, -1, 0.1)], [], prev,
+                             dt=0.02)
+    c = om.discretise_scalar([(A.TIME_CRANK_NICOLSON, 1, 1.0), (A.LAPLACIAN_LINEAR, -1, 0.1)],
+                             [], prev, dt=0.02)
+    assert np.array_equal(e[0], c[0]) and np.array_equal(e[1], c[1])
+
+
+def test_steady_bfs_reference_check(x, A, O, meshes):
+    """test/0_TEST_CASES/2d_incompressible_laminar_BFS.jl:93 with its
+    JacobiSmoother on U and p: after 500 SIMPLE iterations the outlet's
+    average velocity is half the inlet's within 0.1 U."""
+    api = x.api
+    m = meshes("bfs10")
+    model, config = steady_bfs_case(api, m)
+    setup = api.simple_setup(model, config)
+    assert setup.U.smoother_loops == 5 and setup.p.smoother_loops == 5
+    out = O.OracleMesh(m, A).simple_run(setup, model.momentum.U, model.momentum.p, 500)
+    inlet = np.array([0.5, 0.0, 0.0])
+    outlet = patch_face_average(m, out["U"], "outlet")
+    assert np.linalg.norm(outlet - 0.5 * inlet) <= 0.1 * 0.5, outlet
+    assert np.all(np.isfinite(out["U"])) and np.all(np.isfinite(out["p"]))
+
+
+def test_docs_bfs_verification_profile(x, A, O, meshes):
+    """The docs' verification profile (docs/src/examples/figures/01/
+    2D_laminar_BFS_XCALibre.csv, Ux, Uy and p at x = 0.501 from an earlier
+    release of the reference): the converged oracle run of the same case
+    agrees within 2% of the peak velocity in U and 0.01 in p.  A loose
+    end-to-end pin: the release that wrote the file is not known."""
+    api = x.api
+    m = meshes("bfs10")
+    model, config = docs_bfs_case(api, m)
+    setup = api.simple_setup(model, config)
+    out = O.OracleMesh(m, A).simple_run(setup, model.momentum.U, model.momentum.p, 2000)
+    assert out["done"] < 2000  # converged to 1e-7
+    prof = bfs_verification_profile()
+    assert len(prof) == 20
+    cc = m.cell_centre.reshape(-1, 3)
+    cells = [int(np.argmin(np.hypot(cc[:, 0] - xp, cc[:, 1] - yp) +
+                           1e9 * (np.abs(cc[:, 0] - xp) > 0.005) + 1e9 * (np.abs(cc[:, 1] - yp) > 0.005)))
+             for xp, yp in prof[:, :2]]
+    peak = np.abs(prof[:, 2]).max()
+    assert np.abs(out["U"][0, cells] - prof[:, 2]).max() < 0.02 * peak
+    assert np.abs(out["U"][1, cells] - prof[:, 3]).max() < 0.02 * peak
+    assert np.abs(out["p"][cells] - prof[:, 4]).max() < 0.01
+

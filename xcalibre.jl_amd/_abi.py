@@ -8,6 +8,7 @@ import numpy as np
 # operator types
 TIME_STEADY, TIME_EULER, LAPLACIAN_LINEAR, DIV_LINEAR, DIV_UPWIND, DIV_LUST, \
     DIV_BOUNDED_UPWIND, SI = range(8)
+TIME_CRANK_NICOLSON = 8
 # boundary condition kinds
 BC_DIRICHLET, BC_NEUMANN, BC_ZEROGRADIENT, BC_EXTRAPOLATED, BC_WALL, BC_SYMMETRY = range(6)
 BC_KWALL, BC_OMEGAWALL, BC_NUTWALL, BC_EMPTY = 6, 7, 8, 9
@@ -67,7 +68,8 @@ class BC(C.Structure):
 class SolverSetupC(C.Structure):
     _fields_ = [("solver", C.c_int32), ("precon", C.c_int32), ("precision", C.c_int32),
                 ("itmax", C.c_int32), ("convergence", C.c_double), ("relax", C.c_double),
-                ("atol", C.c_double), ("rtol", C.c_double), ("ir_inner_rtol", C.c_double)]
+                ("atol", C.c_double), ("rtol", C.c_double), ("ir_inner_rtol", C.c_double),
+                ("smoother_loops", C.c_int32), ("smoother_omega", C.c_double)]
 
 
 class SimpleSetupC(C.Structure):

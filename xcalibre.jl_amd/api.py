@@ -40,11 +40,15 @@ from .writers import VTK, OpenFOAM, initialise_writer, write_results  # noqa: F4
 # ------------------------------------------------------------- schemes
 class SteadyState: ...
 class Euler: ...
-class CrankNicolson: ...  # assembles as Euler (Discretise_1_schemes.jl:44-58)
+# Crank-Nicolson assembles as Euler (Discretise_1_schemes.jl:44-62); the
+# equation whose first term it is then takes x = 2x - x_old after each solve
+# (Solve_1_api.jl:291-296).
+class CrankNicolson: ...
 
 
 # operator type of each time scheme (xcal_term.type)
-TIME = {SteadyState: A.TIME_STEADY, Euler: A.TIME_EULER, CrankNicolson: A.TIME_EULER}
+TIME = {SteadyState: A.TIME_STEADY, Euler: A.TIME_EULER,
+        CrankNicolson: A.TIME_CRANK_NICOLSON}
 class Linear: ...
 class Upwind: ...
 class LUST: ...
@@ -75,6 +79,25 @@ class DILU: ...
 
 
 @dataclass
+class JacobiSmoother:
+    """Weighted Jacobi smoother (Smoothers_jacobi.jl:36-51): `loops` sweeps
+    with weight `omega` on the field before each Krylov solve.
+    JacobiSmoother(mesh) gives 5 loops and omega 1;
+    JacobiSmoother(domain=mesh, loops=L, omega=2/3) the keyword form."""
+    domain: Any = None
+    loops: int = 5
+    omega: float = 1.0
+
+    def __init__(self, mesh=None, *, domain=None, loops=None, omega=2.0 / 3.0):
+        if mesh is not None:  # JacobiSmoother(mesh): 5 loops, omega 1
+            self.domain, self.loops, self.omega = mesh, 5, 1.0
+            return
+        if domain is None or loops is None:
+            raise TypeError("JacobiSmoother(; domain, loops, omega=2/3) needs domain and loops")
+        self.domain, self.loops, self.omega = domain, int(loops), float(omega)
+
+
+@dataclass
 class SolverSetup:
     solver: Any
     preconditioner: Any
@@ -92,8 +115,8 @@ class SolverSetup:
     def __post_init__(self):
         if self.atol is None:
             self.atol = float(np.finfo(self.float_type).eps) ** 0.9
-        if self.smoother is not None:
-            raise NotImplementedError("smoothers are not part of the device path")
+        if self.smoother is not None and not isinstance(self.smoother, JacobiSmoother):
+            raise NotImplementedError(f"smoother {self.smoother!r} is not supported")
 
     def c(self) -> A.SolverSetupC:
         solver = {Cg: A.SOLVER_CG, Bicgstab: A.SOLVER_BICGSTAB}.get(type(self.solver))
@@ -104,9 +127,11 @@ class SolverSetup:
         if pre is None:
             raise NotImplementedError(f"preconditioner {self.preconditioner!r} is not supported")
         prec = {"fp64": A.PREC_FP64, "fp32-ir": A.PREC_FP32_IR}[self.precision]
+        sm = self.smoother
         return A.SolverSetupC(solver, pre, prec, int(self.itmax), float(self.convergence),
                               float(self.relax), float(self.atol), float(self.rtol),
-                              float(self.inner_rtol))
+                              float(self.inner_rtol), int(sm.loops) if sm else 0,
+                              float(sm.omega) if sm else 0.0)
 
 
 @dataclass
@@ -329,22 +354,16 @@ def save_output(model: Physics, outputWriter, iteration: int, time, config: Conf
                          directory=directory)
 
 
-def _converged(R: np.ndarray, config: Configuration, is3d: bool) -> bool:
-    """The SIMPLE stop test on the last iteration's residuals
-    (Solvers_1_SIMPLE.jl:225-229)."""
-    u, p = config.solvers["U"].convergence, config.solvers["p"].convergence
-    return bool(R[0, -1] <= u and R[1, -1] <= u and (not is3d or R[2, -1] <= u)
-                and R[3, -1] <= p)
-
-
 def run(model: Physics, config: Configuration, pref=None, ncorrectors: int = 0,
-        inner_loops: int = 0, output=VTK(), directory=None):
-    """run!(model, config; output, pref, ncorrectors) for incompressible
-    flow: SIMPLE for a Steady model (laminar or k-omega), PISO with 2
-    pressure correctors per time step for a Transient one (laminar), as run!
-    calls piso!(model, config; pref) (Solvers_3_solver_dispatch.jl:186-190).
-    Returns the residual histories (Ux, Uy, Uz, p) and leaves the final
-    fields in model.momentum.
+        inner_loops: Optional[int] = None, output=VTK(), directory=None):
+    """run!(model, config; output, pref, ncorrectors, inner_loops) for
+    incompressible flow: SIMPLE for a Steady model (laminar or k-omega), PISO
+    with `inner_loops` pressure correctors per time step (default 2) for a
+    Transient one (laminar), as run! calls piso!(model, config; pref,
+    inner_loops) (Solvers_3_solver_dispatch.jl:186-190, 224-235).  Returns
+    the residual histories (Ux, Uy, Uz, p) and leaves the final fields in
+    model.momentum.  The SIMPLE loop stops where the device's convergence
+    test (every residual, a turbulence model's included) stops it.
 
     Results are written as the reference writes them: with
     config.runtime.write_interval w >= 0, after every iteration that is a
@@ -357,6 +376,10 @@ def run(model: Physics, config: Configuration, pref=None, ncorrectors: int = 0,
     whose constant/polyMesh is written at the start of the run."""
     if ncorrectors < 0:
         raise ValueError("ncorrectors must be >= 0")
+    if inner_loops is None:
+        inner_loops = 2
+    elif inner_loops < 1:
+        raise ValueError("inner_loops must be >= 1")
     if not isinstance(config.hardware.backend, HIPBackend):
         raise TypeError("the device path runs on HIPBackend()")
     wi = config.runtime.write_interval
@@ -396,15 +419,14 @@ def run(model: Physics, config: Configuration, pref=None, ncorrectors: int = 0,
     parts, it, time = [], 0, 0.0
     while it < iterations:
         chunk = min(step - it % step, iterations - it)
-        out = sim.run_piso(chunk, 2) if transient else sim.run(chunk)
+        out = sim.run_piso(chunk, inner_loops) if transient else sim.run(chunk)
         parts.append(out["residuals"])
         done = out["done"]
         if transient:
             for _ in range(done):
                 time += config.runtime.time_step
         it += done
-        stop = not transient and done > 0 and (
-            done < chunk or _converged(out["residuals"], config, model.domain.dim == 3))
+        stop = not transient and done > 0 and out["converged"]
         if writer is not None and (stop or it % wi == 0):
             fetch()
             save_output(model, writer, it, time if transient else it, config, directory)

@@ -152,6 +152,7 @@ __global__ void k_set_reference(const int32_t* diag, double* nzval, double* b, d
 
 void discretise(xcal_eqn* e, const TermsK& t, const SourcesK& s, const xcal_field* prev, double dt) {
   xcal_mesh* m = e->mesh;
+  e->cn = t.cn_first != 0;
   MeshK mk = meshk(m);
   double* nz = e->vector ? e->nzval0.p : e->nzval.p;
   const double* px = prev ? prev->comp(0) : nullptr;
@@ -220,7 +221,7 @@ static void check_terms(const xcal_eqn* e, const xcal_term* t, int nt) {
     if (!t[i].flux) continue;
     XCAL_CHECK(t[i].flux->mesh == e->mesh, "term flux belongs to another mesh");
     bool cellwise = t[i].type == XCAL_TIME_STEADY || t[i].type == XCAL_TIME_EULER ||
-                    t[i].type == XCAL_SI;
+                    t[i].type == XCAL_TIME_CRANK_NICOLSON || t[i].type == XCAL_SI;
     XCAL_CHECK(t[i].flux->kind == (cellwise ? XCAL_CELL_SCALAR : XCAL_FACE_SCALAR),
                "term flux must be a face scalar field (Laplacian/Divergence) or a cell scalar "
                "field (Time/Si)");
@@ -270,7 +271,8 @@ int xcal_discretise(xcal_eqn* e, const xcal_term* terms, int32_t nterms, const x
     TermsK t = make_terms(terms, nterms);
     SourcesK s = make_sources(e, src, nsrc);
     bool needs_prev = false;
-    for (int i = 0; i < nterms; ++i) needs_prev |= terms[i].type == XCAL_TIME_EULER;
+    for (int i = 0; i < nterms; ++i)
+      needs_prev |= terms[i].type == XCAL_TIME_EULER || terms[i].type == XCAL_TIME_CRANK_NICOLSON;
     if (prev) check_field(prev, e->mesh, e->vector ? XCAL_CELL_VECTOR : XCAL_CELL_SCALAR, "prev");
     XCAL_CHECK(!needs_prev || prev, "Time{Euler} needs the previous field");
     XCAL_CHECK(!needs_prev, "Time{Euler} needs runtime.dt: use xcal_discretise_dt");
@@ -288,7 +290,8 @@ int xcal_discretise_dt(xcal_eqn* e, const xcal_term* terms, int32_t nterms, cons
     TermsK t = make_terms(terms, nterms);
     SourcesK s = make_sources(e, src, nsrc);
     bool needs_prev = false;
-    for (int i = 0; i < nterms; ++i) needs_prev |= terms[i].type == XCAL_TIME_EULER;
+    for (int i = 0; i < nterms; ++i)
+      needs_prev |= terms[i].type == XCAL_TIME_EULER || terms[i].type == XCAL_TIME_CRANK_NICOLSON;
     if (prev) check_field(prev, e->mesh, e->vector ? XCAL_CELL_VECTOR : XCAL_CELL_SCALAR, "prev");
     XCAL_CHECK(!needs_prev || prev, "Time{Euler} needs the previous field");
     discretise(e, t, s, prev, dt);

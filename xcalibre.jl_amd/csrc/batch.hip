@@ -884,6 +884,7 @@ int solve_system_batch(xcal_eqn* e, const xcal_solver_setup* st, double* x, cons
   const int K = e->mesh->ncase;
   const int itmax = st->itmax == 0 ? (int)(2 * e->mesh->n) : st->itmax;
   std::vector<int> niter(K, 0), nouter(K, 1), rcs(K, 0);
+  solve_prologue(e, st, x, b);
   if (st->precision == XCAL_PREC_FP32_IR) {
     bir_solve(e, st, itmax, x, b, niter, nouter, rcs);
   } else {
@@ -898,6 +899,7 @@ int solve_system_batch(xcal_eqn* e, const xcal_solver_setup* st, double* x, cons
       rcs[k] = (h[k].stop == STOP_ITMAX || h[k].iter >= itmax) ? 1 : 0;
     }
   }
+  solve_epilogue(e, x);
   BSolver<double> S = batch_solver(e);
   std::vector<double> res(K);
   bresidual(S, x, b, res.data());

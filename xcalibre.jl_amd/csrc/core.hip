@@ -24,9 +24,14 @@ TermsK make_terms(const xcal_term* terms, int n) {
   TermsK t{};
   t.n = n;
   for (int i = 0; i < n; ++i) {
-    XCAL_CHECK(terms[i].type >= XCAL_TIME_STEADY && terms[i].type <= XCAL_SI, "unknown operator type");
+    XCAL_CHECK((terms[i].type >= XCAL_TIME_STEADY && terms[i].type <= XCAL_SI) ||
+                   terms[i].type == XCAL_TIME_CRANK_NICOLSON,
+               "unknown operator type");
     XCAL_CHECK(terms[i].sign == 1 || terms[i].sign == -1, "operator sign must be +1 or -1");
-    t.t[i].type = terms[i].type;
+    // Crank-Nicolson's coefficients are Euler's (Discretise_1_schemes.jl:44-62)
+    const bool cn = terms[i].type == XCAL_TIME_CRANK_NICOLSON;
+    if (i == 0) t.cn_first = cn;
+    t.t[i].type = cn ? XCAL_TIME_EULER : terms[i].type;
     t.t[i].sign = terms[i].sign;
     t.t[i].flux = terms[i].flux ? terms[i].flux->comp(0) : nullptr;
     t.t[i].flux_const = terms[i].flux_const;
@@ -1003,7 +1008,7 @@ using namespace xcal;
 extern "C" {
 
 const char* xcal_last_error(void) { return g_last_error.c_str(); }
-int32_t xcal_abi_version(void) { return 1; }
+int32_t xcal_abi_version(void) { return 2; }
 
 int xcal_ctx_create(int32_t device, xcal_ctx** out) {
   return guarded([&] {

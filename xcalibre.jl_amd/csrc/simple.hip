@@ -29,6 +29,9 @@ struct xcal_simple {
   bool initialised = false;
   // serving policy: end the run at the first non-finite residual
   bool stop_nonfinite = false, diverged = false;
+  // the last run stopped on the convergence test of Solvers_1_SIMPLE.jl:226-233
+  // (every residual, the turbulence model's included, below its setting)
+  bool converged = false;
   // k-omega model (RANS_kOmega.jl), off unless set
   bool turb = false;
   xcal_turbulence_setup tset{};
@@ -107,9 +110,13 @@ void check_setup(const xcal_mesh* m, const xcal_simple_setup* s) {
     XCAL_CHECK(s->bc_U[i].kind != XCAL_BC_NEUMANN, "Neumann is not defined for vector fields");
   for (int g : {s->grad_p, s->grad_U})
     XCAL_CHECK(g == XCAL_GRAD_GAUSS || g == XCAL_GRAD_MIDPOINT, "gradient scheme: Gauss or Midpoint");
-  XCAL_CHECK(s->time_U == XCAL_TIME_STEADY || s->time_U == XCAL_TIME_EULER,
-             "momentum time scheme: SteadyState or Euler");
-  XCAL_CHECK(s->time_U != XCAL_TIME_EULER || s->dt > 0.0, "Time{Euler} needs runtime.dt > 0");
+  XCAL_CHECK(s->time_U == XCAL_TIME_STEADY || s->time_U == XCAL_TIME_EULER ||
+                 s->time_U == XCAL_TIME_CRANK_NICOLSON,
+             "momentum time scheme: SteadyState, Euler or CrankNicolson");
+  XCAL_CHECK(s->time_U == XCAL_TIME_STEADY || s->dt > 0.0,
+             "Time{Euler} and Time{CrankNicolson} need runtime.dt > 0");
+  for (const xcal_solver_setup* ss : {&s->U, &s->p})
+    XCAL_CHECK(ss->smoother_loops >= 0, "smoother loops must be >= 0");
   XCAL_CHECK(s->ncorrectors >= 0, "ncorrectors must be >= 0");
 }
 
@@ -298,6 +305,7 @@ int run_impl(xcal_simple* S, int32_t iterations, std::vector<double>& R, std::ve
   {
     xcal_mesh* m = S->mesh;
     const int K = m->ncase;
+    S->converged = false;
     R.assign((size_t)K * 4 * iterations, 0.0);
     KIT.assign((size_t)K * 4 * iterations, 0);
     std::vector<xcal_solve_stats> cst(K);
@@ -328,7 +336,7 @@ int run_impl(xcal_simple* S, int32_t iterations, std::vector<double>& R, std::ve
     xcal_eqn* Ue = S->Ueqn;
     xcal_eqn* pe = S->peqn;
     const bool is3d = m->dim == 3;
-    const double dt = cfg.time_U == XCAL_TIME_EULER ? cfg.dt : 1.0;
+    const double dt = cfg.time_U != XCAL_TIME_STEADY ? cfg.dt : 1.0;
     const int ncorr = inner > 0 ? inner : 1;
     std::memset(S->stage_ms, 0, sizeof(S->stage_ms));
 
@@ -497,6 +505,7 @@ int run_impl(xcal_simple* S, int32_t iterations, std::vector<double>& R, std::ve
       }
       if (inner > 0) conv = false;  // a transient run takes every time step
       if (conv) {
+        S->converged = true;
         ++it;
         break;
       }
@@ -542,6 +551,14 @@ int xcal_simple_run(xcal_simple* S, int32_t iterations, double* residuals, int32
     const int it = run_impl(S, iterations, R, KIT, 0);
     report(S, iterations, R, KIT, residuals, kit);
     *done = it;
+    return 0;
+  });
+}
+
+int xcal_simple_converged(const xcal_simple* S, int32_t* converged) {
+  return guarded([&] {
+    XCAL_CHECK(S && converged, "NULL argument");
+    *converged = S->converged ? 1 : 0;
     return 0;
   });
 }

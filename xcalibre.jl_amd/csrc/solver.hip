@@ -302,6 +302,33 @@ __global__ void __launch_bounds__(kBlock)
   if (i < n) x0[i] = x[i] + x0[i];
 }
 
+// One weighted-Jacobi sweep of apply_smoother! (Smoothers_jacobi.jl:72-81,
+// multiply_row :97-110): the row sum over every entry in column order, less
+// the diagonal's product, then omega/a_ii (b - sum) + (1 - omega) x_i.
+// Every row reads the previous sweep's values (xin) and writes xout.
+template <int G>
+__global__ void __launch_bounds__(kBlock)
+    k_jacobi_smooth(MatK A, const double* __restrict__ b, const double* __restrict__ xin,
+                    double* __restrict__ xout, double omega) {
+  int64_t end;
+  const int64_t i = block_row(A, end);
+  if (i >= end) return;
+  double sum = row_sum<G>(A, i, [&](int j) { return xin[j]; });
+  const double aii = A.nz[A.diag[i]];
+  const double xi = xin[i];
+  sum -= aii * xi;
+  const double rD = 1.0 / aii;
+  xout[i] = (omega * rD) * (b[i] - sum) + (1.0 - omega) * xi;
+}
+
+// Crank-Nicolson's explicit step after the solve (Solve_1_api.jl:291-296):
+// x = 2 x - x_old, x_old the field the Krylov solve started from.
+__global__ void __launch_bounds__(kBlock)
+    k_cn_step(int64_t n, const double* __restrict__ xold, double* __restrict__ x) {
+  const int64_t i = xcd_row();
+  if (i < n) x[i] = 2.0 * x[i] - xold[i];
+}
+
 // --------------------------------------------- mixed-precision refinement
 // r = b - A x in FP64, stored as FP32 for the inner solve; partials of
 // |r|^2 and |b|^2
@@ -981,6 +1008,48 @@ static int ir_solve(xcal_eqn* e, const xcal_solver_setup* st, int itmax, double*
   return rc;
 }
 
+// The steps of solve_system! (Solve_1_api.jl:268-307) around the Krylov
+// solve: apply_smoother! on the field first (:285), and for an equation whose
+// first term is Time{CrankNicolson}, x = 2 x - x_old after it (:291-296).
+// Workspace: vectors 10 and 11 of the equation's FP64 Krylov workspace,
+// which no Krylov form uses.
+void solve_prologue(xcal_eqn* e, const xcal_solver_setup* st, double* x, const double* b) {
+  XCAL_CHECK(st->smoother_loops >= 0, "smoother loops must be >= 0");
+  const xcal_mesh* m = e->mesh;
+  const int64_t n = m->n;
+  if (st->smoother_loops == 0 && !e->cn) return;
+  hipStream_t s = m->ctx->stream;
+  if (e->work.n < 12 * n) {
+    XCAL_HIP(hipStreamSynchronize(s));  // a replaced buffer may be in use
+    e->work.alloc(12 * n);
+  }
+  if (st->smoother_loops > 0) {
+    const MatK A = matk(m, (const double*)e->nzval.p);
+    double* src = x;
+    double* dst = e->work.p + 10 * n;
+    for (int l = 0; l < st->smoother_loops; ++l) {
+      with_row_group(A.rw, [&](auto G) {
+        hipLaunchKernelGGL((k_jacobi_smooth<decltype(G)::value>), dim3(m->nxblk), dim3(kBlock), 0,
+                           s, A, b, (const double*)src, dst, st->smoother_omega);
+      });
+      std::swap(src, dst);
+    }
+    if (src != x)
+      XCAL_HIP(hipMemcpyAsync(x, src, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+  }
+  if (e->cn)
+    XCAL_HIP(hipMemcpyAsync(e->work.p + 11 * n, x, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+  XCAL_HIP(hipGetLastError());
+}
+
+void solve_epilogue(xcal_eqn* e, double* x) {
+  if (!e->cn) return;
+  const xcal_mesh* m = e->mesh;
+  hipLaunchKernelGGL(k_cn_step, dim3(grid_for(m->n)), dim3(kBlock), 0, m->ctx->stream, m->n,
+                     (const double*)(e->work.p + 11 * m->n), x);
+  XCAL_HIP(hipGetLastError());
+}
+
 int solve_system(xcal_eqn* e, const xcal_solver_setup* st, double* x, const double* b,
                  xcal_solve_stats* stats) {
   XCAL_CHECK(st->precision == XCAL_PREC_FP64 || st->precision == XCAL_PREC_FP32_IR,
@@ -1004,6 +1073,7 @@ int solve_system(xcal_eqn* e, const xcal_solver_setup* st, double* x, const doub
   }
   const int itmax = st->itmax == 0 ? (int)(2 * e->mesh->n) : st->itmax;
   int niter = 0, nouter = 1, rc;
+  solve_prologue(e, st, x, b);
   if (st->precision == XCAL_PREC_FP32_IR) {
     rc = ir_solve(e, st, itmax, x, b, niter, nouter);
   } else {
@@ -1013,6 +1083,7 @@ int solve_system(xcal_eqn* e, const xcal_solver_setup* st, double* x, const doub
     else
       rc = bicgstab(S, st->precon, b, x, itmax, st->atol, st->rtol, niter, (double**)nullptr);
   }
+  solve_epilogue(e, x);
   if (stats) {
     stats->niter = niter;
     stats->nouter = nouter;

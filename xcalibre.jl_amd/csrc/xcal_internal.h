@@ -248,6 +248,9 @@ struct xcal_eqn {
   double* bc(int c) const { return b.p + (int64_t)c * mesh->n; }
   // preconditioner storage
   int precon = XCAL_PRECON_NONE;
+  // the last discretise!'s first term was Time{CrankNicolson}: solves take
+  // the step x = 2 x - x_old (Solve_1_api.jl:291-296)
+  bool cn = false;
   xcal::DBuf<double> pS;   // Jacobi 1/|a_ii|
   xcal::DBuf<double> pD;   // DILU D
   xcal::DBuf<double> prD;  // DILU 1/D
@@ -286,6 +289,7 @@ struct TermK {
 };
 struct TermsK {
   int n;
+  int cn_first;  // terms[0] is Time{CrankNicolson} (assembled as Time{Euler})
   TermK t[kMaxTerms];
 };
 struct SourcesK {
@@ -376,6 +380,8 @@ void turb_nueff(const xcal_mesh* m, double nu, const double* nutf, double* nueff
 void spmv(const xcal_eqn* e, const double* x, double* y);
 void apply_M_levels(const xcal_eqn* e, const double* r, double* z);
 void precond_update(xcal_eqn* e, int precon);
+void solve_prologue(xcal_eqn* e, const xcal_solver_setup* st, double* x, const double* b);
+void solve_epilogue(xcal_eqn* e, double* x);
 int solve_system(xcal_eqn* e, const xcal_solver_setup* s, double* x, const double* b,
                  xcal_solve_stats* st);
 // batch meshes: every case solved independently; st has one entry per case

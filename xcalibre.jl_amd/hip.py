@@ -98,6 +98,7 @@ def lib():
             "xcal_simple_set_field": [_vp, C.c_int32, _pd],
             "xcal_simple_get_field": [_vp, C.c_int32, _pd],
             "xcal_simple_run": [_vp, C.c_int32, _pd, _pi32, _pi32],
+            "xcal_simple_converged": [_vp, _pi32],
             "xcal_piso_run": [_vp, C.c_int32, C.c_int32, _pd, _pi32, _pi32],
             "xcal_simple_stage_times": [_vp, _pd],
             "xcal_kernel_timing_enable": [_vp, C.c_int32, C.c_int32],
@@ -380,8 +381,10 @@ class Simple:
         check(lib().xcal_simple_run(self.h, iterations, _pd_of(res), C.byref(done),
                                     kit.ctypes.data_as(_pi32)))
         d = done.value
+        conv = C.c_int32()
+        check(lib().xcal_simple_converged(self.h, C.byref(conv)))
         return dict(done=d, residuals=res.reshape(4, iterations)[:, :d],
-                    krylov_iters=kit.reshape(4, iterations)[:, :d])
+                    krylov_iters=kit.reshape(4, iterations)[:, :d], converged=bool(conv.value))
 
     def run_piso(self, steps: int, inner_loops: int = 2):
         """Transient PISO time steps (setup dt); residuals per step."""

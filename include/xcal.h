@@ -188,6 +188,15 @@ int xcal_mesh_info(const xcal_mesh* mesh, int64_t* n_cells, int64_t* nnz,
  * rows back).  Single meshes: the widest pipelined run of small levels.  No
  * reference counterpart (reports a device code path). */
 int xcal_mesh_sweep_form(const xcal_mesh* mesh, int32_t* fwd, int32_t* bwd);
+/* Cone form of the DILU sweeps (single meshes): rows per block, blocks, and
+ * the cone rows of all blocks for the forward and backward sweeps; all 0
+ * where the sweeps launch level by level.  Each sweep is then one launch in
+ * which the workgroup of a block of cells sweeps the block and every row it
+ * depends on, in the reference's order.  No reference counterpart (reports a
+ * device code path; forward_substitution!/backward_substitution!,
+ * preconditioners_1_DILU.jl:84-114, are what the sweeps compute). */
+int xcal_mesh_cone_form(const xcal_mesh* mesh, int32_t* block_rows, int32_t* blocks,
+                        int64_t* fwd_rows, int64_t* bwd_rows);
 /* CSR pattern, 0-based (rowptr n+1, colval nnz) */
 int xcal_mesh_csr_pattern(const xcal_mesh* mesh, int32_t* rowptr, int32_t* colval);
 

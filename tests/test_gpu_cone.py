@@ -1,0 +1,140 @@
+"""Cone form of the DILU sweeps.
+
+On a randomly numbered mesh of at least 65 536 rows each sweep is one launch:
+the workgroup of a block of cells sweeps the block's rows and every row they
+depend on (the block's dependency cone) in LDS, computing the other blocks'
+rows again instead of waiting for them.  Every row uses the reference's
+formulas in its column order (forward_substitution!, backward_substitution!,
+preconditioners_1_DILU.jl:84-114), so the preconditioner applies bit for bit
+as the oracle and the level launches do; Krylov solves and SIMPLE runs differ
+from the level-launch form only by the order of their dot-product sums.
+"""
+import numpy as np
+import pytest
+
+from conftest import channel_case, rel
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx(x):
+    return x.hip.Context(0)
+
+
+@pytest.fixture(scope="module")
+def box48(x):
+    return x.mesh.box3d(48, 48, 48, jitter=0.1)  # 110 592 cells, random numbering
+
+
+def _mesh(x, ctx, mesh, monkeypatch, **env):
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    dm = x.hip.DeviceMesh(ctx, mesh)
+    for k in env:
+        monkeypatch.delenv(k)
+    return dm
+
+
+def _equation(x, A, dm, mesh, rng, sym):
+    H = x.hip
+    nf = mesh.n_faces
+    g = H.Field(dm, A.FACE_SCALAR).upload(0.5 + rng.random(nf))
+    mdh = np.zeros(nf) if sym else rng.standard_normal(nf) * mesh.face_area * 0.2
+    md = H.Field(dm, A.FACE_SCALAR).upload(mdh)
+    terms = [(A.DIV_UPWIND, 1, md), (A.LAPLACIAN_LINEAR, -1, g)]
+    src = H.Field(dm, A.CELL_SCALAR).upload(rng.standard_normal(mesh.n_cells))
+    eq = H.Equation(dm, False)
+    phi = H.Field(dm, A.CELL_SCALAR)
+    eq.discretise(terms, [(src, 1)], phi)
+    bcs = A.bc_array([(A.BC_DIRICHLET, i, 1.0) for i in range(len(mesh.patch_names))])
+    eq.apply_bcs(terms, bcs, phi)
+    return eq
+
+
+def test_cone_form_is_chosen_on_random_numbering(x, ctx, box48, monkeypatch):
+    dm = _mesh(x, ctx, box48, monkeypatch)
+    rows, nblk, fr, br = dm.cone_form()
+    n = box48.n_cells
+    assert rows == 2048 and nblk == (n + rows - 1) // rows
+    assert n < fr < 3 * n and n < br < 3 * n
+    assert _mesh(x, ctx, box48, monkeypatch, XCAL_NO_CONE="1").cone_form() == (0, 0, 0, 0)
+    natural = x.mesh.box3d(48, 48, 48, jitter=0.1, seed_perm=0)  # a chain crosses the box
+    assert _mesh(x, ctx, natural, monkeypatch).cone_form() == (0, 0, 0, 0)
+
+
+@pytest.mark.parametrize("rows,deep", [("2048", "0"), ("1024", "1"), ("256", "0")])
+def test_cone_preconditioner_bit_exact(x, A, O, ctx, box48, monkeypatch, rows, deep):
+    """z = M^-1 r through both sweeps equals the level launches and the
+    oracle's sequential loops bit for bit, for blocks of 2048, 1024 and 256
+    rows and both prefetch depths."""
+    H = x.hip
+    rng = np.random.default_rng(11)
+    dm = _mesh(x, ctx, box48, monkeypatch, XCAL_CONE_ROWS=rows, XCAL_CONE_DEEP=deep)
+    ref = _mesh(x, ctx, box48, monkeypatch, XCAL_NO_CONE="1")
+    assert dm.cone_form()[0] == int(rows)
+    r_h = rng.standard_normal(box48.n_cells)
+    out = []
+    for m in (dm, ref):
+        eq = _equation(x, A, m, box48, np.random.default_rng(5), sym=False)
+        eq.precond_update(A.PRECON_DILU)
+        z = H.Field(m, A.CELL_SCALAR)
+        eq.precond_apply(H.Field(m, A.CELL_SCALAR).upload(r_h), z)
+        out.append(z.download())
+    assert np.array_equal(out[0], out[1])
+    om = O.OracleMesh(box48, A)
+    rng5 = np.random.default_rng(5)
+    nf = box48.n_faces
+    gam = 0.5 + rng5.random(nf)
+    mdh = rng5.standard_normal(nf) * box48.face_area * 0.2
+    src = rng5.standard_normal(box48.n_cells)
+    oterms = [(A.DIV_UPWIND, 1, mdh), (A.LAPLACIAN_LINEAR, -1, gam)]
+    onz, ob = om.discretise_scalar(oterms, [(src, 1)])
+    bcs = A.bc_array([(A.BC_DIRICHLET, i, 1.0) for i in range(len(box48.patch_names))])
+    om.apply_bcs(oterms, bcs, -1, np.zeros(box48.n_cells), onz, ob)
+    oz = om.dilu_apply(onz, om.dilu_diagonal(onz), r_h)
+    assert np.array_equal(out[0], oz)
+
+
+@pytest.mark.parametrize("solver", ["SOLVER_CG", "SOLVER_BICGSTAB"])
+@pytest.mark.parametrize("prec", ["PREC_FP64", "PREC_FP32_IR"])
+def test_cone_krylov_equals_level_form(x, A, ctx, box48, monkeypatch, solver, prec):
+    """Solutions to rtol 1e-12 agree to 1e-9 relative (the dot products are
+    summed per block instead of per level launch); FP64 iteration counts
+    within 5% (CG) and 25% (BiCGSTAB)."""
+    H = x.hip
+    sym = solver == "SOLVER_CG"
+    res = []
+    for env in ({}, {"XCAL_NO_CONE": "1"}):
+        m = _mesh(x, ctx, box48, monkeypatch, **env)
+        eq = _equation(x, A, m, box48, np.random.default_rng(9), sym)
+        eq.precond_update(A.PRECON_DILU)
+        phi = H.Field(m, A.CELL_SCALAR)
+        setup = A.SolverSetupC(getattr(A, solver), A.PRECON_DILU, getattr(A, prec), 4000, 1e-7,
+                               1.0, 1e-30, 1e-12, 0.0)
+        it, r, rc = eq.solve(setup, phi)
+        assert rc == 0
+        res.append((phi.download(), it))
+    (a, ia), (b, ib) = res
+    assert rel(a, b) < 1e-9
+    if prec == "PREC_FP64":
+        slack = ib // 20 if sym else ib // 4
+        assert abs(ia - ib) <= max(2, slack), (ia, ib)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32-ir"])
+def test_cone_simple_channel_equals_level_form(x, ctx, box48, monkeypatch, prec):
+    """Five SIMPLE iterations of the channel case (p: CG+DILU to 1e-10, U:
+    BiCGSTAB+DILU) with cone and level sweeps agree to 1e-9 in U and p."""
+    api = x.api
+    out = []
+    for env in ({}, {"XCAL_NO_CONE": "1"}):
+        m = _mesh(x, ctx, box48, monkeypatch, **env)
+        model, config = channel_case(api, box48, iterations=5, precision_p=prec)
+        sim = x.hip.Simple(m, api.simple_setup(model, config))
+        sim.set_U(model.momentum.U)
+        sim.set_p(model.momentum.p)
+        sim.run(5)
+        out.append((sim.get("U"), sim.get("p")))
+    assert rel(out[0][0], out[1][0]) < 1e-9
+    assert rel(out[0][1], out[1][1]) < 1e-9

@@ -161,6 +161,15 @@ __global__ void __launch_bounds__(kBlock)
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p < n) Db[p] = D[rows[p]];
 }
+// out[p] = D of cone position p's row (crow holds row or ~row)
+__global__ void __launch_bounds__(kBlock)
+    k_gather_cone_rows(int64_t total, const int32_t* __restrict__ crow,
+                       const double* __restrict__ D, double* __restrict__ out) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= total) return;
+  const int32_t c = crow[p];
+  out[p] = D[c >= 0 ? c : ~c];
+}
 __global__ void __launch_bounds__(kBlock)
     k_recip(int64_t n, const double* __restrict__ D, double* __restrict__ rD) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -252,6 +261,25 @@ void precond_update(xcal_eqn* e, int precon) {
       launch_check();
     }
     hipLaunchKernelGGL(k_recip, dim3(grid_for(n)), dim3(kBlock), 0, s, n, e->pD.p, e->prD.p);
+    if (m->cone) {  // the cone sweeps' factors: a_ij, 1/D_j per entry; D per position
+      const xcal_mesh::Cone &F = m->cone_f, &B = m->cone_b;
+      if (e->cfa.n != F.entries) {
+        e->cfa.alloc(F.entries);
+        e->cfr.alloc(F.entries);
+      }
+      if (e->cba.n != B.entries) e->cba.alloc(B.entries);
+      if (e->cbd.n != B.rows) e->cbd.alloc(B.rows);
+      hipLaunchKernelGGL(k_gather_slots, dim3(grid_for(F.entries)), dim3(kBlock), 0, s, F.entries,
+                         F.slot.p, e->nzval.p, e->cfa.p);
+      hipLaunchKernelGGL(k_gather_cols, dim3(grid_for(F.entries)), dim3(kBlock), 0, s, F.entries,
+                         F.dep.p, e->prD.p, e->cfr.p);
+      hipLaunchKernelGGL(k_gather_slots, dim3(grid_for(B.entries)), dim3(kBlock), 0, s, B.entries,
+                         B.slot.p, e->nzval.p, e->cba.p);
+      hipLaunchKernelGGL(k_gather_cone_rows, dim3(grid_for(B.rows)), dim3(kBlock), 0, s, B.rows,
+                         B.crow.p, e->pD.p, e->cbd.p);
+      launch_check();
+      return;
+    }
     if (m->pipe_wf > 0 || m->pipe_runs) {  // 1/D per packed lower entry, for the pipelined sweeps
       if (e->Lrd.n != m->Lcol.n) e->Lrd.alloc(m->Lcol.n);
       if (m->Lcol.n > 0)

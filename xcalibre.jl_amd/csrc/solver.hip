@@ -29,6 +29,7 @@
 #include <functional>
 
 #include "solver_common.h"
+#include "sweep_cone.h"
 #include "sweep_pipe.h"
 
 namespace xcal {
@@ -464,6 +465,29 @@ __global__ void __launch_bounds__(kPipeBlock)
   if (DOT != 0) store_partial<kPipeBlock>(s0, s1, W.partial, slot);
 }
 
+// Cone form (sweep_cone.h): one workgroup per block, grid = the mesh's
+// block count.  Both return at entry once the solve has stopped.
+template <bool UPD, class T, int WK, int P>
+__global__ void __launch_bounds__(kConeBlock)
+    k_fwd_cone(ConeK C, const T* __restrict__ a, const T* __restrict__ rd, FwdK<T> W) {
+  __shared__ ConeLDS<T> L;
+  if (W.ks && stopped(W.ks)) return;  // uniform over the workgroup
+  fwd_cone<UPD, T, WK, P>(C, a, rd, W, L);
+}
+
+template <int DOT, class T, int WK, int P>
+__global__ void __launch_bounds__(kConeBlock)
+    k_bwd_cone(ConeK C, const T* __restrict__ a, const T* __restrict__ d, BwdK<T> W) {
+  __shared__ ConeLDS<T> L;
+  if (W.ks && stopped(W.ks)) return;
+  bwd_cone<DOT, T, WK, P>(C, a, d, W, L);
+}
+
+inline ConeK cone_k(const xcal_mesh* m, const xcal_mesh::Cone& c) {
+  return ConeK{c.blk.p,        c.base.p,          c.steps.p,   c.crow.p,
+               c.lidx.p,       m->cone_own.p,     m->cone_rows, (int32_t)m->n};
+}
+
 // ------------------------------------------------------------ host side
 // One Krylov solve's view of an equation in value type T: the matrix, the
 // preconditioner arrays and 12 n-vectors of workspace.
@@ -477,6 +501,7 @@ struct Solver {
   const T *D, *rD, *Sj;  // DILU D and 1/D; Jacobi 1/|a_ii|
   const T *Lnz, *Unz, *Db;  // packed DILU factors, D in backward-level order
   const T* Lrd;             // 1/D per packed lower entry (pipelined runs)
+  const T *cfa, *cfr, *cba, *cbd;  // cone form: packed a_ij, 1/D_j, a_ij, D
   T* work;
   KS* ks;
   double* partial;
@@ -530,6 +555,10 @@ static Solver<double> make_solver(xcal_eqn* e) {
   S.Unz = e->Unz.p;
   S.Db = e->Db.p;
   S.Lrd = e->Lrd.p;
+  S.cfa = e->cfa.p;
+  S.cfr = e->cfr.p;
+  S.cba = e->cba.p;
+  S.cbd = e->cbd.p;
   S.work = e->work.p;
   S.ks = (KS*)e->kstate.p;
   S.partial = e->kpart.p;
@@ -551,11 +580,13 @@ static Solver<float> make_solver32(xcal_eqn* e) {
   hipLaunchKernelGGL(k_to_float, dim3(grid_for(m->nzlen)), dim3(kBlock), 0, s, m->nzlen,
                      e->nzval.p, e->nz32.p);
   if (e->precon == XCAL_PRECON_DILU) {
+    hipLaunchKernelGGL(k_to_float, dim3(grid_for(n)), dim3(kBlock), 0, s, n, e->pD.p, e->pD32.p);
+    hipLaunchKernelGGL(k_to_float, dim3(grid_for(n)), dim3(kBlock), 0, s, n, e->prD.p, e->prD32.p);
+  }
+  if (e->precon == XCAL_PRECON_DILU && !m->cone) {  // the level sweeps' packed factors
     if (e->Lnz32.n != e->Lnz.n) e->Lnz32.alloc(e->Lnz.n);
     if (e->Unz32.n != e->Unz.n) e->Unz32.alloc(e->Unz.n);
     if (e->Db32.n != n) e->Db32.alloc(n);
-    hipLaunchKernelGGL(k_to_float, dim3(grid_for(n)), dim3(kBlock), 0, s, n, e->pD.p, e->pD32.p);
-    hipLaunchKernelGGL(k_to_float, dim3(grid_for(n)), dim3(kBlock), 0, s, n, e->prD.p, e->prD32.p);
     hipLaunchKernelGGL(k_to_float, dim3(grid_for(n)), dim3(kBlock), 0, s, n, e->Db.p, e->Db32.p);
     if (e->Lnz.n > 0)
       hipLaunchKernelGGL(k_to_float, dim3(grid_for(e->Lnz.n)), dim3(kBlock), 0, s, e->Lnz.n,
@@ -568,6 +599,17 @@ static Solver<float> make_solver32(xcal_eqn* e) {
       hipLaunchKernelGGL(k_to_float, dim3(grid_for(e->Lrd.n)), dim3(kBlock), 0, s, e->Lrd.n,
                          e->Lrd.p, e->Lrd32.p);
     }
+  } else if (e->precon == XCAL_PRECON_DILU) {  // the cone sweeps' packed factors
+    auto conv = [&](const DBuf<double>& src, DBuf<float>& dst) {
+      if (dst.n != src.n) dst.alloc(src.n);
+      if (src.n > 0)
+        hipLaunchKernelGGL(k_to_float, dim3(grid_for(src.n)), dim3(kBlock), 0, s, src.n, src.p,
+                           dst.p);
+    };
+    conv(e->cfa, e->cfa32);
+    conv(e->cfr, e->cfr32);
+    conv(e->cba, e->cba32);
+    conv(e->cbd, e->cbd32);
   } else if (e->precon == XCAL_PRECON_JACOBI) {
     hipLaunchKernelGGL(k_to_float, dim3(grid_for(n)), dim3(kBlock), 0, s, n, e->pS.p, e->pS32.p);
   }
@@ -585,6 +627,10 @@ static Solver<float> make_solver32(xcal_eqn* e) {
   S.Unz = e->Unz32.p;
   S.Db = e->Db32.p;
   S.Lrd = e->Lrd32.p;
+  S.cfa = e->cfa32.p;
+  S.cfr = e->cfr32.p;
+  S.cba = e->cba32.p;
+  S.cbd = e->cbd32.p;
   S.work = e->work32.p;
   S.ks = (KS*)e->kstate.p;
   S.partial = e->kpart.p;
@@ -655,6 +701,16 @@ struct Fuse {
 template <bool UPD, class T>
 static void sweep_fwd(const Solver<T>& S, const FwdK<T>& W) {
   const xcal_mesh* m = S.m;
+  if (m->cone) {  // y to W.Yb in device order
+    XCAL_CHECK(W.Yb != (UPD ? W.r : W.u) && (!UPD || (W.rout != W.r && W.rout != W.Yb)),
+               "internal: cone sweep operands alias");
+    with_cone_form(m->cone_f.wk, !m->cone_deep, [&](auto WK, auto P) {
+      hipLaunchKernelGGL((k_fwd_cone<UPD, T, decltype(WK)::value, decltype(P)::value>),
+                         dim3(m->cone_nblk), dim3(kConeBlock), 0, S.s, cone_k(m, m->cone_f), S.cfa,
+                         S.cfr, W);
+    });
+    return;
+  }
   for (const SweepStep& st : m->plan_f) {
     if (st.grid == 1 && st.pipe > 0) {
       if (UPD)
@@ -690,6 +746,15 @@ static void sweep_fwd(const Solver<T>& S, const FwdK<T>& W) {
 template <int DOT, class T>
 static void sweep_bwd(const Solver<T>& S, const BwdK<T>& W) {
   const xcal_mesh* m = S.m;
+  if (m->cone) {  // y from W.Yb in device order; partials in slots [0, cone_nblk)
+    XCAL_CHECK(W.Z != W.Yb && (DOT == 0 || W.dv != W.Z), "internal: cone sweep operands alias");
+    with_cone_form(m->cone_b.wk, !m->cone_deep, [&](auto WK, auto P) {
+      hipLaunchKernelGGL((k_bwd_cone<DOT, T, decltype(WK)::value, decltype(P)::value>),
+                         dim3(m->cone_nblk), dim3(kConeBlock), 0, S.s, cone_k(m, m->cone_b), S.cba,
+                         S.cbd, W);
+    });
+    return;
+  }
   for (const SweepStep& st : m->plan_b) {
     if (st.grid == 1 && st.pipe > 0) {
       auto pipe = [&](auto kern) {
@@ -806,6 +871,10 @@ static int cg(Solver<T>& S, int precon, const T* b, T* x0, int itmax, double ato
     finalize_launch(S, S.grid, FIN_CG_INIT);
   }
   int parity = 0;
+  // The cone sweeps read the old r of other workgroups' rows while the fused
+  // update writes the new one, so r alternates between two vectors there.
+  T* rv[2] = {r, w + 8 * n};
+  int rpar = 0;
   const bool split = n >= kSplitRows;
   Fuse<T> F(S, S.m->fuse && !dilu && !split && S.gridA <= kFuseBlocks && S.grid <= kFuseBlocks);
   auto iteration = [&]() {
@@ -835,9 +904,12 @@ static int cg(Solver<T>& S, int precon, const T* b, T* x0, int itmax, double ato
       W.p = pnew;
       W.q = q;
       W.x = x;
-      W.r = r;
+      W.r = rv[rpar];
+      if (S.m->cone) rpar ^= 1;
+      W.rout = rv[rpar];
+      const T* rnew = rv[rpar];
       timed(c, 1, [&] { sweep_fwd<true>(S, W); });
-      timed(c, 2, [&] { sweep_bwd<1>(S, bwdk(S, (const T*)Yb, zd, (const T*)r, (const KS*)S.ks)); });
+      timed(c, 2, [&] { sweep_bwd<1>(S, bwdk(S, (const T*)Yb, zd, rnew, (const KS*)S.ks)); });
       finalize_launch(S, sweep_slots, FIN_CG_ITER);
     } else {
       double* pu = F.part();

@@ -207,6 +207,9 @@ struct FwdK {
   T* x;
   T* r;
   const KS* ks;  // NULL outside a Krylov solve
+  // the cone form's UPD: the updated r goes here (r is read by other
+  // workgroups' cones during the sweep, so it is not updated in place)
+  T* rout = nullptr;
 };
 
 // Loads of one row's entries.  WK > 0: the level's entry count w satisfies

@@ -1,0 +1,257 @@
+// Cone form of the DILU sweeps on single meshes: one launch per sweep, no
+// synchronisation between workgroups.
+//
+// The cells are cut into blocks of consecutive Morton keys (a compact region
+// of space each).  The workgroup of a block sweeps the block's rows together
+// with every row they depend on, directly or through other rows: the block's
+// dependency cone.  It walks the cone in level order, one step (at most
+// kConeBlock rows of one level) between barriers, and keeps every value it
+// computes in LDS, where later rows of the cone gather them.  A cone row of
+// another block is computed again, from the same inputs and in the same
+// summation order as in its own block, so it has the same value bit for bit.
+// Each row sums its entries in the reference's column order
+// (forward_substitution!, backward_substitution!,
+// preconditioners_1_DILU.jl:84-114), so the sweeps equal the sequential loops
+// whatever the blocks.
+//
+// This pays where cones are small.  On a randomly numbered mesh a chain of
+// ascending neighbours is short (18 levels on the 1M-cell box), and a block
+// of 1024 cells has a cone of about 2000 rows; on a lexicographically
+// numbered mesh a chain crosses the domain and the mesh build keeps the level
+// launches.
+//
+// Memory latency is kept off the step-to-step chain as in the pipelined runs
+// (sweep_pipe.h): each row's position-addressed loads (its row index, its
+// entries' cone positions and values) are issued P steps ahead, and the loads
+// addressed by its row index Q steps ahead, all branch-free, so the compiler
+// waits only for the step at hand.  Entries are stored jagged-diagonal (rows
+// of a step by entry count, descending), so a step carries no padding.  A
+// row's outputs are stored by every workgroup that computes it: the values
+// are equal, and a store without a branch keeps the wait counts exact.
+#pragma once
+
+#include <type_traits>
+
+#include "sweep_pipe.h"
+
+namespace xcal {
+
+// Static cone structure of one sweep direction (device copies of ConeHost).
+struct ConeK {
+  const int32_t* blk;    // first step of each block (nblk + 1)
+  const int32_t* base;   // first position of each block (nblk + 1)
+  const int32_t* steps;  // kConeTab ints per step: r0, cnt, eoff[8], ecnt[8], 2 unused
+  const int32_t* crow;   // device row of each position (~row: another block's)
+  const uint16_t* lidx;  // cone-local position of each entry's dependency
+  const int32_t* own;    // device rows in Morton order: block b owns [b*rows, (b+1)*rows)
+  int32_t rows;          // rows per block
+  int32_t n;             // rows of the mesh
+};
+
+struct alignas(16) ConeStep {
+  int32_t r0, cnt, eoff[8], ecnt[8], unused[2];
+};
+static_assert(sizeof(ConeStep) == kConeTab * sizeof(int32_t), "step table row");
+
+template <class T>
+struct ConeLDS {
+  T v[kConeMax];
+  ConeStep tab[kConeSteps];
+};
+
+// Stages block b's step table; returns its step count and first position.
+__device__ inline int cone_stage(const ConeK& C, int b, ConeStep* tab, int32_t& base) {
+  const int s0 = C.blk[b], s1 = C.blk[b + 1];
+  const int4* src = reinterpret_cast<const int4*>(C.steps + (int64_t)s0 * kConeTab);
+  int4* dst = reinterpret_cast<int4*>(tab);
+  for (int j = threadIdx.x; j < (s1 - s0) * (kConeTab / 4); j += blockDim.x) dst[j] = src[j];
+  base = C.base[b];
+  __syncthreads();
+  return s1 - s0;
+}
+
+// Position-addressed part of a record: the row's position and, for each of
+// its entries, the dependency's cone position (-1: none) and entry index.
+template <int WK>
+__device__ inline void cone_fetch_pos(const ConeK& C, const ConeStep& S, int32_t base,
+                                      int32_t& c, int32_t& lp, bool& rep, int32_t (&li)[WK],
+                                      int64_t (&e)[WK]) {
+  const int32_t cnt = S.cnt;
+  rep = (int32_t)threadIdx.x >= cnt;
+  const int32_t q = rep ? cnt - 1 : (int32_t)threadIdx.x;
+  const int32_t pos = S.r0 + q;
+  c = C.crow[pos];
+  lp = pos - base;
+#pragma unroll
+  for (int k = 0; k < WK; ++k) {
+    const bool has = q < S.ecnt[k];
+    e[k] = has ? (int64_t)S.eoff[k] + q : 0;
+    const int l = C.lidx[e[k]];
+    li[k] = has ? l : -1;
+  }
+}
+
+template <class T, int WK>
+struct CFRec {
+  int32_t c, lp;  // crow entry (row or ~row), cone-local position
+  bool rep;       // a repeat of the step's last row
+  T ro, qo;       // gathered: UPD r and q of the row; otherwise ro = u
+  int32_t li[WK];
+  T av[WK], rv[WK];
+};
+
+// Forward sweep y = (I + L D^-1)^-1 u (fwd_row's arithmetic): y to W.Yb in
+// device order (the cone backward sweep reads it there); a and rd hold a_ij
+// and 1/D_j per cone entry.  UPD fuses the CG update: u = r - alpha q, stored
+// to W.rout (other cones read the old r of every row during the sweep), and
+// x += alpha p over the block's own rows after the sweep.
+// The caller returns before the sweep once the solve has stopped, so every
+// store here is live.
+template <bool UPD, class T, int WK, int P>
+__device__ inline void fwd_cone(const ConeK& C, const T* __restrict__ a, const T* __restrict__ rd,
+                                const FwdK<T>& W, ConeLDS<T>& L) {
+  const int b = (int)xcd_block();
+  int32_t base;
+  const int ns = cone_stage(C, b, L.tab, base);
+  const T al = UPD && W.ks ? (T)W.ks->alpha : T(0);
+  auto fetch = [&](CFRec<T, WK>& R, int s) {
+    int64_t e[WK];
+    cone_fetch_pos<WK>(C, L.tab[s < ns ? s : ns - 1], base, R.c, R.lp, R.rep, R.li, e);
+#pragma unroll
+    for (int k = 0; k < WK; ++k) {
+      R.av[k] = a[e[k]];
+      R.rv[k] = rd[e[k]];
+    }
+  };
+  auto gather = [&](CFRec<T, WK>& R) {
+    const int32_t i = R.c >= 0 ? R.c : ~R.c;
+    if (UPD) {
+      R.ro = W.r[i];
+      R.qo = W.q[i];
+    } else {
+      R.ro = W.u[i];
+    }
+  };
+  auto process = [&](const CFRec<T, WK>& R) {
+    T sum = 0;
+#pragma unroll
+    for (int k = 0; k < WK; ++k) {
+      const int li = R.li[k];
+      const T yv = L.v[li >= 0 ? li : 0];
+      const T term = (R.av[k] * yv) * R.rv[k];
+      sum = li >= 0 ? sum + term : sum;
+    }
+    const T u = UPD ? R.ro + (-al) * R.qo : R.ro;
+    const T y = u - sum;
+    L.v[R.lp] = y;
+    const int32_t i = R.c >= 0 ? R.c : ~R.c;
+    W.Yb[i] = y;
+    if (UPD) W.rout[i] = u;
+  };
+  CFRec<T, WK> buf[P];
+  pipe_levels<P>(ns, buf, fetch, gather, process);
+  if (UPD) {  // x += alpha p over the own rows, U rows per thread at a time
+    constexpr int U = 8;
+    const int32_t o0 = b * C.rows, o1 = min(C.n, o0 + C.rows);
+    for (int32_t j0 = o0 + (int32_t)threadIdx.x; j0 < o1; j0 += U * (int32_t)blockDim.x) {
+      int32_t ii[U];
+      T xv[U], pv[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int32_t j = j0 + u * (int32_t)blockDim.x;
+        ii[u] = C.own[j < o1 ? j : o0];
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        xv[u] = W.x[ii[u]];
+        pv[u] = W.p[ii[u]];
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (j0 + u * (int32_t)blockDim.x < o1) W.x[ii[u]] = xv[u] + al * pv[u];
+    }
+  }
+}
+
+template <class T, int WK>
+struct CBRec {
+  int32_t c, lp;
+  bool rep;
+  T d, yb, dvi;
+  int32_t li[WK];
+  T av[WK];
+};
+
+// Backward sweep z = (D + U)^-1 y (bwd_row's arithmetic) with up to two dot
+// products over the block's own rows (DOT as bwd_row), stored as the
+// block's partial pair.  y is read from W.Yb in device order; a holds a_ij
+// per cone entry, d D per cone position.  dv must not alias Z (another
+// workgroup may store a row's z before the row's owner reads its dv).  As
+// for the forward sweep, the caller returns once the solve has stopped.
+template <int DOT, class T, int WK, int P>
+__device__ inline void bwd_cone(const ConeK& C, const T* __restrict__ a, const T* __restrict__ d,
+                                const BwdK<T>& W, ConeLDS<T>& L) {
+  const int b = (int)xcd_block();
+  int32_t base;
+  const int ns = cone_stage(C, b, L.tab, base);
+  auto fetch = [&](CBRec<T, WK>& R, int s) {
+    const ConeStep& S = L.tab[s < ns ? s : ns - 1];
+    int64_t e[WK];
+    cone_fetch_pos<WK>(C, S, base, R.c, R.lp, R.rep, R.li, e);
+    R.d = d[base + R.lp];
+#pragma unroll
+    for (int k = 0; k < WK; ++k) R.av[k] = a[e[k]];
+  };
+  auto gather = [&](CBRec<T, WK>& R) {
+    const int32_t i = R.c >= 0 ? R.c : ~R.c;
+    R.yb = W.Yb[i];
+    R.dvi = DOT != 0 ? W.dv[i] : T(0);
+  };
+  double s0 = 0.0, s1 = 0.0;
+  auto process = [&](const CBRec<T, WK>& R) {
+    T sum = 0;
+#pragma unroll
+    for (int k = 0; k < WK; ++k) {
+      const int li = R.li[k];
+      const T zv = L.v[li >= 0 ? li : 0];
+      const T term = R.av[k] * zv;
+      sum = li >= 0 ? sum + term : sum;
+    }
+    const T zi = (R.yb - sum) / R.d;
+    L.v[R.lp] = zi;
+    const int32_t i = R.c >= 0 ? R.c : ~R.c;
+    W.Z[i] = zi;
+    const bool own = R.c >= 0 && !R.rep;
+    double a0 = 0.0, a1 = 0.0;
+    if (DOT == 1 || DOT == 2) a0 = dd(R.dvi, zi);
+    if (DOT == 3) { a0 = dd(zi, R.dvi); a1 = dd(zi, zi); }
+    if (DOT == 4) { a0 = dd(zi, zi); a1 = dd(R.dvi, zi); }
+    s0 = own ? s0 + a0 : s0;
+    s1 = own ? s1 + a1 : s1;
+  };
+  CBRec<T, WK> buf[P];
+  pipe_levels<P>(ns, buf, fetch, gather, process);
+  if (DOT != 0) store_partial<kConeBlock>(s0, s1, W.partial, b);
+}
+
+// Host: calls f(WK, P) with the cone sweeps' compile-time entry bound and
+// prefetch depth (shallow: 3 steps; deep: 6 steps for up to 4 entries, 4
+// above).  Shallow is the default: on the 1M-cell box its fewer registers
+// (139 against 175 for the CG forward sweep) let more workgroups share a CU,
+// which hid latency better than the deeper prefetch (39.3M against 38.5M
+// cell-iters/s).
+template <class F>
+inline void with_cone_form(int wk, bool shallow, F&& f) {
+  auto go = [&](auto W, auto Pdeep) {
+    if (shallow) f(W, std::integral_constant<int, 3>{});
+    else f(W, Pdeep);
+  };
+  switch (wk) {
+    case 2: go(std::integral_constant<int, 2>{}, std::integral_constant<int, 6>{}); break;
+    case 4: go(std::integral_constant<int, 4>{}, std::integral_constant<int, 6>{}); break;
+    case 6: go(std::integral_constant<int, 6>{}, std::integral_constant<int, 4>{}); break;
+    default: go(std::integral_constant<int, 8>{}, std::integral_constant<int, 4>{});
+  }
+}
+
+}  // namespace xcal

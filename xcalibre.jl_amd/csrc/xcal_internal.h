@@ -78,6 +78,16 @@ constexpr int kPipeRing = 4096;  // LDS ring of recent sweep outputs
 // (level table, ring preload, P levels of prefetch) outweighs its gain.
 constexpr int kPipeMinLevels = 16;
 
+// Cone form of the DILU sweeps on single meshes: one workgroup per block of
+// cells sweeps the block's rows and their dependency cone in LDS.
+constexpr int kConeBlock = 256;       // threads per workgroup = most rows per step
+constexpr int kConeMax = 4608;        // most cone rows of one block (LDS values)
+constexpr int kConeSteps = 64;        // most steps of one block (LDS table)
+constexpr int kConeTab = 20;          // ints per step: r0, cnt, eoff[8], ecnt[8], 2 unused
+constexpr int kConeRows = 2048;       // block size tried first
+constexpr int64_t kConeMinRows = 1 << 16;  // smaller meshes keep the level launches
+constexpr double kConeMaxRatio = 3.0;      // most cone rows in all, per mesh row
+
 // Cut c_x (x = 0..8) of a level of cnt rows into eighths at 64-row
 // boundaries; eighth x is worked by the workgroups on XCD x.
 inline int64_t eighth_cut(int64_t cnt, int x) { return x >= 8 ? cnt : cnt * x / 8 / 64 * 64; }
@@ -225,6 +235,20 @@ struct xcal_mesh {
   // case_iota = 0..ncase; face_case = case of each face (batches only)
   int64_t nblk = 0;
   xcal::DBuf<int32_t> blk_row0, blk_row1, blk_case, case_blk_d, case_iota_d, face_case;
+  // Cone form of both DILU sweeps (single meshes; sweep_cone.h): when `cone`
+  // is set, each sweep is one launch of cone_nblk workgroups, and the
+  // backward sweep's partial sums fill slots [0, cone_nblk).
+  struct Cone {
+    int wk = 0;  // entry bound
+    int64_t rows = 0, entries = 0;
+    xcal::DBuf<int32_t> blk, base, steps, crow, slot, dep;
+    xcal::DBuf<uint16_t> lidx;
+  };
+  xcal::DBuf<int32_t> cone_own;  // block b's own rows: [b * cone_rows, (b + 1) * cone_rows)
+  bool cone = false;
+  int cone_nblk = 0, cone_rows = 0;  // blocks, rows per block
+  bool cone_deep = false;            // deep prefetch (XCAL_CONE_DEEP=1 at creation)
+  Cone cone_f, cone_b;
   int small_level = 0;
   std::vector<SweepStep> plan_f, plan_b;
   int sweep_slots = 0;  // partial-sum slots used by one backward sweep
@@ -257,6 +281,9 @@ struct xcal_eqn {
   xcal::DBuf<double> Lnz, Unz, Db;  // packed factor values; D in backward-level order
   xcal::DBuf<double> Lrd;  // 1/D of each packed lower entry's column (0 on padding)
   xcal::DBuf<double> Lprod;  // a_ij a_ji of each packed lower entry (DILU diagonal update)
+  // cone form: a_ij and 1/D_j per forward cone entry, a_ij per backward cone
+  // entry, D per backward cone position
+  xcal::DBuf<double> cfa, cfr, cba, cbd;
   // Krylov workspace (allocated lazily)
   xcal::DBuf<double> work;    // solver vectors, 12 x n
   xcal::DBuf<double> kpart;   // reduction partial sums
@@ -268,6 +295,7 @@ struct xcal_eqn {
   int cg_batch = 0, bi_batch = 0;   // iterations of the last solve (first-batch size)
   // mixed precision: FP32 copies of the matrix and preconditioner, workspace
   xcal::DBuf<float> work32, nz32, pD32, prD32, pS32, Lnz32, Unz32, Db32, Lrd32;
+  xcal::DBuf<float> cfa32, cfr32, cba32, cbd32;
   // batched solves: first-batch sizes, per-case flags, host staging
   std::vector<int> batch_pred;
   xcal::DBuf<int32_t> case_active;

@@ -46,6 +46,7 @@ def lib():
                                        C.POINTER(_vp)],
             "xcal_mesh_cases": [_vp, _pi32, C.POINTER(C.c_int64)],
             "xcal_mesh_sweep_form": [_vp, _pi32, _pi32],
+            "xcal_mesh_cone_form": [_vp, _pi32, _pi32, C.POINTER(C.c_int64), C.POINTER(C.c_int64)],
             "xcal_solve_system_batch": [_vp, C.POINTER(A.SolverSetupC), _vp, C.c_int32,
                                         C.POINTER(A.SolveStats)],
             "xcal_simple_run_batch": [_vp, C.c_int32, _pd, _pi32, _pi32],
@@ -198,6 +199,14 @@ class DeviceMesh:
         f, b = C.c_int32(), C.c_int32()
         check(lib().xcal_mesh_sweep_form(self.h, C.byref(f), C.byref(b)))
         return f.value, b.value
+
+    def cone_form(self):
+        """(rows per block, blocks, forward cone rows, backward cone rows) of
+        the one-launch cone sweeps; all 0 where the sweeps launch per level."""
+        r, nb = C.c_int32(), C.c_int32()
+        f, b = C.c_int64(), C.c_int64()
+        check(lib().xcal_mesh_cone_form(self.h, C.byref(r), C.byref(nb), C.byref(f), C.byref(b)))
+        return r.value, nb.value, f.value, b.value
 
     def csr_pattern(self):
         rp = np.zeros(self.n + 1, np.int32)

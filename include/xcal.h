@@ -270,6 +270,17 @@ int xcal_precond_update(xcal_eqn* e, int32_t precon);
  * or components of cell vector fields. */
 int xcal_precond_apply(xcal_eqn* e, const xcal_field* r, int32_t rcomp, xcal_field* z,
                        int32_t zcomp);
+/* xcal_precond_apply without allocating: no counterpart in the reference.  It
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops).  The same kernels and results as
+ * xcal_precond_apply, but the DILU sweeps stage through a 16 MiB device array
+ * of the library (held by each GPU it runs on until the process ends)
+ * instead of the equation's workspace, so no hipMalloc or hipFree is called.
+ * Meshes of at most 1048576 rows; larger ones return XCAL_ERR_ARG.  Calls
+ * that overlap on one GPU share the array and race: serialise them.  The
+ * equation's preconditioner must have been set up (xcal_precond_update). */
+int xcal_precond_apply_noalloc(xcal_eqn* e, const xcal_field* r, int32_t rcomp, xcal_field* z,
+                               int32_t zcomp);
 /* y = A x with the equation's A (mul! on SparseXCSR, Multithread/spmvm.jl:85-91) */
 int xcal_spmv(xcal_eqn* e, const xcal_field* x, int32_t xcomp, xcal_field* y, int32_t ycomp);
 /* solve_system! (Solve_1_api.jl:268-307): Krylov solve warm-started from

@@ -1186,6 +1186,7 @@ int xcal_ctx_create(int32_t device, xcal_ctx** out) {
       c->nostop.alloc(32);
       XCAL_HIP(hipMemset(c->nostop.p, 0, sizeof(double) * 32));
       XCAL_HIP(hipHostMalloc((void**)&c->host_pinned, sizeof(double) * 64, hipHostMallocDefault));
+      noalloc_init(device);
     } catch (...) {
       if (c->stream) (void)hipStreamDestroy(c->stream);
       if (c->host_pinned) (void)hipHostFree(c->host_pinned);

@@ -795,13 +795,38 @@ static BwdK<T> bwdk(const Solver<T>& S, const T* Yb, T* Z, const T* dv, const KS
   return BwdK<T>{S.m->Ucol.p, S.Unz, S.m->lev_rows_b.p, S.Db, Yb, Z, dv, S.partial, ks};
 }
 
-// z = M^-1 u through the level sweeps (DILU), used by precond_apply
-static void dilu_apply(Solver<double>& S, const double* u, double* z) {
-  double *Y = S.work, *Yb = S.work + S.n;
+// z = M^-1 u through the sweeps (DILU), used by precond_apply; Y and Yb are
+// two n-vectors of staging (forward output in device and backward order).
+static void dilu_apply(const Solver<double>& S, const double* u, double* z, double* Y,
+                       double* Yb) {
   const KS* live = (const KS*)S.m->ctx->nostop.p;  // a state that never stops
   sweep_fwd<false>(S, fwdk(S, u, Y, Yb, live));
   sweep_bwd<0>(S, bwdk(S, (const double*)Yb, z, (const double*)nullptr, live));
   XCAL_HIP(hipGetLastError());
+}
+
+// z = M^-1 r with the staging vectors Y and Yb given by the caller: no
+// device memory is allocated (xcal_precond_apply_noalloc).
+void precond_apply_staged(xcal_eqn* e, const double* r, double* z, double* Y, double* Yb) {
+  if (e->precon != XCAL_PRECON_DILU) {
+    apply_M_levels(e, r, z);
+    return;
+  }
+  Solver<double> S{};  // the factors only: the sweeps use no Krylov workspace
+  S.e = e;
+  S.m = e->mesh;
+  S.s = e->mesh->ctx->stream;
+  S.n = e->mesh->n;
+  S.rD = e->prD.p;
+  S.Lnz = e->Lnz.p;
+  S.Unz = e->Unz.p;
+  S.Db = e->Db.p;
+  S.Lrd = e->Lrd.p;
+  S.cfa = e->cfa.p;
+  S.cfr = e->cfr.p;
+  S.cba = e->cba.p;
+  S.cbd = e->cbd.p;
+  dilu_apply(S, r, z, Y, Yb);
 }
 
 template <class T>
@@ -1186,7 +1211,7 @@ int xcal_precond_apply(xcal_eqn* e, const xcal_field* r, int32_t rc, xcal_field*
     XCAL_CHECK(rp != zp, "r and z must not alias");
     if (e->precon == XCAL_PRECON_DILU) {
       Solver<double> S = make_solver(e);
-      dilu_apply(S, rp, zp);
+      dilu_apply(S, rp, zp, S.work, S.work + S.n);
     } else {
       apply_M_levels(e, rp, zp);
     }

@@ -408,6 +408,10 @@ void turb_nueff(const xcal_mesh* m, double nu, const double* nutf, double* nueff
 void spmv(const xcal_eqn* e, const double* x, double* y);
 void apply_M_levels(const xcal_eqn* e, const double* r, double* z);
 void precond_update(xcal_eqn* e, int precon);
+void precond_apply_staged(xcal_eqn* e, const double* r, double* z, double* Y, double* Yb);
+// Resolves the current device's address of the no-allocation scratch array
+// (once per device; xcal_ctx_create calls it with the device current).
+void noalloc_init(int device);
 void solve_prologue(xcal_eqn* e, const xcal_solver_setup* st, double* x, const double* b);
 void solve_epilogue(xcal_eqn* e, double* x);
 int solve_system(xcal_eqn* e, const xcal_solver_setup* s, double* x, const double* b,

@@ -77,6 +77,7 @@ def lib():
             "xcal_eqn_download": [_vp, C.c_int32, C.c_int32, _pd, _pd],
             "xcal_precond_update": [_vp, C.c_int32],
             "xcal_precond_apply": [_vp, _vp, C.c_int32, _vp, C.c_int32],
+            "xcal_precond_apply_noalloc": [_vp, _vp, C.c_int32, _vp, C.c_int32],
             "xcal_spmv": [_vp, _vp, C.c_int32, _vp, C.c_int32],
             "xcal_solve_system": [_vp, C.POINTER(A.SolverSetupC), _vp, C.c_int32,
                                   C.POINTER(A.SolveStats)],
@@ -326,6 +327,11 @@ class Equation:
 
     def precond_apply(self, r: Field, z: Field, rc=0, zc=0):
         check(lib().xcal_precond_apply(self.h, r.h, rc, z.h, zc))
+
+    def precond_apply_noalloc(self, r: Field, z: Field, rc=0, zc=0):
+        """precond_apply staged in the library's fixed device array (no
+        allocation; meshes of at most 2**20 rows)."""
+        check(lib().xcal_precond_apply_noalloc(self.h, r.h, rc, z.h, zc))
 
     def spmv(self, xf: Field, y: Field, xc=0, yc=0):
         check(lib().xcal_spmv(self.h, xf.h, xc, y.h, yc))

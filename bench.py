@@ -11,13 +11,18 @@ Workload (a "step" is one SIMPLE iteration over the whole mesh):
 
 value     = cells x steps / seconds of the timed region (inputs resident in HBM)
 roofline  = algorithmic bytes of the dominant pressure-solve kernel per launch
-            / its mean launch time from HIP events on the library's stream
+            / its mean launch time from HIP events on the library's stream;
+            the dominant one of the forward DILU sweep, the backward sweep and
+            the CG SpMV (most time per CG iteration), the other two under
+            roofline.others
 cpu_baseline = the CPU oracle (restatement of the reference's FP64 path)
             timed on the host's cores over a bounded sample of the same run.
 detail.mixed_precision = the same channel with FP32 inner Krylov solves in
             FP64 iterative refinement, and its per-field error against FP64;
 detail.batched = 256 batched 2D cavities (FP64 and FP32-IR pressure), the
             form that --mode batch reports in full.  --quick skips both.
+detail.config1 = the reference's CPU configuration (40x40 cavity, 200
+            iterations): the CPU oracle (median of 3) beside the device.
 
 Other modes: --mode batch (batched cavities), --mode serve (k-omega serving
 with Poisson arrivals: p50/p99 time to converged), --mode piso (the channel as
@@ -215,6 +220,51 @@ def cavity(api, mesh, iterations, precision="fp64"):
     return model, config
 
 
+def config1(x, O, threads, runs=3):
+    """The reference's CPU configuration: the 40x40 lid-driven cavity (Re 100,
+    pref 0 in cell 1), 200 SIMPLE iterations, FP64.  The CPU oracle (the
+    restatement of the reference's FP64 path) on `threads` OpenMP threads,
+    median of `runs` runs after one warm-up, beside the device running the
+    same case the same way."""
+    api, H = x.api, x.hip
+    mesh = x.mesh.cavity2d(40)
+    iters = 200
+    O.set_threads(threads)
+    om = O.OracleMesh(mesh, x._abi)
+    times = []
+    for r in range(runs + 1):
+        model, config = cavity(api, mesh, iters)
+        setup = api.simple_setup(model, config, pref=0.0)
+        t0 = time.perf_counter()
+        ref = om.simple_run(setup, model.momentum.U, model.momentum.p, iters)
+        if r > 0:
+            times.append(time.perf_counter() - t0)
+    tc = float(np.median(times))
+    ctx = H.Context(0)
+    dm = H.DeviceMesh(ctx, mesh)
+    gtimes = []
+    for r in range(runs + 1):
+        model, config = cavity(api, mesh, iters)
+        sim = H.Simple(dm, api.simple_setup(model, config, pref=0.0))
+        sim.set_U(model.momentum.U)
+        sim.set_p(model.momentum.p)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        out = sim.run(iters)
+        ctx.synchronize()
+        if r > 0:
+            gtimes.append(time.perf_counter() - t0)
+        del sim
+    tg = float(np.median(gtimes))
+    n = mesh.n_cells
+    return {"workload": "2D lid-driven cavity, 40x40 quads (1600 cells), Re 100, 200 SIMPLE "
+                        "iterations, FP64",
+            "cpu": {"value": round(n * ref["done"] / tc, 1), "unit": "cell-iters/s",
+                    "cores": threads, "kind": "port", "median_s": round(tc, 4), "runs": runs},
+            "gpu": {"value": round(n * out["done"] / tg, 1), "unit": "cell-iters/s",
+                    "median_s": round(tg, 4), "runs": runs}}
+
+
 def batch_cases(ncase, seed=SEED_BATCH):
     """Cavity sizes m = round(32 * 8^u), u ~ U[0,1), and Re ~ U[50, 400]."""
     rng = np.random.default_rng(seed)
@@ -253,7 +303,7 @@ def batch_line(args, x, precision, steps=None, warmup=None):
     out = sim.run_batch(steps)
     ctx.synchronize()
     t1 = time.perf_counter()
-    launches, kms = ctx.kernel_time()
+    launches, kms = ctx.kernel_time(args.roofline_kernel)
     ctx.time_kernel(args.roofline_kernel, False)
     if out["done"] != steps:
         raise RuntimeError(f"batch stopped after {out['done']} of {steps} steps")
@@ -366,7 +416,7 @@ def run_piso(args, x):
     out = sim.run_piso(args.steps, 2)
     ctx.synchronize()
     t1 = time.perf_counter()
-    launches, kms = ctx.kernel_time()
+    launches, kms = ctx.kernel_time(args.roofline_kernel)
     ctx.time_kernel(args.roofline_kernel, False)
     wall = t1 - t0
     n = mesh.n_cells
@@ -434,21 +484,25 @@ def hbm_traffic(kernel, workload):
 
 
 def kernel_bytes(name, n, nnz, vb=8):
-    """Algorithmic HBM bytes of one timed unit (FP64 values, int32 indices;
-    every array element counted once, gathers included once, slice padding
-    not counted).  cg_spmv is one launch, counted as a CSR SpMV whatever the
-    storage: nnz (value + index) + (n + 1) row pointers + x once + y;
-    dilu_fwd/dilu_bwd are one whole level-synchronous sweep (one launch per
-    large level)."""
+    """Algorithmic HBM bytes of one timed unit (vb-byte values, int32
+    indices; every array element counted once, gathers included once, slice
+    padding and the cone form's recomputed rows not counted).  cg_spmv is
+    one launch, counted as a CSR SpMV whatever the storage: nnz (value +
+    index) + (n + 1) row pointers + x once + y.  dilu_fwd and dilu_bwd are
+    one DILU sweep each (one launch in the cone form), counted as a CSR
+    triangular sweep: the strict triangle's nlow = (nnz - n) / 2 entries
+    (value + index), n + 1 row pointers, n diagonal pointers, and rhs, D
+    and out (3 n values); plus the work fused into them: the forward
+    sweep's CG update (x, p, q read; x, r written: 5 n values) and the
+    backward sweep's r.z (r read: n values)."""
     nlow = (nnz - n) // 2
-    if name == "cg_spmv":   # colval+nz; n+1 row pointers; p read (gathers once); q written
+    if name == "cg_spmv":
         return (4 + vb) * nnz + 4 * (n + 1) + vb * 2 * n
-    if name == "dilu_fwd":  # row begin, diag; lower colval+nz; rD and y gathered; y written;
-        # p, q read; x, r read+written
-        return 8 * n + 12 * nlow + 8 * n * (1 + 1 + 1 + 2 + 4)
-    if name == "dilu_bwd":  # level rows, diag, row end; upper colval+nz; D, y read;
-        # z gathered + written; r read for r.z
-        return 12 * n + 12 * nlow + 8 * n * (1 + 1 + 2 + 1)
+    sweep = (vb + 4) * nlow + 4 * (n + 1) + 4 * n + 3 * vb * n
+    if name == "dilu_fwd":
+        return sweep + 5 * vb * n
+    if name == "dilu_bwd":
+        return sweep + vb * n
     raise KeyError(name)
 
 
@@ -459,7 +513,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--cells", type=int, default=100, help="cells per box edge")
     ap.add_argument("--roofline-kernel", default="cg_spmv",
-                    choices=["cg_spmv", "dilu_fwd", "dilu_bwd"])
+                    choices=["cg_spmv", "dilu_fwd", "dilu_bwd"],
+                    help="batch and piso modes: the kernel whose roofline is reported (the "
+                         "channel reports all three, the dominant one first)")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32-ir"],
                     help="pressure solve: FP64, or FP32 inner Krylov in FP64 refinement")
     ap.add_argument("--inner-rtol", type=float, default=1e-6,
@@ -527,14 +583,17 @@ def main():
     if args.warmup:
         sim.run(args.warmup)
         log(f"{args.warmup} warmup steps done")
-    ctx.time_kernel(args.roofline_kernel, True)
+    kernels = ("dilu_fwd", "dilu_bwd", "cg_spmv")
+    for k in kernels:
+        ctx.time_kernel(k, True)
     ctx.synchronize()
     t0 = time.perf_counter()
     out = sim.run(args.steps)
     ctx.synchronize()
     t1 = time.perf_counter()
-    launches, kms = ctx.kernel_time()
-    ctx.time_kernel(args.roofline_kernel, False)
+    timings = {k: ctx.kernel_time(k) for k in kernels}
+    for k in kernels:
+        ctx.time_kernel(k, False)
     if out["done"] != args.steps:
         raise RuntimeError(f"SIMPLE stopped after {out['done']} of {args.steps} steps")
     wall = t1 - t0
@@ -543,14 +602,23 @@ def main():
     value = n * args.steps / wall
     stages = sim.stage_times()
     kit = out["krylov_iters"]
-    nbytes = kernel_bytes(args.roofline_kernel, n, dm.nnz)
-    avg_s = kms / 1e3 / max(launches, 1)
-    achieved = nbytes / avg_s / 1e9
-    traffic = hbm_traffic(args.roofline_kernel, f"channel_{N}cubed_{args.precision}")
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": args.roofline_kernel, "bytes_per_launch": nbytes,
-                "timed_launches": launches, "avg_us": round(avg_s * 1e6, 2)}
+    vb = 8 if args.precision == "fp64" else 4
+
+    def roof(k):
+        launches, kms = timings[k]
+        nbytes = kernel_bytes(k, n, dm.nnz, 8 if k == "cg_spmv" else vb)
+        avg_s = kms / 1e3 / max(launches, 1)
+        achieved = nbytes / avg_s / 1e9
+        return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": hbm_traffic(k, f"channel_{N}cubed_{args.precision}"),
+                "kernel": k, "bytes_per_launch": nbytes, "timed_launches": launches,
+                "avg_us": round(avg_s * 1e6, 2)}
+    # the dominant kernel: most time per CG iteration (every timed kernel
+    # launches once per iteration)
+    dominant = max(kernels, key=lambda k: timings[k][1] / max(timings[k][0], 1))
+    roofline = roof(dominant)
+    roofline["others"] = [roof(k) for k in kernels if k != dominant]
 
     err = None
     if args.precision != "fp64":
@@ -602,10 +670,13 @@ def main():
             log(f"batch {prec}: {bl['value']:.4g} cell-iters/s")
         batched["workload"] = bl["config"]["workload"].split(", SIMPLE")[0]
 
-    cpu = None
+    cpu = c1 = None
     if not args.no_cpu and args.precision == "fp64":
         O = ge.load_oracle()
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        c1 = config1(x, O, threads)
+        log(f"config 1 (40x40 cavity): CPU {c1['cpu']['value']:.4g}, "
+            f"GPU {c1['gpu']['value']:.4g} cell-iters/s")
         O.set_threads(threads)
         om = O.OracleMesh(mesh, x._abi)
         m2, c2 = channel(api, mesh, args.cpu_steps)
@@ -638,7 +709,8 @@ def main():
                    "device_ms": {k: round(v, 2) for k, v in stages.items()},
                    "p_cg_iters_mean": float(np.mean(kit[3])),
                    "U_bicgstab_iters_mean": float(np.mean(kit[:3])),
-                   "error_vs_fp64": err, "mixed_precision": mixed, "batched": batched},
+                   "error_vs_fp64": err, "mixed_precision": mixed, "batched": batched,
+                   "config1": c1},
     }
     print(json.dumps(line))
 

@@ -468,14 +468,15 @@ int xcal_server_fetch(xcal_server* s, int64_t id, int32_t which, double* host);
 int xcal_server_release(xcal_server* s, int64_t id);
 
 /* Kernel timing with HIP events on the context's stream (no reference
- * counterpart; measurement support).  While enabled, every 8th launch of
- * the selected pressure-solve kernel, starting with the first after the
- * enable, is bracketed by an event pair:
+ * counterpart; measurement support).  While a kernel's timing is enabled,
+ * every 8th launch of it, starting with the first after the enable, is
+ * bracketed by an event pair.  Kernels (each enabled independently):
  *   0 = CG SpMV (p = z + beta p; q = A p; p.q),
- *   1 = forward DILU sweep (fused x, r update), 2 = backward DILU sweep.  xcal_kernel_timing_read returns the number of
- * timed launches and their summed milliseconds since the last enable. */
+ *   1 = forward DILU sweep (fused x, r update), 2 = backward DILU sweep.
+ * xcal_kernel_timing_read returns the kernel's timed launches and their
+ * summed milliseconds since its last enable. */
 int xcal_kernel_timing_enable(xcal_ctx* ctx, int32_t kernel, int32_t enable);
-int xcal_kernel_timing_read(xcal_ctx* ctx, int64_t* launches, double* total_ms);
+int xcal_kernel_timing_read(xcal_ctx* ctx, int32_t kernel, int64_t* launches, double* total_ms);
 
 #ifdef __cplusplus
 }

@@ -1203,7 +1203,7 @@ int xcal_ctx_destroy(xcal_ctx* c) {
     if (!c) return 0;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
-    for (auto& pr : c->prof_pending) c->prof_pool.insert(c->prof_pool.end(), {pr.first, pr.second});
+    for (auto& pr : c->prof_pending) c->prof_pool.insert(c->prof_pool.end(), {pr.a, pr.b});
     for (hipEvent_t ev : c->prof_pool) (void)hipEventDestroy(ev);
     (void)hipStreamDestroy(c->stream);
     if (c->host_pinned) (void)hipHostFree(c->host_pinned);

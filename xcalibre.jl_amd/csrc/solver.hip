@@ -1225,21 +1225,22 @@ int xcal_kernel_timing_enable(xcal_ctx* c, int32_t kernel, int32_t enable) {
     XCAL_CHECK(kernel >= 0 && kernel <= 2, "kernel id must be 0..2");
     XCAL_HIP(hipStreamSynchronize(c->stream));
     prof_resolve(c);
-    c->prof_kernel = enable ? kernel : -1;
-    c->prof_seq = 0;
-    c->prof_launches = 0;
-    c->prof_ms = 0.0;
+    c->prof_on[kernel] = enable != 0;
+    c->prof_seq[kernel] = 0;
+    c->prof_launches[kernel] = 0;
+    c->prof_ms[kernel] = 0.0;
     return 0;
   });
 }
 
-int xcal_kernel_timing_read(xcal_ctx* c, int64_t* launches, double* total_ms) {
+int xcal_kernel_timing_read(xcal_ctx* c, int32_t kernel, int64_t* launches, double* total_ms) {
   return guarded([&] {
     XCAL_CHECK(c && launches && total_ms, "NULL argument");
+    XCAL_CHECK(kernel >= 0 && kernel <= 2, "kernel id must be 0..2");
     XCAL_HIP(hipStreamSynchronize(c->stream));
     prof_resolve(c);
-    *launches = c->prof_launches;
-    *total_ms = c->prof_ms;
+    *launches = c->prof_launches[kernel];
+    *total_ms = c->prof_ms[kernel];
     return 0;
   });
 }

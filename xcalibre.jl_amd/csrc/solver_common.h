@@ -399,7 +399,8 @@ inline hipEvent_t prof_event(xcal_ctx* c) {
 constexpr int64_t kProfEvery = 8;
 template <class Fn>
 inline void timed(xcal_ctx* c, int k, Fn&& fn) {
-  if (c->prof_kernel != k || c->prof_seq++ % kProfEvery != 0) {
+  if (k < 0 || k >= xcal_ctx::kProfKernels || !c->prof_on[k] ||
+      c->prof_seq[k]++ % kProfEvery != 0) {
     fn();
     return;
   }
@@ -407,18 +408,18 @@ inline void timed(xcal_ctx* c, int k, Fn&& fn) {
   XCAL_HIP(hipEventRecord(a, c->stream));
   fn();
   XCAL_HIP(hipEventRecord(b, c->stream));
-  c->prof_pending.emplace_back(a, b);
+  c->prof_pending.push_back({k, a, b});
 }
 
 // after a stream synchronisation: fold completed event pairs into the totals
 inline void prof_resolve(xcal_ctx* c) {
   for (auto& pr : c->prof_pending) {
     float ms = 0.0f;
-    XCAL_HIP(hipEventElapsedTime(&ms, pr.first, pr.second));
-    c->prof_ms += ms;
-    c->prof_launches += 1;
-    c->prof_pool.push_back(pr.first);
-    c->prof_pool.push_back(pr.second);
+    XCAL_HIP(hipEventElapsedTime(&ms, pr.a, pr.b));
+    c->prof_ms[pr.kernel] += ms;
+    c->prof_launches[pr.kernel] += 1;
+    c->prof_pool.push_back(pr.a);
+    c->prof_pool.push_back(pr.b);
   }
   c->prof_pending.clear();
 }

@@ -140,13 +140,19 @@ struct xcal_ctx {
   xcal::DBuf<double> nostop;        // zeroed Krylov state (stop flag clear) for sweeps
                                     // that run outside a Krylov solve
   double* host_pinned = nullptr;    // 64 doubles, pinned
-  // optional HIP-event timing of one pressure-solve kernel
-  int prof_kernel = -1;
+  // optional HIP-event timing of the pressure-solve kernels (0 SpMV, 1
+  // forward sweep, 2 backward sweep), each on or off independently
+  static constexpr int kProfKernels = 3;
+  bool prof_on[kProfKernels] = {};
+  int64_t prof_seq[kProfKernels] = {};  // launches of each kernel since its enable
+  int64_t prof_launches[kProfKernels] = {};
+  double prof_ms[kProfKernels] = {};
   std::vector<hipEvent_t> prof_pool;
-  std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_pending;
-  int64_t prof_launches = 0;
-  int64_t prof_seq = 0;  // launches of the timed kernel since timing was enabled
-  double prof_ms = 0.0;
+  struct ProfPair {
+    int kernel;
+    hipEvent_t a, b;
+  };
+  std::vector<ProfPair> prof_pending;
 };
 
 // One launch of a DILU sweep over levels [lev0, lev1): one level over `grid`
@@ -345,7 +351,6 @@ xcal_mesh* upload_prepared(xcal_ctx* ctx, PreparedMesh& P);
 // residual (the reference's loop would carry on); simple_diverged reports it.
 void simple_stop_nonfinite(xcal_simple* S, bool on);
 bool simple_diverged(const xcal_simple* S);
-
 // device bytes a single-case SIMPLE run on P occupies (estimate); turb adds
 // the k-omega fields
 int64_t prepared_footprint(const PreparedMesh& P, bool turb = false);

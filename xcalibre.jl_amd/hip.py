@@ -104,7 +104,7 @@ def lib():
             "xcal_piso_run": [_vp, C.c_int32, C.c_int32, _pd, _pi32, _pi32],
             "xcal_simple_stage_times": [_vp, _pd],
             "xcal_kernel_timing_enable": [_vp, C.c_int32, C.c_int32],
-            "xcal_kernel_timing_read": [_vp, C.POINTER(C.c_int64), _pd],
+            "xcal_kernel_timing_read": [_vp, C.c_int32, C.POINTER(C.c_int64), _pd],
         }
         for name, argt in sig.items():
             fn = getattr(L, name)
@@ -138,13 +138,14 @@ class Context:
     KERNELS = {"cg_spmv": 0, "dilu_fwd": 1, "dilu_bwd": 2}
 
     def time_kernel(self, name: str, enable: bool = True):
-        """Bracket every launch of one pressure-solve kernel with HIP events."""
+        """Bracket every 8th launch of a pressure-solve kernel with HIP events
+        (kernels are enabled independently)."""
         check(lib().xcal_kernel_timing_enable(self.h, self.KERNELS[name], int(enable)))
 
-    def kernel_time(self):
-        """(launches, total milliseconds) since time_kernel()."""
+    def kernel_time(self, name: str):
+        """(timed launches, total milliseconds) of a kernel since time_kernel()."""
         n, ms = C.c_int64(), C.c_double()
-        check(lib().xcal_kernel_timing_read(self.h, C.byref(n), C.byref(ms)))
+        check(lib().xcal_kernel_timing_read(self.h, self.KERNELS[name], C.byref(n), C.byref(ms)))
         return n.value, ms.value
 
     def __del__(self):

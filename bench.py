@@ -20,7 +20,11 @@ cpu_baseline = the CPU oracle (restatement of the reference's FP64 path)
 detail.mixed_precision = the same channel with FP32 inner Krylov solves in
             FP64 iterative refinement, and its per-field error against FP64;
 detail.batched = 256 batched 2D cavities (FP64 and FP32-IR pressure), the
-            form that --mode batch reports in full.  --quick skips both.
+            form that --mode batch reports in full.
+detail.serving = 8 k-omega step cases served with Poisson arrivals and
+            continuous batching: p50/p99 time to converged over the converged
+            requests, with the counts of converged, capped and diverged ones;
+            --mode serve reports it in full.  --quick skips these three.
 detail.config1 = the reference's CPU configuration (40x40 cavity, 200
             iterations): the CPU oracle (median of 3) beside the device.
 
@@ -123,45 +127,50 @@ def serve_cases(nreq, seed=SEED_SERVE):
     return ratio, cells
 
 
-def run_serve(args, x):
-    """Serving form: Poisson arrivals at a fixed rate, continuous admission."""
+def serve_leg(args, x, requests, rate, max_iterations, alone=True):
+    """Serving form: `requests` step cases arrive as a Poisson process at
+    `rate` per second; the server admits them by HBM footprint into its
+    running batch (continuous batching: the cases in flight step together)
+    and each runs to residual 1e-6 or `max_iterations`.  Returns
+    (cell-iterations per second over the makespan, summary dict)."""
     api, H = x.api, x.hip
-    ratio, cells = serve_cases(args.requests)
+    ratio, cells = serve_cases(requests)
     t0 = time.perf_counter()
-    cases = [step_case(api, x, r, c, args.max_iterations, rans=not args.laminar, precon=args.precon)
+    cases = [step_case(api, x, r, c, max_iterations, rans=not args.laminar, precon=args.precon)
              for r, c in zip(ratio, cells)]
     t_mesh = time.perf_counter() - t0
     rng = np.random.default_rng(SEED_SERVE + 1)
-    gaps = rng.exponential(1.0 / args.rate, args.requests)
-    srv = H.Server(0, slots=args.slots, hbm_budget=int(args.hbm_gb * (1 << 30)),
-                   max_iterations=args.max_iterations)
-    ids = []
+    gaps = rng.exponential(1.0 / rate, requests)
+    budget = int(args.hbm_gb * (1 << 30))
+    srv = H.Server(0, slots=args.slots, hbm_budget=budget, max_iterations=max_iterations)
+    ids, submit_s = [], []
     t0 = time.perf_counter()
     for (mesh, model, config), gap in zip(cases, gaps):
         if ids:
             time.sleep(gap)  # exponential inter-arrival gap
+        ts = time.perf_counter()
         ids.append(submit_case(srv, api, mesh, model, config))
-        print(f"submitted {len(ids)}/{args.requests} at {time.perf_counter() - t0:.2f} s",
-              file=sys.stderr, flush=True)
+        submit_s.append(time.perf_counter() - ts)
+        log(f"submitted {len(ids)}/{requests} at {time.perf_counter() - t0:.2f} s")
     while not srv.wait(0, 30.0):
         done = sum(srv.status(i)["state"] >= x._abi.REQ_DONE for i in ids)
-        print(f"{done}/{len(ids)} finished at {time.perf_counter() - t0:.1f} s", file=sys.stderr,
-              flush=True)
+        log(f"{done}/{len(ids)} finished at {time.perf_counter() - t0:.1f} s")
     wall = time.perf_counter() - t0
     st = [srv.status(i) for i in ids]
     lat = np.array([s["t_end"] - s["t_submit"] for s in st])
+    wait = np.array([s["t_start"] - s["t_submit"] for s in st])
     its = np.array([s["iterations"] for s in st])
     ncell = np.array([s["cells"] for s in st])
     ok = np.array([s["state"] == x._abi.REQ_DONE for s in st])
-    conv = int(np.sum(ok & (its < args.max_iterations)))
+    conv = np.array([s["converged"] for s in st]) & ok
     mix = {i: srv.fetch(i) for i, good in zip(ids, ok) if good}
+    rebuilds = max(s["batch_rebuilds"] for s in st)
     srv.close()
     # each request again, alone on a one-slot server: its run time without
     # the mix, and whether its fields are the same as in the mix
     alone_s, same = [], True
-    if not args.no_alone:
-        srv1 = H.Server(0, slots=1, hbm_budget=int(args.hbm_gb * (1 << 30)),
-                        max_iterations=args.max_iterations)
+    if alone:
+        srv1 = H.Server(0, slots=1, hbm_budget=budget, max_iterations=max_iterations)
         for k, ((mesh, model, config), rid) in enumerate(zip(cases, ids)):
             j = submit_case(srv1, api, mesh, model, config)
             srv1.wait(j, 0.0)
@@ -174,28 +183,50 @@ def run_serve(args, x):
             srv1.release(j)
             log(f"alone {k + 1}/{len(ids)}: {alone_s[-1]:.2f} s")
         srv1.close()
+    pct = lambda a, q: round(float(np.percentile(a, q)), 3) if len(a) else None
+    big = int(np.argmax(ncell))
+    summary = {
+        "workload": f"{requests} 2D backward-facing steps, step ratio U[0.3,0.6], cells "
+                    f"log-uniform [4k,250k], "
+                    + ("laminar (nu 0.05" if args.laminar else "k-omega RANS (nu 1e-5")
+                    + f", U 10), {args.precon}, Poisson arrivals at {rate}/s, residual 1e-6, "
+                    f"at most {max_iterations} iterations",
+        "slots": args.slots, "hbm_budget_gb": args.hbm_gb,
+        "converged": int(conv.sum()), "capped": int(np.sum(ok & ~conv)),
+        "diverged": int(np.sum(~ok)),
+        "p50_time_to_converged_s": pct(lat[conv], 50),
+        "p99_time_to_converged_s": pct(lat[conv], 99),
+        "p50_time_to_stop_all_s": pct(lat, 50), "p99_time_to_stop_all_s": pct(lat, 99),
+        "queue_wait_p50_s": pct(wait, 50), "queue_wait_max_s": pct(wait, 100),
+        "submit_call_p50_s": pct(np.array(submit_s), 50),
+        "submit_call_largest_case_s": round(submit_s[big], 3),
+        "largest_case_cells": int(ncell[big]),
+        "makespan_s": round(wall, 2), "requests_per_s": round(requests / wall, 3),
+        "batch_rebuilds": int(rebuilds), "mesh_build_s": round(t_mesh, 2),
+        "alone_p50_s": pct(alone_s, 50) if alone_s else None,
+        "alone_p99_s": pct(alone_s, 99) if alone_s else None,
+        "identical_alone_vs_mix": same if alone_s else None,
+        "iterations": its.tolist(), "cells": ncell.tolist(),
+        "converged_flags": conv.astype(int).tolist(),
+    }
+    return float(np.sum(ncell * its)) / wall, summary
+
+
+def run_serve(args, x):
+    """--mode serve: the serving leg as the bench line."""
+    value, summary = serve_leg(args, x, args.requests, args.rate, args.max_iterations,
+                               alone=not args.no_alone)
     line = {
-        "metric": METRIC, "value": round(float(np.sum(ncell * its)) / wall, 1),
-        "unit": "cell-iters/s", "n_gpus": 1, "steps": int(np.sum(its)), "warmup": 0,
+        "metric": METRIC, "value": round(value, 1),
+        "unit": "cell-iters/s", "n_gpus": 1, "steps": int(sum(summary["iterations"])), "warmup": 0,
         "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded step ratios, sizes and arrival times)",
-        "config": {"workload": f"serving {args.requests} 2D backward-facing steps, step ratio "
-                               f"U[0.3,0.6], cells log-uniform [4k,250k], "
-                               + ("laminar (nu 0.05" if args.laminar else "k-omega RANS (nu 1e-5")
-                               + f", U 10), Poisson arrivals at {args.rate}/s, residual 1e-6",
+        "config": {"workload": "serving " + summary["workload"],
                    "slots": args.slots, "hbm_budget_gb": args.hbm_gb, "precon": args.precon,
                    "max_iterations": args.max_iterations, "parallelism": "single GPU"},
         "roofline": None, "cpu_baseline": None,
-        "detail": {"mesh_build_s": round(t_mesh, 2), "makespan_s": round(wall, 2),
-                   "latency_p50_s": round(float(np.percentile(lat, 50)), 3),
-                   "latency_p99_s": round(float(np.percentile(lat, 99)), 3),
-                   "requests_per_s": round(args.requests / wall, 3),
-                   "alone_p50_s": round(float(np.percentile(alone_s, 50)), 3) if alone_s else None,
-                   "alone_p99_s": round(float(np.percentile(alone_s, 99)), 3) if alone_s else None,
-                   "identical_alone_vs_mix": same if alone_s else None,
-                   "converged": conv, "failed": int(np.sum(~ok)), "iterations": its.tolist(), "cells": ncell.tolist(),
-                   "run_s": [round(s["t_end"] - s["t_start"], 3) for s in st]},
+        "detail": {"serving": summary},
     }
     print(json.dumps(line))
 
@@ -529,6 +560,8 @@ def main():
                          "serve: Poisson-arrival serving of step cases; piso: the channel "
                          "as a transient PISO case")
     ap.add_argument("--requests", type=int, default=16, help="serve mode: number of requests")
+    ap.add_argument("--serve-requests", type=int, default=8,
+                    help="channel mode: requests of the serving summary (detail.serving)")
     ap.add_argument("--rate", type=float, default=2.0, help="serve mode: arrivals per second")
     ap.add_argument("--slots", type=int, default=4, help="serve mode: concurrent slots")
     ap.add_argument("--hbm-gb", type=float, default=64.0, help="serve mode: HBM budget (GiB)")
@@ -634,7 +667,7 @@ def main():
             return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
         err = {f: rel(sim.get(f), sim64.get(f)) for f in ("U", "p", "mdotf")}
 
-    mixed = batched = None
+    mixed = batched = serving = None
     if args.precision == "fp64" and not args.quick:
         # the same channel with FP32 inner solves in FP64 refinement, and the
         # batched form: the other two parts of the headline metric
@@ -669,6 +702,10 @@ def main():
                              "roofline_frac": bl["roofline"]["frac"]}
             log(f"batch {prec}: {bl['value']:.4g} cell-iters/s")
         batched["workload"] = bl["config"]["workload"].split(", SIMPLE")[0]
+        _, serving = serve_leg(args, x, args.serve_requests, args.rate, args.max_iterations,
+                               alone=False)
+        log(f"serving: {serving['converged']}/{args.serve_requests} converged, p50 "
+            f"{serving['p50_time_to_converged_s']} s, p99 {serving['p99_time_to_converged_s']} s")
 
     cpu = c1 = None
     if not args.no_cpu and args.precision == "fp64":
@@ -710,7 +747,7 @@ def main():
                    "p_cg_iters_mean": float(np.mean(kit[3])),
                    "U_bicgstab_iters_mean": float(np.mean(kit[:3])),
                    "error_vs_fp64": err, "mixed_precision": mixed, "batched": batched,
-                   "config1": c1},
+                   "serving": serving, "config1": c1},
     }
     print(json.dumps(line))
 

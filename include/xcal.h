@@ -184,10 +184,15 @@ int xcal_mesh_info(const xcal_mesh* mesh, int64_t* n_cells, int64_t* nnz,
 /* Entries per row of the pipelined one-workgroup DILU sweeps (forward,
  * backward), or -1 where the sweeps step level by level.  Batch meshes: the
  * per-case sweeps, -1 if any case exceeds their limits (more than 256 rows
- * in a level, 2048 levels, 8 entries per row, or a dependency more than 4096
- * rows back).  Single meshes: the widest pipelined run of small levels.  No
- * reference counterpart (reports a device code path). */
+ * in a level, 2048 levels or 8 entries per row).  Single meshes: the widest
+ * pipelined run of small levels.  No reference counterpart (reports a device
+ * code path). */
 int xcal_mesh_sweep_form(const xcal_mesh* mesh, int32_t* fwd, int32_t* bwd);
+/* 1 where the pipelined sweeps of a batch mesh (forward, backward) read a
+ * dependency more than 4096 rows back, beyond their LDS ring, from global
+ * memory; 0 otherwise.  No reference counterpart (reports a device code
+ * path). */
+int xcal_mesh_sweep_far(const xcal_mesh* mesh, int32_t* fwd, int32_t* bwd);
 /* Cone form of the DILU sweeps (single meshes): rows per block, blocks, and
  * the cone rows of all blocks for the forward and backward sweeps; all 0
  * where the sweeps launch level by level.  Each sweep is then one launch in
@@ -428,17 +433,21 @@ int xcal_simple_set_case_nu(xcal_simple* s, const double* nu);
 int xcal_simple_stage_times(const xcal_simple* s, double* ms6);
 
 /* Serving (no single reference counterpart: it runs many independent
- * run!(model, config) cases).  Requests are built on the host at submission,
- * wait in a FIFO queue and are admitted to one of `slots` slots (each its own
- * HIP stream, so in-flight cases overlap on the device) when their device
- * footprint fits the HBM budget left by the cases in flight.  A case stays
- * resident until it converges (the run!/simple! convergence test) or reaches
- * max_iterations; its fields are then kept on the host until released.
- * A case's result does not depend on which other cases are in flight. */
+ * run!(model, config) cases).  Requests are built on the host at submission
+ * and wait in a FIFO queue.  The cases in flight form one batch mesh and are
+ * stepped together, every launch of a SIMPLE iteration covering all of them
+ * (continuous batching).  A waiting case joins when its device footprint fits
+ * the HBM budget left by the cases in flight, the batch holds fewer than
+ * `slots` cases and its settings (SIMPLE, turbulence, boundary conditions)
+ * equal the batch's.  A case leaves after the iteration in which it converges
+ * (the run!/simple! convergence test), reaches max_iterations, or shows a
+ * non-finite residual (failed); its fields are then kept on the host until
+ * released.  A case's result does not depend on which other cases are in
+ * flight. */
 typedef struct xcal_server xcal_server;
 typedef struct {
   int32_t device;
-  int32_t slots;          /* 1..16 */
+  int32_t slots;          /* most cases in flight together, 1..256 */
   int64_t hbm_budget;     /* bytes of device memory the cases may occupy */
   int32_t max_iterations; /* SIMPLE iterations per case at most */
 } xcal_server_config;
@@ -449,9 +458,12 @@ typedef struct {
   int64_t cells;
   int64_t footprint;    /* admitted device bytes (estimate) */
   double t_submit, t_start, t_end; /* seconds since server creation */
+  int32_t converged;    /* finished on the convergence test (not max_iterations) */
+  int32_t unused;
+  int64_t batch_rebuilds; /* batches the server has formed so far */
 } xcal_request_status;
 int xcal_server_create(const xcal_server_config* cfg, xcal_server** out);
-/* waits for the cases in flight; queued cases are dropped */
+/* waits for the batch in flight; queued cases are dropped */
 int xcal_server_destroy(xcal_server* s);
 /* turb: NULL or XCAL_TURB_LAMINAR for a laminar case.  Initial fields may be
  * NULL (zero start): U0 3*n_cells (x/y/z blocks), p0, k0, omega0, nut0 n_cells

@@ -197,3 +197,54 @@ def test_batch_pipelined_sweeps_equal_level_stepping(x, A, ctx, monkeypatch, pre
         sim.run_batch(15)
         runs.append((sim.get("U"), sim.get("p")))
     assert np.array_equal(runs[0][0], runs[1][0]) and np.array_equal(runs[0][1], runs[1][1])
+
+
+def test_batch_far_dependencies_equal_level_stepping(x, A, ctx, monkeypatch):
+    """A 2D step mesh of about 60k cells has DILU dependencies more than 4096
+    rows back; the pipelined batch sweeps read those from the global copy of
+    the forward sweep's output, which each sweep must write afresh.  Solves on
+    such a batch equal the level-stepping kernels bit for bit, twice in a row
+    on the same equation (the second sweep finds the first one's values in the
+    work vectors)."""
+    ms = [x.mesh.backward_facing_step_2d(0.4, 1.0, 1.0, 4.0, nx_in=113, nx_out=452,
+                                         ny_low=45, ny_up=68),
+          x.mesh.backward_facing_step_2d(0.5, 1.0, 1.0, 4.0, nx_in=20, nx_out=80,
+                                         ny_low=10, ny_up=10)]
+    dm = x.hip.DeviceMesh.batch(ctx, ms)
+    assert dm.n > 60000
+    assert dm.sweep_form()[0] > 0 and dm.sweep_far()[0]
+    monkeypatch.setenv("XCAL_NO_PIPE", "1")
+    dm_ref = x.hip.DeviceMesh.batch(ctx, ms)
+    monkeypatch.delenv("XCAL_NO_PIPE")
+    H = x.hip
+    rng = np.random.default_rng(7)
+    nf = dm.mesh.n_faces
+    gamma = 0.5 + rng.random(nf)
+    mdot = rng.standard_normal(nf) * np.concatenate([m.face_area for m in ms]) * 0.2
+    src = rng.standard_normal(dm.n)
+    for solver in ("SOLVER_CG", "SOLVER_BICGSTAB"):
+        sym = solver == "SOLVER_CG"
+        setup = A.SolverSetupC(getattr(A, solver), A.PRECON_DILU, A.PREC_FP64, 60, 1e-7,
+                               1.0, 0.0, 1e-12, 1e-4)
+        out = []
+        for d in (dm, dm_ref):
+            g = H.Field(d, A.FACE_SCALAR).upload(gamma)
+            md = H.Field(d, A.FACE_SCALAR).upload(np.zeros(nf) if sym else mdot)
+            terms = [(A.DIV_UPWIND, 1, md), (A.LAPLACIAN_LINEAR, -1, g)]
+            s = H.Field(d, A.CELL_SCALAR).upload(src)
+            phi = H.Field(d, A.CELL_SCALAR).upload(np.zeros(d.n))
+            eq = H.Equation(d, False)
+            eq.discretise(terms, [(s, 1)], phi)
+            bcs = A.bc_array([(A.BC_DIRICHLET, i, 1.0) for i in range(len(ms[0].patch_names))])
+            eq.apply_bcs(terms, bcs, phi)
+            eq.precond_update(A.PRECON_DILU)
+            runs = []
+            for _ in range(2):
+                phi.upload(np.zeros(d.n))
+                its, _, res, rc = eq.solve_batch(setup, phi)
+                assert rc in (0, 1)  # 1: a case stopped at 60 iterations
+                runs.append((phi.download(), list(its), list(res)))
+            assert np.array_equal(runs[0][0], runs[1][0]), solver
+            out.append(runs[0])
+        assert np.array_equal(out[0][0], out[1][0]), solver
+        assert out[0][1] == out[1][1] and out[0][2] == out[1][2], solver

@@ -91,7 +91,8 @@ class ServerConfigC(C.Structure):
 class RequestStatusC(C.Structure):
     _fields_ = [("state", C.c_int32), ("iterations", C.c_int32), ("cells", C.c_int64),
                 ("footprint", C.c_int64), ("t_submit", C.c_double), ("t_start", C.c_double),
-                ("t_end", C.c_double)]
+                ("t_end", C.c_double), ("converged", C.c_int32), ("unused", C.c_int32),
+                ("batch_rebuilds", C.c_int64)]
 
 
 REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED = 0, 1, 2, 3

@@ -296,7 +296,9 @@ __global__ void __launch_bounds__(kBlock)
   r[i] = ri;
 }
 
-// UPD: x and r were updated by kb_cg_xr; the sweep's input is u = r
+// UPD: x and r were updated by kb_cg_xr; the sweep's input is u = r.  With
+// FAR, y is also stored to W.Y (device order): a row gathers a dependency
+// that lies before the LDS ring from there.
 template <bool UPD, bool FAR, class T, int WK>
 __global__ void __launch_bounds__(kPipeBlock)
     kb_fwd_pipe(FwdK<T> W, const T* __restrict__ Lrd, const int32_t* __restrict__ lev_ptr,
@@ -306,8 +308,8 @@ __global__ void __launch_bounds__(kPipeBlock)
   const int k = blockIdx.x;
   if (ks && stopped(ks + k)) return;  // uniform over the workgroup
   const int u0 = case_lev[k];
-  fwd_pipe<false, FAR, T, WK, pipe_depth<WK>()>(W, UPD ? W.r : W.u, Lrd, lev_ptr, meta, u0,
-                                           case_lev[k + 1] - u0, false, L);
+  fwd_pipe<FAR, FAR, T, WK, pipe_depth<WK>()>(W, UPD ? W.r : W.u, Lrd, lev_ptr, meta, u0,
+                                         case_lev[k + 1] - u0, false, L);
 }
 
 template <int DOT, bool FAR, class T, int WK>

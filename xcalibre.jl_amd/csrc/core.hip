@@ -1022,6 +1022,23 @@ std::shared_ptr<PreparedMesh> prepare_mesh(const xcal_mesh_host* const* hosts, i
   return P;
 }
 
+// The batch of several prepared single cases (the serving engine's batch of
+// cases in flight); the parts are copied, not consumed.
+std::shared_ptr<PreparedMesh> prepare_union(const std::vector<const PreparedMesh*>& parts) {
+  XCAL_CHECK(!parts.empty(), "empty batch");
+  std::vector<HostMesh> C;
+  C.reserve(parts.size());
+  for (const PreparedMesh* q : parts) {
+    XCAL_CHECK(q && q->ncase == 1, "batch parts must be single cases");
+    C.push_back(q->U);
+  }
+  auto P = std::make_shared<PreparedMesh>();
+  union_cases(C, P->U, P->cc, P->cb, P->ci, P->lf, P->lb);
+  P->ncase = (int)parts.size();
+  P->batch = true;
+  return P;
+}
+
 int64_t prepared_cells(const PreparedMesh& P) { return P.U.n; }
 
 // Device bytes of a prepared mesh plus a steady SIMPLE run on it (fields,
@@ -1044,6 +1061,11 @@ int64_t prepared_footprint(const PreparedMesh& P, bool turb) {
 }
 
 xcal_mesh* upload_prepared(xcal_ctx* ctx, PreparedMesh& P) {
+  if (P.batch) {  // batches run the batched solvers, which have no cone form
+    P.U.cone = false;
+    P.U.cone_f = P.U.cone_b = ConeHost();
+    P.U.cone_own.clear();
+  }
   xcal_mesh* m = upload_mesh(ctx, P.U);
   std::unique_ptr<xcal_mesh> guard(m);
   const int ncase = P.ncase;
@@ -1278,6 +1300,15 @@ int xcal_mesh_cone_form(const xcal_mesh* m, int32_t* block_rows, int32_t* blocks
     if (blocks) *blocks = m->cone ? m->cone_nblk : 0;
     if (fwd_rows) *fwd_rows = m->cone ? m->cone_f.rows : 0;
     if (bwd_rows) *bwd_rows = m->cone ? m->cone_b.rows : 0;
+    return 0;
+  });
+}
+
+int xcal_mesh_sweep_far(const xcal_mesh* m, int32_t* fwd, int32_t* bwd) {
+  return guarded([&] {
+    XCAL_CHECK(m && fwd && bwd, "NULL argument");
+    *fwd = m->pipe_wf > 0 && m->pipe_far_f ? 1 : 0;
+    *bwd = m->pipe_wb > 0 && m->pipe_far_b ? 1 : 0;
     return 0;
   });
 }

@@ -1,17 +1,23 @@
 // Serving engine: many independent steady SIMPLE cases on one device with
-// continuous admission.  A case (request) is built on the host when it is
-// submitted, waits in a FIFO queue, and is admitted to a free slot as soon as
-// its device footprint fits the HBM budget left by the cases in flight.  Its
-// mesh and fields stay resident on the device until it has converged (or hit
-// its iteration limit); then its fields are copied back and its memory is
-// released, which may admit the next waiting case.
-//
-// Each slot is a host thread with its own context (HIP stream), so the work
-// of the cases in flight overlaps on the device.  A case's arithmetic does not
-// depend on which other cases are in flight (separate buffers, deterministic
-// kernels), so its result is the same alone and in the mix.
+// continuous batching.  A case (request) is built on the host when it is
+// submitted and waits in a FIFO queue.  One scheduler thread keeps the cases
+// in flight as one batch mesh (the union of their meshes, each case in its
+// own numbering) and steps them together: every kernel launch of a SIMPLE
+// iteration covers all of them, and each case's Krylov solves stop on their
+// own.  After every iteration each case is tested on its own: a converged
+// case (the run!/simple! test), one with a non-finite residual, or one at its
+// iteration limit leaves the batch with its fields copied back.  Waiting
+// cases join as soon as their device footprint fits the HBM budget left by
+// the cases in flight.  When the set changes the batch is rebuilt: the
+// continuing cases' fields move on the device, bit for bit, into the new
+// batch, and only the joining cases take the initial calculations.  A case's
+// arithmetic does not depend on which other cases share the batch (separate
+// rows, per-case reductions, deterministic kernels), so its result is the same
+// alone and in the mix.
 #include <chrono>
 #include <condition_variable>
+#include <cstdlib>
+#include <cstring>
 #include <deque>
 #include <map>
 #include <memory>
@@ -35,10 +41,27 @@ struct Request {
   // initial fields indexed by XCAL_SIMPLE_* (empty: zero start), then results
   std::vector<double> f0[6], f[6];
   int64_t footprint = 0, cells = 0;
-  int32_t state = XCAL_REQ_QUEUED, iterations = 0;
+  int32_t state = XCAL_REQ_QUEUED, iterations = 0, converged = 0;
   double t_submit = 0, t_start = 0, t_end = 0;
   std::string error;
 };
+
+// Two requests can share a batch: the same SIMPLE and turbulence settings
+// and boundary conditions (compared bytewise, pointers aside).
+bool compatible(const Request& a, const Request& b) {
+  xcal_simple_setup sa = a.setup, sb = b.setup;
+  sa.bc_U = sa.bc_p = sb.bc_U = sb.bc_p = nullptr;
+  if (std::memcmp(&sa, &sb, sizeof sa) != 0 || a.turb != b.turb) return false;
+  auto same = [](const std::vector<xcal_bc>& x, const std::vector<xcal_bc>& y) {
+    return x.size() == y.size() && std::memcmp(x.data(), y.data(), sizeof(xcal_bc) * x.size()) == 0;
+  };
+  if (!same(a.bcU, b.bcU) || !same(a.bcp, b.bcp)) return false;
+  if (!a.turb) return true;
+  xcal_turbulence_setup ta = a.tset, tb = b.tset;
+  ta.bc_k = ta.bc_omega = ta.bc_nut = tb.bc_k = tb.bc_omega = tb.bc_nut = nullptr;
+  return std::memcmp(&ta, &tb, sizeof ta) == 0 && same(a.bcK, b.bcK) && same(a.bcW, b.bcW) &&
+         same(a.bcN, b.bcN);
+}
 
 }  // namespace
 
@@ -48,9 +71,12 @@ struct xcal_server {
   std::condition_variable cv;  // queue / budget / completion changes
   std::deque<int64_t> queue;
   std::map<int64_t, std::shared_ptr<Request>> reqs;
-  int64_t next_id = 1, in_use = 0, running = 0;
+  int64_t next_id = 1, in_use = 0, running = 0, rebuilds = 0;
   bool stop = false;
-  std::vector<std::thread> workers;
+  // XCAL_SERVE_REBUILD=1 at creation: rebuild the batch after every iteration
+  // (moving every case's state each time; a test of the state transfer)
+  bool rebuild_always = false;
+  std::thread scheduler;
   std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
   double now() const {
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -59,79 +85,194 @@ struct xcal_server {
 
 namespace {
 
-void run_request(xcal_ctx* ctx, const xcal_server_config& cfg, Request& r) {
-  xcal_mesh* m = upload_prepared(ctx, *r.mesh);
-  std::unique_ptr<xcal_mesh, int (*)(xcal_mesh*)> mg(m, xcal_mesh_destroy);
-  xcal_simple_setup s = r.setup;
-  s.bc_U = r.bcU.data();
-  s.bc_p = r.bcp.data();
-  xcal_simple* sim = nullptr;
-  int rc = xcal_simple_create(m, &s, &sim);
+using Sim = std::unique_ptr<xcal_simple, int (*)(xcal_simple*)>;
+using Mesh = std::unique_ptr<xcal_mesh, int (*)(xcal_mesh*)>;
+
+void check_rc(int rc) {
   if (rc) throw Error(rc, xcal_last_error());
-  std::unique_ptr<xcal_simple, int (*)(xcal_simple*)> sg(sim, xcal_simple_destroy);
-  simple_stop_nonfinite(sim, true);
-  if (r.turb) {
-    xcal_turbulence_setup t = r.tset;
-    t.bc_k = r.bcK.data();
-    t.bc_omega = r.bcW.data();
-    t.bc_nut = r.bcN.data();
-    if ((rc = xcal_simple_set_turbulence(sim, &t))) throw Error(rc, xcal_last_error());
-  }
-  for (int w = 0; w < 6; ++w)
-    if (!r.f0[w].empty() && (rc = xcal_simple_set_field(sim, w, r.f0[w].data())))
-      throw Error(rc, xcal_last_error());
-  std::vector<double> res(4 * (size_t)cfg.max_iterations);
-  int32_t done = 0;
-  if ((rc = xcal_simple_run(sim, cfg.max_iterations, res.data(), &done, nullptr)))
-    throw Error(rc, xcal_last_error());
-  r.iterations = done;
-  if (simple_diverged(sim)) throw Error(XCAL_ERR_STATE, "diverged: non-finite residual");
-  for (int w : {XCAL_SIMPLE_U, XCAL_SIMPLE_P, XCAL_SIMPLE_K, XCAL_SIMPLE_OMEGA, XCAL_SIMPLE_NUT}) {
-    if (w >= XCAL_SIMPLE_K && !r.turb) break;
-    r.f[w].resize((w == XCAL_SIMPLE_U ? 3 : 1) * (size_t)r.cells);
-    if ((rc = xcal_simple_get_field(sim, w, r.f[w].data()))) throw Error(rc, xcal_last_error());
-  }
 }
 
-void worker(xcal_server* S) {
+// The batch of cases in flight: requests in batch order, their device mesh
+// and SIMPLE state.
+struct Batch {
+  std::vector<std::shared_ptr<Request>> cases;
+  Mesh mesh{nullptr, xcal_mesh_destroy};
+  Sim sim{nullptr, xcal_simple_destroy};
+};
+
+// A new batch of `keep` (cases of `old`, continuing) followed by `join`.
+void rebuild(xcal_ctx* ctx, Batch& old, const std::vector<std::shared_ptr<Request>>& keep,
+             const std::vector<std::shared_ptr<Request>>& join) {
+  Batch nb;
+  nb.cases = keep;
+  nb.cases.insert(nb.cases.end(), join.begin(), join.end());
+  if (nb.cases.empty()) {
+    old = Batch();
+    return;
+  }
+  std::vector<const PreparedMesh*> parts;
+  for (const auto& r : nb.cases) parts.push_back(r->mesh.get());
+  auto P = prepare_union(parts);
+  nb.mesh.reset(upload_prepared(ctx, *P));
+  xcal_mesh* m = nb.mesh.get();
+  Request& r0 = *nb.cases[0];
+  xcal_simple_setup s = r0.setup;
+  s.bc_U = r0.bcU.data();
+  s.bc_p = r0.bcp.data();
+  xcal_simple* sim = nullptr;
+  check_rc(xcal_simple_create(m, &s, &sim));
+  nb.sim.reset(sim);
+  if (r0.turb) {
+    xcal_turbulence_setup t = r0.tset;
+    t.bc_k = r0.bcK.data();
+    t.bc_omega = r0.bcW.data();
+    t.bc_nut = r0.bcN.data();
+    check_rc(xcal_simple_set_turbulence(sim, &t));
+  }
+  // initial fields in the batch's reference numbering (cases concatenated,
+  // vector components in blocks over the whole batch): the joining cases'
+  // own, zero for the continuing ones (overwritten below)
+  const int64_t n = m->n;
+  for (int w : {XCAL_SIMPLE_U, XCAL_SIMPLE_P, XCAL_SIMPLE_K, XCAL_SIMPLE_OMEGA, XCAL_SIMPLE_NUT}) {
+    if (w >= XCAL_SIMPLE_K && !r0.turb) break;
+    const int nc = w == XCAL_SIMPLE_U ? 3 : 1;
+    std::vector<double> h((size_t)nc * n, 0.0);
+    bool any = false;
+    for (size_t k = keep.size(); k < nb.cases.size(); ++k) {
+      const Request& r = *nb.cases[k];
+      if (r.f0[w].empty()) continue;
+      any = true;
+      for (int c = 0; c < nc; ++c)
+        std::copy(r.f0[w].begin() + c * r.cells, r.f0[w].begin() + (c + 1) * r.cells,
+                  h.begin() + c * n + m->case_cell[k]);
+    }
+    if (any) check_rc(xcal_simple_set_field(sim, w, h.data()));
+  }
+  simple_initialise(sim);
+  for (size_t k = 0; k < keep.size(); ++k) {
+    int ko = -1;
+    for (size_t j = 0; j < old.cases.size(); ++j)
+      if (old.cases[j] == keep[k]) ko = (int)j;
+    XCAL_CHECK(ko >= 0, "internal: continuing case not in the old batch");
+    simple_copy_case(sim, (int)k, old.sim.get(), ko);
+  }
+  XCAL_HIP(hipStreamSynchronize(ctx->stream));
+  old.sim.reset();
+  old.mesh.reset();
+  old = std::move(nb);
+}
+
+void scheduler(xcal_server* S) {
   xcal_ctx* ctx = nullptr;
-  if (xcal_ctx_create(S->cfg.device, &ctx) != 0) ctx = nullptr;
+  const bool ok = xcal_ctx_create(S->cfg.device, &ctx) == 0;
+  Batch B;
+  std::vector<double> res;
+  bool dirty = false;  // cases left the batch since it was built
   for (;;) {
-    std::shared_ptr<Request> r;
+    std::vector<std::shared_ptr<Request>> join;
     {
       std::unique_lock<std::mutex> lk(S->mu);
-      S->cv.wait(lk, [&] {
-        if (S->stop) return true;
-        if (S->queue.empty()) return false;
-        const auto& q = S->reqs[S->queue.front()];
-        return q->footprint <= S->cfg.hbm_budget - S->in_use;
-      });
-      if (S->stop) break;
-      r = S->reqs[S->queue.front()];
-      S->queue.pop_front();
-      S->in_use += r->footprint;
-      S->running += 1;
-      r->state = XCAL_REQ_RUNNING;
-      r->t_start = S->now();
+      auto admissible = [&] {
+        if (S->stop || S->queue.empty()) return false;
+        const Request& q = *S->reqs[S->queue.front()];
+        if (q.footprint > S->cfg.hbm_budget - S->in_use) return false;
+        int64_t live = (int64_t)join.size();
+        for (const auto& r : B.cases) live += r->state == XCAL_REQ_RUNNING;
+        if (live >= S->cfg.slots) return false;
+        const Request* first = !B.cases.empty() ? B.cases[0].get()
+                               : !join.empty()  ? join[0].get()
+                                                : nullptr;
+        return !first || compatible(*first, q) || !ok;
+      };
+      if (B.cases.empty() && !dirty)
+        S->cv.wait(lk, [&] { return S->stop || admissible(); });
+      if (S->stop && B.cases.empty()) break;
+      while (admissible()) {
+        auto r = S->reqs[S->queue.front()];
+        S->queue.pop_front();
+        S->in_use += r->footprint;
+        S->running += 1;
+        r->state = XCAL_REQ_RUNNING;
+        r->t_start = S->now();
+        join.push_back(r);
+      }
     }
+    std::vector<std::shared_ptr<Request>> failed;
     std::string err;
-    try {
-      if (!ctx) throw Error(XCAL_ERR_HIP, "slot could not create a device context");
-      run_request(ctx, S->cfg, *r);
-    } catch (const std::exception& e) {
-      err = e.what();
+    if (!join.empty() || dirty) {
+      try {
+        if (!ok) throw Error(XCAL_ERR_HIP, "server could not create a device context");
+        std::vector<std::shared_ptr<Request>> keep;
+        for (const auto& r : B.cases)
+          if (r->state == XCAL_REQ_RUNNING) keep.push_back(r);
+        rebuild(ctx, B, keep, join);
+        ++S->rebuilds;
+      } catch (const std::exception& e) {  // every case of the failed batch fails
+        err = e.what();
+        for (const auto& r : B.cases)
+          if (r->state == XCAL_REQ_RUNNING) failed.push_back(r);
+        failed.insert(failed.end(), join.begin(), join.end());
+        B = Batch();
+      }
+      dirty = false;
+    }
+    std::vector<std::shared_ptr<Request>> done;
+    if (err.empty() && !B.cases.empty()) {
+      try {
+        simple_step(B.sim.get(), res);
+        {
+          std::lock_guard<std::mutex> lk(S->mu);  // status readers see whole iterations
+          for (const auto& r : B.cases) r->iterations += 1;
+        }
+        for (size_t k = 0; k < B.cases.size(); ++k) {
+          Request& r = *B.cases[k];
+          if (simple_case_bad(B.sim.get(), (int)k)) {
+            r.error = "diverged: non-finite residual";
+            done.push_back(B.cases[k]);
+            continue;
+          }
+          const bool conv = simple_case_converged(B.sim.get(), (int)k);
+          if (!conv && r.iterations < S->cfg.max_iterations) continue;
+          {
+            std::lock_guard<std::mutex> lk(S->mu);
+            r.converged = conv ? 1 : 0;
+          }
+          for (int w : {XCAL_SIMPLE_U, XCAL_SIMPLE_P, XCAL_SIMPLE_K, XCAL_SIMPLE_OMEGA,
+                        XCAL_SIMPLE_NUT}) {
+            if (w >= XCAL_SIMPLE_K && !r.turb) break;
+            r.f[w].resize((w == XCAL_SIMPLE_U ? 3 : 1) * (size_t)r.cells);
+            simple_case_field(B.sim.get(), w, (int)k, r.f[w].data());
+          }
+          done.push_back(B.cases[k]);
+        }
+      } catch (const std::exception& e) {
+        err = e.what();
+        for (const auto& r : B.cases) failed.push_back(r);
+        done.clear();
+        B = Batch();
+      }
     }
     {
       std::lock_guard<std::mutex> lk(S->mu);
-      S->in_use -= r->footprint;
-      S->running -= 1;
-      r->mesh.reset();
-      r->error = err;
-      r->state = err.empty() ? XCAL_REQ_DONE : XCAL_REQ_FAILED;
-      r->t_end = S->now();
+      const double t = S->now();
+      for (const auto& r : failed) r->error = err;
+      for (auto* list : {&done, &failed})
+        for (const auto& r : *list) {
+          if (r->state != XCAL_REQ_RUNNING) continue;
+          S->in_use -= r->footprint;
+          S->running -= 1;
+          r->mesh.reset();
+          r->state = r->error.empty() ? XCAL_REQ_DONE : XCAL_REQ_FAILED;
+          r->t_end = t;
+        }
     }
-    S->cv.notify_all();
+    if (!done.empty() || !failed.empty()) {
+      dirty = !B.cases.empty();
+      S->cv.notify_all();
+    }
+    if (S->rebuild_always && !B.cases.empty()) dirty = true;
   }
+  B = Batch();
   if (ctx) xcal_ctx_destroy(ctx);
 }
 
@@ -142,7 +283,7 @@ extern "C" {
 int xcal_server_create(const xcal_server_config* cfg, xcal_server** out) {
   return guarded([&] {
     XCAL_CHECK(cfg && out, "NULL argument");
-    XCAL_CHECK(cfg->slots >= 1 && cfg->slots <= 16, "slots must be 1..16");
+    XCAL_CHECK(cfg->slots >= 1 && cfg->slots <= 256, "slots must be 1..256");
     XCAL_CHECK(cfg->hbm_budget > 0, "HBM budget must be positive");
     XCAL_CHECK(cfg->max_iterations >= 1, "max_iterations must be >= 1");
     int ndev = 0;
@@ -150,7 +291,9 @@ int xcal_server_create(const xcal_server_config* cfg, xcal_server** out) {
     XCAL_CHECK(cfg->device >= 0 && cfg->device < ndev, "device index out of range");
     auto S = std::make_unique<xcal_server>();
     S->cfg = *cfg;
-    for (int i = 0; i < cfg->slots; ++i) S->workers.emplace_back(worker, S.get());
+    const char* rb = std::getenv("XCAL_SERVE_REBUILD");
+    S->rebuild_always = rb && *rb == '1';
+    S->scheduler = std::thread(scheduler, S.get());
     *out = S.release();
     return 0;
   });
@@ -164,7 +307,7 @@ int xcal_server_destroy(xcal_server* S) {
       S->stop = true;
     }
     S->cv.notify_all();
-    for (auto& t : S->workers) t.join();  // cases in flight run to completion first
+    S->scheduler.join();  // the batch in flight runs to completion first
     delete S;
     return 0;
   });
@@ -227,6 +370,9 @@ int xcal_server_poll(xcal_server* S, int64_t id, xcal_request_status* st) {
     st->t_submit = r.t_submit;
     st->t_start = r.t_start;
     st->t_end = r.t_end;
+    st->converged = r.converged;
+    st->unused = 0;
+    st->batch_rebuilds = S->rebuilds;
     return 0;
   });
 }

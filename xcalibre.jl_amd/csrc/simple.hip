@@ -41,6 +41,11 @@ struct xcal_simple {
              *Pw = nullptr, *Dk = nullptr, *Dw = nullptr, *mk = nullptr, *mw = nullptr,
              *tmp = nullptr;
   double res_k = 0.0, res_w = 0.0;
+  // per case of the last iteration: k and omega residuals, the convergence
+  // test of Solvers_1_SIMPLE.jl:226-233 (turbulence included), and any
+  // non-finite residual
+  std::vector<double> case_res_k, case_res_w;
+  std::vector<char> case_conv, case_bad;
   // batch meshes: viscosity per case, reference cell (device id) per case
   std::vector<double> case_nu;
   xcal::DBuf<double> case_nu_d;
@@ -253,6 +258,12 @@ namespace {
 // turbulence!(KOmegaModel) (RANS_kOmega.jl:175-235), after the velocity
 // correction; gradU and Uf are current.  Returns whether k and omega met
 // their convergence criteria.
+// k and omega of case k met their criteria at the last turbulence step
+bool case_turb_conv(const xcal_simple* S, int k) {
+  const xcal_turbulence_setup& T = S->tset;
+  return S->case_res_k[k] < T.k.convergence && S->case_res_w[k] < T.omega.convergence;
+}
+
 bool komega_step(xcal_simple* S) {
   xcal_mesh* m = S->mesh;
   const xcal_turbulence_setup& T = S->tset;
@@ -279,20 +290,39 @@ bool komega_step(xcal_simple* S) {
     relax_diagdom(e, phi->comp(0), e->bc(0), ss.relax);
     if (constrain) turb_omega_constrain(e, bc, nu, S->k->comp(0));
     precond_update(e, ss.precon);
-    xcal_solve_stats st{};
-    solve_system(e, &ss, phi->comp(0), e->bc(0), &st);
+    std::vector<xcal_solve_stats> st(m->ncase);
+    if (m->batch) solve_system_batch(e, &ss, phi->comp(0), e->bc(0), st.data());
+    else solve_system(e, &ss, phi->comp(0), e->bc(0), st.data());
     turb_bound(m, phi->comp(0), S->tmp->comp(0));
-    return st.residual;
+    std::vector<double> r(m->ncase);
+    for (int k = 0; k < m->ncase; ++k) r[k] = st[k].residual;
+    return r;
   };
-  S->res_w = transport(S->omega, S->Dw, S->mw, S->Pw, S->bW, T.omega, true);
-  S->res_k = transport(S->k, S->Dk, S->mk, S->Pk, S->bK, T.k, false);
+  S->case_res_w = transport(S->omega, S->Dw, S->mw, S->Pw, S->bW, T.omega, true);
+  S->case_res_k = transport(S->k, S->Dk, S->mk, S->Pk, S->bK, T.k, false);
+  S->res_w = *std::max_element(S->case_res_w.begin(), S->case_res_w.end());
+  S->res_k = *std::max_element(S->case_res_k.begin(), S->case_res_k.end());
   turb_nut(m, S->k->comp(0), S->omega->comp(0), S->nut->comp(0));
   interpolate(m, S->nut->comp(0), S->nutf->comp(0));
   const double* ph[1] = {S->nut->comp(0)};
   double* fc[1] = {S->nutf->comp(0)};
   correct_boundaries(m, S->bN, ph, fc, 1);
   turb_nut_wall(m, S->bN, nu, S->k->comp(0), S->nutf->comp(0));
-  return S->res_k < T.k.convergence && S->res_w < T.omega.convergence;
+  bool all = true;
+  for (int k = 0; k < m->ncase; ++k) all = all && case_turb_conv(S, k);
+  return all;
+}
+
+// initial calculations (Solvers_1_SIMPLE.jl:150-156), once after the fields
+// were set
+void initialise(xcal_simple* S) {
+  if (S->initialised) return;
+  xcal_mesh* m = S->mesh;
+  interp_vec(S, S->U);
+  flux(m, S->Uf->comp(0), S->Uf->comp(1), S->Uf->comp(2), S->mdotf->comp(0));
+  grad_p(S);
+  update_nueff(S);  // update_nueff! (laminar)
+  S->initialised = true;
 }
 
 // The SIMPLE loop (inner == 0), or PISO's (inner >= 1 pressure correctors
@@ -340,14 +370,7 @@ int run_impl(xcal_simple* S, int32_t iterations, std::vector<double>& R, std::ve
     const int ncorr = inner > 0 ? inner : 1;
     std::memset(S->stage_ms, 0, sizeof(S->stage_ms));
 
-    // initial calculations (Solvers_1_SIMPLE.jl:150-156)
-    if (!S->initialised) {
-      interp_vec(S, S->U);
-      flux(m, S->Uf->comp(0), S->Uf->comp(1), S->Uf->comp(2), S->mdotf->comp(0));
-      grad_p(S);
-      update_nueff(S);  // update_nueff! (laminar)
-      S->initialised = true;
-    }
+    initialise(S);
     float acc[6] = {0, 0, 0, 0, 0, 0};
     auto mark = [&](int k) { XCAL_HIP(hipEventRecord(S->ev[k], st)); };
     int it = 0;
@@ -489,19 +512,28 @@ int run_impl(xcal_simple* S, int32_t iterations, std::vector<double>& R, std::ve
         acc[3] += c;
       }
       bool conv = turb_conv;
+      S->case_conv.assign(K, 0);
+      S->case_bad.assign(K, 0);
+      for (int k = 0; k < K; ++k) {
+        const double* r = &R[(size_t)k * 4 * iterations];
+        bool bad = false;
+        for (int f = 0; f < 4; ++f) bad = bad || !std::isfinite(r[(size_t)f * iterations + it]);
+        if (S->turb)
+          bad = bad || !std::isfinite(S->case_res_k[k]) || !std::isfinite(S->case_res_w[k]);
+        const bool zc = is3d ? r[2 * iterations + it] <= cfg.U.convergence : true;
+        const bool ck = r[it] <= cfg.U.convergence && r[iterations + it] <= cfg.U.convergence &&
+                        zc && r[3 * iterations + it] <= cfg.p.convergence &&
+                        (!S->turb || case_turb_conv(S, k));
+        S->case_conv[k] = ck;
+        S->case_bad[k] = bad;
+        conv = conv && ck;
+      }
       if (S->stop_nonfinite) {
-        for (int k = 0; k < K * 4; ++k)
-          S->diverged = S->diverged || !std::isfinite(R[(size_t)k * iterations + it]);
+        for (int k = 0; k < K; ++k) S->diverged = S->diverged || S->case_bad[k];
         if (S->diverged) {
           ++it;
           break;
         }
-      }
-      for (int k = 0; k < K; ++k) {
-        const double* r = &R[(size_t)k * 4 * iterations];
-        const bool zc = is3d ? r[2 * iterations + it] <= cfg.U.convergence : true;
-        conv = conv && r[it] <= cfg.U.convergence && r[iterations + it] <= cfg.U.convergence &&
-               zc && r[3 * iterations + it] <= cfg.p.convergence;
       }
       if (inner > 0) conv = false;  // a transient run takes every time step
       if (conv) {
@@ -539,6 +571,89 @@ void report(const xcal_simple* S, int32_t iterations, const std::vector<double>&
 }
 
 }  // namespace
+
+namespace xcal {
+
+// One SIMPLE iteration of every case of S (the serving engine steps its
+// batch this way).  res: 6 per case (Ux, Uy, Uz, p, k, omega residuals).
+void simple_step(xcal_simple* S, std::vector<double>& res) {
+  std::vector<double> R;
+  std::vector<int32_t> KIT;
+  run_impl(S, 1, R, KIT, 0);
+  const int K = S->mesh->ncase;
+  res.assign((size_t)6 * K, 0.0);
+  for (int k = 0; k < K; ++k) {
+    for (int f = 0; f < 4; ++f) res[6 * k + f] = R[(size_t)k * 4 + f];
+    if (S->turb) {
+      res[6 * k + 4] = S->case_res_k[k];
+      res[6 * k + 5] = S->case_res_w[k];
+    }
+  }
+}
+bool simple_case_converged(const xcal_simple* S, int k) { return S->case_conv.at(k) != 0; }
+bool simple_case_bad(const xcal_simple* S, int k) { return S->case_bad.at(k) != 0; }
+void simple_initialise(xcal_simple* S) {
+  XCAL_HIP(hipSetDevice(S->mesh->ctx->device));
+  initialise(S);
+}
+
+// Case ks of src becomes case kd of dst: the fields a SIMPLE iteration
+// carries to the next (U, p, grad p, mdotf, nueff; k, omega, nut, nutf with
+// k-omega) are copied on the device.  Both hold the same case mesh, in its
+// own device numbering, on one context.
+void simple_copy_case(xcal_simple* dst, int kd, const xcal_simple* src, int ks) {
+  const xcal_mesh *md = dst->mesh, *ms = src->mesh;
+  XCAL_CHECK(md->ctx == ms->ctx, "cases copied between contexts");
+  const int64_t nc = ms->case_cell[ks + 1] - ms->case_cell[ks];
+  const int64_t nb = ms->case_bface[ks + 1] - ms->case_bface[ks];
+  const int64_t ni = ms->case_iface[ks + 1] - ms->case_iface[ks];
+  XCAL_CHECK(md->case_cell[kd + 1] - md->case_cell[kd] == nc &&
+                 md->case_bface[kd + 1] - md->case_bface[kd] == nb &&
+                 md->case_iface[kd + 1] - md->case_iface[kd] == ni,
+             "copied case has another mesh");
+  XCAL_CHECK(dst->turb == src->turb, "copied case has another turbulence model");
+  hipStream_t st = md->ctx->stream;
+  auto cells = [&](xcal_field* d, const xcal_field* s) {
+    for (int c = 0; c < s->ncomp; ++c)
+      XCAL_HIP(hipMemcpyAsync(d->comp(c) + md->case_cell[kd], s->comp(c) + ms->case_cell[ks],
+                              sizeof(double) * nc, hipMemcpyDeviceToDevice, st));
+  };
+  auto faces = [&](xcal_field* d, const xcal_field* s) {
+    XCAL_HIP(hipMemcpyAsync(d->comp(0) + md->case_bface[kd], s->comp(0) + ms->case_bface[ks],
+                            sizeof(double) * nb, hipMemcpyDeviceToDevice, st));
+    XCAL_HIP(hipMemcpyAsync(d->comp(0) + md->case_iface[kd], s->comp(0) + ms->case_iface[ks],
+                            sizeof(double) * ni, hipMemcpyDeviceToDevice, st));
+  };
+  cells(dst->U, src->U);
+  cells(dst->p, src->p);
+  cells(dst->gradp, src->gradp);
+  faces(dst->mdotf, src->mdotf);
+  faces(dst->nueff, src->nueff);
+  if (src->turb) {
+    cells(dst->k, src->k);
+    cells(dst->omega, src->omega);
+    cells(dst->nut, src->nut);
+    faces(dst->nutf, src->nutf);
+  }
+}
+
+// Field `which` (XCAL_SIMPLE_*) of case k in its reference numbering.
+void simple_case_field(const xcal_simple* S, int which, int k, double* host) {
+  const xcal_mesh* m = S->mesh;
+  const xcal_field* f = which == XCAL_SIMPLE_U       ? S->U
+                        : which == XCAL_SIMPLE_P     ? S->p
+                        : which == XCAL_SIMPLE_K     ? S->k
+                        : which == XCAL_SIMPLE_OMEGA ? S->omega
+                        : which == XCAL_SIMPLE_NUT   ? S->nut
+                                                     : nullptr;
+  XCAL_CHECK(f, "unknown cell field selector (or no turbulence model)");
+  const int64_t c0 = m->case_cell[k], nk = m->case_cell[k + 1] - c0;
+  const std::vector<int32_t> perm(m->cperm.begin() + c0, m->cperm.begin() + c0 + nk);
+  for (int c = 0; c < f->ncomp; ++c)
+    download_permuted(m, perm, f->comp(c), host + (int64_t)c * nk, nk, f->len);
+}
+
+}  // namespace xcal
 
 extern "C" {
 
@@ -601,7 +716,9 @@ int xcal_simple_set_turbulence(xcal_simple* S, const xcal_turbulence_setup* t) {
       return 0;
     }
     XCAL_CHECK(t->model == XCAL_TURB_KOMEGA, "unknown turbulence model");
-    XCAL_CHECK(!m->batch, "turbulence models run on single meshes");
+    if (m->batch)  // the k-omega kernels take one viscosity
+      for (double nu : S->case_nu)
+        XCAL_CHECK(nu == S->setup.nu, "k-omega on a batch mesh needs one viscosity for all cases");
     XCAL_CHECK(t->bc_k && t->bc_omega && t->bc_nut, "k, omega and nut need boundary conditions");
     XCAL_CHECK(t->div_scheme >= XCAL_DIV_LINEAR && t->div_scheme <= XCAL_DIV_BOUNDED_UPWIND,
                "k/omega divergence scheme must be a Divergence type");

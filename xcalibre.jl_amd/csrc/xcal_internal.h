@@ -346,11 +346,22 @@ void make_sweep_plans(xcal_mesh* m, int small);
 // meshes built on the host first (server admission), uploaded later
 struct PreparedMesh;
 std::shared_ptr<PreparedMesh> prepare_mesh(const xcal_mesh_host* const* hosts, int ncase, bool batch);
+std::shared_ptr<PreparedMesh> prepare_union(const std::vector<const PreparedMesh*>& parts);
 xcal_mesh* upload_prepared(xcal_ctx* ctx, PreparedMesh& P);
 // Serving policy of a SIMPLE object: stop a run at the first non-finite
 // residual (the reference's loop would carry on); simple_diverged reports it.
 void simple_stop_nonfinite(xcal_simple* S, bool on);
 bool simple_diverged(const xcal_simple* S);
+// Serving engine: one iteration of every case (res: Ux, Uy, Uz, p, k, omega
+// residuals per case), per-case outcome, initial calculations, state
+// transfer between batches and per-case read-back.
+void simple_step(xcal_simple* S, std::vector<double>& res);
+bool simple_case_converged(const xcal_simple* S, int k);
+bool simple_case_bad(const xcal_simple* S, int k);
+void simple_initialise(xcal_simple* S);
+void simple_copy_case(xcal_simple* dst, int kd, const xcal_simple* src, int ks);
+void simple_case_field(const xcal_simple* S, int which, int k, double* host);
+
 // device bytes a single-case SIMPLE run on P occupies (estimate); turb adds
 // the k-omega fields
 int64_t prepared_footprint(const PreparedMesh& P, bool turb = false);

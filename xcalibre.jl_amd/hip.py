@@ -46,6 +46,7 @@ def lib():
                                        C.POINTER(_vp)],
             "xcal_mesh_cases": [_vp, _pi32, C.POINTER(C.c_int64)],
             "xcal_mesh_sweep_form": [_vp, _pi32, _pi32],
+            "xcal_mesh_sweep_far": [_vp, _pi32, _pi32],
             "xcal_mesh_cone_form": [_vp, _pi32, _pi32, C.POINTER(C.c_int64), C.POINTER(C.c_int64)],
             "xcal_solve_system_batch": [_vp, C.POINTER(A.SolverSetupC), _vp, C.c_int32,
                                         C.POINTER(A.SolveStats)],
@@ -201,6 +202,13 @@ class DeviceMesh:
         f, b = C.c_int32(), C.c_int32()
         check(lib().xcal_mesh_sweep_form(self.h, C.byref(f), C.byref(b)))
         return f.value, b.value
+
+    def sweep_far(self):
+        """(forward, backward): whether the pipelined batch sweeps read
+        dependencies beyond their LDS ring from global memory."""
+        f, b = C.c_int32(), C.c_int32()
+        check(lib().xcal_mesh_sweep_far(self.h, C.byref(f), C.byref(b)))
+        return bool(f.value), bool(b.value)
 
     def cone_form(self):
         """(rows per block, blocks, forward cone rows, backward cone rows) of
@@ -442,8 +450,9 @@ class Simple:
 
 
 class Server:
-    """Serving engine (xcal_server_*): continuous admission of independent
-    steady SIMPLE cases into `slots` concurrent slots under an HBM budget."""
+    """Serving engine (xcal_server_*): continuous batching of independent
+    steady SIMPLE cases; at most `slots` cases in flight, stepped together in
+    shared launches, under an HBM budget."""
 
     def __init__(self, device=0, slots=4, hbm_budget=64 << 30, max_iterations=1000):
         cfg = A.ServerConfigC(device, slots, int(hbm_budget), max_iterations)
@@ -472,7 +481,8 @@ class Server:
         check(lib().xcal_server_poll(self.h, rid, C.byref(st)))
         return dict(state=st.state, iterations=st.iterations, cells=st.cells,
                     footprint=st.footprint, t_submit=st.t_submit, t_start=st.t_start,
-                    t_end=st.t_end)
+                    t_end=st.t_end, converged=bool(st.converged),
+                    batch_rebuilds=st.batch_rebuilds)
 
     def wait(self, rid: int = 0, timeout: float = 0.0) -> bool:
         """True when finished, False on timeout."""

@@ -185,6 +185,15 @@ def serve_leg(args, x, requests, rate, max_iterations, alone=True):
         srv1.close()
     pct = lambda a, q: round(float(np.percentile(a, q)), 3) if len(a) else None
     big = int(np.argmax(ncell))
+    # admission cost of the largest request's matrix connectivity (CSR
+    # pattern, slot maps, DILU levels): device kernels against the host loops
+    ctx = H.Context(0)
+    Kd = [H.Connectivity(ctx, cases[big][0], on_device=True) for _ in range(2)][-1]
+    Kh = H.Connectivity(ctx, cases[big][0], on_device=False)
+    ingest = {"cells": int(ncell[big]), "levels": list(Kd.levels),
+              "device_build_ms": round(Kd.wall_ms, 2), "device_kernels_ms": round(Kd.device_ms, 2),
+              "host_build_ms": round(Kh.wall_ms, 2)}
+    del Kd, Kh, ctx
     summary = {
         "workload": f"{requests} 2D backward-facing steps, step ratio U[0.3,0.6], cells "
                     f"log-uniform [4k,250k], "
@@ -200,6 +209,7 @@ def serve_leg(args, x, requests, rate, max_iterations, alone=True):
         "queue_wait_p50_s": pct(wait, 50), "queue_wait_max_s": pct(wait, 100),
         "submit_call_p50_s": pct(np.array(submit_s), 50),
         "submit_call_largest_case_s": round(submit_s[big], 3),
+        "connectivity_largest_case": ingest,
         "largest_case_cells": int(ncell[big]),
         "makespan_s": round(wall, 2), "requests_per_s": round(requests / wall, 3),
         "batch_rebuilds": int(rebuilds), "mesh_build_s": round(t_mesh, 2),

@@ -205,6 +205,29 @@ int xcal_mesh_cone_form(const xcal_mesh* mesh, int32_t* block_rows, int32_t* blo
 /* CSR pattern, 0-based (rowptr n+1, colval nnz) */
 int xcal_mesh_csr_pattern(const xcal_mesh* mesh, int32_t* rowptr, int32_t* colval);
 
+/* Mesh ingest: the matrix connectivity of a mesh in the reference numbering,
+ * built by device kernels (on_device = 1) or by the host loops (0); the two
+ * are equal element for element.  Replaces sparse_matrix_connectivity +
+ * sparsecsr (ModelFramework_0_types.jl:139-140, 223-242) and the spindex
+ * searches of assembly (:246-258).  xcal_mesh_create builds its mesh with the
+ * device form (XCAL_HOST_INGEST=1 selects the host loops); this handle
+ * exposes the arrays for checks and timing.  The device form runs on the
+ * context's stream and synchronises it. */
+typedef struct xcal_connectivity xcal_connectivity;
+int xcal_connectivity_create(xcal_ctx* ctx, const xcal_mesh_host* mesh, int32_t on_device,
+                             xcal_connectivity** out);
+/* nnz, cell-face pairs, forward and backward DILU level counts, wall time
+ * of the build (host clock, uploads and downloads included) and, for the
+ * device form, the kernel time between events on the stream (ms) */
+int xcal_connectivity_info(const xcal_connectivity* c, int64_t* nnz, int64_t* n_cell_faces,
+                           int32_t* nlev_f, int32_t* nlev_b, double* wall_ms, double* device_ms);
+/* which: 0 rowptr (n+1), 1 colval (nnz), 2 diagonal slot (n), 3 slot of each
+ * cell_faces entry (n_cell_faces, faces_range order), 4 slot of each entry's
+ * transpose (nnz, -1 if absent), 5 owner-row slot of each face (n_faces, -1
+ * on boundary faces), 6 forward DILU level (n), 7 backward DILU level (n) */
+int xcal_connectivity_get(const xcal_connectivity* c, int32_t which, int32_t* host);
+int xcal_connectivity_destroy(xcal_connectivity* c);
+
 /* Batched form (no single reference counterpart: it runs many independent
  * cases of the reference's mesh/equation/run! path at once).  The cases'
  * meshes become one device mesh whose fields, equations and SIMPLE runs
@@ -433,8 +456,9 @@ int xcal_simple_set_case_nu(xcal_simple* s, const double* nu);
 int xcal_simple_stage_times(const xcal_simple* s, double* ms6);
 
 /* Serving (no single reference counterpart: it runs many independent
- * run!(model, config) cases).  Requests are built on the host at submission
- * and wait in a FIFO queue.  The cases in flight form one batch mesh and are
+ * run!(model, config) cases).  Requests are built at submission (the matrix
+ * connectivity on the device, as xcal_connectivity_create; the rest on the
+ * host) and wait in a FIFO queue.  The cases in flight form one batch mesh and are
  * stepped together, every launch of a SIMPLE iteration covering all of them
  * (continuous batching).  A waiting case joins when its device footprint fits
  * the HBM budget left by the cases in flight, the batch holds fewer than

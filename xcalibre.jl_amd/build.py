@@ -30,7 +30,7 @@ HIP_FLAGS = [
     "-Wall", "-Wno-unused-function", f"-I{INC}", f"-I{CSRC}",
 ]
 HIP_SOURCES = ["core.hip", "assemble.hip", "calculate.hip", "krylov.hip", "solver.hip", "batch.hip",
-               "turbulence.hip", "simple.hip", "serve.hip", "noalloc.hip"]
+               "turbulence.hip", "simple.hip", "serve.hip", "noalloc.hip", "ingest.hip"]
 HEADERS = ["xcal_internal.h", "device_common.h"]
 
 

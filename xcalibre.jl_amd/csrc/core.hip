@@ -297,49 +297,21 @@ struct HostMesh {
   std::vector<int8_t> cf_sign, bpatch;
 };
 
-static void build_host_mesh(const xcal_mesh_host* h, HostMesh& M, bool cones) {
-  XCAL_CHECK(h->dim == 2 || h->dim == 3, "dim must be 2 or 3");
-  XCAL_CHECK(h->n_cells > 0 && h->n_faces > 0, "empty mesh");
-  XCAL_CHECK(h->n_cells < (1ll << 30) && h->n_faces < (1ll << 30), "mesh too large for int32 indices");
-  XCAL_CHECK(h->n_bfaces >= 0 && h->n_bfaces <= h->n_faces, "bad n_bfaces");
-  M.dim = h->dim;
-  const int64_t n = h->n_cells, nf = h->n_faces, nbf = h->n_bfaces;
-  M.n = n;
-  M.nf = nf;
-  M.nbf = nbf;
-  M.n_patches = h->n_patches;
-  XCAL_CHECK(h->n_patches <= kMaxPatches, "mesh has more than 32 patches");
-  for (int p = 0; p < h->n_patches; ++p) {
-    int64_t a = h->patch_range[2 * p] - 1, b = h->patch_range[2 * p + 1] - 1;
-    XCAL_CHECK(a >= 0 && b < nbf && a <= b + 1, "patch range outside boundary faces");
-    M.patch_first.push_back(a);
-    M.patch_last.push_back(b);
-  }
-  // cell -> internal face lists in the reference's faces_range order
-  std::vector<int32_t> cf_ptr(n + 1, 0), cf_face, cf_nbr;
-  std::vector<int8_t> cf_sign;
-  cf_face.reserve(h->n_cell_faces);
-  cf_nbr.reserve(h->n_cell_faces);
-  cf_sign.reserve(h->n_cell_faces);
-  for (int64_t c = 0; c < n; ++c) {
-    int64_t a = h->cell_faces_range[2 * c], b = h->cell_faces_range[2 * c + 1];
-    for (int64_t k = a; k <= b; ++k) {
-      XCAL_CHECK(k >= 1 && k <= h->n_cell_faces, "faces_range outside cell_faces");
-      int64_t f = h->cell_faces[k - 1] - 1, nb = h->cell_neighbours[k - 1] - 1;
-      XCAL_CHECK(f >= nbf && f < nf, "cell face is not an internal face");
-      XCAL_CHECK(nb >= 0 && nb < n, "neighbour index out of range");
-      XCAL_CHECK(h->cell_nsign[k - 1] == 1 || h->cell_nsign[k - 1] == -1, "nsign must be +-1");
-      cf_face.push_back((int32_t)f);
-      cf_nbr.push_back((int32_t)nb);
-      cf_sign.push_back((int8_t)h->cell_nsign[k - 1]);
-    }
-    cf_ptr[c + 1] = (int32_t)cf_face.size();
-  }
-  M.ncf = (int64_t)cf_face.size();
-  // CSR pattern: row i = {i} U neighbours(i), ascending, unique
-  // (sparse_matrix_connectivity + sparsecsr, ModelFramework_0_types.jl:223-242)
-  std::vector<int32_t> rowptr(n + 1, 0), colval;
-  colval.reserve(n + cf_face.size());
+// Row i of the matrix is {i} U neighbours(i), ascending and unique
+// (sparse_matrix_connectivity + sparsecsr, ModelFramework_0_types.jl:223-242).
+// The slot maps replace spindex (ModelFramework_0_types.jl:246-258): the
+// entry of each cell-face pair, of each internal face's owner row, of each
+// diagonal and of each entry's transpose.  DILU level schedules: forward
+// level(i) = 1 + max level of its lower neighbours, backward the same over
+// upper neighbours.  Rows inside a level are independent; level order
+// preserves every i<j dependency of the sequential sweeps.
+void host_connectivity(const CellLists& L, Connectivity& K) {
+  const int64_t n = L.n, nf = L.nf, nbf = L.nbf;
+  const std::vector<int32_t>&cf_ptr = L.cf_ptr, &cf_nbr = L.cf_nbr, &own = L.own, &nei = L.nei;
+  std::vector<int32_t>&rowptr = K.rowptr, &colval = K.colval;
+  rowptr.assign(n + 1, 0);
+  colval.clear();
+  colval.reserve(n + cf_nbr.size());
   std::vector<int32_t> row;
   for (int64_t c = 0; c < n; ++c) {
     row.clear();
@@ -351,29 +323,122 @@ static void build_host_mesh(const xcal_mesh_host* h, HostMesh& M, bool cones) {
     rowptr[c + 1] = (int32_t)colval.size();
   }
   const int64_t nnz = (int64_t)colval.size();
-  M.nnz = nnz;
   auto find = [&](int64_t i, int32_t j) -> int32_t {
     const int32_t* b = colval.data() + rowptr[i];
     const int32_t* e = colval.data() + rowptr[i + 1];
     const int32_t* it = std::lower_bound(b, e, j);
     return (it != e && *it == j) ? (int32_t)(it - colval.data()) : -1;
   };
-  std::vector<int32_t> diag(n), cf_slot(cf_face.size()), trans(nnz);
+  K.diag.assign(n, 0);
+  K.cf_slot.assign(cf_nbr.size(), 0);
+  K.trans.assign(nnz, 0);
   for (int64_t c = 0; c < n; ++c) {
-    diag[c] = find(c, (int32_t)c);
-    for (int32_t k = cf_ptr[c]; k < cf_ptr[c + 1]; ++k) cf_slot[k] = find(c, cf_nbr[k]);
-    for (int32_t k = rowptr[c]; k < rowptr[c + 1]; ++k) trans[k] = find(colval[k], (int32_t)c);
+    K.diag[c] = find(c, (int32_t)c);
+    for (int32_t k = cf_ptr[c]; k < cf_ptr[c + 1]; ++k) K.cf_slot[k] = find(c, cf_nbr[k]);
+    for (int32_t k = rowptr[c]; k < rowptr[c + 1]; ++k) K.trans[k] = find(colval[k], (int32_t)c);
   }
+  K.fslot.assign(nf, -1);
+  for (int64_t f = nbf; f < nf; ++f) K.fslot[f] = find(own[f], nei[f]);
+  K.flev.assign(n, 0);
+  K.blev.assign(n, 0);
+  int maxl = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    int l = 0;
+    for (int32_t k = rowptr[i]; k < K.diag[i]; ++k) l = std::max(l, K.flev[colval[k]] + 1);
+    K.flev[i] = l;
+    maxl = std::max(maxl, l);
+  }
+  K.nlev_f = maxl + 1;
+  maxl = 0;
+  for (int64_t i = n - 1; i >= 0; --i) {
+    int l = 0;
+    for (int32_t k = K.diag[i] + 1; k < rowptr[i + 1]; ++k) l = std::max(l, K.blev[colval[k]] + 1);
+    K.blev[i] = l;
+    maxl = std::max(maxl, l);
+  }
+  K.nlev_b = maxl + 1;
+}
+
+void cell_lists(const xcal_mesh_host* h, CellLists& L) {
+  XCAL_CHECK(h->dim == 2 || h->dim == 3, "dim must be 2 or 3");
+  XCAL_CHECK(h->n_cells > 0 && h->n_faces > 0, "empty mesh");
+  XCAL_CHECK(h->n_cells < (1ll << 30) && h->n_faces < (1ll << 30), "mesh too large for int32 indices");
+  XCAL_CHECK(h->n_bfaces >= 0 && h->n_bfaces <= h->n_faces, "bad n_bfaces");
+  XCAL_CHECK(h->n_cell_faces >= 0 && h->n_cell_faces < (1ll << 31), "bad n_cell_faces");
+  const int64_t n = h->n_cells, nf = h->n_faces, nbf = h->n_bfaces;
+  L.n = n;
+  L.nf = nf;
+  L.nbf = nbf;
+  // cell -> internal face lists in the reference's faces_range order
+  L.cf_ptr.assign(n + 1, 0);
+  L.cf_face.clear();
+  L.cf_nbr.clear();
+  L.cf_sign.clear();
+  L.cf_face.reserve(h->n_cell_faces);
+  L.cf_nbr.reserve(h->n_cell_faces);
+  L.cf_sign.reserve(h->n_cell_faces);
+  for (int64_t c = 0; c < n; ++c) {
+    int64_t a = h->cell_faces_range[2 * c], b = h->cell_faces_range[2 * c + 1];
+    for (int64_t k = a; k <= b; ++k) {
+      XCAL_CHECK(k >= 1 && k <= h->n_cell_faces, "faces_range outside cell_faces");
+      int64_t f = h->cell_faces[k - 1] - 1, nb = h->cell_neighbours[k - 1] - 1;
+      XCAL_CHECK(f >= nbf && f < nf, "cell face is not an internal face");
+      XCAL_CHECK(nb >= 0 && nb < n, "neighbour index out of range");
+      XCAL_CHECK(h->cell_nsign[k - 1] == 1 || h->cell_nsign[k - 1] == -1, "nsign must be +-1");
+      L.cf_face.push_back((int32_t)f);
+      L.cf_nbr.push_back((int32_t)nb);
+      L.cf_sign.push_back((int8_t)h->cell_nsign[k - 1]);
+    }
+    XCAL_CHECK(L.cf_face.size() < (size_t)(1ll << 31), "cell-face lists exceed int32 indices");
+    L.cf_ptr[c + 1] = (int32_t)L.cf_face.size();
+  }
+  L.own.assign(nf, 0);
+  L.nei.assign(nf, 0);
+  for (int64_t f = 0; f < nf; ++f) {
+    L.own[f] = (int32_t)(h->face_owner[2 * f] - 1);
+    L.nei[f] = (int32_t)(h->face_owner[2 * f + 1] - 1);
+    XCAL_CHECK(L.own[f] >= 0 && L.own[f] < n && L.nei[f] >= 0 && L.nei[f] < n,
+               "ownerCells out of range");
+  }
+}
+
+static void build_host_mesh(const xcal_mesh_host* h, HostMesh& M, bool cones, xcal_ctx* ingest) {
+  CellLists L;
+  cell_lists(h, L);
+  M.dim = h->dim;
+  const int64_t n = L.n, nf = L.nf, nbf = L.nbf;
+  M.n = n;
+  M.nf = nf;
+  M.nbf = nbf;
+  M.n_patches = h->n_patches;
+  XCAL_CHECK(h->n_patches <= kMaxPatches, "mesh has more than 32 patches");
+  for (int p = 0; p < h->n_patches; ++p) {
+    int64_t a = h->patch_range[2 * p] - 1, b = h->patch_range[2 * p + 1] - 1;
+    XCAL_CHECK(a >= 0 && b < nbf && a <= b + 1, "patch range outside boundary faces");
+    M.patch_first.push_back(a);
+    M.patch_last.push_back(b);
+  }
+  std::vector<int32_t>&cf_ptr = L.cf_ptr, &cf_face = L.cf_face, &cf_nbr = L.cf_nbr, &own = L.own,
+                      &nei = L.nei;
+  std::vector<int8_t>& cf_sign = L.cf_sign;
+  M.ncf = (int64_t)cf_face.size();
+  // CSR pattern, slot maps and DILU levels, on the device when a context is
+  // given (the arrays are equal either way)
+  Connectivity K;
+  if (ingest) device_connectivity(ingest, L, K);
+  else host_connectivity(L, K);
+  std::vector<int32_t>&rowptr = K.rowptr, &colval = K.colval, &diag = K.diag, &cf_slot = K.cf_slot,
+                      &trans = K.trans, &fslot = K.fslot, &flev = K.flev, &blev = K.blev;
+  const int64_t nnz = (int64_t)colval.size();
+  M.nnz = nnz;
+  M.nlev_f = K.nlev_f;
+  M.nlev_b = K.nlev_b;
   // faces
-  std::vector<int32_t> own(nf), nei(nf);
   std::vector<double> area(h->face_area, h->face_area + nf), weight(h->face_weight, h->face_weight + nf),
       delta(h->face_delta, h->face_delta + nf), nx(nf), ny(nf), nz(nf), ex(nf), ey(nf), ez(nf),
       sfx(nf), sfy(nf), sfz(nf), lapEf(nf), rix(nf, 0.0), riy(nf, 0.0), riz(nf, 0.0),
       ncw(nf, 0.0), ntx(nf, 0.0), nty(nf, 0.0), ntz(nf, 0.0);
   for (int64_t f = 0; f < nf; ++f) {
-    own[f] = (int32_t)(h->face_owner[2 * f] - 1);
-    nei[f] = (int32_t)(h->face_owner[2 * f + 1] - 1);
-    XCAL_CHECK(own[f] >= 0 && own[f] < n && nei[f] >= 0 && nei[f] < n, "ownerCells out of range");
     nx[f] = h->face_normal[3 * f];
     ny[f] = h->face_normal[3 * f + 1];
     nz[f] = h->face_normal[3 * f + 2];
@@ -420,8 +485,6 @@ static void build_host_mesh(const xcal_mesh_host* h, HostMesh& M, bool cones) {
       ntz[f] = area[f] * n3[2] - (c1c2[2] * fac) * area[f];
     }
   }
-  std::vector<int32_t> fslot(nf, -1);
-  for (int64_t f = nbf; f < nf; ++f) fslot[f] = find(own[f], nei[f]);
   // boundary faces grouped per cell, ascending face id
   std::vector<int8_t> bpatch(nbf, -1);
   for (int p = 0; p < h->n_patches; ++p)
@@ -451,26 +514,6 @@ static void build_host_mesh(const xcal_mesh_host* h, HostMesh& M, bool cones) {
     }
   }
   M.nbcell = (int64_t)bc_cells.size();
-  // DILU level schedules: forward level(i) = 1 + max level of lower
-  // neighbours; backward symmetric over upper neighbours.  Rows inside a
-  // level are independent; level order preserves every i<j dependency.
-  std::vector<int32_t> flev(n, 0), blev(n, 0);
-  int maxl = 0;
-  for (int64_t i = 0; i < n; ++i) {
-    int l = 0;
-    for (int32_t k = rowptr[i]; k < diag[i]; ++k) l = std::max(l, flev[colval[k]] + 1);
-    flev[i] = l;
-    maxl = std::max(maxl, l);
-  }
-  M.nlev_f = maxl + 1;
-  maxl = 0;
-  for (int64_t i = n - 1; i >= 0; --i) {
-    int l = 0;
-    for (int32_t k = diag[i] + 1; k < rowptr[i + 1]; ++k) l = std::max(l, blev[colval[k]] + 1);
-    blev[i] = l;
-    maxl = std::max(maxl, l);
-  }
-  M.nlev_b = maxl + 1;
   std::vector<double> vol(h->cell_volume, h->cell_volume + n);
 
   // ---- device numbering ------------------------------------------------
@@ -1008,12 +1051,19 @@ struct PreparedMesh {
   bool batch = false;
 };
 
-std::shared_ptr<PreparedMesh> prepare_mesh(const xcal_mesh_host* const* hosts, int ncase, bool batch) {
+bool host_ingest_forced() {
+  const char* e = std::getenv("XCAL_HOST_INGEST");
+  return e && *e == '1';
+}
+
+std::shared_ptr<PreparedMesh> prepare_mesh(const xcal_mesh_host* const* hosts, int ncase, bool batch,
+                                           xcal_ctx* ingest, bool cones) {
   XCAL_CHECK(hosts && ncase >= 1, "NULL argument or empty batch");
+  if (host_ingest_forced()) ingest = nullptr;
   std::vector<HostMesh> C(ncase);
   for (int k = 0; k < ncase; ++k) {
     XCAL_CHECK(hosts[k], "NULL mesh in batch");
-    build_host_mesh(hosts[k], C[k], !batch && ncase == 1);
+    build_host_mesh(hosts[k], C[k], cones && !batch && ncase == 1, ingest);
   }
   auto P = std::make_shared<PreparedMesh>();
   union_cases(C, P->U, P->cc, P->cb, P->ci, P->lf, P->lb);
@@ -1180,7 +1230,7 @@ xcal_mesh* upload_prepared(xcal_ctx* ctx, PreparedMesh& P) {
 static xcal_mesh* create_mesh(xcal_ctx* ctx, const xcal_mesh_host* const* hosts, int ncase,
                               bool batch) {
   XCAL_CHECK(ctx, "NULL context");
-  auto P = prepare_mesh(hosts, ncase, batch);
+  auto P = prepare_mesh(hosts, ncase, batch, ctx);
   return upload_prepared(ctx, *P);
 }
 

@@ -1,6 +1,7 @@
 // Serving engine: many independent steady SIMPLE cases on one device with
-// continuous batching.  A case (request) is built on the host when it is
-// submitted and waits in a FIFO queue.  One scheduler thread keeps the cases
+// continuous batching.  A case (request) is built when it is submitted (its
+// matrix connectivity on the device, the rest on the host) and waits in a
+// FIFO queue.  One scheduler thread keeps the cases
 // in flight as one batch mesh (the union of their meshes, each case in its
 // own numbering) and steps them together: every kernel launch of a SIMPLE
 // iteration covers all of them, and each case's Krylov solves stop on their
@@ -77,6 +78,11 @@ struct xcal_server {
   // (moving every case's state each time; a test of the state transfer)
   bool rebuild_always = false;
   std::thread scheduler;
+  // mesh ingest at submission: the CSR pattern, slot maps and DILU levels of
+  // a request's mesh are built on the device, on this context's stream (one
+  // submission at a time)
+  xcal_ctx* ingest = nullptr;
+  std::mutex ingest_mu;
   std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
   double now() const {
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -291,6 +297,7 @@ int xcal_server_create(const xcal_server_config* cfg, xcal_server** out) {
     XCAL_CHECK(cfg->device >= 0 && cfg->device < ndev, "device index out of range");
     auto S = std::make_unique<xcal_server>();
     S->cfg = *cfg;
+    check_rc(xcal_ctx_create(cfg->device, &S->ingest));
     const char* rb = std::getenv("XCAL_SERVE_REBUILD");
     S->rebuild_always = rb && *rb == '1';
     S->scheduler = std::thread(scheduler, S.get());
@@ -308,6 +315,7 @@ int xcal_server_destroy(xcal_server* S) {
     }
     S->cv.notify_all();
     S->scheduler.join();  // the batch in flight runs to completion first
+    if (S->ingest) xcal_ctx_destroy(S->ingest);
     delete S;
     return 0;
   });
@@ -325,7 +333,10 @@ int xcal_server_submit(xcal_server* S, const xcal_mesh_host* mesh, const xcal_si
     XCAL_CHECK(rans || !(k0 || omega0 || nut0), "k, omega and nut fields need a turbulence model");
     auto r = std::make_shared<Request>();
     r->t_submit = S->now();
-    r->mesh = prepare_mesh(&mesh, 1, false);  // host-side build happens at submission
+    {  // the mesh is built at submission (batches carry no cone form)
+      std::lock_guard<std::mutex> lk(S->ingest_mu);
+      r->mesh = prepare_mesh(&mesh, 1, false, S->ingest, false);
+    }
     r->cells = prepared_cells(*r->mesh);
     r->footprint = prepared_footprint(*r->mesh, rans);
     XCAL_CHECK(r->footprint <= S->cfg.hbm_budget, "case does not fit the server's HBM budget");

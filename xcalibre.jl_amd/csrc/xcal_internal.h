@@ -345,7 +345,31 @@ struct BCsK {
 void make_sweep_plans(xcal_mesh* m, int small);
 // meshes built on the host first (server admission), uploaded later
 struct PreparedMesh;
-std::shared_ptr<PreparedMesh> prepare_mesh(const xcal_mesh_host* const* hosts, int ncase, bool batch);
+// Host build of one mesh or a batch.  With `ingest`, the CSR pattern, slot
+// maps and DILU levels are built on that context's device (unless
+// XCAL_HOST_INGEST=1); `cones` allows the cone form of single meshes.
+std::shared_ptr<PreparedMesh> prepare_mesh(const xcal_mesh_host* const* hosts, int ncase, bool batch,
+                                           xcal_ctx* ingest = nullptr, bool cones = true);
+bool host_ingest_forced();
+// Matrix connectivity of one mesh in the reference numbering: CSR pattern,
+// slot maps (cell-face pair, internal face's owner row, diagonal, transpose)
+// and the forward/backward DILU level of every row.
+struct Connectivity {
+  std::vector<int32_t> rowptr, colval, diag, cf_slot, trans, fslot, flev, blev;
+  int nlev_f = 0, nlev_b = 0;
+  double device_ms = 0.0;  // device build: kernel time (events on the stream)
+};
+// A mesh's cell-face lists (0-based, validated): each cell's internal faces,
+// neighbours and nsign in faces_range order, and the two cells of every face
+// (boundary faces first, nbf of them).
+struct CellLists {
+  int64_t n = 0, nf = 0, nbf = 0;
+  std::vector<int32_t> cf_ptr, cf_face, cf_nbr, own, nei;
+  std::vector<int8_t> cf_sign;
+};
+void cell_lists(const xcal_mesh_host* h, CellLists& L);
+void host_connectivity(const CellLists& L, Connectivity& K);
+void device_connectivity(xcal_ctx* ctx, const CellLists& L, Connectivity& K);
 std::shared_ptr<PreparedMesh> prepare_union(const std::vector<const PreparedMesh*>& parts);
 xcal_mesh* upload_prepared(xcal_ctx* ctx, PreparedMesh& P);
 // Serving policy of a SIMPLE object: stop a run at the first non-finite

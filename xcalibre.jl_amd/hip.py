@@ -42,6 +42,11 @@ def lib():
             "xcal_mesh_info": [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), _pi32, _pi32,
                                C.POINTER(C.c_int64)],
             "xcal_mesh_csr_pattern": [_vp, _pi32, _pi32],
+            "xcal_connectivity_create": [_vp, C.POINTER(A.MeshHost), C.c_int32, C.POINTER(_vp)],
+            "xcal_connectivity_info": [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), _pi32,
+                                       _pi32, _pd, _pd],
+            "xcal_connectivity_get": [_vp, C.c_int32, _pi32],
+            "xcal_connectivity_destroy": [_vp],
             "xcal_mesh_create_batch": [_vp, C.POINTER(C.POINTER(A.MeshHost)), C.c_int32,
                                        C.POINTER(_vp)],
             "xcal_mesh_cases": [_vp, _pi32, C.POINTER(C.c_int64)],
@@ -152,6 +157,40 @@ class Context:
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:
             _lib.xcal_ctx_destroy(self.h)
+            self.h = None
+
+
+class Connectivity:
+    """xcal_connectivity: a mesh's CSR pattern, slot maps and DILU levels
+    built on the device (on_device=True) or by the host loops."""
+
+    NAMES = ("rowptr", "colval", "diag", "cf_slot", "trans", "face_slot", "flev", "blev")
+
+    def __init__(self, ctx: Context | None, mesh, on_device: bool = True):
+        self.h = _vp()
+        host = A.mesh_host(mesh)
+        check(lib().xcal_connectivity_create(ctx.h if ctx is not None else None, C.byref(host),
+                                             int(on_device), C.byref(self.h)))
+        nnz, ncf_, fl, bl = C.c_int64(), C.c_int64(), C.c_int32(), C.c_int32()
+        wall, dev = C.c_double(), C.c_double()
+        check(lib().xcal_connectivity_info(self.h, C.byref(nnz), C.byref(ncf_), C.byref(fl),
+                                           C.byref(bl), C.byref(wall), C.byref(dev)))
+        self.nnz, self.levels = nnz.value, (fl.value, bl.value)
+        self.wall_ms, self.device_ms = wall.value, dev.value
+        n, nf, ncf = mesh.n_cells, mesh.n_faces, ncf_.value
+        sizes = (n + 1, self.nnz, n, ncf, self.nnz, nf, n, n)
+        self.arrays = {}
+        for w, (name, size) in enumerate(zip(self.NAMES, sizes)):
+            a = np.zeros(size, np.int32)
+            check(lib().xcal_connectivity_get(self.h, w, a.ctypes.data_as(_pi32)))
+            self.arrays[name] = a
+
+    def __getitem__(self, name):
+        return self.arrays[name]
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.xcal_connectivity_destroy(self.h)
             self.h = None
 
 

@@ -20,7 +20,10 @@ cpu_baseline = the CPU oracle (restatement of the reference's FP64 path)
 detail.mixed_precision = the same channel with FP32 inner Krylov solves in
             FP64 iterative refinement, and its per-field error against FP64;
 detail.batched = 256 batched 2D cavities (FP64 and FP32-IR pressure), the
-            form that --mode batch reports in full.
+            form that --mode batch reports in full: value with the Krylov
+            iterations replayed from HIP graphs, the same run with eager
+            launches beside it, and the rows per thread the built-in tuner
+            picked for the DILU sweeps with its timings.
 detail.serving = 8 k-omega step cases served with Poisson arrivals and
             continuous batching: p50/p99 time to converged over the converged
             requests, with the counts of converged, capped and diverged ones;
@@ -332,24 +335,39 @@ def batch_line(args, x, precision, steps=None, warmup=None):
     ctx = H.Context(0)
     t0 = time.perf_counter()
     dm = H.DeviceMesh.batch(ctx, meshes)
-    sim = H.Simple(dm, setup)
-    sim.set_case_nu(1.0 / Re)
     ctx.synchronize()
     t_upload = time.perf_counter() - t0
-    if warmup:
-        sim.run_batch(warmup)
-    ctx.time_kernel(args.roofline_kernel, True)
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    out = sim.run_batch(steps)
-    ctx.synchronize()
-    t1 = time.perf_counter()
+
+    def timed_run(graph, kernel_timing):
+        """warmup + steps on a fresh SIMPLE state; seconds of the timed steps.
+        graph: Krylov iterations replayed from HIP graphs (otherwise eager)."""
+        os.environ["XCAL_GRAPH"] = "1" if graph else "0"
+        try:
+            sm = H.Simple(dm, setup)
+            sm.set_case_nu(1.0 / Re)
+            if warmup:
+                sm.run_batch(warmup)
+            if kernel_timing:
+                ctx.time_kernel(args.roofline_kernel, True)
+            ctx.synchronize()
+            ta = time.perf_counter()
+            o = sm.run_batch(steps)
+            ctx.synchronize()
+            tb = time.perf_counter()
+        finally:
+            os.environ.pop("XCAL_GRAPH", None)
+        if o["done"] != steps:
+            raise RuntimeError(f"batch stopped after {o['done']} of {steps} steps")
+        return sm, o, tb - ta
+
+    # eager launches with the roofline kernel timed, then the same run from
+    # the same start with HIP-graph replay (the headline value)
+    _, _, wall_eager = timed_run(False, True)
     launches, kms = ctx.kernel_time(args.roofline_kernel)
     ctx.time_kernel(args.roofline_kernel, False)
-    if out["done"] != steps:
-        raise RuntimeError(f"batch stopped after {out['done']} of {steps} steps")
-    wall = t1 - t0
+    sim, out, wall = timed_run(True, False)
     n = dm.n
+    f_rows, b_rows, tune_us = dm.pipe_tuning()
     vb = 4 if precision == "fp32-ir" else 8
     nbytes = kernel_bytes(args.roofline_kernel, n, dm.nnz, vb)
     avg_s = kms / 1e3 / max(launches, 1)
@@ -387,6 +405,12 @@ def batch_line(args, x, precision, steps=None, warmup=None):
         "cpu_baseline": None,
         "detail": {"mesh_build_s": round(t_mesh, 2), "upload_s": round(t_upload, 2),
                    "case_iters_per_s": round(args.cases * steps / wall, 1),
+                   "graph": {"on_ms_per_step": round(1e3 * wall / steps, 3),
+                             "off_ms_per_step": round(1e3 * wall_eager / steps, 3),
+                             "speedup": round(wall_eager / wall, 4)},
+                   "tuned": {"dilu_rows_per_thread": [f_rows, b_rows],
+                             "tuner_us_per_launch": {f"{d}_rw{r}": round(v, 2)
+                                                     for (d, r), v in tune_us.items()}},
                    "device_ms": {k: round(v, 2) for k, v in sim.stage_times().items()},
                    "p_iters_max_mean": float(np.mean(kit[:, 3].max(axis=0))),
                    "error_vs_fp64": err},
@@ -709,7 +733,8 @@ def main():
             batched[prec] = {"value": bl["value"], "unit": bl["unit"],
                              "case_iters_per_s": bl["detail"]["case_iters_per_s"],
                              "ms_per_step": bl["ms_per_step"], "cells": bl["config"]["cells"],
-                             "roofline_frac": bl["roofline"]["frac"]}
+                             "roofline_frac": bl["roofline"]["frac"],
+                             "graph": bl["detail"]["graph"], "tuned": bl["detail"]["tuned"]}
             log(f"batch {prec}: {bl['value']:.4g} cell-iters/s")
         batched["workload"] = bl["config"]["workload"].split(", SIMPLE")[0]
         _, serving = serve_leg(args, x, args.serve_requests, args.rate, args.max_iterations,

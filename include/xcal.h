@@ -193,6 +193,14 @@ int xcal_mesh_sweep_form(const xcal_mesh* mesh, int32_t* fwd, int32_t* bwd);
  * memory; 0 otherwise.  No reference counterpart (reports a device code
  * path). */
 int xcal_mesh_sweep_far(const xcal_mesh* mesh, int32_t* fwd, int32_t* bwd);
+/* Rows of a level per thread (1, 2 or 4; -1 before the first DILU solve) of
+ * the pipelined batch sweeps, forward and backward, as the built-in tuner
+ * chose them by timing every choice on the first DILU solve on a mesh of this
+ * shape (XCAL_PIPE_ROWS=1|2|4 fixes them), and its timings: us[6] = mean µs
+ * per launch, forward at 1, 2, 4 rows per thread, then backward (0 where not
+ * timed).  Results do not depend on the choice.  No reference counterpart. */
+int xcal_mesh_pipe_tuning(const xcal_mesh* mesh, int32_t* fwd_rows, int32_t* bwd_rows,
+                          double* us);
 /* Cone form of the DILU sweeps (single meshes): rows per block, blocks, and
  * the cone rows of all blocks for the forward and backward sweeps; all 0
  * where the sweeps launch level by level.  Each sweep is then one launch in

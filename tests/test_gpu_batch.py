@@ -123,8 +123,8 @@ def test_batch_krylov_alone_equals_mix_and_oracle(x, A, O, ctx, solver, precon, 
         off += m.n_cells
 
 
-def _batch_setup(api, meshes, iterations):
-    model, config = cavity_case(api, meshes[0], iterations=iterations)
+def _batch_setup(api, meshes, iterations, precon="DILU"):
+    model, config = cavity_case(api, meshes[0], iterations=iterations, precon=precon)
     return api.simple_setup(model, config, pref=0.0)
 
 
@@ -248,3 +248,62 @@ def test_batch_far_dependencies_equal_level_stepping(x, A, ctx, monkeypatch):
             out.append(runs[0])
         assert np.array_equal(out[0][0], out[1][0]), solver
         assert out[0][1] == out[1][1] and out[0][2] == out[1][2], solver
+
+
+@pytest.mark.parametrize("prec", ["PREC_FP64", "PREC_FP32_IR"])
+def test_batch_pipe_rows_per_thread_bit_identical(x, A, ctx, monkeypatch, prec):
+    """The pipelined batch sweeps at 1, 2 and 4 rows of a level per thread
+    (256, 128 and 64 threads per case) give the same CG and BiCGSTAB solves
+    bit for bit, dot products included; the tuner picks one of them and
+    reports a timing for each."""
+    ms = _cases(x)
+    out = {}
+    for rows in ("1", "2", "4", None):
+        if rows:
+            monkeypatch.setenv("XCAL_PIPE_ROWS", rows)
+        else:
+            monkeypatch.delenv("XCAL_PIPE_ROWS", raising=False)
+        dm = x.hip.DeviceMesh.batch(ctx, ms)
+        res = []
+        for solver in ("SOLVER_CG", "SOLVER_BICGSTAB"):
+            sym = solver == "SOLVER_CG"
+            setup = A.SolverSetupC(getattr(A, solver), A.PRECON_DILU, getattr(A, prec), 4000, 1e-7,
+                                   1.0, 0.0, 1e-12, 1e-4)
+            eq, phi, _, _, _ = _system(x, A, dm, ms, sym)
+            eq.precond_update(A.PRECON_DILU)
+            its, outer, r, rc = eq.solve_batch(setup, phi)
+            assert rc == 0
+            res.append((phi.download(), list(its), list(r)))
+        f, b, t = dm.pipe_tuning()
+        if rows:
+            assert (f, b) == (int(rows), int(rows))
+        else:
+            assert f in (1, 2, 4) and b in (1, 2, 4)
+            assert all(v > 0 for v in t.values()), t
+            print("tuned rows per thread", f, b, t)
+        out[rows] = res
+    for rows in ("2", "4", None):
+        for (u, iu, ru), (v, iv, rv) in zip(out["1"], out[rows]):
+            assert np.array_equal(u, v) and iu == iv and ru == rv, rows
+
+
+def test_batch_graph_replay_equals_eager(x, A, ctx, monkeypatch):
+    """Batched Krylov iterations replayed from HIP graphs (two iterations per
+    graph, captured on first use and replayed on every later solve, with
+    assembly and host reads of the stop flags between replays) give the same
+    SIMPLE run bit for bit as eager launches (XCAL_GRAPH=0), for DILU and
+    Jacobi preconditioning."""
+    ms = _cases(x)
+    api = x.api
+    for precon in ("DILU", "Jacobi"):
+        runs = []
+        for graph in ("0", "1"):
+            monkeypatch.setenv("XCAL_GRAPH", graph)
+            dm = x.hip.DeviceMesh.batch(ctx, ms)
+            setup = _batch_setup(api, ms, 12, precon=precon)
+            sim = x.hip.Simple(dm, setup)
+            sim.set_case_nu([0.01, 0.004, 0.0025])
+            out = sim.run_batch(12)
+            runs.append((sim.get("U"), sim.get("p"), out))
+        monkeypatch.delenv("XCAL_GRAPH")
+        assert np.array_equal(runs[0][0], runs[1][0]) and np.array_equal(runs[0][1], runs[1][1])

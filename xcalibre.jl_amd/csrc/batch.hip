@@ -14,7 +14,11 @@
 //    many and short), all cases in one launch.
 #include <cmath>
 #include <cstring>
+#include <cstdlib>
 #include <functional>
+#include <map>
+#include <mutex>
+#include <vector>
 
 #include "solver_common.h"
 #include "sweep_pipe.h"
@@ -299,8 +303,10 @@ __global__ void __launch_bounds__(kBlock)
 // UPD: x and r were updated by kb_cg_xr; the sweep's input is u = r.  With
 // FAR, y is also stored to W.Y (device order): a row gathers a dependency
 // that lies before the LDS ring from there.
-template <bool UPD, bool FAR, class T, int WK>
-__global__ void __launch_bounds__(kPipeBlock)
+// RW: rows of a level per thread, kPipeBlock / RW threads per workgroup
+// (the same rows, values and sums for every RW).
+template <bool UPD, bool FAR, class T, int WK, int RW>
+__global__ void __launch_bounds__(kPipeBlock / RW)
     kb_fwd_pipe(FwdK<T> W, const T* __restrict__ Lrd, const int32_t* __restrict__ lev_ptr,
                 const int32_t* __restrict__ meta, const int32_t* __restrict__ case_lev,
                 const KS* ks) {
@@ -308,12 +314,12 @@ __global__ void __launch_bounds__(kPipeBlock)
   const int k = blockIdx.x;
   if (ks && stopped(ks + k)) return;  // uniform over the workgroup
   const int u0 = case_lev[k];
-  fwd_pipe<FAR, FAR, T, WK, pipe_depth<WK>()>(W, UPD ? W.r : W.u, Lrd, lev_ptr, meta, u0,
-                                         case_lev[k + 1] - u0, false, L);
+  fwd_pipe<FAR, FAR, T, WK, pipe_depth<WK, RW>(), RW>(W, UPD ? W.r : W.u, Lrd, lev_ptr, meta, u0,
+                                                   case_lev[k + 1] - u0, false, L);
 }
 
-template <int DOT, bool FAR, class T, int WK>
-__global__ void __launch_bounds__(kPipeBlock)
+template <int DOT, bool FAR, class T, int WK, int RW>
+__global__ void __launch_bounds__(kPipeBlock / RW)
     kb_bwd_pipe(BwdK<T> W, const int32_t* __restrict__ Upos, const int32_t* __restrict__ lev_ptr,
                 const int32_t* __restrict__ meta, const int32_t* __restrict__ case_lev,
                 const KS* ks) {
@@ -321,22 +327,32 @@ __global__ void __launch_bounds__(kPipeBlock)
   const int k = blockIdx.x;
   if (ks && stopped(ks + k)) return;
   const int u0 = case_lev[k];
-  double s0 = 0.0, s1 = 0.0;
-  bwd_pipe<DOT, FAR, T, WK, pipe_depth<WK>()>(W, Upos, lev_ptr, meta, u0, case_lev[k + 1] - u0, false,
-                                         L, s0, s1);
-  if (DOT != 0) store_partial<kPipeBlock>(s0, s1, W.partial, k);
+  double s0[RW], s1[RW];
+#pragma unroll
+  for (int j = 0; j < RW; ++j) s0[j] = s1[j] = 0.0;
+  bwd_pipe<DOT, FAR, T, WK, pipe_depth<WK, RW>(), RW>(W, Upos, lev_ptr, meta, u0,
+                                                   case_lev[k + 1] - u0, false, L, s0, s1);
+  if (DOT != 0) store_partial_rows<kPipeBlock, RW>(s0, s1, W.partial, k);
 }
 
-template <bool FAR, int WK>
-__global__ void __launch_bounds__(kPipeBlock)
+template <bool FAR, int WK, int RW>
+__global__ void __launch_bounds__(kPipeBlock / RW)
     kb_dilu_diag_pipe(double* D, const double* __restrict__ prod, const int32_t* __restrict__ Lcol,
                       const int32_t* __restrict__ lev_ptr, const int32_t* __restrict__ meta,
                       const int32_t* __restrict__ case_lev) {
   __shared__ PipeLDS<double> L;
   const int k = blockIdx.x;
   const int u0 = case_lev[k];
-  diag_pipe<FAR, WK, pipe_depth<WK>()>(D, prod, Lcol, lev_ptr, meta, u0, case_lev[k + 1] - u0,
-                                       false, L);
+  diag_pipe<FAR, WK, pipe_depth<WK, RW>(), RW>(D, prod, Lcol, lev_ptr, meta, u0,
+                                               case_lev[k + 1] - u0, false, L);
+}
+
+// Host: calls f(RW) for a rows-per-thread count of 1, 2 or 4.
+template <class F>
+inline void with_pipe_rows(int rw, F&& f) {
+  if (rw == 4) f(std::integral_constant<int, 4>{});
+  else if (rw == 2) f(std::integral_constant<int, 2>{});
+  else f(std::integral_constant<int, 1>{});
 }
 
 // update_dilu_diagonal! per case: one workgroup walks the case's levels
@@ -532,33 +548,39 @@ static void fin_cases(const BSolver<T>& S, int fin) {
 }
 
 template <bool UPD, class T>
-static void bsweep_fwd(const BSolver<T>& S, const FwdK<T>& W, const KS* ks) {
+static void bsweep_fwd(const BSolver<T>& S, const FwdK<T>& W, const KS* ks, int rw = 0) {
   const xcal_mesh* m = S.m;
+  if (rw <= 0) rw = m->pipe_rw_f > 0 ? m->pipe_rw_f : 1;
   auto pipe = [&](auto kern) {
-    hipLaunchKernelGGL(kern, dim3(S.ncase), dim3(kPipeBlock), 0, S.s, W, S.Lrd, m->lev_ptr_f_d.p,
-                       m->lmeta_f_d.p, m->case_lev_f_d.p, ks);
+    hipLaunchKernelGGL(kern, dim3(S.ncase), dim3(kPipeBlock / rw), 0, S.s, W, S.Lrd,
+                       m->lev_ptr_f_d.p, m->lmeta_f_d.p, m->case_lev_f_d.p, ks);
   };
   if (UPD && m->pipe_wf > 0)
     hipLaunchKernelGGL(kb_cg_xr<T>, dim3(S.nblk), dim3(kBlock), 0, S.s, S.B, W.x, W.p, W.r, W.q,
                        ks);
   if (m->pipe_wf > 0)
     with_pipe_form(m->pipe_wf, m->pipe_far_f, [&](auto WK, auto FAR) {
-      pipe(kb_fwd_pipe<UPD, decltype(FAR)::value, T, decltype(WK)::value>);
+      with_pipe_rows(rw, [&](auto RW) {
+        pipe(kb_fwd_pipe<UPD, decltype(FAR)::value, T, decltype(WK)::value, decltype(RW)::value>);
+      });
     });
   else
     hipLaunchKernelGGL((kb_fwd<UPD, T>), dim3(S.ncase), dim3(kRunBlock), 0, S.s, W,
                        m->lev_ptr_f_d.p, m->lmeta_f_d.p, m->case_lev_f_d.p, ks);
 }
 template <int DOT, class T>
-static void bsweep_bwd(const BSolver<T>& S, const BwdK<T>& W, const KS* ks) {
+static void bsweep_bwd(const BSolver<T>& S, const BwdK<T>& W, const KS* ks, int rw = 0) {
   const xcal_mesh* m = S.m;
+  if (rw <= 0) rw = m->pipe_rw_b > 0 ? m->pipe_rw_b : 1;
   auto pipe = [&](auto kern) {
-    hipLaunchKernelGGL(kern, dim3(S.ncase), dim3(kPipeBlock), 0, S.s, W, m->Upos.p,
+    hipLaunchKernelGGL(kern, dim3(S.ncase), dim3(kPipeBlock / rw), 0, S.s, W, m->Upos.p,
                        m->lev_ptr_b_d.p, m->lmeta_b_d.p, m->case_lev_b_d.p, ks);
   };
   if (m->pipe_wb > 0)
     with_pipe_form(m->pipe_wb, m->pipe_far_b, [&](auto WK, auto FAR) {
-      pipe(kb_bwd_pipe<DOT, decltype(FAR)::value, T, decltype(WK)::value>);
+      with_pipe_rows(rw, [&](auto RW) {
+        pipe(kb_bwd_pipe<DOT, decltype(FAR)::value, T, decltype(WK)::value, decltype(RW)::value>);
+      });
     });
   else
     hipLaunchKernelGGL((kb_bwd<DOT, T>), dim3(S.ncase), dim3(kRunBlock), 0, S.s, W,
@@ -602,12 +624,63 @@ static bool bread(BSolver<T>& S) {
 
 // iterations in batches until every case has stopped; the first batch is
 // sized from the previous solve (longest case)
+// Launch signature of a batched solve's iteration: every pointer and size
+// its kernels take, with the solver kind and value type.  Two solves with the
+// same signature launch the same kernels with the same arguments.
 template <class T>
-static void brun(BSolver<T>& S, int tail, const std::function<void()>& iteration) {
+static std::vector<uint64_t> graph_key(const BSolver<T>& S, int kind, int precon, const void* b) {
+  auto u = [](const void* p) { return (uint64_t)(uintptr_t)p; };
+  const xcal_mesh* m = S.m;
+  return {(uint64_t)kind, sizeof(T), (uint64_t)precon, (uint64_t)S.n, (uint64_t)S.ncase,
+          (uint64_t)S.nblk, u(S.s), u(S.work), u(S.ks), u(S.partial), u(S.A.nz), u(S.A.colval),
+          u(S.A.rb), u(S.Sj), u(S.rD), u(S.Lnz), u(S.Unz), u(S.Db), u(S.Lrd), u(b),
+          (uint64_t)m->pipe_rw_f, (uint64_t)m->pipe_rw_b, (uint64_t)m->pipe_wf,
+          (uint64_t)m->pipe_wb};
+}
+
+// Batched iterations are captured in HIP graphs of two iterations (the
+// iteration's ping-pong buffers return to their roles after two) unless
+// XCAL_GRAPH=0 or kernel timing is on.  A graph holds kernel launches only;
+// every kernel reads the cases' Krylov state from device memory, so a replay
+// does what the eager launches would.
+static bool graphs_enabled(const xcal_ctx* c) {
+  const char* e = std::getenv("XCAL_GRAPH");
+  if (e && *e == '0') return false;
+  for (int k = 0; k < xcal_ctx::kProfKernels; ++k)
+    if (c->prof_on[k]) return false;
+  return true;
+}
+
+// Runs iterations until every case has stopped: a first batch sized from the
+// last solve (less a tenth), then `tail` at a time, reading the stop flags in
+// between.  With a graph key, pairs of iterations are replayed from the
+// equation's graph for the current parity (captured on first use).
+template <class T>
+static void brun(BSolver<T>& S, int tail, const std::function<void()>& iteration,
+                 const std::vector<uint64_t>* gkey = nullptr, const int* parity = nullptr) {
   if (bread(S)) return;
   int batch = *S.pred > 0 ? std::max(1, (*S.pred * 9) / 10) : 8;
+  auto run = [&](int count) {
+    int done = 0;
+    if (gkey && parity)
+      for (; done + 2 <= count; done += 2) {
+        hipGraphExec_t& g = S.e->graphs[*gkey][*parity];
+        if (!g) {
+          hipGraph_t graph = nullptr;
+          XCAL_HIP(hipStreamBeginCapture(S.s, hipStreamCaptureModeThreadLocal));
+          iteration();
+          iteration();
+          XCAL_HIP(hipStreamEndCapture(S.s, &graph));
+          const hipError_t rc = hipGraphInstantiate(&g, graph, nullptr, nullptr, 0);
+          (void)hipGraphDestroy(graph);
+          XCAL_HIP(rc);
+        }
+        XCAL_HIP(hipGraphLaunch(g, S.s));
+      }
+    for (; done < count; ++done) iteration();
+  };
   for (;;) {
-    for (int b = 0; b < batch; ++b) iteration();
+    run(batch);
     XCAL_HIP(hipGetLastError());
     if (bread(S)) break;
     batch = tail;
@@ -616,6 +689,88 @@ static void brun(BSolver<T>& S, int tail, const std::function<void()>& iteration
   int mx = 0;
   for (int k = 0; k < S.ncase; ++k) mx = std::max(mx, h[k].iter);
   *S.pred = mx;
+}
+
+// ------------------------------------------------- rows-per-thread tuner
+// The pipelined batch sweeps run the same rows, values and sums whatever
+// their rows per thread (RW), so RW is picked by time alone: on the first
+// DILU solve on a batch mesh each direction is timed at RW = 1, 2 and 4 (one
+// warm launch, then kTuneReps launches between HIP events, every case
+// swept), and the fastest is kept for the mesh.  Meshes of the same shape
+// (cells, entries, cases, levels, sweep form) reuse the choice and the
+// timings.  XCAL_PIPE_ROWS=1|2|4 fixes RW instead.
+constexpr int kTuneReps = 3;
+constexpr int kTuneRows[3] = {1, 2, 4};
+
+using PipeTune = xcal_mesh::PipeTuning;
+
+static std::mutex g_tune_mu;
+static std::map<std::vector<int64_t>, PipeTune> g_tune;
+
+static std::vector<int64_t> tune_key(const xcal_mesh* m, int bytes) {
+  return {m->n, m->nnz, m->ncase, (int64_t)m->lev_ptr_f.size(), (int64_t)m->lev_ptr_b.size(),
+          m->pipe_wf, m->pipe_wb, m->pipe_far_f, m->pipe_far_b, bytes};
+}
+
+template <class T>
+static void tune_pipes(const BSolver<T>& S, const T* u, T* Y, T* Yb, T* z) {
+  const xcal_mesh* m = S.m;
+  if (m->pipe_rw_f > 0 || m->pipe_wf <= 0 || m->pipe_wb <= 0) return;
+  const char* env = std::getenv("XCAL_PIPE_ROWS");
+  if (env && (*env == '1' || *env == '2' || *env == '4') && env[1] == 0) {
+    m->pipe_rw_f = m->pipe_rw_b = *env - '0';
+    return;
+  }
+  const auto key = tune_key(m, (int)sizeof(T));
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune.find(key);
+    if (it != g_tune.end()) {
+      m->pipe_tune = it->second;
+      m->pipe_rw_f = it->second.rw_f;
+      m->pipe_rw_b = it->second.rw_b;
+      return;
+    }
+  }
+  struct Ev {
+    hipEvent_t a = nullptr, b = nullptr;
+    ~Ev() {
+      if (a) (void)hipEventDestroy(a);
+      if (b) (void)hipEventDestroy(b);
+    }
+  } ev;
+  XCAL_HIP(hipEventCreate(&ev.a));
+  XCAL_HIP(hipEventCreate(&ev.b));
+  PipeTune t;
+  for (int dir = 0; dir < 2; ++dir) {
+    double best = 1e300;
+    for (int c = 0; c < 3; ++c) {
+      const int rw = kTuneRows[c];
+      auto launch = [&] {
+        if (dir == 0) bsweep_fwd<false>(S, bfwdk(S, u, Y, Yb), nullptr, rw);
+        else bsweep_bwd<1>(S, bbwdk(S, (const T*)Yb, z, u), nullptr, rw);
+      };
+      launch();
+      XCAL_HIP(hipEventRecord(ev.a, S.s));
+      for (int r = 0; r < kTuneReps; ++r) launch();
+      XCAL_HIP(hipEventRecord(ev.b, S.s));
+      XCAL_HIP(hipEventSynchronize(ev.b));
+      float ms = 0.f;
+      XCAL_HIP(hipEventElapsedTime(&ms, ev.a, ev.b));
+      const double us = 1e3 * ms / kTuneReps;
+      t.us[dir][c] = us;
+      if (us < best) {
+        best = us;
+        (dir == 0 ? t.rw_f : t.rw_b) = rw;
+      }
+    }
+  }
+  XCAL_HIP(hipGetLastError());
+  m->pipe_tune = t;
+  m->pipe_rw_f = t.rw_f;
+  m->pipe_rw_b = t.rw_b;
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tune.emplace(key, t);
 }
 
 // x0 == NULL: zero start, solution left in *xsol
@@ -633,6 +788,7 @@ static void bcg(BSolver<T>& S, int precon, const T* b, T* x0, int itmax, double 
   const T* Sd = precon == XCAL_PRECON_JACOBI ? S.Sj : nullptr;
   const KS* ks = S.ks;
   if (dilu) {
+    tune_pipes(S, (const T*)r, Y, Yb, zd);
     bsweep_fwd<false>(S, bfwdk(S, (const T*)r, Y, Yb), ks);
     bsweep_bwd<1>(S, bbwdk(S, (const T*)Yb, zd, (const T*)r), ks);
     fin_cases(S, FIN_CG_INIT);
@@ -671,7 +827,8 @@ static void bcg(BSolver<T>& S, int precon, const T* b, T* x0, int itmax, double 
       fin_blocks(S, FIN_CG_ITER);
     }
   };
-  brun(S, 4, iteration);
+  const auto key = graph_key(S, 0, precon, nullptr);
+  brun(S, 4, iteration, graphs_enabled(c) ? &key : nullptr, &parity);
   if (x0) hipLaunchKernelGGL(kb_add, dim3(grid_for(n)), dim3(kBlock), 0, S.s, n, (const double*)x,
                              (double*)x0);
   if (xsol) *xsol = x;
@@ -692,6 +849,7 @@ static void bbicg(BSolver<T>& S, int precon, const T* b, T* x0, int itmax, doubl
   const T* Sd = precon == XCAL_PRECON_JACOBI ? S.Sj : nullptr;
   const KS* ks = S.ks;
   if (dilu) {
+    tune_pipes(S, (const T*)r, Y, Yb, t);
     bsweep_fwd<false>(S, bfwdk(S, (const T*)r, Y, Yb), ks);
     bsweep_bwd<4>(S, bbwdk(S, (const T*)Yb, r, b), ks);
     fin_cases(S, FIN_BI_INIT);
@@ -745,7 +903,8 @@ static void bbicg(BSolver<T>& S, int precon, const T* b, T* x0, int itmax, doubl
                        (const T*)t, (const T*)v, x, r, pnew, b, S.partial, ks);
     fin_blocks(S, FIN_BI_END);
   };
-  brun(S, 2, iteration);
+  const auto key = graph_key(S, 1, precon, b);
+  brun(S, 2, iteration, graphs_enabled(S.m->ctx) ? &key : nullptr, &parity);
   if (x0) hipLaunchKernelGGL(kb_add, dim3(grid_for(n)), dim3(kBlock), 0, S.s, n, (const double*)x,
                              (double*)x0);
   if (xsol) *xsol = x;
@@ -865,13 +1024,16 @@ void precond_update_batch(xcal_eqn* e) {
     if (e->Lprod.n != nl) e->Lprod.alloc(nl);
     hipLaunchKernelGGL(k_lower_products, dim3(grid_for(nl)), dim3(kBlock), 0, s, nl, m->Lslot.p,
                        m->trans.p, e->nzval.p, e->Lprod.p);
+    const int rw = m->pipe_rw_f > 0 ? m->pipe_rw_f : 1;  // tuned on the first solve
     auto pipe = [&](auto kern) {
-      hipLaunchKernelGGL(kern, dim3(m->ncase), dim3(kPipeBlock), 0, s, e->pD.p,
+      hipLaunchKernelGGL(kern, dim3(m->ncase), dim3(kPipeBlock / rw), 0, s, e->pD.p,
                          (const double*)e->Lprod.p, m->Lcol.p, m->lev_ptr_f_d.p, m->lmeta_f_d.p,
                          m->case_lev_f_d.p);
     };
     with_pipe_form(m->pipe_wf, m->pipe_far_f, [&](auto WK, auto FAR) {
-      pipe(kb_dilu_diag_pipe<decltype(FAR)::value, decltype(WK)::value>);
+      with_pipe_rows(rw, [&](auto RW) {
+        pipe(kb_dilu_diag_pipe<decltype(FAR)::value, decltype(WK)::value, decltype(RW)::value>);
+      });
     });
     return;
   }

@@ -1363,6 +1363,17 @@ int xcal_mesh_sweep_far(const xcal_mesh* m, int32_t* fwd, int32_t* bwd) {
   });
 }
 
+int xcal_mesh_pipe_tuning(const xcal_mesh* m, int32_t* fwd_rows, int32_t* bwd_rows, double* us) {
+  return guarded([&] {
+    XCAL_CHECK(m && fwd_rows && bwd_rows && us, "NULL argument");
+    *fwd_rows = m->pipe_rw_f;
+    *bwd_rows = m->pipe_rw_b;
+    for (int d = 0; d < 2; ++d)
+      for (int c = 0; c < 3; ++c) us[3 * d + c] = m->pipe_tune.us[d][c];
+    return 0;
+  });
+}
+
 int xcal_mesh_sweep_form(const xcal_mesh* m, int32_t* fwd, int32_t* bwd) {
   return guarded([&] {
     XCAL_CHECK(m && fwd && bwd, "NULL argument");

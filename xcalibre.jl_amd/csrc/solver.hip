@@ -459,10 +459,10 @@ __global__ void __launch_bounds__(kPipeBlock)
                    const int32_t* __restrict__ meta, int lev0, int lev1, int slot) {
   __shared__ PipeLDS<T> L;
   if (stopped(W.ks)) return;
-  double s0 = 0.0, s1 = 0.0;
+  double s0[1] = {0.0}, s1[1] = {0.0};
   bwd_pipe<DOT, FAR, T, WK, pipe_depth<WK>()>(W, Upos, lev_ptr, meta, lev0, lev1 - lev0, true, L, s0,
                                          s1);
-  if (DOT != 0) store_partial<kPipeBlock>(s0, s1, W.partial, slot);
+  if (DOT != 0) store_partial<kPipeBlock>(s0[0], s1[0], W.partial, slot);
 }
 
 // Cone form (sweep_cone.h): one workgroup per block, grid = the mesh's

@@ -179,6 +179,40 @@ __device__ inline void store_partial(double s0, double s1, double* partial, int 
 }
 
 
+// store_partial when each thread holds RW row slots, slot j of thread t
+// standing for thread t + j blockDim.x of a workgroup RW times as wide: the
+// same wavefront sums, added in the same wavefront order, so the partials are
+// those of that wider workgroup bit for bit.
+template <int RW>
+__device__ inline double block_sum_rows(const double (&v)[RW], double* lds) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int j = 0; j < RW; ++j) {
+    double x = v[j];
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
+    if (lane == 0) lds[(j * (int)blockDim.x + (int)threadIdx.x) >> 6] = x;
+  }
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0) {
+    const int nw = (RW * (int)blockDim.x + 63) >> 6;
+    for (int w = 0; w < nw; ++w) s += lds[w];
+  }
+  __syncthreads();
+  return s;
+}
+template <int NV, int RW>
+__device__ inline void store_partial_rows(const double (&s0)[RW], const double (&s1)[RW],
+                                          double* partial, int slot) {
+  __shared__ double lds[NV / 64];
+  const double t0 = block_sum_rows<RW>(s0, lds);
+  const double t1 = block_sum_rows<RW>(s1, lds);
+  if (threadIdx.x == 0) {
+    partial[2 * slot] = t0;
+    partial[2 * slot + 1] = t1;
+  }
+}
+
 __device__ inline double dd(double a, double b) { return a * b; }  // products of dots in double
 
 // ------------------------------------------------------------ DILU sweeps

@@ -39,6 +39,17 @@ struct PipeLDS {
   int32_t r0[kPipeLev + 1], base[kPipeLev], w[kPipeLev];
 };
 
+// A thread's records of one level when each thread works RW rows of it
+// (kPipeBlock / RW threads): row slot j of thread t is the level's row
+// t + j kPipeBlock / RW, the row thread t + j kPipeBlock / RW would have in
+// a workgroup of kPipeBlock threads.  Rows, values and summation orders are
+// the same for every RW; fewer threads take fewer barrier round trips per
+// level but issue more per thread.
+template <class Rec, int RW>
+struct PipeRows {
+  Rec r[RW];
+};
+
 // Stages levels [u0, u0 + nl) (row starts, packed bases, entry counts).
 __device__ inline void stage_levels(const int32_t* __restrict__ lev_ptr,
                                     const int32_t* __restrict__ meta, int u0, int nl,
@@ -107,19 +118,20 @@ __device__ inline void preload_ring(T* ring, int32_t first, V&& value) {
 
 // Forward sweep y = (I + L D^-1)^-1 u over levels [u0, u0 + nl): y to the
 // ring, to Yb (backward order) and, with STOREY, to Y (for later launches).
-template <bool STOREY, bool FAR, class T, int WK, int P>
+template <bool STOREY, bool FAR, class T, int WK, int P, int RW = 1>
 __device__ inline void fwd_pipe(const FwdK<T>& W, const T* __restrict__ uin,
                                 const T* __restrict__ Lrd, const int32_t* __restrict__ lev_ptr,
                                 const int32_t* __restrict__ meta, int u0, int nl, bool preload,
                                 PipeLDS<T>& L) {
+  constexpr int NT = kPipeBlock / RW;
   stage_levels(lev_ptr, meta, u0, nl, L.r0, L.base, L.w);
   const int32_t first = L.r0[0];
   if (preload) preload_ring(L.ring, first, [&](int32_t p) { return W.Y[p]; });
   const int32_t ring_lo = preload ? first - kPipeRing : first;
-  auto fetch = [&](FRec<T, WK>& R, int l) {
+  auto fetch1 = [&](FRec<T, WK>& R, int l, int32_t vt) {
     const int lc = l < nl ? l : nl - 1;
     const int32_t r0 = L.r0[lc], cnt = L.r0[lc + 1] - r0;
-    const int32_t q = (int32_t)threadIdx.x < cnt ? (int32_t)threadIdx.x : cnt - 1;
+    const int32_t q = vt < cnt ? vt : cnt - 1;
     const int w = L.w[lc];
     const int64_t e0 = (int64_t)L.base[lc] + q;
     const int32_t i = r0 + q;
@@ -136,13 +148,13 @@ __device__ inline void fwd_pipe(const FwdK<T>& W, const T* __restrict__ uin,
       R.rv[t] = Lrd[e];
     }
   };
-  auto gather = [&](FRec<T, WK>& R) {
+  auto gather1 = [&](FRec<T, WK>& R) {
     if (FAR) {
 #pragma unroll
       for (int t = 0; t < WK; ++t) R.yg[t] = W.Y[R.col[t] >= 0 ? R.col[t] : 0];
     }
   };
-  auto process = [&](const FRec<T, WK>& R) {
+  auto process1 = [&](const FRec<T, WK>& R) {
     T sum = 0;
 #pragma unroll
     for (int t = 0; t < WK; ++t) {
@@ -157,7 +169,20 @@ __device__ inline void fwd_pipe(const FwdK<T>& W, const T* __restrict__ uin,
     if (STOREY) W.Y[R.i] = y;
     W.Yb[R.pb] = y;
   };
-  FRec<T, WK> buf[P];
+  using Rec = PipeRows<FRec<T, WK>, RW>;
+  auto fetch = [&](Rec& R, int l) {
+#pragma unroll
+    for (int j = 0; j < RW; ++j) fetch1(R.r[j], l, (int32_t)threadIdx.x + j * NT);
+  };
+  auto gather = [&](Rec& R) {
+#pragma unroll
+    for (int j = 0; j < RW; ++j) gather1(R.r[j]);
+  };
+  auto process = [&](const Rec& R) {
+#pragma unroll
+    for (int j = 0; j < RW; ++j) process1(R.r[j]);
+  };
+  Rec buf[P];
   pipe_levels<P>(nl, buf, fetch, gather, process);
 }
 
@@ -174,20 +199,21 @@ struct BRec {
 // up to two dot products (DOT as bwd_row) into s0, s1.  The dot operand
 // dv[i] is gathered with the record's dependencies, Q levels before its
 // row is processed (its row index arrives with the record, P levels ahead).
-template <int DOT, bool FAR, class T, int WK, int P>
+template <int DOT, bool FAR, class T, int WK, int P, int RW = 1>
 __device__ inline void bwd_pipe(const BwdK<T>& W, const int32_t* __restrict__ Upos,
                                 const int32_t* __restrict__ lev_ptr,
                                 const int32_t* __restrict__ meta, int u0, int nl, bool preload,
-                                PipeLDS<T>& L, double& s0, double& s1) {
+                                PipeLDS<T>& L, double (&s0)[RW], double (&s1)[RW]) {
+  constexpr int NT = kPipeBlock / RW;
   stage_levels(lev_ptr, meta, u0, nl, L.r0, L.base, L.w);
   const int32_t first = L.r0[0];
   if (preload) preload_ring(L.ring, first, [&](int32_t p) { return W.Z[W.rows[p]]; });
   const int32_t ring_lo = preload ? first - kPipeRing : first;
-  auto fetch = [&](BRec<T, WK>& R, int l) {
+  auto fetch1 = [&](BRec<T, WK>& R, int l, int32_t vt) {
     const int lc = l < nl ? l : nl - 1;
     const int32_t p0 = L.r0[lc], cnt = L.r0[lc + 1] - p0;
-    R.own = l < nl && (int32_t)threadIdx.x < cnt;
-    const int32_t q = (int32_t)threadIdx.x < cnt ? (int32_t)threadIdx.x : cnt - 1;
+    R.own = l < nl && vt < cnt;
+    const int32_t q = vt < cnt ? vt : cnt - 1;
     const int w = L.w[lc];
     const int64_t e0 = (int64_t)L.base[lc] + q;
     const int32_t pos = p0 + q;
@@ -205,14 +231,14 @@ __device__ inline void bwd_pipe(const BwdK<T>& W, const int32_t* __restrict__ Up
       R.av[t] = W.Unz[e];
     }
   };
-  auto gather = [&](BRec<T, WK>& R) {
+  auto gather1 = [&](BRec<T, WK>& R) {
     if (DOT != 0) R.dvi = W.dv[R.i];
     if (FAR) {
 #pragma unroll
       for (int t = 0; t < WK; ++t) R.zg[t] = W.Z[R.up[t] >= 0 ? R.col[t] : 0];
     }
   };
-  auto process = [&](const BRec<T, WK>& R) {
+  auto process1 = [&](const BRec<T, WK>& R, double& s0, double& s1) {
     T sum = 0;
 #pragma unroll
     for (int t = 0; t < WK; ++t) {
@@ -233,7 +259,20 @@ __device__ inline void bwd_pipe(const BwdK<T>& W, const int32_t* __restrict__ Up
     s0 = R.own ? s0 + a0 : s0;
     s1 = R.own ? s1 + a1 : s1;
   };
-  BRec<T, WK> buf[P];
+  using Rec = PipeRows<BRec<T, WK>, RW>;
+  auto fetch = [&](Rec& R, int l) {
+#pragma unroll
+    for (int j = 0; j < RW; ++j) fetch1(R.r[j], l, (int32_t)threadIdx.x + j * NT);
+  };
+  auto gather = [&](Rec& R) {
+#pragma unroll
+    for (int j = 0; j < RW; ++j) gather1(R.r[j]);
+  };
+  auto process = [&](const Rec& R) {
+#pragma unroll
+    for (int j = 0; j < RW; ++j) process1(R.r[j], s0[j], s1[j]);
+  };
+  Rec buf[P];
   pipe_levels<P>(nl, buf, fetch, gather, process);
 }
 
@@ -241,7 +280,7 @@ __device__ inline void bwd_pipe(const BwdK<T>& W, const int32_t* __restrict__ Up
 // [u0, u0 + nl): D_j starts from a_jj and takes D_j -= (a_ij a_ji) / D_i over
 // its lower entries in column order; prod holds a_ij a_ji per packed entry,
 // D_i comes from the ring (or, with FAR, from D itself).
-template <bool FAR, int WK, int P>
+template <bool FAR, int WK, int P, int RW = 1>
 __device__ inline void diag_pipe(double* D, const double* __restrict__ prod,
                                  const int32_t* __restrict__ Lcol,
                                  const int32_t* __restrict__ lev_ptr,
@@ -257,10 +296,10 @@ __device__ inline void diag_pipe(double* D, const double* __restrict__ prod,
     int32_t col[WK];
     double pr[WK], dg[WK];
   };
-  auto fetch = [&](Rec& R, int l) {
+  auto fetch1 = [&](Rec& R, int l, int32_t vt) {
     const int lc = l < nl ? l : nl - 1;
     const int32_t r0 = L.r0[lc], cnt = L.r0[lc + 1] - r0;
-    const int32_t q = (int32_t)threadIdx.x < cnt ? (int32_t)threadIdx.x : cnt - 1;
+    const int32_t q = vt < cnt ? vt : cnt - 1;
     const int w = L.w[lc];
     const int64_t e0 = (int64_t)L.base[lc] + q;
     R.j = r0 + q;
@@ -274,13 +313,13 @@ __device__ inline void diag_pipe(double* D, const double* __restrict__ prod,
       R.pr[t] = prod[e];
     }
   };
-  auto gather = [&](Rec& R) {
+  auto gather1 = [&](Rec& R) {
     if (FAR) {
 #pragma unroll
       for (int t = 0; t < WK; ++t) R.dg[t] = D[R.col[t] >= 0 ? R.col[t] : 0];
     }
   };
-  auto process = [&](const Rec& R) {
+  auto process1 = [&](const Rec& R) {
     double Dj = R.d0;
 #pragma unroll
     for (int t = 0; t < WK; ++t) {
@@ -293,7 +332,21 @@ __device__ inline void diag_pipe(double* D, const double* __restrict__ prod,
     L.ring[R.j & (kPipeRing - 1)] = Dj;
     D[R.j] = Dj;
   };
-  Rec buf[P];
+  constexpr int NT = kPipeBlock / RW;
+  using Recs = PipeRows<Rec, RW>;
+  auto fetch = [&](Recs& R, int l) {
+#pragma unroll
+    for (int j = 0; j < RW; ++j) fetch1(R.r[j], l, (int32_t)threadIdx.x + j * NT);
+  };
+  auto gather = [&](Recs& R) {
+#pragma unroll
+    for (int j = 0; j < RW; ++j) gather1(R.r[j]);
+  };
+  auto process = [&](const Recs& R) {
+#pragma unroll
+    for (int j = 0; j < RW; ++j) process1(R.r[j]);
+  };
+  Recs buf[P];
   pipe_levels<P>(nl, buf, fetch, gather, process);
 }
 
@@ -327,9 +380,10 @@ inline void with_pipe_form(int wk, bool far, F&& f) {
 
 // Prefetch depth by entry bound: as deep as the 63 outstanding loads of a
 // wavefront allow, with whole groups of levels per loop iteration.
-template <int WK>
+template <int WK, int RW = 1>
 constexpr int pipe_depth() {
-  return WK <= 2 ? 8 : WK <= 4 ? 6 : 4;
+  constexpr int d = WK <= 2 ? 8 : WK <= 4 ? 6 : 4;  // levels ahead at one row per thread
+  return d / RW >= 2 ? d / RW : 2;                 // about as many records in flight
 }
 
 }  // namespace xcal

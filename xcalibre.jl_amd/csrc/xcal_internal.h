@@ -7,6 +7,8 @@
 #include <stdexcept>
 #include <memory>
 #include <string>
+#include <map>
+#include <array>
 #include <vector>
 
 #include "xcal.h"
@@ -215,6 +217,15 @@ struct xcal_mesh {
   // sweeps (forward, backward), or -1 when a case exceeds their limits.
   int pipe_wf = -1, pipe_wb = -1;
   bool pipe_far_f = false, pipe_far_b = false;  // some dependency before the ring
+  // rows of a level per thread of the pipelined batch sweeps (1, 2 or 4):
+  // set by the tuner on the first DILU solve (or XCAL_PIPE_ROWS), -1 untuned;
+  // the tuner's timings (µs per launch, [forward, backward][RW 1, 2, 4])
+  mutable int pipe_rw_f = -1, pipe_rw_b = -1;
+  struct PipeTuning {
+    int rw_f = 1, rw_b = 1;
+    double us[2][3] = {};
+  };
+  mutable PipeTuning pipe_tune;
   // Single meshes: some forward run is pipelined (needs Lrd and Lprod)
   bool pipe_runs = false;
   std::vector<int32_t> xblk;
@@ -307,7 +318,14 @@ struct xcal_eqn {
   xcal::DBuf<int32_t> case_active;
   std::vector<double> case_host;
   int ir_pred[2][16] = {};  // FP32 inner solves: iterations of the last solve per sweep index
+  // batched solves: instantiated HIP graphs of two Krylov iterations, by the
+  // solve's launch signature (its kernels' arguments) and the iteration
+  // parity at the graph's start
+  std::map<std::vector<uint64_t>, std::array<hipGraphExec_t, 2>> graphs;
   ~xcal_eqn() {
+    for (auto& g : graphs)
+      for (hipGraphExec_t x : g.second)
+        if (x) (void)hipGraphExecDestroy(x);
     if (kstate_host) (void)hipHostFree(kstate_host);
   }
 };

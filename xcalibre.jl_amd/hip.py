@@ -52,6 +52,7 @@ def lib():
             "xcal_mesh_cases": [_vp, _pi32, C.POINTER(C.c_int64)],
             "xcal_mesh_sweep_form": [_vp, _pi32, _pi32],
             "xcal_mesh_sweep_far": [_vp, _pi32, _pi32],
+            "xcal_mesh_pipe_tuning": [_vp, _pi32, _pi32, _pd],
             "xcal_mesh_cone_form": [_vp, _pi32, _pi32, C.POINTER(C.c_int64), C.POINTER(C.c_int64)],
             "xcal_solve_system_batch": [_vp, C.POINTER(A.SolverSetupC), _vp, C.c_int32,
                                         C.POINTER(A.SolveStats)],
@@ -248,6 +249,16 @@ class DeviceMesh:
         f, b = C.c_int32(), C.c_int32()
         check(lib().xcal_mesh_sweep_far(self.h, C.byref(f), C.byref(b)))
         return bool(f.value), bool(b.value)
+
+    def pipe_tuning(self):
+        """(forward rows per thread, backward rows per thread, {(direction,
+        rows): µs per launch}) of the pipelined batch sweeps as tuned."""
+        f, b = C.c_int32(), C.c_int32()
+        us = np.zeros(6)
+        check(lib().xcal_mesh_pipe_tuning(self.h, C.byref(f), C.byref(b), _pd_of(us)))
+        t = {(d, r): float(us[3 * i + j]) for i, d in enumerate(("fwd", "bwd"))
+             for j, r in enumerate((1, 2, 4))}
+        return f.value, b.value, t
 
     def cone_form(self):
         """(rows per block, blocks, forward cone rows, backward cone rows) of

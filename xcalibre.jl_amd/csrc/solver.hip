@@ -42,7 +42,7 @@ __global__ void __launch_bounds__(kBlock)
   KS k;
   if (threadIdx.x == 0) k = *ks;  // loaded while the partials load
   double t0, t1;
-  reduce_partials(partial, count, t0, t1, lds);
+  reduce_partials<16>(partial, count, t0, t1, lds);
   if (threadIdx.x == 0 && k.stop == STOP_NONE) {
     finalize(&k, fin, t0, t1);
     *ks = k;
@@ -1090,11 +1090,15 @@ static int ir_solve(xcal_eqn* e, const xcal_solver_setup* st, int itmax, double*
     S32.cg_batch = &e->ir_pred[0][std::min(nouter, 15)];
     S32.bi_batch = &e->ir_pred[1][std::min(nouter, 15)];
     const double ia = single ? st->atol : 0.0;
+    // a sweep reduces its residual by inner_rtol, or only by what the outer
+    // tolerance still asks (with a factor 2 of margin for the FP32
+    // recurrence's drift from the true residual): the last sweep stops
+    // there instead of overshooting by decades
+    const double rt = single ? inner_rtol : std::max(inner_rtol, std::min(0.5 * tol / res, 0.5));
     if (st->solver == XCAL_SOLVER_CG)
-      cg(S32, st->precon, (const float*)r32, (float*)nullptr, itmax - niter, ia, inner_rtol, ni, &d);
+      cg(S32, st->precon, (const float*)r32, (float*)nullptr, itmax - niter, ia, rt, ni, &d);
     else
-      bicgstab(S32, st->precon, (const float*)r32, (float*)nullptr, itmax - niter, ia, inner_rtol,
-               ni, &d);
+      bicgstab(S32, st->precon, (const float*)r32, (float*)nullptr, itmax - niter, ia, rt, ni, &d);
     niter += ni;
     hipLaunchKernelGGL(k_ir_update, dim3(S.grid), dim3(kBlock), 0, S.s, S.n, (const float*)d, x);
     prev = res;

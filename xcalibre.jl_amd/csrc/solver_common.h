@@ -109,9 +109,12 @@ __device__ inline bool stopped(const KS* ks) { return ks->stop != STOP_NONE; }
 // Sum of `count` partial pairs in slot order, the same in every kernel that
 // reduces them: thread t < kBlock adds slots t, t + kBlock, ... in turn,
 // then block_sum; every thread receives both sums.  lds: >= kBlock / 64.
+// U slots per thread are loaded together, then added in slot order: U does
+// not change the sums, only how many round trips a long list takes (the
+// one-workgroup finalize takes 16, one round trip for up to 4096 slots).
+template <int U = 8>
 __device__ inline void reduce_partials(const double* __restrict__ partial, int count, double& t0,
                                        double& t1, double* lds) {
-  constexpr int U = 8;  // slots loaded together, then added in slot order
   double a = 0.0, b = 0.0;
   if ((int)threadIdx.x < kBlock)
     for (int s0 = threadIdx.x; s0 < count; s0 += U * kBlock) {

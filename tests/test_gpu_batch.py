@@ -291,7 +291,7 @@ def test_batch_graph_replay_equals_eager(x, A, ctx, monkeypatch):
     """Batched Krylov iterations replayed from HIP graphs (two iterations per
     graph, captured on first use and replayed on every later solve, with
     assembly and host reads of the stop flags between replays) give the same
-    SIMPLE run bit for bit as eager launches (XCAL_GRAPH=0), for DILU and
+    SIMPLE run bit for bit as eager launches (the default), for DILU and
     Jacobi preconditioning."""
     ms = _cases(x)
     api = x.api

@@ -67,3 +67,33 @@ def test_fused_finalizes_bit_identical_komega(x, meshes, monkeypatch):
     assert np.array_equal(a["residuals"], b["residuals"])
     for u, v in zip(fa, fb):
         assert np.array_equal(u, v)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32-ir"])
+def test_fused_batch_chains_bit_identical(x, monkeypatch, prec):
+    """Batched CG and BiCGSTAB with Jacobi: each kernel of an iteration does
+    the previous kernel's per-case finalize itself (2 and 3 launches per
+    iteration).  Three cavities of different size and Reynolds number run as
+    one batch match the separate-finalize chain (XCAL_NO_FUSE=1) bit for
+    bit: fields, residuals and Krylov iteration counts per case."""
+    api, H = x.api, x.hip
+    ms = [x.mesh.cavity2d(n) for n in (12, 20, 16)]
+    model, config = cavity_case(api, ms[0], iterations=25, precon="Jacobi", rtol_p=1e-3,
+                                precision_p=prec)
+    setup = api.simple_setup(model, config, pref=0.0)
+    ctx = x.api._context(0)
+    runs = []
+    for fuse in (True, False):
+        if not fuse:
+            monkeypatch.setenv("XCAL_NO_FUSE", "1")
+        dm = H.DeviceMesh.batch(ctx, ms)
+        monkeypatch.delenv("XCAL_NO_FUSE", raising=False)
+        sim = H.Simple(dm, setup)
+        sim.set_case_nu([0.01, 0.004, 0.0025])
+        out = sim.run_batch(25)
+        runs.append((out, sim.get("U"), sim.get("p")))
+    (a, Ua, pa), (b, Ub, pb) = runs
+    assert a["done"] == b["done"] == 25
+    assert np.array_equal(a["krylov_iters"], b["krylov_iters"])
+    assert np.array_equal(a["residuals"], b["residuals"])
+    assert np.array_equal(Ua, Ub) and np.array_equal(pa, pb)

@@ -34,14 +34,27 @@ struct BlkK {
 
 // Sum of segment k ([seg[k], seg[k+1])) of the partial pairs by one
 // wavefront: fixed per-lane strides, then a fixed shuffle tree (the result
-// depends only on the segment).
+// depends only on the segment).  A lane loads U of its slots together, then
+// adds them in slot order, so a long segment takes few round trips.
 __device__ inline void wave_sum_segment(const double* __restrict__ partial, int32_t j0, int32_t j1,
                                         double& t0, double& t1) {
+  constexpr int U = 8;
   t0 = 0.0;
   t1 = 0.0;
-  for (int32_t j = j0 + (int32_t)threadIdx.x; j < j1; j += 64) {
-    t0 += partial[2 * j];
-    t1 += partial[2 * j + 1];
+  for (int32_t jb = j0 + (int32_t)threadIdx.x; jb < j1; jb += 64 * U) {
+    double a[U], b[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int32_t j = jb + 64 * u < j1 ? jb + 64 * u : j0;
+      a[u] = partial[2 * j];
+      b[u] = partial[2 * j + 1];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (jb + 64 * u < j1) {
+        t0 += a[u];
+        t1 += b[u];
+      }
   }
   for (int o = 32; o > 0; o >>= 1) {
     t0 += __shfl_down(t0, o, 64);
@@ -61,6 +74,53 @@ __global__ void __launch_bounds__(64)
   if (threadIdx.x == 0 && st.stop == STOP_NONE) {
     finalize(&st, fin, t0, t1);
     ks[k] = st;
+  }
+}
+
+// The Krylov state of case k a kernel of an iteration works with.  Unfused
+// (fin == FIN_NONE, out == nullptr) it is in[k], as kb_finalize left it.
+// Fused, the kernel first does the pending finalize itself: every workgroup
+// of the case sums the case's pending partials as kb_finalize does (one
+// wavefront, the same order) and applies the same scalar update, so all
+// hold the same state, and the case's first workgroup stores it to out[k]
+// (never in[k], which the case's other workgroups are still reading).
+struct BStep {
+  const KS* in;
+  KS* out;
+  const double* pin;
+  int fin;
+};
+inline BStep bstep_plain(const KS* ks) { return BStep{ks, nullptr, nullptr, FIN_NONE}; }
+__device__ inline KS bstate(const BStep& K, int k, const int32_t* __restrict__ seg, double* lds) {
+  KS st = K.in[k];
+  if (K.fin != FIN_NONE) {  // uniform over the launch
+    double t0 = 0.0, t1 = 0.0;
+    if (threadIdx.x < 64) wave_sum_segment(K.pin, seg[k], seg[k + 1], t0, t1);
+    if (threadIdx.x == 0) {
+      lds[0] = t0;
+      lds[1] = t1;
+    }
+    __syncthreads();
+    t0 = lds[0];
+    t1 = lds[1];
+    if (st.stop == STOP_NONE) finalize(&st, K.fin, t0, t1);
+  }
+  if (K.out && (int32_t)blockIdx.x == seg[k] && threadIdx.x == 0) K.out[k] = st;
+  return st;
+}
+
+// The pending finalize of a fused chain (fin, partial) applied to in[k] and
+// stored to out[k] (in place allowed), before the host reads the states.
+__global__ void __launch_bounds__(64)
+    kb_flush(const double* __restrict__ partial, const int32_t* __restrict__ seg, const KS* in,
+             KS* out, int fin) {
+  const int k = blockIdx.x;
+  KS st = in[k];
+  double t0 = 0.0, t1 = 0.0;
+  if (fin != FIN_NONE) wave_sum_segment(partial, seg[k], seg[k + 1], t0, t1);
+  if (threadIdx.x == 0) {
+    if (fin != FIN_NONE && st.stop == STOP_NONE) finalize(&st, fin, t0, t1);
+    out[k] = st;
   }
 }
 
@@ -146,20 +206,58 @@ __global__ void __launch_bounds__(kBlock)
   store_partial<kBlock>(s0, 0.0, partial, blockIdx.x);
 }
 
+// Fused form of kb_cg_p + kb_cg_spmv (fused chains): p = z + beta p formed
+// for the row and for each neighbour it gathers, in the same arithmetic as
+// kb_cg_p, so q and p.q equal the separate kernels' bit for bit.
+template <int G, class T>
+__global__ void __launch_bounds__(kBlock)
+    kb_cg_pspmv(BlkK B, MatKT<T> A, const T* __restrict__ z, const T* __restrict__ pold,
+                T* __restrict__ pnew, T* __restrict__ q, double* partial, BStep K,
+                const int32_t* __restrict__ seg) {
+  __shared__ double lds[2];
+  const int32_t i = B.row0[blockIdx.x] + threadIdx.x;
+  const bool in = i < B.row1[blockIdx.x];
+  const int32_t ic = in ? i : B.row0[blockIdx.x];
+  // the row's loads go out before the state is formed (its partials' round
+  // trip overlaps the row's)
+  const RowPair<G, T> R = row_pair_load<G>(A, ic, z, pold);
+  const T zi = z[ic], poi = pold[ic];
+  const KS st = bstate(K, B.kcase[blockIdx.x], seg, lds);
+  const bool live = st.stop == STOP_NONE;
+  const T beta = (T)st.beta;
+  double s0 = 0.0;
+  if (in) {
+    const T acc = row_pair_sum(A, R, z, pold, [&](T u, T w) { return u + beta * w; });
+    const T pi = zi + beta * poi;
+    if (live) {
+      pnew[i] = pi;
+      q[i] = acc;
+    }
+    s0 = dd(pi, acc);
+  }
+  store_partial<kBlock>(s0, 0.0, partial, blockIdx.x);
+}
+
 template <class T>
 __global__ void __launch_bounds__(kBlock)
     kb_cg_update_diag(BlkK B, const T* __restrict__ S, const T* __restrict__ p,
                       const T* __restrict__ q, T* __restrict__ x, T* __restrict__ r,
-                      T* __restrict__ z, double* partial, const KS* ks) {
-  const KS* K = ks + B.kcase[blockIdx.x];
+                      T* __restrict__ z, double* partial, BStep Kst,
+                      const int32_t* __restrict__ seg) {
+  __shared__ double lds[2];
+  const int32_t i = B.row0[blockIdx.x] + threadIdx.x;
+  const bool in = i < B.row1[blockIdx.x];
+  const int32_t ic = in ? i : B.row0[blockIdx.x];
+  const T x0 = x[ic], p0 = p[ic], r0 = r[ic], q0 = q[ic], S0 = S ? S[ic] : T(1);
+  const KS st = bstate(Kst, B.kcase[blockIdx.x], seg, lds);
+  const KS* K = &st;
   const bool live = !stopped(K);
   const T a = (T)K->alpha;
-  const int32_t i = B.row0[blockIdx.x] + threadIdx.x;
   double s0 = 0.0;
-  if (i < B.row1[blockIdx.x]) {
-    const T xi = x[i] + a * p[i];
-    const T ri = r[i] + (-a) * q[i];
-    const T zi = S ? S[i] * ri : ri;
+  if (in) {
+    const T xi = x0 + a * p0;
+    const T ri = r0 + (-a) * q0;
+    const T zi = S ? S0 * ri : ri;
     if (live) {
       x[i] = xi;
       r[i] = ri;
@@ -174,21 +272,29 @@ template <int G, bool VDIAG, class T>
 __global__ void __launch_bounds__(kBlock)
     kb_bi_spmv_p(BlkK B, MatKT<T> A, const T* __restrict__ r, const T* __restrict__ pold,
                  T* __restrict__ pnew, const T* __restrict__ S, T* __restrict__ qv,
-                 const T* __restrict__ c, double* partial, const KS* ks) {
-  const KS* K = ks + B.kcase[blockIdx.x];
+                 const T* __restrict__ c, double* partial, BStep Kst,
+                 const int32_t* __restrict__ seg) {
+  __shared__ double lds[2];
+  const int32_t i = B.row0[blockIdx.x] + threadIdx.x;
+  const bool in = i < B.row1[blockIdx.x];
+  const int32_t ic = in ? i : B.row0[blockIdx.x];
+  const RowPair<G, T> R = row_pair_load<G>(A, ic, r, pold);
+  const T ri0 = r[ic], poi = pold[ic];
+  const T Si = VDIAG && S ? S[ic] : T(1), ci = VDIAG ? c[ic] : T(0);
+  const KS st = bstate(Kst, B.kcase[blockIdx.x], seg, lds);
+  const KS* K = &st;
   const bool live = !stopped(K);
   const T beta = (T)K->beta;
-  const int32_t i = B.row0[blockIdx.x] + threadIdx.x;
   double s0 = 0.0;
-  if (i < B.row1[blockIdx.x]) {
-    const T acc = row_sum<G>(A, i, [&](int j) { return r[j] + beta * pold[j]; });
-    const T pi = r[i] + beta * pold[i];
-    const T vi = VDIAG && S ? S[i] * acc : acc;
+  if (in) {
+    const T acc = row_pair_sum(A, R, r, pold, [&](T u, T w) { return u + beta * w; });
+    const T pi = ri0 + beta * poi;
+    const T vi = VDIAG && S ? Si * acc : acc;
     if (live) {
       pnew[i] = pi;
       qv[i] = vi;
     }
-    if (VDIAG) s0 = dd(c[i], vi);
+    if (VDIAG) s0 = dd(ci, vi);
   }
   if (VDIAG) store_partial<kBlock>(s0, 0.0, partial, blockIdx.x);
 }
@@ -197,17 +303,25 @@ template <int G, bool TDIAG, class T>
 __global__ void __launch_bounds__(kBlock)
     kb_bi_spmv_s(BlkK B, MatKT<T> A, const T* __restrict__ r, const T* __restrict__ v,
                  const T* __restrict__ p, T* __restrict__ s, T* __restrict__ x,
-                 const T* __restrict__ S, T* __restrict__ dt, double* partial, const KS* ks) {
-  const KS* K = ks + B.kcase[blockIdx.x];
+                 const T* __restrict__ S, T* __restrict__ dt, double* partial, BStep Kst,
+                 const int32_t* __restrict__ seg) {
+  __shared__ double lds[2];
+  const int32_t i = B.row0[blockIdx.x] + threadIdx.x;
+  const bool in = i < B.row1[blockIdx.x];
+  const int32_t ic = in ? i : B.row0[blockIdx.x];
+  const RowPair<G, T> R = row_pair_load<G>(A, ic, r, v);
+  const T ri0 = r[ic], vi0 = v[ic], xi0 = x[ic], pi0 = p[ic];
+  const T Si = TDIAG && S ? S[ic] : T(1);
+  const KS st = bstate(Kst, B.kcase[blockIdx.x], seg, lds);
+  const KS* K = &st;
   const bool live = !stopped(K);
   const T a = (T)K->alpha;
-  const int32_t i = B.row0[blockIdx.x] + threadIdx.x;
   double s0 = 0.0, s1 = 0.0;
-  if (i < B.row1[blockIdx.x]) {
-    const T acc = row_sum<G>(A, i, [&](int j) { return r[j] + (-a) * v[j]; });
-    const T si = r[i] + (-a) * v[i];
-    const T xi = x[i] + a * p[i];
-    const T ti = TDIAG && S ? S[i] * acc : acc;
+  if (in) {
+    const T acc = row_pair_sum(A, R, r, v, [&](T u, T w) { return u + (-a) * w; });
+    const T si = ri0 + (-a) * vi0;
+    const T xi = xi0 + a * pi0;
+    const T ti = TDIAG && S ? Si * acc : acc;
     if (live) {
       s[i] = si;
       x[i] = xi;
@@ -225,22 +339,27 @@ template <class T>
 __global__ void __launch_bounds__(kBlock)
     kb_bi_tail(BlkK B, const T* __restrict__ s, const T* __restrict__ t, const T* __restrict__ v,
                T* __restrict__ x, T* __restrict__ r, T* __restrict__ p, const T* __restrict__ c,
-               double* partial, const KS* ks) {
-  const KS* K = ks + B.kcase[blockIdx.x];
+               double* partial, BStep Kst, const int32_t* __restrict__ seg) {
+  __shared__ double lds[2];
+  const int32_t i = B.row0[blockIdx.x] + threadIdx.x;
+  const bool in = i < B.row1[blockIdx.x];
+  const int32_t ic = in ? i : B.row0[blockIdx.x];
+  const T x0 = x[ic], s0v = s[ic], t0 = t[ic], p0 = p[ic], v0 = v[ic], c0 = c[ic];
+  const KS st = bstate(Kst, B.kcase[blockIdx.x], seg, lds);
+  const KS* K = &st;
   const bool live = !stopped(K);
   const T w = (T)K->omega;
-  const int32_t i = B.row0[blockIdx.x] + threadIdx.x;
   double s0 = 0.0, s1 = 0.0;
-  if (i < B.row1[blockIdx.x]) {
-    const T xi = x[i] + w * s[i];
-    const T ri = s[i] + (-w) * t[i];
-    const T pi = p[i] + (-w) * v[i];
+  if (in) {
+    const T xi = x0 + w * s0v;
+    const T ri = s0v + (-w) * t0;
+    const T pi = p0 + (-w) * v0;
     if (live) {
       x[i] = xi;
       r[i] = ri;
       p[i] = pi;
     }
-    s0 = dd(c[i], ri);
+    s0 = dd(c0, ri);
     s1 = dd(ri, ri);
   }
   store_partial<kBlock>(s0, s1, partial, blockIdx.x);
@@ -426,6 +545,8 @@ struct BSolver {
   T* work;
   KS* ks;
   double* partial;  // nblk slots (also used for the ncase sweep slots)
+  KS* fks;          // fused chains: the second state array (ncase)
+  double* fpart;    // fused chains: the second partial area (nblk slots)
   BlkK B;
   int nblk;
   int* pred;  // iterations of the last solve (first-batch size)
@@ -438,11 +559,15 @@ static void batch_workspace(xcal_eqn* e, bool fp32) {
   const int64_t ksb = (int64_t)sizeof(KS) * m->ncase;
   const bool grow = (e->work.n > 0 && e->work.n < 12 * n) ||
                     (e->kpart.n > 0 && e->kpart.n < 2 * slots) ||
+                    (e->fpart.n > 0 && e->fpart.n < 2 * slots) ||
+                    (e->fstate.n > 0 && e->fstate.n < ksb) ||
                     (e->kstate.n > 0 && e->kstate.n < ksb) ||
                     (fp32 && e->work32.n > 0 && e->work32.n < 12 * n);
   if (grow) XCAL_HIP(hipStreamSynchronize(m->ctx->stream));
   if (e->work.n < 12 * n) e->work.alloc(12 * n);
   if (e->kpart.n < 2 * slots) e->kpart.alloc(2 * slots);
+  if (e->fpart.n < 2 * slots) e->fpart.alloc(2 * slots);
+  if (e->fstate.n < ksb) e->fstate.alloc(ksb);
   if (e->kstate.n < ksb) {
     e->kstate.alloc(ksb);
     if (e->kstate_host) (void)hipHostFree(e->kstate_host);
@@ -483,6 +608,8 @@ static BSolver<double> batch_solver(xcal_eqn* e) {
   S.work = e->work.p;
   S.ks = (KS*)e->kstate.p;
   S.partial = e->kpart.p;
+  S.fks = (KS*)e->fstate.p;
+  S.fpart = e->fpart.p;
   S.B = blkk(m);
   S.nblk = (int)m->nblk;
   S.pred = &e->batch_pred[0];
@@ -530,6 +657,8 @@ static BSolver<float> batch_solver32(xcal_eqn* e) {
   S.work = e->work32.p;
   S.ks = (KS*)e->kstate.p;
   S.partial = e->kpart.p;
+  S.fks = (KS*)e->fstate.p;
+  S.fpart = e->fpart.p;
   S.B = blkk(m);
   S.nblk = (int)m->nblk;
   S.pred = &e->batch_pred[2];
@@ -545,6 +674,70 @@ template <class T>
 static void fin_cases(const BSolver<T>& S, int fin) {
   hipLaunchKernelGGL(kb_finalize, dim3(S.ncase), dim3(64), 0, S.s, S.partial, S.m->case_iota_d.p,
                      S.ncase, S.ks, fin);
+}
+
+// Fused chains for batched CG and BiCGSTAB with a diagonal (or no)
+// preconditioner: each kernel of an iteration does the pending finalize of
+// the kernel before it (bstate), so an iteration takes 2 (CG) or 3
+// (BiCGSTAB) launches instead of 5 or 6.  States alternate between S.ks and
+// S.fks, partial areas between S.partial and S.fpart (a kernel never writes
+// what another workgroup of it still reads); flush() applies the last
+// pending finalize into S.ks before the host reads the states.  Results are
+// those of the unfused chain bit for bit (the same sums in the same order,
+// the same scalar updates).  Off with XCAL_NO_FUSE=1 at mesh creation, for
+// DILU (its sweeps take the state from S.ks) and above kBatchFuseRows rows,
+// where a second gather per entry costs more than a launch.
+constexpr int64_t kBatchFuseRows = 1 << 20;
+
+template <class T>
+struct BFuse {
+  const BSolver<T>& S;
+  bool on;
+  int cur = 0;         // the state array holding the current state: 0 S.ks, 1 S.fks
+  int pend = -1;       // partial area of the pending finalize: 0 S.partial, 1 S.fpart
+  int pfin = FIN_NONE;
+  KS* ks(int a) const { return a ? S.fks : S.ks; }
+  double* pa(int a) const { return a ? S.fpart : S.partial; }
+  // a kernel that reads no state: its partials, then done(fin)
+  double* start() {
+    if (!on) return S.partial;
+    pend = 0;
+    return pa(0);
+  }
+  // a kernel of the chain: its state and, in `out`, its partial area
+  BStep step(double*& out) {
+    if (!on) {
+      out = S.partial;
+      return bstep_plain(S.ks);
+    }
+    const int po = pend == 0 ? 1 : 0;
+    const BStep K{ks(cur), ks(cur ^ 1), pend >= 0 ? pa(pend) : nullptr,
+                  pend >= 0 ? pfin : FIN_NONE};
+    cur ^= 1;
+    pend = po;
+    pfin = FIN_NONE;
+    out = pa(po);
+    return K;
+  }
+  void done(int fin) {
+    if (on) pfin = fin;
+    else hipLaunchKernelGGL(kb_finalize, dim3(S.ncase), dim3(64), 0, S.s, S.partial,
+                            S.m->case_blk_d.p, S.ncase, S.ks, fin);
+  }
+  void flush() {
+    if (!on || (cur == 0 && pfin == FIN_NONE)) return;
+    hipLaunchKernelGGL(kb_flush, dim3(S.ncase), dim3(64), 0, S.s,
+                       (const double*)(pend >= 0 ? pa(pend) : S.partial), S.m->case_blk_d.p,
+                       (const KS*)ks(cur), S.ks, pfin);
+    cur = 0;
+    pend = -1;
+    pfin = FIN_NONE;
+  }
+};
+
+template <class T>
+static bool fuse_on(const BSolver<T>& S, int precon) {
+  return precon != XCAL_PRECON_DILU && S.m->fuse && S.n <= kBatchFuseRows;
 }
 
 template <bool UPD, class T>
@@ -658,7 +851,9 @@ static bool graphs_enabled(const xcal_ctx* c) {
 // equation's graph for the current parity (captured on first use).
 template <class T>
 static void brun(BSolver<T>& S, int tail, const std::function<void()>& iteration,
-                 const std::vector<uint64_t>* gkey = nullptr, const int* parity = nullptr) {
+                 const std::vector<uint64_t>* gkey = nullptr, const int* parity = nullptr,
+                 const std::function<void()>& flush = nullptr) {
+  if (flush) flush();
   if (bread(S)) return;
   int batch = *S.pred > 0 ? std::max(1, (*S.pred * 9) / 10) : 8;
   auto run = [&](int count) {
@@ -683,6 +878,7 @@ static void brun(BSolver<T>& S, int tail, const std::function<void()>& iteration
   for (;;) {
     run(batch);
     XCAL_HIP(hipGetLastError());
+    if (flush) flush();
     if (bread(S)) break;
     batch = tail;
   }
@@ -788,6 +984,8 @@ static void bcg(BSolver<T>& S, int precon, const T* b, T* x0, int itmax, double 
   const bool dilu = precon == XCAL_PRECON_DILU;
   const T* Sd = precon == XCAL_PRECON_JACOBI ? S.Sj : nullptr;
   const KS* ks = S.ks;
+  BFuse<T> F{S, fuse_on(S, precon)};
+  const int32_t* seg = S.m->case_blk_d.p;
   if (dilu) {
     tune_pipes(S, (const T*)r, Y, Yb, zd);
     bsweep_fwd<false>(S, bfwdk(S, (const T*)r, Y, Yb), ks);
@@ -795,8 +993,8 @@ static void bcg(BSolver<T>& S, int precon, const T* b, T* x0, int itmax, double 
     fin_cases(S, FIN_CG_INIT);
   } else {
     hipLaunchKernelGGL((kb_precond_diag_start<false, T>), dim3(S.nblk), dim3(kBlock), 0, S.s, S.B,
-                       Sd, r, zd, (const T*)nullptr, S.partial);
-    fin_blocks(S, FIN_CG_INIT);
+                       Sd, r, zd, (const T*)nullptr, F.start());
+    F.done(FIN_CG_INIT);
   }
   int parity = 0;
   xcal_ctx* c = S.m->ctx;
@@ -804,6 +1002,20 @@ static void bcg(BSolver<T>& S, int precon, const T* b, T* x0, int itmax, double 
     T* pold = parity ? pb : pa;
     T* pnew = parity ? pa : pb;
     parity ^= 1;
+    if (F.on) {  // two launches: [p, q = A p, p.q], [x, r, z, r.z]
+      double* P;
+      const BStep K1 = F.step(P);
+      with_row_group(S.A.rw, [&](auto G) {
+        hipLaunchKernelGGL((kb_cg_pspmv<decltype(G)::value, T>), dim3(S.nblk), dim3(kBlock), 0,
+                           S.s, S.B, S.A, (const T*)zd, (const T*)pold, pnew, q, P, K1, seg);
+      });
+      F.done(FIN_CG_PAP);
+      const BStep K2 = F.step(P);
+      hipLaunchKernelGGL(kb_cg_update_diag<T>, dim3(S.nblk), dim3(kBlock), 0, S.s, S.B, Sd,
+                         (const T*)pnew, (const T*)q, x, r, zd, P, K2, seg);
+      F.done(FIN_CG_ITER);
+      return;
+    }
     hipLaunchKernelGGL(kb_cg_p<T>, dim3(S.nblk), dim3(kBlock), 0, S.s, S.B, (const T*)zd,
                        (const T*)pold, pnew, ks);
     timed(c, 0, [&] {
@@ -824,12 +1036,15 @@ static void bcg(BSolver<T>& S, int precon, const T* b, T* x0, int itmax, double 
       fin_cases(S, FIN_CG_ITER);
     } else {
       hipLaunchKernelGGL(kb_cg_update_diag<T>, dim3(S.nblk), dim3(kBlock), 0, S.s, S.B, Sd,
-                         (const T*)pnew, (const T*)q, x, r, zd, S.partial, ks);
+                         (const T*)pnew, (const T*)q, x, r, zd, S.partial, bstep_plain(ks), seg);
       fin_blocks(S, FIN_CG_ITER);
     }
   };
   const auto key = graph_key(S, 0, precon, nullptr);
-  brun(S, 4, iteration, graphs_enabled(c) ? &key : nullptr, &parity);
+  // fused chains carry their pending finalize between launches, so they are
+  // not captured
+  brun(S, 4, iteration, graphs_enabled(c) && !F.on ? &key : nullptr, &parity,
+       [&] { F.flush(); });
   if (x0) hipLaunchKernelGGL(kb_add, dim3(grid_for(n)), dim3(kBlock), 0, S.s, n, (const double*)x,
                              (double*)x0);
   if (xsol) *xsol = x;
@@ -849,6 +1064,8 @@ static void bbicg(BSolver<T>& S, int precon, const T* b, T* x0, int itmax, doubl
   const bool dilu = precon == XCAL_PRECON_DILU;
   const T* Sd = precon == XCAL_PRECON_JACOBI ? S.Sj : nullptr;
   const KS* ks = S.ks;
+  BFuse<T> F{S, fuse_on(S, precon)};
+  const int32_t* seg = S.m->case_blk_d.p;
   if (dilu) {
     tune_pipes(S, (const T*)r, Y, Yb, t);
     bsweep_fwd<false>(S, bfwdk(S, (const T*)r, Y, Yb), ks);
@@ -856,8 +1073,8 @@ static void bbicg(BSolver<T>& S, int precon, const T* b, T* x0, int itmax, doubl
     fin_cases(S, FIN_BI_INIT);
   } else {
     hipLaunchKernelGGL((kb_precond_diag_start<true, T>), dim3(S.nblk), dim3(kBlock), 0, S.s, S.B, Sd,
-                       r, (T*)nullptr, b, S.partial);
-    fin_blocks(S, FIN_BI_INIT);
+                       r, (T*)nullptr, b, F.start());
+    F.done(FIN_BI_INIT);
   }
   int parity = 0;
   auto iteration = [&]() {
@@ -869,7 +1086,8 @@ static void bbicg(BSolver<T>& S, int precon, const T* b, T* x0, int itmax, doubl
         constexpr int g = decltype(G)::value;
         hipLaunchKernelGGL((kb_bi_spmv_p<g, false, T>), dim3(S.nblk), dim3(kBlock), 0,
                            S.s, S.B, S.A,
-                           (const T*)r, (const T*)pold, pnew, (const T*)nullptr, d, b, S.partial, ks);
+                           (const T*)r, (const T*)pold, pnew, (const T*)nullptr, d, b, S.partial,
+                           bstep_plain(ks), seg);
       });
       bsweep_fwd<false>(S, bfwdk(S, (const T*)d, Y, Yb), ks);
       bsweep_bwd<2>(S, bbwdk(S, (const T*)Yb, v, b), ks);
@@ -879,33 +1097,39 @@ static void bbicg(BSolver<T>& S, int precon, const T* b, T* x0, int itmax, doubl
         hipLaunchKernelGGL((kb_bi_spmv_s<g, false, T>), dim3(S.nblk), dim3(kBlock), 0,
                            S.s, S.B, S.A,
                            (const T*)r, (const T*)v, (const T*)pnew, sv, x, (const T*)nullptr, d,
-                           S.partial, ks);
+                           S.partial, bstep_plain(ks), seg);
       });
       bsweep_fwd<false>(S, bfwdk(S, (const T*)d, Y, Yb), ks);
       bsweep_bwd<3>(S, bbwdk(S, (const T*)Yb, t, (const T*)sv), ks);
       fin_cases(S, FIN_BI_OMEGA);
-    } else {
+    } else {  // fused: three launches per iteration
+      double* P;
+      const BStep Kp = F.step(P);
       with_row_group(S.A.rw, [&](auto G) {
         constexpr int g = decltype(G)::value;
         hipLaunchKernelGGL((kb_bi_spmv_p<g, true, T>), dim3(S.nblk), dim3(kBlock), 0,
                            S.s, S.B, S.A,
-                           (const T*)r, (const T*)pold, pnew, Sd, v, b, S.partial, ks);
+                           (const T*)r, (const T*)pold, pnew, Sd, v, b, P, Kp, seg);
       });
-      fin_blocks(S, FIN_BI_ALPHA);
+      F.done(FIN_BI_ALPHA);
+      const BStep Ks = F.step(P);
       with_row_group(S.A.rw, [&](auto G) {
         constexpr int g = decltype(G)::value;
         hipLaunchKernelGGL((kb_bi_spmv_s<g, true, T>), dim3(S.nblk), dim3(kBlock), 0,
                            S.s, S.B, S.A,
-                           (const T*)r, (const T*)v, (const T*)pnew, sv, x, Sd, t, S.partial, ks);
+                           (const T*)r, (const T*)v, (const T*)pnew, sv, x, Sd, t, P, Ks, seg);
       });
-      fin_blocks(S, FIN_BI_OMEGA);
+      F.done(FIN_BI_OMEGA);
     }
+    double* P;
+    const BStep Kt = F.step(P);
     hipLaunchKernelGGL(kb_bi_tail<T>, dim3(S.nblk), dim3(kBlock), 0, S.s, S.B, (const T*)sv,
-                       (const T*)t, (const T*)v, x, r, pnew, b, S.partial, ks);
-    fin_blocks(S, FIN_BI_END);
+                       (const T*)t, (const T*)v, x, r, pnew, b, P, Kt, seg);
+    F.done(FIN_BI_END);
   };
   const auto key = graph_key(S, 1, precon, b);
-  brun(S, 2, iteration, graphs_enabled(S.m->ctx) ? &key : nullptr, &parity);
+  brun(S, 2, iteration, graphs_enabled(S.m->ctx) && !F.on ? &key : nullptr, &parity,
+       [&] { F.flush(); });
   if (x0) hipLaunchKernelGGL(kb_add, dim3(grid_for(n)), dim3(kBlock), 0, S.s, n, (const double*)x,
                              (double*)x0);
   if (xsol) *xsol = x;

@@ -32,7 +32,8 @@ detail.config1 = the reference's CPU configuration (40x40 cavity, 200
             iterations): the CPU oracle (median of 3) beside the device.
 
 Other modes: --mode batch (batched cavities), --mode serve (k-omega serving
-with Poisson arrivals: p50/p99 time to converged), --mode piso (the channel as
+with Poisson arrivals: p50/p99 time to converged), --mode stepbatch (the
+serving workload's first --cases requests stepped as one batch), --mode piso (the channel as
 a transient PISO case: cell-steps/s), --mode calib (a plain 256 MiB copy for
 HBM counter calibration).
 
@@ -223,6 +224,54 @@ def serve_leg(args, x, requests, rate, max_iterations, alone=True):
         "converged_flags": conv.astype(int).tolist(),
     }
     return float(np.sum(ncell * its)) / wall, summary
+
+
+def run_stepbatch(args, x):
+    """--mode stepbatch: the serving workload's first `--cases` requests
+    stepped together as one batch mesh (no arrivals, no server): k-omega
+    SIMPLE iterations per second of the batch the server would form, with
+    per-stage device time and Krylov counts."""
+    api, H = x.api, x.hip
+    k = min(args.cases, 8)
+    ratio, cells = serve_cases(max(8, k))
+    cases = [step_case(api, x, r, c, args.warmup + args.steps, rans=not args.laminar,
+                       precon=args.precon) for r, c in zip(ratio[:k], cells[:k])]
+    meshes = [c[0] for c in cases]
+    ctx = H.Context(0)
+    dm = H.DeviceMesh.batch(ctx, meshes)
+    model, config = cases[0][1], cases[0][2]
+    sim = H.Simple(dm, api.simple_setup(model, config))
+    if not args.laminar:
+        sim.set_turbulence(api.turbulence_setup(model, config))
+        for f in ("k", "omega", "nut"):
+            sim.set_scalar(f, np.concatenate([np.asarray(getattr(c[1].turbulence, f),
+                                                         np.float64).reshape(-1) for c in cases]))
+    sim.set_U(np.concatenate([np.asarray(c[1].momentum.U, np.float64).reshape(3, -1)
+                              for c in cases], axis=1))
+    if args.warmup:
+        sim.run_batch(args.warmup)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    out = sim.run_batch(args.steps)
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    kit = out["krylov_iters"]
+    line = {
+        "metric": METRIC, "value": round(dm.n * out["done"] / wall, 1), "unit": "cell-iters/s",
+        "n_gpus": 1, "steps": out["done"], "warmup": args.warmup,
+        "ms_per_step": round(1e3 * wall / max(out["done"], 1), 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (the serving workload's seeded step ratios and sizes)",
+        "config": {"workload": f"first {k} serving requests as one batch: 2D backward-facing "
+                               f"steps, " + ("laminar" if args.laminar else "k-omega RANS")
+                               + f", {args.precon}",
+                   "cells": dm.n, "case_cells": [m.n_cells for m in meshes],
+                   "parallelism": "single GPU, one batch mesh"},
+        "roofline": None, "cpu_baseline": None,
+        "detail": {"device_ms": {kk: round(v, 2) for kk, v in sim.stage_times().items()},
+                   "krylov_iters_mean_per_case_Ux_Uy_Uz_p": kit.mean(axis=2).round(1).tolist()},
+    }
+    print(json.dumps(line))
 
 
 def run_serve(args, x):
@@ -590,7 +639,7 @@ def main():
     ap.add_argument("--quick", action="store_true",
                     help="channel mode: skip the mixed-precision and batched summaries")
     ap.add_argument("--mode", default="channel",
-                    choices=["channel", "batch", "serve", "piso", "calib"],
+                    choices=["channel", "batch", "serve", "stepbatch", "piso", "calib"],
                     help="channel: the 1M-cell FP64 channel; batch: batched cavities; "
                          "serve: Poisson-arrival serving of step cases; piso: the channel "
                          "as a transient PISO case")
@@ -624,6 +673,9 @@ def main():
         return
     if args.mode == "serve":
         run_serve(args, x)
+        return
+    if args.mode == "stepbatch":
+        run_stepbatch(args, x)
         return
     if args.mode == "calib":
         run_calib(args)

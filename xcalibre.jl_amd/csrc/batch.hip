@@ -1213,7 +1213,7 @@ static void bir_solve(xcal_eqn* e, const xcal_solver_setup* st, int itmax, doubl
     }
     if (!any) break;
     float* d = nullptr;
-    S32.pred = &e->batch_pred[4 + std::min(sweep, 12)];
+    S32.pred = e->pred_slot(x, 16 + 2 * std::min(sweep, 12) + (st->solver == XCAL_SOLVER_CG ? 0 : 1));
     const double ia = single ? st->atol : 0.0;
     if (st->solver == XCAL_SOLVER_CG)
       bcg(S32, st->precon, (const float*)r32, (float*)nullptr, itmax, ia, inner_rtol, frozen.data(),
@@ -1278,6 +1278,7 @@ int solve_system_batch(xcal_eqn* e, const xcal_solver_setup* st, double* x, cons
     bir_solve(e, st, itmax, x, b, niter, nouter, rcs);
   } else {
     BSolver<double> S = batch_solver(e);
+    S.pred = e->pred_slot(x, st->solver == XCAL_SOLVER_CG ? 0 : 1);
     if (st->solver == XCAL_SOLVER_CG)
       bcg(S, st->precon, b, x, itmax, st->atol, st->rtol, (const int*)nullptr, (double**)nullptr);
     else

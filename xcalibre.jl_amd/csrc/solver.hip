@@ -1087,8 +1087,8 @@ static int ir_solve(xcal_eqn* e, const xcal_solver_setup* st, int itmax, double*
     }
     int ni = 0;
     float* d = nullptr;
-    S32.cg_batch = &e->ir_pred[0][std::min(nouter, 15)];
-    S32.bi_batch = &e->ir_pred[1][std::min(nouter, 15)];
+    S32.cg_batch = e->pred_slot(x, 16 + 2 * std::min(nouter, 15));
+    S32.bi_batch = e->pred_slot(x, 17 + 2 * std::min(nouter, 15));
     const double ia = single ? st->atol : 0.0;
     // a sweep reduces its residual by inner_rtol, or only by what the outer
     // tolerance still asks (with a factor 2 of margin for the FP32
@@ -1179,6 +1179,8 @@ int solve_system(xcal_eqn* e, const xcal_solver_setup* st, double* x, const doub
     rc = ir_solve(e, st, itmax, x, b, niter, nouter);
   } else {
     Solver<double> S = make_solver(e);
+    S.cg_batch = e->pred_slot(x, 0);
+    S.bi_batch = e->pred_slot(x, 1);
     if (st->solver == XCAL_SOLVER_CG)
       rc = cg(S, st->precon, b, x, itmax, st->atol, st->rtol, niter, (double**)nullptr);
     else

@@ -318,6 +318,12 @@ struct xcal_eqn {
   xcal::DBuf<int32_t> case_active;
   std::vector<double> case_host;
   int ir_pred[2][16] = {};  // FP32 inner solves: iterations of the last solve per sweep index
+  // Iterations of the last solve of each field, by (field pointer, solve
+  // kind): the first batch of iterations a solve enqueues is sized from the
+  // same field's last solve, not from another field sharing the equation (p,
+  // k and omega share one), whose count can differ by 10x.
+  std::map<std::pair<const void*, int>, int> preds;
+  int* pred_slot(const void* field, int kind) { return &preds[{field, kind}]; }
   // batched solves: instantiated HIP graphs of two Krylov iterations, by the
   // solve's launch signature (its kernels' arguments) and the iteration
   // parity at the graph's start

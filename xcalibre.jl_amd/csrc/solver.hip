@@ -378,6 +378,21 @@ __global__ void __launch_bounds__(kBlock)
   if (i < n) b[i] = (float)a[i];
 }
 
+// b = a * rd rounded to FP32 (the folded forward factor a_ij / D_j)
+__global__ void __launch_bounds__(kBlock)
+    k_fold_float(int64_t n, const double* __restrict__ a, const double* __restrict__ rd,
+                 float* __restrict__ b) {
+  const int64_t i = xcd_row();
+  if (i < n) b[i] = (float)(a[i] * rd[i]);
+}
+
+// b = 1 / a rounded to FP32
+__global__ void __launch_bounds__(kBlock)
+    k_recip_float(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
+  const int64_t i = xcd_row();
+  if (i < n) b[i] = (float)(1.0 / a[i]);
+}
+
 // one level: rows [row0, row0 + cnt) (forward levels are contiguous)
 template <bool UPD, int WK, class T>
 __global__ void __launch_bounds__(kBlock)
@@ -606,10 +621,17 @@ static Solver<float> make_solver32(xcal_eqn* e) {
         hipLaunchKernelGGL(k_to_float, dim3(grid_for(src.n)), dim3(kBlock), 0, s, src.n, src.p,
                            dst.p);
     };
-    conv(e->cfa, e->cfa32);
-    conv(e->cfr, e->cfr32);
+    // folded factors of the FP32 cone sweeps (cone_folded): a_ij / D_j per
+    // forward entry, 1 / D per backward position
+    if (e->cfa32.n != e->cfa.n) e->cfa32.alloc(e->cfa.n);
+    if (e->cfa.n > 0)
+      hipLaunchKernelGGL(k_fold_float, dim3(grid_for(e->cfa.n)), dim3(kBlock), 0, s, e->cfa.n,
+                         (const double*)e->cfa.p, (const double*)e->cfr.p, e->cfa32.p);
     conv(e->cba, e->cba32);
-    conv(e->cbd, e->cbd32);
+    if (e->cbd32.n != e->cbd.n) e->cbd32.alloc(e->cbd.n);
+    if (e->cbd.n > 0)
+      hipLaunchKernelGGL(k_recip_float, dim3(grid_for(e->cbd.n)), dim3(kBlock), 0, s, e->cbd.n,
+                         (const double*)e->cbd.p, e->cbd32.p);
   } else if (e->precon == XCAL_PRECON_JACOBI) {
     hipLaunchKernelGGL(k_to_float, dim3(grid_for(n)), dim3(kBlock), 0, s, n, e->pS.p, e->pS32.p);
   }
@@ -628,7 +650,7 @@ static Solver<float> make_solver32(xcal_eqn* e) {
   S.Db = e->Db32.p;
   S.Lrd = e->Lrd32.p;
   S.cfa = e->cfa32.p;
-  S.cfr = e->cfr32.p;
+  S.cfr = e->cfa32.p;  // unused by the folded FP32 forward cone sweep
   S.cba = e->cba32.p;
   S.cbd = e->cbd32.p;
   S.work = e->work32.p;

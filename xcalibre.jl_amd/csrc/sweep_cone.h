@@ -107,6 +107,16 @@ struct CFRec {
 // x += alpha p over the block's own rows after the sweep.
 // The caller returns before the sweep once the solve has stopped, so every
 // store here is live.
+// FP32 (the inner solves of the mixed-precision form, refined in FP64 to
+// the FP64 tolerance) takes folded factors: a holds a_ij / D_j per forward
+// entry (rd unused) and d holds 1 / D per backward position, so a forward
+// entry loads one value and a backward row multiplies.  FP64 keeps the
+// reference's (a_ij y_j) / D_j and division.
+template <class T>
+constexpr bool cone_folded() {
+  return std::is_same<T, float>::value;
+}
+
 template <bool UPD, class T, int WK, int P>
 __device__ inline void fwd_cone(const ConeK& C, const T* __restrict__ a, const T* __restrict__ rd,
                                 const FwdK<T>& W, ConeLDS<T>& L) {
@@ -120,7 +130,7 @@ __device__ inline void fwd_cone(const ConeK& C, const T* __restrict__ a, const T
 #pragma unroll
     for (int k = 0; k < WK; ++k) {
       R.av[k] = a[e[k]];
-      R.rv[k] = rd[e[k]];
+      if (!cone_folded<T>()) R.rv[k] = rd[e[k]];
     }
   };
   auto gather = [&](CFRec<T, WK>& R) {
@@ -138,7 +148,7 @@ __device__ inline void fwd_cone(const ConeK& C, const T* __restrict__ a, const T
     for (int k = 0; k < WK; ++k) {
       const int li = R.li[k];
       const T yv = L.v[li >= 0 ? li : 0];
-      const T term = (R.av[k] * yv) * R.rv[k];
+      const T term = cone_folded<T>() ? R.av[k] * yv : (R.av[k] * yv) * R.rv[k];
       sum = li >= 0 ? sum + term : sum;
     }
     const T u = UPD ? R.ro + (-al) * R.qo : R.ro;
@@ -217,7 +227,7 @@ __device__ inline void bwd_cone(const ConeK& C, const T* __restrict__ a, const T
       const T term = R.av[k] * zv;
       sum = li >= 0 ? sum + term : sum;
     }
-    const T zi = (R.yb - sum) / R.d;
+    const T zi = cone_folded<T>() ? (R.yb - sum) * R.d : (R.yb - sum) / R.d;
     L.v[R.lp] = zi;
     const int32_t i = R.c >= 0 ? R.c : ~R.c;
     W.Z[i] = zi;

@@ -26,7 +26,7 @@ __global__ void __launch_bounds__(kBlock)
   // once, then the faces are summed in list order.  Only Laplacian and
   // Divergence terms read a face flux; a term without a face flux field
   // (a constant, or the cell field of a Time or Si term, which has no face
-  // contribution) reads the weights in its place and never uses them.
+  // contribution) takes the weight in its place and never uses it.
   struct Rec {
     double ns, w, lapEf, delta, J[kMaxTerms];
     int32_t slot;
@@ -38,14 +38,14 @@ __global__ void __launch_bounds__(kBlock)
         Rec r;
         r.ns = (double)m.cf_sign[k];
         r.slot = m.cf_slot[k];
-        r.w = m.weight[f];
-        r.lapEf = m.lapEf[f];
-        r.delta = m.delta[f];
+        // the face's geometry streams in cf order; only the fluxes gather
+        r.w = m.cf_w[k];
+        r.lapEf = m.cf_lapEf[k];
+        r.delta = m.cf_delta[k];
 #pragma unroll
         for (int t = 0; t < kMaxTerms; ++t) {
           const bool face_flux = t < T.n && T.t[t].flux && face_term(T.t[t].type);
-          const double* src = face_flux ? T.t[t].flux : m.weight;
-          r.J[t] = src[f];
+          r.J[t] = face_flux ? T.t[t].flux[f] : r.w;
         }
         return r;
       },

@@ -123,16 +123,16 @@ __global__ void __launch_bounds__(kBlock)
   if (i >= m.n) return;
   double rx = 0.0, ry = 0.0, rz = 0.0;
   struct Rec {
-    double v, ns, sx, sy, sz;
+    double v, sx, sy, sz;  // sx, sy, sz: S_f nsign, streamed in cf order
   };
   cell_faces(
       m.cf_ptr[i], m.cf_ptr[i + 1],
       [&](int32_t k) {
         const int f = m.cf_face[k];
-        return Rec{phif[f], (double)m.cf_sign[k], m.sfx[f], m.sfy[f], m.sfz[f]};
+        return Rec{phif[f], m.cf_sx[k], m.cf_sy[k], m.cf_sz[k]};
       },
       [&](const Rec& r, bool ok) {
-        const double tx = r.v * (r.sx * r.ns), ty = r.v * (r.sy * r.ns), tz = r.v * (r.sz * r.ns);
+        const double tx = r.v * r.sx, ty = r.v * r.sy, tz = r.v * r.sz;
         rx = ok ? rx + tx : rx;
         ry = ok ? ry + ty : ry;
         rz = ok ? rz + tz : rz;
@@ -179,8 +179,7 @@ __global__ void __launch_bounds__(kBlock)
       m.cf_ptr[i], m.cf_ptr[i + 1],
       [&](int32_t k) {
         const int f = m.cf_face[k];
-        const double ns = (double)m.cf_sign[k];
-        return Rec{{m.sfx[f] * ns, m.sfy[f] * ns, m.sfz[f] * ns}, {fx[f], fy[f], fz[f]}};
+        return Rec{{m.cf_sx[k], m.cf_sy[k], m.cf_sz[k]}, {fx[f], fy[f], fz[f]}};
       },
       [&](const Rec& e, bool ok) {
 #pragma unroll

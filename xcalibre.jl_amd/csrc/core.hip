@@ -933,6 +933,28 @@ static void union_cases(std::vector<HostMesh>& C, HostMesh& U, std::vector<int64
   U.lev_ptr_b.push_back((int32_t)N);
 }
 
+// Face values per cf entry k (face f, nsign ns): weight, |E_f| and delta of
+// f, and S_f ns (exact: ns is +-1)
+__global__ void __launch_bounds__(kBlock)
+    k_cf_face_values(int64_t ncf, const int32_t* __restrict__ cf_face,
+                     const int8_t* __restrict__ cf_sign, const double* __restrict__ w,
+                     const double* __restrict__ lapEf, const double* __restrict__ delta,
+                     const double* __restrict__ sx, const double* __restrict__ sy,
+                     const double* __restrict__ sz, double* __restrict__ ow,
+                     double* __restrict__ olap, double* __restrict__ odel, double* __restrict__ osx,
+                     double* __restrict__ osy, double* __restrict__ osz) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= ncf) return;
+  const int32_t f = cf_face[k];
+  const double ns = (double)cf_sign[k];
+  ow[k] = w[f];
+  olap[k] = lapEf[f];
+  odel[k] = delta[f];
+  osx[k] = sx[f] * ns;
+  osy[k] = sy[f] * ns;
+  osz[k] = sz[f] * ns;
+}
+
 static xcal_mesh* upload_mesh(xcal_ctx* ctx, HostMesh& M) {
   XCAL_HIP(hipSetDevice(ctx->device));
   auto m = std::make_unique<xcal_mesh>();
@@ -994,6 +1016,18 @@ static xcal_mesh* upload_mesh(xcal_ctx* ctx, HostMesh& M) {
   upload(m->ntx, M.ntx, bytes);
   upload(m->nty, M.nty, bytes);
   upload(m->ntz, M.ntz, bytes);
+  {  // cf-ordered copies of the face values assembly reads per cell face
+    const int64_t ncf = (int64_t)M.cf_face.size();
+    for (auto* b : {&m->cf_w, &m->cf_lapEf, &m->cf_delta, &m->cf_sx, &m->cf_sy, &m->cf_sz})
+      b->alloc(ncf);
+    bytes += 6 * 8 * ncf;
+    if (ncf > 0)
+      hipLaunchKernelGGL(k_cf_face_values, dim3(grid_for(ncf)), dim3(kBlock), 0, ctx->stream, ncf,
+                         m->cf_face.p, m->cf_sign.p, m->weight.p, m->lapEf.p, m->delta.p,
+                         m->sfx.p, m->sfy.p, m->sfz.p, m->cf_w.p, m->cf_lapEf.p, m->cf_delta.p,
+                         m->cf_sx.p, m->cf_sy.p, m->cf_sz.p);
+    XCAL_HIP(hipGetLastError());
+  }
   upload(m->bc_cells, M.bc_cells, bytes);
   upload(m->bc_ptr, M.bc_ptr, bytes);
   upload(m->bc_face, M.bc_face, bytes);

@@ -13,6 +13,8 @@ struct MeshK {
   const double* vol;
   const int32_t *cf_ptr, *cf_face, *cf_nbr, *cf_slot;
   const int8_t* cf_sign;
+  const double *cf_w, *cf_lapEf, *cf_delta;  // face values per cf entry
+  const double *cf_sx, *cf_sy, *cf_sz;       // nsign S_f per cf entry
   const int32_t* diag;
   const int32_t *own, *nei, *fslot;
   const double *area, *weight, *delta, *nx, *ny, *nz, *ex, *ey, *ez, *sfx, *sfy, *sfz, *lapEf;
@@ -59,6 +61,12 @@ inline MeshK meshk(const xcal_mesh* m) {
   k.cf_nbr = m->cf_nbr.p;
   k.cf_slot = m->cf_slot.p;
   k.cf_sign = m->cf_sign.p;
+  k.cf_w = m->cf_w.p;
+  k.cf_lapEf = m->cf_lapEf.p;
+  k.cf_delta = m->cf_delta.p;
+  k.cf_sx = m->cf_sx.p;
+  k.cf_sy = m->cf_sy.p;
+  k.cf_sz = m->cf_sz.p;
   k.diag = m->diag.p;
   k.own = m->own.p;
   k.nei = m->nei.p;

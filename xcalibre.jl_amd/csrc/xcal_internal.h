@@ -177,6 +177,10 @@ struct xcal_mesh {
   // cells
   xcal::DBuf<double> vol;
   xcal::DBuf<int32_t> cf_ptr, cf_face, cf_nbr, cf_slot;
+  // per cell-face entry, in cf order: the face's weight, |E_f| and delta,
+  // and S_f times the entry's nsign (assembly and Green-Gauss stream them
+  // instead of gathering them by face)
+  xcal::DBuf<double> cf_w, cf_lapEf, cf_delta, cf_sx, cf_sy, cf_sz;
   xcal::DBuf<int8_t> cf_sign;
   xcal::DBuf<int32_t> diag;
   // faces

@@ -137,8 +137,15 @@ typedef struct {
 /* Linear solver settings: SolverSetup (Solve_1_api.jl:8-83). */
 enum { XCAL_SOLVER_CG = 0, XCAL_SOLVER_BICGSTAB = 1 };
 enum { XCAL_PRECON_NONE = 0, XCAL_PRECON_JACOBI = 1, XCAL_PRECON_DILU = 2 };
-enum { XCAL_PREC_FP64 = 0,      /* plain FP64 Krylov                         */
-       XCAL_PREC_FP32_IR = 1 }; /* FP32 inner Krylov in FP64 refinement loop */
+enum { XCAL_PREC_FP64 = 0,       /* plain FP64 Krylov                         */
+       XCAL_PREC_FP32_IR = 1,    /* FP32 inner Krylov in FP64 refinement loop */
+       /* FP64 CG (SpMV, vectors, dot products, residual recurrence) with
+        * the DILU sweeps computed in FP32 from FP32 factors, on single
+        * meshes in the cone sweep form.  BiCGSTAB (its left-preconditioned
+        * recurrence runs through the sweeps), other sweep forms and other
+        * preconditioners keep FP64 under this mode; batch meshes reject it
+        * (XCAL_ERR_UNSUPPORTED). */
+       XCAL_PREC_FP32_PRECON = 2 };
 typedef struct {
   int32_t solver;
   int32_t precon;

@@ -97,11 +97,15 @@ def test_cone_preconditioner_bit_exact(x, A, O, ctx, box48, monkeypatch, rows, d
 
 
 @pytest.mark.parametrize("solver", ["SOLVER_CG", "SOLVER_BICGSTAB"])
-@pytest.mark.parametrize("prec", ["PREC_FP64", "PREC_FP32_IR"])
+@pytest.mark.parametrize("prec", ["PREC_FP64", "PREC_FP32_IR", "PREC_FP32_PRECON"])
 def test_cone_krylov_equals_level_form(x, A, ctx, box48, monkeypatch, solver, prec):
     """Solutions to rtol 1e-12 agree to 1e-9 relative (the dot products are
     summed per block instead of per level launch); FP64 iteration counts
-    within 5% (CG) and 25% (BiCGSTAB)."""
+    within 5% (CG) and 25% (BiCGSTAB).  Under PREC_FP32_PRECON the cone
+    sweeps of CG compute in FP32 and the level form keeps FP64, so the two
+    differ by the preconditioner's rounding as well: CG's FP64 residual
+    recurrence still reaches the same solution.  BiCGSTAB keeps FP64 sweeps
+    under that mode."""
     H = x.hip
     sym = solver == "SOLVER_CG"
     res = []
@@ -122,10 +126,11 @@ def test_cone_krylov_equals_level_form(x, A, ctx, box48, monkeypatch, solver, pr
         assert abs(ia - ib) <= max(2, slack), (ia, ib)
 
 
-@pytest.mark.parametrize("prec", ["fp64", "fp32-ir"])
+@pytest.mark.parametrize("prec", ["fp64", "fp32-ir", "fp32-precon"])
 def test_cone_simple_channel_equals_level_form(x, ctx, box48, monkeypatch, prec):
     """Five SIMPLE iterations of the channel case (p: CG+DILU to 1e-10, U:
-    BiCGSTAB+DILU) with cone and level sweeps agree to 1e-9 in U and p."""
+    BiCGSTAB+DILU) with cone and level sweeps agree to 1e-9 in U and p
+    (under fp32-precon the level form's sweeps stay FP64)."""
     api = x.api
     out = []
     for env in ({}, {"XCAL_NO_CONE": "1"}):
@@ -138,3 +143,40 @@ def test_cone_simple_channel_equals_level_form(x, ctx, box48, monkeypatch, prec)
         out.append((sim.get("U"), sim.get("p")))
     assert rel(out[0][0], out[1][0]) < 1e-9
     assert rel(out[0][1], out[1][1]) < 1e-9
+
+
+def test_fp32_preconditioner_iterations_and_batch_rejection(x, A, ctx, box48, monkeypatch):
+    """The FP32 sweeps inside FP64 CG keep the Krylov space: the iteration
+    count to rtol 1e-10 stays within 10% of the FP64 preconditioner's, and
+    the solutions agree to 1e-8.  A batch mesh rejects the mode."""
+    H = x.hip
+    m = _mesh(x, ctx, box48, monkeypatch)
+    res = {}
+    for prec in ("PREC_FP64", "PREC_FP32_PRECON"):
+        eq = _equation(x, A, m, box48, np.random.default_rng(9), sym=True)
+        eq.precond_update(A.PRECON_DILU)
+        phi = H.Field(m, A.CELL_SCALAR)
+        setup = A.SolverSetupC(A.SOLVER_CG, A.PRECON_DILU, getattr(A, prec), 4000, 1e-7,
+                               1.0, 1e-30, 1e-10, 0.0)
+        it, r, rc = eq.solve(setup, phi)
+        assert rc == 0
+        res[prec] = (phi.download(), it)
+    (a, ia), (b, ib) = res["PREC_FP64"], res["PREC_FP32_PRECON"]
+    print(f"CG+DILU to 1e-10: FP64 sweeps {ia} iterations, FP32 sweeps {ib}")
+    assert rel(b, a) < 1e-8
+    assert abs(ib - ia) <= max(2, ia // 10), (ia, ib)
+    cav = [x.mesh.cavity2d(8), x.mesh.cavity2d(12)]
+    bm = H.DeviceMesh.batch(ctx, cav)
+    eq = H.Equation(bm, False)
+    g = H.Field(bm, A.FACE_SCALAR)
+    g.fill(1.0)
+    phi = H.Field(bm, A.CELL_SCALAR)
+    src = H.Field(bm, A.CELL_SCALAR)
+    src.fill(1.0)
+    terms = [(A.LAPLACIAN_LINEAR, -1, g)]
+    eq.discretise(terms, [(src, 1)], phi)
+    eq.precond_update(A.PRECON_DILU)
+    setup = A.SolverSetupC(A.SOLVER_CG, A.PRECON_DILU, A.PREC_FP32_PRECON, 100, 1e-7, 1.0,
+                           1e-30, 1e-8, 0.0)
+    with pytest.raises(H.XcalError, match="single meshes"):
+        eq.solve(setup, phi)

@@ -49,13 +49,15 @@ def test_channel_1m_matches_oracle(x, A, O, channel_mesh):
 def test_channel_1m_mixed_precision_within_1e8(x, channel_mesh):
     api = x.api
     res = {}
-    for prec in ("fp64", "fp32-ir"):
+    for prec in ("fp64", "fp32-ir", "fp32-precon"):
         model, config = bench.channel(api, channel_mesh, 3, precision=prec)
         res[prec] = _device_run(x, channel_mesh, model, config, 3)
-    (_, U64, p64), (_, U32, p32) = res["fp64"], res["fp32-ir"]
-    eU, ep = rel(U32, U64), rel(p32, p64)
-    print(f"1M channel, 3 iterations, FP32-IR vs FP64: U {eU:.2e} p {ep:.2e}")
-    assert eU < 1e-8 and ep < 1e-8
+    _, U64, p64 = res["fp64"]
+    for prec in ("fp32-ir", "fp32-precon"):
+        _, U32, p32 = res[prec]
+        eU, ep = rel(U32, U64), rel(p32, p64)
+        print(f"1M channel, 3 iterations, {prec} vs FP64: U {eU:.2e} p {ep:.2e}")
+        assert eU < 1e-8 and ep < 1e-8
 
 
 def test_batch_256_cases_alone_equal_mix(x):

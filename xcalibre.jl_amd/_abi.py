@@ -15,7 +15,7 @@ BC_KWALL, BC_OMEGAWALL, BC_NUTWALL, BC_EMPTY = 6, 7, 8, 9
 # solvers / preconditioners / precision
 SOLVER_CG, SOLVER_BICGSTAB = 0, 1
 PRECON_NONE, PRECON_JACOBI, PRECON_DILU = 0, 1, 2
-PREC_FP64, PREC_FP32_IR = 0, 1
+PREC_FP64, PREC_FP32_IR, PREC_FP32_PRECON = 0, 1, 2
 # field kinds
 CELL_SCALAR, CELL_VECTOR, FACE_SCALAR, FACE_VECTOR, CELL_TENSOR = range(5)
 # status

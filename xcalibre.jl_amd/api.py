@@ -109,7 +109,9 @@ class SolverSetup:
     itmax: int = 1000
     atol: Optional[float] = None
     rtol: float = 0.1
-    precision: str = "fp64"      # "fp64" or "fp32-ir" (FP32 inner Krylov, FP64 refinement)
+    # "fp64"; "fp32-ir" (FP32 inner Krylov, FP64 refinement); "fp32-precon"
+    # (FP64 Krylov with the DILU sweeps in FP32, single meshes in cone form)
+    precision: str = "fp64"
     inner_rtol: float = 1e-6     # fp32-ir: relative tolerance of each inner solve
 
     def __post_init__(self):
@@ -126,7 +128,8 @@ class SolverSetup:
             type(self.preconditioner))
         if pre is None:
             raise NotImplementedError(f"preconditioner {self.preconditioner!r} is not supported")
-        prec = {"fp64": A.PREC_FP64, "fp32-ir": A.PREC_FP32_IR}[self.precision]
+        prec = {"fp64": A.PREC_FP64, "fp32-ir": A.PREC_FP32_IR,
+                "fp32-precon": A.PREC_FP32_PRECON}[self.precision]
         sm = self.smoother
         return A.SolverSetupC(solver, pre, prec, int(self.itmax), float(self.convergence),
                               float(self.relax), float(self.atol), float(self.rtol),

@@ -481,21 +481,23 @@ __global__ void __launch_bounds__(kPipeBlock)
 }
 
 // Cone form (sweep_cone.h): one workgroup per block, grid = the mesh's
-// block count.  Both return at entry once the solve has stopped.
-template <bool UPD, class T, int WK, int P>
+// block count.  Both return at entry once the solve has stopped.  The sweep
+// computes in T over vectors of type V; Yt is the forward output in T.
+template <bool UPD, class T, int WK, int P, class V>
 __global__ void __launch_bounds__(kConeBlock)
-    k_fwd_cone(ConeK C, const T* __restrict__ a, const T* __restrict__ rd, FwdK<T> W) {
+    k_fwd_cone(ConeK C, const T* __restrict__ a, const T* __restrict__ rd, FwdK<V> W, T* Yt) {
   __shared__ ConeLDS<T> L;
   if (W.ks && stopped(W.ks)) return;  // uniform over the workgroup
-  fwd_cone<UPD, T, WK, P>(C, a, rd, W, L);
+  fwd_cone<UPD, T, WK, P, V>(C, a, rd, W, Yt, L);
 }
 
-template <int DOT, class T, int WK, int P>
+template <int DOT, class T, int WK, int P, class V>
 __global__ void __launch_bounds__(kConeBlock)
-    k_bwd_cone(ConeK C, const T* __restrict__ a, const T* __restrict__ d, BwdK<T> W) {
+    k_bwd_cone(ConeK C, const T* __restrict__ a, const T* __restrict__ d, BwdK<V> W,
+               const T* Yt) {
   __shared__ ConeLDS<T> L;
   if (W.ks && stopped(W.ks)) return;
-  bwd_cone<DOT, T, WK, P>(C, a, d, W, L);
+  bwd_cone<DOT, T, WK, P, V>(C, a, d, W, Yt, L);
 }
 
 inline ConeK cone_k(const xcal_mesh* m, const xcal_mesh::Cone& c) {
@@ -517,6 +519,10 @@ struct Solver {
   const T *Lnz, *Unz, *Db;  // packed DILU factors, D in backward-level order
   const T* Lrd;             // 1/D per packed lower entry (pipelined runs)
   const T *cfa, *cfr, *cba, *cbd;  // cone form: packed a_ij, 1/D_j, a_ij, D
+  // FP32 preconditioner of an FP64 solve (XCAL_PREC_FP32_PRECON, cone form):
+  // folded FP32 factors a_ij / D_j, a_ij and 1 / D of the cone sweeps
+  bool pc32 = false;
+  const float *cfa32 = nullptr, *cba32 = nullptr, *cbd32 = nullptr;
   T* work;
   KS* ks;
   double* partial;
@@ -586,6 +592,26 @@ static Solver<double> make_solver(xcal_eqn* e) {
   return S;
 }
 
+// The FP32 cone sweeps' factors, folded (cone_folded) from the FP64 ones:
+// a_ij / D_j per forward entry, a_ij per backward entry, 1 / D per backward
+// position.
+static void cone_factors32(xcal_eqn* e) {
+  hipStream_t s = e->mesh->ctx->stream;
+  if (e->cfa32.n != e->cfa.n) e->cfa32.alloc(e->cfa.n);
+  if (e->cfa.n > 0)
+    hipLaunchKernelGGL(k_fold_float, dim3(grid_for(e->cfa.n)), dim3(kBlock), 0, s, e->cfa.n,
+                       (const double*)e->cfa.p, (const double*)e->cfr.p, e->cfa32.p);
+  if (e->cba32.n != e->cba.n) e->cba32.alloc(e->cba.n);
+  if (e->cba.n > 0)
+    hipLaunchKernelGGL(k_to_float, dim3(grid_for(e->cba.n)), dim3(kBlock), 0, s, e->cba.n,
+                       e->cba.p, e->cba32.p);
+  if (e->cbd32.n != e->cbd.n) e->cbd32.alloc(e->cbd.n);
+  if (e->cbd.n > 0)
+    hipLaunchKernelGGL(k_recip_float, dim3(grid_for(e->cbd.n)), dim3(kBlock), 0, s, e->cbd.n,
+                       (const double*)e->cbd.p, e->cbd32.p);
+  XCAL_HIP(hipGetLastError());
+}
+
 // FP32 copies of the matrix and the preconditioner, made from the FP64 ones
 static Solver<float> make_solver32(xcal_eqn* e) {
   ensure_workspace(e, true);
@@ -615,23 +641,7 @@ static Solver<float> make_solver32(xcal_eqn* e) {
                          e->Lrd.p, e->Lrd32.p);
     }
   } else if (e->precon == XCAL_PRECON_DILU) {  // the cone sweeps' packed factors
-    auto conv = [&](const DBuf<double>& src, DBuf<float>& dst) {
-      if (dst.n != src.n) dst.alloc(src.n);
-      if (src.n > 0)
-        hipLaunchKernelGGL(k_to_float, dim3(grid_for(src.n)), dim3(kBlock), 0, s, src.n, src.p,
-                           dst.p);
-    };
-    // folded factors of the FP32 cone sweeps (cone_folded): a_ij / D_j per
-    // forward entry, 1 / D per backward position
-    if (e->cfa32.n != e->cfa.n) e->cfa32.alloc(e->cfa.n);
-    if (e->cfa.n > 0)
-      hipLaunchKernelGGL(k_fold_float, dim3(grid_for(e->cfa.n)), dim3(kBlock), 0, s, e->cfa.n,
-                         (const double*)e->cfa.p, (const double*)e->cfr.p, e->cfa32.p);
-    conv(e->cba, e->cba32);
-    if (e->cbd32.n != e->cbd.n) e->cbd32.alloc(e->cbd.n);
-    if (e->cbd.n > 0)
-      hipLaunchKernelGGL(k_recip_float, dim3(grid_for(e->cbd.n)), dim3(kBlock), 0, s, e->cbd.n,
-                         (const double*)e->cbd.p, e->cbd32.p);
+    cone_factors32(e);
   } else if (e->precon == XCAL_PRECON_JACOBI) {
     hipLaunchKernelGGL(k_to_float, dim3(grid_for(n)), dim3(kBlock), 0, s, n, e->pS.p, e->pS32.p);
   }
@@ -727,9 +737,14 @@ static void sweep_fwd(const Solver<T>& S, const FwdK<T>& W) {
     XCAL_CHECK(W.Yb != (UPD ? W.r : W.u) && (!UPD || (W.rout != W.r && W.rout != W.Yb)),
                "internal: cone sweep operands alias");
     with_cone_form(m->cone_f.wk, !m->cone_deep, [&](auto WK, auto P) {
-      hipLaunchKernelGGL((k_fwd_cone<UPD, T, decltype(WK)::value, decltype(P)::value>),
-                         dim3(m->cone_nblk), dim3(kConeBlock), 0, S.s, cone_k(m, m->cone_f), S.cfa,
-                         S.cfr, W);
+      constexpr int wk = decltype(WK)::value, p = decltype(P)::value;
+      if (std::is_same<T, double>::value && S.pc32)  // y in float, in the Yb vector's storage
+        hipLaunchKernelGGL((k_fwd_cone<UPD, float, wk, p, T>), dim3(m->cone_nblk),
+                           dim3(kConeBlock), 0, S.s, cone_k(m, m->cone_f), S.cfa32, S.cfa32, W,
+                           reinterpret_cast<float*>(W.Yb));
+      else
+        hipLaunchKernelGGL((k_fwd_cone<UPD, T, wk, p, T>), dim3(m->cone_nblk), dim3(kConeBlock),
+                           0, S.s, cone_k(m, m->cone_f), S.cfa, S.cfr, W, W.Yb);
     });
     return;
   }
@@ -771,9 +786,14 @@ static void sweep_bwd(const Solver<T>& S, const BwdK<T>& W) {
   if (m->cone) {  // y from W.Yb in device order; partials in slots [0, cone_nblk)
     XCAL_CHECK(W.Z != W.Yb && (DOT == 0 || W.dv != W.Z), "internal: cone sweep operands alias");
     with_cone_form(m->cone_b.wk, !m->cone_deep, [&](auto WK, auto P) {
-      hipLaunchKernelGGL((k_bwd_cone<DOT, T, decltype(WK)::value, decltype(P)::value>),
-                         dim3(m->cone_nblk), dim3(kConeBlock), 0, S.s, cone_k(m, m->cone_b), S.cba,
-                         S.cbd, W);
+      constexpr int wk = decltype(WK)::value, p = decltype(P)::value;
+      if (std::is_same<T, double>::value && S.pc32)
+        hipLaunchKernelGGL((k_bwd_cone<DOT, float, wk, p, T>), dim3(m->cone_nblk),
+                           dim3(kConeBlock), 0, S.s, cone_k(m, m->cone_b), S.cba32, S.cbd32, W,
+                           reinterpret_cast<const float*>(W.Yb));
+      else
+        hipLaunchKernelGGL((k_bwd_cone<DOT, T, wk, p, T>), dim3(m->cone_nblk), dim3(kConeBlock),
+                           0, S.s, cone_k(m, m->cone_b), S.cba, S.cbd, W, W.Yb);
     });
     return;
   }
@@ -1175,13 +1195,17 @@ void solve_epilogue(xcal_eqn* e, double* x) {
 
 int solve_system(xcal_eqn* e, const xcal_solver_setup* st, double* x, const double* b,
                  xcal_solve_stats* stats) {
-  XCAL_CHECK(st->precision == XCAL_PREC_FP64 || st->precision == XCAL_PREC_FP32_IR,
+  XCAL_CHECK(st->precision == XCAL_PREC_FP64 || st->precision == XCAL_PREC_FP32_IR ||
+                 st->precision == XCAL_PREC_FP32_PRECON,
              "unknown precision mode");
   XCAL_CHECK(st->solver == XCAL_SOLVER_CG || st->solver == XCAL_SOLVER_BICGSTAB,
              "unknown Krylov solver");
   XCAL_CHECK(st->precon == e->precon,
              "preconditioner of the solver setup has not been updated on this equation");
   if (e->mesh->batch) {  // batch mesh: the worst case is reported
+    if (st->precision == XCAL_PREC_FP32_PRECON)
+      throw Error(XCAL_ERR_UNSUPPORTED,
+                  "the FP32 preconditioner form runs on single meshes; batch meshes take FP32_IR");
     std::vector<xcal_solve_stats> cs(e->mesh->ncase);
     const int rc = solve_system_batch(e, st, x, b, cs.data());
     if (stats) {
@@ -1201,8 +1225,22 @@ int solve_system(xcal_eqn* e, const xcal_solver_setup* st, double* x, const doub
     rc = ir_solve(e, st, itmax, x, b, niter, nouter);
   } else {
     Solver<double> S = make_solver(e);
-    S.cg_batch = e->pred_slot(x, 0);
-    S.bi_batch = e->pred_slot(x, 1);
+    // CG only: its residual recurrence r -= alpha A p uses the FP64 matrix,
+    // so FP32 sweeps change the search directions and not the residual.
+    // Left-preconditioned BiCGSTAB updates M^-1 r through the sweeps, where
+    // FP32 rounding would move the residual itself (1.7e-8 in the solution
+    // at rtol 1e-12 on a 110 592-cell box), so it keeps FP64 sweeps.
+    const bool pc32 = st->precision == XCAL_PREC_FP32_PRECON;
+    if (pc32 && st->solver == XCAL_SOLVER_CG && st->precon == XCAL_PRECON_DILU &&
+        e->mesh->cone) {
+      cone_factors32(e);
+      S.pc32 = true;
+      S.cfa32 = e->cfa32.p;
+      S.cba32 = e->cba32.p;
+      S.cbd32 = e->cbd32.p;
+    }
+    S.cg_batch = e->pred_slot(x, pc32 ? 2 : 0);
+    S.bi_batch = e->pred_slot(x, pc32 ? 3 : 1);
     if (st->solver == XCAL_SOLVER_CG)
       rc = cg(S, st->precon, b, x, itmax, st->atol, st->rtol, niter, (double**)nullptr);
     else

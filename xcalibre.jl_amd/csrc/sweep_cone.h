@@ -91,18 +91,21 @@ __device__ inline void cone_fetch_pos(const ConeK& C, const ConeStep& S, int32_t
   }
 }
 
-template <class T, int WK>
+template <class T, int WK, class V>
 struct CFRec {
   int32_t c, lp;  // crow entry (row or ~row), cone-local position
   bool rep;       // a repeat of the step's last row
-  T ro, qo;       // gathered: UPD r and q of the row; otherwise ro = u
+  V ro, qo;       // gathered: UPD r and q of the row; otherwise ro = u
   int32_t li[WK];
   T av[WK], rv[WK];
 };
 
-// Forward sweep y = (I + L D^-1)^-1 u (fwd_row's arithmetic): y to W.Yb in
+// Forward sweep y = (I + L D^-1)^-1 u (fwd_row's arithmetic): y to Yt in
 // device order (the cone backward sweep reads it there); a and rd hold a_ij
-// and 1/D_j per cone entry.  UPD fuses the CG update: u = r - alpha q, stored
+// and 1/D_j per cone entry.  The vectors of W have type V and the sweep
+// computes in T: V = double with T = float is the FP32 preconditioner of an
+// FP64 Krylov solve (u is rounded to float as the sweep reads it, y stays
+// float); V = T otherwise, with Yt = W.Yb.  UPD fuses the CG update: u = r - alpha q, stored
 // to W.rout (other cones read the old r of every row during the sweep), and
 // x += alpha p over the block's own rows after the sweep.
 // The caller returns before the sweep once the solve has stopped, so every
@@ -117,14 +120,14 @@ constexpr bool cone_folded() {
   return std::is_same<T, float>::value;
 }
 
-template <bool UPD, class T, int WK, int P>
+template <bool UPD, class T, int WK, int P, class V = T>
 __device__ inline void fwd_cone(const ConeK& C, const T* __restrict__ a, const T* __restrict__ rd,
-                                const FwdK<T>& W, ConeLDS<T>& L) {
+                                const FwdK<V>& W, T* __restrict__ Yt, ConeLDS<T>& L) {
   const int b = (int)xcd_block();
   int32_t base;
   const int ns = cone_stage(C, b, L.tab, base);
-  const T al = UPD && W.ks ? (T)W.ks->alpha : T(0);
-  auto fetch = [&](CFRec<T, WK>& R, int s) {
+  const V al = UPD && W.ks ? (V)W.ks->alpha : V(0);
+  auto fetch = [&](CFRec<T, WK, V>& R, int s) {
     int64_t e[WK];
     cone_fetch_pos<WK>(C, L.tab[s < ns ? s : ns - 1], base, R.c, R.lp, R.rep, R.li, e);
 #pragma unroll
@@ -133,7 +136,7 @@ __device__ inline void fwd_cone(const ConeK& C, const T* __restrict__ a, const T
       if (!cone_folded<T>()) R.rv[k] = rd[e[k]];
     }
   };
-  auto gather = [&](CFRec<T, WK>& R) {
+  auto gather = [&](CFRec<T, WK, V>& R) {
     const int32_t i = R.c >= 0 ? R.c : ~R.c;
     if (UPD) {
       R.ro = W.r[i];
@@ -142,7 +145,7 @@ __device__ inline void fwd_cone(const ConeK& C, const T* __restrict__ a, const T
       R.ro = W.u[i];
     }
   };
-  auto process = [&](const CFRec<T, WK>& R) {
+  auto process = [&](const CFRec<T, WK, V>& R) {
     T sum = 0;
 #pragma unroll
     for (int k = 0; k < WK; ++k) {
@@ -151,21 +154,21 @@ __device__ inline void fwd_cone(const ConeK& C, const T* __restrict__ a, const T
       const T term = cone_folded<T>() ? R.av[k] * yv : (R.av[k] * yv) * R.rv[k];
       sum = li >= 0 ? sum + term : sum;
     }
-    const T u = UPD ? R.ro + (-al) * R.qo : R.ro;
-    const T y = u - sum;
+    const V u = UPD ? R.ro + (-al) * R.qo : R.ro;
+    const T y = (T)u - sum;
     L.v[R.lp] = y;
     const int32_t i = R.c >= 0 ? R.c : ~R.c;
-    W.Yb[i] = y;
+    Yt[i] = y;
     if (UPD) W.rout[i] = u;
   };
-  CFRec<T, WK> buf[P];
+  CFRec<T, WK, V> buf[P];
   pipe_levels<P>(ns, buf, fetch, gather, process);
   if (UPD) {  // x += alpha p over the own rows, U rows per thread at a time
     constexpr int U = 8;
     const int32_t o0 = b * C.rows, o1 = min(C.n, o0 + C.rows);
     for (int32_t j0 = o0 + (int32_t)threadIdx.x; j0 < o1; j0 += U * (int32_t)blockDim.x) {
       int32_t ii[U];
-      T xv[U], pv[U];
+      V xv[U], pv[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const int32_t j = j0 + u * (int32_t)blockDim.x;
@@ -183,28 +186,31 @@ __device__ inline void fwd_cone(const ConeK& C, const T* __restrict__ a, const T
   }
 }
 
-template <class T, int WK>
+template <class T, int WK, class V>
 struct CBRec {
   int32_t c, lp;
   bool rep;
-  T d, yb, dvi;
+  T d, yb;
+  V dvi;
   int32_t li[WK];
   T av[WK];
 };
 
 // Backward sweep z = (D + U)^-1 y (bwd_row's arithmetic) with up to two dot
 // products over the block's own rows (DOT as bwd_row), stored as the
-// block's partial pair.  y is read from W.Yb in device order; a holds a_ij
-// per cone entry, d D per cone position.  dv must not alias Z (another
+// block's partial pair.  y is read from Yt in device order; a holds a_ij
+// per cone entry, d D per cone position.  Types as for the forward sweep: z
+// is computed in T and stored to W.Z in V, and the dot products take the
+// stored value.  dv must not alias Z (another
 // workgroup may store a row's z before the row's owner reads its dv).  As
 // for the forward sweep, the caller returns once the solve has stopped.
-template <int DOT, class T, int WK, int P>
+template <int DOT, class T, int WK, int P, class V = T>
 __device__ inline void bwd_cone(const ConeK& C, const T* __restrict__ a, const T* __restrict__ d,
-                                const BwdK<T>& W, ConeLDS<T>& L) {
+                                const BwdK<V>& W, const T* __restrict__ Yt, ConeLDS<T>& L) {
   const int b = (int)xcd_block();
   int32_t base;
   const int ns = cone_stage(C, b, L.tab, base);
-  auto fetch = [&](CBRec<T, WK>& R, int s) {
+  auto fetch = [&](CBRec<T, WK, V>& R, int s) {
     const ConeStep& S = L.tab[s < ns ? s : ns - 1];
     int64_t e[WK];
     cone_fetch_pos<WK>(C, S, base, R.c, R.lp, R.rep, R.li, e);
@@ -212,13 +218,13 @@ __device__ inline void bwd_cone(const ConeK& C, const T* __restrict__ a, const T
 #pragma unroll
     for (int k = 0; k < WK; ++k) R.av[k] = a[e[k]];
   };
-  auto gather = [&](CBRec<T, WK>& R) {
+  auto gather = [&](CBRec<T, WK, V>& R) {
     const int32_t i = R.c >= 0 ? R.c : ~R.c;
-    R.yb = W.Yb[i];
-    R.dvi = DOT != 0 ? W.dv[i] : T(0);
+    R.yb = Yt[i];
+    R.dvi = DOT != 0 ? W.dv[i] : V(0);
   };
   double s0 = 0.0, s1 = 0.0;
-  auto process = [&](const CBRec<T, WK>& R) {
+  auto process = [&](const CBRec<T, WK, V>& R) {
     T sum = 0;
 #pragma unroll
     for (int k = 0; k < WK; ++k) {
@@ -227,9 +233,10 @@ __device__ inline void bwd_cone(const ConeK& C, const T* __restrict__ a, const T
       const T term = R.av[k] * zv;
       sum = li >= 0 ? sum + term : sum;
     }
-    const T zi = cone_folded<T>() ? (R.yb - sum) * R.d : (R.yb - sum) / R.d;
-    L.v[R.lp] = zi;
+    const T zt = cone_folded<T>() ? (R.yb - sum) * R.d : (R.yb - sum) / R.d;
+    L.v[R.lp] = zt;
     const int32_t i = R.c >= 0 ? R.c : ~R.c;
+    const V zi = (V)zt;
     W.Z[i] = zi;
     const bool own = R.c >= 0 && !R.rep;
     double a0 = 0.0, a1 = 0.0;
@@ -239,7 +246,7 @@ __device__ inline void bwd_cone(const ConeK& C, const T* __restrict__ a, const T
     s0 = own ? s0 + a0 : s0;
     s1 = own ? s1 + a1 : s1;
   };
-  CBRec<T, WK> buf[P];
+  CBRec<T, WK, V> buf[P];
   pipe_levels<P>(ns, buf, fetch, gather, process);
   if (DOT != 0) store_partial<kConeBlock>(s0, s1, W.partial, b);
 }

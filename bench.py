@@ -146,7 +146,8 @@ def serve_leg(args, x, requests, rate, max_iterations, alone=True):
     rng = np.random.default_rng(SEED_SERVE + 1)
     gaps = rng.exponential(1.0 / rate, requests)
     budget = int(args.hbm_gb * (1 << 30))
-    srv = H.Server(0, slots=args.slots, hbm_budget=budget, max_iterations=max_iterations)
+    srv = H.Server(0, slots=args.slots, hbm_budget=budget, max_iterations=max_iterations,
+                   lanes=args.lanes)
     ids, submit_s = [], []
     t0 = time.perf_counter()
     for (mesh, model, config), gap in zip(cases, gaps):
@@ -204,7 +205,7 @@ def serve_leg(args, x, requests, rate, max_iterations, alone=True):
                     + ("laminar (nu 0.05" if args.laminar else "k-omega RANS (nu 1e-5")
                     + f", U 10), {args.precon}, Poisson arrivals at {rate}/s, residual 1e-6, "
                     f"at most {max_iterations} iterations",
-        "slots": args.slots, "hbm_budget_gb": args.hbm_gb,
+        "slots": args.slots, "lanes": args.lanes, "hbm_budget_gb": args.hbm_gb,
         "converged": int(conv.sum()), "capped": int(np.sum(ok & ~conv)),
         "diverged": int(np.sum(~ok)),
         "p50_time_to_converged_s": pct(lat[conv], 50),
@@ -222,6 +223,7 @@ def serve_leg(args, x, requests, rate, max_iterations, alone=True):
         "identical_alone_vs_mix": same if alone_s else None,
         "iterations": its.tolist(), "cells": ncell.tolist(),
         "converged_flags": conv.astype(int).tolist(),
+        "lane_of_request": [s["lane"] for s in st],
     }
     return float(np.sum(ncell * its)) / wall, summary
 
@@ -653,6 +655,9 @@ def main():
                     help="channel mode: requests of the serving summary (detail.serving)")
     ap.add_argument("--rate", type=float, default=2.0, help="serve mode: arrivals per second")
     ap.add_argument("--slots", type=int, default=4, help="serve mode: concurrent slots")
+    ap.add_argument("--lanes", type=int, default=2,
+                    help="serving: batches stepped concurrently (a case joins the lane "
+                         "closest to it in size)")
     ap.add_argument("--hbm-gb", type=float, default=64.0, help="serve mode: HBM budget (GiB)")
     ap.add_argument("--max-iterations", type=int, default=1000,
                     help="serve mode: SIMPLE iterations per request at most")

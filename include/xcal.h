@@ -473,12 +473,16 @@ int xcal_simple_stage_times(const xcal_simple* s, double* ms6);
 /* Serving (no single reference counterpart: it runs many independent
  * run!(model, config) cases).  Requests are built at submission (the matrix
  * connectivity on the device, as xcal_connectivity_create; the rest on the
- * host) and wait in a FIFO queue.  The cases in flight form one batch mesh and are
- * stepped together, every launch of a SIMPLE iteration covering all of them
- * (continuous batching).  A waiting case joins when its device footprint fits
- * the HBM budget left by the cases in flight, the batch holds fewer than
- * `slots` cases and its settings (SIMPLE, turbulence, boundary conditions)
- * equal the batch's.  A case leaves after the iteration in which it converges
+ * host) and wait in a FIFO queue.  Each of `lanes` scheduler threads (one by
+ * default, each on its own stream) keeps its cases in flight as one batch
+ * mesh and steps them together, every launch of a SIMPLE iteration covering
+ * all of them (continuous batching); the lanes' launches overlap on the
+ * device.  The queue head joins when its device footprint fits the HBM
+ * budget left by the cases in flight and fewer than `slots` cases are in
+ * flight over all lanes.  It joins the lane whose cases are closest to it
+ * in size (largest case within a factor 4, else an empty lane), among lanes
+ * that are empty or whose settings (SIMPLE, turbulence, boundary
+ * conditions) equal its own.  A case leaves after the iteration in which it converges
  * (the run!/simple! convergence test), reaches max_iterations, or shows a
  * non-finite residual (failed); its fields are then kept on the host until
  * released.  A case's result does not depend on which other cases are in
@@ -489,6 +493,11 @@ typedef struct {
   int32_t slots;          /* most cases in flight together, 1..256 */
   int64_t hbm_budget;     /* bytes of device memory the cases may occupy */
   int32_t max_iterations; /* SIMPLE iterations per case at most */
+  /* batches stepped concurrently, 0..16 (0: one).  Each lane has its own
+   * stream, as has the submission path: past the process's hardware queues
+   * (GPU_MAX_HW_QUEUES, 4 by default) streams share queues, and 4 lanes
+   * measured slower than 2. */
+  int32_t lanes;
 } xcal_server_config;
 enum { XCAL_REQ_QUEUED = 0, XCAL_REQ_RUNNING = 1, XCAL_REQ_DONE = 2, XCAL_REQ_FAILED = 3 };
 typedef struct {
@@ -498,7 +507,7 @@ typedef struct {
   int64_t footprint;    /* admitted device bytes (estimate) */
   double t_submit, t_start, t_end; /* seconds since server creation */
   int32_t converged;    /* finished on the convergence test (not max_iterations) */
-  int32_t unused;
+  int32_t lane;         /* the lane that admitted it (-1 while queued) */
   int64_t batch_rebuilds; /* batches the server has formed so far */
 } xcal_request_status;
 int xcal_server_create(const xcal_server_config* cfg, xcal_server** out);

@@ -155,3 +155,28 @@ def test_serve_rejects_oversized_case(x):
     with pytest.raises(Exception, match="HBM budget"):
         _submit(srv, x, case)
     srv.close()
+
+
+def test_serve_lanes_group_cases_by_size(x):
+    """With two lanes, cases join the lane closest to them in size: the two
+    60k-cell cases share one lane, the two 4.5k-cell cases the other, the
+    lanes step concurrently, and every case equals its alone run bit for
+    bit."""
+    cases = [_case(x, 0.4, 60000, iters=10), _case(x, 0.55, 4500, iters=10),
+             _case(x, 0.35, 56000, iters=10), _case(x, 0.5, 4800, iters=10)]
+    alone = _alone(x, cases, iters=10)
+    srv = x.hip.Server(0, slots=4, max_iterations=10, lanes=2)
+    ids = [_submit(srv, x, c) for c in cases]
+    assert srv.wait(0, 300.0)
+    st = [srv.status(i) for i in ids]
+    lanes = [s["lane"] for s in st]
+    assert lanes[0] == lanes[2] and lanes[1] == lanes[3] and lanes[0] != lanes[1], lanes
+    # the lanes were in flight at the same time
+    span = {l: (min(s["t_start"] for s in st if s["lane"] == l),
+                max(s["t_end"] for s in st if s["lane"] == l)) for l in set(lanes)}
+    (a0, a1), (b0, b1) = span.values()
+    assert a0 < b1 and b0 < a1, span
+    for rid, (ref, _) in zip(ids, alone):
+        assert srv.status(rid)["iterations"] == 10
+        _same(srv.fetch(rid), ref)
+    srv.close()

@@ -85,13 +85,13 @@ GRAD_GAUSS, GRAD_MIDPOINT = 0, 1
 
 class ServerConfigC(C.Structure):
     _fields_ = [("device", C.c_int32), ("slots", C.c_int32), ("hbm_budget", C.c_int64),
-                ("max_iterations", C.c_int32)]
+                ("max_iterations", C.c_int32), ("lanes", C.c_int32)]
 
 
 class RequestStatusC(C.Structure):
     _fields_ = [("state", C.c_int32), ("iterations", C.c_int32), ("cells", C.c_int64),
                 ("footprint", C.c_int64), ("t_submit", C.c_double), ("t_start", C.c_double),
-                ("t_end", C.c_double), ("converged", C.c_int32), ("unused", C.c_int32),
+                ("t_end", C.c_double), ("converged", C.c_int32), ("lane", C.c_int32),
                 ("batch_rebuilds", C.c_int64)]
 
 

@@ -1,11 +1,13 @@
 // Serving engine: many independent steady SIMPLE cases on one device with
 // continuous batching.  A case (request) is built when it is submitted (its
 // matrix connectivity on the device, the rest on the host) and waits in a
-// FIFO queue.  One scheduler thread keeps the cases
-// in flight as one batch mesh (the union of their meshes, each case in its
-// own numbering) and steps them together: every kernel launch of a SIMPLE
-// iteration covers all of them, and each case's Krylov solves stop on their
-// own.  After every iteration each case is tested on its own: a converged
+// FIFO queue.  Each lane (a scheduler thread with its own stream) keeps its
+// cases in flight as one batch mesh (the union of their meshes, each case in
+// its own numbering) and steps them together: every kernel launch of a
+// SIMPLE iteration covers all of them, and each case's Krylov solves stop on
+// their own.  A batch iteration lasts as long as its slowest case's solves,
+// so with several lanes a case joins the lane whose cases are closest to it
+// in size, and the lanes' launches overlap on the device.  After every iteration each case is tested on its own: a converged
 // case (the run!/simple! test), one with a non-finite residual, or one at its
 // iteration limit leaves the batch with its fields copied back.  Waiting
 // cases join as soon as their device footprint fits the HBM budget left by
@@ -16,6 +18,7 @@
 // rows, per-case reductions, deterministic kernels), so its result is the same
 // alone and in the mix.
 #include <chrono>
+#include <cmath>
 #include <condition_variable>
 #include <cstdlib>
 #include <cstring>
@@ -43,6 +46,7 @@ struct Request {
   std::vector<double> f0[6], f[6];
   int64_t footprint = 0, cells = 0;
   int32_t state = XCAL_REQ_QUEUED, iterations = 0, converged = 0;
+  int32_t lane = -1;  // the lane that admitted it
   double t_submit = 0, t_start = 0, t_end = 0;
   std::string error;
 };
@@ -77,7 +81,8 @@ struct xcal_server {
   // XCAL_SERVE_REBUILD=1 at creation: rebuild the batch after every iteration
   // (moving every case's state each time; a test of the state transfer)
   bool rebuild_always = false;
-  std::thread scheduler;
+  int nlanes = 1;
+  std::vector<std::thread> lanes;  // one scheduler thread per lane
   // mesh ingest at submission: the CSR pattern, slot maps and DILU levels of
   // a request's mesh are built on the device, on this context's stream (one
   // submission at a time)
@@ -168,7 +173,41 @@ void rebuild(xcal_ctx* ctx, Batch& old, const std::vector<std::shared_ptr<Reques
   old = std::move(nb);
 }
 
-void scheduler(xcal_server* S) {
+// A lane whose largest case differs from a request by this size factor or
+// more is no better for it than an empty lane.
+constexpr double kLaneSpread = 4.0;
+
+// The lane that request q joins (S->mu held): among the empty lanes and the
+// lanes whose cases share q's settings, the one whose largest running case
+// is closest to q in size (|log| of the cell ratio; an empty lane counts as
+// log kLaneSpread), the lowest index on a tie; -1 when none can take it.
+int best_lane(const xcal_server* S, const Request& q) {
+  const int nl = S->nlanes;
+  std::vector<int64_t> big(nl, 0);
+  std::vector<const Request*> first(nl, nullptr);
+  for (const auto& kv : S->reqs) {
+    const Request& r = *kv.second;
+    if (r.state != XCAL_REQ_RUNNING || r.lane < 0) continue;
+    big[r.lane] = std::max(big[r.lane], r.cells);
+    if (!first[r.lane]) first[r.lane] = &r;
+  }
+  int best = -1;
+  double bd = INFINITY;
+  for (int l = 0; l < nl; ++l) {
+    double d = std::log(kLaneSpread);
+    if (first[l]) {
+      if (!compatible(*first[l], q)) continue;
+      d = std::fabs(std::log((double)q.cells / (double)big[l]));
+    }
+    if (d < bd) {
+      bd = d;
+      best = l;
+    }
+  }
+  return best;
+}
+
+void scheduler(xcal_server* S, int lane) {
   xcal_ctx* ctx = nullptr;
   const bool ok = xcal_ctx_create(S->cfg.device, &ctx) == 0;
   Batch B;
@@ -182,13 +221,8 @@ void scheduler(xcal_server* S) {
         if (S->stop || S->queue.empty()) return false;
         const Request& q = *S->reqs[S->queue.front()];
         if (q.footprint > S->cfg.hbm_budget - S->in_use) return false;
-        int64_t live = (int64_t)join.size();
-        for (const auto& r : B.cases) live += r->state == XCAL_REQ_RUNNING;
-        if (live >= S->cfg.slots) return false;
-        const Request* first = !B.cases.empty() ? B.cases[0].get()
-                               : !join.empty()  ? join[0].get()
-                                                : nullptr;
-        return !first || compatible(*first, q) || !ok;
+        if (S->running >= S->cfg.slots) return false;  // cases in flight over all lanes
+        return best_lane(S, q) == lane;  // a lane without a context fails what it takes
       };
       if (B.cases.empty() && !dirty)
         S->cv.wait(lk, [&] { return S->stop || admissible(); });
@@ -199,9 +233,12 @@ void scheduler(xcal_server* S) {
         S->in_use += r->footprint;
         S->running += 1;
         r->state = XCAL_REQ_RUNNING;
+        r->lane = lane;
         r->t_start = S->now();
         join.push_back(r);
       }
+      // the new queue head may be another lane's
+      if (!join.empty() && !S->queue.empty()) S->cv.notify_all();
     }
     std::vector<std::shared_ptr<Request>> failed;
     std::string err;
@@ -212,6 +249,7 @@ void scheduler(xcal_server* S) {
         for (const auto& r : B.cases)
           if (r->state == XCAL_REQ_RUNNING) keep.push_back(r);
         rebuild(ctx, B, keep, join);
+        std::lock_guard<std::mutex> lk(S->mu);
         ++S->rebuilds;
       } catch (const std::exception& e) {  // every case of the failed batch fails
         err = e.what();
@@ -292,6 +330,7 @@ int xcal_server_create(const xcal_server_config* cfg, xcal_server** out) {
     XCAL_CHECK(cfg->slots >= 1 && cfg->slots <= 256, "slots must be 1..256");
     XCAL_CHECK(cfg->hbm_budget > 0, "HBM budget must be positive");
     XCAL_CHECK(cfg->max_iterations >= 1, "max_iterations must be >= 1");
+    XCAL_CHECK(cfg->lanes >= 0 && cfg->lanes <= 16, "lanes must be 0..16");
     int ndev = 0;
     XCAL_HIP(hipGetDeviceCount(&ndev));
     XCAL_CHECK(cfg->device >= 0 && cfg->device < ndev, "device index out of range");
@@ -300,7 +339,8 @@ int xcal_server_create(const xcal_server_config* cfg, xcal_server** out) {
     check_rc(xcal_ctx_create(cfg->device, &S->ingest));
     const char* rb = std::getenv("XCAL_SERVE_REBUILD");
     S->rebuild_always = rb && *rb == '1';
-    S->scheduler = std::thread(scheduler, S.get());
+    S->nlanes = std::max(1, (int)cfg->lanes);
+    for (int l = 0; l < S->nlanes; ++l) S->lanes.emplace_back(scheduler, S.get(), l);
     *out = S.release();
     return 0;
   });
@@ -314,7 +354,7 @@ int xcal_server_destroy(xcal_server* S) {
       S->stop = true;
     }
     S->cv.notify_all();
-    S->scheduler.join();  // the batch in flight runs to completion first
+    for (auto& t : S->lanes) t.join();  // the batches in flight run to completion first
     if (S->ingest) xcal_ctx_destroy(S->ingest);
     delete S;
     return 0;
@@ -382,7 +422,7 @@ int xcal_server_poll(xcal_server* S, int64_t id, xcal_request_status* st) {
     st->t_start = r.t_start;
     st->t_end = r.t_end;
     st->converged = r.converged;
-    st->unused = 0;
+    st->lane = r.lane;
     st->batch_rebuilds = S->rebuilds;
     return 0;
   });

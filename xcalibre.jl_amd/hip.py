@@ -502,10 +502,11 @@ class Simple:
 class Server:
     """Serving engine (xcal_server_*): continuous batching of independent
     steady SIMPLE cases; at most `slots` cases in flight, stepped together in
-    shared launches, under an HBM budget."""
+    shared launches, under an HBM budget.  `lanes` batches step concurrently,
+    each on its own stream; a case joins the lane closest to it in size."""
 
-    def __init__(self, device=0, slots=4, hbm_budget=64 << 30, max_iterations=1000):
-        cfg = A.ServerConfigC(device, slots, int(hbm_budget), max_iterations)
+    def __init__(self, device=0, slots=4, hbm_budget=64 << 30, max_iterations=1000, lanes=1):
+        cfg = A.ServerConfigC(device, slots, int(hbm_budget), max_iterations, lanes)
         self.h = _vp()
         check(lib().xcal_server_create(C.byref(cfg), C.byref(self.h)))
         self._cells = {}
@@ -531,7 +532,7 @@ class Server:
         check(lib().xcal_server_poll(self.h, rid, C.byref(st)))
         return dict(state=st.state, iterations=st.iterations, cells=st.cells,
                     footprint=st.footprint, t_submit=st.t_submit, t_start=st.t_start,
-                    t_end=st.t_end, converged=bool(st.converged),
+                    t_end=st.t_end, converged=bool(st.converged), lane=st.lane,
                     batch_rebuilds=st.batch_rebuilds)
 
     def wait(self, rid: int = 0, timeout: float = 0.0) -> bool:

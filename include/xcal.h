@@ -208,15 +208,21 @@ int xcal_mesh_sweep_far(const xcal_mesh* mesh, int32_t* fwd, int32_t* bwd);
  * timed).  Results do not depend on the choice.  No reference counterpart. */
 int xcal_mesh_pipe_tuning(const xcal_mesh* mesh, int32_t* fwd_rows, int32_t* bwd_rows,
                           double* us);
-/* Cone form of the DILU sweeps (single meshes): rows per block, blocks, and
- * the cone rows of all blocks for the forward and backward sweeps; all 0
- * where the sweeps launch level by level.  Each sweep is then one launch in
- * which the workgroup of a block of cells sweeps the block and every row it
- * depends on, in the reference's order.  No reference counterpart (reports a
- * device code path; forward_substitution!/backward_substitution!,
+/* Cone form of the DILU sweeps (single meshes): rows per block, forward
+ * blocks, and the cone positions of all blocks for the forward and backward
+ * sweeps; all 0 where the sweeps launch level by level.  Each sweep is then
+ * one launch per band of consecutive levels (one band where the cones fit,
+ * as on randomly numbered meshes), in which the workgroup of a block of
+ * cells sweeps the block and every row of the band it depends on, in the
+ * reference's order, reading earlier bands' rows from global memory.  No
+ * reference counterpart (reports a device code path;
+ * forward_substitution!/backward_substitution!,
  * preconditioners_1_DILU.jl:84-114, are what the sweeps compute). */
 int xcal_mesh_cone_form(const xcal_mesh* mesh, int32_t* block_rows, int32_t* blocks,
                         int64_t* fwd_rows, int64_t* bwd_rows);
+/* Bands (launches) per sweep of the cone form: forward, backward (0: level
+ * launches).  No reference counterpart. */
+int xcal_mesh_cone_bands(const xcal_mesh* mesh, int32_t* fwd_bands, int32_t* bwd_bands);
 /* CSR pattern, 0-based (rowptr n+1, colval nnz) */
 int xcal_mesh_csr_pattern(const xcal_mesh* mesh, int32_t* rowptr, int32_t* colval);
 

@@ -27,6 +27,14 @@ def box48(x):
     return x.mesh.box3d(48, 48, 48, jitter=0.1)  # 110 592 cells, random numbering
 
 
+@pytest.fixture(scope="module")
+def box48n(x):
+    # natural (lexicographic) numbering: a dependency chain crosses the box
+    # (142 levels), so the cones of all levels do not fit and the sweeps run
+    # in bands of levels
+    return x.mesh.box3d(48, 48, 48, jitter=0.1, seed_perm=0)
+
+
 def _mesh(x, ctx, mesh, monkeypatch, **env):
     for k, v in env.items():
         monkeypatch.setenv(k, v)
@@ -52,27 +60,40 @@ def _equation(x, A, dm, mesh, rng, sym):
     return eq
 
 
-def test_cone_form_is_chosen_on_random_numbering(x, ctx, box48, monkeypatch):
+def test_cone_form_is_chosen_on_random_numbering(x, ctx, box48, box48n, monkeypatch):
     dm = _mesh(x, ctx, box48, monkeypatch)
     rows, nblk, fr, br = dm.cone_form()
     n = box48.n_cells
     assert rows == 2048 and nblk == (n + rows - 1) // rows
     assert n < fr < 3 * n and n < br < 3 * n
-    assert _mesh(x, ctx, box48, monkeypatch, XCAL_NO_CONE="1").cone_form() == (0, 0, 0, 0)
-    natural = x.mesh.box3d(48, 48, 48, jitter=0.1, seed_perm=0)  # a chain crosses the box
-    assert _mesh(x, ctx, natural, monkeypatch).cone_form() == (0, 0, 0, 0)
+    assert dm.cone_bands() == (1, 1)  # every level in one launch
+    no = _mesh(x, ctx, box48, monkeypatch, XCAL_NO_CONE="1")
+    assert no.cone_form() == (0, 0, 0, 0) and no.cone_bands() == (0, 0)
+    # natural numbering: bands of at least 4 levels, far fewer launches than levels
+    dn = _mesh(x, ctx, box48n, monkeypatch)
+    fb, bb = dn.cone_bands()
+    lf, lb = dn.levels
+    print(f"natural 48^3: {lf} levels, {fb} forward and {bb} backward bands")
+    assert dn.cone_form()[0] > 0 and 1 < fb <= lf // 4 and 1 < bb <= lb // 4
 
 
-@pytest.mark.parametrize("rows,deep", [("2048", "0"), ("1024", "1"), ("256", "0")])
-def test_cone_preconditioner_bit_exact(x, A, O, ctx, box48, monkeypatch, rows, deep):
+@pytest.mark.parametrize("rows,deep,numbering", [
+    ("2048", "0", "random"), ("1024", "1", "random"), ("256", "0", "random"),
+    ("2048", "0", "natural"), ("512", "1", "natural")])
+def test_cone_preconditioner_bit_exact(x, A, O, ctx, box48, box48n, monkeypatch, rows, deep,
+                                       numbering):
     """z = M^-1 r through both sweeps equals the level launches and the
-    oracle's sequential loops bit for bit, for blocks of 2048, 1024 and 256
-    rows and both prefetch depths."""
+    oracle's sequential loops bit for bit, for blocks of 2048, 1024, 512 and
+    256 rows, both prefetch depths, and both numberings (natural numbering
+    sweeps in bands of levels, each block reading the rows of earlier bands
+    from global memory)."""
     H = x.hip
     rng = np.random.default_rng(11)
+    box48 = box48 if numbering == "random" else box48n
     dm = _mesh(x, ctx, box48, monkeypatch, XCAL_CONE_ROWS=rows, XCAL_CONE_DEEP=deep)
     ref = _mesh(x, ctx, box48, monkeypatch, XCAL_NO_CONE="1")
     assert dm.cone_form()[0] == int(rows)
+    assert (dm.cone_bands() == (1, 1)) == (numbering == "random")
     r_h = rng.standard_normal(box48.n_cells)
     out = []
     for m in (dm, ref):

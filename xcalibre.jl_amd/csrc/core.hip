@@ -148,7 +148,7 @@ void make_sweep_plans(xcal_mesh* m, int small) {
   m->plan_b = plan_levels(m->lev_ptr_b, m->nlev_b, small);
   m->sweep_slots = 0;
   for (const auto& st : m->plan_b) m->sweep_slots = std::max(m->sweep_slots, st.slot + st.grid);
-  if (m->cone) m->sweep_slots = m->cone_nblk;  // the cone form: one partial per block
+  if (m->cone) m->sweep_slots = m->cone_b.nblk();  // the cone form: one partial per block
   int64_t bytes = 0;
   if (m->lev_ptr_f_d.n == 0) {
     upload(m->lev_ptr_f_d, m->lev_ptr_f, bytes);
@@ -174,13 +174,19 @@ static inline double dot3(const double* a, const double* b) {
 }
 static inline double norm3(const double* a) { return std::sqrt(dot3(a, a)); }
 
-// Cone form of one DILU sweep (host side; see sweep_cone.h).  Block b owns
-// the device rows ord[b*S, (b+1)*S) of the Morton order; its cone is those
-// rows and every row they depend on, directly or through other rows.  Cone
-// positions of block b are [base[b], base[b+1]), ordered by level and, inside
-// a level, by entry count (descending), cut into steps of at most kConeBlock
-// rows.  Entry t of the q-th row of a step is entry eoff[t] + q, present for
-// q < ecnt[t] (jagged-diagonal storage: no padding).
+// Cone form of one DILU sweep (host side; see sweep_cone.h).  The sweep's
+// levels are cut into bands of consecutive levels, one launch each (one band
+// holding every level where the cones fit, as on randomly numbered meshes).
+// The rows of a band, in Morton order, are cut into blocks of S rows.  A
+// block's cone is its rows and every row of the band they depend on,
+// directly or through other rows; the rows of earlier bands that the cone
+// reads (its halo) come first among its positions and are loaded from the
+// sweep's output in global memory, where earlier launches stored them.
+// Cone positions of block b are [base[b], base[b+1]): the halo, then the
+// cone rows ordered by level and, inside a level, by entry count
+// (descending), cut into steps of at most kConeBlock rows.  Entry t of the
+// q-th row of a step is entry eoff[t] + q, present for q < ecnt[t]
+// (jagged-diagonal storage: no padding).
 struct ConeHost {
   int wk = 0;                      // entry bound (2, 4, 6 or 8) of the cone kernels
   std::vector<int32_t> blk, base;  // per block: first step, first position (nblk + 1)
@@ -188,17 +194,30 @@ struct ConeHost {
   std::vector<int32_t> crow;       // per position: device row (own) or ~row (another block's)
   std::vector<uint16_t> lidx;      // per entry: cone-local position of the dependency
   std::vector<int32_t> slot, dep;  // per entry: matrix slot, device row of the dependency
+  std::vector<int32_t> own, ownp;  // block b owns rows own[ownp[b], ownp[b+1])
+  std::vector<int32_t> band;       // first block of each band (nbands + 1)
 };
 
-// Builds the cone form of the forward (lower) or backward (upper) sweep with
-// blocks of S rows; false when a block's cone or step count exceeds the
-// kernels' limits or the cones hold more than kConeMaxRatio n rows.
-static bool build_cone(int64_t n, int S, bool lower, const std::vector<int32_t>& ord,
-                       const std::vector<int32_t>& lev, const std::vector<int32_t>& rbeg,
-                       const std::vector<int32_t>& rend, const std::vector<int32_t>& diag,
-                       const std::vector<int32_t>& colval, ConeHost& C) {
-  C = ConeHost();
-  const int64_t nblk = (n + S - 1) / S;
+namespace {
+// Work arrays of a cone build; `mark` holds build stamps, so a failed band
+// leaves nothing a later one could mistake for its own.
+struct ConeScratch {
+  std::vector<int32_t> owner, mark, loc, stack, cone, halo;
+  int32_t stamp = 0;
+  explicit ConeScratch(int64_t n) : owner(n, -1), mark(n, -1), loc(n, 0) {}
+};
+}  // namespace
+
+// Appends to C the blocks of the band of levels [lo, hi); false (C
+// unchanged) when a block's cone or step count exceeds the kernels' limits,
+// or (ratio > 0) the band's cone positions so far exceed ratio times its
+// rows so far by more than one block's limit (checked after every block, so
+// a band that will not fit stops early).
+static bool build_band(int S, double ratio, bool lower, const std::vector<int32_t>& ord,
+                       const std::vector<int32_t>& lev, int lo, int hi,
+                       const std::vector<int32_t>& rbeg, const std::vector<int32_t>& rend,
+                       const std::vector<int32_t>& diag, const std::vector<int32_t>& colval,
+                       ConeHost& C, ConeScratch& W) {
   auto span = [&](int32_t c, int32_t& k0, int32_t& k1) {
     k0 = lower ? rbeg[c] : diag[c] + kSlice;
     k1 = lower ? diag[c] : rend[c];
@@ -208,19 +227,31 @@ static bool build_cone(int64_t n, int S, bool lower, const std::vector<int32_t>&
     span(c, k0, k1);
     return (k1 - k0) / kSlice;
   };
-  std::vector<int32_t> owner(n), mark(n, -1), loc(n, 0), stack, cone;
-  for (int64_t t = 0; t < n; ++t) owner[ord[t]] = (int32_t)(t / S);
+  std::vector<int32_t> rows;  // the band's rows in Morton order
+  for (int32_t c : ord)
+    if (lev[c] >= lo && lev[c] < hi) rows.push_back(c);
+  const int64_t nb = ((int64_t)rows.size() + S - 1) / S;
+  const int32_t b0 = (int32_t)C.blk.size() - 1;  // global index of the band's first block
+  for (size_t t = 0; t < rows.size(); ++t) W.owner[rows[t]] = b0 + (int32_t)(t / S);
+  ConeHost B;  // the band's blocks, positions and entries from 0
+  B.blk.push_back(0);
+  B.base.push_back(0);
+  B.ownp.push_back(0);
   int maxw = 0;
-  C.blk.push_back(0);
-  C.base.push_back(0);
-  for (int64_t b = 0; b < nblk; ++b) {
+  std::vector<int32_t>&cone = W.cone, &halo = W.halo, &stack = W.stack, &mark = W.mark,
+  &loc = W.loc;
+  for (int64_t b = 0; b < nb; ++b) {
+    const int32_t stamp = ++W.stamp;
     cone.clear();
-    const int64_t a0 = b * S, a1 = std::min<int64_t>(n, a0 + S);
-    for (int64_t t = a0; t < a1; ++t) {
-      mark[ord[t]] = (int32_t)b;
-      cone.push_back(ord[t]);
-      stack.push_back(ord[t]);
+    halo.clear();
+    const size_t a0 = (size_t)b * S, a1 = std::min(rows.size(), a0 + S);
+    for (size_t t = a0; t < a1; ++t) {
+      mark[rows[t]] = stamp;
+      cone.push_back(rows[t]);
+      stack.push_back(rows[t]);
+      B.own.push_back(rows[t]);
     }
+    B.ownp.push_back((int32_t)B.own.size());
     while (!stack.empty()) {
       const int32_t c = stack.back();
       stack.pop_back();
@@ -228,55 +259,142 @@ static bool build_cone(int64_t n, int S, bool lower, const std::vector<int32_t>&
       span(c, k0, k1);
       for (int32_t k = k0; k < k1; k += kSlice) {
         const int32_t d = colval[k];
-        if (mark[d] == (int32_t)b) continue;
-        mark[d] = (int32_t)b;
-        cone.push_back(d);
-        stack.push_back(d);
-        if ((int64_t)cone.size() > kConeMax) return false;
+        if (mark[d] == stamp) continue;
+        mark[d] = stamp;
+        if (lev[d] < lo) {  // an earlier band's row: read, not swept
+          halo.push_back(d);
+        } else {
+          cone.push_back(d);
+          stack.push_back(d);
+        }
+        if ((int64_t)(cone.size() + halo.size()) > kConeMax) {
+          stack.clear();
+          return false;
+        }
       }
     }
+    std::sort(halo.begin(), halo.end());
     std::sort(cone.begin(), cone.end(), [&](int32_t x, int32_t y) {
       if (lev[x] != lev[y]) return lev[x] < lev[y];
       const int ex = nent(x), ey = nent(y);
       if (ex != ey) return ex > ey;
       return x < y;
     });
-    const int32_t base = C.base.back();
-    for (size_t j = 0; j < cone.size(); ++j) loc[cone[j]] = (int32_t)j;
+    const int32_t base = B.base.back();
+    const int32_t nh = (int32_t)halo.size();
+    for (size_t j = 0; j < halo.size(); ++j) loc[halo[j]] = (int32_t)j;
+    for (size_t j = 0; j < cone.size(); ++j) loc[cone[j]] = nh + (int32_t)j;
     int nsteps = 0;
     for (size_t j0 = 0; j0 < cone.size();) {
       size_t j1 = j0;
       while (j1 < cone.size() && lev[cone[j1]] == lev[cone[j0]] && j1 - j0 < (size_t)kConeBlock) ++j1;
       int32_t tab[kConeTab] = {};  // r0, cnt, eoff[8], ecnt[8], 2 unused
-      tab[0] = base + (int32_t)j0;
+      tab[0] = base + nh + (int32_t)j0;
       tab[1] = (int32_t)(j1 - j0);
       for (int t = 0; t < 8; ++t) {
-        tab[2 + t] = (int32_t)C.lidx.size();
+        tab[2 + t] = (int32_t)B.lidx.size();
         for (size_t j = j0; j < j1 && nent(cone[j]) > t; ++j) {
           int32_t k0, k1;
           span(cone[j], k0, k1);
           const int32_t k = k0 + t * kSlice, d = colval[k];
-          XCAL_CHECK(loc[d] < (int32_t)j0, "internal: cone dependency after its row");
-          C.lidx.push_back((uint16_t)loc[d]);
-          C.slot.push_back(k);
-          C.dep.push_back(d);
+          XCAL_CHECK(loc[d] < nh + (int32_t)j0, "internal: cone dependency after its row");
+          B.lidx.push_back((uint16_t)loc[d]);
+          B.slot.push_back(k);
+          B.dep.push_back(d);
           tab[10 + t]++;
         }
       }
       for (size_t j = j0; j < j1; ++j) maxw = std::max(maxw, nent(cone[j]));
       if (maxw > 8) return false;
-      C.steps.insert(C.steps.end(), tab, tab + kConeTab);
+      B.steps.insert(B.steps.end(), tab, tab + kConeTab);
       ++nsteps;
       j0 = j1;
     }
     if (nsteps > kConeSteps) return false;
-    for (int32_t c : cone) C.crow.push_back(owner[c] == (int32_t)b ? c : ~c);
-    C.base.push_back(base + (int32_t)cone.size());
-    C.blk.push_back(C.blk.back() + nsteps);
+    for (int32_t c : halo) B.crow.push_back(~c);
+    const int32_t gb = b0 + (int32_t)b;
+    for (int32_t c : cone) B.crow.push_back(W.owner[c] == gb ? c : ~c);
+    B.base.push_back(base + (int32_t)(halo.size() + cone.size()));
+    B.blk.push_back(B.blk.back() + nsteps);
+    if (ratio > 0 && B.base.back() > ratio * (double)B.own.size() + kConeMax) return false;
+  }
+  // append the band: positions, entry offsets, steps and own rows move up
+  const int32_t p0 = C.base.back(), e0 = (int32_t)C.lidx.size(), s0 = C.blk.back(),
+                o0 = C.ownp.back();
+  for (size_t t = 0; t < B.steps.size(); t += kConeTab) {
+    B.steps[t] += p0;
+    for (int k = 0; k < 8; ++k) B.steps[t + 2 + k] += e0;
+  }
+  for (size_t b = 1; b < B.blk.size(); ++b) {
+    C.blk.push_back(B.blk[b] + s0);
+    C.base.push_back(B.base[b] + p0);
+    C.ownp.push_back(B.ownp[b] + o0);
+  }
+  C.steps.insert(C.steps.end(), B.steps.begin(), B.steps.end());
+  C.crow.insert(C.crow.end(), B.crow.begin(), B.crow.end());
+  C.lidx.insert(C.lidx.end(), B.lidx.begin(), B.lidx.end());
+  C.slot.insert(C.slot.end(), B.slot.begin(), B.slot.end());
+  C.dep.insert(C.dep.end(), B.dep.begin(), B.dep.end());
+  C.own.insert(C.own.end(), B.own.begin(), B.own.end());
+  C.wk = std::max(C.wk, maxw);
+  return true;
+}
+
+// Builds the cone form of the forward (lower) or backward (upper) sweep with
+// blocks of S rows over nlev levels.  First every level as one band; where
+// those cones do not fit, bands of consecutive levels, each launch then
+// covering only the band's rows: a band's block size is the power of two
+// (kConeMinBlock..S) that gives about kConeBandBlocks blocks, so a launch
+// still fills the chip, doubled while its cone positions exceed
+// kConeBandRatio times its rows (larger blocks recompute fewer rows).  A
+// band starts at twice the previous band's width (the levels left, for the
+// first) and halves when no block size fits.  False
+// when a band of kConeMinBand levels does not fit, the sweep needs more than
+// kConeMaxBands bands, or the cones hold more than kConeMaxRatio n
+// positions in all.
+static bool build_cone(int64_t n, int S, bool lower, const std::vector<int32_t>& ord,
+                       const std::vector<int32_t>& lev, int nlev, const std::vector<int32_t>& rbeg,
+                       const std::vector<int32_t>& rend, const std::vector<int32_t>& diag,
+                       const std::vector<int32_t>& colval, ConeHost& C) {
+  C = ConeHost();
+  C.blk.push_back(0);
+  C.base.push_back(0);
+  C.ownp.push_back(0);
+  C.band.push_back(0);
+  ConeScratch W(n);
+  std::vector<int64_t> lcnt(nlev + 1, 0);  // rows below each level
+  for (int64_t c = 0; c < n; ++c) lcnt[lev[c] + 1]++;
+  for (int l = 0; l < nlev; ++l) lcnt[l + 1] += lcnt[l];
+  double bratio = kConeBandRatio;  // XCAL_CONE_BAND_RATIO at creation overrides it
+  if (const char* env = std::getenv("XCAL_CONE_BAND_RATIO")) bratio = std::max(1.0, std::atof(env));
+  const double maxratio = std::max(kConeMaxRatio, bratio + 0.5);
+  int lo = 0, H = nlev;
+  if (build_band(S, 0, lower, ord, lev, 0, nlev, rbeg, rend, diag, colval, C, W)) {
+    C.band.push_back((int32_t)C.blk.size() - 1);
+    lo = nlev;
     if ((double)C.crow.size() > kConeMaxRatio * (double)n) return false;
   }
-  if (maxw == 0 || C.lidx.empty()) return false;  // masked loads read entry 0
-  C.wk = maxw <= 2 ? 2 : maxw <= 4 ? 4 : maxw <= 6 ? 6 : 8;
+  while (lo < nlev) {
+    H = std::min(H, nlev - lo);
+    for (;;) {
+      const int64_t rows = lcnt[lo + H] - lcnt[lo];
+      int Sb = kConeMinBlock;
+      while (Sb < S && rows / (2 * Sb) >= kConeBandBlocks) Sb *= 2;
+      bool ok = false;
+      for (; Sb <= S && !ok; Sb *= 2)
+        ok = build_band(Sb, bratio, lower, ord, lev, lo, lo + H, rbeg, rend, diag, colval, C, W);
+      if (ok) break;
+      if (H <= kConeMinBand) return false;
+      H = std::max(kConeMinBand, H / 2);
+    }
+    C.band.push_back((int32_t)C.blk.size() - 1);
+    if ((int)C.band.size() - 1 > kConeMaxBands) return false;
+    if ((double)C.crow.size() > maxratio * (double)n) return false;
+    lo += H;
+    H *= 2;
+  }
+  if (C.wk == 0 || C.lidx.empty()) return false;  // masked loads read entry 0
+  C.wk = C.wk <= 2 ? 2 : C.wk <= 4 ? 4 : C.wk <= 6 ? 6 : 8;
   return true;
 }
 
@@ -285,7 +403,6 @@ struct HostMesh {
   bool cone = false;   // the cone form of both sweeps was built
   int cone_rows = 0;   // its block size
   ConeHost cone_f, cone_b;
-  std::vector<int32_t> cone_own;  // device rows in Morton order: block b owns a range
   int dim = 3, n_patches = 0, nlev_f = 0, nlev_b = 0;
   int64_t n = 0, nf = 0, nbf = 0, nnz = 0, ncf = 0, nzlen = 0, nbcell = 0;
   std::vector<int64_t> patch_first, patch_last;
@@ -688,13 +805,14 @@ static void build_host_mesh(const xcal_mesh_host* h, HostMesh& M, bool cones, xc
     std::stable_sort(ord.begin(), ord.end(),
                      [&](int32_t a, int32_t b) { return dmort[a] < dmort[b]; });
     for (int S = rows; S >= 64 && !M.cone; S /= 2) {
-      M.cone = build_cone(n, S, true, ord, flev_d, rbeg, rend, diag_n, colval_n, M.cone_f) &&
-               build_cone(n, S, false, ord, blev_d, rbeg, rend, diag_n, colval_n, M.cone_b);
+      M.cone = build_cone(n, S, true, ord, flev_d, M.nlev_f, rbeg, rend, diag_n, colval_n,
+                          M.cone_f) &&
+               build_cone(n, S, false, ord, blev_d, M.nlev_b, rbeg, rend, diag_n, colval_n,
+                          M.cone_b);
       if (M.cone) M.cone_rows = S;
       if (S <= 256) break;
     }
-    if (M.cone) M.cone_own.swap(ord);
-    else M.cone_f = M.cone_b = ConeHost();
+    if (!M.cone) M.cone_f = M.cone_b = ConeHost();
   }
   std::vector<int32_t> cf_ptr_n(n + 1, 0), cf_face_n(cf_face.size()), cf_nbr_n(cf_face.size()),
       cf_slot_n(cf_face.size());
@@ -1062,12 +1180,14 @@ static xcal_mesh* upload_mesh(xcal_ctx* ctx, HostMesh& M) {
       upload(D.slot, H.slot, bytes);
       upload(D.dep, H.dep, bytes);
       upload(D.lidx, H.lidx, bytes);
+      D.band = H.band;
     };
     up(m->cone_f, M.cone_f);
     up(m->cone_b, M.cone_b);
-    upload(m->cone_own, M.cone_own, bytes);
+    upload(m->cone_own, M.cone_f.own, bytes);
+    upload(m->cone_ownp, M.cone_f.ownp, bytes);
     m->cone = true;
-    m->cone_nblk = (int)M.cone_f.blk.size() - 1;
+    m->cone_nblk = m->cone_f.nblk();
     m->cone_rows = M.cone_rows;
     const char* deep = std::getenv("XCAL_CONE_DEEP");
     m->cone_deep = deep && *deep == '1';
@@ -1148,7 +1268,6 @@ xcal_mesh* upload_prepared(xcal_ctx* ctx, PreparedMesh& P) {
   if (P.batch) {  // batches run the batched solvers, which have no cone form
     P.U.cone = false;
     P.U.cone_f = P.U.cone_b = ConeHost();
-    P.U.cone_own.clear();
   }
   xcal_mesh* m = upload_mesh(ctx, P.U);
   std::unique_ptr<xcal_mesh> guard(m);
@@ -1384,6 +1503,15 @@ int xcal_mesh_cone_form(const xcal_mesh* m, int32_t* block_rows, int32_t* blocks
     if (blocks) *blocks = m->cone ? m->cone_nblk : 0;
     if (fwd_rows) *fwd_rows = m->cone ? m->cone_f.rows : 0;
     if (bwd_rows) *bwd_rows = m->cone ? m->cone_b.rows : 0;
+    return 0;
+  });
+}
+
+int xcal_mesh_cone_bands(const xcal_mesh* m, int32_t* fwd, int32_t* bwd) {
+  return guarded([&] {
+    XCAL_CHECK(m && fwd && bwd, "NULL argument");
+    *fwd = m->cone ? (int32_t)m->cone_f.band.size() - 1 : 0;
+    *bwd = m->cone ? (int32_t)m->cone_b.band.size() - 1 : 0;
     return 0;
   });
 }

@@ -500,9 +500,9 @@ __global__ void __launch_bounds__(kConeBlock)
   bwd_cone<DOT, T, WK, P, V>(C, a, d, W, Yt, L);
 }
 
-inline ConeK cone_k(const xcal_mesh* m, const xcal_mesh::Cone& c) {
-  return ConeK{c.blk.p,        c.base.p,          c.steps.p,   c.crow.p,
-               c.lidx.p,       m->cone_own.p,     m->cone_rows, (int32_t)m->n};
+inline ConeK cone_k(const xcal_mesh* m, const xcal_mesh::Cone& c, int32_t blk0) {
+  return ConeK{c.blk.p, c.base.p, c.steps.p, c.crow.p, c.lidx.p, m->cone_own.p, m->cone_ownp.p,
+               blk0};
 }
 
 // ------------------------------------------------------------ host side
@@ -736,15 +736,19 @@ static void sweep_fwd(const Solver<T>& S, const FwdK<T>& W) {
   if (m->cone) {  // y to W.Yb in device order
     XCAL_CHECK(W.Yb != (UPD ? W.r : W.u) && (!UPD || (W.rout != W.r && W.rout != W.Yb)),
                "internal: cone sweep operands alias");
+    const std::vector<int32_t>& band = m->cone_f.band;
     with_cone_form(m->cone_f.wk, !m->cone_deep, [&](auto WK, auto P) {
       constexpr int wk = decltype(WK)::value, p = decltype(P)::value;
-      if (std::is_same<T, double>::value && S.pc32)  // y in float, in the Yb vector's storage
-        hipLaunchKernelGGL((k_fwd_cone<UPD, float, wk, p, T>), dim3(m->cone_nblk),
-                           dim3(kConeBlock), 0, S.s, cone_k(m, m->cone_f), S.cfa32, S.cfa32, W,
-                           reinterpret_cast<float*>(W.Yb));
-      else
-        hipLaunchKernelGGL((k_fwd_cone<UPD, T, wk, p, T>), dim3(m->cone_nblk), dim3(kConeBlock),
-                           0, S.s, cone_k(m, m->cone_f), S.cfa, S.cfr, W, W.Yb);
+      for (size_t k = 0; k + 1 < band.size(); ++k) {  // one launch per band of levels
+        const ConeK C = cone_k(m, m->cone_f, band[k]);
+        const dim3 grid(band[k + 1] - band[k]);
+        if (std::is_same<T, double>::value && S.pc32)  // y in float, in the Yb vector's storage
+          hipLaunchKernelGGL((k_fwd_cone<UPD, float, wk, p, T>), grid, dim3(kConeBlock), 0, S.s,
+                             C, S.cfa32, S.cfa32, W, reinterpret_cast<float*>(W.Yb));
+        else
+          hipLaunchKernelGGL((k_fwd_cone<UPD, T, wk, p, T>), grid, dim3(kConeBlock), 0, S.s, C,
+                             S.cfa, S.cfr, W, W.Yb);
+      }
     });
     return;
   }
@@ -783,17 +787,21 @@ static void sweep_fwd(const Solver<T>& S, const FwdK<T>& W) {
 template <int DOT, class T>
 static void sweep_bwd(const Solver<T>& S, const BwdK<T>& W) {
   const xcal_mesh* m = S.m;
-  if (m->cone) {  // y from W.Yb in device order; partials in slots [0, cone_nblk)
+  if (m->cone) {  // y from W.Yb in device order; partials in slots [0, backward blocks)
     XCAL_CHECK(W.Z != W.Yb && (DOT == 0 || W.dv != W.Z), "internal: cone sweep operands alias");
+    const std::vector<int32_t>& band = m->cone_b.band;
     with_cone_form(m->cone_b.wk, !m->cone_deep, [&](auto WK, auto P) {
       constexpr int wk = decltype(WK)::value, p = decltype(P)::value;
-      if (std::is_same<T, double>::value && S.pc32)
-        hipLaunchKernelGGL((k_bwd_cone<DOT, float, wk, p, T>), dim3(m->cone_nblk),
-                           dim3(kConeBlock), 0, S.s, cone_k(m, m->cone_b), S.cba32, S.cbd32, W,
-                           reinterpret_cast<const float*>(W.Yb));
-      else
-        hipLaunchKernelGGL((k_bwd_cone<DOT, T, wk, p, T>), dim3(m->cone_nblk), dim3(kConeBlock),
-                           0, S.s, cone_k(m, m->cone_b), S.cba, S.cbd, W, W.Yb);
+      for (size_t k = 0; k + 1 < band.size(); ++k) {  // one launch per band of levels
+        const ConeK C = cone_k(m, m->cone_b, band[k]);
+        const dim3 grid(band[k + 1] - band[k]);
+        if (std::is_same<T, double>::value && S.pc32)
+          hipLaunchKernelGGL((k_bwd_cone<DOT, float, wk, p, T>), grid, dim3(kConeBlock), 0, S.s,
+                             C, S.cba32, S.cbd32, W, reinterpret_cast<const float*>(W.Yb));
+        else
+          hipLaunchKernelGGL((k_bwd_cone<DOT, T, wk, p, T>), grid, dim3(kConeBlock), 0, S.s, C,
+                             S.cba, S.cbd, W, W.Yb);
+      }
     });
     return;
   }

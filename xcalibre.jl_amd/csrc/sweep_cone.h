@@ -43,9 +43,9 @@ struct ConeK {
   const int32_t* steps;  // kConeTab ints per step: r0, cnt, eoff[8], ecnt[8], 2 unused
   const int32_t* crow;   // device row of each position (~row: another block's)
   const uint16_t* lidx;  // cone-local position of each entry's dependency
-  const int32_t* own;    // device rows in Morton order: block b owns [b*rows, (b+1)*rows)
-  int32_t rows;          // rows per block
-  int32_t n;             // rows of the mesh
+  const int32_t* own;    // forward blocks' own rows: block b owns own[ownp[b], ownp[b+1])
+  const int32_t* ownp;
+  int32_t blk0;          // this launch's band: global index of its first block
 };
 
 struct alignas(16) ConeStep {
@@ -60,12 +60,22 @@ struct ConeLDS {
 };
 
 // Stages block b's step table; returns its step count and first position.
-__device__ inline int cone_stage(const ConeK& C, int b, ConeStep* tab, int32_t& base) {
+// The positions before the first step's are the block's halo: rows of
+// earlier bands, whose values `src` holds (stored by earlier launches); they
+// are loaded into the cone's first LDS slots.
+template <class T, class S>
+__device__ inline int cone_stage(const ConeK& C, int b, ConeStep* tab, int32_t& base,
+                                 const S* __restrict__ src, T* v) {
   const int s0 = C.blk[b], s1 = C.blk[b + 1];
-  const int4* src = reinterpret_cast<const int4*>(C.steps + (int64_t)s0 * kConeTab);
+  const int4* srct = reinterpret_cast<const int4*>(C.steps + (int64_t)s0 * kConeTab);
   int4* dst = reinterpret_cast<int4*>(tab);
-  for (int j = threadIdx.x; j < (s1 - s0) * (kConeTab / 4); j += blockDim.x) dst[j] = src[j];
+  for (int j = threadIdx.x; j < (s1 - s0) * (kConeTab / 4); j += blockDim.x) dst[j] = srct[j];
   base = C.base[b];
+  const int32_t nh = C.steps[(int64_t)s0 * kConeTab] - base;  // halo positions
+  for (int32_t p = threadIdx.x; p < nh; p += blockDim.x) {
+    const int32_t c = C.crow[base + p];
+    v[p] = (T)src[c >= 0 ? c : ~c];
+  }
   __syncthreads();
   return s1 - s0;
 }
@@ -123,9 +133,9 @@ constexpr bool cone_folded() {
 template <bool UPD, class T, int WK, int P, class V = T>
 __device__ inline void fwd_cone(const ConeK& C, const T* __restrict__ a, const T* __restrict__ rd,
                                 const FwdK<V>& W, T* __restrict__ Yt, ConeLDS<T>& L) {
-  const int b = (int)xcd_block();
+  const int b = C.blk0 + (int)xcd_block();
   int32_t base;
-  const int ns = cone_stage(C, b, L.tab, base);
+  const int ns = cone_stage(C, b, L.tab, base, (const T*)Yt, L.v);
   const V al = UPD && W.ks ? (V)W.ks->alpha : V(0);
   auto fetch = [&](CFRec<T, WK, V>& R, int s) {
     int64_t e[WK];
@@ -165,7 +175,7 @@ __device__ inline void fwd_cone(const ConeK& C, const T* __restrict__ a, const T
   pipe_levels<P>(ns, buf, fetch, gather, process);
   if (UPD) {  // x += alpha p over the own rows, U rows per thread at a time
     constexpr int U = 8;
-    const int32_t o0 = b * C.rows, o1 = min(C.n, o0 + C.rows);
+    const int32_t o0 = C.ownp[b], o1 = C.ownp[b + 1];
     for (int32_t j0 = o0 + (int32_t)threadIdx.x; j0 < o1; j0 += U * (int32_t)blockDim.x) {
       int32_t ii[U];
       V xv[U], pv[U];
@@ -207,9 +217,9 @@ struct CBRec {
 template <int DOT, class T, int WK, int P, class V = T>
 __device__ inline void bwd_cone(const ConeK& C, const T* __restrict__ a, const T* __restrict__ d,
                                 const BwdK<V>& W, const T* __restrict__ Yt, ConeLDS<T>& L) {
-  const int b = (int)xcd_block();
+  const int b = C.blk0 + (int)xcd_block();
   int32_t base;
-  const int ns = cone_stage(C, b, L.tab, base);
+  const int ns = cone_stage(C, b, L.tab, base, (const V*)W.Z, L.v);
   auto fetch = [&](CBRec<T, WK, V>& R, int s) {
     const ConeStep& S = L.tab[s < ns ? s : ns - 1];
     int64_t e[WK];

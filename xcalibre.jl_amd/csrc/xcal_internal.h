@@ -89,6 +89,11 @@ constexpr int kConeTab = 20;          // ints per step: r0, cnt, eoff[8], ecnt[8
 constexpr int kConeRows = 2048;       // block size tried first
 constexpr int64_t kConeMinRows = 1 << 16;  // smaller meshes keep the level launches
 constexpr double kConeMaxRatio = 3.0;      // most cone rows in all, per mesh row
+constexpr int kConeMinBand = 4;            // narrowest band of levels
+constexpr int kConeMaxBands = 128;         // most bands (launches) per sweep
+constexpr int kConeMinBlock = 64;          // smallest block of a band
+constexpr int kConeBandBlocks = 512;       // blocks a band's block size aims at
+constexpr double kConeBandRatio = 4.0;     // most cone positions per row of a band
 
 // Cut c_x (x = 0..8) of a level of cnt rows into eighths at 64-row
 // boundaries; eighth x is worked by the workgroups on XCD x.
@@ -257,17 +262,20 @@ struct xcal_mesh {
   int64_t nblk = 0;
   xcal::DBuf<int32_t> blk_row0, blk_row1, blk_case, case_blk_d, case_iota_d, face_case;
   // Cone form of both DILU sweeps (single meshes; sweep_cone.h): when `cone`
-  // is set, each sweep is one launch of cone_nblk workgroups, and the
-  // backward sweep's partial sums fill slots [0, cone_nblk).
+  // is set, each sweep is one launch per band of levels, of the band's
+  // blocks, and the backward sweep's partial sums fill one slot per block.
   struct Cone {
     int wk = 0;  // entry bound
     int64_t rows = 0, entries = 0;
     xcal::DBuf<int32_t> blk, base, steps, crow, slot, dep;
     xcal::DBuf<uint16_t> lidx;
+    std::vector<int32_t> band;  // host: first block of each band (nbands + 1)
+    int nblk() const { return band.empty() ? 0 : band.back(); }
   };
-  xcal::DBuf<int32_t> cone_own;  // block b's own rows: [b * cone_rows, (b + 1) * cone_rows)
+  // the forward sweep's block b owns rows cone_own[cone_ownp[b], cone_ownp[b+1])
+  xcal::DBuf<int32_t> cone_own, cone_ownp;
   bool cone = false;
-  int cone_nblk = 0, cone_rows = 0;  // blocks, rows per block
+  int cone_nblk = 0, cone_rows = 0;  // forward blocks, rows per block
   bool cone_deep = false;            // deep prefetch (XCAL_CONE_DEEP=1 at creation)
   Cone cone_f, cone_b;
   int small_level = 0;

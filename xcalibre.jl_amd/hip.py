@@ -54,6 +54,7 @@ def lib():
             "xcal_mesh_sweep_far": [_vp, _pi32, _pi32],
             "xcal_mesh_pipe_tuning": [_vp, _pi32, _pi32, _pd],
             "xcal_mesh_cone_form": [_vp, _pi32, _pi32, C.POINTER(C.c_int64), C.POINTER(C.c_int64)],
+            "xcal_mesh_cone_bands": [_vp, _pi32, _pi32],
             "xcal_solve_system_batch": [_vp, C.POINTER(A.SolverSetupC), _vp, C.c_int32,
                                         C.POINTER(A.SolveStats)],
             "xcal_simple_run_batch": [_vp, C.c_int32, _pd, _pi32, _pi32],
@@ -260,9 +261,17 @@ class DeviceMesh:
              for j, r in enumerate((1, 2, 4))}
         return f.value, b.value, t
 
+    def cone_bands(self):
+        """(forward, backward) bands of levels, one launch each, of the cone
+        sweeps; (0, 0) where the sweeps launch per level."""
+        f, b = C.c_int32(), C.c_int32()
+        check(lib().xcal_mesh_cone_bands(self.h, C.byref(f), C.byref(b)))
+        return f.value, b.value
+
     def cone_form(self):
-        """(rows per block, blocks, forward cone rows, backward cone rows) of
-        the one-launch cone sweeps; all 0 where the sweeps launch per level."""
+        """(rows per block, forward blocks, forward cone positions, backward
+        cone positions) of the cone sweeps; all 0 where the sweeps launch per
+        level."""
         r, nb = C.c_int32(), C.c_int32()
         f, b = C.c_int64(), C.c_int64()
         check(lib().xcal_mesh_cone_form(self.h, C.byref(r), C.byref(nb), C.byref(f), C.byref(b)))

@@ -1,5 +1,6 @@
-// Cone form of the DILU sweeps on single meshes: one launch per sweep, no
-// synchronisation between workgroups.
+// Cone form of the DILU sweeps on single meshes: one launch per band of
+// levels (one band where the cones fit), no synchronisation between
+// workgroups.
 //
 // The cells are cut into blocks of consecutive Morton keys (a compact region
 // of space each).  The workgroup of a block sweeps the block's rows together
@@ -16,9 +17,11 @@
 //
 // This pays where cones are small.  On a randomly numbered mesh a chain of
 // ascending neighbours is short (18 levels on the 1M-cell box), and a block
-// of 1024 cells has a cone of about 2000 rows; on a lexicographically
-// numbered mesh a chain crosses the domain and the mesh build keeps the level
-// launches.
+// of 1024 cells has a cone of about 2000 rows: one band holds every level.
+// On a lexicographically numbered mesh a chain crosses the domain, so the
+// levels are cut into bands, one launch each: a block's cone stops at its
+// band's first level, and the rows of earlier bands it reads (its halo) are
+// loaded from global memory, where the earlier launches stored them.
 //
 // Memory latency is kept off the step-to-step chain as in the pipelined runs
 // (sweep_pipe.h): each row's position-addressed loads (its row index, its

@@ -261,7 +261,7 @@ void precond_update(xcal_eqn* e, int precon) {
       launch_check();
     }
     hipLaunchKernelGGL(k_recip, dim3(grid_for(n)), dim3(kBlock), 0, s, n, e->pD.p, e->prD.p);
-    if (m->cone) {  // the cone sweeps' factors: a_ij, 1/D_j per entry; D per position
+    if (m->cone) {  // the cone sweeps' factors: a_ij, 1/D_j per entry; 1/D, D per position
       const xcal_mesh::Cone &F = m->cone_f, &B = m->cone_b;
       if (e->cfa.n != F.entries) {
         e->cfa.alloc(F.entries);
@@ -277,6 +277,9 @@ void precond_update(xcal_eqn* e, int precon) {
                          B.slot.p, e->nzval.p, e->cba.p);
       hipLaunchKernelGGL(k_gather_cone_rows, dim3(grid_for(B.rows)), dim3(kBlock), 0, s, B.rows,
                          B.crow.p, e->pD.p, e->cbd.p);
+      if (e->cfd.n != F.rows) e->cfd.alloc(F.rows);
+      hipLaunchKernelGGL(k_gather_cone_rows, dim3(grid_for(F.rows)), dim3(kBlock), 0, s, F.rows,
+                         F.crow.p, e->prD.p, e->cfd.p);
       launch_check();
       return;
     }

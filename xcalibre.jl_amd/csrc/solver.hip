@@ -486,7 +486,7 @@ __global__ void __launch_bounds__(kPipeBlock)
 template <bool UPD, class T, int WK, int P, class V>
 __global__ void __launch_bounds__(kConeBlock)
     k_fwd_cone(ConeK C, const T* __restrict__ a, const T* __restrict__ rd, FwdK<V> W, T* Yt) {
-  __shared__ ConeLDS<T> L;
+  __shared__ ConeLDS<T, !cone_folded<T>()> L;
   if (W.ks && stopped(W.ks)) return;  // uniform over the workgroup
   fwd_cone<UPD, T, WK, P, V>(C, a, rd, W, Yt, L);
 }
@@ -518,7 +518,9 @@ struct Solver {
   const T *D, *rD, *Sj;  // DILU D and 1/D; Jacobi 1/|a_ii|
   const T *Lnz, *Unz, *Db;  // packed DILU factors, D in backward-level order
   const T* Lrd;             // 1/D per packed lower entry (pipelined runs)
-  const T *cfa, *cfr, *cba, *cbd;  // cone form: packed a_ij, 1/D_j, a_ij, D
+  // cone form: a_ij and 1/D_j per forward entry, 1/D per forward position,
+  // a_ij per backward entry, D per backward position
+  const T *cfa, *cfr, *cfd, *cba, *cbd;
   // FP32 preconditioner of an FP64 solve (XCAL_PREC_FP32_PRECON, cone form):
   // folded FP32 factors a_ij / D_j, a_ij and 1 / D of the cone sweeps
   bool pc32 = false;
@@ -578,6 +580,7 @@ static Solver<double> make_solver(xcal_eqn* e) {
   S.Lrd = e->Lrd.p;
   S.cfa = e->cfa.p;
   S.cfr = e->cfr.p;
+  S.cfd = e->cfd.p;
   S.cba = e->cba.p;
   S.cbd = e->cbd.p;
   S.work = e->work.p;
@@ -661,6 +664,7 @@ static Solver<float> make_solver32(xcal_eqn* e) {
   S.Lrd = e->Lrd32.p;
   S.cfa = e->cfa32.p;
   S.cfr = e->cfa32.p;  // unused by the folded FP32 forward cone sweep
+  S.cfd = e->cfa32.p;  // likewise
   S.cba = e->cba32.p;
   S.cbd = e->cbd32.p;
   S.work = e->work32.p;
@@ -747,7 +751,7 @@ static void sweep_fwd(const Solver<T>& S, const FwdK<T>& W) {
                              C, S.cfa32, S.cfa32, W, reinterpret_cast<float*>(W.Yb));
         else
           hipLaunchKernelGGL((k_fwd_cone<UPD, T, wk, p, T>), grid, dim3(kConeBlock), 0, S.s, C,
-                             S.cfa, S.cfr, W, W.Yb);
+                             S.cfa, S.cfd, W, W.Yb);
       }
     });
     return;
@@ -874,6 +878,7 @@ void precond_apply_staged(xcal_eqn* e, const double* r, double* z, double* Y, do
   S.Lrd = e->Lrd.p;
   S.cfa = e->cfa.p;
   S.cfr = e->cfr.p;
+  S.cfd = e->cfd.p;
   S.cba = e->cba.p;
   S.cbd = e->cbd.p;
   dilu_apply(S, r, z, Y, Yb);

@@ -56,19 +56,24 @@ struct alignas(16) ConeStep {
 };
 static_assert(sizeof(ConeStep) == kConeTab * sizeof(int32_t), "step table row");
 
-template <class T>
+// Cone values in LDS; RD adds the 1/D of each position's row (the FP64
+// forward sweep, whose entries read it there).
+template <class T, bool RD = false>
 struct ConeLDS {
   T v[kConeMax];
+  T rd[RD ? kConeMax : 1];
   ConeStep tab[kConeSteps];
 };
 
 // Stages block b's step table; returns its step count and first position.
 // The positions before the first step's are the block's halo: rows of
 // earlier bands, whose values `src` holds (stored by earlier launches); they
-// are loaded into the cone's first LDS slots.
+// are loaded into the cone's first LDS slots (and, with rdl, their 1/D from
+// the per-position rdp).
 template <class T, class S>
 __device__ inline int cone_stage(const ConeK& C, int b, ConeStep* tab, int32_t& base,
-                                 const S* __restrict__ src, T* v) {
+                                 const S* __restrict__ src, T* v,
+                                 const T* __restrict__ rdp = nullptr, T* rdl = nullptr) {
   const int s0 = C.blk[b], s1 = C.blk[b + 1];
   const int4* srct = reinterpret_cast<const int4*>(C.steps + (int64_t)s0 * kConeTab);
   int4* dst = reinterpret_cast<int4*>(tab);
@@ -78,6 +83,7 @@ __device__ inline int cone_stage(const ConeK& C, int b, ConeStep* tab, int32_t& 
   for (int32_t p = threadIdx.x; p < nh; p += blockDim.x) {
     const int32_t c = C.crow[base + p];
     v[p] = (T)src[c >= 0 ? c : ~c];
+    if (rdl) rdl[p] = rdp[base + p];
   }
   __syncthreads();
   return s1 - s0;
@@ -109,13 +115,15 @@ struct CFRec {
   int32_t c, lp;  // crow entry (row or ~row), cone-local position
   bool rep;       // a repeat of the step's last row
   V ro, qo;       // gathered: UPD r and q of the row; otherwise ro = u
+  T rdi;          // 1/D of the row (unfolded factors)
   int32_t li[WK];
-  T av[WK], rv[WK];
+  T av[WK];
 };
 
 // Forward sweep y = (I + L D^-1)^-1 u (fwd_row's arithmetic): y to Yt in
-// device order (the cone backward sweep reads it there); a and rd hold a_ij
-// and 1/D_j per cone entry.  The vectors of W have type V and the sweep
+// device order (the cone backward sweep reads it there); a holds a_ij per
+// cone entry and rd 1/D per cone position: each row loads its own 1/D and
+// keeps it in LDS beside y, where its dependents' entries read both.  The vectors of W have type V and the sweep
 // computes in T: V = double with T = float is the FP32 preconditioner of an
 // FP64 Krylov solve (u is rounded to float as the sweep reads it, y stays
 // float); V = T otherwise, with Yt = W.Yb.  UPD fuses the CG update: u = r - alpha q, stored
@@ -127,7 +135,7 @@ struct CFRec {
 // the FP64 tolerance) takes folded factors: a holds a_ij / D_j per forward
 // entry (rd unused) and d holds 1 / D per backward position, so a forward
 // entry loads one value and a backward row multiplies.  FP64 keeps the
-// reference's (a_ij y_j) / D_j and division.
+// reference's (a_ij y_j) (1/D_j) and division.
 template <class T>
 constexpr bool cone_folded() {
   return std::is_same<T, float>::value;
@@ -135,19 +143,20 @@ constexpr bool cone_folded() {
 
 template <bool UPD, class T, int WK, int P, class V = T>
 __device__ inline void fwd_cone(const ConeK& C, const T* __restrict__ a, const T* __restrict__ rd,
-                                const FwdK<V>& W, T* __restrict__ Yt, ConeLDS<T>& L) {
+                                const FwdK<V>& W, T* __restrict__ Yt,
+                                ConeLDS<T, !cone_folded<T>()>& L) {
+  constexpr bool fold = cone_folded<T>();
   const int b = C.blk0 + (int)xcd_block();
   int32_t base;
-  const int ns = cone_stage(C, b, L.tab, base, (const T*)Yt, L.v);
+  const int ns = cone_stage(C, b, L.tab, base, (const T*)Yt, L.v, fold ? nullptr : rd,
+                            fold ? nullptr : L.rd);
   const V al = UPD && W.ks ? (V)W.ks->alpha : V(0);
   auto fetch = [&](CFRec<T, WK, V>& R, int s) {
     int64_t e[WK];
     cone_fetch_pos<WK>(C, L.tab[s < ns ? s : ns - 1], base, R.c, R.lp, R.rep, R.li, e);
+    R.rdi = fold ? T(0) : rd[base + R.lp];
 #pragma unroll
-    for (int k = 0; k < WK; ++k) {
-      R.av[k] = a[e[k]];
-      if (!cone_folded<T>()) R.rv[k] = rd[e[k]];
-    }
+    for (int k = 0; k < WK; ++k) R.av[k] = a[e[k]];
   };
   auto gather = [&](CFRec<T, WK, V>& R) {
     const int32_t i = R.c >= 0 ? R.c : ~R.c;
@@ -163,13 +172,15 @@ __device__ inline void fwd_cone(const ConeK& C, const T* __restrict__ a, const T
 #pragma unroll
     for (int k = 0; k < WK; ++k) {
       const int li = R.li[k];
-      const T yv = L.v[li >= 0 ? li : 0];
-      const T term = cone_folded<T>() ? R.av[k] * yv : (R.av[k] * yv) * R.rv[k];
+      const int lj = li >= 0 ? li : 0;
+      const T yv = L.v[lj];
+      const T term = fold ? R.av[k] * yv : (R.av[k] * yv) * L.rd[lj];
       sum = li >= 0 ? sum + term : sum;
     }
     const V u = UPD ? R.ro + (-al) * R.qo : R.ro;
     const T y = (T)u - sum;
     L.v[R.lp] = y;
+    if (!fold) L.rd[R.lp] = R.rdi;
     const int32_t i = R.c >= 0 ? R.c : ~R.c;
     Yt[i] = y;
     if (UPD) W.rout[i] = u;

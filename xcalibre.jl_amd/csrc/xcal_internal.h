@@ -312,7 +312,7 @@ struct xcal_eqn {
   xcal::DBuf<double> Lprod;  // a_ij a_ji of each packed lower entry (DILU diagonal update)
   // cone form: a_ij and 1/D_j per forward cone entry, a_ij per backward cone
   // entry, D per backward cone position
-  xcal::DBuf<double> cfa, cfr, cba, cbd;
+  xcal::DBuf<double> cfa, cfr, cfd, cba, cbd;
   // Krylov workspace (allocated lazily)
   xcal::DBuf<double> work;    // solver vectors, 12 x n
   xcal::DBuf<double> kpart;   // reduction partial sums

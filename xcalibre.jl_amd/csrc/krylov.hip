@@ -13,6 +13,7 @@
 #include <cmath>
 
 #include "device_common.h"
+#include "sweep_cone.h"
 #include "sweep_pipe.h"
 
 namespace xcal {
@@ -130,6 +131,25 @@ __global__ void __launch_bounds__(kRunBlock)
     __syncthreads();
   }
 }
+// update_dilu_diagonal! in the cone form (sweep_cone.h), one launch per band
+template <int WK, int P>
+__global__ void __launch_bounds__(kConeBlock)
+    k_dilu_diag_cone(ConeK C, const double* __restrict__ prod, const double* __restrict__ ajj,
+                     double* D) {
+  __shared__ ConeLDS<double> L;
+  diag_cone<WK, P>(C, prod, ajj, D, L);
+}
+// a_ji a_ij per cone entry (0 where the transpose is absent, which a face
+// matrix's symmetric pattern never has)
+__global__ void __launch_bounds__(kBlock)
+    k_cone_products(int64_t n, const int32_t* __restrict__ slot, const int32_t* __restrict__ trans,
+                    const double* __restrict__ nz, double* __restrict__ out) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const int32_t k = slot[e], kt = trans[k];
+  out[e] = kt >= 0 ? nz[kt] * nz[k] : 0.0;
+}
+
 // update_dilu_diagonal! over a run of small levels, pipelined (sweep_pipe.h)
 template <bool FAR, int WK>
 __global__ void __launch_bounds__(kPipeBlock)
@@ -240,7 +260,26 @@ void precond_update(xcal_eqn* e, int precon) {
                          m->Lslot.n, m->Lslot.p, m->trans.p, e->nzval.p, e->Lprod.p);
     }
     if (m->batch) precond_update_batch(e);  // one workgroup per case
-    else
+    else if (m->cone) {  // one launch per forward band; a_jj and a_ji a_ij in cone order
+      const xcal_mesh::Cone& F = m->cone_f;
+      if (e->cfd.n != F.rows) e->cfd.alloc(F.rows);  // a_jj here, 1/D after the update
+      if (e->cfa.n != F.entries) {  // cfr: products here, 1/D_j after
+        e->cfa.alloc(F.entries);
+        e->cfr.alloc(F.entries);
+      }
+      hipLaunchKernelGGL(k_gather_cone_rows, dim3(grid_for(F.rows)), dim3(kBlock), 0, s, F.rows,
+                         F.crow.p, e->pD.p, e->cfd.p);
+      hipLaunchKernelGGL(k_cone_products, dim3(grid_for(F.entries)), dim3(kBlock), 0, s,
+                         F.entries, F.slot.p, m->trans.p, e->nzval.p, e->cfr.p);
+      with_cone_form(F.wk, !m->cone_deep, [&](auto WK, auto P) {
+        for (size_t k = 0; k + 1 < F.band.size(); ++k)
+          hipLaunchKernelGGL((k_dilu_diag_cone<decltype(WK)::value, decltype(P)::value>),
+                             dim3(F.band[k + 1] - F.band[k]), dim3(kConeBlock), 0, s,
+                             cone_k(m, F, F.band[k]), (const double*)e->cfr.p,
+                             (const double*)e->cfd.p, e->pD.p);
+      });
+      launch_check();
+    } else
     for (const SweepStep& st : m->plan_f) {
       if (st.lev1 == 1) continue;  // level 0 rows have no lower neighbours
       if (st.grid == 1 && st.pipe > 0) {

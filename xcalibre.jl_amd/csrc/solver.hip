@@ -112,6 +112,43 @@ __global__ void __launch_bounds__(kBlock)
   if (i < end && live) pnew[i] = z[i] + beta * pold[i];
 }
 
+// CG with DILU on large meshes: the backward sweep's pending r.z finalize
+// (FIN_CG_ITER from its partials, kstate: every workgroup reduces them in the
+// same order and workgroup 0 stores the state to K.out), then p = z + beta p
+// over the rows of A's row blocks blockIdx.x + u gridDim.x, u < kPFinU.  The
+// grid is a multiple of 8, so row block b is handled on the XCD of the SpMV
+// workgroup that reads it.  The rows' loads are issued before the
+// reduction, which they do not depend on.
+constexpr int kPFinU = 4;
+inline int pfin_grid(int nblk) { return ((nblk + kPFinU - 1) / kPFinU + 7) / 8 * 8; }
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_cg_p_fin(MatKT<T> A, int nblk, const T* __restrict__ z, const T* __restrict__ pold,
+               T* __restrict__ pnew, KStep K) {
+  __shared__ double lds[kBlock / 64];
+  int64_t i[kPFinU];
+  T zv[kPFinU], pv[kPFinU];
+#pragma unroll
+  for (int u = 0; u < kPFinU; ++u) {
+    const int b = (int)blockIdx.x + u * (int)gridDim.x;
+    const int bb = b < nblk ? b : 0;
+    const int64_t r = (int64_t)A.blk[2 * bb] + threadIdx.x;
+    i[u] = b < nblk && r < A.blk[2 * bb + 1] ? r : -1;
+  }
+#pragma unroll
+  for (int u = 0; u < kPFinU; ++u) {
+    const int64_t ii = i[u] >= 0 ? i[u] : 0;
+    zv[u] = z[ii];
+    pv[u] = pold[ii];
+  }
+  const KS k = kstate(K, lds);
+  if (k.stop != STOP_NONE) return;  // uniform over the grid
+  const T beta = (T)k.beta;
+#pragma unroll
+  for (int u = 0; u < kPFinU; ++u)
+    if (i[u] >= 0) pnew[i[u]] = zv[u] + beta * pv[u];
+}
+
 // CG in one launch for meshes below kSplitRows rows: p = z + beta p on the
 // fly for every neighbour ; q = A p ; p.q.  There the launch costs more than
 // the second gather per entry.
@@ -500,10 +537,6 @@ __global__ void __launch_bounds__(kConeBlock)
   bwd_cone<DOT, T, WK, P, V>(C, a, d, W, Yt, L);
 }
 
-inline ConeK cone_k(const xcal_mesh* m, const xcal_mesh::Cone& c, int32_t blk0) {
-  return ConeK{c.blk.p, c.base.p, c.steps.p, c.crow.p, c.lidx.p, m->cone_own.p, m->cone_ownp.p,
-               blk0};
-}
 
 // ------------------------------------------------------------ host side
 // One Krylov solve's view of an equation in value type T: the matrix, the
@@ -957,14 +990,28 @@ static int cg(Solver<T>& S, int precon, const T* b, T* x0, int itmax, double ato
   int rpar = 0;
   const bool split = n >= kSplitRows;
   Fuse<T> F(S, S.m->fuse && !dilu && !split && S.gridA <= kFuseBlocks && S.grid <= kFuseBlocks);
+  // Cone sweeps on a large mesh: the backward sweep's r.z finalize is left
+  // pending and done by k_cg_p_fin at the start of the next iteration (state
+  // into S.fks, then the p.q finalize moves it back to S.ks), or by flush()
+  // before the host reads the state.
+  const bool pfin = dilu && split && S.m->cone && sweep_slots <= 1024;
+  bool pend = false;
   auto iteration = [&]() {
     T* pold = parity ? pb : pa;
     T* pnew = parity ? pa : pb;
     parity ^= 1;
     xcal_ctx* c = S.m->ctx;
-    if (split)
+    const KS* cur = S.ks;  // the state the SpMV and its finalize read
+    if (split && pend) {
+      const KStep Kp{S.ks, S.fks, S.partial, sweep_slots, FIN_CG_ITER};
+      hipLaunchKernelGGL(k_cg_p_fin<T>, dim3(pfin_grid(S.gridA)), dim3(kBlock), 0, S.s, S.A,
+                         S.gridA, (const T*)zd, (const T*)pold, pnew, Kp);
+      cur = S.fks;
+      pend = false;
+    } else if (split) {
       hipLaunchKernelGGL(k_cg_p<T>, dim3(S.gridA), dim3(kBlock), 0, S.s, S.A, (const T*)zd,
                          (const T*)pold, pnew, (const KS*)S.ks);
+    }
     double* pp = F.part();
     const KStep K = F.step();
     timed(c, 0, [&] {
@@ -972,13 +1019,17 @@ static int cg(Solver<T>& S, int precon, const T* b, T* x0, int itmax, double ato
         constexpr int g = decltype(G)::value;
         if (split)
           hipLaunchKernelGGL((k_cg_spmv<g, T>), dim3(S.gridA), dim3(kBlock), 0, S.s, S.A,
-                             (const T*)pnew, q, S.partial, (const KS*)S.ks);
+                             (const T*)pnew, q, S.partial, cur);
         else
           hipLaunchKernelGGL((k_cg_pspmv<g, T>), dim3(S.gridA), dim3(kBlock), 0, S.s, S.A,
                              (const T*)zd, (const T*)pold, pnew, q, pp, K);
       });
     });
-    F.done(pp, S.gridA, FIN_CG_PAP);
+    if (cur != S.ks)
+      hipLaunchKernelGGL(k_finalize_to, dim3(1), dim3(kBlock), 0, S.s, (const double*)S.partial,
+                         S.gridA, cur, S.ks, (int)FIN_CG_PAP);
+    else
+      F.done(pp, S.gridA, FIN_CG_PAP);
     if (dilu) {
       FwdK<T> W = fwdk(S, (const T*)nullptr, Y, Yb, (const KS*)S.ks);
       W.p = pnew;
@@ -990,7 +1041,8 @@ static int cg(Solver<T>& S, int precon, const T* b, T* x0, int itmax, double ato
       const T* rnew = rv[rpar];
       timed(c, 1, [&] { sweep_fwd<true>(S, W); });
       timed(c, 2, [&] { sweep_bwd<1>(S, bwdk(S, (const T*)Yb, zd, rnew, (const KS*)S.ks)); });
-      finalize_launch(S, sweep_slots, FIN_CG_ITER);
+      if (pfin) pend = true;
+      else finalize_launch(S, sweep_slots, FIN_CG_ITER);
     } else {
       double* pu = F.part();
       const KStep Ku = F.step();
@@ -1000,7 +1052,12 @@ static int cg(Solver<T>& S, int precon, const T* b, T* x0, int itmax, double ato
     }
   };
   KS st = read_state(S);
-  if (st.stop == STOP_NONE) run_batches(S, S.cg_batch, 4, iteration, st, [&] { F.flush(); });
+  if (st.stop == STOP_NONE)
+    run_batches(S, S.cg_batch, 4, iteration, st, [&] {
+      F.flush();
+      if (pend) finalize_launch(S, sweep_slots, FIN_CG_ITER);
+      pend = false;
+    });
   if (x0) hipLaunchKernelGGL(k_add, dim3(S.grid), dim3(kBlock), 0, S.s, n, (const double*)x,
                              (double*)x0);
   if (xsol) *xsol = x;

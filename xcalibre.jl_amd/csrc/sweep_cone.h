@@ -275,6 +275,57 @@ __device__ inline void bwd_cone(const ConeK& C, const T* __restrict__ a, const T
   if (DOT != 0) store_partial<kConeBlock>(s0, s1, W.partial, b);
 }
 
+template <int WK>
+struct CDRec {
+  int32_t c, lp;
+  bool rep;
+  double d0;  // a_jj
+  int32_t li[WK];
+  double pr[WK];
+};
+
+// update_dilu_diagonal! over the forward cone (the sweep's dependencies):
+// D_j = a_jj, then D_j -= (a_ji a_ij) / D_i over its lower entries in column
+// order (dilu_diag_row's arithmetic).  prod holds a_ji a_ij per forward cone
+// entry, ajj a_jj per forward cone position; D_j goes to D in device order,
+// where later bands read their halo.
+template <int WK, int P>
+__device__ inline void diag_cone(const ConeK& C, const double* __restrict__ prod,
+                                 const double* __restrict__ ajj, double* __restrict__ D,
+                                 ConeLDS<double>& L) {
+  const int b = C.blk0 + (int)xcd_block();
+  int32_t base;
+  const int ns = cone_stage(C, b, L.tab, base, (const double*)D, L.v);
+  auto fetch = [&](CDRec<WK>& R, int s) {
+    int64_t e[WK];
+    cone_fetch_pos<WK>(C, L.tab[s < ns ? s : ns - 1], base, R.c, R.lp, R.rep, R.li, e);
+    R.d0 = ajj[base + R.lp];
+#pragma unroll
+    for (int k = 0; k < WK; ++k) R.pr[k] = prod[e[k]];
+  };
+  auto gather = [&](CDRec<WK>&) {};
+  auto process = [&](const CDRec<WK>& R) {
+    double dj = R.d0;
+#pragma unroll
+    for (int k = 0; k < WK; ++k) {
+      const int li = R.li[k];
+      const double di = L.v[li >= 0 ? li : 0];
+      dj = li >= 0 ? dj - R.pr[k] / di : dj;
+    }
+    L.v[R.lp] = dj;
+    D[R.c >= 0 ? R.c : ~R.c] = dj;
+  };
+  CDRec<WK> buf[P];
+  pipe_levels<P>(ns, buf, fetch, gather, process);
+}
+
+// Host: the kernel view of one sweep direction's cone, for the launch of
+// the band whose first block is blk0.
+inline ConeK cone_k(const xcal_mesh* m, const xcal_mesh::Cone& c, int32_t blk0) {
+  return ConeK{c.blk.p, c.base.p, c.steps.p, c.crow.p, c.lidx.p, m->cone_own.p, m->cone_ownp.p,
+               blk0};
+}
+
 // Host: calls f(WK, P) with the cone sweeps' compile-time entry bound and
 // prefetch depth (shallow: 3 steps; deep: 6 steps for up to 4 entries, 4
 // above).  Shallow is the default: on the 1M-cell box its fewer registers

@@ -42,6 +42,7 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--quick]
 from __future__ import annotations
 
 import argparse
+import gc
 import json
 import os
 import sys
@@ -766,6 +767,7 @@ def main():
         err = {f: rel(sim.get(f), sim64.get(f)) for f in ("U", "p", "mdotf")}
 
     mixed = batched = serving = None
+    nnz, levels = dm.nnz, list(dm.levels)
     if args.precision == "fp64" and not args.quick:
         # the same channel with FP32 inner solves in FP64 refinement, and the
         # batched form: the other two parts of the headline metric
@@ -807,6 +809,11 @@ def main():
                              "graph": bl["detail"]["graph"], "tuned": bl["detail"]["tuned"]}
             log(f"batch {prec}: {bl['value']:.4g} cell-iters/s")
         batched["workload"] = bl["config"]["workload"].split(", SIMPLE")[0]
+        # the server's lanes and submission path each take a stream; the
+        # channel's context (one more) is released first, so that the
+        # process holds no more streams than in --mode serve
+        del sim, dm, ctx
+        gc.collect()
         _, serving = serve_leg(args, x, args.serve_requests, args.rate, args.max_iterations,
                                alone=False)
         log(f"serving: {serving['converged']}/{args.serve_requests} converged, p50 "
@@ -840,7 +847,7 @@ def main():
         "data": "synthetic (seeded jittered, renumbered hex box)",
         "config": {"workload": f"3D laminar channel SIMPLE {args.precision.upper()}, {N}x{N}x{N} "
                                f"jittered+renumbered hex box ({n} cells)",
-                   "cells": n, "nnz": dm.nnz, "dilu_levels": list(dm.levels),
+                   "cells": n, "nnz": nnz, "dilu_levels": levels,
                    "p_solver": {"fp64": "CG+DILU rtol 1e-10",
                                 "fp32-ir": f"FP32 CG+DILU (inner rtol {args.inner_rtol:g}) in "
                                            f"FP64 refinement to 1e-10",

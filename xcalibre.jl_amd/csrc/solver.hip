@@ -517,15 +517,29 @@ __global__ void __launch_bounds__(kPipeBlock)
   if (DOT != 0) store_partial<kPipeBlock>(s0[0], s1[0], W.partial, slot);
 }
 
-// Cone form (sweep_cone.h): one workgroup per block, grid = the mesh's
-// block count.  Both return at entry once the solve has stopped.  The sweep
-// computes in T over vectors of type V; Yt is the forward output in T.
+// Cone form (sweep_cone.h): one workgroup per block of the launch's band.
+// Both return at entry once the solve has stopped.  The sweep computes in T
+// over vectors of type V; Yt is the forward output in T.  With K.fin set the
+// forward sweep first does the SpMV's p.q finalize itself (kstate: every
+// workgroup from the same partials, workgroup 0 storing the state to K.out)
+// and takes alpha from it; otherwise it reads the state W.ks.
+static_assert(kConeBlock == kBlock, "kstate reduces with kBlock threads");
 template <bool UPD, class T, int WK, int P, class V>
 __global__ void __launch_bounds__(kConeBlock)
-    k_fwd_cone(ConeK C, const T* __restrict__ a, const T* __restrict__ rd, FwdK<V> W, T* Yt) {
+    k_fwd_cone(ConeK C, const T* __restrict__ a, const T* __restrict__ rd, FwdK<V> W, T* Yt,
+               KStep K) {
   __shared__ ConeLDS<T, !cone_folded<T>()> L;
-  if (W.ks && stopped(W.ks)) return;  // uniform over the workgroup
-  fwd_cone<UPD, T, WK, P, V>(C, a, rd, W, Yt, L);
+  __shared__ double lds[kConeBlock / 64];
+  double al = 0.0;
+  if (K.fin != FIN_NONE) {
+    const KS k = kstate(K, lds);
+    if (k.stop != STOP_NONE) return;  // uniform over the grid
+    al = k.alpha;
+  } else if (W.ks) {
+    if (stopped(W.ks)) return;  // uniform over the workgroup
+    al = W.ks->alpha;
+  }
+  fwd_cone<UPD, T, WK, P, V>(C, a, rd, W, Yt, L, al);
 }
 
 template <int DOT, class T, int WK, int P, class V>
@@ -766,9 +780,10 @@ struct Fuse {
   }
 };
 
-// forward sweep over the launch plan
+// forward sweep over the launch plan; K (cone form, CG): a p.q finalize
+// the sweep does first, its state then in K.out
 template <bool UPD, class T>
-static void sweep_fwd(const Solver<T>& S, const FwdK<T>& W) {
+static void sweep_fwd(const Solver<T>& S, const FwdK<T>& W, const KStep* K = nullptr) {
   const xcal_mesh* m = S.m;
   if (m->cone) {  // y to W.Yb in device order
     XCAL_CHECK(W.Yb != (UPD ? W.r : W.u) && (!UPD || (W.rout != W.r && W.rout != W.Yb)),
@@ -776,15 +791,16 @@ static void sweep_fwd(const Solver<T>& S, const FwdK<T>& W) {
     const std::vector<int32_t>& band = m->cone_f.band;
     with_cone_form(m->cone_f.wk, !m->cone_deep, [&](auto WK, auto P) {
       constexpr int wk = decltype(WK)::value, p = decltype(P)::value;
+      const KStep Kp = K ? *K : KStep{nullptr, nullptr, nullptr, 0, FIN_NONE};
       for (size_t k = 0; k + 1 < band.size(); ++k) {  // one launch per band of levels
         const ConeK C = cone_k(m, m->cone_f, band[k]);
         const dim3 grid(band[k + 1] - band[k]);
         if (std::is_same<T, double>::value && S.pc32)  // y in float, in the Yb vector's storage
           hipLaunchKernelGGL((k_fwd_cone<UPD, float, wk, p, T>), grid, dim3(kConeBlock), 0, S.s,
-                             C, S.cfa32, S.cfa32, W, reinterpret_cast<float*>(W.Yb));
+                             C, S.cfa32, S.cfa32, W, reinterpret_cast<float*>(W.Yb), Kp);
         else
           hipLaunchKernelGGL((k_fwd_cone<UPD, T, wk, p, T>), grid, dim3(kConeBlock), 0, S.s, C,
-                             S.cfa, S.cfd, W, W.Yb);
+                             S.cfa, S.cfd, W, W.Yb, Kp);
       }
     });
     return;
@@ -990,27 +1006,29 @@ static int cg(Solver<T>& S, int precon, const T* b, T* x0, int itmax, double ato
   int rpar = 0;
   const bool split = n >= kSplitRows;
   Fuse<T> F(S, S.m->fuse && !dilu && !split && S.gridA <= kFuseBlocks && S.grid <= kFuseBlocks);
-  // Cone sweeps on a large mesh: the backward sweep's r.z finalize is left
-  // pending and done by k_cg_p_fin at the start of the next iteration (state
-  // into S.fks, then the p.q finalize moves it back to S.ks), or by flush()
-  // before the host reads the state.
+  // Cone sweeps on a large mesh: no separate finalize.  The backward
+  // sweep's r.z finalize is left pending and done by k_cg_p_fin at the start
+  // of the next iteration, the SpMV's p.q finalize by the forward sweep; each
+  // moves the state between S.ks and S.fks (`cur` holds it), and flush()
+  // brings it back to S.ks before the host reads it.
   const bool pfin = dilu && split && S.m->cone && sweep_slots <= 1024;
   bool pend = false;
+  KS* cur = S.ks;
+  auto other = [&](KS* k) { return k == S.ks ? S.fks : S.ks; };
   auto iteration = [&]() {
     T* pold = parity ? pb : pa;
     T* pnew = parity ? pa : pb;
     parity ^= 1;
     xcal_ctx* c = S.m->ctx;
-    const KS* cur = S.ks;  // the state the SpMV and its finalize read
     if (split && pend) {
-      const KStep Kp{S.ks, S.fks, S.partial, sweep_slots, FIN_CG_ITER};
+      const KStep Kp{cur, other(cur), S.partial, sweep_slots, FIN_CG_ITER};
       hipLaunchKernelGGL(k_cg_p_fin<T>, dim3(pfin_grid(S.gridA)), dim3(kBlock), 0, S.s, S.A,
                          S.gridA, (const T*)zd, (const T*)pold, pnew, Kp);
-      cur = S.fks;
+      cur = other(cur);
       pend = false;
     } else if (split) {
       hipLaunchKernelGGL(k_cg_p<T>, dim3(S.gridA), dim3(kBlock), 0, S.s, S.A, (const T*)zd,
-                         (const T*)pold, pnew, (const KS*)S.ks);
+                         (const T*)pold, pnew, (const KS*)cur);
     }
     double* pp = F.part();
     const KStep K = F.step();
@@ -1019,19 +1037,15 @@ static int cg(Solver<T>& S, int precon, const T* b, T* x0, int itmax, double ato
         constexpr int g = decltype(G)::value;
         if (split)
           hipLaunchKernelGGL((k_cg_spmv<g, T>), dim3(S.gridA), dim3(kBlock), 0, S.s, S.A,
-                             (const T*)pnew, q, S.partial, cur);
+                             (const T*)pnew, q, S.partial, (const KS*)cur);
         else
           hipLaunchKernelGGL((k_cg_pspmv<g, T>), dim3(S.gridA), dim3(kBlock), 0, S.s, S.A,
                              (const T*)zd, (const T*)pold, pnew, q, pp, K);
       });
     });
-    if (cur != S.ks)
-      hipLaunchKernelGGL(k_finalize_to, dim3(1), dim3(kBlock), 0, S.s, (const double*)S.partial,
-                         S.gridA, cur, S.ks, (int)FIN_CG_PAP);
-    else
-      F.done(pp, S.gridA, FIN_CG_PAP);
+    if (!pfin) F.done(pp, S.gridA, FIN_CG_PAP);
     if (dilu) {
-      FwdK<T> W = fwdk(S, (const T*)nullptr, Y, Yb, (const KS*)S.ks);
+      FwdK<T> W = fwdk(S, (const T*)nullptr, Y, Yb, (const KS*)cur);
       W.p = pnew;
       W.q = q;
       W.x = x;
@@ -1039,10 +1053,17 @@ static int cg(Solver<T>& S, int precon, const T* b, T* x0, int itmax, double ato
       if (S.m->cone) rpar ^= 1;
       W.rout = rv[rpar];
       const T* rnew = rv[rpar];
-      timed(c, 1, [&] { sweep_fwd<true>(S, W); });
-      timed(c, 2, [&] { sweep_bwd<1>(S, bwdk(S, (const T*)Yb, zd, rnew, (const KS*)S.ks)); });
-      if (pfin) pend = true;
-      else finalize_launch(S, sweep_slots, FIN_CG_ITER);
+      if (pfin) {  // the forward sweep does the p.q finalize
+        const KStep Kq{cur, other(cur), S.partial, S.gridA, FIN_CG_PAP};
+        cur = other(cur);
+        timed(c, 1, [&] { sweep_fwd<true>(S, W, &Kq); });
+        timed(c, 2, [&] { sweep_bwd<1>(S, bwdk(S, (const T*)Yb, zd, rnew, (const KS*)cur)); });
+        pend = true;
+      } else {
+        timed(c, 1, [&] { sweep_fwd<true>(S, W); });
+        timed(c, 2, [&] { sweep_bwd<1>(S, bwdk(S, (const T*)Yb, zd, rnew, (const KS*)S.ks)); });
+        finalize_launch(S, sweep_slots, FIN_CG_ITER);
+      }
     } else {
       double* pu = F.part();
       const KStep Ku = F.step();
@@ -1055,8 +1076,12 @@ static int cg(Solver<T>& S, int precon, const T* b, T* x0, int itmax, double ato
   if (st.stop == STOP_NONE)
     run_batches(S, S.cg_batch, 4, iteration, st, [&] {
       F.flush();
-      if (pend) finalize_launch(S, sweep_slots, FIN_CG_ITER);
+      if (pend || cur != S.ks)  // the pending r.z finalize, or a copy, into S.ks
+        hipLaunchKernelGGL(k_finalize_to, dim3(1), dim3(kBlock), 0, S.s,
+                           (const double*)S.partial, sweep_slots, (const KS*)cur, S.ks,
+                           (int)(pend ? FIN_CG_ITER : FIN_NONE));
       pend = false;
+      cur = S.ks;
     });
   if (x0) hipLaunchKernelGGL(k_add, dim3(S.grid), dim3(kBlock), 0, S.s, n, (const double*)x,
                              (double*)x0);

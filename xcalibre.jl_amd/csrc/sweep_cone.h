@@ -128,7 +128,7 @@ struct CFRec {
 // FP64 Krylov solve (u is rounded to float as the sweep reads it, y stays
 // float); V = T otherwise, with Yt = W.Yb.  UPD fuses the CG update: u = r - alpha q, stored
 // to W.rout (other cones read the old r of every row during the sweep), and
-// x += alpha p over the block's own rows after the sweep.
+// x += alpha p over the block's own rows after the sweep (alpha: al_in).
 // The caller returns before the sweep once the solve has stopped, so every
 // store here is live.
 // FP32 (the inner solves of the mixed-precision form, refined in FP64 to
@@ -144,13 +144,13 @@ constexpr bool cone_folded() {
 template <bool UPD, class T, int WK, int P, class V = T>
 __device__ inline void fwd_cone(const ConeK& C, const T* __restrict__ a, const T* __restrict__ rd,
                                 const FwdK<V>& W, T* __restrict__ Yt,
-                                ConeLDS<T, !cone_folded<T>()>& L) {
+                                ConeLDS<T, !cone_folded<T>()>& L, double al_in) {
   constexpr bool fold = cone_folded<T>();
   const int b = C.blk0 + (int)xcd_block();
   int32_t base;
   const int ns = cone_stage(C, b, L.tab, base, (const T*)Yt, L.v, fold ? nullptr : rd,
                             fold ? nullptr : L.rd);
-  const V al = UPD && W.ks ? (V)W.ks->alpha : V(0);
+  const V al = UPD ? (V)al_in : V(0);
   auto fetch = [&](CFRec<T, WK, V>& R, int s) {
     int64_t e[WK];
     cone_fetch_pos<WK>(C, L.tab[s < ns ? s : ns - 1], base, R.c, R.lp, R.rep, R.li, e);

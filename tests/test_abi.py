@@ -57,3 +57,31 @@ def test_reference_shaped_api_validation(x, meshes):
     U = x.api.Physics(domain=m, fluid=api.Fluid(nu=1.0)).momentum.U
     api.initialise(U, [1, 2, 3])
     assert (U[:, 5] == [1, 2, 3]).all()
+
+
+def test_ctypes_structs_match_the_c_header(x, tmp_path):
+    """Every ctypes mirror of an include/xcal.h struct has the C struct's size
+    and each field's offset, as gcc lays them out (host only)."""
+    A = x._abi
+    pairs = {"MeshHost": "xcal_mesh_host", "BC": "xcal_bc", "SolverSetupC": "xcal_solver_setup",
+             "SolveStats": "xcal_solve_stats", "SimpleSetupC": "xcal_simple_setup",
+             "TurbulenceSetupC": "xcal_turbulence_setup", "ServerConfigC": "xcal_server_config",
+             "RequestStatusC": "xcal_request_status"}
+    lines = ["#include <stddef.h>", "#include <stdio.h>", '#include "xcal.h"', "int main(void) {"]
+    expect = {}
+    for py, c in pairs.items():
+        S = getattr(A, py)
+        lines.append(f'  printf("{c} sizeof %zu\\n", sizeof({c}));')
+        expect[f"{c} sizeof"] = C.sizeof(S)
+        for f in S._fields_:
+            lines.append(f'  printf("{c} {f[0]} %zu\\n", offsetof({c}, {f[0]}));')
+            expect[f"{c} {f[0]}"] = getattr(S, f[0]).offset
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", f"-I{ROOT / 'include'}", str(src), "-o", str(exe)],
+                   check=True, capture_output=True, text=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
+    got = {" ".join(l.split()[:2]): int(l.split()[2]) for l in out.splitlines()}
+    assert got == expect

@@ -117,17 +117,20 @@ def test_cone_preconditioner_bit_exact(x, A, O, ctx, box48, box48n, monkeypatch,
     assert np.array_equal(out[0], oz)
 
 
+@pytest.mark.parametrize("numbering", ["random", "natural"])
 @pytest.mark.parametrize("solver", ["SOLVER_CG", "SOLVER_BICGSTAB"])
 @pytest.mark.parametrize("prec", ["PREC_FP64", "PREC_FP32_IR", "PREC_FP32_PRECON"])
-def test_cone_krylov_equals_level_form(x, A, ctx, box48, monkeypatch, solver, prec):
+def test_cone_krylov_equals_level_form(x, A, ctx, box48, box48n, monkeypatch, solver, prec,
+                                       numbering):
     """Solutions to rtol 1e-12 agree to 1e-9 relative (the dot products are
     summed per block instead of per level launch); FP64 iteration counts
     within 5% (CG) and 25% (BiCGSTAB).  Under PREC_FP32_PRECON the cone
     sweeps of CG compute in FP32 and the level form keeps FP64, so the two
     differ by the preconditioner's rounding as well: CG's FP64 residual
     recurrence still reaches the same solution.  BiCGSTAB keeps FP64 sweeps
-    under that mode."""
+    under that mode.  Natural numbering runs the cone sweeps in bands."""
     H = x.hip
+    box48 = box48 if numbering == "random" else box48n
     sym = solver == "SOLVER_CG"
     res = []
     for env in ({}, {"XCAL_NO_CONE": "1"}):

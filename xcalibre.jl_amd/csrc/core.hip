@@ -196,6 +196,7 @@ struct ConeHost {
   std::vector<int32_t> slot, dep;  // per entry: matrix slot, device row of the dependency
   std::vector<int32_t> own, ownp;  // block b owns rows own[ownp[b], ownp[b+1])
   std::vector<int32_t> band;       // first block of each band (nbands + 1)
+  std::vector<int32_t> hnum;       // per block: halo positions (rows of earlier bands)
 };
 
 namespace {
@@ -312,6 +313,7 @@ static bool build_band(int S, double ratio, bool lower, const std::vector<int32_
     }
     if (nsteps > kConeSteps) return false;
     for (int32_t c : halo) B.crow.push_back(~c);
+    B.hnum.push_back(nh);
     const int32_t gb = b0 + (int32_t)b;
     for (int32_t c : cone) B.crow.push_back(W.owner[c] == gb ? c : ~c);
     B.base.push_back(base + (int32_t)(halo.size() + cone.size()));
@@ -336,6 +338,7 @@ static bool build_band(int S, double ratio, bool lower, const std::vector<int32_
   C.slot.insert(C.slot.end(), B.slot.begin(), B.slot.end());
   C.dep.insert(C.dep.end(), B.dep.begin(), B.dep.end());
   C.own.insert(C.own.end(), B.own.begin(), B.own.end());
+  C.hnum.insert(C.hnum.end(), B.hnum.begin(), B.hnum.end());
   C.wk = std::max(C.wk, maxw);
   return true;
 }
@@ -1180,6 +1183,7 @@ static xcal_mesh* upload_mesh(xcal_ctx* ctx, HostMesh& M) {
       upload(D.slot, H.slot, bytes);
       upload(D.dep, H.dep, bytes);
       upload(D.lidx, H.lidx, bytes);
+      upload(D.hnum, H.hnum, bytes);
       D.band = H.band;
     };
     up(m->cone_f, M.cone_f);

@@ -48,6 +48,7 @@ struct ConeK {
   const uint16_t* lidx;  // cone-local position of each entry's dependency
   const int32_t* own;    // forward blocks' own rows: block b owns own[ownp[b], ownp[b+1])
   const int32_t* ownp;
+  const int32_t* hnum;   // halo positions of each block
   int32_t blk0;          // this launch's band: global index of its first block
 };
 
@@ -79,7 +80,7 @@ __device__ inline int cone_stage(const ConeK& C, int b, ConeStep* tab, int32_t& 
   int4* dst = reinterpret_cast<int4*>(tab);
   for (int j = threadIdx.x; j < (s1 - s0) * (kConeTab / 4); j += blockDim.x) dst[j] = srct[j];
   base = C.base[b];
-  const int32_t nh = C.steps[(int64_t)s0 * kConeTab] - base;  // halo positions
+  const int32_t nh = C.hnum[b];  // halo positions
   for (int32_t p = threadIdx.x; p < nh; p += blockDim.x) {
     const int32_t c = C.crow[base + p];
     v[p] = (T)src[c >= 0 ? c : ~c];
@@ -322,8 +323,8 @@ __device__ inline void diag_cone(const ConeK& C, const double* __restrict__ prod
 // Host: the kernel view of one sweep direction's cone, for the launch of
 // the band whose first block is blk0.
 inline ConeK cone_k(const xcal_mesh* m, const xcal_mesh::Cone& c, int32_t blk0) {
-  return ConeK{c.blk.p, c.base.p, c.steps.p, c.crow.p, c.lidx.p, m->cone_own.p, m->cone_ownp.p,
-               blk0};
+  return ConeK{c.blk.p,       c.base.p,        c.steps.p, c.crow.p, c.lidx.p,
+               m->cone_own.p, m->cone_ownp.p, c.hnum.p,  blk0};
 }
 
 // Host: calls f(WK, P) with the cone sweeps' compile-time entry bound and

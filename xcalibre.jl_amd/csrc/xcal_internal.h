@@ -267,7 +267,7 @@ struct xcal_mesh {
   struct Cone {
     int wk = 0;  // entry bound
     int64_t rows = 0, entries = 0;
-    xcal::DBuf<int32_t> blk, base, steps, crow, slot, dep;
+    xcal::DBuf<int32_t> blk, base, steps, crow, slot, dep, hnum;
     xcal::DBuf<uint16_t> lidx;
     std::vector<int32_t> band;  // host: first block of each band (nbands + 1)
     int nblk() const { return band.empty() ? 0 : band.back(); }

@@ -29,8 +29,10 @@
 // addressed by its row index Q steps ahead, all branch-free, so the compiler
 // waits only for the step at hand.  Entries are stored jagged-diagonal (rows
 // of a step by entry count, descending), so a step carries no padding.  A
-// row's outputs are stored by every workgroup that computes it: the values
-// are equal, and a store without a branch keeps the wait counts exact.
+// row's outputs are stored, and the row-indexed inputs that only its own
+// rows use (the backward sweep's dot operand) loaded, by its owner block
+// only: the other cones hold equal copies, and their stores to the same
+// lines from other XCDs cost more than the predicated access.
 #pragma once
 
 #include <type_traits>
@@ -183,8 +185,10 @@ __device__ inline void fwd_cone(const ConeK& C, const T* __restrict__ a, const T
     L.v[R.lp] = y;
     if (!fold) L.rd[R.lp] = R.rdi;
     const int32_t i = R.c >= 0 ? R.c : ~R.c;
-    Yt[i] = y;
-    if (UPD) W.rout[i] = u;
+    if (R.c >= 0) {  // the row's owner stores it (other cones hold equal copies)
+      Yt[i] = y;
+      if (UPD) W.rout[i] = u;
+    }
   };
   CFRec<T, WK, V> buf[P];
   pipe_levels<P>(ns, buf, fetch, gather, process);
@@ -246,7 +250,7 @@ __device__ inline void bwd_cone(const ConeK& C, const T* __restrict__ a, const T
   auto gather = [&](CBRec<T, WK, V>& R) {
     const int32_t i = R.c >= 0 ? R.c : ~R.c;
     R.yb = Yt[i];
-    R.dvi = DOT != 0 ? W.dv[i] : V(0);
+    R.dvi = DOT != 0 && R.c >= 0 ? W.dv[R.c] : V(0);  // own rows only enter the dots
   };
   double s0 = 0.0, s1 = 0.0;
   auto process = [&](const CBRec<T, WK, V>& R) {
@@ -262,7 +266,7 @@ __device__ inline void bwd_cone(const ConeK& C, const T* __restrict__ a, const T
     L.v[R.lp] = zt;
     const int32_t i = R.c >= 0 ? R.c : ~R.c;
     const V zi = (V)zt;
-    W.Z[i] = zi;
+    if (R.c >= 0) W.Z[i] = zi;
     const bool own = R.c >= 0 && !R.rep;
     double a0 = 0.0, a1 = 0.0;
     if (DOT == 1 || DOT == 2) a0 = dd(R.dvi, zi);
@@ -314,7 +318,7 @@ __device__ inline void diag_cone(const ConeK& C, const double* __restrict__ prod
       dj = li >= 0 ? dj - R.pr[k] / di : dj;
     }
     L.v[R.lp] = dj;
-    D[R.c >= 0 ? R.c : ~R.c] = dj;
+    if (R.c >= 0) D[R.c] = dj;
   };
   CDRec<WK> buf[P];
   pipe_levels<P>(ns, buf, fetch, gather, process);

@@ -140,14 +140,17 @@ __global__ void __launch_bounds__(kConeBlock)
   diag_cone<WK, P>(C, prod, ajj, D, L);
 }
 // a_ji a_ij per cone entry (0 where the transpose is absent, which a face
-// matrix's symmetric pattern never has)
+// matrix's symmetric pattern never has), and a_ij itself
 __global__ void __launch_bounds__(kBlock)
     k_cone_products(int64_t n, const int32_t* __restrict__ slot, const int32_t* __restrict__ trans,
-                    const double* __restrict__ nz, double* __restrict__ out) {
+                    const double* __restrict__ nz, double* __restrict__ out,
+                    double* __restrict__ a) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n) return;
   const int32_t k = slot[e], kt = trans[k];
-  out[e] = kt >= 0 ? nz[kt] * nz[k] : 0.0;
+  const double aij = nz[k];
+  out[e] = kt >= 0 ? nz[kt] * aij : 0.0;
+  a[e] = aij;
 }
 
 // update_dilu_diagonal! over a run of small levels, pipelined (sweep_pipe.h)
@@ -270,7 +273,7 @@ void precond_update(xcal_eqn* e, int precon) {
       hipLaunchKernelGGL(k_gather_cone_rows, dim3(grid_for(F.rows)), dim3(kBlock), 0, s, F.rows,
                          F.crow.p, e->pD.p, e->cfd.p);
       hipLaunchKernelGGL(k_cone_products, dim3(grid_for(F.entries)), dim3(kBlock), 0, s,
-                         F.entries, F.slot.p, m->trans.p, e->nzval.p, e->cfr.p);
+                         F.entries, F.slot.p, m->trans.p, e->nzval.p, e->cfr.p, e->cfa.p);
       with_cone_form(F.wk, !m->cone_deep, [&](auto WK, auto P) {
         for (size_t k = 0; k + 1 < F.band.size(); ++k)
           hipLaunchKernelGGL((k_dilu_diag_cone<decltype(WK)::value, decltype(P)::value>),
@@ -308,8 +311,7 @@ void precond_update(xcal_eqn* e, int precon) {
       }
       if (e->cba.n != B.entries) e->cba.alloc(B.entries);
       if (e->cbd.n != B.rows) e->cbd.alloc(B.rows);
-      hipLaunchKernelGGL(k_gather_slots, dim3(grid_for(F.entries)), dim3(kBlock), 0, s, F.entries,
-                         F.slot.p, e->nzval.p, e->cfa.p);
+      // cfa (a_ij per forward entry) was gathered with the diagonal update's products
       hipLaunchKernelGGL(k_gather_cols, dim3(grid_for(F.entries)), dim3(kBlock), 0, s, F.entries,
                          F.dep.p, e->prD.p, e->cfr.p);
       hipLaunchKernelGGL(k_gather_slots, dim3(grid_for(B.entries)), dim3(kBlock), 0, s, B.entries,
